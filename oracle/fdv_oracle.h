@@ -1,0 +1,117 @@
+/* fdv_oracle.h -- CPU restatement of fdv's P-frame encode-analysis and
+ * reconstruction path (motion search, motion compensation, 4x4/8x8 integer
+ * transforms, quantisation + RDOQ, residual coding, intra-in-P, mode decision,
+ * deblocking) and of the host entropy packing of a P-frame payload.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is the checker the HIP path is compared
+ * against: only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+ * leg may load it.  The product never links it.
+ *
+ * Every function cites the reference (yhbsh/fdv, include/fdv.h) line it follows.
+ * Planes handed in here are "padded": `data` points at pixel (0,0) of a buffer
+ * with a replicated border of ORC_BORDER pixels on every side (fdv.h:1548-1630).
+ */
+#ifndef FDV_ORACLE_H
+#define FDV_ORACLE_H
+#include <stddef.h>
+#include <stdint.h>
+
+#define ORC_BORDER 32
+#define ORC_NS 16            /* coded streams of a P-frame (fdv.h:6198)          */
+#define ORC_NSTRUCT 9        /* PS_NSTRUCT: 4 mode contexts + REF MVX MVY SUB MERGE */
+
+/* Stream indices (fdv.h:4170, 6199-6207). */
+enum { ORC_S_MODE0 = 0, ORC_S_REF = 4, ORC_S_MVX = 5, ORC_S_MVY = 6, ORC_S_SUB = 7,
+       ORC_S_MERGE = 8, ORC_S_CN = 9, ORC_S_CL = 10, ORC_S_CN8 = 11, ORC_S_CL8 = 12,
+       ORC_S_FL = 13, ORC_S_CNC = 14, ORC_S_CLC = 15 };
+
+/* Motion-search flavours of the analysis. */
+enum { ORC_ME_DIAMOND = 0,   /* the reference's EPZS diamond (fdv.h:3375)      */
+       ORC_ME_EXHAUSTIVE = 1 /* full +-range integer search, same sub-pel step */ };
+
+/* One macroblock's decision.  mode: 0 SKIP, 1 INTER16, 2 INTRA, 3 INTER8. */
+typedef struct {
+    uint8_t mode, ref, merge_idx, sub;
+    int16_t mv[4][2];        /* [0] for SKIP / INTER16, all four for INTER8 */
+} orc_mbinfo;
+
+/* The entropy state carried between P-frames (fdv_tabcache, fdv.h:599-612). */
+typedef struct {
+    uint16_t f1[256];
+    uint16_t fs[16][256];
+    int      have;
+    uint32_t hist[16][256];
+    int      hn;
+} orc_tabcache;
+
+/* Padded plane helpers (plane_alloc / plane_extend, fdv.h:1548, 1603). */
+int  orc_plane_stride(int w);
+void orc_plane_extend(uint8_t *data, int w, int h, int stride);
+void orc_plane_from_tight(uint8_t *data, int stride, const uint8_t *src, int w, int h);
+
+/* Block kernels. */
+int  orc_bits_val(uint32_t v);
+void orc_fdct4(const int16_t r[16], int32_t c[16]);
+void orc_idct4(const int32_t c[16], int16_t r[16]);
+void orc_quant4(const int32_t c[16], int16_t l[16], int qp);
+void orc_dequant4(const int16_t l[16], int32_t c[16], int qp);
+void orc_rdoq4(int16_t l[16], const int16_t res[16], int qp, double lambda);
+void orc_tables8(int32_t c8[64], int32_t qstep8[52]);
+void orc_fdct8(const int16_t x[64], int32_t y[64]);
+void orc_idct8(const int32_t y[64], int16_t x[64]);
+void orc_quant8(const int32_t y[64], int16_t l[64], int qp);
+void orc_dequant8(const int16_t l[64], int32_t y[64], int qp);
+int  orc_satd4(const uint8_t *src, int sstride, const uint8_t *pred);
+void orc_intra_nxn(int mode, const uint8_t *top, const uint8_t *left, uint8_t tl, int n,
+                   int ht, int hl, uint8_t *pred);
+void orc_mc_luma(const uint8_t *ref, int stride, int bx, int by, int bw, int bh,
+                 int mvx, int mvy, uint8_t *dst, int dstride);
+void orc_mc_chroma(const uint8_t *ref, int stride, int bx, int by, int bw, int bh,
+                   int mvx, int mvy, uint8_t *dst, int dstride);
+int  orc_me_search(const uint8_t *cur, int cstride, const uint8_t *ref, int rstride,
+                   int bx, int by, int bw, int bh, int range, int pmx, int pmy,
+                   int me_mode, int *mvx, int *mvy);
+void orc_deblock_plane(uint8_t *p, int w, int h, int stride, int qp);
+
+/* code_residual (fdv.h:5300): returns the 7 coefficient streams' bytes through
+ * bufs[0..6] (n l n8 l8 fl nc lc) and their lengths. */
+void orc_code_residual(const uint8_t *cur, int cstride, int ox, int oy,
+                       const uint8_t *pred, int n, int qp, double lambda, int chroma,
+                       uint8_t *rec, uint8_t *const bufs[7], size_t lens[7],
+                       double *ssd, int *bits);
+
+/* The lambda of a P-frame (fdv.h:5804). */
+double orc_lambda(int qp, double lscale);
+
+/* P-frame analysis + reconstruction over a whole w x h frame (pframe_encode,
+ * fdv.h:5764-6170).  refs are padded planes (ystride for luma, cstride for
+ * chroma).  Outputs: deblocked tight recon ry/ru/rv; per-MB decisions (may be
+ * NULL); the 16 streams in bufs[k] (each of capacity scap) with lengths sn[k].
+ * `given` (may be NULL) replaces search and mode decision by fixed decisions.
+ * Returns 0, or -1 if a stream buffer would overflow. */
+int orc_pframe_analyse(const uint8_t *cy, const uint8_t *cu, const uint8_t *cv,
+                       int w, int h,
+                       const uint8_t *const ry_ref[2], const uint8_t *const ru_ref[2],
+                       const uint8_t *const rv_ref[2], int ystride, int cstride,
+                       int navail, int qp, double lambda, int force_skip, int me_mode,
+                       const orc_mbinfo *given,
+                       uint8_t *ry, uint8_t *ru, uint8_t *rv, orc_mbinfo *mbinfo,
+                       uint8_t *const bufs[ORC_NS], size_t scap, size_t sn[ORC_NS]);
+
+/* Entropy packing of a P-frame payload from its 16 streams (fdv.h:6172-6421).
+ * tc may be NULL.  Returns the payload length, or 0 on overflow. */
+size_t orc_pframe_pack(int qp, const uint8_t *const sv[ORC_NS], const size_t sn[ORC_NS],
+                       orc_tabcache *tc, uint8_t *out, size_t cap);
+
+/* Convenience: analyse + pack with tight I420 references (built into padded
+ * planes internally).  Mirrors pframe_encode's argument meaning. */
+size_t orc_pframe_encode(const uint8_t *cy, const uint8_t *cu, const uint8_t *cv,
+                         int w, int h,
+                         const uint8_t *r0y, const uint8_t *r0u, const uint8_t *r0v,
+                         const uint8_t *r1y, const uint8_t *r1u, const uint8_t *r1v,
+                         int navail, int qp, uint8_t *out, size_t cap,
+                         uint8_t *ry, uint8_t *ru, uint8_t *rv, int force_skip,
+                         orc_tabcache *tc, double lscale, int me_mode,
+                         orc_mbinfo *mbinfo);
+
+#endif
