@@ -1,0 +1,171 @@
+"""Encode-analysis throughput on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): 1280x720 4:2:0,
+QP 24, 8 tiles -- the reference's tile-parallel video with 6 MB rows per band
+(fdv_vtile_encode, fdv.h:7161; 7 bands of 1280x96 and one of 1280x48) -- on
+seeded synthetic content (global pan over a blurred-noise texture, 12 textured
+rectangles, Gaussian grain sigma 3).  Frame 0 is the key frame: its source is
+installed as the reference (I-frame coding is outside this path); every step
+is one P-frame of all 8 bands in one batched device pass: motion search, mode
+decision, residual coding, reconstruction, deblocking and serialisation of the
+16 symbol streams, plus their copy back to the host.  Source frames are resident
+in HBM before timing starts.
+
+Prints one JSON line (see the keys below).  --gpus above 1 is refused: this
+path runs on one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+METRIC = "encode-analysis Mpix/s at 720p/1080p, learned vs exhaustive, at ≤1.5% BD-rate"
+HBM_PEAK_GBS = 8000.0
+
+
+def bands(h: int, band_mbrows: int):
+    bh = band_mbrows * 16
+    out, y = [], 0
+    while y < h:
+        out.append((y, min(bh, h - y)))
+        y += bh
+    return out
+
+
+def band_frames(frame: np.ndarray, w: int, h: int, layout):
+    """Split a tight I420 frame into the tight I420 frames of its bands."""
+    ys = w * h
+    cw, ch = w // 2, h // 2
+    Y = frame[:ys].reshape(h, w)
+    U = frame[ys:ys + cw * ch].reshape(ch, cw)
+    V = frame[ys + cw * ch:].reshape(ch, cw)
+    out = []
+    for y0, bh in layout:
+        out.append(np.concatenate([Y[y0:y0 + bh].ravel(), U[y0 // 2:(y0 + bh) // 2].ravel(),
+                                   V[y0 // 2:(y0 + bh) // 2].ravel()]))
+    return out
+
+
+def cpu_baseline(frames, w, layout, qp, seconds=12.0):
+    """The reference P-frame encoder (built from its own sources, when present)
+    or else the oracle port, one thread, on the first frames of the workload."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    try:
+        import orclib as L
+    except Exception as e:  # pragma: no cover
+        return {"value": None, "unit": "Mpix/s", "cores": 1, "kind": "port", "sample": "unavailable: %s" % e}
+    use_ref = L.ref() is not None
+    kind = "reference" if use_ref else "port"
+    px = 0
+    t0 = time.perf_counter()
+    nf = 0
+    tcs = [L.TabCache() for _ in layout]
+    near = band_frames(frames[0], w, sum(b for _, b in layout), layout)
+    far = [None] * len(layout)
+    for f in range(1, len(frames)):
+        cur = band_frames(frames[f], w, sum(b for _, b in layout), layout)
+        for b, (_, bh) in enumerate(layout):
+            navail = 1 if f == 1 else 2
+            if use_ref:
+                _, rec = L.ref_pframe(cur[b], near[b], far[b], w, bh, navail, qp, tcs[b])
+            else:
+                _, rec, _ = L.oracle_pframe(cur[b], near[b], far[b], w, bh, navail, qp, tcs[b])
+            far[b], near[b] = near[b], rec
+            px += w * bh
+        nf += 1
+        if time.perf_counter() - t0 > seconds:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": round(px / dt / 1e6, 3), "unit": "Mpix/s", "cores": 1, "kind": kind,
+            "sample": "%d P-frames of the same 1280x720 8-band workload, one thread (%.1f s)" % (nf, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--mode", default="diamond", choices=["diamond", "exhaustive", "learned"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this encode-analysis path runs on one GPU; --gpus %d is not supported "
+              "(independent streams or bands would shard across GPUs with no exchange)" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+
+    from fdv_amd import content
+    from fdv_amd import gpu as G
+
+    W, H, QP, BAND_MBROWS = 1280, 720, 24, 6
+    layout = bands(H, BAND_MBROWS)
+    nsteps = args.warmup + args.steps
+    nframes = min(nsteps + 1, 60)
+    scene = content.scene_pan(W, H, seed=2)
+    frames = [scene.frame(i) for i in range(nframes)]
+    mode = {"diamond": G.ME_DIAMOND, "exhaustive": G.ME_EXHAUSTIVE, "learned": G.ME_LEARNED}[args.mode]
+
+    an = G.Analyser([(W, bh) for _, bh in layout], nsrc=nframes)
+    per_band = [band_frames(f, W, H, layout) for f in frames]
+    for b in range(len(layout)):
+        an.put_ref(b, 0, per_band[0][b])
+        for i in range(1, nframes):
+            an.put_source(b, i, per_band[i][b])
+    nb = len(layout)
+
+    def step(i):
+        src = 1 + (i % (nframes - 1))
+        navail = 1 if i == 0 else 2
+        an.pframe([src] * nb, [QP] * nb, [1.0] * nb, [navail] * nb, [0] * nb, mode)
+        an.fetch()
+        return an.timings()
+
+    for i in range(args.warmup):
+        step(i)
+    kt = []
+    t0 = time.perf_counter()
+    for i in range(args.warmup, nsteps):
+        kt.append(step(i))
+    dt = time.perf_counter() - t0
+    kt = np.array(kt)
+
+    px_frame = W * H
+    value = args.steps * px_frame / dt / 1e6
+    k_ms = float(kt[:, 1].mean())                  # the MB wavefront kernel
+    bytes_per_px = 6.0                              # 1.5 B x (source + 2 references + recon)
+    algo_bytes = bytes_per_px * px_frame
+    achieved = algo_bytes / (k_ms * 1e-3) / 1e9
+    roof = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
+            "kernel": "k_analyse", "kernel_ms": round(k_ms, 4),
+            "bytes_per_launch": int(algo_bytes)}
+    stage_ms = {"motion_tables": round(float(kt[:, 0].mean()), 4), "wavefront": round(k_ms, 4),
+                "deblock": round(float(kt[:, 2].mean()), 4), "serialise": round(float(kt[:, 3].mean()), 4)}
+    cpu = None if args.no_cpu else cpu_baseline(frames[:min(nframes, 9)], W, layout, QP, args.cpu_seconds)
+    an.close()
+    out = {
+        "metric": METRIC, "value": round(value, 3), "unit": "Mpix/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt / args.steps * 1e3, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u8",
+        "data": "synthetic (seeded pan + objects + grain)",
+        "config": {"workload": "1280x720 4:2:0 P-frames, QP 24, 8 tiles (6-MB-row bands), "
+                               "%s motion search" % args.mode,
+                   "frame": [W, H], "bands": len(layout), "qp": QP, "search": args.mode},
+        "stage_ms": stage_ms,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

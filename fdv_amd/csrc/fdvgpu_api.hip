@@ -1,0 +1,390 @@
+// fdvgpu_api.hip -- host side of the C ABI declared in include/fdvgpu.h.
+//
+// Device memory per sequence (sized for the frame; 288 GB of HBM holds thousands):
+//   3 padded frames (fdv.h plane layout: stride = align32(w + 64), h + 64 rows,
+//     data at (32, 32)) rotating as near / far / output reference,
+//   nsrc tight I420 source slots,
+//   per-MB records (decision + levels), handoff words, stream counts.
+// One HIP stream (default priority) per context; every frame step is the same
+// fixed launch sequence, so it can be captured into a graph by a caller.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include "fdvgpu_common.h"
+#include "../../include/fdvgpu.h"
+
+extern "C" int fdvg_set_tables(const int32_t *c8, const int32_t *qs8);
+extern "C" int fdvg_launch_analyse(const fdvg_seq *, const int2 *, int, unsigned *, fdvg_mbrec *,
+                                   uint64_t *, hipStream_t);
+extern "C" int fdvg_launch_deblock(const void *, int, int, hipStream_t);
+extern "C" int fdvg_launch_border(const void *, int, int, hipStream_t);
+extern "C" int fdvg_launch_pad(const void *, const uint8_t *const *, int, int, hipStream_t);
+extern "C" int fdvg_launch_zero(uint32_t *, int, hipStream_t);
+extern "C" int fdvg_launch_symbols(const void *, int, int, const fdvg_mbrec *, uint32_t *, uint32_t *,
+                                   uint32_t *, uint8_t *, size_t, hipStream_t);
+extern "C" int fdvg_launch_metab(const fdvg_seq *, int, int, void *, hipStream_t);
+
+static thread_local char g_err[256];
+static void seterr(const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+}
+extern "C" const char *fdvgpu_last_error(void) { return g_err; }
+
+#define CK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { \
+    seterr("%s: %s (%s:%d)", #x, hipGetErrorString(e_), __FILE__, __LINE__); return -1; } } while (0)
+#define CKN(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { \
+    seterr("%s: %s (%s:%d)", #x, hipGetErrorString(e_), __FILE__, __LINE__); goto fail; } } while (0)
+
+struct SeqMem {
+    int w, h, cw, ch, ys, cs, mw, mh, nmb, mb0;
+    size_t ysz, csz, fsz;          // padded Y plane bytes, chroma plane bytes, frame bytes
+    uint8_t *frames;               // 3 padded frames
+    int near_, far_, out_;
+    uint8_t *src;                  // nsrc tight frames
+    size_t srcsz;
+    int has_given, tag;
+};
+
+struct fdvgpu {
+    int nseq, nsrc, nrows, total_mb, maxmb, maxpx;
+    SeqMem *S;
+    hipStream_t st;
+    fdvg_seq *seq_d, *seq_h;        // seq_h: RING staging copies, one per frame in flight
+    int2 *rowtab_d;
+    unsigned *ticket_d;
+    fdvg_mbrec *recs_d;
+    uint64_t *hand_d;
+    fdvgpu_mbinfo *info_d, *given_d;
+    void *metab_d;
+    PlaneRef *planes_d, *planes_h;  // planes_h: RING staging copies
+    hipEvent_t ring_ev[8];
+    int ring_pos;
+    SymSeq *sym_d;
+    uint32_t *cnt_d, *totals_d, *bases_d, *bases_h;
+    uint8_t *compact_d, *compact_h;
+    size_t compact_cap;
+    uint32_t *flags_h;
+    hipEvent_t ev[5];
+    int have_frame, fetched;
+};
+
+static uint8_t *fy(const SeqMem &m, int f) { return m.frames + (size_t)f * m.fsz + (size_t)FDVG_BORDER * m.ys + FDVG_BORDER; }
+static uint8_t *fu(const SeqMem &m, int f) { return m.frames + (size_t)f * m.fsz + m.ysz + (size_t)FDVG_BORDER * m.cs + FDVG_BORDER; }
+static uint8_t *fv(const SeqMem &m, int f) { return m.frames + (size_t)f * m.fsz + m.ysz + m.csz + (size_t)FDVG_BORDER * m.cs + FDVG_BORDER; }
+
+// 8x8 DCT basis and quantiser steps, from libm exactly as the reference builds
+// them (c8_init, fdv.h:2597-2612).
+static void tables8(int32_t c8[64], int32_t qs8[52]) {
+    const double pi = 3.14159265358979323846;
+    for (int u = 0; u < 8; ++u) {
+        double a = u ? sqrt(2.0 / 8.0) : sqrt(1.0 / 8.0);
+        for (int x = 0; x < 8; ++x) c8[u * 8 + x] = (int32_t)lround(a * cos((2 * x + 1) * u * pi / 16.0) * 512.0);
+    }
+    for (int u = 1; u < 8; ++u) {
+        int32_t s = 0;
+        for (int x = 0; x < 8; ++x) s += c8[u * 8 + x];
+        c8[u * 8] -= s;
+    }
+    for (int q = 0; q < 52; ++q) qs8[q] = (int32_t)lround(262144.0 * (0.625 * pow(2.0, q / 6.0)));
+}
+
+extern "C" void fdvgpu_close(fdvgpu *g) {
+    if (!g) return;
+    if (g->S) for (int s = 0; s < g->nseq; ++s) { (void)hipFree(g->S[s].frames); (void)hipFree(g->S[s].src); }
+    (void)hipFree(g->seq_d); (void)hipHostFree(g->seq_h); (void)hipFree(g->rowtab_d); (void)hipFree(g->ticket_d);
+    (void)hipFree(g->recs_d); (void)hipFree(g->hand_d); (void)hipFree(g->info_d); (void)hipFree(g->given_d);
+    (void)hipFree(g->metab_d); (void)hipFree(g->planes_d); (void)hipHostFree(g->planes_h); (void)hipFree(g->sym_d);
+    (void)hipFree(g->cnt_d); (void)hipFree(g->totals_d); (void)hipFree(g->bases_d); (void)hipHostFree(g->bases_h);
+    (void)hipFree(g->compact_d); (void)hipHostFree(g->compact_h); (void)hipHostFree(g->flags_h);
+    for (int i = 0; i < 5; ++i) if (g->ev[i]) (void)hipEventDestroy(g->ev[i]);
+    for (int i = 0; i < 8; ++i) if (g->ring_ev[i]) (void)hipEventDestroy(g->ring_ev[i]);
+    if (g->st) (void)hipStreamDestroy(g->st);
+    free(g->S);
+    free(g);
+}
+
+extern "C" fdvgpu *fdvgpu_open(int nseq, const int *w, const int *h, int nsrc) {
+    if (nseq <= 0 || !w || !h || nsrc <= 0) { seterr("fdvgpu_open: bad arguments"); return NULL; }
+    for (int s = 0; s < nseq; ++s)
+        if (w[s] <= 0 || h[s] <= 0 || (w[s] & 15) || (h[s] & 15) || w[s] > 8192 || h[s] > 8192) {
+            seterr("fdvgpu_open: sequence %d has size %dx%d (multiples of 16, at most 8192)", s, w[s], h[s]);
+            return NULL;
+        }
+    fdvgpu *g = (fdvgpu *)calloc(1, sizeof *g);
+    if (!g) { seterr("out of host memory"); return NULL; }
+    g->nseq = nseq; g->nsrc = nsrc;
+    g->S = (SeqMem *)calloc((size_t)nseq, sizeof(SeqMem));
+    int maxmh = 0;
+    int2 *rowtab_h = NULL;
+    SymSeq *sym_h = NULL;
+    int32_t c8[64], qs8[52];
+    if (!g->S) { seterr("out of host memory"); goto fail; }
+    for (int s = 0; s < nseq; ++s) {
+        SeqMem &m = g->S[s];
+        m.w = w[s]; m.h = h[s]; m.cw = m.w / 2; m.ch = m.h / 2;
+        m.ys = (m.w + 2 * FDVG_BORDER + 31) & ~31;
+        m.cs = (m.cw + 2 * FDVG_BORDER + 31) & ~31;
+        m.mw = m.w / 16; m.mh = m.h / 16; m.nmb = m.mw * m.mh;
+        m.ysz = (size_t)m.ys * (m.h + 2 * FDVG_BORDER);
+        m.csz = (size_t)m.cs * (m.ch + 2 * FDVG_BORDER);
+        m.fsz = (m.ysz + 2 * m.csz + 255) & ~(size_t)255;
+        m.srcsz = ((size_t)m.w * m.h + 2 * (size_t)m.cw * m.ch + 255) & ~(size_t)255;
+        m.mb0 = g->total_mb;
+        g->total_mb += m.nmb;
+        g->nrows += m.mh;
+        if (m.mh > maxmh) maxmh = m.mh;
+        if (m.nmb > g->maxmb) g->maxmb = m.nmb;
+        if ((int)((size_t)m.ys * (m.h + 2 * FDVG_BORDER)) > g->maxpx) g->maxpx = (int)((size_t)m.ys * (m.h + 2 * FDVG_BORDER));
+        m.near_ = 0; m.far_ = 1; m.out_ = 2;
+        CKN(hipMalloc(&m.frames, 3 * m.fsz));
+        CKN(hipMemset(m.frames, 0, 3 * m.fsz));
+        CKN(hipMalloc(&m.src, (size_t)nsrc * m.srcsz));
+        g->compact_cap += (size_t)m.nmb * 2048 + 4096;
+    }
+    CKN(hipStreamCreate(&g->st));
+    for (int i = 0; i < 5; ++i) CKN(hipEventCreate(&g->ev[i]));
+    for (int i = 0; i < 8; ++i) CKN(hipEventCreateWithFlags(&g->ring_ev[i], hipEventDisableTiming));
+    CKN(hipMalloc(&g->seq_d, nseq * sizeof(fdvg_seq)));
+    CKN(hipHostMalloc(&g->seq_h, 8 * nseq * sizeof(fdvg_seq), hipHostMallocDefault));
+    CKN(hipMalloc(&g->rowtab_d, g->nrows * sizeof(int2)));
+    CKN(hipMalloc(&g->ticket_d, 64));
+    CKN(hipMalloc(&g->recs_d, (size_t)g->total_mb * sizeof(fdvg_mbrec)));
+    CKN(hipMalloc(&g->hand_d, (size_t)g->total_mb * FDVG_HWORDS * sizeof(uint64_t)));
+    CKN(hipMemset(g->hand_d, 0, (size_t)g->total_mb * FDVG_HWORDS * sizeof(uint64_t)));
+    CKN(hipMalloc(&g->info_d, (size_t)g->total_mb * sizeof(fdvgpu_mbinfo)));
+    CKN(hipMalloc(&g->given_d, (size_t)g->total_mb * sizeof(fdvgpu_mbinfo)));
+    CKN(hipMalloc(&g->metab_d, (size_t)g->total_mb * sizeof(fdvg_metab)));
+    CKN(hipMemset(g->metab_d, 0, (size_t)g->total_mb * sizeof(fdvg_metab)));
+    CKN(hipMalloc(&g->planes_d, 3 * nseq * sizeof(PlaneRef)));
+    CKN(hipHostMalloc(&g->planes_h, 8 * 3 * nseq * sizeof(PlaneRef), hipHostMallocDefault));
+    CKN(hipMalloc(&g->sym_d, nseq * sizeof(SymSeq)));
+    CKN(hipMalloc(&g->cnt_d, (size_t)g->total_mb * FDVG_NS * sizeof(uint32_t)));
+    CKN(hipMalloc(&g->totals_d, (size_t)nseq * FDVG_NS * sizeof(uint32_t)));
+    CKN(hipMalloc(&g->bases_d, ((size_t)nseq * FDVG_NS + 1) * sizeof(uint32_t)));
+    CKN(hipHostMalloc(&g->bases_h, ((size_t)nseq * FDVG_NS + 1) * sizeof(uint32_t), hipHostMallocDefault));
+    CKN(hipMalloc(&g->compact_d, g->compact_cap));
+    CKN(hipHostMalloc(&g->compact_h, g->compact_cap, hipHostMallocDefault));
+    CKN(hipHostMalloc(&g->flags_h, 64, hipHostMallocDefault));
+
+    // rows in (row, sequence) order: every sequence's row r before any row r+1
+    rowtab_h = (int2 *)malloc(g->nrows * sizeof(int2));
+    sym_h = (SymSeq *)malloc(nseq * sizeof(SymSeq));
+    if (!rowtab_h || !sym_h) { seterr("out of host memory"); goto fail; }
+    {
+        int k = 0;
+        for (int r = 0; r < maxmh; ++r)
+            for (int s = 0; s < nseq; ++s)
+                if (r < g->S[s].mh) rowtab_h[k++] = make_int2(s, r);
+        for (int s = 0; s < nseq; ++s) sym_h[s] = SymSeq{g->S[s].mb0, g->S[s].mw, g->S[s].mh, 0};
+    }
+    CKN(hipMemcpy(g->rowtab_d, rowtab_h, g->nrows * sizeof(int2), hipMemcpyHostToDevice));
+    CKN(hipMemcpy(g->sym_d, sym_h, nseq * sizeof(SymSeq), hipMemcpyHostToDevice));
+    free(rowtab_h); rowtab_h = NULL;
+    free(sym_h); sym_h = NULL;
+    tables8(c8, qs8);
+    if (fdvg_set_tables(c8, qs8) != 0) { seterr("could not load the 8x8 tables"); goto fail; }
+    return g;
+fail:
+    free(rowtab_h); free(sym_h);
+    fdvgpu_close(g);
+    return NULL;
+}
+
+static int seq_ok(fdvgpu *g, int seq) {
+    if (!g || seq < 0 || seq >= g->nseq) { seterr("bad context or sequence index"); return 0; }
+    return 1;
+}
+
+extern "C" int fdvgpu_put_source(fdvgpu *g, int seq, int slot, const uint8_t *i420) {
+    if (!seq_ok(g, seq) || slot < 0 || slot >= g->nsrc || !i420) { seterr("fdvgpu_put_source: bad arguments"); return -1; }
+    SeqMem &m = g->S[seq];
+    CK(hipMemcpyAsync(m.src + (size_t)slot * m.srcsz, i420, (size_t)m.w * m.h + 2 * (size_t)m.cw * m.ch,
+                      hipMemcpyHostToDevice, g->st));
+    CK(hipStreamSynchronize(g->st));
+    return 0;
+}
+
+extern "C" int fdvgpu_put_ref(fdvgpu *g, int seq, int which, const uint8_t *i420) {
+    if (!seq_ok(g, seq) || (which != 0 && which != 1) || !i420) { seterr("fdvgpu_put_ref: bad arguments"); return -1; }
+    SeqMem &m = g->S[seq];
+    int f = which == 0 ? m.near_ : m.far_;
+    size_t tsz = (size_t)m.w * m.h + 2 * (size_t)m.cw * m.ch;
+    uint8_t *tmp = NULL;
+    uint8_t **srcs_d = NULL;
+    PlaneRef *pl_d = NULL;
+    CK(hipMalloc(&tmp, tsz));
+    PlaneRef pl[3] = {{fy(m, f), m.w, m.h, m.ys, 0}, {fu(m, f), m.cw, m.ch, m.cs, 0}, {fv(m, f), m.cw, m.ch, m.cs, 0}};
+    const uint8_t *srcs[3] = {tmp, tmp + (size_t)m.w * m.h, tmp + (size_t)m.w * m.h + (size_t)m.cw * m.ch};
+    int rc = -1;
+    if (hipMalloc(&srcs_d, sizeof srcs) != hipSuccess || hipMalloc(&pl_d, sizeof pl) != hipSuccess) {
+        seterr("fdvgpu_put_ref: device allocation failed");
+    } else if (hipMemcpy(tmp, i420, tsz, hipMemcpyHostToDevice) != hipSuccess ||
+               hipMemcpy(srcs_d, srcs, sizeof srcs, hipMemcpyHostToDevice) != hipSuccess ||
+               hipMemcpy(pl_d, pl, sizeof pl, hipMemcpyHostToDevice) != hipSuccess) {
+        seterr("fdvgpu_put_ref: copy failed");
+    } else if (fdvg_launch_pad(pl_d, srcs_d, 3, (int)(m.ysz), g->st) != 0 ||
+               hipStreamSynchronize(g->st) != hipSuccess) {
+        seterr("fdvgpu_put_ref: padding kernel failed");
+    } else rc = 0;
+    (void)hipFree(tmp); (void)hipFree(srcs_d); (void)hipFree(pl_d);
+    return rc;
+}
+
+extern "C" int fdvgpu_set_given(fdvgpu *g, int seq, const fdvgpu_mbinfo *dec) {
+    if (!seq_ok(g, seq)) return -1;
+    SeqMem &m = g->S[seq];
+    if (!dec) { m.has_given = 0; return 0; }
+    for (int i = 0; i < m.nmb; ++i) {
+        const fdvgpu_mbinfo &d = dec[i];
+        int nq = d.mode == 3 ? 4 : 1;
+        if (d.mode > 3 || d.ref > 1 || d.merge_idx > 2 || d.sub > 3) { seterr("given decision %d out of range", i); return -1; }
+        for (int k = 0; k < nq; ++k)
+            if (d.mv[k][0] < -FDVG_LIM - 3 || d.mv[k][0] > FDVG_LIM + 3 || d.mv[k][1] < -FDVG_LIM - 3 || d.mv[k][1] > FDVG_LIM + 3) {
+                seterr("given vector of MB %d outside the search range", i); return -1;
+            }
+    }
+    CK(hipMemcpy(g->given_d + m.mb0, dec, (size_t)m.nmb * sizeof(fdvgpu_mbinfo), hipMemcpyHostToDevice));
+    m.has_given = 1;
+    return 0;
+}
+
+extern "C" int fdvgpu_pframe(fdvgpu *g, const int *src_slot, const int *qp, const double *lscale,
+                             const int *navail, const int *force_skip, int me_mode) {
+    if (!g || !src_slot || !qp || !lscale || !navail) { seterr("fdvgpu_pframe: bad arguments"); return -1; }
+    if (me_mode < FDVGPU_ME_DIAMOND || me_mode > FDVGPU_ME_LEARNED) { seterr("fdvgpu_pframe: bad me_mode"); return -1; }
+    for (int s = 0; s < g->nseq; ++s) {
+        if (src_slot[s] < 0 || src_slot[s] >= g->nsrc || qp[s] < 0 || qp[s] > 51 ||
+            navail[s] < 1 || navail[s] > 2 || !(lscale[s] > 0.0)) {
+            seterr("fdvgpu_pframe: bad parameters for sequence %d", s); return -1;
+        }
+    }
+    CK(hipEventRecord(g->ev[0], g->st));
+    // staging slot for this frame's descriptors: the copy of the frame 8 steps
+    // back must have run before the slot is rewritten
+    const int slot = g->ring_pos;
+    g->ring_pos = (g->ring_pos + 1) % 8;
+    CK(hipEventSynchronize(g->ring_ev[slot]));
+    fdvg_seq *seq_h = g->seq_h + (size_t)slot * g->nseq;
+    PlaneRef *planes_h = g->planes_h + (size_t)slot * 3 * g->nseq;
+    for (int s = 0; s < g->nseq; ++s) {
+        SeqMem &m = g->S[s];
+        fdvg_seq &q = seq_h[s];
+        memset(&q, 0, sizeof q);
+        const uint8_t *src = m.src + (size_t)src_slot[s] * m.srcsz;
+        q.cy = src; q.cu = src + (size_t)m.w * m.h; q.cv = q.cu + (size_t)m.cw * m.ch;
+        const int r1 = navail[s] >= 2 ? m.far_ : m.near_;
+        q.ry[0] = fy(m, m.near_); q.ru[0] = fu(m, m.near_); q.rv[0] = fv(m, m.near_);
+        q.ry[1] = fy(m, r1);      q.ru[1] = fu(m, r1);      q.rv[1] = fv(m, r1);
+        q.oy = fy(m, m.out_); q.ou = fu(m, m.out_); q.ov = fv(m, m.out_);
+        q.given = m.has_given ? (const fdvg_mbinfo *)(g->given_d + m.mb0) : NULL;
+        q.metab = me_mode != FDVGPU_ME_DIAMOND ? (const fdvg_metab *)g->metab_d + m.mb0 : NULL;
+        q.info = (fdvg_mbinfo *)(g->info_d + m.mb0);
+        // lambda = lscale * 0.425 * 2^((qp-12)/3), in the reference's order (fdv.h:5804)
+        q.lambda = lscale[s] * 0.425 * pow(2.0, (qp[s] - 12) / 3.0);
+        q.w = m.w; q.h = m.h; q.ys = m.ys; q.cs = m.cs;
+        q.navail = navail[s]; q.qp = qp[s];
+        q.me_mode = me_mode == FDVGPU_ME_DIAMOND ? FDVG_ME_DIAMOND : FDVG_ME_TABLE;
+        q.force_skip = force_skip ? force_skip[s] : 0;
+        q.mb0 = m.mb0;
+        m.tag = m.tag % 255 + 1;
+        q.tag = m.tag;
+        PlaneRef *pl = &planes_h[3 * s];
+        pl[0] = PlaneRef{q.oy, m.w, m.h, m.ys, qp[s]};
+        pl[1] = PlaneRef{q.ou, m.cw, m.ch, m.cs, qp[s]};
+        pl[2] = PlaneRef{q.ov, m.cw, m.ch, m.cs, qp[s]};
+    }
+    CK(hipMemcpyAsync(g->seq_d, seq_h, g->nseq * sizeof(fdvg_seq), hipMemcpyHostToDevice, g->st));
+    CK(hipMemcpyAsync(g->planes_d, planes_h, 3 * g->nseq * sizeof(PlaneRef), hipMemcpyHostToDevice, g->st));
+    CK(hipEventRecord(g->ring_ev[slot], g->st));
+    if (me_mode != FDVGPU_ME_DIAMOND &&
+        fdvg_launch_metab(g->seq_d, g->nseq, g->maxmb, g->metab_d, g->st) != 0) {
+        seterr("motion table launch failed"); return -1;
+    }
+    if (fdvg_launch_zero(g->ticket_d, 2, g->st) != 0) { seterr("ticket reset launch failed"); return -1; }
+    CK(hipEventRecord(g->ev[1], g->st));
+    if (fdvg_launch_analyse(g->seq_d, g->rowtab_d, g->nrows, g->ticket_d, g->recs_d, g->hand_d, g->st) != 0) {
+        seterr("analysis launch failed: %s", hipGetErrorString(hipGetLastError())); return -1;
+    }
+    CK(hipEventRecord(g->ev[2], g->st));
+    if (fdvg_launch_deblock(g->planes_d, 3 * g->nseq, g->maxpx, g->st) != 0 ||
+        fdvg_launch_border(g->planes_d, 3 * g->nseq, g->maxpx, g->st) != 0) {
+        seterr("deblock launch failed"); return -1;
+    }
+    CK(hipEventRecord(g->ev[3], g->st));
+    if (fdvg_launch_symbols(g->sym_d, g->nseq, g->maxmb, g->recs_d, g->cnt_d, g->totals_d, g->bases_d,
+                            g->compact_d, g->compact_cap, g->st) != 0) {
+        seterr("serialiser launch failed"); return -1;
+    }
+    CK(hipEventRecord(g->ev[4], g->st));
+    for (int s = 0; s < g->nseq; ++s) {
+        SeqMem &m = g->S[s];
+        int old_far = m.far_;
+        m.far_ = m.near_; m.near_ = m.out_; m.out_ = old_far;
+        m.has_given = 0;
+    }
+    g->have_frame = 1;
+    g->fetched = 0;
+    return 0;
+}
+
+extern "C" int fdvgpu_fetch(fdvgpu *g) {
+    if (!g || !g->have_frame) { seterr("fdvgpu_fetch: no frame analysed"); return -1; }
+    if (g->fetched) return 0;
+    const int n = g->nseq * FDVG_NS;
+    CK(hipMemcpyAsync(g->bases_h, g->bases_d, (n + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, g->st));
+    CK(hipMemcpyAsync(g->flags_h, g->ticket_d, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, g->st));
+    CK(hipStreamSynchronize(g->st));
+    if (g->flags_h[1]) { seterr("MB wavefront: a row waited too long for the row above"); return -1; }
+    size_t total = g->bases_h[n];
+    if (total > g->compact_cap) { seterr("symbol streams overflowed (%zu bytes)", total); return -1; }
+    if (total) {
+        CK(hipMemcpyAsync(g->compact_h, g->compact_d, total, hipMemcpyDeviceToHost, g->st));
+        CK(hipStreamSynchronize(g->st));
+    }
+    g->fetched = 1;
+    return 0;
+}
+
+extern "C" int fdvgpu_streams(fdvgpu *g, int seq, const uint8_t **sv, size_t *sn) {
+    if (!seq_ok(g, seq) || !sv || !sn) return -1;
+    if (fdvgpu_fetch(g) != 0) return -1;
+    for (int k = 0; k < FDVG_NS; ++k) {
+        uint32_t a = g->bases_h[seq * FDVG_NS + k], b = g->bases_h[seq * FDVG_NS + k + 1];
+        sv[k] = g->compact_h + a;
+        sn[k] = b - a;
+    }
+    return 0;
+}
+
+extern "C" int fdvgpu_recon(fdvgpu *g, int seq, uint8_t *i420) {
+    if (!seq_ok(g, seq) || !i420) return -1;
+    SeqMem &m = g->S[seq];
+    CK(hipMemcpy2DAsync(i420, m.w, fy(m, m.near_), m.ys, m.w, m.h, hipMemcpyDeviceToHost, g->st));
+    uint8_t *u = i420 + (size_t)m.w * m.h, *v = u + (size_t)m.cw * m.ch;
+    CK(hipMemcpy2DAsync(u, m.cw, fu(m, m.near_), m.cs, m.cw, m.ch, hipMemcpyDeviceToHost, g->st));
+    CK(hipMemcpy2DAsync(v, m.cw, fv(m, m.near_), m.cs, m.cw, m.ch, hipMemcpyDeviceToHost, g->st));
+    CK(hipStreamSynchronize(g->st));
+    return 0;
+}
+
+extern "C" int fdvgpu_decisions(fdvgpu *g, int seq, fdvgpu_mbinfo *out) {
+    if (!seq_ok(g, seq) || !out) return -1;
+    SeqMem &m = g->S[seq];
+    CK(hipMemcpyAsync(out, g->info_d + m.mb0, (size_t)m.nmb * sizeof(fdvgpu_mbinfo), hipMemcpyDeviceToHost, g->st));
+    CK(hipStreamSynchronize(g->st));
+    return 0;
+}
+
+extern "C" int fdvgpu_timings(fdvgpu *g, float ms[4]) {
+    if (!g || !ms || !g->have_frame) return -1;
+    CK(hipEventSynchronize(g->ev[4]));
+    for (int i = 0; i < 4; ++i) CK(hipEventElapsedTime(&ms[i], g->ev[i], g->ev[i + 1]));
+    return 0;
+}
+

@@ -1,0 +1,85 @@
+// fdvgpu_common.h -- device-side layouts shared by the encode-analysis kernels
+// and their host driver.  Plain C structs only: the C-ABI never exposes them.
+//
+// Terms follow the reference codec (yhbsh/fdv include/fdv.h): a *sequence* is one
+// independently coded picture stream -- a whole video, or one band of a
+// tile-parallel video (fdv.h:7092-7138) -- and its frames are split into 16x16
+// macroblocks (MB) coded in raster order.
+#pragma once
+#include <stdint.h>
+
+#define FDVG_BORDER 32          // replicated border of every padded plane (fdv.h:217)
+#define FDVG_RANGE 16           // integer search radius in pels (fdv.h:5169)
+#define FDVG_LIM (FDVG_RANGE * 4)
+#define FDVG_NS 16              // coded streams per P-frame (fdv.h:6198)
+
+// Stream indices (fdv.h:4170 structure models, then fdv_cw order 618-627).
+enum { SG_REF = 4, SG_MVX = 5, SG_MVY = 6, SG_SUB = 7, SG_MERGE = 8, SG_CN = 9,
+       SG_CL = 10, SG_CN8 = 11, SG_CL8 = 12, SG_FL = 13, SG_CNC = 14, SG_CLC = 15 };
+
+// Motion-search flavours.
+enum { FDVG_ME_DIAMOND = 0,    // the reference's EPZS diamond, run inside the wavefront
+       FDVG_ME_TABLE = 1 };    // per-MB vectors precomputed block-parallel (exhaustive
+                               // search, or a learned proposal refined on the GPU)
+
+// One MB decision as supplied by a caller (given-decisions mode) or reported back.
+typedef struct {
+    uint8_t mode, ref, merge_idx, sub;   // mode: 0 SKIP 1 INTER16 2 INTRA 3 INTER8
+    int16_t mv[4][2];
+} fdvg_mbinfo;
+
+// Precomputed motion for one MB: 16x16 vector and SAD per reference, the four
+// 8x8 quadrant vectors on reference 0, and a skip hint (learned proposer only).
+typedef struct {
+    int16_t mv16[2][2];
+    int32_t sad16[2];
+    int16_t mv8[4][2];
+    int32_t sad8[4];
+    int32_t flags;      // bit0: the proposer is confident the MB is SKIP
+} fdvg_metab;
+
+// What the wavefront hands to the serialiser for one MB: the decision and the
+// winning candidate's quantised levels in symbolic form (region flags + levels).
+// Luma region r (raster 8x8 of the MB) keeps its levels at lev[r*64..]: four
+// raster 4x4 blocks of 16 (flag 1) or one raster 8x8 (flag 2).  Chroma plane p
+// keeps clev[p*64..] in the same region layout.
+typedef struct {
+    uint8_t mode, ref, merge_idx, ncand;
+    uint8_t sub, ckind, pad0, pad1;   // ckind: 0 none(SKIP) 1 any-flag 4x4 (INTER16)
+                                      //        2 region flag (INTRA) 3 flagless 4x4 (INTER8)
+    uint8_t lflag[4];
+    uint8_t cflag[2];
+    uint8_t pad2[2];
+    int16_t mvd[4][2];                // coded vector differences (INTER16: [0]; INTER8: 4)
+    int16_t mv[4][2];                 // the vectors themselves
+    int16_t lev[256];
+    int16_t clev[128];
+} fdvg_mbrec;
+
+// One sequence's frame job.
+typedef struct {
+    const uint8_t *cy, *cu, *cv;       // tight source planes (w, w/2 strides)
+    const uint8_t *ry[2], *ru[2], *rv[2];  // padded reference planes, at pixel (0,0)
+    uint8_t *oy, *ou, *ov;             // padded output planes (the next reference)
+    const fdvg_mbinfo *given;          // NULL, or one decision per MB
+    const fdvg_metab *metab;           // used when me_mode == FDVG_ME_TABLE
+    fdvg_mbinfo *info;                 // per-MB decisions out (may be NULL)
+    double lambda;
+    int w, h, ys, cs;                  // luma dims, padded luma / chroma strides
+    int navail, qp, me_mode, force_skip;
+    int mb0;                           // first MB of this sequence in the global MB arrays
+    int tag;                           // 1..255, differs between consecutive frames
+    int pad[2];
+} fdvg_seq;
+
+// Per-MB handoff to the MB row below: the bottom recon rows (16 Y, 8 U, 8 V),
+// the motion record and the mode, packed 7 bytes to a 64-bit word whose top byte
+// is the frame tag.  Written and read with relaxed device-scope atomics, so a
+// reader on any XCD sees either the old frame's word or this frame's whole word.
+#define FDVG_HWORDS 6
+
+// A plane for the block-parallel passes (deblock, borders, padding).
+typedef struct { uint8_t *p; int w, h, stride, qp; } PlaneRef;
+
+// Where one sequence's MBs live, for the serialiser.
+typedef struct { int mb0, mw, mh, pad; } SymSeq;

@@ -1,0 +1,112 @@
+// fdvgpu_dev.h -- device helpers shared by the analysis and motion-table kernels:
+// scan / quantiser tables, wave reductions, and quarter-pel interpolation from
+// LDS windows (reference: yhbsh/fdv include/fdv.h).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+static __device__ __constant__ uint8_t k_SCAN4[16] = {0, 1, 4, 8, 5, 2, 3, 6, 9, 12, 13, 10, 7, 11, 14, 15};
+static __device__ __constant__ uint8_t k_INV4[16] = {0, 1, 5, 6, 2, 4, 7, 12, 3, 8, 11, 13, 9, 10, 14, 15};
+static __device__ __constant__ uint8_t k_INV8[64] = {
+     0,  1,  5,  6, 14, 15, 27, 28,  2,  4,  7, 13, 16, 26, 29, 42,
+     3,  8, 12, 17, 25, 30, 41, 43,  9, 11, 18, 24, 31, 40, 44, 53,
+    10, 19, 23, 32, 39, 45, 52, 54, 20, 22, 33, 38, 46, 51, 55, 60,
+    21, 34, 37, 47, 50, 56, 59, 61, 35, 36, 48, 49, 57, 58, 62, 63};
+static __device__ __constant__ int k_MF[6][3] = {{13107, 5243, 8066}, {11916, 4660, 7490}, {10082, 4194, 6554},
+                                          {9362, 3647, 5825},  {8192, 3355, 5243},  {7282, 2893, 4559}};
+static __device__ __constant__ int k_DV[6][3] = {{10, 16, 13}, {11, 18, 14}, {13, 20, 16},
+                                          {14, 23, 18}, {16, 25, 20}, {18, 29, 23}};
+static __device__ __constant__ int8_t k_LDSP[8][2] = {{0, 8}, {0, -8}, {8, 0}, {-8, 0}, {4, 4}, {4, -4}, {-4, 4}, {-4, -4}};
+static __device__ __constant__ int8_t k_SDSP[4][2] = {{0, 4}, {0, -4}, {4, 0}, {-4, 0}};
+static __device__ __constant__ int8_t k_CROSS[4][2] = {{0, -1}, {0, 1}, {-1, 0}, {1, 0}};
+
+#define WIN 64        // luma window side, origin (bx-24, by-24)
+#define WOFF 24
+#define CWIN 32       // chroma window side, origin (cbx-12, cby-12)
+#define CWOFF 12
+
+// Exact floors of the coded modes, in bits (fdv.h:5887-5896; mode 3, flag 2).
+#define FL_INTRA 14
+#define FL_INTER16 16
+#define FL_INTER8 19
+
+// ------------------------------------------------------------------ helpers
+
+__device__ __forceinline__ int clip8(int v) { return v < 0 ? 0 : (v > 255 ? 255 : v); }
+__device__ __forceinline__ uint32_t zmap(int v) { return ((uint32_t)v << 1) ^ (uint32_t)(v >> 31); }
+__device__ __forceinline__ int egbits(uint32_t v) { return 2 * (31 - __clz((int)(v + 1u))) + 1; }
+__device__ __forceinline__ int leblen(uint32_t u) {
+    return u < (1u << 7) ? 1 : u < (1u << 14) ? 2 : u < (1u << 21) ? 3 : u < (1u << 28) ? 4 : 5;
+}
+__device__ __forceinline__ int pclass(int i) { int r = (i >> 2) & 1, c = i & 1; return r == c ? r : 2; }
+
+template <int W> __device__ __forceinline__ int gsum(int v) {
+#pragma unroll
+    for (int o = 1; o < W; o <<= 1) v += __shfl_xor(v, o);
+    return v;
+}
+template <int W> __device__ __forceinline__ int gmax(int v) {
+#pragma unroll
+    for (int o = 1; o < W; o <<= 1) v = max(v, __shfl_xor(v, o));
+    return v;
+}
+template <int W> __device__ __forceinline__ int gor(int v) {
+#pragma unroll
+    for (int o = 1; o < W; o <<= 1) v |= __shfl_xor(v, o);
+    return v;
+}
+
+// ---- luma / chroma interpolation from LDS windows (H.264 fig. 8-4 naming;
+//      fdv.h:2931-2990 and the block form 3082-3230 it equals)
+__device__ __forceinline__ int tap6(int a, int b, int c, int d, int e, int f) {
+    return a - 5 * b + 20 * c + 20 * d - 5 * e + f;
+}
+__device__ __forceinline__ int wpx(const uint8_t *w, int x, int y) { return w[y * WIN + x]; }
+__device__ __forceinline__ int whr(const uint8_t *w, int x, int y) {
+    const uint8_t *p = w + y * WIN + x;
+    return tap6(p[-2], p[-1], p[0], p[1], p[2], p[3]);
+}
+__device__ __forceinline__ int wvr(const uint8_t *w, int x, int y) {
+    const uint8_t *p = w + y * WIN + x;
+    return tap6(p[-2 * WIN], p[-WIN], p[0], p[WIN], p[2 * WIN], p[3 * WIN]);
+}
+__device__ __forceinline__ int wb(const uint8_t *w, int x, int y) { return clip8((whr(w, x, y) + 16) >> 5); }
+__device__ __forceinline__ int wh(const uint8_t *w, int x, int y) { return clip8((wvr(w, x, y) + 16) >> 5); }
+__device__ __forceinline__ int wj(const uint8_t *w, int x, int y) {
+    int v = tap6(whr(w, x, y - 2), whr(w, x, y - 1), whr(w, x, y), whr(w, x, y + 1),
+                 whr(w, x, y + 2), whr(w, x, y + 3));
+    return clip8((v + 512) >> 10);
+}
+__device__ __forceinline__ int av2(int a, int b) { return (a + b + 1) >> 1; }
+// Predicted luma sample for the MB-relative pixel (x, y) displaced by (mvx, mvy).
+__device__ __forceinline__ int lumaq(const uint8_t *w, int x, int y, int mvx, int mvy) {
+    int X = WOFF + x + (mvx >> 2), Y = WOFF + y + (mvy >> 2);
+    switch (((mvy & 3) << 2) | (mvx & 3)) {
+    case 0:  return wpx(w, X, Y);
+    case 1:  return av2(wpx(w, X, Y), wb(w, X, Y));
+    case 2:  return wb(w, X, Y);
+    case 3:  return av2(wb(w, X, Y), wpx(w, X + 1, Y));
+    case 4:  return av2(wpx(w, X, Y), wh(w, X, Y));
+    case 5:  return av2(wb(w, X, Y), wh(w, X, Y));
+    case 6:  return av2(wb(w, X, Y), wj(w, X, Y));
+    case 7:  return av2(wb(w, X, Y), wh(w, X + 1, Y));
+    case 8:  return wh(w, X, Y);
+    case 9:  return av2(wh(w, X, Y), wj(w, X, Y));
+    case 10: return wj(w, X, Y);
+    case 11: return av2(wj(w, X, Y), wh(w, X + 1, Y));
+    case 12: return av2(wh(w, X, Y), wpx(w, X, Y + 1));
+    case 13: return av2(wh(w, X, Y), wb(w, X, Y + 1));
+    case 14: return av2(wj(w, X, Y), wb(w, X, Y + 1));
+    default: return av2(wh(w, X + 1, Y), wb(w, X, Y + 1));
+    }
+}
+// Bilinear chroma sample for chroma-MB-relative (x, y) with a chroma quarter-pel
+// vector (fdv.h:3260-3283).
+__device__ __forceinline__ int chromaq(const uint8_t *w, int x, int y, int mvx, int mvy) {
+    int X = CWOFF + x + (mvx >> 2), Y = CWOFF + y + (mvy >> 2), fx = mvx & 3, fy = mvy & 3;
+    const uint8_t *p = w + Y * CWIN + X;
+    int top = p[0] * (4 - fx) + p[1] * fx;
+    int bot = p[CWIN] * (4 - fx) + p[CWIN + 1] * fx;
+    return (top * (4 - fy) + bot * fy + 8) >> 4;
+}
+

@@ -1,0 +1,155 @@
+// fdvgpu_metab.hip -- block-parallel motion tables for the exhaustive search mode.
+//
+// The reference's EPZS diamond (fdv.h:3375-3466) starts from the running vector
+// predictor, so it can only run inside the MB wavefront.  The exhaustive search
+// scores every whole-pel vector within +-16 pels (1089 points) and so depends on
+// nothing but the source block and the reference: it runs for all MBs of all
+// sequences at once, before the wavefront, and the wavefront only reads its
+// results.  Ties go to the smaller |dx|+|dy|, then to raster order.  The best
+// point is then refined with the reference's own sub-pel step (half- then
+// quarter-pel four-point cross, unclamped, skipped at SAD 0: fdv.h:3453-3461).
+//
+// One 256-thread workgroup per MB; the source MB and a 64x64 window of each
+// reference are staged in LDS.  Per reference-0 point the four 8x8 quadrant
+// SADs are accumulated separately (the INTER8 searches) and summed for 16x16.
+// SADs use v_sad_u8 on packed bytes, with v_alignbyte for unaligned windows.
+#include <hip/hip_runtime.h>
+#include "fdvgpu_common.h"
+#include "fdvgpu_dev.h"
+
+struct __align__(16) MSmem {
+    uint8_t win[2][WIN * WIN];
+    uint32_t cur32[64];                 // the source MB, 4 words a row
+    uint8_t cur[256];
+    unsigned long long key[6][4];       // per block, per wave: best packed key
+    int32_t sad[2][8];
+    int32_t res[6][3];                  // per block: mvx, mvy, sad after sub-pel
+};
+
+__device__ __forceinline__ uint32_t win_word(const uint8_t *w, int y, int x) {
+    // 4 bytes at an arbitrary x of row y, from two aligned words
+    const uint32_t *r = (const uint32_t *)(w + y * WIN);
+    const int a = x >> 2, sh = x & 3;
+    const uint32_t lo = r[a], hi = r[a + 1];
+    return __builtin_amdgcn_alignbyte(hi, lo, sh);
+}
+
+__device__ __forceinline__ unsigned long long umin64(unsigned long long a, unsigned long long b) { return a < b ? a : b; }
+
+// Refine block b (0,1: 16x16 on ref 0,1; 2..5: 8x8 quadrants on ref 0).
+__device__ void refine(MSmem &s, int &par, int b, int bs, int ox, int oy, const uint8_t *w,
+                       int mx, int my, int best) {
+    for (int st = 2; st >= 1 && best > 0; st >>= 1) {
+        const int t = threadIdx.x, k = t >> 6, tt = t & 63;
+        const int px = mx + k_CROSS[k][0] * st, py = my + k_CROSS[k][1] * st;
+        int acc = 0;
+        const int npx = bs * bs / 64;
+        for (int j = 0; j < npx; ++j) {
+            const int p = tt * npx + j, y = p / bs, x = p % bs;
+            const int d = s.cur[(oy + y) * 16 + ox + x] - lumaq(w, ox + x, oy + y, px, py);
+            acc += d < 0 ? -d : d;
+        }
+        acc = gsum<64>(acc);
+        if (tt == 0) s.sad[par][k] = acc;
+        __syncthreads();
+        const int cx = mx, cy = my;
+        for (int k2 = 0; k2 < 4; ++k2) {
+            const int v = s.sad[par][k2];
+            if (v < best) { best = v; mx = cx + k_CROSS[k2][0] * st; my = cy + k_CROSS[k2][1] * st; }
+        }
+        par ^= 1;
+    }
+    if (threadIdx.x == 0) { s.res[b][0] = mx; s.res[b][1] = my; s.res[b][2] = best; }
+}
+
+extern "C" __global__ void __launch_bounds__(256)
+k_me_exhaustive(const fdvg_seq *__restrict__ seqs, fdvg_metab *__restrict__ out) {
+    __shared__ MSmem s;
+    const fdvg_seq q = seqs[blockIdx.y];
+    const int mw = q.w >> 4, nmb = mw * (q.h >> 4);
+    const int i = blockIdx.x;
+    if (i >= nmb) return;
+    const int tid = threadIdx.x, bx = (i % mw) * 16, by = (i / mw) * 16;
+    const int nr = q.navail;
+    s.cur[tid] = q.cy[(size_t)(by + (tid >> 4)) * q.w + bx + (tid & 15)];
+    for (int r = 0; r < nr; ++r) {
+        const int wy = tid >> 2, wx = (tid & 3) * 16;
+        const uint8_t *src = q.ry[r] + (ptrdiff_t)(by - WOFF + wy) * q.ys + bx - WOFF + wx;
+        *(uint64_t *)&s.win[r][wy * WIN + wx] = *(const uint64_t *)src;
+        *(uint64_t *)&s.win[r][wy * WIN + wx + 8] = *(const uint64_t *)(src + 8);
+    }
+    __syncthreads();
+    if (tid < 64) s.cur32[tid] = ((const uint32_t *)s.cur)[tid];
+    __syncthreads();
+
+    // integer search: thread t takes points t, t+256, ...; packed key
+    // (sad, |dx|+|dy|, raster index) so the minimum is the reference tie order.
+    unsigned long long best[6];
+    for (int b = 0; b < 6; ++b) best[b] = ~0ull;
+    for (int pidx = tid; pidx < 33 * 33; pidx += 256) {
+        const int dy = pidx / 33 - 16, dx = pidx % 33 - 16;
+        const unsigned long long tie = ((unsigned long long)(abs(dx) + abs(dy)) << 16) | (unsigned)pidx;
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            if (r >= nr) break;
+            const uint8_t *w = s.win[r];
+            uint32_t qs[4] = {0, 0, 0, 0};
+#pragma unroll
+            for (int y = 0; y < 16; ++y) {
+                const int wy = WOFF + y + dy, wx = WOFF + dx;
+                const uint32_t r0 = win_word(w, wy, wx), r1 = win_word(w, wy, wx + 4);
+                const uint32_t r2 = win_word(w, wy, wx + 8), r3 = win_word(w, wy, wx + 12);
+                const int qr = (y >> 3) * 2;
+                qs[qr] = __builtin_amdgcn_sad_u8(s.cur32[y * 4 + 0], r0, qs[qr]);
+                qs[qr] = __builtin_amdgcn_sad_u8(s.cur32[y * 4 + 1], r1, qs[qr]);
+                qs[qr + 1] = __builtin_amdgcn_sad_u8(s.cur32[y * 4 + 2], r2, qs[qr + 1]);
+                qs[qr + 1] = __builtin_amdgcn_sad_u8(s.cur32[y * 4 + 3], r3, qs[qr + 1]);
+            }
+            const unsigned s16 = qs[0] + qs[1] + qs[2] + qs[3];
+            best[r] = umin64(best[r], ((unsigned long long)s16 << 32) | tie);
+            if (r == 0)
+#pragma unroll
+                for (int qd = 0; qd < 4; ++qd)
+                    best[2 + qd] = umin64(best[2 + qd], ((unsigned long long)qs[qd] << 32) | tie);
+        }
+    }
+    const int wv = tid >> 6, L = tid & 63;
+#pragma unroll
+    for (int b = 0; b < 6; ++b) {
+        unsigned long long v = best[b];
+        for (int o = 1; o < 64; o <<= 1) {
+            unsigned long long u = __shfl_xor(v, o);
+            v = umin64(v, u);
+        }
+        if (L == 0) s.key[b][wv] = v;
+    }
+    __syncthreads();
+    int par = 0;
+    for (int b = 0; b < 6; ++b) {
+        if (b == 1 && nr < 2) continue;
+        const unsigned long long k = umin64(umin64(s.key[b][0], s.key[b][1]), umin64(s.key[b][2], s.key[b][3]));
+        const int pidx = (int)(k & 0xffff), sad = (int)(k >> 32);
+        const int mx = (pidx % 33 - 16) * 4, my = (pidx / 33 - 16) * 4;
+        const int bs = b < 2 ? 16 : 8;
+        const int ox = b < 2 ? 0 : ((b - 2) & 1) * 8, oy = b < 2 ? 0 : ((b - 2) >> 1) * 8;
+        refine(s, par, b, bs, ox, oy, s.win[b == 1 ? 1 : 0], mx, my, sad);
+    }
+    __syncthreads();
+    if (tid == 0) {
+        fdvg_metab &m = out[q.mb0 + i];
+        for (int r = 0; r < 2; ++r) {
+            const int b = (r == 1 && nr < 2) ? 0 : r;
+            m.mv16[r][0] = (int16_t)s.res[b][0]; m.mv16[r][1] = (int16_t)s.res[b][1]; m.sad16[r] = s.res[b][2];
+        }
+        for (int qd = 0; qd < 4; ++qd) {
+            m.mv8[qd][0] = (int16_t)s.res[2 + qd][0]; m.mv8[qd][1] = (int16_t)s.res[2 + qd][1];
+            m.sad8[qd] = s.res[2 + qd][2];
+        }
+        m.flags = 0;
+    }
+}
+
+extern "C" int fdvg_launch_metab(const fdvg_seq *seqs_d, int nseq, int maxmb, void *metab, hipStream_t st) {
+    hipLaunchKernelGGL(k_me_exhaustive, dim3(maxmb, nseq), dim3(256), 0, st, seqs_d, (fdvg_metab *)metab);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}

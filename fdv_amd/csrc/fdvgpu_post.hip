@@ -1,0 +1,296 @@
+// fdvgpu_post.hip -- the block-parallel work after the MB wavefront:
+//   * in-loop deblocking (fdv.h:3479-3636): all vertical 8x8-grid edges, then all
+//     horizontal ones.  Edges within a pass touch disjoint samples (an edge writes
+//     p0,q0 and reads p1,q1, and edges are 8 apart), so each pass is one launch
+//     with a thread per edge segment;
+//   * border replication of the new reference (plane_extend, fdv.h:1603);
+//   * serialisation of the MB records into the 16 coded streams in raster order
+//     (code_block4 / fdv_coeff8_encode / pframe_encode emission, fdv.h:5268-5277,
+//     2197-2203, 6036-6160, 6180-6193): a count pass, a per-stream exclusive scan,
+//     and a write pass.
+#include <hip/hip_runtime.h>
+#include "fdvgpu_common.h"
+
+__device__ __constant__ uint8_t p_SCAN4[16] = {0, 1, 4, 8, 5, 2, 3, 6, 9, 12, 13, 10, 7, 11, 14, 15};
+__device__ __constant__ uint8_t p_SCAN8[64] = {
+     0,  1,  8, 16,  9,  2,  3, 10, 17, 24, 32, 25, 18, 11,  4,  5,
+    12, 19, 26, 33, 40, 48, 41, 34, 27, 20, 13,  6,  7, 14, 21, 28,
+    35, 42, 49, 56, 57, 50, 43, 36, 29, 22, 15, 23, 30, 37, 44, 51,
+    58, 59, 52, 45, 38, 31, 39, 46, 53, 60, 61, 54, 47, 55, 62, 63};
+
+// ------------------------------------------------------------------ deblock
+
+
+__device__ __forceinline__ int clipb(int v) { return v < 0 ? 0 : (v > 255 ? 255 : v); }
+
+__device__ __forceinline__ void edge(uint8_t *q, int st, int al, int be, int tc) {
+    int p1 = q[-2 * st], p0 = q[-st], q0 = q[0], q1 = q[st];
+    if (abs(p0 - q0) < al && abs(p1 - p0) < be && abs(q1 - q0) < be) {
+        int d = ((q0 - p0) * 4 + (p1 - q1) + 4) >> 3;
+        d = d < -tc ? -tc : (d > tc ? tc : d);
+        q[-st] = (uint8_t)clipb(p0 + d);
+        q[0] = (uint8_t)clipb(q0 - d);
+    }
+}
+
+// pass 0: thread per (row, vertical edge); pass 1: thread per (horizontal edge, column).
+extern "C" __global__ void __launch_bounds__(256)
+k_deblock(const PlaneRef *__restrict__ planes, int pass) {
+    const PlaneRef P = planes[blockIdx.y];
+    const int al = 6 + P.qp, be = 2 + P.qp / 4, tc = 1 + P.qp / 10;
+    const int ne_v = P.w / 8 - 1;                   // vertical edges per row
+    const int ne_h = (P.h - 1) / 8;                 // horizontal edges
+    const int n = pass == 0 ? P.h * (ne_v > 0 ? ne_v : 0) : ne_h * P.w;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        if (pass == 0) {
+            const int y = i / ne_v, e = i % ne_v;
+            edge(P.p + (ptrdiff_t)y * P.stride + 8 * (e + 1), 1, al, be, tc);
+        } else {
+            const int e = i / P.w, x = i % P.w;
+            edge(P.p + (ptrdiff_t)(8 * (e + 1)) * P.stride + x, P.stride, al, be, tc);
+        }
+    }
+}
+
+// Replicate the outermost samples into the FDVG_BORDER-wide border.  Each border
+// sample copies the interior sample at its clamped coordinate, so there is no
+// ordering between rows and columns.
+extern "C" __global__ void __launch_bounds__(256)
+k_border(const PlaneRef *__restrict__ planes) {
+    const PlaneRef P = planes[blockIdx.y];
+    const int B = FDVG_BORDER, fw = P.w + 2 * B;
+    const int ntb = B * fw;                 // one of the top/bottom bands
+    const int n = 2 * ntb + P.h * 2 * B;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        int x, y;
+        if (i < ntb)            { y = -B + i / fw;   x = -B + i % fw; }
+        else if (i < 2 * ntb)   { int j = i - ntb; y = P.h + j / fw; x = -B + j % fw; }
+        else                    { int j = i - 2 * ntb; y = j / (2 * B); int k = j % (2 * B); x = k < B ? -B + k : P.w + (k - B); }
+        const int sx = x < 0 ? 0 : (x >= P.w ? P.w - 1 : x);
+        const int sy = y < 0 ? 0 : (y >= P.h ? P.h - 1 : y);
+        P.p[(ptrdiff_t)y * P.stride + x] = P.p[(ptrdiff_t)sy * P.stride + sx];
+    }
+}
+
+// Copy a padded plane's interior to a tight buffer (recon readback).
+extern "C" __global__ void __launch_bounds__(256)
+k_unpad(const PlaneRef *__restrict__ planes, uint8_t *const *__restrict__ dsts) {
+    const PlaneRef P = planes[blockIdx.y];
+    uint8_t *d = dsts[blockIdx.y];
+    const int n = P.w * P.h;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        d[i] = P.p[(ptrdiff_t)(i / P.w) * P.stride + i % P.w];
+}
+
+// Tight I420 source -> padded planes with borders (a reference loaded from the host).
+extern "C" __global__ void __launch_bounds__(256)
+k_pad(const PlaneRef *__restrict__ planes, const uint8_t *const *__restrict__ srcs) {
+    const PlaneRef P = planes[blockIdx.y];
+    const uint8_t *s = srcs[blockIdx.y];
+    const int B = FDVG_BORDER, fw = P.w + 2 * B, n = fw * (P.h + 2 * B);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int y = i / fw - B, x = i % fw - B;
+        const int sx = x < 0 ? 0 : (x >= P.w ? P.w - 1 : x);
+        const int sy = y < 0 ? 0 : (y >= P.h ? P.h - 1 : y);
+        P.p[(ptrdiff_t)y * P.stride + x] = s[(size_t)sy * P.w + sx];
+    }
+}
+
+// ------------------------------------------------------------------ symbols
+
+
+__device__ __forceinline__ uint32_t zz(int v) { return ((uint32_t)v << 1) ^ (uint32_t)(v >> 31); }
+__device__ __forceinline__ int leb_n(uint32_t u) {
+    return u < (1u << 7) ? 1 : u < (1u << 14) ? 2 : u < (1u << 21) ? 3 : u < (1u << 28) ? 4 : 5;
+}
+
+// A byte sink that either counts (out == nullptr) or writes.
+struct Sink {
+    uint8_t *base[FDVG_NS];
+    uint32_t n[FDVG_NS];
+    uint32_t lim[FDVG_NS];         // writable bytes from base[k] (capacity guard)
+    __device__ __forceinline__ void byte(int k, unsigned v) {
+        if (base[k] && n[k] < lim[k]) base[k][n[k]] = (uint8_t)v;
+        n[k]++;
+    }
+    __device__ __forceinline__ void leb(int k, uint32_t u) {
+        while (u >= 0x80) { byte(k, 0x80 | (u & 0x7f)); u >>= 7; }
+        byte(k, u);
+    }
+};
+
+__device__ void emit4(Sink &S, const int16_t *l, int cn, int cl) {
+    int last = -1;
+    for (int k = 0; k < 16; ++k) if (l[p_SCAN4[k]]) last = k;
+    S.byte(cn, (unsigned)(last + 1));
+    for (int k = 0; k <= last; ++k) S.leb(cl, zz(l[p_SCAN4[k]]));
+}
+__device__ void emit8(Sink &S, const int16_t *l) {
+    int last = -1;
+    for (int k = 0; k < 64; ++k) if (l[p_SCAN8[k]]) last = k;
+    S.byte(SG_CN8, (unsigned)(last + 1));
+    for (int k = 0; k <= last; ++k) S.leb(SG_CL8, zz(l[p_SCAN8[k]]));
+}
+// Luma regions of a coded MB, or one chroma region (fdv.h:5385-5410).
+__device__ void emit_region(Sink &S, int flag, const int16_t *l, int chroma) {
+    S.byte(SG_FL, (unsigned)flag);
+    if (flag == 1) for (int b = 0; b < 4; ++b) emit4(S, l + 16 * b, chroma ? SG_CNC : SG_CN, chroma ? SG_CLC : SG_CL);
+    else if (flag == 2) emit8(S, l);
+}
+
+__device__ void emit_mb(Sink &S, const fdvg_mbrec *R, int ctx) {
+    S.byte(ctx, R->mode);
+    if (R->mode == 0) {
+        if (R->ncand > 1) S.byte(SG_MERGE, R->merge_idx);
+        return;
+    }
+    if (R->mode == 1) {
+        S.byte(SG_REF, R->ref);
+        S.leb(SG_MVX, zz(R->mvd[0][0]));
+        S.leb(SG_MVY, zz(R->mvd[0][1]));
+    } else if (R->mode == 2) {
+        S.byte(SG_SUB, R->sub);
+    } else {
+        for (int q = 0; q < 4; ++q) { S.leb(SG_MVX, zz(R->mvd[q][0])); S.leb(SG_MVY, zz(R->mvd[q][1])); }
+    }
+    for (int r = 0; r < 4; ++r) emit_region(S, R->lflag[r], R->lev + 64 * r, 0);
+    for (int p = 0; p < 2; ++p) {
+        const int16_t *cl = R->clev + 64 * p;
+        if (R->ckind == 1) {
+            S.byte(SG_FL, R->cflag[p]);
+            if (R->cflag[p]) for (int b = 0; b < 4; ++b) emit4(S, cl + 16 * b, SG_CNC, SG_CLC);
+        } else if (R->ckind == 2) {
+            emit_region(S, R->cflag[p], cl, 1);
+        } else {
+            for (int b = 0; b < 4; ++b) emit4(S, cl + 16 * b, SG_CNC, SG_CLC);
+        }
+    }
+}
+
+__device__ __forceinline__ int mb_ctx(const fdvg_mbrec *recs, const SymSeq &q, int mx, int my) {
+    const int mb = q.mb0 + my * q.mw + mx;
+    const int L = mx > 0 ? recs[mb - 1].mode : 0;
+    const int A = my > 0 ? recs[mb - q.mw].mode : 0;
+    return (L ? 2 : 0) + (A ? 1 : 0);
+}
+
+// pass 0: per-MB byte counts into cnt[mb*16 + k]; pass 1: write at off[mb*16 + k].
+// Pass 1 writes stream k of sequence s at compact + bases[s*16 + k] + offset.
+extern "C" __global__ void __launch_bounds__(256)
+k_sym(const SymSeq *__restrict__ seqs, const fdvg_mbrec *__restrict__ recs,
+      uint32_t *__restrict__ cnt, int pass, uint8_t *__restrict__ compact,
+      const uint32_t *__restrict__ bases, uint32_t cap) {
+    const SymSeq q = seqs[blockIdx.y];
+    const int nmb = q.mw * q.mh;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nmb; i += gridDim.x * blockDim.x) {
+        const int mb = q.mb0 + i;
+        Sink S;
+        for (int k = 0; k < FDVG_NS; ++k) {
+            if (pass == 0) { S.base[k] = nullptr; S.n[k] = 0; S.lim[k] = 0; }
+            else {
+                const uint32_t b0 = bases[blockIdx.y * FDVG_NS + k];
+                S.base[k] = compact + b0;
+                S.n[k] = cnt[(size_t)mb * FDVG_NS + k];
+                S.lim[k] = b0 < cap ? cap - b0 : 0;
+            }
+        }
+        emit_mb(S, &recs[mb], mb_ctx(recs, q, i % q.mw, i / q.mw));
+        if (pass == 0) for (int k = 0; k < FDVG_NS; ++k) cnt[(size_t)mb * FDVG_NS + k] = S.n[k];
+    }
+}
+
+// Exclusive scan of one (sequence, stream) column of cnt[], in place; the total
+// goes to totals[seq*16 + stream].  One block per column.
+extern "C" __global__ void __launch_bounds__(256)
+k_scan(const SymSeq *__restrict__ seqs, uint32_t *__restrict__ cnt, uint32_t *__restrict__ totals) {
+    const SymSeq q = seqs[blockIdx.y];
+    const int k = blockIdx.x;
+    const int nmb = q.mw * q.mh;
+    __shared__ uint32_t part[256];
+    __shared__ uint32_t carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int base = 0; base < nmb; base += 256) {
+        const int i = base + threadIdx.x;
+        const uint32_t v = i < nmb ? cnt[(size_t)(q.mb0 + i) * FDVG_NS + k] : 0;
+        part[threadIdx.x] = v;
+        __syncthreads();
+        for (int o = 1; o < 256; o <<= 1) {
+            uint32_t t = threadIdx.x >= o ? part[threadIdx.x - o] : 0;
+            __syncthreads();
+            part[threadIdx.x] += t;
+            __syncthreads();
+        }
+        const uint32_t incl = part[threadIdx.x];
+        if (i < nmb) cnt[(size_t)(q.mb0 + i) * FDVG_NS + k] = carry + incl - v;
+        __syncthreads();
+        if (threadIdx.x == 255) carry += incl;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) totals[blockIdx.y * FDVG_NS + k] = carry;
+}
+
+extern "C" __global__ void k_zero_u32(uint32_t *p, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
+}
+
+// Start of every (sequence, stream) run in the compact output, and the total.
+extern "C" __global__ void k_bases(const uint32_t *__restrict__ totals, uint32_t *__restrict__ bases, int n) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        uint32_t run = 0;
+        for (int i = 0; i < n; ++i) { bases[i] = run; run += totals[i]; }
+        bases[n] = run;
+    }
+}
+
+// ------------------------------------------------------------ host launchers
+
+extern "C" int fdvg_launch_deblock(const void *planes, int nplanes, int maxpx, hipStream_t st) {
+    const PlaneRef *P = (const PlaneRef *)planes;
+    int gx = (maxpx / 8 + 255) / 256;
+    gx = gx < 1 ? 1 : gx > 1024 ? 1024 : gx;
+    hipLaunchKernelGGL(k_deblock, dim3(gx, nplanes), dim3(256), 0, st, P, 0);
+    hipLaunchKernelGGL(k_deblock, dim3(gx, nplanes), dim3(256), 0, st, P, 1);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+extern "C" int fdvg_launch_border(const void *planes, int nplanes, int maxbpx, hipStream_t st) {
+    int gx = (maxbpx + 255) / 256;
+    gx = gx < 1 ? 1 : gx > 1024 ? 1024 : gx;
+    hipLaunchKernelGGL(k_border, dim3(gx, nplanes), dim3(256), 0, st, (const PlaneRef *)planes);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+extern "C" int fdvg_launch_pad(const void *planes, const uint8_t *const *srcs, int nplanes, int maxpx,
+                               hipStream_t st) {
+    int gx = (maxpx + 255) / 256;
+    gx = gx < 1 ? 1 : gx > 2048 ? 2048 : gx;
+    hipLaunchKernelGGL(k_pad, dim3(gx, nplanes), dim3(256), 0, st, (const PlaneRef *)planes, srcs);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+extern "C" int fdvg_launch_unpad(const void *planes, uint8_t *const *dsts, int nplanes, int maxpx,
+                                 hipStream_t st) {
+    int gx = (maxpx + 255) / 256;
+    gx = gx < 1 ? 1 : gx > 2048 ? 2048 : gx;
+    hipLaunchKernelGGL(k_unpad, dim3(gx, nplanes), dim3(256), 0, st, (const PlaneRef *)planes, dsts);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+extern "C" int fdvg_launch_zero(uint32_t *p, int n, hipStream_t st) {
+    hipLaunchKernelGGL(k_zero_u32, dim3((n + 255) / 256), dim3(256), 0, st, p, n);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+extern "C" int fdvg_launch_symbols(const void *symseqs, int nseq, int maxmb, const fdvg_mbrec *recs,
+                                   uint32_t *cnt, uint32_t *totals, uint32_t *bases, uint8_t *compact,
+                                   size_t cap, hipStream_t st) {
+    const uint32_t cap32 = cap > 0xffffffffu ? 0xffffffffu : (uint32_t)cap;
+    const SymSeq *S = (const SymSeq *)symseqs;
+    int gx = (maxmb + 255) / 256;
+    hipLaunchKernelGGL(k_sym, dim3(gx, nseq), dim3(256), 0, st, S, recs, cnt, 0, compact, (const uint32_t *)bases, cap32);
+    hipLaunchKernelGGL(k_scan, dim3(FDVG_NS, nseq), dim3(256), 0, st, S, cnt, totals);
+    hipLaunchKernelGGL(k_bases, dim3(1), dim3(64), 0, st, (const uint32_t *)totals, bases, nseq * FDVG_NS);
+    hipLaunchKernelGGL(k_sym, dim3(gx, nseq), dim3(256), 0, st, S, recs, cnt, 1, compact, (const uint32_t *)bases, cap32);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}

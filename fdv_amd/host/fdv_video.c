@@ -1,0 +1,153 @@
+/* fdv_video.c -- the P-frame encoder on the MI355X path: device analysis through
+ * the fdvgpu C ABI, then host entropy packing, one thread per sequence.
+ *
+ * Reference behaviour mirrored: pframe_encode's contract (fdv.h:5764-6421) and
+ * the near/far reference pool of fdv_video_encode (fdv.h:6749-6754, 6842-6846),
+ * which here lives in device memory and rotates on every P-frame.
+ */
+#include "fdv_amd.h"
+#include "fdvgpu.h"
+
+#include <pthread.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+static _Thread_local char amd_err[256];
+static void amd_seterr(const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(amd_err, sizeof amd_err, fmt, ap);
+    va_end(ap);
+}
+const char *fdvamd_last_error(void) { return amd_err; }
+
+struct fdvamd_encoder {
+    fdvgpu *g;
+    int nseq, me_mode;
+    int *w, *h;
+    uint8_t *tmp;            /* contiguous I420 staging for fdvamd_pframe_encode */
+};
+
+fdvamd_encoder *fdvamd_encoder_open(int nseq, const int *w, const int *h, int me_mode) {
+    if (nseq <= 0 || !w || !h || me_mode < 0 || me_mode > 2) { amd_seterr("bad arguments"); return NULL; }
+    fdvamd_encoder *e = calloc(1, sizeof *e);
+    if (!e) return NULL;
+    e->nseq = nseq; e->me_mode = me_mode;
+    e->w = malloc(nseq * sizeof(int)); e->h = malloc(nseq * sizeof(int));
+    if (!e->w || !e->h) { fdvamd_encoder_close(e); return NULL; }
+    memcpy(e->w, w, nseq * sizeof(int)); memcpy(e->h, h, nseq * sizeof(int));
+    e->g = fdvgpu_open(nseq, w, h, 1);
+    if (!e->g) { amd_seterr("%s", fdvgpu_last_error()); fdvamd_encoder_close(e); return NULL; }
+    return e;
+}
+
+void fdvamd_encoder_close(fdvamd_encoder *e) {
+    if (!e) return;
+    fdvgpu_close(e->g);
+    free(e->w); free(e->h); free(e->tmp);
+    free(e);
+}
+
+int fdvamd_set_ref(fdvamd_encoder *e, int seq, int which, const uint8_t *i420) {
+    if (!e || fdvgpu_put_ref(e->g, seq, which, i420) != 0) {
+        amd_seterr("%s", fdvgpu_last_error()); return -1;
+    }
+    return 0;
+}
+
+typedef struct {
+    fdvamd_encoder *e;
+    const int *qp;
+    fdvamd_tabcache *const *tc;
+    uint8_t *const *out;
+    const size_t *cap;
+    size_t *len;
+    int first, step, rc;
+} packjob;
+
+static void *pack_worker(void *arg) {
+    packjob *j = arg;
+    for (int s = j->first; s < j->e->nseq; s += j->step) {
+        const uint8_t *sv[16];
+        size_t sn[16];
+        if (fdvgpu_streams(j->e->g, s, sv, sn) != 0) { j->len[s] = 0; j->rc = -1; continue; }
+        j->len[s] = fdvamd_pack_pframe(j->qp[s], sv, sn, j->tc ? j->tc[s] : NULL, j->out[s], j->cap[s]);
+        if (!j->len[s]) j->rc = -1;
+    }
+    return NULL;
+}
+
+int fdvamd_pframe_batch(fdvamd_encoder *e, const uint8_t *const *frames, const int *navail,
+                        const int *qp, const double *lscale, const int *force_skip,
+                        fdvamd_tabcache *const *tc, uint8_t *const *out, const size_t *cap,
+                        size_t *len, uint8_t *const *recon) {
+    if (!e || !frames || !navail || !qp || !out || !cap || !len) { amd_seterr("bad arguments"); return -1; }
+    int *slots = calloc((size_t)e->nseq, sizeof(int));
+    double *ls = malloc((size_t)e->nseq * sizeof(double));
+    if (!slots || !ls) { free(slots); free(ls); amd_seterr("out of memory"); return -1; }
+    for (int s = 0; s < e->nseq; ++s) ls[s] = lscale ? lscale[s] : 1.0;
+    int rc = 0;
+    for (int s = 0; s < e->nseq && rc == 0; ++s)
+        if (fdvgpu_put_source(e->g, s, 0, frames[s]) != 0) rc = -1;
+    if (rc == 0 && fdvgpu_pframe(e->g, slots, qp, ls, navail, force_skip, e->me_mode) != 0) rc = -1;
+    if (rc == 0 && fdvgpu_fetch(e->g) != 0) rc = -1;
+    free(slots); free(ls);
+    if (rc != 0) { amd_seterr("%s", fdvgpu_last_error()); return -1; }
+
+    /* host entropy: the sequences are independent, so they pack in parallel */
+    int nt = e->nseq < 16 ? e->nseq : 16;
+    pthread_t tid[16];
+    packjob *jobs = calloc((size_t)nt, sizeof *jobs);
+    if (!jobs) { amd_seterr("out of memory"); return -1; }
+    int spawned[16] = {0};
+    for (int t = 0; t < nt; ++t) {
+        jobs[t] = (packjob){.e = e, .qp = qp, .tc = tc, .out = out, .cap = cap, .len = len,
+                            .first = t, .step = nt, .rc = 0};
+        if (t > 0) spawned[t] = pthread_create(&tid[t], NULL, pack_worker, &jobs[t]) == 0;
+    }
+    pack_worker(&jobs[0]);
+    for (int t = 1; t < nt; ++t) {
+        if (spawned[t]) pthread_join(tid[t], NULL);
+        else pack_worker(&jobs[t]);
+    }
+    for (int t = 0; t < nt; ++t) if (jobs[t].rc) rc = -1;
+    free(jobs);
+    if (recon)
+        for (int s = 0; s < e->nseq; ++s)
+            if (recon[s] && fdvgpu_recon(e->g, s, recon[s]) != 0) { amd_seterr("%s", fdvgpu_last_error()); rc = -1; }
+    if (rc) amd_seterr("a sequence failed to pack");
+    return rc;
+}
+
+size_t fdvamd_pframe_encode(fdvamd_encoder *e, const uint8_t *cy, const uint8_t *cu,
+                            const uint8_t *cv, int w, int h, int navail, int qp,
+                            uint8_t *out, size_t cap, uint8_t *ry, uint8_t *ru, uint8_t *rv,
+                            int force_skip, fdvamd_tabcache *tc, double lscale) {
+    if (!e || w != e->w[0] || h != e->h[0]) { amd_seterr("frame size does not match the encoder"); return 0; }
+    const size_t ys = (size_t)w * h, cs = (size_t)(w / 2) * (h / 2);
+    if (!e->tmp && !(e->tmp = malloc(ys + 2 * cs))) { amd_seterr("out of memory"); return 0; }
+    memcpy(e->tmp, cy, ys); memcpy(e->tmp + ys, cu, cs); memcpy(e->tmp + ys + cs, cv, cs);
+    const uint8_t *fr[1] = {e->tmp};
+    uint8_t *o[1] = {out};
+    size_t len = 0;
+    fdvamd_tabcache *t[1] = {tc};
+    uint8_t *rec = NULL;
+    if (ry || ru || rv) {
+        rec = malloc(ys + 2 * cs);
+        if (!rec) { amd_seterr("out of memory"); return 0; }
+    }
+    uint8_t *recs[1] = {rec};
+    int fs = force_skip;
+    if (fdvamd_pframe_batch(e, fr, &navail, &qp, &lscale, &fs, t, o, &cap, &len, recs) != 0) {
+        free(rec); return 0;
+    }
+    if (rec) {
+        if (ry) memcpy(ry, rec, ys);
+        if (ru) memcpy(ru, rec + ys, cs);
+        if (rv) memcpy(rv, rec + ys + cs, cs);
+        free(rec);
+    }
+    return len;
+}

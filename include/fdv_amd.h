@@ -1,0 +1,82 @@
+/* fdv_amd.h -- host C11 side of the MI355X encode-analysis path.
+ *
+ * The drop-in point is the reference's P-frame encoder call
+ *
+ *   fdv.h:6793   blob = pframe_encode(cy, cu, cv, w, h, 0, h, refs, navail, qp, tmp,
+ *                                     tmp_cap, ry, ru, rv, 0, &tc, 1.0);
+ *   fdv.h:8278   (the streaming encoder's call, same arguments + force_skip, lscale)
+ *
+ * fdvamd_pframe_encode takes the same arguments except the reference frames,
+ * which the encoder keeps resident in device memory (the reference's near / far
+ * pool, fdv.h:6749-6754, 6842-6846): after a key frame the caller installs the
+ * key frame's reconstruction with fdvamd_set_ref, exactly where the reference
+ * calls load_ref (fdv.h:6844).  The output bytes, the reconstruction and the
+ * entropy state `tc` are identical to pframe_encode's.
+ *
+ * fdvamd_pframe_batch codes one frame of many sequences at once -- the bands of
+ * a tile-parallel video (fdv_vtile_encode, fdv.h:7161) or independent streams
+ * -- in a single batched device pass, then packs each sequence's payload on
+ * host threads.
+ *
+ * The entropy coder is the reference's (static rANS with four table strategies
+ * and the adaptive range coder, fdv.h:1747-2143, 6172-6421) restated in C11; it
+ * is exposed as fdvamd_pack_pframe for callers that bring their own streams.
+ * Errors: sizes are 0 and status codes -1, as in fdv.h.
+ */
+#ifndef FDV_AMD_H
+#define FDV_AMD_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Entropy state carried between P-frames; the same layout as fdv_tabcache
+ * (fdv.h:599-612), so a reference encoder's cache can be passed straight in. */
+typedef struct {
+    uint16_t f1[256];
+    uint16_t fs[16][256];
+    int      have;
+    uint32_t hist[16][256];
+    int      hn;
+} fdvamd_tabcache;
+
+/* Entropy-code a P-frame from its 16 symbol streams (fdv.h:6172-6421).
+ * Returns the payload length, or 0 if it does not fit in cap. */
+size_t fdvamd_pack_pframe(int qp, const uint8_t *const sv[16], const size_t sn[16],
+                          fdvamd_tabcache *tc, uint8_t *out, size_t cap);
+
+typedef struct fdvamd_encoder fdvamd_encoder;
+
+/* An encoder for nseq sequences of the given sizes (multiples of 16).
+ * me_mode: 0 the reference's diamond search (bit-exact with pframe_encode),
+ * 1 exhaustive search, 2 learned proposer with exhaustive fallback. */
+fdvamd_encoder *fdvamd_encoder_open(int nseq, const int *w, const int *h, int me_mode);
+void fdvamd_encoder_close(fdvamd_encoder *e);
+const char *fdvamd_last_error(void);
+
+/* Install a reference (tight I420): which = 0 near, 1 far (load_ref, fdv.h:5183). */
+int fdvamd_set_ref(fdvamd_encoder *e, int seq, int which, const uint8_t *i420);
+
+/* pframe_encode for sequence 0 of the encoder (fdv.h:5764): returns the payload
+ * length, or 0 on error.  ry/ru/rv (tight, may be NULL) receive the deblocked
+ * reconstruction; tc (may be NULL) is the entropy state. */
+size_t fdvamd_pframe_encode(fdvamd_encoder *e, const uint8_t *cy, const uint8_t *cu,
+                            const uint8_t *cv, int w, int h, int navail, int qp,
+                            uint8_t *out, size_t cap, uint8_t *ry, uint8_t *ru, uint8_t *rv,
+                            int force_skip, fdvamd_tabcache *tc, double lscale);
+
+/* One frame of every sequence: frames[s] tight I420; out[s]/cap[s] payload
+ * buffers; len[s] receives each payload length (0 on error); recon[s] (may be
+ * NULL) the deblocked reconstruction; tc[s] (may be NULL) entropy states.
+ * Returns 0, or -1 if any sequence failed. */
+int fdvamd_pframe_batch(fdvamd_encoder *e, const uint8_t *const *frames, const int *navail,
+                        const int *qp, const double *lscale, const int *force_skip,
+                        fdvamd_tabcache *const *tc, uint8_t *const *out, const size_t *cap,
+                        size_t *len, uint8_t *const *recon);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -161,3 +161,69 @@ def ref_pframe(cur, r0, r1, w, h, navail, qp, tc=None, lscale=1.0, force_skip=0)
                                  ry.ctypes.data, ru.ctypes.data, rv.ctypes.data, force_skip,
                                  tc.buf.ctypes.data if tc is not None else None, lscale)
     return out[:n].copy(), rec
+
+
+def padded(plane: np.ndarray, w: int, h: int):
+    """A bordered copy of a tight plane in the reference layout (fdv.h:1548):
+    stride align32(w + 64), 32 replicated samples each side.  Returns (buffer,
+    byte offset of pixel (0,0), stride)."""
+    stride = (w + 64 + 31) & ~31
+    buf = np.zeros((h + 64) * stride, dtype=np.uint8)
+    p = np.pad(plane.reshape(h, w), 32, mode="edge")
+    buf.reshape(h + 64, stride)[:, :w + 64] = p
+    return buf, 32 * stride + 32, stride
+
+
+def oracle_analyse(cur, r0, r1, w, h, navail, qp, lscale=1.0, force_skip=0, me_mode=0, given=None):
+    """Oracle analysis of one P-frame: (16 streams as bytes, deblocked recon I420,
+    per-MB MbInfo array).  `given` is an MbInfo array or None."""
+    lib = oracle()
+    cy, cu, cv = split_i420(cur, w, h)
+    keep = []
+    ptrs = [[0, 0], [0, 0], [0, 0]]
+    refs = [r0, r1 if r1 is not None else r0]
+    ys = cs = 0
+    for r in range(2):
+        pl = split_i420(refs[r], w, h)
+        for k, (p, pw, ph) in enumerate(((pl[0], w, h), (pl[1], w // 2, h // 2), (pl[2], w // 2, h // 2))):
+            buf, off, st = padded(p, pw, ph)
+            keep.append(buf)
+            ptrs[k][r] = buf.ctypes.data + off
+            if k == 0:
+                ys = st
+            else:
+                cs = st
+    RY = (C.c_void_p * 2)(*ptrs[0])
+    RU = (C.c_void_p * 2)(*ptrs[1])
+    RV = (C.c_void_p * 2)(*ptrs[2])
+    nmb = (w // 16) * (h // 16)
+    scap = nmb * 1600 + 1024
+    sb = np.zeros(NS * scap, dtype=np.uint8)
+    bufs = (C.c_void_p * NS)(*[sb.ctypes.data + k * scap for k in range(NS)])
+    sn = (C.c_size_t * NS)()
+    rec = np.zeros_like(cur)
+    ry, ru, rv = split_i420(rec, w, h)
+    info = (MbInfo * nmb)()
+    lam = lib.orc_lambda(qp, lscale)
+    rc = lib.orc_pframe_analyse(cy.ctypes.data, cu.ctypes.data, cv.ctypes.data, w, h,
+                                C.addressof(RY), C.addressof(RU), C.addressof(RV), ys, cs,
+                                navail, qp, lam, force_skip, me_mode,
+                                C.addressof(given) if given is not None else None,
+                                ry.ctypes.data, ru.ctypes.data, rv.ctypes.data, C.addressof(info),
+                                C.addressof(bufs), scap, C.addressof(sn))
+    assert rc == 0, "oracle stream overflow"
+    streams = [bytes(sb[k * scap:k * scap + sn[k]]) for k in range(NS)]
+    return streams, rec, info
+
+
+def oracle_pack(qp, streams, tc=None):
+    lib = oracle()
+    arrs = [np.frombuffer(s, dtype=np.uint8) if s else np.zeros(1, dtype=np.uint8) for s in streams]
+    sv = (C.c_void_p * NS)(*[a.ctypes.data for a in arrs])
+    sn = (C.c_size_t * NS)(*[len(s) for s in streams])
+    total = sum(len(s) for s in streams)
+    cap = total * 3 + 65536
+    out = np.zeros(cap, dtype=np.uint8)
+    n = lib.orc_pframe_pack(qp, C.addressof(sv), C.addressof(sn),
+                            tc.buf.ctypes.data if tc is not None else None, out.ctypes.data, cap)
+    return out[:n].tobytes()

@@ -1,0 +1,45 @@
+"""GPU (HIP) encode-analysis path vs the oracle, bit for bit: the 16 symbol
+streams, the deblocked reconstruction and every MB decision."""
+import pytest
+
+from fdv_amd import content
+from fdv_amd import gpu as G
+from gpucases import run_parity
+
+pytestmark = pytest.mark.gpu
+
+
+def test_tiny_diamond():
+    assert run_parity([content.scene_objects(64, 48, 3)], 4, [28], label="tiny") == 0
+
+
+def test_batch_mixed_sizes_diamond():
+    scenes = [content.scene_objects(96, 64, 5, nobj=3, smin=8, smax=32),
+              content.scene_pan(64, 64, 6, nobj=3, smin=8, smax=32),
+              content.scene_blobs(128, 48, 7, nblob=30, vmax=10, grain=2.0)]
+    assert run_parity(scenes, 4, [24, 30, 18], lscales=[1.0, 0.8, 1.2], label="batch") == 0
+
+
+@pytest.mark.parametrize("qp", [0, 12, 40, 51])
+def test_qp_extremes(qp):
+    assert run_parity([content.scene_pan(80, 48, 11 + qp, grain=4.0, nobj=2, smin=8, smax=24)],
+                      3, [qp], label=f"qp{qp}") == 0
+
+
+def test_config1_content():
+    assert run_parity([content.scene_objects(320, 180, 1)], 4, [28], label="cfg1") == 0
+
+
+def test_force_skip():
+    assert run_parity([content.scene_objects(64, 64, 9)], 4, [20], force_skip_at=2, label="hold") == 0
+
+
+def test_given_decisions():
+    scenes = [content.scene_pan(96, 64, 12, grain=3.0, nobj=3, smin=8, smax=32)]
+    assert run_parity(scenes, 4, [24], given=True, label="given") == 0
+
+
+def test_exhaustive_vs_oracle():
+    scenes = [content.scene_objects(96, 64, 13, nobj=3, smin=8, smax=32, vmax=12),
+              content.scene_pan(64, 48, 14, nobj=2, smin=8, smax=24)]
+    assert run_parity(scenes, 3, [26, 32], me_mode=G.ME_EXHAUSTIVE, label="exh") == 0
