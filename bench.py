@@ -96,7 +96,19 @@ def main():
     ap.add_argument("--mode", default="diamond", choices=["diamond", "exhaustive", "learned"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (reads MIB, writes MIB): "
+                         "the calibration for HBM byte counters")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        x = torch.ones(args.copy_probe << 20, dtype=torch.uint8, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            torch.add(x, 0, out=y)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_mib": args.copy_probe, "launches": 3}))
+        return
     if args.gpus != 1:
         print("bench.py: this encode-analysis path runs on one GPU; --gpus %d is not supported "
               "(independent streams or bands would shard across GPUs with no exchange)" % args.gpus,

@@ -14,21 +14,32 @@
 // atomics; the row below spins on them.  No cache flushes are needed: a
 // device-scope load sees either last frame's word (old tag) or this frame's.
 //
-// Inside an MB the four wavefronts work on the four 8x8 residual regions in
-// parallel: 16 lanes per 4x4 block (quantisation + RDOQ) and then 64 lanes per
-// 8x8 transform (int32 forward, int64 inverse), with cross-lane shuffles.
-// Motion search evaluates the 8 large-diamond points at once (32 lanes each)
-// from a 64x64 reference window staged in LDS.
+// Inside an MB the four wavefronts first run the independent searches side by
+// side -- the 16x16 diamond on reference 0, on reference 1, the four chained
+// 8x8 quadrant diamonds, and the intra SATD screen -- each wave-local (v_sad_u8
+// on packed bytes, DPP reductions, scalar control).  The reference evaluates
+// some of these only when earlier candidates leave room (fdv.h:5913, 5950,
+// 5996); the searches are pure functions of the MB and its neighbours, so
+// computing them early changes no result, only latency.  Residual coding then
+// takes the four wavefronts as the four 8x8 regions (fdvgpu_resid.h).
 //
 // All rate-distortion costs are IEEE doubles computed without contraction
 // (-ffp-contract=off) in the reference's operation order.
 #include <hip/hip_runtime.h>
 #include "fdvgpu_common.h"
 #include "fdvgpu_dev.h"
+#include "fdvgpu_resid.h"
 
-__constant__ int32_t c_T8[64];      // 8x8 DCT basis (built on the host from libm)
-__constant__ int32_t c_QS8[52];     // 8x8 quantiser steps
-
+// Sub-pel refinement reads four planes around the whole-pel best vector: G
+// (integer samples), b and h (horizontal / vertical half samples) and j (centre),
+// for block-relative X, Y in [-1, bs] (stride SPN).  Built once per search from
+// an unclipped horizontal-tap plane, they turn every quarter-pel sample into the
+// rounded mean of two lookups (H.264 fig. 8-4; fdv.h:2931-2990).
+#define SPN 18
+struct SubpelWs {
+    uint8_t pl[4 * SPN * SPN];
+    int16_t hr[SPN * (16 + 7)];
+};
 struct __align__(16) Smem {
     uint8_t win[2][WIN * WIN];
     uint8_t cwin[2][2][CWIN * CWIN];
@@ -43,285 +54,184 @@ struct __align__(16) Smem {
     int16_t clev[128];
     uint8_t flagT[4], flagI[4], flagN[4], flag8[4];
     uint8_t cflag[4];
-    int32_t x32[4][64];
-    int64_t x64[4][64];
-    uint8_t recA[4][64], recB[4][64];
-    int32_t re[4], rb[4];
-    int32_t sad[2][8];
+    RegionWs ws;
+    SubpelWs sp[3];                   // half-pel planes of the three searching waves
     int32_t red[2][4][4];
-    int32_t satd[4];
+    int32_t me[8][4];                 // [0],[1]: 16x16 on ref 0/1 (mvx, mvy, sad); [2..5]:
+                                      // quadrants; [6]: intra SATD per mode; [7][0]: quadrant SAD sum
     uint64_t hand[3][FDVG_HWORDS];
     uint8_t hbytes[48];
     uint8_t lcol[32];                 // previous MB's right recon column: Y 16, U 8, V 8
     int32_t row;
 };
 
-// ---- 4x4 transforms across a 16-lane group: lane i holds raster element i.
-//      Forward core rows then columns (fdv.h:2233-2273); inverse with the >>1
-//      half weights and (x+32)>>6 (fdv.h:2246-2292).
-__device__ __forceinline__ int fdct4_lane(int v, int gb, int i) {
-    int r = i >> 2, c = i & 3;
-    int a = __shfl(v, gb + 4 * r), b = __shfl(v, gb + 4 * r + 1);
-    int cc = __shfl(v, gb + 4 * r + 2), d = __shfl(v, gb + 4 * r + 3);
-    int s03 = a + d, d03 = a - d, s12 = b + cc, d12 = b - cc;
-    int t = c == 0 ? s03 + s12 : c == 1 ? 2 * d03 + d12 : c == 2 ? s03 - s12 : d03 - 2 * d12;
-    a = __shfl(t, gb + c); b = __shfl(t, gb + 4 + c); cc = __shfl(t, gb + 8 + c); d = __shfl(t, gb + 12 + c);
-    s03 = a + d; d03 = a - d; s12 = b + cc; d12 = b - cc;
-    return r == 0 ? s03 + s12 : r == 1 ? 2 * d03 + d12 : r == 2 ? s03 - s12 : d03 - 2 * d12;
-}
-__device__ __forceinline__ int idct4_lane(int v, int gb, int i) {
-    int r = i >> 2, c = i & 3;
-    int a = __shfl(v, gb + 4 * r), b = __shfl(v, gb + 4 * r + 1);
-    int cc = __shfl(v, gb + 4 * r + 2), d = __shfl(v, gb + 4 * r + 3);
-    int e0 = a + cc, e1 = a - cc, e2 = (b >> 1) - d, e3 = b + (d >> 1);
-    int t = c == 0 ? e0 + e3 : c == 1 ? e1 + e2 : c == 2 ? e1 - e2 : e0 - e3;
-    a = __shfl(t, gb + c); b = __shfl(t, gb + 4 + c); cc = __shfl(t, gb + 8 + c); d = __shfl(t, gb + 12 + c);
-    e0 = a + cc; e1 = a - cc; e2 = (b >> 1) - d; e3 = b + (d >> 1);
-    int o = r == 0 ? e0 + e3 : r == 1 ? e1 + e2 : r == 2 ? e1 - e2 : e0 - e3;
-    return (o + 32) >> 6;
-}
-__device__ __forceinline__ int quant4(int co, int i, int qp) {          // fdv.h:2430
-    int qb = 15 + qp / 6;
-    long long off = (1 << qb) / 3;
-    long long a = co < 0 ? -(long long)co : co;
-    int v = (int)((a * k_MF[qp % 6][pclass(i)] + off) >> qb);
-    return co < 0 ? -v : v;
-}
-__device__ __forceinline__ int dqscale(int i, int qp) { return k_DV[qp % 6][pclass(i)]; }
+// ---- wave-local motion search ------------------------------------------------
 
-// RDOQ of one 4x4 block held by 16 lanes (fdv.h:2493-2536): truncation points
-// L = last..-1 scored in order, each only where L is -1 or scan position L is
-// nonzero; J = SSD(res, recon of the truncated block) + lambda*bits(L), with
-// bits(L) the running total the reference keeps (exp-Golomb of count and kept
-// levels, minus 8 bits per LEB byte of every dropped position).  Kept when the
-// best so far is negative or J is strictly smaller.
-__device__ __forceinline__ int rdoq4_lane(int lv, int res, int gb, int i, int qp, double lambda) {
-    const int zi = k_INV4[i];
-    const int last = gmax<16>(lv ? zi : -1);
-    const int mask = gor<16>(lv ? (1 << zi) : 0);
-    int bits = egbits((uint32_t)(last + 1)) + gsum<16>(zi <= last ? egbits(zmap(lv)) : 0);
-    const int drop = 8 * leblen(zmap(lv));
-    const int Lmax = gmax<64>(last);
-    const int dv = dqscale(i, qp), sh = 1 << (qp / 6);
-    double bestJ = -1.0;
-    int bestL = last;
-    for (int L = Lmax; L >= -1; --L) {
-        int tr = zi <= L ? lv : 0;
-        int rr = idct4_lane((tr * dv) * sh, gb, i);
-        int d = res - rr;
-        int D = gsum<16>(d * d);
-        bool active = L <= last;
-        if (active && (L < 0 || ((mask >> L) & 1))) {
-            double J = (double)D + lambda * (double)bits;
-            if (bestJ < 0.0 || J < bestJ) { bestJ = J; bestL = L; }
-        }
-        int dL = __shfl(drop, gb + k_SCAN4[L < 0 ? 0 : L]);
-        if (active && L >= 0) bits -= dL;
-    }
-    return zi <= bestL ? lv : 0;
+__device__ __forceinline__ uint32_t win_word(const uint8_t *w, int y, int x) {
+    // 4 bytes at an arbitrary x of window row y, from two aligned words
+    const uint32_t *r = (const uint32_t *)(w + y * WIN);
+    const int a = x >> 2;
+    return __builtin_amdgcn_alignbyte(r[a + 1], r[a], (unsigned)(x & 3));
 }
 
-// ---- residual regions.  Region r (one 8x8: of a luma MB, or a whole chroma
-// plane block) is coded by wavefront r.  kind 0: four RDOQ'd 4x4 against one
-// 8x8, ties to 4x4, empty option costs only the flag (fdv.h:5319-5410);
-// kind 1: four 4x4 without RDOQ behind an any-coefficient flag (chroma of
-// INTER16, fdv.h:5576-5626); kind 2: four RDOQ'd 4x4, no flag (4x4 chroma of
-// INTER8, code_residual n==4, fdv.h:5306-5311).
-// Pixel operands live in LDS at base + off[r] with a common stride; results:
-// recon at recbase + off[r] (stride rst), levels at lev[r*64..], flag[r],
-// s.re[r] (SSD), s.rb[r] (bits).  Every thread of the block must call it.
-__device__ __forceinline__ void code_regions(Smem &s, int kind, int nreg,
-        const uint8_t *cb, int cst, const uint8_t *pb, int pst, uint8_t *rb_, int rst,
-        int o0, int o1, int o2, int o3, int16_t *lev, uint8_t *flag, int qp, double lambda) {
-    const int tid = threadIdx.x, r = tid >> 6, L = tid & 63;
-    const bool live = r < nreg;
-    const int ro = live ? (r == 0 ? o0 : r == 1 ? o1 : r == 2 ? o2 : o3) : o0;
-    const int gb = L & ~15, blk = L >> 4, i = L & 15;
-    // 4x4 path
-    const int y4 = (blk >> 1) * 4 + (i >> 2), x4 = (blk & 1) * 4 + (i & 3);
-    const int c4 = cb[ro + y4 * cst + x4], p4 = pb[ro + y4 * pst + x4];
-    const int res4 = c4 - p4;
-    int lv = quant4(fdct4_lane(res4, gb, i), i, qp);
-    if (kind != 1) lv = rdoq4_lane(lv, res4, gb, i, qp, lambda);
-    const int rr4 = idct4_lane((lv * dqscale(i, qp)) * (1 << (qp / 6)), gb, i);
-    const int rec4 = clip8(p4 + rr4);
-    const int d4 = c4 - rec4;
-    const int zi = k_INV4[i];
-    const int last4 = gmax<16>(lv ? zi : -1);
-    const int eA = gsum<64>(d4 * d4);
-    int bA = 2 + gsum<64>((zi <= last4 ? egbits(zmap(lv)) : 0) + (i == 0 ? egbits((uint32_t)(last4 + 1)) : 0));
-    const int anyA = gmax<64>(lv != 0);
-    s.recA[r][y4 * 8 + x4] = (uint8_t)rec4;
-    if (kind == 0) {
-        // 8x8 path: int32 forward, quant by division, saturating dequant, int64
-        // inverse (fdv.h:2614-2685)
-        const int y8 = L >> 3, x8 = L & 7;
-        const int c8 = cb[ro + y8 * cst + x8], p8 = pb[ro + y8 * pst + x8];
-        s.x32[r][L] = c8 - p8;
-        __syncthreads();
-        int a = 0;
-#pragma unroll
-        for (int k = 0; k < 8; ++k) a += c_T8[y8 * 8 + k] * s.x32[r][k * 8 + x8];
-        __syncthreads();
-        s.x32[r][L] = a;
-        __syncthreads();
-        int Y = 0;
-#pragma unroll
-        for (int k = 0; k < 8; ++k) Y += s.x32[r][y8 * 8 + k] * c_T8[x8 * 8 + k];
-        const int q = c_QS8[qp], f = q / 3;
-        const int aY = Y < 0 ? -Y : Y;
-        int lv8 = (aY + f) / q;
-        lv8 = Y < 0 ? -lv8 : lv8;
-        long long dqv = (long long)lv8 * q;
-        const int dq8 = dqv > 2147483647LL ? 2147483647 : dqv < -2147483648LL ? (-2147483647 - 1) : (int)dqv;
-        __syncthreads();
-        s.x32[r][L] = dq8;
-        __syncthreads();
-        long long B = 0;
-#pragma unroll
-        for (int k = 0; k < 8; ++k) B += (long long)c_T8[k * 8 + y8] * s.x32[r][k * 8 + x8];
-        s.x64[r][L] = B;
-        __syncthreads();
-        long long X = 0;
-#pragma unroll
-        for (int k = 0; k < 8; ++k) X += s.x64[r][y8 * 8 + k] * (long long)c_T8[k * 8 + x8];
-        const int rr8 = (int16_t)((X + (1LL << 35)) >> 36);
-        const int rec8 = clip8(p8 + rr8);
-        const int d8 = c8 - rec8;
-        const int eB = gsum<64>(d8 * d8);
-        const int z8 = k_INV8[L];
-        const int last8 = gmax<64>(lv8 ? z8 : -1);
-        int bB = 2 + egbits((uint32_t)(last8 + 1)) + gsum<64>(z8 <= last8 ? egbits(zmap(lv8)) : 0);
-        if (!anyA) bA = 2;
-        if (last8 < 0) bB = 2;
-        const bool takeA = (double)eA + lambda * (double)bA <= (double)eB + lambda * (double)bB;
-        s.recB[r][L] = (uint8_t)rec8;
-        __syncthreads();
-        if (live) {
-            rb_[ro + y8 * rst + x8] = takeA ? s.recA[r][L] : s.recB[r][L];
-            if (takeA) lev[r * 64 + blk * 16 + i] = (int16_t)lv;
-            else       lev[r * 64 + L] = (int16_t)lv8;
-            if (L == 0) {
-                flag[r] = takeA ? (anyA ? 1 : 0) : (last8 >= 0 ? 2 : 0);
-                s.re[r] = takeA ? eA : eB;
-                s.rb[r] = takeA ? bA : bB;
-            }
-        }
-    } else {
-        __syncthreads();
-        if (live) {
-            rb_[ro + (L >> 3) * rst + (L & 7)] = s.recA[r][L];
-            lev[r * 64 + blk * 16 + i] = (int16_t)lv;
-            if (L == 0) { flag[r] = anyA ? 1 : 0; s.re[r] = eA; s.rb[r] = bA; }
-        }
-    }
-    __syncthreads();
-}
-
-// Block-wide sums of up to four ints; every thread gets the totals.
-__device__ __forceinline__ void bsum4(Smem &s, int &par, int a, int b, int c, int d) {
-    const int w = threadIdx.x >> 6, L = threadIdx.x & 63;
-    a = gsum<64>(a); b = gsum<64>(b); c = gsum<64>(c); d = gsum<64>(d);
-    if (L == 0) { s.red[par][w][0] = a; s.red[par][w][1] = b; s.red[par][w][2] = c; s.red[par][w][3] = d; }
-    __syncthreads();
-}
-
-// ---- motion search on an LDS window.  Points are evaluated 8 at a time, 32
-//      lanes each; the caller applies them in order.
-struct Pts { int x[8], y[8]; int n; };
-
-__device__ __forceinline__ void eval_pts(Smem &s, int &par, const uint8_t *w, const Pts &P,
-                                         int bs, int ox, int oy) {
-    const int t = threadIdx.x, k = t >> 5, tt = t & 31;
-    int acc = 0;
+// SADs of up to 8 whole-pel vectors (quarter-pel units, multiples of 4) for the
+// bs x bs block at (ox, oy) of the MB: 8 lanes per point.  Results are uniform.
+__device__ __forceinline__ void pts_int8(const uint8_t *w, const uint8_t *cur, int bs, int ox, int oy,
+                                         const int *px, const int *py, int *sad) {
+    const int L = threadIdx.x & 63, k = L >> 3, j = L & 7;
     int mx = 0, my = 0;
 #pragma unroll
-    for (int j = 0; j < 8; ++j) if (j == k) { mx = P.x[j]; my = P.y[j]; }
-    if (k < P.n) {
-        if ((mx | my) & 3) {
-            const int npx = bs * bs / 32;
-            for (int j = 0; j < npx; ++j) {
-                int p = tt * npx + j, y = p / bs, x = p % bs;
-                int v = lumaq(w, ox + x, oy + y, mx, my);
-                int d = s.cur[(oy + y) * 16 + ox + x] - v;
-                acc += d < 0 ? -d : d;
-            }
-        } else {
-            const int npx = bs * bs / 32;
-            const int dx = mx >> 2, dy = my >> 2;
-            for (int j = 0; j < npx; ++j) {
-                int p = tt * npx + j, y = p / bs, x = p % bs;
-                int v = w[(WOFF + oy + y + dy) * WIN + WOFF + ox + x + dx];
-                int d = s.cur[(oy + y) * 16 + ox + x] - v;
-                acc += d < 0 ? -d : d;
-            }
+    for (int kk = 0; kk < 8; ++kk) if (kk == k) { mx = px[kk]; my = py[kk]; }
+    const int dx = mx >> 2, dy = my >> 2;
+    uint32_t acc = 0;
+    if (bs == 16) {
+#pragma unroll
+        for (int yy = 0; yy < 2; ++yy) {
+            const int y = 2 * j + yy;
+            const uint32_t *cw = (const uint32_t *)(cur + y * 16);
+            const int Y = WOFF + y + dy, X = WOFF + dx;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) acc = __builtin_amdgcn_sad_u8(cw[q], win_word(w, Y, X + 4 * q), acc);
         }
+    } else {
+        const uint32_t *cw = (const uint32_t *)(cur + (oy + j) * 16 + ox);
+        const int Y = WOFF + oy + j + dy, X = WOFF + ox + dx;
+        acc = __builtin_amdgcn_sad_u8(cw[0], win_word(w, Y, X), acc);
+        acc = __builtin_amdgcn_sad_u8(cw[1], win_word(w, Y, X + 4), acc);
     }
-    acc = gsum<32>(acc);
-    if (tt == 0 && k < 8) s.sad[par][k] = acc;
-    __syncthreads();
+    const int t = oct_sum8((int)acc);
+#pragma unroll
+    for (int kk = 0; kk < 8; ++kk) sad[kk] = __builtin_amdgcn_readlane(t, 8 * kk + 4);
 }
 
-// EPZS diamond + sub-pel cross (fdv.h:3375-3466).  Returns the SAD; (*mx,*my)
-// the quarter-pel vector.  Uniform across the block.
-__device__ int me_diamond(Smem &s, int &par, const uint8_t *w, int bs, int ox, int oy,
+// quarter-pel phase (fy*4+fx) -> two (plane, dx, dy) sources
+__device__ __constant__ int8_t k_PHSRC[16][6] = {
+    {0, 0, 0, 0, 0, 0}, {0, 0, 0, 1, 0, 0}, {1, 0, 0, 1, 0, 0}, {1, 0, 0, 0, 1, 0},
+    {0, 0, 0, 2, 0, 0}, {1, 0, 0, 2, 0, 0}, {1, 0, 0, 3, 0, 0}, {1, 0, 0, 2, 1, 0},
+    {2, 0, 0, 2, 0, 0}, {2, 0, 0, 3, 0, 0}, {3, 0, 0, 3, 0, 0}, {3, 0, 0, 2, 1, 0},
+    {2, 0, 0, 0, 0, 1}, {2, 0, 0, 1, 0, 1}, {3, 0, 0, 1, 0, 1}, {2, 1, 0, 1, 0, 1}};
+
+__device__ __forceinline__ void subpel_planes(const uint8_t *w, SubpelWs &P, int bs, int ox, int oy,
+                                              int dx, int dy) {
+    const int L = threadIdx.x & 63, n = bs + 2, X0 = WOFF + ox + dx - 1, Y0 = WOFF + oy + dy;
+    for (int i = L; i < n * (bs + 7); i += 64) {               // hraw rows Y = -3 .. bs+3
+        const int X = i % n, Y = i / n;
+        P.hr[Y * SPN + X] = (int16_t)whr(w, X0 + X, Y0 - 3 + Y);
+    }
+    wave_sync();
+    for (int i = L; i < n * n; i += 64) {                      // X, Y = -1 .. bs
+        const int X = i % n, Y = i / n, o = Y * SPN + X;
+        const int Wx = X0 + X, Wy = Y0 - 1 + Y;
+        P.pl[o] = w[Wy * WIN + Wx];
+        P.pl[SPN * SPN + o] = (uint8_t)clip8((P.hr[(Y + 2) * SPN + X] + 16) >> 5);
+        P.pl[2 * SPN * SPN + o] = (uint8_t)clip8((wvr(w, Wx, Wy) + 16) >> 5);
+        const int jj = tap6(P.hr[Y * SPN + X], P.hr[(Y + 1) * SPN + X], P.hr[(Y + 2) * SPN + X],
+                            P.hr[(Y + 3) * SPN + X], P.hr[(Y + 4) * SPN + X], P.hr[(Y + 5) * SPN + X]);
+        P.pl[3 * SPN * SPN + o] = (uint8_t)clip8((jj + 512) >> 10);
+    }
+    wave_sync();
+}
+
+// SADs of 4 quarter-pel vectors within 3 quarter pels of the whole-pel vector
+// (ix, iy) the planes were built for: 16 lanes per point, no divergence.
+__device__ __forceinline__ void pts_frac4(const SubpelWs &P, const uint8_t *cur, int bs, int ox, int oy,
+                                          int ix, int iy, const int *px, const int *py, int *sad) {
+    const int L = threadIdx.x & 63, k = L >> 4, j = L & 15;
+    int mx = 0, my = 0;
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) if (kk == k) { mx = px[kk]; my = py[kk]; }
+    const int dxq = mx - ix, dyq = my - iy;
+    const int ph = ((dyq & 3) << 2) | (dxq & 3), bxo = (dxq >> 2) + 1, byo = (dyq >> 2) + 1;
+    const int a0 = k_PHSRC[ph][0] * SPN * SPN + (byo + k_PHSRC[ph][2]) * SPN + bxo + k_PHSRC[ph][1];
+    const int a1 = k_PHSRC[ph][3] * SPN * SPN + (byo + k_PHSRC[ph][5]) * SPN + bxo + k_PHSRC[ph][4];
+    int acc = 0;
+    if (bs == 16) {
+#pragma unroll
+        for (int x = 0; x < 16; ++x) {
+            const int v = (P.pl[a0 + j * SPN + x] + P.pl[a1 + j * SPN + x] + 1) >> 1;
+            const int d = cur[j * 16 + x] - v;
+            acc += d < 0 ? -d : d;
+        }
+    } else {
+        const int y = j >> 1, x0 = (j & 1) * 4;
+#pragma unroll
+        for (int x = 0; x < 4; ++x) {
+            const int v = (P.pl[a0 + y * SPN + x0 + x] + P.pl[a1 + y * SPN + x0 + x] + 1) >> 1;
+            const int d = cur[(oy + y) * 16 + ox + x0 + x] - v;
+            acc += d < 0 ? -d : d;
+        }
+    }
+    acc = row_sum16(acc);
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) sad[kk] = __builtin_amdgcn_readlane(acc, 16 * kk);
+}
+
+// The reference's EPZS diamond + sub-pel cross (fdv_me_search_inner,
+// fdv.h:3375-3466) on one wavefront: (0,0) and the whole-pel-rounded, clamped
+// predictor; large diamonds (8 points, strict improvement, first wins, at most
+// 64 rounds), one small diamond, then the half- and quarter-pel crosses
+// (unclamped, skipped at SAD 0).  Returns the SAD, vector in (*omx, *omy).
+__device__ __noinline__ int me_diamond(const uint8_t *w, const uint8_t *cur, SubpelWs &SP, int bs, int ox, int oy,
                           int pmx, int pmy, int *omx, int *omy) {
     const int lim = FDVG_LIM;
     int px = (pmx >> 2) * 4, py = (pmy >> 2) * 4;
     px = px < -lim ? -lim : px > lim ? lim : px;
     py = py < -lim ? -lim : py > lim ? lim : py;
-    Pts P;
-    P.n = 2; P.x[0] = 0; P.y[0] = 0; P.x[1] = px; P.y[1] = py;
-    eval_pts(s, par, w, P, bs, ox, oy);
-    int best = s.sad[par][0], bxv = 0, byv = 0;
-    if ((px || py) && s.sad[par][1] < best) { best = s.sad[par][1]; bxv = px; byv = py; }
-    par ^= 1;
+    int P[8], Q[8], S[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) { P[k] = 0; Q[k] = 0; }
+    P[1] = px; Q[1] = py;
+    pts_int8(w, cur, bs, ox, oy, P, Q, S);
+    int best = S[0], bxv = 0, byv = 0;
+    if ((px || py) && S[1] < best) { best = S[1]; bxv = px; byv = py; }
     for (int it = 0; it < 64; ++it) {
         const int cx = bxv, cy = byv;
         int valid = 0;
-        P.n = 8;
+#pragma unroll
         for (int k = 0; k < 8; ++k) {
-            int nx = cx + k_LDSP[k][0], ny = cy + k_LDSP[k][1];
-            bool ok = !(nx < -lim || nx > lim || ny < -lim || ny > lim);
-            valid |= ok << k;
-            P.x[k] = ok ? nx : 0; P.y[k] = ok ? ny : 0;
+            const int nx = cx + k_LDSP[k][0], ny = cy + k_LDSP[k][1];
+            const bool ok = !(nx < -lim || nx > lim || ny < -lim || ny > lim);
+            valid |= (int)ok << k;
+            P[k] = ok ? nx : cx; Q[k] = ok ? ny : cy;
         }
-        eval_pts(s, par, w, P, bs, ox, oy);
+        pts_int8(w, cur, bs, ox, oy, P, Q, S);
         int moved = 0;
+#pragma unroll
         for (int k = 0; k < 8; ++k)
-            if (((valid >> k) & 1) && s.sad[par][k] < best) { best = s.sad[par][k]; bxv = P.x[k]; byv = P.y[k]; moved = 1; }
-        par ^= 1;
+            if (((valid >> k) & 1) && S[k] < best) { best = S[k]; bxv = P[k]; byv = Q[k]; moved = 1; }
         if (!moved) break;
     }
     {
         const int cx = bxv, cy = byv;
         int valid = 0;
-        P.n = 4;
-        for (int k = 0; k < 4; ++k) {
-            int nx = cx + k_SDSP[k][0], ny = cy + k_SDSP[k][1];
-            bool ok = !(nx < -lim || nx > lim || ny < -lim || ny > lim);
-            valid |= ok << k;
-            P.x[k] = ok ? nx : 0; P.y[k] = ok ? ny : 0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int nx = k < 4 ? cx + k_SDSP[k & 3][0] : cx, ny = k < 4 ? cy + k_SDSP[k & 3][1] : cy;
+            const bool ok = k < 4 && !(nx < -lim || nx > lim || ny < -lim || ny > lim);
+            valid |= (int)ok << k;
+            P[k] = ok ? nx : cx; Q[k] = ok ? ny : cy;
         }
-        eval_pts(s, par, w, P, bs, ox, oy);
+        pts_int8(w, cur, bs, ox, oy, P, Q, S);
+#pragma unroll
         for (int k = 0; k < 4; ++k)
-            if (((valid >> k) & 1) && s.sad[par][k] < best) { best = s.sad[par][k]; bxv = P.x[k]; byv = P.y[k]; }
-        par ^= 1;
+            if (((valid >> k) & 1) && S[k] < best) { best = S[k]; bxv = P[k]; byv = Q[k]; }
     }
+    const int ix = bxv, iy = byv;
+    if (best > 0) subpel_planes(w, SP, bs, ox, oy, ix >> 2, iy >> 2);
     for (int st = 2; st >= 1 && best > 0; st >>= 1) {
         const int cx = bxv, cy = byv;
-        P.n = 4;
-        for (int k = 0; k < 4; ++k) { P.x[k] = cx + k_CROSS[k][0] * st; P.y[k] = cy + k_CROSS[k][1] * st; }
-        eval_pts(s, par, w, P, bs, ox, oy);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) { P[k] = cx + k_CROSS[k][0] * st; Q[k] = cy + k_CROSS[k][1] * st; }
+        pts_frac4(SP, cur, bs, ox, oy, ix, iy, P, Q, S);
+#pragma unroll
         for (int k = 0; k < 4; ++k)
-            if (s.sad[par][k] < best) { best = s.sad[par][k]; bxv = P.x[k]; byv = P.y[k]; }
-        par ^= 1;
+            if (S[k] < best) { best = S[k]; bxv = P[k]; byv = Q[k]; }
     }
     *omx = bxv; *omy = byv;
     return best;
 }
 
-// ---- intra 16x16 / 8x8 prediction (fdv.h:2851-2911).  top[-1] is the corner.
+// ---- intra 16x16 / 8x8 prediction (fdv.h:2851-2911).
 struct Nb { int ht, hl, dc, a, b, c; };
 __device__ __forceinline__ Nb nb_setup(const uint8_t *top, const uint8_t *left, int tl, int n, int ht, int hl) {
     Nb q; q.ht = ht; q.hl = hl;
@@ -358,6 +268,49 @@ __device__ __forceinline__ int ipred(int m, const Nb &q, const uint8_t *top, con
     return q.dc;
 }
 
+// SATD of the four 16x16 intra sub-modes (mb_satd, fdv.h:4411): one wavefront,
+// lane = (mode, 4x4 block).  Returns mode m's total in sat[m] (uniform).
+__device__ __noinline__ void satd_modes(const uint8_t *cur, const Nb &nb, const uint8_t *top,
+                                           const uint8_t *left, int *sat) {
+    const int L = threadIdx.x & 63, m = L >> 4, b = L & 15, oy = (b >> 2) * 4, ox = (b & 3) * 4;
+    int d[16], t[16], sa = 0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k)
+        d[k] = cur[(oy + (k >> 2)) * 16 + ox + (k & 3)] - ipred(m, nb, top, left, 16, ox + (k & 3), oy + (k >> 2));
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {                     // rows (fdv.h:3803-3822)
+        const int p0 = d[4 * k] + d[4 * k + 1], m0 = d[4 * k] - d[4 * k + 1];
+        const int p1 = d[4 * k + 2] + d[4 * k + 3], m1 = d[4 * k + 2] - d[4 * k + 3];
+        t[4 * k] = p0 + p1; t[4 * k + 1] = m0 + m1; t[4 * k + 2] = p0 - p1; t[4 * k + 3] = m0 - m1;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {                     // columns
+        const int p0 = t[k] + t[4 + k], m0 = t[k] - t[4 + k];
+        const int p1 = t[8 + k] + t[12 + k], m1 = t[8 + k] - t[12 + k];
+        sa += abs(p0 + p1) + abs(m0 + m1) + abs(p0 - p1) + abs(m0 - m1);
+    }
+    sa = row_sum16(sa);
+#pragma unroll
+    for (int mm = 0; mm < 4; ++mm) sat[mm] = __builtin_amdgcn_readlane(sa, 16 * mm);
+}
+
+// ---- residual coding of a whole MB's luma (four regions, one per wavefront)
+//      or of the two chroma blocks (waves 0, 1)
+__device__ __forceinline__ void code_luma(Smem &s, const uint8_t *pred, uint8_t *rec, int16_t *lev,
+                                          uint8_t *flag, int qp, double lambda) {
+    const int w = threadIdx.x >> 6, o = (w >> 1) * 128 + (w & 1) * 8;
+    region_code<0>(s.ws, w, s.cur + o, 16, pred + o, 16, rec + o, 16, lev, flag, qp, lambda);
+    __syncthreads();
+}
+template <int KIND>
+__device__ __forceinline__ void code_chroma(Smem &s, int qp, double lambda) {
+    const int w = threadIdx.x >> 6;
+    if (w < 2)
+        region_code<KIND>(s.ws, w, s.ccur + 64 * w, 8, s.cpred + 64 * w, 8, s.crec + 64 * w, 8,
+                          s.clev, s.cflag, qp, lambda);
+    __syncthreads();
+}
+
 // ---- handoff words
 __device__ __forceinline__ uint64_t hload(const uint64_t *p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -371,18 +324,39 @@ __device__ __forceinline__ int hbyte(const Smem &s, int j, int b) {
 
 // ------------------------------------------------------------------ kernel
 
+// Optional per-phase cycle accounting (thread 0 of every workgroup), compiled in
+// only with -DFDVG_PHASE_PROF for performance investigation.
+#ifdef FDVG_PHASE_PROF
+__device__ unsigned long long g_phase[16];
+#define PH(k) do { if (tid == 0) { long long t_ = clock64(); ph[k] += t_ - ph_last; ph_last = t_; } } while (0)
+extern "C" int fdvg_phase_read(unsigned long long *out) {
+    return hipMemcpyFromSymbol(out, HIP_SYMBOL(g_phase), sizeof g_phase) == hipSuccess ? 0 : -1;
+}
+#else
+#define PH(k) do { } while (0)
+#endif
+
 extern "C" __global__ void __launch_bounds__(256)
 k_analyse(const fdvg_seq *__restrict__ seqs, const int2 *__restrict__ rowtab, int nrows,
           unsigned *ticket, fdvg_mbrec *__restrict__ recs, uint64_t *__restrict__ hand) {
     __shared__ Smem s;
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, wave = tid >> 6;
     int par = 0;
+#ifdef FDVG_PHASE_PROF
+    long long ph[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    long long ph_last = clock64();
+#endif
     for (;;) {
         if (tid == 0) s.row = (int)atomicAdd(ticket, 1u);
         __syncthreads();
         const int row = s.row;
         __syncthreads();
-        if (row >= nrows) return;
+        if (row >= nrows) {
+#ifdef FDVG_PHASE_PROF
+            if (tid == 0) for (int k = 0; k < 8; ++k) atomicAdd(&g_phase[k], (unsigned long long)ph[k]);
+#endif
+            return;
+        }
         const int2 rt = rowtab[row];
         const fdvg_seq q = seqs[rt.x];
         const int my = rt.y, mw = q.w >> 4, cw = q.w >> 1;
@@ -438,9 +412,12 @@ k_analyse(const fdvg_seq *__restrict__ seqs, const int2 *__restrict__ rowtab, in
                 }
             }
             __syncthreads();
+            PH(0);
 
             // ---- neighbours from the row above / the previous MB
             const int ht = my > 0, hl = mx > 0;
+            if (tid < 16) s.hbytes[1 + tid] = ht ? (uint8_t)hbyte(s, 1, tid) : 0;
+            if (tid == 16) s.hbytes[0] = (ht && hl) ? (uint8_t)hbyte(s, 0, 15) : 128;
             // merge candidates (fdv.h:5711-5732)
             int cx_[3] = {pmx, 0, 0}, cy_[3] = {pmy, 0, 0}, cr_[3] = {0, 0, 0}, ncand = 1;
             if (ht) {
@@ -462,11 +439,12 @@ k_analyse(const fdvg_seq *__restrict__ seqs, const int2 *__restrict__ rowtab, in
                     }
                 }
             }
+            __syncthreads();
 
             const fdvg_mbinfo *gv = q.given ? &q.given[mb - q.mb0] : nullptr;
             const fdvg_metab *mt = (q.me_mode == FDVG_ME_TABLE && q.metab) ? &q.metab[mb - q.mb0] : nullptr;
             int mode, bmi = 0, iref = 0, imx = 0, imy = 0, nsub = 0;
-            int qmv[4][2];
+            int qmv[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
             fdvg_mbrec *R = &recs[mb];
 
             if (q.force_skip) {
@@ -485,12 +463,16 @@ k_analyse(const fdvg_seq *__restrict__ seqs, const int2 *__restrict__ rowtab, in
                     const int d = s.cur[tid] - pv[c];
                     e[c] = d * d;
                 }
-                bsum4(s, par, e[0], e[1], e[2], 0);
+                {
+                    const int e0 = wave_sum(e[0]), e1 = wave_sum(e[1]), e2 = wave_sum(e[2]);
+                    if ((tid & 63) == 0) { s.red[par][wave][0] = e0; s.red[par][wave][1] = e1; s.red[par][wave][2] = e2; }
+                }
+                __syncthreads();
                 int Ds = 0;
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
                     if (c >= ncand) continue;
-                    int ec = s.red[par][0][c] + s.red[par][1][c] + s.red[par][2][c] + s.red[par][3][c];
+                    const int ec = s.red[par][0][c] + s.red[par][1][c] + s.red[par][2][c] + s.red[par][3][c];
                     if (gv) { if (gv->mode == 0 && gv->merge_idx == c) { Ds = ec; bmi = c; } }
                     else if (c == 0 || ec < Ds) { Ds = ec; bmi = c; }
                 }
@@ -501,6 +483,34 @@ k_analyse(const fdvg_seq *__restrict__ seqs, const int2 *__restrict__ rowtab, in
                 bool skip_wins = Js <= lambda * (double)FL_INTRA;
                 if (mt && (mt->flags & 1)) skip_wins = true;
                 if (gv) skip_wins = false;
+                PH(1);
+
+                // ---- the independent searches, one per wavefront
+                const Nb nbL = nb_setup(&s.hbytes[1], s.lcol, s.hbytes[0], 16, ht, hl);
+                if (!gv && !skip_wins) {
+                    if (wave == 0 || (wave == 1 && q.navail == 2)) {
+                        int a, b, sd;
+                        if (mt) { a = mt->mv16[wave][0]; b = mt->mv16[wave][1]; sd = mt->sad16[wave]; }
+                        else sd = me_diamond(s.win[wave], s.cur, s.sp[wave], 16, 0, 0, pmx, pmy, &a, &b);
+                        if ((tid & 63) == 0) { s.me[wave][0] = a; s.me[wave][1] = b; s.me[wave][2] = sd; }
+                    } else if (wave == 2) {
+                        int sx = pmx, sy = pmy, ssad = 0;
+                        for (int qd = 0; qd < 4; ++qd) {
+                            int a, b;
+                            if (mt) { a = mt->mv8[qd][0]; b = mt->mv8[qd][1]; ssad += mt->sad8[qd]; }
+                            else ssad += me_diamond(s.win[0], s.cur, s.sp[2], 8, (qd & 1) * 8, (qd >> 1) * 8, sx, sy, &a, &b);
+                            if ((tid & 63) == 0) { s.me[2 + qd][0] = a; s.me[2 + qd][1] = b; }
+                            sx = a; sy = b;
+                        }
+                        if ((tid & 63) == 0) s.me[7][0] = ssad;
+                    } else if (wave == 3) {
+                        int sat[4];
+                        satd_modes(s.cur, nbL, &s.hbytes[1], s.lcol, sat);
+                        if ((tid & 63) == 0) for (int m = 0; m < 4; ++m) s.me[6][m] = sat[m];
+                    }
+                }
+                __syncthreads();
+                PH(2);
 
                 // ---- INTER16 over the references (fdv.h:5909-5943)
                 double Ji = 1e30;
@@ -511,15 +521,13 @@ k_analyse(const fdvg_seq *__restrict__ seqs, const int2 *__restrict__ rowtab, in
                     if (!gv && r > 0 && Jbest <= lambda * (double)FL_INTER16) break;
                     int rmx, rmy, rsad = 0;
                     if (gv) { rmx = gv->mv[0][0]; rmy = gv->mv[0][1]; }
-                    else if (mt) { rmx = mt->mv16[r][0]; rmy = mt->mv16[r][1]; rsad = mt->sad16[r]; }
-                    else rsad = me_diamond(s, par, s.win[r], 16, 0, 0, pmx, pmy, &rmx, &rmy);
+                    else { rmx = s.me[r][0]; rmy = s.me[r][1]; rsad = s.me[r][2]; }
                     s.pred[tid] = (uint8_t)lumaq(s.win[r], tid & 15, tid >> 4, rmx, rmy);
                     __syncthreads();
-                    code_regions(s, 0, 4, s.cur, 16, s.pred, 16, s.recT, 16, 0, 8, 128, 136,
-                                 s.levT, s.flagT, qp, lambda);
+                    code_luma(s, s.pred, s.recT, s.levT, s.flagT, qp, lambda);
                     const int bb = 6 + egbits(zmap(rmx - pmx)) + egbits(zmap(rmy - pmy)) +
-                                   s.rb[0] + s.rb[1] + s.rb[2] + s.rb[3];
-                    const double dd = (double)(s.re[0] + s.re[1] + s.re[2] + s.re[3]);
+                                   s.ws.b[0] + s.ws.b[1] + s.ws.b[2] + s.ws.b[3];
+                    const double dd = (double)(s.ws.e[0] + s.ws.e[1] + s.ws.e[2] + s.ws.e[3]);
                     const double J = dd + lambda * (double)bb;
                     if (J < Jbest) Jbest = J;
                     if (J < Ji) {
@@ -529,74 +537,44 @@ k_analyse(const fdvg_seq *__restrict__ seqs, const int2 *__restrict__ rowtab, in
                         if (tid < 4) s.flagI[tid] = s.flagT[tid];
                     }
                     __syncthreads();
+                    PH(3);
                 }
 
                 // ---- INTRA 16x16: SATD screen, one coding (fdv.h:5950-5988)
                 double Jn = 1e30;
-                const uint8_t *top = &s.hbytes[1];
-                if (tid < 16) s.hbytes[1 + tid] = ht ? (uint8_t)hbyte(s, 1, tid) : 0;
-                if (tid == 16) s.hbytes[0] = (ht && hl) ? (uint8_t)hbyte(s, 0, 15) : 128;
-                __syncthreads();
                 const bool want_n = gv ? gv->mode == 2 : (!skip_wins && Jbest > lambda * (double)FL_INTRA);
                 if (want_n) {
-                    const uint8_t *tp = top, *lft = s.lcol;
-                    const Nb nb = nb_setup(tp, lft, s.hbytes[0], 16, ht, hl);
                     int sel = 0;
                     if (gv) sel = gv->sub;
                     else {
-                        if (tid < 64) {
-                            const int m = tid >> 4, b = tid & 15, oy = (b >> 2) * 4, ox = (b & 3) * 4;
-                            int dd4[16];
-#pragma unroll
-                            for (int k = 0; k < 16; ++k)
-                                dd4[k] = s.cur[(oy + (k >> 2)) * 16 + ox + (k & 3)] -
-                                         ipred(m, nb, tp, lft, 16, ox + (k & 3), oy + (k >> 2));
-                            int t4[16], sa = 0;
-#pragma unroll
-                            for (int k = 0; k < 4; ++k) {
-                                int p0 = dd4[4 * k] + dd4[4 * k + 1], m0 = dd4[4 * k] - dd4[4 * k + 1];
-                                int p1 = dd4[4 * k + 2] + dd4[4 * k + 3], m1 = dd4[4 * k + 2] - dd4[4 * k + 3];
-                                t4[4 * k] = p0 + p1; t4[4 * k + 1] = m0 + m1; t4[4 * k + 2] = p0 - p1; t4[4 * k + 3] = m0 - m1;
-                            }
-#pragma unroll
-                            for (int k = 0; k < 4; ++k) {
-                                int p0 = t4[k] + t4[4 + k], m0 = t4[k] - t4[4 + k];
-                                int p1 = t4[8 + k] + t4[12 + k], m1 = t4[8 + k] - t4[12 + k];
-                                sa += abs(p0 + p1) + abs(m0 + m1) + abs(p0 - p1) + abs(m0 - m1);
-                            }
-                            sa = gsum<16>(sa);
-                            if (b == 0) s.satd[m] = sa;
-                        }
-                        __syncthreads();
                         int sbest = 0; sel = -1;
                         for (int m = 0; m < 4; ++m) {
-                            if (!nn_ok(m, nb)) continue;
-                            if (sel < 0 || s.satd[m] < sbest) { sbest = s.satd[m]; sel = m; }
+                            if (!nn_ok(m, nbL)) continue;
+                            if (sel < 0 || s.me[6][m] < sbest) { sbest = s.me[6][m]; sel = m; }
                         }
                     }
-                    s.pred[tid] = (uint8_t)ipred(sel, nb, tp, lft, 16, tid & 15, tid >> 4);
+                    s.pred[tid] = (uint8_t)ipred(sel, nbL, &s.hbytes[1], s.lcol, 16, tid & 15, tid >> 4);
                     __syncthreads();
-                    code_regions(s, 0, 4, s.cur, 16, s.pred, 16, s.recN, 16, 0, 8, 128, 136,
-                                 s.levN, s.flagN, qp, lambda);
-                    const int bn = 6 + s.rb[0] + s.rb[1] + s.rb[2] + s.rb[3];
-                    Jn = (double)(s.re[0] + s.re[1] + s.re[2] + s.re[3]) + lambda * (double)bn;
+                    code_luma(s, s.pred, s.recN, s.levN, s.flagN, qp, lambda);
+                    const int bn = 6 + s.ws.b[0] + s.ws.b[1] + s.ws.b[2] + s.ws.b[3];
+                    Jn = (double)(s.ws.e[0] + s.ws.e[1] + s.ws.e[2] + s.ws.e[3]) + lambda * (double)bn;
                     nsub = sel;
                     if (Jn < Jbest) Jbest = Jn;
                 }
+                PH(4);
 
-                // ---- INTER8: four chained quadrant searches, the SAD split gate,
-                //      four 8x8 residuals (fdv.h:5996-6033)
+                // ---- INTER8: the quadrant vectors, the SAD split gate, four 8x8
+                //      residuals (fdv.h:5996-6033)
                 double J8 = 1e30;
                 bool want_8 = gv ? gv->mode == 3 : (!skip_wins && Jbest > lambda * (double)FL_INTER8);
                 if (want_8) {
-                    int ssad = 0, sx = pmx, sy = pmy;
+#pragma unroll
                     for (int qd = 0; qd < 4; ++qd) {
-                        int a, b;
-                        if (gv) { a = gv->mv[qd][0]; b = gv->mv[qd][1]; }
-                        else if (mt) { a = mt->mv8[qd][0]; b = mt->mv8[qd][1]; ssad += mt->sad8[qd]; }
-                        else ssad += me_diamond(s, par, s.win[0], 8, (qd & 1) * 8, (qd >> 1) * 8, sx, sy, &a, &b);
-                        qmv[qd][0] = a; qmv[qd][1] = b; sx = a; sy = b;
+                        qmv[qd][0] = gv ? gv->mv[qd][0] : s.me[2 + qd][0];
+                        qmv[qd][1] = gv ? gv->mv[qd][1] : s.me[2 + qd][1];
                     }
+                    const int ssad = s.me[7][0];
+                    PH(5);
                     if (!gv && sad16 > 0 && ssad * 10 >= sad16 * 9) want_8 = false;
                     if (want_8) {
                         const int x = tid & 15, y = tid >> 4, qd = (y >> 3) * 2 + (x >> 3);
@@ -604,16 +582,17 @@ k_analyse(const fdvg_seq *__restrict__ seqs, const int2 *__restrict__ rowtab, in
                         const int vy = qd == 0 ? qmv[0][1] : qd == 1 ? qmv[1][1] : qd == 2 ? qmv[2][1] : qmv[3][1];
                         s.pred[tid] = (uint8_t)lumaq(s.win[0], x, y, vx, vy);
                         __syncthreads();
-                        code_regions(s, 0, 4, s.cur, 16, s.pred, 16, s.rec8, 16, 0, 8, 128, 136,
-                                     s.lev8, s.flag8, qp, lambda);
-                        int b8 = 3 + s.rb[0] + s.rb[1] + s.rb[2] + s.rb[3];
+                        code_luma(s, s.pred, s.rec8, s.lev8, s.flag8, qp, lambda);
+                        int b8 = 3 + s.ws.b[0] + s.ws.b[1] + s.ws.b[2] + s.ws.b[3];
                         int qpx = pmx, qpy = pmy;
+#pragma unroll
                         for (int qd2 = 0; qd2 < 4; ++qd2) {
                             b8 += egbits(zmap(qmv[qd2][0] - qpx)) + egbits(zmap(qmv[qd2][1] - qpy));
                             qpx = qmv[qd2][0]; qpy = qmv[qd2][1];
                         }
-                        J8 = (double)(s.re[0] + s.re[1] + s.re[2] + s.re[3]) + lambda * (double)b8;
+                        J8 = (double)(s.ws.e[0] + s.ws.e[1] + s.ws.e[2] + s.ws.e[3]) + lambda * (double)b8;
                     }
+                    PH(6);
                 }
 
                 // ---- commit: strict < in order SKIP, INTER16, INTRA, INTER8
@@ -645,7 +624,7 @@ k_analyse(const fdvg_seq *__restrict__ seqs, const int2 *__restrict__ rowtab, in
                     s.cpred[tid] = (uint8_t)chromaq(s.cwin[iref][pl], p & 7, p >> 3, imx / 2, imy / 2);
                 }
                 __syncthreads();
-                code_regions(s, 1, 2, s.ccur, 8, s.cpred, 8, s.crec, 8, 0, 64, 0, 0, s.clev, s.cflag, qp, lambda);
+                code_chroma<1>(s, qp, lambda);
                 ckind = 1;
             } else if (mode == 2) {
                 if (tid < 2) {
@@ -664,7 +643,7 @@ k_analyse(const fdvg_seq *__restrict__ seqs, const int2 *__restrict__ rowtab, in
                     s.cpred[tid] = (uint8_t)ipred(nsub, nb, tp, lf, 8, p & 7, p >> 3);
                 }
                 __syncthreads();
-                code_regions(s, 0, 2, s.ccur, 8, s.cpred, 8, s.crec, 8, 0, 64, 0, 0, s.clev, s.cflag, qp, lambda);
+                code_chroma<0>(s, qp, lambda);
                 ckind = 2;
             } else {
                 if (tid < 128) {
@@ -674,7 +653,7 @@ k_analyse(const fdvg_seq *__restrict__ seqs, const int2 *__restrict__ rowtab, in
                     s.cpred[tid] = (uint8_t)chromaq(s.cwin[0][pl], x, y, vx / 2, vy / 2);
                 }
                 __syncthreads();
-                code_regions(s, 2, 2, s.ccur, 8, s.cpred, 8, s.crec, 8, 0, 64, 0, 0, s.clev, s.cflag, qp, lambda);
+                code_chroma<2>(s, qp, lambda);
                 ckind = 3;
             }
 
@@ -743,6 +722,7 @@ k_analyse(const fdvg_seq *__restrict__ seqs, const int2 *__restrict__ rowtab, in
             }
             if (mode != 2) { pmx = nmx; pmy = nmy; }
             __syncthreads();
+            PH(7);
         }
     }
 }

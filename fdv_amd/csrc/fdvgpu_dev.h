@@ -110,3 +110,60 @@ __device__ __forceinline__ int chromaq(const uint8_t *w, int x, int y, int mvx, 
     return (top * (4 - fy) + bot * fy + 8) >> 4;
 }
 
+
+// ---- cross-lane reductions with DPP (no LDS round trip).  On gfx950 row_ror:N
+//      gives lane i the value of lane (i - N) mod 16 of its 16-lane row.
+template <int CTRL> __device__ __forceinline__ int dpp_mov(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
+}
+#define DPP_XOR1 0xB1     // quad_perm [1,0,3,2]
+#define DPP_XOR2 0x4E     // quad_perm [2,3,0,1]
+#define DPP_ROR4 0x124
+#define DPP_ROR8 0x128
+// every lane of each 16-lane row receives the row's sum / max
+__device__ __forceinline__ int row_sum16(int v) {
+    v += dpp_mov<DPP_XOR1>(v); v += dpp_mov<DPP_XOR2>(v);
+    v += dpp_mov<DPP_ROR4>(v); v += dpp_mov<DPP_ROR8>(v);
+    return v;
+}
+__device__ __forceinline__ int row_max16(int v) {
+    v = max(v, dpp_mov<DPP_XOR1>(v)); v = max(v, dpp_mov<DPP_XOR2>(v));
+    v = max(v, dpp_mov<DPP_ROR4>(v)); v = max(v, dpp_mov<DPP_ROR8>(v));
+    return v;
+}
+// sum over each aligned group of 8 lanes; valid in lanes 8g+4 .. 8g+7
+__device__ __forceinline__ int oct_sum8(int v) {
+    v += dpp_mov<DPP_XOR1>(v); v += dpp_mov<DPP_XOR2>(v);
+    v += dpp_mov<DPP_ROR4>(v);
+    return v;
+}
+// wave-uniform totals
+__device__ __forceinline__ int wave_sum(int v) {
+    v = row_sum16(v);
+    return __builtin_amdgcn_readlane(v, 0) + __builtin_amdgcn_readlane(v, 16) +
+           __builtin_amdgcn_readlane(v, 32) + __builtin_amdgcn_readlane(v, 48);
+}
+__device__ __forceinline__ int wave_max(int v) {
+    v = row_max16(v);
+    return max(max(__builtin_amdgcn_readlane(v, 0), __builtin_amdgcn_readlane(v, 16)),
+               max(__builtin_amdgcn_readlane(v, 32), __builtin_amdgcn_readlane(v, 48)));
+}
+// order this wave's LDS writes before its later LDS reads by other lanes
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Scan tables as constant expressions, for indexing register arrays in fully
+// unrolled loops (fdv.h:1699).
+__device__ __forceinline__ constexpr int k_SCAN4c(int k) {
+    return k == 0 ? 0 : k == 1 ? 1 : k == 2 ? 4 : k == 3 ? 8 : k == 4 ? 5 : k == 5 ? 2 : k == 6 ? 3 :
+           k == 7 ? 6 : k == 8 ? 9 : k == 9 ? 12 : k == 10 ? 13 : k == 11 ? 10 : k == 12 ? 7 :
+           k == 13 ? 11 : k == 14 ? 14 : 15;
+}
+__device__ __forceinline__ constexpr int k_INV4c(int i) {
+    return i == 0 ? 0 : i == 1 ? 1 : i == 2 ? 5 : i == 3 ? 6 : i == 4 ? 2 : i == 5 ? 4 : i == 6 ? 7 :
+           i == 7 ? 12 : i == 8 ? 3 : i == 9 ? 8 : i == 10 ? 11 : i == 11 ? 13 : i == 12 ? 9 :
+           i == 13 ? 10 : i == 14 ? 14 : 15;
+}
