@@ -161,3 +161,89 @@ def pack_pframe(qp: int, streams, tc: TabCache | None = None) -> bytes:
     if n == 0:
         raise GpuError("fdvamd_pack_pframe failed")
     return out[:n].tobytes()
+
+
+class Encoder:
+    """The drop-in P-frame encoder (include/fdv_amd.h): device analysis plus host
+    entropy packing, with the near/far reference pool kept on the device and
+    rotated after every P-frame (fdv_video_encode's pool, fdv.h:6842-6846)."""
+
+    def __init__(self, sizes, me_mode: int = ME_DIAMOND):
+        self.L = lib()
+        L = self.L
+        vp = C.c_void_p
+        L.fdvamd_encoder_open.restype = vp
+        L.fdvamd_encoder_open.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_int]
+        L.fdvamd_encoder_close.argtypes = [vp]
+        L.fdvamd_last_error.restype = C.c_char_p
+        L.fdvamd_set_ref.argtypes = [vp, C.c_int, C.c_int, vp]
+        L.fdvamd_pframe_encode.restype = C.c_size_t
+        L.fdvamd_pframe_encode.argtypes = [vp, vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int,
+                                           vp, C.c_size_t, vp, vp, vp, C.c_int, vp, C.c_double]
+        L.fdvamd_pframe_batch.argtypes = [vp] * 11
+        self.sizes = [(int(w), int(h)) for w, h in sizes]
+        n = len(self.sizes)
+        self.h = L.fdvamd_encoder_open(n, _ints([w for w, _ in self.sizes]),
+                                       _ints([h for _, h in self.sizes]), int(me_mode))
+        if not self.h:
+            raise GpuError("fdvamd_encoder_open: " + L.fdvamd_last_error().decode())
+
+    def close(self):
+        if self.h:
+            self.L.fdvamd_encoder_close(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_ref(self, seq, which, frame):
+        f = np.ascontiguousarray(frame, dtype=np.uint8)
+        if self.L.fdvamd_set_ref(self.h, seq, which, f.ctypes.data) != 0:
+            raise GpuError("fdvamd_set_ref: " + self.L.fdvamd_last_error().decode())
+
+    def pframe_encode(self, frame, navail, qp, tc: TabCache | None = None, lscale=1.0, force_skip=0):
+        """One P-frame of sequence 0: (payload bytes, recon I420)."""
+        w, h = self.sizes[0]
+        f = np.ascontiguousarray(frame, dtype=np.uint8)
+        ys, cs = w * h, (w // 2) * (h // 2)
+        cap = ys * 3 + 65536
+        out = np.zeros(cap, np.uint8)
+        rec = np.zeros(ys + 2 * cs, np.uint8)
+        n = self.L.fdvamd_pframe_encode(self.h, f.ctypes.data, f.ctypes.data + ys, f.ctypes.data + ys + cs,
+                                        w, h, int(navail), int(qp), out.ctypes.data, cap,
+                                        rec.ctypes.data, rec.ctypes.data + ys, rec.ctypes.data + ys + cs,
+                                        int(force_skip), tc.buf.ctypes.data if tc is not None else None,
+                                        float(lscale))
+        if n == 0:
+            raise GpuError("fdvamd_pframe_encode: " + self.L.fdvamd_last_error().decode())
+        return out[:n].tobytes(), rec
+
+    def pframe_batch(self, frames, navail, qp, tcs=None, lscale=None, force_skip=None):
+        """One P-frame of every sequence in one device pass: lists of payloads and recons."""
+        n = len(self.sizes)
+        fr = [np.ascontiguousarray(f, dtype=np.uint8) for f in frames]
+        caps = [w * h * 3 + 65536 for w, h in self.sizes]
+        outs = [np.zeros(c, np.uint8) for c in caps]
+        recs = [np.zeros(w * h + 2 * (w // 2) * (h // 2), np.uint8) for w, h in self.sizes]
+        P = C.c_void_p * n
+        lens = (C.c_size_t * n)()
+        ls = (C.c_double * n)(*([1.0] * n if lscale is None else lscale))
+        tcp = P(*[t.buf.ctypes.data for t in tcs]) if tcs is not None else None
+        # every argument array is bound to a name so that it outlives the call
+        fp = P(*[f.ctypes.data for f in fr])
+        nav = _ints(navail)
+        qps = _ints(qp)
+        fsk = _ints(force_skip if force_skip is not None else [0] * n)
+        op = P(*[o.ctypes.data for o in outs])
+        cp = (C.c_size_t * n)(*caps)
+        rp = P(*[r.ctypes.data for r in recs])
+        rc = self.L.fdvamd_pframe_batch(
+            self.h, C.addressof(fp), C.addressof(nav), C.addressof(qps), C.addressof(ls),
+            C.addressof(fsk), C.addressof(tcp) if tcp is not None else None, C.addressof(op),
+            C.addressof(cp), C.addressof(lens), C.addressof(rp))
+        if rc != 0:
+            raise GpuError("fdvamd_pframe_batch: " + self.L.fdvamd_last_error().decode())
+        return [outs[s][:lens[s]].tobytes() for s in range(n)], recs

@@ -1,0 +1,99 @@
+"""The drop-in encoder (include/fdv_amd.h) on the MI355X path against the
+oracle's pframe_encode: identical payload bytes, reconstruction and carried
+entropy state over chains of P-frames, one sequence at a time and batched; and
+the 1280x720 eight-band workload (the reference's tiled encoder with 6-MB-row
+bands, fdv.h:7161) through the analysis path at full size."""
+import numpy as np
+import pytest
+
+import orclib as L
+from fdv_amd import content
+from fdv_amd import gpu as G
+from gpucases import run_parity, say
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle_chain(frames, w, h, qp, lscale=1.0, fs_at=None):
+    tc = L.TabCache()
+    near, far = frames[0], None
+    out = []
+    for f in range(1, len(frames)):
+        navail = 1 if f == 1 else 2
+        pay, rec, _ = L.oracle_pframe(frames[f], near, far, w, h, navail, qp, tc, lscale,
+                                      1 if fs_at == f else 0)
+        out.append((pay.tobytes(), rec, tc.buf.copy()))
+        far, near = near, rec
+    return out
+
+
+def test_pframe_encode_chain_matches_oracle():
+    sc = content.scene_pan(96, 64, 21, nobj=3, smin=8, smax=32)
+    frames = [sc.frame(i) for i in range(6)]
+    want = _oracle_chain(frames, sc.w, sc.h, 26)
+    enc = G.Encoder([(sc.w, sc.h)])
+    enc.set_ref(0, 0, frames[0])
+    tc = G.TabCache()
+    for f in range(1, 6):
+        say(f"  dropin pframe_encode 96x64 frame {f}")
+        pay, rec = enc.pframe_encode(frames[f], 1 if f == 1 else 2, 26, tc)
+        wp, wr, wt = want[f - 1]
+        assert pay == wp, f"frame {f}: payload {len(pay)} B vs oracle {len(wp)} B"
+        assert np.array_equal(rec, wr), f"frame {f}: recon"
+        assert np.array_equal(tc.buf, wt), f"frame {f}: entropy state"
+    enc.close()
+
+
+def test_pframe_batch_mixed_sizes_matches_oracle():
+    scenes = [content.scene_objects(64, 48, 22, nobj=2, smin=8, smax=24),
+              content.scene_pan(128, 64, 23, nobj=3, smin=8, smax=40),
+              content.scene_objects(320, 180, 24)]
+    qps = [20, 32, 28]
+    ls = [1.0, 0.8, 1.0]
+    nfr = 4
+    frames = [[s.frame(i) for i in range(nfr)] for s in scenes]
+    want = [_oracle_chain(frames[k], s.w, s.h, qps[k], ls[k], fs_at=2 if k == 0 else None)
+            for k, s in enumerate(scenes)]
+    enc = G.Encoder([(s.w, s.h) for s in scenes])
+    for k in range(3):
+        enc.set_ref(k, 0, frames[k][0])
+    tcs = [G.TabCache() for _ in scenes]
+    for f in range(1, nfr):
+        say(f"  dropin batch of 3 sequences frame {f}")
+        pays, recs = enc.pframe_batch([frames[k][f] for k in range(3)], [1 if f == 1 else 2] * 3, qps, tcs,
+                                      ls, [1 if (k == 0 and f == 2) else 0 for k in range(3)])
+        for k in range(3):
+            wp, wr, wt = want[k][f - 1]
+            assert pays[k] == wp, f"seq {k} frame {f}: payload"
+            assert np.array_equal(recs[k], wr), f"seq {k} frame {f}: recon"
+            assert np.array_equal(tcs[k].buf, wt), f"seq {k} frame {f}: entropy state"
+    enc.close()
+
+
+class _Band:
+    """One 6-MB-row band of a 1280x720 scene as a sequence of its own."""
+
+    def __init__(self, scene, y0, bh):
+        self.scene, self.y0, self.w, self.h = scene, y0, scene.w, bh
+        self._cache = {}
+
+    def frame(self, i):
+        if i not in self._cache:
+            y, u, v = self.scene.planes(i)
+            y0, bh = self.y0, self.h
+            self._cache[i] = np.concatenate([y[y0:y0 + bh].ravel(), u[y0 // 2:(y0 + bh) // 2].ravel(),
+                                             v[y0 // 2:(y0 + bh) // 2].ravel()])
+        return self._cache[i]
+
+
+def test_720p_eight_bands_full_size():
+    sc = content.scene_pan(1280, 720, 2)
+    bands = [_Band(sc, y0, min(96, 720 - y0)) for y0 in range(0, 720, 96)]
+    assert [b.h for b in bands] == [96] * 7 + [48]
+    assert run_parity(bands, 3, [24] * 8, label="720p-bands") == 0
+
+
+def test_720p_eight_bands_given_reference_decisions():
+    sc = content.scene_pan(1280, 720, 3)
+    bands = [_Band(sc, y0, min(96, 720 - y0)) for y0 in range(0, 720, 96)]
+    assert run_parity(bands, 3, [24] * 8, given=True, label="720p-given") == 0
