@@ -72,6 +72,7 @@ struct fdvgpu {
     uint32_t *flags_h;
     hipEvent_t ev[5];
     int have_frame, fetched;
+    int can_rewind;                 // the last call was fdvgpu_pframe: its rotation can be undone
 };
 
 static uint8_t *fy(const SeqMem &m, int f) { return m.frames + (size_t)f * m.fsz + (size_t)FDVG_BORDER * m.ys + FDVG_BORDER; }
@@ -213,6 +214,7 @@ extern "C" int fdvgpu_put_source(fdvgpu *g, int seq, int slot, const uint8_t *i4
 extern "C" int fdvgpu_put_ref(fdvgpu *g, int seq, int which, const uint8_t *i420) {
     if (!seq_ok(g, seq) || (which != 0 && which != 1) || !i420) { seterr("fdvgpu_put_ref: bad arguments"); return -1; }
     SeqMem &m = g->S[seq];
+    g->can_rewind = 0;
     int f = which == 0 ? m.near_ : m.far_;
     size_t tsz = (size_t)m.w * m.h + 2 * (size_t)m.cw * m.ch;
     uint8_t *tmp = NULL;
@@ -330,6 +332,20 @@ extern "C" int fdvgpu_pframe(fdvgpu *g, const int *src_slot, const int *qp, cons
     }
     g->have_frame = 1;
     g->fetched = 0;
+    g->can_rewind = 1;
+    return 0;
+}
+
+extern "C" int fdvgpu_rewind(fdvgpu *g) {
+    if (!g || !g->can_rewind) { seterr("fdvgpu_rewind: no P-frame to undo"); return -1; }
+    // inverse of the rotation in fdvgpu_pframe: the pre-frame far reference still
+    // sits untouched in the output slot until the next frame writes it
+    for (int s = 0; s < g->nseq; ++s) {
+        SeqMem &m = g->S[s];
+        int out = m.near_;
+        m.near_ = m.far_; m.far_ = m.out_; m.out_ = out;
+    }
+    g->can_rewind = 0;
     return 0;
 }
 

@@ -204,6 +204,22 @@ class Encoder:
         if self.L.fdvamd_set_ref(self.h, seq, which, f.ctypes.data) != 0:
             raise GpuError("fdvamd_set_ref: " + self.L.fdvamd_last_error().decode())
 
+    def set_ref_planes(self, seq, which, frame):
+        """Install a reference from three separate tight planes (the form the
+        reference's encoders hold their reconstruction in)."""
+        w, h = self.sizes[seq]
+        ys, cs = w * h, (w // 2) * (h // 2)
+        f = np.ascontiguousarray(frame, dtype=np.uint8)
+        y, u, v = f[:ys].copy(), f[ys:ys + cs].copy(), f[ys + cs:].copy()
+        self.L.fdvamd_set_ref_planes.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+        if self.L.fdvamd_set_ref_planes(self.h, seq, which, y.ctypes.data, u.ctypes.data, v.ctypes.data) != 0:
+            raise GpuError("fdvamd_set_ref_planes: " + self.L.fdvamd_last_error().decode())
+
+    def rewind(self):
+        self.L.fdvamd_rewind.argtypes = [C.c_void_p]
+        if self.L.fdvamd_rewind(self.h) != 0:
+            raise GpuError("fdvamd_rewind: " + self.L.fdvamd_last_error().decode())
+
     def pframe_encode(self, frame, navail, qp, tc: TabCache | None = None, lscale=1.0, force_skip=0):
         """One P-frame of sequence 0: (payload bytes, recon I420)."""
         w, h = self.sizes[0]

@@ -57,6 +57,18 @@ int fdvamd_set_ref(fdvamd_encoder *e, int seq, int which, const uint8_t *i420) {
     return 0;
 }
 
+int fdvamd_set_ref_planes(fdvamd_encoder *e, int seq, int which,
+                          const uint8_t *y, const uint8_t *u, const uint8_t *v) {
+    if (!e || seq < 0 || seq >= e->nseq || !y || !u || !v) { amd_seterr("bad arguments"); return -1; }
+    const size_t ys = (size_t)e->w[seq] * e->h[seq], cs = (size_t)(e->w[seq] / 2) * (e->h[seq] / 2);
+    uint8_t *t = malloc(ys + 2 * cs);
+    if (!t) { amd_seterr("out of memory"); return -1; }
+    memcpy(t, y, ys); memcpy(t + ys, u, cs); memcpy(t + ys + cs, v, cs);
+    int rc = fdvamd_set_ref(e, seq, which, t);
+    free(t);
+    return rc;
+}
+
 typedef struct {
     fdvamd_encoder *e;
     const int *qp;
@@ -150,4 +162,9 @@ size_t fdvamd_pframe_encode(fdvamd_encoder *e, const uint8_t *cy, const uint8_t 
         free(rec);
     }
     return len;
+}
+
+int fdvamd_rewind(fdvamd_encoder *e) {
+    if (!e || fdvgpu_rewind(e->g) != 0) { amd_seterr("%s", e ? fdvgpu_last_error() : "bad arguments"); return -1; }
+    return 0;
 }

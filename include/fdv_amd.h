@@ -59,6 +59,12 @@ const char *fdvamd_last_error(void);
 /* Install a reference (tight I420): which = 0 near, 1 far (load_ref, fdv.h:5183). */
 int fdvamd_set_ref(fdvamd_encoder *e, int seq, int which, const uint8_t *i420);
 
+/* The same from three tight planes (w x h luma, w/2 x h/2 chroma), as the
+ * reference's encoders hold their reconstruction (ry/ru/rv, fdv.h:6766-6770):
+ * after an I-frame, install its reconstruction as the near reference. */
+int fdvamd_set_ref_planes(fdvamd_encoder *e, int seq, int which,
+                          const uint8_t *y, const uint8_t *u, const uint8_t *v);
+
 /* pframe_encode for sequence 0 of the encoder (fdv.h:5764): returns the payload
  * length, or 0 on error.  ry/ru/rv (tight, may be NULL) receive the deblocked
  * reconstruction; tc (may be NULL) is the entropy state. */
@@ -75,6 +81,13 @@ int fdvamd_pframe_batch(fdvamd_encoder *e, const uint8_t *const *frames, const i
                         const int *qp, const double *lscale, const int *force_skip,
                         fdvamd_tabcache *const *tc, uint8_t *const *out, const size_t *cap,
                         size_t *len, uint8_t *const *recon);
+
+/* Undo the last frame's reference rotation (all sequences) so the frame can be
+ * coded again, e.g. at another QP: the requantising retry of the streaming
+ * encoder (fdv.h:8303-8334).  The caller restores its entropy state as the
+ * reference does (tc_before, fdv.h:8273).  Returns 0, or -1 if there is no
+ * frame to undo. */
+int fdvamd_rewind(fdvamd_encoder *e);
 
 #ifdef __cplusplus
 }

@@ -86,6 +86,13 @@ int fdvgpu_pframe(fdvgpu *g, const int *src_slot, const int *qp, const double *l
 int fdvgpu_fetch(fdvgpu *g);
 int fdvgpu_streams(fdvgpu *g, int seq, const uint8_t **sv, size_t *sn);
 
+/* Undo the reference rotation of the last fdvgpu_pframe, so that the same
+ * frame can be coded again from the same references -- the requantising retry
+ * of the streaming encoder (fdv.h:8303-8334), which calls pframe_encode twice
+ * on one frame.  Valid once, directly after fdvgpu_pframe (fetching its results
+ * first is allowed); -1 otherwise. */
+int fdvgpu_rewind(fdvgpu *g);
+
 /* The deblocked reconstruction of the last frame (tight I420, host memory). */
 int fdvgpu_recon(fdvgpu *g, int seq, uint8_t *i420);
 

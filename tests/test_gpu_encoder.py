@@ -97,3 +97,31 @@ def test_720p_eight_bands_given_reference_decisions():
     sc = content.scene_pan(1280, 720, 3)
     bands = [_Band(sc, y0, min(96, 720 - y0)) for y0 in range(0, 720, 96)]
     assert run_parity(bands, 3, [24] * 8, given=True, label="720p-given") == 0
+
+
+def test_requantising_retry_after_rewind():
+    """The streaming encoder codes a frame twice when the first attempt misses
+    its budget (fdv.h:8303-8334): the second attempt starts from the same
+    references and the entropy state from before the first."""
+    sc = content.scene_objects(96, 64, 31, nobj=3, smin=8, smax=32)
+    frames = [sc.frame(i) for i in range(4)]
+    enc = G.Encoder([(sc.w, sc.h)])
+    enc.set_ref_planes(0, 0, frames[0])
+    tc, tco = G.TabCache(), L.TabCache()
+    near, far = frames[0], None
+    for f in range(1, 4):
+        navail = 1 if f == 1 else 2
+        before = tc.buf.copy()
+        say(f"  retry: frame {f} at qp 20, rewind, again at qp 34")
+        enc.pframe_encode(frames[f], navail, 20, tc)
+        enc.rewind()
+        tc.buf[:] = before
+        pay, rec = enc.pframe_encode(frames[f], navail, 34, tc)
+        wp, wr, _ = L.oracle_pframe(frames[f], near, far, sc.w, sc.h, navail, 34, tco)
+        assert pay == wp.tobytes(), f"frame {f}: payload"
+        assert np.array_equal(rec, wr), f"frame {f}: recon"
+        far, near = near, wr
+    with pytest.raises(G.GpuError):
+        enc.rewind()
+        enc.rewind()
+    enc.close()

@@ -417,3 +417,35 @@ def test_rdoq_only_truncates_and_empties_at_huge_lambda():
         big = before.copy()
         lib.orc_rdoq4(_p(big), _p(res), qp, 1e12)
         assert not big.any()
+
+
+def test_exhaustive_search_is_brute_force_minimum_then_subpel():
+    """The exhaustive mode (this build's own search): the whole-pel minimum over
+    +-range (ties to the smaller |dx|+|dy|, then raster order), then the
+    reference's sub-pel cross; the returned SAD is that of the motion-compensated
+    prediction at the returned vector."""
+    lib = L.oracle()
+    rng = 6
+    for case in V.me_cases()[:16]:
+        cur, ref, w, h, bx, by, bw, bh, pmx, pmy = case
+        buf, off, st = _padded_plane(ref)
+        P = np.pad(ref, 32, mode="edge").astype(int)
+        blk = cur[by:by + bh, bx:bx + bw].astype(int)
+        best = None
+        for dy in range(-rng, rng + 1):
+            for dx in range(-rng, rng + 1):
+                s = int(np.abs(blk - P[32 + by + dy:32 + by + dy + bh, 32 + bx + dx:32 + bx + dx + bw]).sum())
+                key = (s, abs(dx) + abs(dy))
+                if best is None or key < best[0]:
+                    best = (key, dx * 4, dy * 4)
+        (es, _), ex, ey = best
+        mx, my = C.c_int(), C.c_int()
+        s = lib.orc_me_search(_p(np.ascontiguousarray(cur)), w, C.c_void_p(buf.ctypes.data + off), st,
+                              bx, by, bw, bh, rng, pmx, pmy, 1, C.byref(mx), C.byref(my))
+        assert s <= es
+        assert abs(mx.value - ex) <= 3 and abs(my.value - ey) <= 3
+        if es == 0:
+            assert (mx.value, my.value, s) == (ex, ey, 0)
+        pred = np.zeros(bw * bh, np.uint8)
+        lib.orc_mc_luma(C.c_void_p(buf.ctypes.data + off), st, bx, by, bw, bh, mx.value, my.value, _p(pred), bw)
+        assert s == int(np.abs(blk - pred.reshape(bh, bw).astype(int)).sum())
