@@ -119,26 +119,10 @@ struct Sink {
     }
 };
 
-__device__ void emit4(Sink &S, const int16_t *l, int cn, int cl) {
-    int last = -1;
-    for (int k = 0; k < 16; ++k) if (l[p_SCAN4[k]]) last = k;
-    S.byte(cn, (unsigned)(last + 1));
-    for (int k = 0; k <= last; ++k) S.leb(cl, zz(l[p_SCAN4[k]]));
-}
-__device__ void emit8(Sink &S, const int16_t *l) {
-    int last = -1;
-    for (int k = 0; k < 64; ++k) if (l[p_SCAN8[k]]) last = k;
-    S.byte(SG_CN8, (unsigned)(last + 1));
-    for (int k = 0; k <= last; ++k) S.leb(SG_CL8, zz(l[p_SCAN8[k]]));
-}
-// Luma regions of a coded MB, or one chroma region (fdv.h:5385-5410).
-__device__ void emit_region(Sink &S, int flag, const int16_t *l, int chroma) {
-    S.byte(SG_FL, (unsigned)flag);
-    if (flag == 1) for (int b = 0; b < 4; ++b) emit4(S, l + 16 * b, chroma ? SG_CNC : SG_CN, chroma ? SG_CLC : SG_CL);
-    else if (flag == 2) emit8(S, l);
-}
-
-__device__ void emit_mb(Sink &S, const fdvg_mbrec *R, int ctx) {
+// Structure symbols and region / any-flags of one MB, in emission order
+// (pframe_encode, fdv.h:6036-6160; code_residual flags 5385-5410; chroma_block
+// any-flags 5599-5606).  Coefficient blocks are emitted by the lanes (below).
+__device__ void emit_structure(Sink &S, const fdvg_mbrec *R, int ctx) {
     S.byte(ctx, R->mode);
     if (R->mode == 0) {
         if (R->ncand > 1) S.byte(SG_MERGE, R->merge_idx);
@@ -153,18 +137,9 @@ __device__ void emit_mb(Sink &S, const fdvg_mbrec *R, int ctx) {
     } else {
         for (int q = 0; q < 4; ++q) { S.leb(SG_MVX, zz(R->mvd[q][0])); S.leb(SG_MVY, zz(R->mvd[q][1])); }
     }
-    for (int r = 0; r < 4; ++r) emit_region(S, R->lflag[r], R->lev + 64 * r, 0);
-    for (int p = 0; p < 2; ++p) {
-        const int16_t *cl = R->clev + 64 * p;
-        if (R->ckind == 1) {
-            S.byte(SG_FL, R->cflag[p]);
-            if (R->cflag[p]) for (int b = 0; b < 4; ++b) emit4(S, cl + 16 * b, SG_CNC, SG_CLC);
-        } else if (R->ckind == 2) {
-            emit_region(S, R->cflag[p], cl, 1);
-        } else {
-            for (int b = 0; b < 4; ++b) emit4(S, cl + 16 * b, SG_CNC, SG_CLC);
-        }
-    }
+    for (int r = 0; r < 4; ++r) S.byte(SG_FL, R->lflag[r]);
+    if (R->ckind == 1 || R->ckind == 2)
+        for (int p = 0; p < 2; ++p) S.byte(SG_FL, R->cflag[p]);
 }
 
 __device__ __forceinline__ int mb_ctx(const fdvg_mbrec *recs, const SymSeq &q, int mx, int my) {
@@ -174,28 +149,107 @@ __device__ __forceinline__ int mb_ctx(const fdvg_mbrec *recs, const SymSeq &q, i
     return (L ? 2 : 0) + (A ? 1 : 0);
 }
 
-// pass 0: per-MB byte counts into cnt[mb*16 + k]; pass 1: write at off[mb*16 + k].
-// Pass 1 writes stream k of sequence s at compact + bases[s*16 + k] + offset.
+// The coefficient blocks of one MB as 30 unit slots, one per lane, numbered so
+// that lane order is emission order within every stream:
+//   0-15  luma 4x4 (region r = u/4, block u%4)          -> CN / CL   when lflag[r] == 1
+//   16-19 luma 8x8 (region u-16)                         -> CN8 / CL8 when lflag[r] == 2
+//   20-27 chroma 4x4 (plane (u-20)/4, block (u-20)%4)    -> CNC / CLC
+//   28-29 chroma 8x8 (plane u-28, INTRA region flag 2)   -> CN8 / CL8
+// Chroma 4x4 units are coded when the MB is INTER8 (no flag), when the INTER16
+// any-flag is set, or when the INTRA region flag is 1 (fdv.h:5385-5410, 5599-5626).
+__device__ __forceinline__ int unit_of(const fdvg_mbrec *R, int u, const int16_t **lev, int *n,
+                                       int *cn, int *cl) {
+    if (R->mode == 0 || u >= 30) return 0;
+    if (u < 16) {
+        if (R->lflag[u >> 2] != 1) return 0;
+        *lev = R->lev + 64 * (u >> 2) + 16 * (u & 3); *n = 16; *cn = SG_CN; *cl = SG_CL; return 1;
+    }
+    if (u < 20) {
+        if (R->lflag[u - 16] != 2) return 0;
+        *lev = R->lev + 64 * (u - 16); *n = 64; *cn = SG_CN8; *cl = SG_CL8; return 1;
+    }
+    if (u < 28) {
+        const int p = (u - 20) >> 2;
+        const int on = R->ckind == 3 || (R->ckind == 1 && R->cflag[p]) || (R->ckind == 2 && R->cflag[p] == 1);
+        if (!on) return 0;
+        *lev = R->clev + 64 * p + 16 * ((u - 20) & 3); *n = 16; *cn = SG_CNC; *cl = SG_CLC; return 1;
+    }
+    if (R->ckind != 2 || R->cflag[u - 28] != 2) return 0;
+    *lev = R->clev + 64 * (u - 28); *n = 64; *cn = SG_CN8; *cl = SG_CL8; return 1;
+}
+
+__device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v) {
+    const int L = threadIdx.x & 63;
+    uint32_t x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t t = __shfl_up(x, o, 64);
+        if (L >= o) x += t;
+    }
+    return x - v;
+}
+
+// One wavefront per MB.  pass 0: per-MB byte counts into cnt[mb*16 + k];
+// pass 1: write stream k of sequence s at compact + bases[s*16 + k] + cnt[mb*16 + k]
+// (cnt holding the exclusive scan over the sequence's MBs by then).
 extern "C" __global__ void __launch_bounds__(256)
 k_sym(const SymSeq *__restrict__ seqs, const fdvg_mbrec *__restrict__ recs,
       uint32_t *__restrict__ cnt, int pass, uint8_t *__restrict__ compact,
       const uint32_t *__restrict__ bases, uint32_t cap) {
     const SymSeq q = seqs[blockIdx.y];
     const int nmb = q.mw * q.mh;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nmb; i += gridDim.x * blockDim.x) {
-        const int mb = q.mb0 + i;
-        Sink S;
-        for (int k = 0; k < FDVG_NS; ++k) {
-            if (pass == 0) { S.base[k] = nullptr; S.n[k] = 0; S.lim[k] = 0; }
-            else {
-                const uint32_t b0 = bases[blockIdx.y * FDVG_NS + k];
-                S.base[k] = compact + b0;
-                S.n[k] = cnt[(size_t)mb * FDVG_NS + k];
-                S.lim[k] = b0 < cap ? cap - b0 : 0;
-            }
+    const int L = threadIdx.x & 63;
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= nmb) return;
+    const int mb = q.mb0 + i;
+    const fdvg_mbrec *R = &recs[mb];
+
+    // this lane's coefficient unit: EOB count and LEB bytes of its levels
+    const int16_t *lev = nullptr;
+    int n = 0, cn = 0, cl = 0;
+    const int on = unit_of(R, L, &lev, &n, &cn, &cl);
+    int last = -1, lb = 0;
+    if (on) {
+        const uint8_t *scan = n == 16 ? p_SCAN4 : p_SCAN8;
+        for (int k = 0; k < n; ++k) if (lev[scan[k]]) last = k;
+        for (int k = 0; k <= last; ++k) lb += leb_n(zz(lev[scan[k]]));
+    }
+    // per coefficient-stream pair: lanes' byte counts and their in-MB offsets
+    const int pr = !on ? -1 : cn == SG_CN ? 0 : cn == SG_CN8 ? 1 : 2;
+    uint32_t offn[3], offl[3], totn[3], totl[3];
+    for (int c = 0; c < 3; ++c) {
+        const uint32_t vn = pr == c ? 1u : 0u, vl = pr == c ? (uint32_t)lb : 0u;
+        offn[c] = wave_excl_scan(vn);
+        offl[c] = wave_excl_scan(vl);
+        totn[c] = __shfl(offn[c] + vn, 63, 64);
+        totl[c] = __shfl(offl[c] + vl, 63, 64);
+    }
+
+    Sink S;
+    for (int k = 0; k < FDVG_NS; ++k) {
+        if (pass == 0) { S.base[k] = nullptr; S.n[k] = 0; S.lim[k] = 0; }
+        else {
+            const uint32_t b0 = bases[blockIdx.y * FDVG_NS + k];
+            S.base[k] = compact + b0;
+            S.n[k] = cnt[(size_t)mb * FDVG_NS + k];
+            S.lim[k] = b0 < cap ? cap - b0 : 0;
         }
-        emit_mb(S, &recs[mb], mb_ctx(recs, q, i % q.mw, i / q.mw));
-        if (pass == 0) for (int k = 0; k < FDVG_NS; ++k) cnt[(size_t)mb * FDVG_NS + k] = S.n[k];
+    }
+    if (pass == 1 && on) {
+        // lanes write their blocks at their offsets inside the MB's run
+        Sink W = S;
+        W.n[cn] += offn[pr];
+        W.n[cl] += offl[pr];
+        const uint8_t *scan = n == 16 ? p_SCAN4 : p_SCAN8;
+        W.byte(cn, (unsigned)(last + 1));
+        for (int k = 0; k <= last; ++k) W.leb(cl, zz(lev[scan[k]]));
+    }
+    if (L == 0) {
+        emit_structure(S, R, mb_ctx(recs, q, i % q.mw, i / q.mw));
+        if (pass == 0) {
+            const int sn[3] = {SG_CN, SG_CN8, SG_CNC}, sl[3] = {SG_CL, SG_CL8, SG_CLC};
+            for (int c = 0; c < 3; ++c) { S.n[sn[c]] += totn[c]; S.n[sl[c]] += totl[c]; }
+            for (int k = 0; k < FDVG_NS; ++k) cnt[(size_t)mb * FDVG_NS + k] = S.n[k];
+        }
     }
 }
 
@@ -287,7 +341,7 @@ extern "C" int fdvg_launch_symbols(const void *symseqs, int nseq, int maxmb, con
                                    size_t cap, hipStream_t st) {
     const uint32_t cap32 = cap > 0xffffffffu ? 0xffffffffu : (uint32_t)cap;
     const SymSeq *S = (const SymSeq *)symseqs;
-    int gx = (maxmb + 255) / 256;
+    int gx = (maxmb + 3) / 4;             // four MBs (wavefronts) per workgroup
     hipLaunchKernelGGL(k_sym, dim3(gx, nseq), dim3(256), 0, st, S, recs, cnt, 0, compact, (const uint32_t *)bases, cap32);
     hipLaunchKernelGGL(k_scan, dim3(FDVG_NS, nseq), dim3(256), 0, st, S, cnt, totals);
     hipLaunchKernelGGL(k_bases, dim3(1), dim3(64), 0, st, (const uint32_t *)totals, bases, nseq * FDVG_NS);
