@@ -118,7 +118,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--mode", default="diamond", choices=["diamond", "exhaustive", "learned"])
+    ap.add_argument("--mode", default="diamond", choices=["diamond", "exhaustive", "learned", "parallel"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -149,7 +149,8 @@ def main():
     nframes = min(nsteps + 1, 60)
     scene = content.scene_pan(W, H, seed=2)
     frames = [scene.frame(i) for i in range(nframes)]
-    mode = {"diamond": G.ME_DIAMOND, "exhaustive": G.ME_EXHAUSTIVE, "learned": G.ME_LEARNED}[args.mode]
+    mode = {"diamond": G.ME_DIAMOND, "exhaustive": G.ME_EXHAUSTIVE, "learned": G.ME_LEARNED,
+            "parallel": G.ME_PARALLEL}[args.mode]
 
     an = G.Analyser([(W, bh) for _, bh in layout], nsrc=nframes)
     per_band = [band_frames(f, W, H, layout) for f in frames]

@@ -61,8 +61,23 @@ struct __align__(16) Smem {
                                       // quadrants; [6]: intra SATD per mode; [7][0]: quadrant SAD sum
     uint64_t hand[3][FDVG_HWORDS];
     uint8_t hbytes[48];
-    uint8_t lcol[32];                 // previous MB's right recon column: Y 16, U 8, V 8
+    uint8_t lcol[32];                 // left neighbour's right recon column: Y 16, U 8, V 8
+    uint8_t ctop[2][9];               // chroma: corner, then the 8 samples above (U, V)
     int32_t row;
+};
+
+// Inputs of one MB's decision that come from its neighbours: the running
+// vector predictor, the merge candidates (fdv.h:5711-5732) and whether the
+// top / left neighbours exist.  Luma intra neighbours are in s.hbytes (corner,
+// then 16 above) and s.lcol; chroma ones in s.ctop and s.lcol.
+struct MbCtx {
+    int pmx, pmy, ncand, ht, hl;
+    int cx[3], cy[3], cr[3];
+};
+// The decision and what the finishing stage needs of it.
+struct MbDec {
+    int mode, bmi, iref, imx, imy, nsub;
+    int qmv[4][2];
 };
 
 // ---- wave-local motion search ------------------------------------------------
@@ -322,6 +337,315 @@ __device__ __forceinline__ int hbyte(const Smem &s, int j, int b) {
     return (int)((s.hand[j][b / 7] >> (8 * (b % 7))) & 0xff);
 }
 
+// ---- the MB body, shared by the wavefront kernel and the block-parallel ones
+
+// Stage the source MB and, when the MB may use them, the 64x64 luma and 32x32
+// chroma windows of each reference around it.  The caller synchronises.
+__device__ __forceinline__ void mb_stage(Smem &s, const fdvg_seq &q, int mb, int bx, int by) {
+    const int tid = threadIdx.x, cw = q.w >> 1, cbx = bx >> 1, cby = by >> 1;
+    s.cur[tid] = q.cy[(size_t)(by + (tid >> 4)) * q.w + bx + (tid & 15)];
+    if (tid < 128) {
+        const int pl = tid >> 6, p = tid & 63;
+        const uint8_t *src = pl ? q.cv : q.cu;
+        s.ccur[tid] = src[(size_t)(cby + (p >> 3)) * cw + cbx + (p & 7)];
+    }
+    const bool need_ref = !q.force_skip && !(q.given && q.given[mb - q.mb0].mode == 2);
+    const int nwin = q.force_skip ? 1 : q.navail;
+    if (need_ref || q.force_skip) {
+        for (int r = 0; r < nwin; ++r) {
+            // 64 rows x 64 bytes: 16 bytes a thread, two 8-byte loads (8-aligned)
+            const int wy = tid >> 2, wx = (tid & 3) * 16;
+            const uint8_t *src = (r ? q.ry[1] : q.ry[0]) + (ptrdiff_t)(by - WOFF + wy) * q.ys + bx - WOFF + wx;
+            uint64_t a = *(const uint64_t *)src, b = *(const uint64_t *)(src + 8);
+            *(uint64_t *)&s.win[r][wy * WIN + wx] = a;
+            *(uint64_t *)&s.win[r][wy * WIN + wx + 8] = b;
+            // 2 planes x 32 rows x 32 bytes: 8 bytes a thread (4-aligned)
+            const int pl = tid >> 7, t2 = tid & 127, cyy = t2 >> 2, cxx = (t2 & 3) * 8;
+            const uint8_t *cp = (pl ? (r ? q.rv[1] : q.rv[0]) : (r ? q.ru[1] : q.ru[0])) +
+                                (ptrdiff_t)(cby - CWOFF + cyy) * q.cs + cbx - CWOFF + cxx;
+            const uint32_t *c32 = (const uint32_t *)cp;
+            uint32_t u0 = c32[0], u1 = c32[1];
+            *(uint32_t *)&s.cwin[r][pl][cyy * CWIN + cxx] = u0;
+            *(uint32_t *)&s.cwin[r][pl][cyy * CWIN + cxx + 4] = u1;
+        }
+    }
+}
+
+// The mode decision of one MB (pframe_encode's MB loop, fdv.h:5820-6040): SKIP
+// over the merge candidates, INTER16 per reference, INTRA 16x16, INTER8, then
+// the strict-< commit.  With `gv` the decision is given and only its mode's
+// candidate is built; with `mt` the motion vectors come from a table instead of
+// the diamond search.  Leaves the chosen luma reconstruction / levels / flags in
+// s.recS (SKIP), s.recI/levI/flagI (INTER16), s.recN/... (INTRA), s.rec8/...
+// (INTER8).
+__device__ __forceinline__ void mb_decide(Smem &s, const fdvg_seq &q, MbCtx &c, const fdvg_mbinfo *gv,
+                                          const fdvg_metab *mt, int &par, MbDec &D) {
+    const int tid = threadIdx.x, wave = tid >> 6;
+    const double lambda = q.lambda;
+    const int qp = q.qp, pmx = c.pmx, pmy = c.pmy;
+    D.bmi = 0; D.iref = 0; D.imx = 0; D.imy = 0; D.nsub = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) { D.qmv[k][0] = 0; D.qmv[k][1] = 0; }
+
+    if (q.force_skip) {
+        // a held frame: SKIP with the zero vector everywhere (fdv.h:5820-5836)
+        D.mode = 0; c.ncand = 1; c.cx[0] = 0; c.cy[0] = 0; c.cr[0] = 0;
+        s.recS[tid] = s.win[0][(WOFF + (tid >> 4)) * WIN + WOFF + (tid & 15)];
+        __syncthreads();
+        return;
+    }
+    // ---- SKIP: best merge candidate by SSD, first wins (fdv.h:5847-5866)
+    const int ncand = c.ncand;
+    const int x = tid & 15, y = tid >> 4;
+    int pv[3] = {0, 0, 0}, e[3] = {0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        if (k >= ncand || (gv && !(gv->mode == 0 && gv->merge_idx == k))) continue;
+        pv[k] = lumaq(s.win[c.cr[k]], x, y, c.cx[k], c.cy[k]);
+        const int d = s.cur[tid] - pv[k];
+        e[k] = d * d;
+    }
+    {
+        const int e0 = wave_sum(e[0]), e1 = wave_sum(e[1]), e2 = wave_sum(e[2]);
+        if ((tid & 63) == 0) { s.red[par][wave][0] = e0; s.red[par][wave][1] = e1; s.red[par][wave][2] = e2; }
+    }
+    __syncthreads();
+    int Ds = 0, bmi = 0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        if (k >= ncand) continue;
+        const int ec = s.red[par][0][k] + s.red[par][1][k] + s.red[par][2][k] + s.red[par][3][k];
+        if (gv) { if (gv->mode == 0 && gv->merge_idx == k) { Ds = ec; bmi = k; } }
+        else if (k == 0 || ec < Ds) { Ds = ec; bmi = k; }
+    }
+    par ^= 1;
+    D.bmi = bmi;
+    s.recS[tid] = (uint8_t)(bmi == 0 ? pv[0] : bmi == 1 ? pv[1] : pv[2]);
+    const double Js = (double)Ds + lambda * (double)(3 + (ncand > 1 ? 3 : 0));
+    double Jbest = Js;
+    bool skip_wins = Js <= lambda * (double)FL_INTRA;
+    if (mt && (mt->flags & 1)) skip_wins = true;
+    if (gv) skip_wins = false;
+
+    // ---- the independent searches, one per wavefront
+    const Nb nbL = nb_setup(&s.hbytes[1], s.lcol, s.hbytes[0], 16, c.ht, c.hl);
+    if (!gv && !skip_wins) {
+        if (wave == 0 || (wave == 1 && q.navail == 2)) {
+            int a, b, sd;
+            if (mt) { a = mt->mv16[wave][0]; b = mt->mv16[wave][1]; sd = mt->sad16[wave]; }
+            else sd = me_diamond(s.win[wave], s.cur, s.sp[wave], 16, 0, 0, pmx, pmy, &a, &b);
+            if ((tid & 63) == 0) { s.me[wave][0] = a; s.me[wave][1] = b; s.me[wave][2] = sd; }
+        } else if (wave == 2) {
+            int sx = pmx, sy = pmy, ssad = 0;
+            for (int qd = 0; qd < 4; ++qd) {
+                int a, b;
+                if (mt) { a = mt->mv8[qd][0]; b = mt->mv8[qd][1]; ssad += mt->sad8[qd]; }
+                else ssad += me_diamond(s.win[0], s.cur, s.sp[2], 8, (qd & 1) * 8, (qd >> 1) * 8, sx, sy, &a, &b);
+                if ((tid & 63) == 0) { s.me[2 + qd][0] = a; s.me[2 + qd][1] = b; }
+                sx = a; sy = b;
+            }
+            if ((tid & 63) == 0) s.me[7][0] = ssad;
+        } else if (wave == 3) {
+            int sat[4];
+            satd_modes(s.cur, nbL, &s.hbytes[1], s.lcol, sat);
+            if ((tid & 63) == 0) for (int m = 0; m < 4; ++m) s.me[6][m] = sat[m];
+        }
+    }
+    __syncthreads();
+
+    // ---- INTER16 over the references (fdv.h:5909-5943)
+    double Ji = 1e30;
+    int sad16 = 0;
+    const bool want_i = gv ? gv->mode == 1 : !skip_wins;
+    for (int r = 0; r < q.navail && want_i; ++r) {
+        if (gv && r != gv->ref) continue;
+        if (!gv && r > 0 && Jbest <= lambda * (double)FL_INTER16) break;
+        int rmx, rmy, rsad = 0;
+        if (gv) { rmx = gv->mv[0][0]; rmy = gv->mv[0][1]; }
+        else { rmx = s.me[r][0]; rmy = s.me[r][1]; rsad = s.me[r][2]; }
+        s.pred[tid] = (uint8_t)lumaq(s.win[r], tid & 15, tid >> 4, rmx, rmy);
+        __syncthreads();
+        code_luma(s, s.pred, s.recT, s.levT, s.flagT, qp, lambda);
+        const int bb = 6 + egbits(zmap(rmx - pmx)) + egbits(zmap(rmy - pmy)) +
+                       s.ws.b[0] + s.ws.b[1] + s.ws.b[2] + s.ws.b[3];
+        const double dd = (double)(s.ws.e[0] + s.ws.e[1] + s.ws.e[2] + s.ws.e[3]);
+        const double J = dd + lambda * (double)bb;
+        if (J < Jbest) Jbest = J;
+        if (J < Ji) {
+            Ji = J; D.iref = r; D.imx = rmx; D.imy = rmy; sad16 = rsad;
+            s.recI[tid] = s.recT[tid];
+            s.levI[tid] = s.levT[tid];
+            if (tid < 4) s.flagI[tid] = s.flagT[tid];
+        }
+        __syncthreads();
+    }
+
+    // ---- INTRA 16x16: SATD screen, one coding (fdv.h:5950-5988)
+    double Jn = 1e30;
+    const bool want_n = gv ? gv->mode == 2 : (!skip_wins && Jbest > lambda * (double)FL_INTRA);
+    if (want_n) {
+        int sel = 0;
+        if (gv) sel = gv->sub;
+        else {
+            int sbest = 0; sel = -1;
+            for (int m = 0; m < 4; ++m) {
+                if (!nn_ok(m, nbL)) continue;
+                if (sel < 0 || s.me[6][m] < sbest) { sbest = s.me[6][m]; sel = m; }
+            }
+        }
+        s.pred[tid] = (uint8_t)ipred(sel, nbL, &s.hbytes[1], s.lcol, 16, tid & 15, tid >> 4);
+        __syncthreads();
+        code_luma(s, s.pred, s.recN, s.levN, s.flagN, qp, lambda);
+        const int bn = 6 + s.ws.b[0] + s.ws.b[1] + s.ws.b[2] + s.ws.b[3];
+        Jn = (double)(s.ws.e[0] + s.ws.e[1] + s.ws.e[2] + s.ws.e[3]) + lambda * (double)bn;
+        D.nsub = sel;
+        if (Jn < Jbest) Jbest = Jn;
+    }
+
+    // ---- INTER8: the quadrant vectors, the SAD split gate, four 8x8 residuals
+    //      (fdv.h:5996-6033)
+    double J8 = 1e30;
+    bool want_8 = gv ? gv->mode == 3 : (!skip_wins && Jbest > lambda * (double)FL_INTER8);
+    if (want_8) {
+#pragma unroll
+        for (int qd = 0; qd < 4; ++qd) {
+            D.qmv[qd][0] = gv ? gv->mv[qd][0] : s.me[2 + qd][0];
+            D.qmv[qd][1] = gv ? gv->mv[qd][1] : s.me[2 + qd][1];
+        }
+        const int ssad = s.me[7][0];
+        if (!gv && sad16 > 0 && ssad * 10 >= sad16 * 9) want_8 = false;
+        if (want_8) {
+            const int qd = (y >> 3) * 2 + (x >> 3);
+            const int vx = qd == 0 ? D.qmv[0][0] : qd == 1 ? D.qmv[1][0] : qd == 2 ? D.qmv[2][0] : D.qmv[3][0];
+            const int vy = qd == 0 ? D.qmv[0][1] : qd == 1 ? D.qmv[1][1] : qd == 2 ? D.qmv[2][1] : D.qmv[3][1];
+            s.pred[tid] = (uint8_t)lumaq(s.win[0], x, y, vx, vy);
+            __syncthreads();
+            code_luma(s, s.pred, s.rec8, s.lev8, s.flag8, qp, lambda);
+            int b8 = 3 + s.ws.b[0] + s.ws.b[1] + s.ws.b[2] + s.ws.b[3];
+            int qpx = pmx, qpy = pmy;
+#pragma unroll
+            for (int qd2 = 0; qd2 < 4; ++qd2) {
+                b8 += egbits(zmap(D.qmv[qd2][0] - qpx)) + egbits(zmap(D.qmv[qd2][1] - qpy));
+                qpx = D.qmv[qd2][0]; qpy = D.qmv[qd2][1];
+            }
+            J8 = (double)(s.ws.e[0] + s.ws.e[1] + s.ws.e[2] + s.ws.e[3]) + lambda * (double)b8;
+        }
+    }
+
+    // ---- commit: strict < in order SKIP, INTER16, INTRA, INTER8
+    if (gv) D.mode = gv->mode;
+    else {
+        double Jm = Js; D.mode = 0;
+        if (Ji < Jm) { Jm = Ji; D.mode = 1; }
+        if (Jn < Jm) { Jm = Jn; D.mode = 2; }
+        if (J8 < Jm) { Jm = J8; D.mode = 3; }
+    }
+}
+
+// The motion record an MB leaves for its neighbours (fdv_mbmv, fdv.h:5680):
+// vector, reference, whether it is inter.
+__device__ __forceinline__ void mb_motion(const MbCtx &c, const MbDec &D, int *nmx, int *nmy, int *nref,
+                                          int *ninter) {
+    const int b = D.bmi;
+    if (D.mode == 0) {
+        *nmx = b == 0 ? c.cx[0] : b == 1 ? c.cx[1] : c.cx[2];
+        *nmy = b == 0 ? c.cy[0] : b == 1 ? c.cy[1] : c.cy[2];
+        *nref = b == 0 ? c.cr[0] : b == 1 ? c.cr[1] : c.cr[2];
+        *ninter = 1;
+    } else if (D.mode == 1) { *nmx = D.imx; *nmy = D.imy; *nref = D.iref; *ninter = 1; }
+    else if (D.mode == 2)   { *nmx = 0; *nmy = 0; *nref = 0; *ninter = 0; }
+    else                    { *nmx = D.qmv[3][0]; *nmy = D.qmv[3][1]; *nref = 0; *ninter = 1; }
+}
+
+// Chroma of the chosen mode, the reconstruction into the output planes, the
+// record for the serialiser and the reported decision.  Returns the luma
+// reconstruction (in LDS); the chroma one is in s.crec.
+__device__ __forceinline__ const uint8_t *mb_finish(Smem &s, const fdvg_seq &q, int mb, int bx, int by,
+                                                    const MbCtx &c, const MbDec &D, fdvg_mbrec *R) {
+    const int tid = threadIdx.x, cbx = bx >> 1, cby = by >> 1;
+    const double lambda = q.lambda;
+    const int qp = q.qp, mode = D.mode;
+    int nmx, nmy, nref, ninter;
+    mb_motion(c, D, &nmx, &nmy, &nref, &ninter);
+    const uint8_t *yrec = mode == 0 ? s.recS : mode == 1 ? s.recI : mode == 2 ? s.recN : s.rec8;
+    int ckind = 0;
+    if (mode == 0) {
+        if (tid < 128) {
+            const int pl = tid >> 6, p = tid & 63;
+            s.crec[tid] = (uint8_t)chromaq(s.cwin[nref][pl], p & 7, p >> 3, nmx / 2, nmy / 2);
+        }
+        __syncthreads();
+    } else if (mode == 1) {
+        if (tid < 128) {
+            const int pl = tid >> 6, p = tid & 63;
+            s.cpred[tid] = (uint8_t)chromaq(s.cwin[D.iref][pl], p & 7, p >> 3, D.imx / 2, D.imy / 2);
+        }
+        __syncthreads();
+        code_chroma<1>(s, qp, lambda);
+        ckind = 1;
+    } else if (mode == 2) {
+        if (tid < 128) {
+            const int pl = tid >> 6, p = tid & 63;
+            const uint8_t *tp = &s.ctop[pl][1], *lf = &s.lcol[16 + 8 * pl];
+            const Nb nb = nb_setup(tp, lf, s.ctop[pl][0], 8, c.ht, c.hl);
+            s.cpred[tid] = (uint8_t)ipred(D.nsub, nb, tp, lf, 8, p & 7, p >> 3);
+        }
+        __syncthreads();
+        code_chroma<0>(s, qp, lambda);
+        ckind = 2;
+    } else {
+        if (tid < 128) {
+            const int pl = tid >> 6, p = tid & 63, x = p & 7, y = p >> 3, qd = (y >> 2) * 2 + (x >> 2);
+            const int vx = qd == 0 ? D.qmv[0][0] : qd == 1 ? D.qmv[1][0] : qd == 2 ? D.qmv[2][0] : D.qmv[3][0];
+            const int vy = qd == 0 ? D.qmv[0][1] : qd == 1 ? D.qmv[1][1] : qd == 2 ? D.qmv[2][1] : D.qmv[3][1];
+            s.cpred[tid] = (uint8_t)chromaq(s.cwin[0][pl], x, y, vx / 2, vy / 2);
+        }
+        __syncthreads();
+        code_chroma<2>(s, qp, lambda);
+        ckind = 3;
+    }
+
+    // reconstruction out (the next frame's reference, deblocked later)
+    q.oy[(ptrdiff_t)(by + (tid >> 4)) * q.ys + bx + (tid & 15)] = yrec[tid];
+    if (tid < 128) {
+        const int pl = tid >> 6, p = tid & 63;
+        (pl ? q.ov : q.ou)[(ptrdiff_t)(cby + (p >> 3)) * q.cs + cbx + (p & 7)] = s.crec[tid];
+    }
+    // decision record for the serialiser
+    const int16_t *lsrc = mode == 1 ? s.levI : mode == 2 ? s.levN : s.lev8;
+    const uint8_t *fsrc = mode == 1 ? s.flagI : mode == 2 ? s.flagN : s.flag8;
+    R->lev[tid] = mode == 0 ? (int16_t)0 : lsrc[tid];
+    if (tid < 128) R->clev[tid] = ckind ? s.clev[tid] : (int16_t)0;
+    if (tid == 0) {
+        const int pmx = c.pmx, pmy = c.pmy;
+        R->mode = (uint8_t)mode; R->ref = (uint8_t)(mode == 0 ? nref : mode == 1 ? D.iref : 0);
+        R->merge_idx = (uint8_t)D.bmi; R->ncand = (uint8_t)c.ncand;
+        R->sub = (uint8_t)D.nsub; R->ckind = (uint8_t)ckind;
+        for (int k = 0; k < 4; ++k) R->lflag[k] = mode == 0 ? 0 : fsrc[k];
+        R->cflag[0] = s.cflag[0]; R->cflag[1] = s.cflag[1];
+        for (int k = 0; k < 4; ++k) { R->mvd[k][0] = R->mvd[k][1] = 0; R->mv[k][0] = R->mv[k][1] = 0; }
+        if (mode == 0) { R->mv[0][0] = (int16_t)nmx; R->mv[0][1] = (int16_t)nmy; }
+        if (mode == 1) {
+            R->mv[0][0] = (int16_t)D.imx; R->mv[0][1] = (int16_t)D.imy;
+            R->mvd[0][0] = (int16_t)(D.imx - pmx); R->mvd[0][1] = (int16_t)(D.imy - pmy);
+        }
+        if (mode == 3) {
+            int qpx = pmx, qpy = pmy;
+            for (int k = 0; k < 4; ++k) {
+                R->mv[k][0] = (int16_t)D.qmv[k][0]; R->mv[k][1] = (int16_t)D.qmv[k][1];
+                R->mvd[k][0] = (int16_t)(D.qmv[k][0] - qpx); R->mvd[k][1] = (int16_t)(D.qmv[k][1] - qpy);
+                qpx = D.qmv[k][0]; qpy = D.qmv[k][1];
+            }
+        }
+        if (q.info) {
+            fdvg_mbinfo *I = &q.info[mb - q.mb0];
+            I->mode = (uint8_t)mode; I->ref = R->ref; I->merge_idx = (uint8_t)D.bmi; I->sub = (uint8_t)D.nsub;
+            for (int k = 0; k < 4; ++k) { I->mv[k][0] = R->mv[k][0]; I->mv[k][1] = R->mv[k][1]; }
+        }
+    }
+    return yrec;
+}
+
 // ------------------------------------------------------------------ kernel
 
 // Optional per-phase cycle accounting (thread 0 of every workgroup), compiled in
@@ -340,7 +664,7 @@ extern "C" __global__ void __launch_bounds__(256)
 k_analyse(const fdvg_seq *__restrict__ seqs, const int2 *__restrict__ rowtab, int nrows,
           unsigned *ticket, fdvg_mbrec *__restrict__ recs, uint64_t *__restrict__ hand) {
     __shared__ Smem s;
-    const int tid = threadIdx.x, wave = tid >> 6;
+    const int tid = threadIdx.x;
     int par = 0;
 #ifdef FDVG_PHASE_PROF
     long long ph[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -359,14 +683,12 @@ k_analyse(const fdvg_seq *__restrict__ seqs, const int2 *__restrict__ rowtab, in
         }
         const int2 rt = rowtab[row];
         const fdvg_seq q = seqs[rt.x];
-        const int my = rt.y, mw = q.w >> 4, cw = q.w >> 1;
-        const double lambda = q.lambda;
-        const int qp = q.qp;
+        const int my = rt.y, mw = q.w >> 4;
         const uint64_t tag = (uint64_t)(q.tag & 0xff) << 56;
         int pmx = 0, pmy = 0;
 
         for (int mx = 0; mx < mw; ++mx) {
-            const int bx = mx * 16, by = my * 16, cbx = bx >> 1, cby = by >> 1;
+            const int bx = mx * 16, by = my * 16;
             const int mb = q.mb0 + my * mw + mx;
             // ---- wait for the row above (MBs mx-1 .. mx+1) and stage its handoff
             if (tid < 18) {
@@ -385,43 +707,26 @@ k_analyse(const fdvg_seq *__restrict__ seqs, const int2 *__restrict__ rowtab, in
                 }
                 s.hand[j][k] = v;
             }
-            // ---- stage the source MB and the reference windows
-            s.cur[tid] = q.cy[(size_t)(by + (tid >> 4)) * q.w + bx + (tid & 15)];
-            if (tid < 128) {
-                const int pl = tid >> 6, p = tid & 63;
-                const uint8_t *src = pl ? q.cv : q.cu;
-                s.ccur[tid] = src[(size_t)(cby + (p >> 3)) * cw + cbx + (p & 7)];
-            }
-            const bool need_ref = !q.force_skip && !(q.given && q.given[mb - q.mb0].mode == 2);
-            const int nwin = q.force_skip ? 1 : q.navail;
-            if (need_ref || q.force_skip) {
-                for (int r = 0; r < nwin; ++r) {
-                    // 64 rows x 64 bytes: 16 bytes a thread, two 8-byte loads (8-aligned)
-                    const int wy = tid >> 2, wx = (tid & 3) * 16;
-                    const uint8_t *src = q.ry[r] + (ptrdiff_t)(by - WOFF + wy) * q.ys + bx - WOFF + wx;
-                    uint64_t a = *(const uint64_t *)src, b = *(const uint64_t *)(src + 8);
-                    *(uint64_t *)&s.win[r][wy * WIN + wx] = a;
-                    *(uint64_t *)&s.win[r][wy * WIN + wx + 8] = b;
-                    // 2 planes x 32 rows x 32 bytes: 8 bytes a thread (4-aligned)
-                    const int pl = tid >> 7, t2 = tid & 127, cyy = t2 >> 2, cxx = (t2 & 3) * 8;
-                    const uint8_t *cp = (pl ? q.rv[r] : q.ru[r]) + (ptrdiff_t)(cby - CWOFF + cyy) * q.cs + cbx - CWOFF + cxx;
-                    const uint32_t *c32 = (const uint32_t *)cp;
-                    uint32_t u0 = c32[0], u1 = c32[1];
-                    *(uint32_t *)&s.cwin[r][pl][cyy * CWIN + cxx] = u0;
-                    *(uint32_t *)&s.cwin[r][pl][cyy * CWIN + cxx + 4] = u1;
-                }
-            }
+            mb_stage(s, q, mb, bx, by);
             __syncthreads();
             PH(0);
 
             // ---- neighbours from the row above / the previous MB
-            const int ht = my > 0, hl = mx > 0;
-            if (tid < 16) s.hbytes[1 + tid] = ht ? (uint8_t)hbyte(s, 1, tid) : 0;
-            if (tid == 16) s.hbytes[0] = (ht && hl) ? (uint8_t)hbyte(s, 0, 15) : 128;
+            MbCtx c;
+            c.ht = my > 0; c.hl = mx > 0;
+            c.pmx = pmx; c.pmy = pmy;
+            if (tid < 16) s.hbytes[1 + tid] = c.ht ? (uint8_t)hbyte(s, 1, tid) : 0;
+            if (tid == 16) s.hbytes[0] = (c.ht && c.hl) ? (uint8_t)hbyte(s, 0, 15) : 128;
+            if (tid >= 32 && tid < 50) {
+                const int pl = (tid - 32) / 9, k = (tid - 32) % 9;
+                s.ctop[pl][k] = k ? (c.ht ? (uint8_t)hbyte(s, 1, 16 + 8 * pl + k - 1) : 0)
+                                  : ((c.ht && c.hl) ? (uint8_t)hbyte(s, 0, 16 + 8 * pl + 7) : 128);
+            }
             // merge candidates (fdv.h:5711-5732)
-            int cx_[3] = {pmx, 0, 0}, cy_[3] = {pmy, 0, 0}, cr_[3] = {0, 0, 0}, ncand = 1;
-            if (ht) {
-                for (int k = 0; k < 2 && ncand < 3; ++k) {
+            c.cx[0] = pmx; c.cy[0] = pmy; c.cr[0] = 0; c.ncand = 1;
+            c.cx[1] = c.cy[1] = c.cr[1] = c.cx[2] = c.cy[2] = c.cr[2] = 0;
+            if (c.ht) {
+                for (int k = 0; k < 2 && c.ncand < 3; ++k) {
                     const int j = 1 + k;
                     if (mx + k >= mw) continue;
                     const int inter = hbyte(s, j, 37);
@@ -431,11 +736,11 @@ k_analyse(const fdvg_seq *__restrict__ seqs, const int2 *__restrict__ rowtab, in
                     const int vr = hbyte(s, j, 36);
                     bool dup = false;
 #pragma unroll
-                    for (int c = 0; c < 3; ++c) dup |= (c < ncand && cx_[c] == vx && cy_[c] == vy && cr_[c] == vr);
+                    for (int m = 0; m < 3; ++m) dup |= (m < c.ncand && c.cx[m] == vx && c.cy[m] == vy && c.cr[m] == vr);
                     if (!dup) {
-                        if (ncand == 1) { cx_[1] = vx; cy_[1] = vy; cr_[1] = vr; }
-                        else            { cx_[2] = vx; cy_[2] = vy; cr_[2] = vr; }
-                        ++ncand;
+                        if (c.ncand == 1) { c.cx[1] = vx; c.cy[1] = vy; c.cr[1] = vr; }
+                        else              { c.cx[2] = vx; c.cy[2] = vy; c.cr[2] = vr; }
+                        ++c.ncand;
                     }
                 }
             }
@@ -443,263 +748,14 @@ k_analyse(const fdvg_seq *__restrict__ seqs, const int2 *__restrict__ rowtab, in
 
             const fdvg_mbinfo *gv = q.given ? &q.given[mb - q.mb0] : nullptr;
             const fdvg_metab *mt = (q.me_mode == FDVG_ME_TABLE && q.metab) ? &q.metab[mb - q.mb0] : nullptr;
-            int mode, bmi = 0, iref = 0, imx = 0, imy = 0, nsub = 0;
-            int qmv[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
-            fdvg_mbrec *R = &recs[mb];
-
-            if (q.force_skip) {
-                // a held frame: SKIP with the zero vector everywhere (fdv.h:5820-5836)
-                mode = 0; ncand = 1; cx_[0] = 0; cy_[0] = 0; cr_[0] = 0;
-                s.recS[tid] = s.win[0][(WOFF + (tid >> 4)) * WIN + WOFF + (tid & 15)];
-                __syncthreads();
-            } else {
-                // ---- SKIP: best merge candidate by SSD, first wins (fdv.h:5847-5866)
-                const int x = tid & 15, y = tid >> 4;
-                int pv[3] = {0, 0, 0}, e[3] = {0, 0, 0};
-#pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    if (c >= ncand || (gv && !(gv->mode == 0 && gv->merge_idx == c))) continue;
-                    pv[c] = lumaq(s.win[cr_[c]], x, y, cx_[c], cy_[c]);
-                    const int d = s.cur[tid] - pv[c];
-                    e[c] = d * d;
-                }
-                {
-                    const int e0 = wave_sum(e[0]), e1 = wave_sum(e[1]), e2 = wave_sum(e[2]);
-                    if ((tid & 63) == 0) { s.red[par][wave][0] = e0; s.red[par][wave][1] = e1; s.red[par][wave][2] = e2; }
-                }
-                __syncthreads();
-                int Ds = 0;
-#pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    if (c >= ncand) continue;
-                    const int ec = s.red[par][0][c] + s.red[par][1][c] + s.red[par][2][c] + s.red[par][3][c];
-                    if (gv) { if (gv->mode == 0 && gv->merge_idx == c) { Ds = ec; bmi = c; } }
-                    else if (c == 0 || ec < Ds) { Ds = ec; bmi = c; }
-                }
-                par ^= 1;
-                s.recS[tid] = (uint8_t)(bmi == 0 ? pv[0] : bmi == 1 ? pv[1] : pv[2]);
-                const double Js = (double)Ds + lambda * (double)(3 + (ncand > 1 ? 3 : 0));
-                double Jbest = Js;
-                bool skip_wins = Js <= lambda * (double)FL_INTRA;
-                if (mt && (mt->flags & 1)) skip_wins = true;
-                if (gv) skip_wins = false;
-                PH(1);
-
-                // ---- the independent searches, one per wavefront
-                const Nb nbL = nb_setup(&s.hbytes[1], s.lcol, s.hbytes[0], 16, ht, hl);
-                if (!gv && !skip_wins) {
-                    if (wave == 0 || (wave == 1 && q.navail == 2)) {
-                        int a, b, sd;
-                        if (mt) { a = mt->mv16[wave][0]; b = mt->mv16[wave][1]; sd = mt->sad16[wave]; }
-                        else sd = me_diamond(s.win[wave], s.cur, s.sp[wave], 16, 0, 0, pmx, pmy, &a, &b);
-                        if ((tid & 63) == 0) { s.me[wave][0] = a; s.me[wave][1] = b; s.me[wave][2] = sd; }
-                    } else if (wave == 2) {
-                        int sx = pmx, sy = pmy, ssad = 0;
-                        for (int qd = 0; qd < 4; ++qd) {
-                            int a, b;
-                            if (mt) { a = mt->mv8[qd][0]; b = mt->mv8[qd][1]; ssad += mt->sad8[qd]; }
-                            else ssad += me_diamond(s.win[0], s.cur, s.sp[2], 8, (qd & 1) * 8, (qd >> 1) * 8, sx, sy, &a, &b);
-                            if ((tid & 63) == 0) { s.me[2 + qd][0] = a; s.me[2 + qd][1] = b; }
-                            sx = a; sy = b;
-                        }
-                        if ((tid & 63) == 0) s.me[7][0] = ssad;
-                    } else if (wave == 3) {
-                        int sat[4];
-                        satd_modes(s.cur, nbL, &s.hbytes[1], s.lcol, sat);
-                        if ((tid & 63) == 0) for (int m = 0; m < 4; ++m) s.me[6][m] = sat[m];
-                    }
-                }
-                __syncthreads();
-                PH(2);
-
-                // ---- INTER16 over the references (fdv.h:5909-5943)
-                double Ji = 1e30;
-                int sad16 = 0;
-                const bool want_i = gv ? gv->mode == 1 : !skip_wins;
-                for (int r = 0; r < q.navail && want_i; ++r) {
-                    if (gv && r != gv->ref) continue;
-                    if (!gv && r > 0 && Jbest <= lambda * (double)FL_INTER16) break;
-                    int rmx, rmy, rsad = 0;
-                    if (gv) { rmx = gv->mv[0][0]; rmy = gv->mv[0][1]; }
-                    else { rmx = s.me[r][0]; rmy = s.me[r][1]; rsad = s.me[r][2]; }
-                    s.pred[tid] = (uint8_t)lumaq(s.win[r], tid & 15, tid >> 4, rmx, rmy);
-                    __syncthreads();
-                    code_luma(s, s.pred, s.recT, s.levT, s.flagT, qp, lambda);
-                    const int bb = 6 + egbits(zmap(rmx - pmx)) + egbits(zmap(rmy - pmy)) +
-                                   s.ws.b[0] + s.ws.b[1] + s.ws.b[2] + s.ws.b[3];
-                    const double dd = (double)(s.ws.e[0] + s.ws.e[1] + s.ws.e[2] + s.ws.e[3]);
-                    const double J = dd + lambda * (double)bb;
-                    if (J < Jbest) Jbest = J;
-                    if (J < Ji) {
-                        Ji = J; iref = r; imx = rmx; imy = rmy; sad16 = rsad;
-                        s.recI[tid] = s.recT[tid];
-                        s.levI[tid] = s.levT[tid];
-                        if (tid < 4) s.flagI[tid] = s.flagT[tid];
-                    }
-                    __syncthreads();
-                    PH(3);
-                }
-
-                // ---- INTRA 16x16: SATD screen, one coding (fdv.h:5950-5988)
-                double Jn = 1e30;
-                const bool want_n = gv ? gv->mode == 2 : (!skip_wins && Jbest > lambda * (double)FL_INTRA);
-                if (want_n) {
-                    int sel = 0;
-                    if (gv) sel = gv->sub;
-                    else {
-                        int sbest = 0; sel = -1;
-                        for (int m = 0; m < 4; ++m) {
-                            if (!nn_ok(m, nbL)) continue;
-                            if (sel < 0 || s.me[6][m] < sbest) { sbest = s.me[6][m]; sel = m; }
-                        }
-                    }
-                    s.pred[tid] = (uint8_t)ipred(sel, nbL, &s.hbytes[1], s.lcol, 16, tid & 15, tid >> 4);
-                    __syncthreads();
-                    code_luma(s, s.pred, s.recN, s.levN, s.flagN, qp, lambda);
-                    const int bn = 6 + s.ws.b[0] + s.ws.b[1] + s.ws.b[2] + s.ws.b[3];
-                    Jn = (double)(s.ws.e[0] + s.ws.e[1] + s.ws.e[2] + s.ws.e[3]) + lambda * (double)bn;
-                    nsub = sel;
-                    if (Jn < Jbest) Jbest = Jn;
-                }
-                PH(4);
-
-                // ---- INTER8: the quadrant vectors, the SAD split gate, four 8x8
-                //      residuals (fdv.h:5996-6033)
-                double J8 = 1e30;
-                bool want_8 = gv ? gv->mode == 3 : (!skip_wins && Jbest > lambda * (double)FL_INTER8);
-                if (want_8) {
-#pragma unroll
-                    for (int qd = 0; qd < 4; ++qd) {
-                        qmv[qd][0] = gv ? gv->mv[qd][0] : s.me[2 + qd][0];
-                        qmv[qd][1] = gv ? gv->mv[qd][1] : s.me[2 + qd][1];
-                    }
-                    const int ssad = s.me[7][0];
-                    PH(5);
-                    if (!gv && sad16 > 0 && ssad * 10 >= sad16 * 9) want_8 = false;
-                    if (want_8) {
-                        const int x = tid & 15, y = tid >> 4, qd = (y >> 3) * 2 + (x >> 3);
-                        const int vx = qd == 0 ? qmv[0][0] : qd == 1 ? qmv[1][0] : qd == 2 ? qmv[2][0] : qmv[3][0];
-                        const int vy = qd == 0 ? qmv[0][1] : qd == 1 ? qmv[1][1] : qd == 2 ? qmv[2][1] : qmv[3][1];
-                        s.pred[tid] = (uint8_t)lumaq(s.win[0], x, y, vx, vy);
-                        __syncthreads();
-                        code_luma(s, s.pred, s.rec8, s.lev8, s.flag8, qp, lambda);
-                        int b8 = 3 + s.ws.b[0] + s.ws.b[1] + s.ws.b[2] + s.ws.b[3];
-                        int qpx = pmx, qpy = pmy;
-#pragma unroll
-                        for (int qd2 = 0; qd2 < 4; ++qd2) {
-                            b8 += egbits(zmap(qmv[qd2][0] - qpx)) + egbits(zmap(qmv[qd2][1] - qpy));
-                            qpx = qmv[qd2][0]; qpy = qmv[qd2][1];
-                        }
-                        J8 = (double)(s.ws.e[0] + s.ws.e[1] + s.ws.e[2] + s.ws.e[3]) + lambda * (double)b8;
-                    }
-                    PH(6);
-                }
-
-                // ---- commit: strict < in order SKIP, INTER16, INTRA, INTER8
-                if (gv) mode = gv->mode;
-                else {
-                    double Jm = Js; mode = 0;
-                    if (Ji < Jm) { Jm = Ji; mode = 1; }
-                    if (Jn < Jm) { Jm = Jn; mode = 2; }
-                    if (J8 < Jm) { Jm = J8; mode = 3; }
-                }
-            }
-
-            const int selx = bmi == 0 ? cx_[0] : bmi == 1 ? cx_[1] : cx_[2];
-            const int sely = bmi == 0 ? cy_[0] : bmi == 1 ? cy_[1] : cy_[2];
-            const int selr = bmi == 0 ? cr_[0] : bmi == 1 ? cr_[1] : cr_[2];
-            // ---- chroma of the chosen mode, then publish the MB
-            uint8_t *yrec = mode == 0 ? s.recS : mode == 1 ? s.recI : mode == 2 ? s.recN : s.rec8;
-            int ckind = 0;
-            if (mode == 0) {
-                const int sr = selr, smx = selx / 2, smy = sely / 2;
-                if (tid < 128) {
-                    const int pl = tid >> 6, p = tid & 63;
-                    s.crec[tid] = (uint8_t)chromaq(s.cwin[sr][pl], p & 7, p >> 3, smx, smy);
-                }
-                __syncthreads();
-            } else if (mode == 1) {
-                if (tid < 128) {
-                    const int pl = tid >> 6, p = tid & 63;
-                    s.cpred[tid] = (uint8_t)chromaq(s.cwin[iref][pl], p & 7, p >> 3, imx / 2, imy / 2);
-                }
-                __syncthreads();
-                code_chroma<1>(s, qp, lambda);
-                ckind = 1;
-            } else if (mode == 2) {
-                if (tid < 2) {
-                    // chroma neighbours: top from the handoff, left from the previous MB
-                    const int pl = tid;
-                    uint8_t *ct = &s.hbytes[18 + pl * 10];
-                    for (int k = 0; k < 8; ++k) ct[1 + k] = ht ? (uint8_t)hbyte(s, 1, 16 + 8 * pl + k) : 0;
-                    ct[0] = (ht && hl) ? (uint8_t)hbyte(s, 0, 16 + 8 * pl + 7) : 128;
-                }
-                __syncthreads();
-                if (tid < 128) {
-                    const int pl = tid >> 6, p = tid & 63;
-                    const uint8_t *ct = &s.hbytes[18 + pl * 10];
-                    const uint8_t *tp = ct + 1, *lf = &s.lcol[16 + 8 * pl];
-                    const Nb nb = nb_setup(tp, lf, ct[0], 8, ht, hl);
-                    s.cpred[tid] = (uint8_t)ipred(nsub, nb, tp, lf, 8, p & 7, p >> 3);
-                }
-                __syncthreads();
-                code_chroma<0>(s, qp, lambda);
-                ckind = 2;
-            } else {
-                if (tid < 128) {
-                    const int pl = tid >> 6, p = tid & 63, x = p & 7, y = p >> 3, qd = (y >> 2) * 2 + (x >> 2);
-                    const int vx = qd == 0 ? qmv[0][0] : qd == 1 ? qmv[1][0] : qd == 2 ? qmv[2][0] : qmv[3][0];
-                    const int vy = qd == 0 ? qmv[0][1] : qd == 1 ? qmv[1][1] : qd == 2 ? qmv[2][1] : qmv[3][1];
-                    s.cpred[tid] = (uint8_t)chromaq(s.cwin[0][pl], x, y, vx / 2, vy / 2);
-                }
-                __syncthreads();
-                code_chroma<2>(s, qp, lambda);
-                ckind = 3;
-            }
-
-            // reconstruction out (the next frame's reference, deblocked later)
-            q.oy[(ptrdiff_t)(by + (tid >> 4)) * q.ys + bx + (tid & 15)] = yrec[tid];
-            if (tid < 128) {
-                const int pl = tid >> 6, p = tid & 63;
-                (pl ? q.ov : q.ou)[(ptrdiff_t)(cby + (p >> 3)) * q.cs + cbx + (p & 7)] = s.crec[tid];
-            }
-            // decision record for the serialiser
-            const int16_t *lsrc = mode == 1 ? s.levI : mode == 2 ? s.levN : s.lev8;
-            const uint8_t *fsrc = mode == 1 ? s.flagI : mode == 2 ? s.flagN : s.flag8;
-            R->lev[tid] = mode == 0 ? (int16_t)0 : lsrc[tid];
-            if (tid < 128) R->clev[tid] = ckind ? s.clev[tid] : (int16_t)0;
-            // new vector predictor and motion record
+            MbDec D;
+            mb_decide(s, q, c, gv, mt, par, D);
+            PH(1);
+            const uint8_t *yrec = mb_finish(s, q, mb, bx, by, c, D, &recs[mb]);
             int nmx, nmy, nref, ninter;
-            if (mode == 0)      { nmx = selx; nmy = sely; nref = selr; ninter = 1; }
-            else if (mode == 1) { nmx = imx; nmy = imy; nref = iref; ninter = 1; }
-            else if (mode == 2) { nmx = 0; nmy = 0; nref = 0; ninter = 0; }
-            else                { nmx = qmv[3][0]; nmy = qmv[3][1]; nref = 0; ninter = 1; }
-            if (tid == 0) {
-                R->mode = (uint8_t)mode; R->ref = (uint8_t)(mode == 0 ? selr : mode == 1 ? iref : 0);
-                R->merge_idx = (uint8_t)bmi; R->ncand = (uint8_t)ncand;
-                R->sub = (uint8_t)nsub; R->ckind = (uint8_t)ckind;
-                for (int k = 0; k < 4; ++k) R->lflag[k] = mode == 0 ? 0 : fsrc[k];
-                R->cflag[0] = s.cflag[0]; R->cflag[1] = s.cflag[1];
-                for (int k = 0; k < 4; ++k) { R->mvd[k][0] = R->mvd[k][1] = 0; R->mv[k][0] = R->mv[k][1] = 0; }
-                if (mode == 0) { R->mv[0][0] = (int16_t)nmx; R->mv[0][1] = (int16_t)nmy; }
-                if (mode == 1) {
-                    R->mv[0][0] = (int16_t)imx; R->mv[0][1] = (int16_t)imy;
-                    R->mvd[0][0] = (int16_t)(imx - pmx); R->mvd[0][1] = (int16_t)(imy - pmy);
-                }
-                if (mode == 3) {
-                    int qpx = pmx, qpy = pmy;
-                    for (int k = 0; k < 4; ++k) {
-                        R->mv[k][0] = (int16_t)qmv[k][0]; R->mv[k][1] = (int16_t)qmv[k][1];
-                        R->mvd[k][0] = (int16_t)(qmv[k][0] - qpx); R->mvd[k][1] = (int16_t)(qmv[k][1] - qpy);
-                        qpx = qmv[k][0]; qpy = qmv[k][1];
-                    }
-                }
-                if (q.info) {
-                    fdvg_mbinfo *I = &q.info[mb - q.mb0];
-                    I->mode = (uint8_t)mode; I->ref = R->ref; I->merge_idx = (uint8_t)bmi; I->sub = (uint8_t)nsub;
-                    for (int k = 0; k < 4; ++k) { I->mv[k][0] = R->mv[k][0]; I->mv[k][1] = R->mv[k][1]; }
-                }
-            }
+            mb_motion(c, D, &nmx, &nmy, &nref, &ninter);
+            const int mode = D.mode;
+
             // handoff bytes: bottom rows, motion record, mode
             if (tid < 16) s.hbytes[tid] = yrec[240 + tid];
             else if (tid < 24) s.hbytes[tid] = s.crec[56 + (tid - 16)];
@@ -725,6 +781,578 @@ k_analyse(const fdvg_seq *__restrict__ seqs, const int2 *__restrict__ rowtab, in
             PH(7);
         }
     }
+}
+
+// ---------------------------------------------------- block-parallel decisions
+//
+// The parallel and learned modes replace the decision wavefront by
+//   k_pcost    every MB's candidate costs, all MBs at once;
+//   k_prepair  the reference's decision logic (fdv.h:5847-6040) replayed in a
+//              light wavefront (one wavefront per MB row, motion records only)
+//              on those costs, with the true vector predictor and merge list;
+//   k_pencode  the decided non-intra MBs coded exactly, all at once;
+//   k_pintra   the intra MBs coded in dependency order (an intra MB waits only
+//              for intra neighbours; inter ones are already reconstructed).
+// The one approximation is that intra costs and sub-modes are computed from
+// the source frame's neighbours instead of the reconstruction's.
+
+// Neighbours of an MB from a plane (gather_nb, fdv.h:5201-5212): luma corner +
+// top row into s.hbytes, left column into s.lcol[0..15]; with `chroma`, also
+// the chroma corners + top rows (s.ctop) and left columns (s.lcol[16..31]).
+__device__ __forceinline__ void nb_load(Smem &s, const uint8_t *py, int ys, const uint8_t *pu,
+                                        const uint8_t *pv, int cs, int bx, int by, int ht, int hl,
+                                        bool chroma) {
+    const int tid = threadIdx.x, cbx = bx >> 1, cby = by >> 1;
+    if (tid < 16) s.hbytes[1 + tid] = ht ? py[(ptrdiff_t)(by - 1) * ys + bx + tid] : 0;
+    else if (tid == 16) s.hbytes[0] = (ht && hl) ? py[(ptrdiff_t)(by - 1) * ys + bx - 1] : 128;
+    else if (tid >= 32 && tid < 48) s.lcol[tid - 32] = hl ? py[(ptrdiff_t)(by + tid - 32) * ys + bx - 1] : 0;
+    if (chroma) {
+        if (tid >= 64 && tid < 82) {
+            const int pl = (tid - 64) / 9, k = (tid - 64) % 9;
+            const uint8_t *p = pl ? pv : pu;
+            s.ctop[pl][k] = k ? (ht ? p[(ptrdiff_t)(cby - 1) * cs + cbx + k - 1] : 0)
+                              : ((ht && hl) ? p[(ptrdiff_t)(cby - 1) * cs + cbx - 1] : 128);
+        } else if (tid >= 96 && tid < 112) {
+            const int pl = (tid - 96) >> 3, k = (tid - 96) & 7;
+            const uint8_t *p = pl ? pv : pu;
+            s.lcol[16 + 8 * pl + k] = hl ? p[(ptrdiff_t)(cby + k) * cs + cbx - 1] : 0;
+        }
+    }
+}
+
+__device__ __forceinline__ int mb_count(const fdvg_seq &q) { return (q.w >> 4) * (q.h >> 4); }
+
+extern "C" __global__ void __launch_bounds__(256)
+k_pcost(const fdvg_seq *__restrict__ seqs, fdvg_pcost *__restrict__ pcs) {
+    __shared__ Smem s;
+    __shared__ int32_t sred[FDVG_NSC][4];
+    fdvg_seq q = seqs[blockIdx.y];
+    const int i = blockIdx.x;
+    if (i >= mb_count(q)) return;
+    q.given = nullptr;
+    const int tid = threadIdx.x, wave = tid >> 6;
+    const int mw = q.w >> 4, mx = i % mw, my = i / mw, bx = mx * 16, by = my * 16;
+    const int mb = q.mb0 + i;
+    const fdvg_metab *T = &q.metab[i];
+    fdvg_pcost *P = &pcs[mb];
+    mb_stage(s, q, mb, bx, by);
+    nb_load(s, q.cy, q.w, nullptr, nullptr, 0, bx, by, my > 0, mx > 0, false);
+    __syncthreads();
+
+    // SKIP SSDs of likely merge candidates: (0,0), and the table vectors of the
+    // left, above, above-right (16x16 on both references, last quadrant), above-
+    // left and left-left MBs; deduplicated, references that exist only
+    int ncs = 0, csx[FDVG_NSC], csy[FDVG_NSC], csr[FDVG_NSC];
+    auto add = [&](int x, int y, int r) {
+        if (r >= q.navail || ncs >= FDVG_NSC) return;
+        for (int k = 0; k < ncs; ++k) if (csx[k] == x && csy[k] == y && csr[k] == r) return;
+        csx[ncs] = x; csy[ncs] = y; csr[ncs] = r; ++ncs;
+    };
+    add(0, 0, 0);
+    const int nbx[5] = {mx - 1, mx, mx + 1, mx - 1, mx - 2}, nby[5] = {my, my - 1, my - 1, my - 1, my};
+    for (int k = 0; k < 5; ++k) {
+        if (nbx[k] < 0 || nbx[k] >= mw || nby[k] < 0) continue;
+        const fdvg_metab *A = &q.metab[nby[k] * mw + nbx[k]];
+        add(A->mv16[0][0], A->mv16[0][1], 0);
+        if (k < 3) { add(A->mv16[1][0], A->mv16[1][1], 1); add(A->mv8[3][0], A->mv8[3][1], 0); }
+    }
+    {
+        const int x = tid & 15, y = tid >> 4;
+        for (int c = 0; c < ncs; ++c) {
+            const int d = s.cur[tid] - lumaq(s.win[csr[c]], x, y, csx[c], csy[c]);
+            const int e = wave_sum(d * d);
+            if ((tid & 63) == 0) sred[c][wave] = e;
+        }
+    }
+    __syncthreads();
+    if (tid < FDVG_NSC) {
+        const int c = tid;
+        P->scx[c] = (int16_t)(c < ncs ? csx[c] : 0);
+        P->scy[c] = (int16_t)(c < ncs ? csy[c] : 0);
+        P->scr[c] = (uint8_t)(c < ncs ? csr[c] : 0);
+        P->sce[c] = c < ncs ? sred[c][0] + sred[c][1] + sred[c][2] + sred[c][3] : 0;
+    }
+    if (tid == 0) P->nsc = (uint8_t)ncs;
+    if (T->flags & 1) {                      // a forced SKIP needs no mode costs
+        if (tid == 0) { P->d16[0] = P->d16[1] = P->b16[0] = P->b16[1] = 0; P->dn = P->bn = P->d8 = P->b8 = 0; P->nsub = 0; }
+        return;
+    }
+    const int qp = q.qp;
+    const double lambda = q.lambda;
+    // INTER16 on each reference
+    for (int r = 0; r < q.navail; ++r) {
+        s.pred[tid] = (uint8_t)lumaq(s.win[r], tid & 15, tid >> 4, T->mv16[r][0], T->mv16[r][1]);
+        __syncthreads();
+        code_luma(s, s.pred, s.recT, s.levT, s.flagT, qp, lambda);
+        if (tid == 0) {
+            P->d16[r] = s.ws.e[0] + s.ws.e[1] + s.ws.e[2] + s.ws.e[3];
+            P->b16[r] = 6 + s.ws.b[0] + s.ws.b[1] + s.ws.b[2] + s.ws.b[3];
+        }
+        __syncthreads();
+    }
+    // INTRA 16x16 from the source neighbours: SATD screen, then one coding
+    const Nb nbL = nb_setup(&s.hbytes[1], s.lcol, s.hbytes[0], 16, my > 0, mx > 0);
+    if (wave == 0) {
+        int sat[4];
+        satd_modes(s.cur, nbL, &s.hbytes[1], s.lcol, sat);
+        if (tid == 0) for (int m = 0; m < 4; ++m) s.me[6][m] = sat[m];
+    }
+    __syncthreads();
+    int sel = -1, sbest = 0;
+    for (int m = 0; m < 4; ++m) {
+        if (!nn_ok(m, nbL)) continue;
+        if (sel < 0 || s.me[6][m] < sbest) { sbest = s.me[6][m]; sel = m; }
+    }
+    s.pred[tid] = (uint8_t)ipred(sel, nbL, &s.hbytes[1], s.lcol, 16, tid & 15, tid >> 4);
+    __syncthreads();
+    code_luma(s, s.pred, s.recN, s.levN, s.flagN, qp, lambda);
+    if (tid == 0) {
+        P->dn = s.ws.e[0] + s.ws.e[1] + s.ws.e[2] + s.ws.e[3];
+        P->bn = 6 + s.ws.b[0] + s.ws.b[1] + s.ws.b[2] + s.ws.b[3];
+        P->nsub = (uint8_t)sel;
+    }
+    __syncthreads();
+    // INTER8 from the quadrant vectors
+    {
+        const int x = tid & 15, y = tid >> 4, qd = (y >> 3) * 2 + (x >> 3);
+        s.pred[tid] = (uint8_t)lumaq(s.win[0], x, y, T->mv8[qd][0], T->mv8[qd][1]);
+    }
+    __syncthreads();
+    code_luma(s, s.pred, s.rec8, s.lev8, s.flag8, qp, lambda);
+    if (tid == 0) {
+        P->d8 = s.ws.e[0] + s.ws.e[1] + s.ws.e[2] + s.ws.e[3];
+        P->b8 = 3 + s.ws.b[0] + s.ws.b[1] + s.ws.b[2] + s.ws.b[3];
+    }
+}
+
+// Motion record handoff of the decision wavefront: tag | inter | ref | x | y.
+__device__ __forceinline__ uint64_t mrec_pack(uint64_t tag, int inter, int ref, int x, int y) {
+    return tag | ((uint64_t)(inter & 1) << 33) | ((uint64_t)(ref & 1) << 32) |
+           ((uint64_t)(uint16_t)x << 16) | (uint64_t)(uint16_t)y;
+}
+
+// Quarter-pel luma sample from a small tile of stride S whose integer sample
+// for the output pixel is at (X, Y): the H.264 positions of fdv.h:2931-2990.
+template <int S> __device__ __forceinline__ int t_h(const uint8_t *w, int x, int y) {
+    const uint8_t *p = w + y * S + x;
+    return tap6(p[-2], p[-1], p[0], p[1], p[2], p[3]);
+}
+template <int S> __device__ __forceinline__ int t_v(const uint8_t *w, int x, int y) {
+    const uint8_t *p = w + y * S + x;
+    return tap6(p[-2 * S], p[-S], p[0], p[S], p[2 * S], p[3 * S]);
+}
+template <int S> __device__ __forceinline__ int t_b(const uint8_t *w, int x, int y) { return clip8((t_h<S>(w, x, y) + 16) >> 5); }
+template <int S> __device__ __forceinline__ int t_hh(const uint8_t *w, int x, int y) { return clip8((t_v<S>(w, x, y) + 16) >> 5); }
+template <int S> __device__ __forceinline__ int t_j(const uint8_t *w, int x, int y) {
+    const int v = tap6(t_h<S>(w, x, y - 2), t_h<S>(w, x, y - 1), t_h<S>(w, x, y), t_h<S>(w, x, y + 1),
+                       t_h<S>(w, x, y + 2), t_h<S>(w, x, y + 3));
+    return clip8((v + 512) >> 10);
+}
+template <int S> __device__ __forceinline__ int lumaq_t(const uint8_t *w, int X, int Y, int ph) {
+    const int G = w[Y * S + X];
+    switch (ph) {
+    case 0:  return G;
+    case 1:  return av2(G, t_b<S>(w, X, Y));
+    case 2:  return t_b<S>(w, X, Y);
+    case 3:  return av2(t_b<S>(w, X, Y), w[Y * S + X + 1]);
+    case 4:  return av2(G, t_hh<S>(w, X, Y));
+    case 5:  return av2(t_b<S>(w, X, Y), t_hh<S>(w, X, Y));
+    case 6:  return av2(t_b<S>(w, X, Y), t_j<S>(w, X, Y));
+    case 7:  return av2(t_b<S>(w, X, Y), t_hh<S>(w, X + 1, Y));
+    case 8:  return t_hh<S>(w, X, Y);
+    case 9:  return av2(t_hh<S>(w, X, Y), t_j<S>(w, X, Y));
+    case 10: return t_j<S>(w, X, Y);
+    case 11: return av2(t_j<S>(w, X, Y), t_hh<S>(w, X + 1, Y));
+    case 12: return av2(t_hh<S>(w, X, Y), w[(Y + 1) * S + X]);
+    case 13: return av2(t_hh<S>(w, X, Y), t_b<S>(w, X, Y + 1));
+    case 14: return av2(t_j<S>(w, X, Y), t_b<S>(w, X, Y + 1));
+    default: return av2(t_hh<S>(w, X + 1, Y), t_b<S>(w, X, Y + 1));
+    }
+}
+
+// The decision wavefront of one sequence inside one workgroup: wave w replays
+// rows w, w + NW, ...; a row's motion records go to an LDS ring slot per row
+// and its progress to prog[w], which the wave of the row below polls.  Each
+// wave stages its row's costs and tables PR_CHUNK MBs at a time.
+#ifdef FDVG_PHASE_PROF
+#define RPP(k) do { if (L == 0) { long long t_ = clock64(); atomicAdd(&g_phase[8 + (k)], (unsigned long long)(t_ - rp_last)); rp_last = t_; } } while (0)
+#else
+#define RPP(k) do { } while (0)
+#endif
+#define PR_CHUNK 8
+#define PR_TS 24                         // tile stride: 21 samples a row
+struct PrWave {
+    fdvg_pcost pc[PR_CHUNK];
+    fdvg_metab mt[PR_CHUNK];
+    uint8_t cur[PR_CHUNK][256];          // the source MBs, for SSDs outside the precomputed set
+    uint8_t tile[21 * PR_TS];
+};
+
+// A chunk of a row in flight: every lane's share of the costs, tables and
+// source MBs, fixed trip counts so the loads issue back to back.
+#define PR_PW ((PR_CHUNK * (int)sizeof(fdvg_pcost) / 4 + 63) / 64)
+#define PR_MW ((PR_CHUNK * (int)sizeof(fdvg_metab) / 4 + 63) / 64)
+#define PR_CW (PR_CHUNK * 64 / 64)
+struct PrPre {
+    uint32_t p[PR_PW], m[PR_MW], c[PR_CW];
+    int n;
+};
+__device__ __forceinline__ void pr_load(PrPre &R, const fdvg_seq &q, const fdvg_pcost *pcs, int my, int mx0, int L) {
+    const int mw = q.w >> 4, i0 = my * mw + mx0;
+    const int n = mw - mx0 < PR_CHUNK ? mw - mx0 : PR_CHUNK;
+    const uint32_t *gp = (const uint32_t *)(pcs + q.mb0 + i0);
+    const uint32_t *gm = (const uint32_t *)(q.metab + i0);
+    const int np = n * (int)(sizeof(fdvg_pcost) / 4), nm = n * (int)(sizeof(fdvg_metab) / 4);
+#pragma unroll
+    for (int j = 0; j < PR_PW; ++j) { const int k = L + 64 * j; R.p[j] = k < np ? gp[k] : 0u; }
+#pragma unroll
+    for (int j = 0; j < PR_MW; ++j) { const int k = L + 64 * j; R.m[j] = k < nm ? gm[k] : 0u; }
+#pragma unroll
+    for (int j = 0; j < PR_CW; ++j) {
+        const int k = L + 64 * j, m = k >> 6, rr = (k >> 2) & 15, cc = (k & 3) * 4;
+        R.c[j] = m < n ? *(const uint32_t *)(q.cy + (size_t)(my * 16 + rr) * q.w + (mx0 + m) * 16 + cc) : 0u;
+    }
+    R.n = n;
+}
+__device__ __forceinline__ void pr_store(PrWave &W, const PrPre &R, int L) {
+    uint32_t *lp = (uint32_t *)W.pc, *lm = (uint32_t *)W.mt, *lc = (uint32_t *)W.cur;
+    const int np = R.n * (int)(sizeof(fdvg_pcost) / 4), nm = R.n * (int)(sizeof(fdvg_metab) / 4);
+#pragma unroll
+    for (int j = 0; j < PR_PW; ++j) { const int k = L + 64 * j; if (k < np) lp[k] = R.p[j]; }
+#pragma unroll
+    for (int j = 0; j < PR_MW; ++j) { const int k = L + 64 * j; if (k < nm) lm[k] = R.m[j]; }
+#pragma unroll
+    for (int j = 0; j < PR_CW; ++j) lc[L + 64 * j] = R.c[j];
+}
+
+extern "C" __global__ void __launch_bounds__(512)
+k_prepair(const fdvg_seq *__restrict__ seqs, int mwmax, unsigned *ctl,
+          const fdvg_pcost *__restrict__ pcs, fdvg_mbinfo *__restrict__ given, fdvg_pctx *__restrict__ pctx,
+          unsigned *__restrict__ ilist) {
+    extern __shared__ uint64_t dyn[];
+    const int NW = blockDim.x >> 6, wv = threadIdx.x >> 6, L = threadIdx.x & 63;
+    uint64_t *ring = dyn;                                    // NW slots x mwmax motion records
+    int *prog = (int *)(ring + (size_t)NW * mwmax);          // NW progress words
+    PrWave *pw = (PrWave *)(((uintptr_t)(prog + NW) + 15) & ~(uintptr_t)15);
+    PrWave &W = pw[wv];
+    const fdvg_seq q = seqs[blockIdx.x];
+    const int mw = q.w >> 4, mh = q.h >> 4;
+    const double lambda = q.lambda;
+    if (L == 0) __hip_atomic_store(&prog[wv], -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __syncthreads();
+#ifdef FDVG_PHASE_PROF
+    long long rp_last = clock64();
+#endif
+    for (int my = wv; my < mh; my += NW) {
+        uint64_t *cur_slot = ring + (size_t)(my % NW) * mwmax;
+        const uint64_t *above = ring + (size_t)((my + NW - 1) % NW) * mwmax;
+        int *up = &prog[(wv + NW - 1) % NW];
+        int pmx = 0, pmy = 0;
+        PrPre pre;
+        for (int mx = 0; mx < mw; ++mx) {
+            const int i = my * mw + mx, mb = q.mb0 + i, bx = mx * 16, by = my * 16;
+            if (mx % PR_CHUNK == 0) {
+                // this chunk's costs, tables and source MBs into LDS (from the
+                // registers loaded during the previous chunk, or now at a row
+                // start), then the next chunk's loads issued into registers
+                if (mx == 0) pr_load(pre, q, pcs, my, 0, L);
+                pr_store(W, pre, L);
+                wave_sync();
+                if (mx + PR_CHUNK < mw) pr_load(pre, q, pcs, my, mx + PR_CHUNK, L);
+            }
+            RPP(0);
+            const fdvg_pcost &P = W.pc[mx % PR_CHUNK];
+            const fdvg_metab &T = W.mt[mx % PR_CHUNK];
+            // everything this MB reads from LDS, before the wait
+            const int nsc = P.nsc;
+            const int sx_l = L < FDVG_NSC ? P.scx[L] : 0, sy_l = L < FDVG_NSC ? P.scy[L] : 0;
+            const int sr_l = L < FDVG_NSC ? P.scr[L] : 0, se_l = L < FDVG_NSC ? P.sce[L] : 0;
+            const int d16_0 = P.d16[0], d16_1 = P.d16[1], b16_0 = P.b16[0], b16_1 = P.b16[1];
+            const int pdn = P.dn, pbn = P.bn, pd8 = P.d8, pb8 = P.b8, psub = P.nsub;
+            const int m0x = T.mv16[0][0], m0y = T.mv16[0][1], m1x = T.mv16[1][0], m1y = T.mv16[1][1];
+            const int s16_0 = T.sad16[0], s16_1 = T.sad16[1], tfl = T.flags;
+            int q8x[4], q8y[4];
+#pragma unroll
+            for (int qd = 0; qd < 4; ++qd) { q8x[qd] = T.mv8[qd][0]; q8y[qd] = T.mv8[qd][1]; }
+            const int ssad = T.sad8[0] + T.sad8[1] + T.sad8[2] + T.sad8[3];
+            // the row above must have finished the above-right MB (or the last one)
+            if (my > 0) {
+                const int need = (my - 1) * 65536 + (mx + 2 < mw ? mx + 2 : mw);
+                int v = __hip_atomic_load(up, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+                for (unsigned spins = 0; v < need; ++spins) {
+                    if (spins > (1u << 24)) { if (L == 0) atomicOr(ctl + 1, 1u); break; }
+                    __builtin_amdgcn_s_sleep(1);
+                    v = __hip_atomic_load(up, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+                }
+            }
+            RPP(1);
+            const uint64_t va = my > 0 ? above[mx] : 0;
+            const uint64_t vr = (my > 0 && mx + 1 < mw) ? above[mx + 1] : 0;
+            // merge list: predictor, then the inter MBs above and above-right (fdv.h:5711-5732)
+            int cx0 = pmx, cy0 = pmy, cx1 = 0, cy1 = 0, cr1 = 0, cx2 = 0, cy2 = 0, cr2 = 0, ncand = 1;
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                const uint64_t w = k ? vr : va;
+                if (!((w >> 33) & 1) || ncand >= 3) continue;
+                const int x = (int16_t)((w >> 16) & 0xffff), y = (int16_t)(w & 0xffff), r = (int)((w >> 32) & 1);
+                const bool dup = (cx0 == x && cy0 == y && r == 0) || (ncand > 1 && cx1 == x && cy1 == y && cr1 == r);
+                if (dup) continue;
+                if (ncand == 1) { cx1 = x; cy1 = y; cr1 = r; } else { cx2 = x; cy2 = y; cr2 = r; }
+                ++ncand;
+            }
+            // SKIP SSDs: from the precomputed set, else from a 21x21 tile of the reference
+            int ssd[3] = {0, 0, 0};
+            const uint8_t *curb = W.cur[mx % PR_CHUNK];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                if (c >= ncand) continue;
+                const int x = c == 0 ? cx0 : c == 1 ? cx1 : cx2, y = c == 0 ? cy0 : c == 1 ? cy1 : cy2;
+                const int r = c == 0 ? 0 : c == 1 ? cr1 : cr2;
+                const bool hit = L < nsc && sx_l == x && sy_l == y && sr_l == r;
+                const uint64_t hm = __ballot(hit);
+                if (hm) {
+                    ssd[c] = __builtin_amdgcn_readlane(se_l, __builtin_ctzll(hm));
+                    RPP(2);
+                } else {
+                    const uint8_t *rp = (r ? q.ry[1] : q.ry[0]) +
+                                        (ptrdiff_t)(by + (y >> 2) - 2) * q.ys + bx + (x >> 2) - 2;
+                    uint8_t tv[7];
+#pragma unroll
+                    for (int j = 0; j < 7; ++j) {
+                        const int k = L + 64 * j;
+                        tv[j] = k < 21 * 21 ? rp[(ptrdiff_t)(k / 21) * q.ys + k % 21] : 0;
+                    }
+#pragma unroll
+                    for (int j = 0; j < 7; ++j) {
+                        const int k = L + 64 * j;
+                        if (k < 21 * 21) W.tile[(k / 21) * PR_TS + k % 21] = tv[j];
+                    }
+                    wave_sync();
+                    const int ph = ((y & 3) << 2) | (x & 3);
+                    int e = 0;
+                    for (int k = L; k < 256; k += 64) {
+                        const int d = curb[k] - lumaq_t<PR_TS>(W.tile, (k & 15) + 2, (k >> 4) + 2, ph);
+                        e += d * d;
+                    }
+                    ssd[c] = wave_sum(e);
+                    wave_sync();
+                    RPP(3);
+                }
+            }
+            int Ds = ssd[0], bmi = 0;
+            if (ncand > 1 && ssd[1] < Ds) { Ds = ssd[1]; bmi = 1; }
+            if (ncand > 2 && ssd[2] < Ds) { Ds = ssd[2]; bmi = 2; }
+            RPP(4);
+            // the reference's gates and commit on the stored costs
+            const double Js = (double)Ds + lambda * (double)(3 + (ncand > 1 ? 3 : 0));
+            double Jbest = Js;
+            const bool skip_wins = Js <= lambda * (double)FL_INTRA || (tfl & 1);
+            double Ji = 1e30;
+            int iref = 0, sad16 = 0;
+            if (!skip_wins) {
+                const int bb = b16_0 + egbits(zmap(m0x - pmx)) + egbits(zmap(m0y - pmy));
+                const double J = (double)d16_0 + lambda * (double)bb;
+                if (J < Jbest) Jbest = J;
+                Ji = J; sad16 = s16_0;
+                if (q.navail > 1 && !(Jbest <= lambda * (double)FL_INTER16)) {
+                    const int bb1 = b16_1 + egbits(zmap(m1x - pmx)) + egbits(zmap(m1y - pmy));
+                    const double J1 = (double)d16_1 + lambda * (double)bb1;
+                    if (J1 < Jbest) Jbest = J1;
+                    if (J1 < Ji) { Ji = J1; iref = 1; sad16 = s16_1; }
+                }
+            }
+            double Jn = 1e30;
+            if (!skip_wins && Jbest > lambda * (double)FL_INTRA) {
+                Jn = (double)pdn + lambda * (double)pbn;
+                if (Jn < Jbest) Jbest = Jn;
+            }
+            double J8 = 1e30;
+            if (!skip_wins && Jbest > lambda * (double)FL_INTER8) {
+                if (!(sad16 > 0 && ssad * 10 >= sad16 * 9)) {
+                    int b8 = pb8, qpx = pmx, qpy = pmy;
+#pragma unroll
+                    for (int qd = 0; qd < 4; ++qd) {
+                        b8 += egbits(zmap(q8x[qd] - qpx)) + egbits(zmap(q8y[qd] - qpy));
+                        qpx = q8x[qd]; qpy = q8y[qd];
+                    }
+                    J8 = (double)pd8 + lambda * (double)b8;
+                }
+            }
+            int mode = 0;
+            double Jm = Js;
+            if (Ji < Jm) { Jm = Ji; mode = 1; }
+            if (Jn < Jm) { Jm = Jn; mode = 2; }
+            if (J8 < Jm) { Jm = J8; mode = 3; }
+            int nx = 0, ny = 0, nr = 0, ni = 1;
+            if (mode == 0) {
+                nx = bmi == 0 ? cx0 : bmi == 1 ? cx1 : cx2; ny = bmi == 0 ? cy0 : bmi == 1 ? cy1 : cy2;
+                nr = bmi == 0 ? 0 : bmi == 1 ? cr1 : cr2;
+            } else if (mode == 1) { nx = iref ? m1x : m0x; ny = iref ? m1y : m0y; nr = iref; }
+            else if (mode == 2) ni = 0;
+            else { nx = q8x[3]; ny = q8y[3]; }
+            RPP(5);
+            if (L == 0) {
+                // the row below needs only the motion record: publish it first
+                cur_slot[mx] = mrec_pack(0, ni, nr, nx, ny);
+                __hip_atomic_store(&prog[wv], my * 65536 + mx + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+                fdvg_mbinfo G;
+                G.mode = (uint8_t)mode; G.ref = (uint8_t)(mode == 0 || mode == 1 ? nr : 0);
+                G.merge_idx = (uint8_t)(mode == 0 ? bmi : 0); G.sub = (uint8_t)(mode == 2 ? psub : 0);
+                for (int k = 0; k < 4; ++k) { G.mv[k][0] = 0; G.mv[k][1] = 0; }
+                if (mode == 0 || mode == 1) { G.mv[0][0] = (int16_t)nx; G.mv[0][1] = (int16_t)ny; }
+                if (mode == 3) for (int k = 0; k < 4; ++k) { G.mv[k][0] = (int16_t)q8x[k]; G.mv[k][1] = (int16_t)q8y[k]; }
+                given[mb] = G;
+                fdvg_pctx X;
+                X.pmx = (int16_t)pmx; X.pmy = (int16_t)pmy; X.ncand = (uint8_t)ncand;
+                X.cx[0] = (int16_t)cx0; X.cy[0] = (int16_t)cy0; X.cr[0] = 0;
+                X.cx[1] = (int16_t)cx1; X.cy[1] = (int16_t)cy1; X.cr[1] = (uint8_t)cr1;
+                X.cx[2] = (int16_t)cx2; X.cy[2] = (int16_t)cy2; X.cr[2] = (uint8_t)cr2;
+                pctx[mb] = X;
+            }
+            RPP(6);
+            if (mode != 2) { pmx = nx; pmy = ny; }
+        }
+    }
+}
+
+// The intra MBs of the batch in MB order (sequence by sequence, raster within
+// each), so every intra MB's left / top / top-left intra neighbours come first:
+// ilist[0] = count, then the MB indices.  One workgroup.
+extern "C" __global__ void __launch_bounds__(1024)
+k_plist(const fdvg_mbinfo *__restrict__ given, int total_mb, unsigned *__restrict__ ilist) {
+    __shared__ unsigned wsum[16];
+    __shared__ unsigned base;
+    const int t = threadIdx.x, L = t & 63, wv = t >> 6;
+    if (t == 0) base = 0;
+    __syncthreads();
+    for (int c = 0; c < total_mb; c += 1024) {
+        const int mb = c + t;
+        const bool is = mb < total_mb && given[mb].mode == 2;
+        const uint64_t bal = __ballot(is);
+        const unsigned before = __builtin_popcountll(bal & ((1ull << L) - 1));
+        if (L == 0) wsum[wv] = __builtin_popcountll(bal);
+        __syncthreads();
+        unsigned off = base;
+        for (int k = 0; k < wv; ++k) off += wsum[k];
+        if (is) ilist[1 + off + before] = (unsigned)mb;
+        __syncthreads();
+        if (t == 0) { unsigned tot = 0; for (int k = 0; k < 16; ++k) tot += wsum[k]; base += tot; }
+        __syncthreads();
+    }
+    if (t == 0) ilist[0] = base;
+}
+
+// LDS bytes of k_prepair for NW waves and rows up to mwmax MBs wide.
+static size_t prepair_lds(int nw, int mwmax) {
+    return ((size_t)nw * mwmax * 8 + (size_t)nw * 4 + 15) / 16 * 16 + (size_t)nw * sizeof(PrWave);
+}
+
+__device__ __forceinline__ void ctx_from(MbCtx &c, const fdvg_pctx &X, int ht, int hl) {
+    c.pmx = X.pmx; c.pmy = X.pmy; c.ncand = X.ncand; c.ht = ht; c.hl = hl;
+    for (int k = 0; k < 3; ++k) { c.cx[k] = X.cx[k]; c.cy[k] = X.cy[k]; c.cr[k] = X.cr[k]; }
+}
+
+// Decided non-intra MBs, coded exactly (the given-decisions path of mb_decide).
+extern "C" __global__ void __launch_bounds__(256)
+k_pencode(const fdvg_seq *__restrict__ seqs, const fdvg_mbinfo *__restrict__ given,
+          const fdvg_pctx *__restrict__ pctx, fdvg_mbrec *__restrict__ recs) {
+    __shared__ Smem s;
+    fdvg_seq q = seqs[blockIdx.y];
+    const int i = blockIdx.x;
+    if (i >= mb_count(q)) return;
+    const int mb = q.mb0 + i;
+    const fdvg_mbinfo *gv = &given[mb];
+    if (gv->mode == 2) return;
+    q.given = given + q.mb0;
+    const int mw = q.w >> 4, mx = i % mw, my = i / mw, bx = mx * 16, by = my * 16;
+    mb_stage(s, q, mb, bx, by);
+    MbCtx c;
+    ctx_from(c, pctx[mb], my > 0, mx > 0);
+    __syncthreads();
+    int par = 0;
+    MbDec D;
+    mb_decide(s, q, c, gv, nullptr, par, D);
+    mb_finish(s, q, mb, bx, by, c, D, &recs[mb]);
+}
+
+// Decided intra MBs, in the order of ilist (every intra neighbour first): each
+// waits for its left, top and top-left neighbours when those are intra, then
+// reads the unfiltered reconstruction around it.
+extern "C" __global__ void __launch_bounds__(256)
+k_pintra(const fdvg_seq *__restrict__ seqs, int nseq, const fdvg_mbinfo *__restrict__ given,
+         const fdvg_pctx *__restrict__ pctx, fdvg_mbrec *__restrict__ recs, unsigned *ctl,
+         const unsigned *__restrict__ ilist, unsigned *__restrict__ done) {
+    __shared__ Smem s;
+    __shared__ int job_s;
+    const int tid = threadIdx.x;
+    const unsigned n = ilist[0];
+    for (;;) {
+        if (tid == 0) job_s = (int)atomicAdd(ctl + 4, 1u);
+        __syncthreads();
+        const unsigned job = (unsigned)job_s;
+        __syncthreads();
+        if (job >= n) return;
+        const int mb = (int)ilist[1 + job];
+        int sq = 0;
+        for (int k = 1; k < nseq; ++k) if (mb >= seqs[k].mb0) sq = k;
+        fdvg_seq q = seqs[sq];
+        q.given = given + q.mb0;
+        const int i = mb - q.mb0, mw = q.w >> 4, mx = i % mw, my = i / mw, bx = mx * 16, by = my * 16;
+        if (tid < 3) {
+            // left, top, top-left
+            const int ax = tid == 0 ? mx - 1 : tid == 1 ? mx : mx - 1, ay = tid == 0 ? my : my - 1;
+            if (ax >= 0 && ay >= 0 && given[q.mb0 + ay * mw + ax].mode == 2) {
+                const unsigned *p = done + q.mb0 + ay * mw + ax;
+                unsigned v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                for (unsigned spins = 0; v == 0; ++spins) {
+                    if (spins > (1u << 24)) { atomicOr(ctl + 1, 1u); break; }
+                    __builtin_amdgcn_s_sleep(1);
+                    v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
+            }
+        }
+        __syncthreads();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");   // every wave reads the neighbours
+        mb_stage(s, q, mb, bx, by);
+        nb_load(s, q.oy, q.ys, q.ou, q.ov, q.cs, bx, by, my > 0, mx > 0, true);
+        MbCtx c;
+        ctx_from(c, pctx[mb], my > 0, mx > 0);
+        __syncthreads();
+        int par = 0;
+        MbDec D;
+        mb_decide(s, q, c, &given[mb], nullptr, par, D);
+        mb_finish(s, q, mb, bx, by, c, D, &recs[mb]);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");   // every wave's reconstruction stores
+        __syncthreads();
+        if (tid == 0) __hip_atomic_store(done + mb, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+extern "C" int fdvg_launch_parallel(const fdvg_seq *seqs, int nseq, int maxmb, int maxmh, int mwmax,
+                                    unsigned *ctl, fdvg_pcost *pcs, fdvg_mbinfo *given, fdvg_pctx *pctx,
+                                    unsigned *ilist, unsigned *done, int total_mb,
+                                    fdvg_mbrec *recs, hipStream_t st) {
+    static int cap_i = 0;
+    if (!cap_i) {
+        int dev = 0, ncu = 0, pi = 0;
+        if (hipGetDevice(&dev) != hipSuccess) return -1;
+        if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return -1;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&pi, (const void *)k_pintra, 256, 0) != hipSuccess) return -1;
+        cap_i = ncu * (pi > 0 ? pi : 1);
+    }
+    int nw = maxmh < 8 ? maxmh : 8;
+    size_t lds = prepair_lds(nw, mwmax);
+    while (nw > 1 && lds > 64 * 1024) lds = prepair_lds(--nw, mwmax);
+    if (lds > 64 * 1024) return -1;
+    hipLaunchKernelGGL(k_pcost, dim3(maxmb, nseq), dim3(256), 0, st, seqs, pcs);
+    hipLaunchKernelGGL(k_prepair, dim3(nseq), dim3(64 * nw), lds, st, seqs, mwmax, ctl, (const fdvg_pcost *)pcs,
+                       given, pctx, ilist);
+    hipLaunchKernelGGL(k_plist, dim3(1), dim3(1024), 0, st, (const fdvg_mbinfo *)given, total_mb, ilist);
+    hipLaunchKernelGGL(k_pencode, dim3(maxmb, nseq), dim3(256), 0, st, seqs, (const fdvg_mbinfo *)given,
+                       (const fdvg_pctx *)pctx, recs);
+    hipLaunchKernelGGL(k_pintra, dim3(total_mb < cap_i ? total_mb : cap_i), dim3(256), 0, st, seqs, nseq,
+                       (const fdvg_mbinfo *)given, (const fdvg_pctx *)pctx, recs, ctl, (const unsigned *)ilist, done);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 // ------------------------------------------------------------ host launchers

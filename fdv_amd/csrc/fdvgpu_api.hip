@@ -26,6 +26,9 @@ extern "C" int fdvg_launch_zero(uint32_t *, int, hipStream_t);
 extern "C" int fdvg_launch_symbols(const void *, int, int, const fdvg_mbrec *, uint32_t *, uint32_t *,
                                    uint32_t *, uint8_t *, size_t, hipStream_t);
 extern "C" int fdvg_launch_metab(const fdvg_seq *, int, int, void *, hipStream_t);
+extern "C" int fdvg_launch_parallel(const fdvg_seq *, int, int, int, int, unsigned *, fdvg_pcost *,
+                                    fdvg_mbinfo *, fdvg_pctx *, unsigned *, unsigned *, int,
+                                    fdvg_mbrec *, hipStream_t);
 
 static thread_local char g_err[256];
 static void seterr(const char *fmt, ...) {
@@ -52,7 +55,7 @@ struct SeqMem {
 };
 
 struct fdvgpu {
-    int nseq, nsrc, nrows, total_mb, maxmb, maxpx;
+    int nseq, nsrc, nrows, total_mb, maxmb, maxpx, maxmh, maxmw;
     SeqMem *S;
     hipStream_t st;
     fdvg_seq *seq_d, *seq_h;        // seq_h: RING staging copies, one per frame in flight
@@ -68,6 +71,12 @@ struct fdvgpu {
     SymSeq *sym_d;
     uint32_t *cnt_d, *totals_d, *bases_d, *bases_h;
     uint8_t *compact_d, *compact_h;
+    // block-parallel decisions
+    fdvg_pcost *pcost_d;
+    fdvg_pctx *pctx_d;
+    fdvg_mbinfo *pgiven_d;
+    uint64_t *mrec_d;               // motion-record handoff of the decision wavefront
+    unsigned *ilist_d, *done_d;     // intra MBs in dependency order (count first), per-MB done flags
     size_t compact_cap;
     uint32_t *flags_h;
     hipEvent_t ev[5];
@@ -103,6 +112,8 @@ extern "C" void fdvgpu_close(fdvgpu *g) {
     (void)hipFree(g->metab_d); (void)hipFree(g->planes_d); (void)hipHostFree(g->planes_h); (void)hipFree(g->sym_d);
     (void)hipFree(g->cnt_d); (void)hipFree(g->totals_d); (void)hipFree(g->bases_d); (void)hipHostFree(g->bases_h);
     (void)hipFree(g->compact_d); (void)hipHostFree(g->compact_h); (void)hipHostFree(g->flags_h);
+    (void)hipFree(g->pcost_d); (void)hipFree(g->pctx_d); (void)hipFree(g->pgiven_d); (void)hipFree(g->mrec_d);
+    (void)hipFree(g->ilist_d); (void)hipFree(g->done_d);
     for (int i = 0; i < 5; ++i) if (g->ev[i]) (void)hipEventDestroy(g->ev[i]);
     for (int i = 0; i < 8; ++i) if (g->ring_ev[i]) (void)hipEventDestroy(g->ring_ev[i]);
     if (g->st) (void)hipStreamDestroy(g->st);
@@ -140,6 +151,8 @@ extern "C" fdvgpu *fdvgpu_open(int nseq, const int *w, const int *h, int nsrc) {
         g->total_mb += m.nmb;
         g->nrows += m.mh;
         if (m.mh > maxmh) maxmh = m.mh;
+        if (m.mh > g->maxmh) g->maxmh = m.mh;
+        if (m.mw > g->maxmw) g->maxmw = m.mw;
         if (m.nmb > g->maxmb) g->maxmb = m.nmb;
         if ((int)((size_t)m.ys * (m.h + 2 * FDVG_BORDER)) > g->maxpx) g->maxpx = (int)((size_t)m.ys * (m.h + 2 * FDVG_BORDER));
         m.near_ = 0; m.far_ = 1; m.out_ = 2;
@@ -172,6 +185,13 @@ extern "C" fdvgpu *fdvgpu_open(int nseq, const int *w, const int *h, int nsrc) {
     CKN(hipMalloc(&g->compact_d, g->compact_cap));
     CKN(hipHostMalloc(&g->compact_h, g->compact_cap, hipHostMallocDefault));
     CKN(hipHostMalloc(&g->flags_h, 64, hipHostMallocDefault));
+    CKN(hipMalloc(&g->pcost_d, (size_t)g->total_mb * sizeof(fdvg_pcost)));
+    CKN(hipMalloc(&g->pctx_d, (size_t)g->total_mb * sizeof(fdvg_pctx)));
+    CKN(hipMalloc(&g->pgiven_d, (size_t)g->total_mb * sizeof(fdvg_mbinfo)));
+    CKN(hipMalloc(&g->mrec_d, (size_t)g->total_mb * sizeof(uint64_t)));
+    CKN(hipMemset(g->mrec_d, 0, (size_t)g->total_mb * sizeof(uint64_t)));
+    CKN(hipMalloc(&g->ilist_d, ((size_t)g->total_mb + 1) * sizeof(unsigned)));
+    CKN(hipMalloc(&g->done_d, (size_t)g->total_mb * sizeof(unsigned)));
 
     // rows in (row, sequence) order: every sequence's row r before any row r+1
     rowtab_h = (int2 *)malloc(g->nrows * sizeof(int2));
@@ -259,7 +279,13 @@ extern "C" int fdvgpu_set_given(fdvgpu *g, int seq, const fdvgpu_mbinfo *dec) {
 extern "C" int fdvgpu_pframe(fdvgpu *g, const int *src_slot, const int *qp, const double *lscale,
                              const int *navail, const int *force_skip, int me_mode) {
     if (!g || !src_slot || !qp || !lscale || !navail) { seterr("fdvgpu_pframe: bad arguments"); return -1; }
-    if (me_mode < FDVGPU_ME_DIAMOND || me_mode > FDVGPU_ME_LEARNED) { seterr("fdvgpu_pframe: bad me_mode"); return -1; }
+    if (me_mode < FDVGPU_ME_DIAMOND || me_mode > FDVGPU_ME_PARALLEL) { seterr("fdvgpu_pframe: bad me_mode"); return -1; }
+    // block-parallel decisions (learned / parallel modes) need neither held
+    // frames nor given decisions; those frames take the wavefront, whose
+    // table mode they share
+    bool par = me_mode == FDVGPU_ME_LEARNED || me_mode == FDVGPU_ME_PARALLEL;
+    for (int s = 0; s < g->nseq; ++s)
+        if ((force_skip && force_skip[s]) || g->S[s].has_given) par = false;
     for (int s = 0; s < g->nseq; ++s) {
         if (src_slot[s] < 0 || src_slot[s] >= g->nsrc || qp[s] < 0 || qp[s] > 51 ||
             navail[s] < 1 || navail[s] > 2 || !(lscale[s] > 0.0)) {
@@ -308,9 +334,16 @@ extern "C" int fdvgpu_pframe(fdvgpu *g, const int *src_slot, const int *qp, cons
         fdvg_launch_metab(g->seq_d, g->nseq, g->maxmb, g->metab_d, g->st) != 0) {
         seterr("motion table launch failed"); return -1;
     }
-    if (fdvg_launch_zero(g->ticket_d, 2, g->st) != 0) { seterr("ticket reset launch failed"); return -1; }
+    if (fdvg_launch_zero(g->ticket_d, 8, g->st) != 0) { seterr("ticket reset launch failed"); return -1; }
     CK(hipEventRecord(g->ev[1], g->st));
-    if (fdvg_launch_analyse(g->seq_d, g->rowtab_d, g->nrows, g->ticket_d, g->recs_d, g->hand_d, g->st) != 0) {
+    if (par) {
+        if (fdvg_launch_zero(g->ilist_d, 1, g->st) != 0 || fdvg_launch_zero(g->done_d, g->total_mb, g->st) != 0 ||
+            fdvg_launch_parallel(g->seq_d, g->nseq, g->maxmb, g->maxmh, g->maxmw, g->ticket_d, g->pcost_d,
+                                 g->pgiven_d, g->pctx_d, g->ilist_d, g->done_d, g->total_mb,
+                                 g->recs_d, g->st) != 0) {
+            seterr("parallel analysis launch failed: %s", hipGetErrorString(hipGetLastError())); return -1;
+        }
+    } else if (fdvg_launch_analyse(g->seq_d, g->rowtab_d, g->nrows, g->ticket_d, g->recs_d, g->hand_d, g->st) != 0) {
         seterr("analysis launch failed: %s", hipGetErrorString(hipGetLastError())); return -1;
     }
     CK(hipEventRecord(g->ev[2], g->st));

@@ -38,6 +38,29 @@ typedef struct {
     int32_t flags;      // bit0: the proposer is confident the MB is SKIP
 } fdvg_metab;
 
+// Block-parallel decisions (parallel and learned modes).  Per MB: the residual
+// SSD and bits of every candidate mode, without the vector-difference bits
+// (which depend on the predictor), and the SKIP SSDs of a set of likely merge
+// candidates (the MB's own and its neighbours' table vectors).
+#define FDVG_NSC 12
+typedef struct {
+    int32_t d16[2], b16[2];           // INTER16 on each reference (bits incl. the 6 mode/ref bits)
+    int32_t dn, bn;                   // INTRA 16x16 from the source frame's neighbours
+    int32_t d8, b8;                   // INTER8 (bits incl. the 3 mode bits)
+    int16_t scx[FDVG_NSC], scy[FDVG_NSC];
+    int32_t sce[FDVG_NSC];
+    uint8_t scr[FDVG_NSC];
+    uint8_t nsub, nsc, pad[2];
+} fdvg_pcost;
+
+// The true vector predictor and merge list of an MB once its neighbours are
+// decided (fdv.h:5680-5732).
+typedef struct {
+    int16_t pmx, pmy;
+    int16_t cx[3], cy[3];
+    uint8_t cr[3], ncand;
+} fdvg_pctx;
+
 // What the wavefront hands to the serialiser for one MB: the decision and the
 // winning candidate's quantised levels in symbolic form (region flags + levels).
 // Luma region r (raster 8x8 of the MB) keeps its levels at lev[r*64..]: four

@@ -11,7 +11,7 @@ import numpy as np
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libfdvgpu.so")
 
-ME_DIAMOND, ME_EXHAUSTIVE, ME_LEARNED = 0, 1, 2
+ME_DIAMOND, ME_EXHAUSTIVE, ME_LEARNED, ME_PARALLEL = 0, 1, 2, 3
 NS = 16
 
 MBINFO_DTYPE = np.dtype([("mode", "u1"), ("ref", "u1"), ("merge_idx", "u1"), ("sub", "u1"),

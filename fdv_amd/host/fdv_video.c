@@ -31,7 +31,7 @@ struct fdvamd_encoder {
 };
 
 fdvamd_encoder *fdvamd_encoder_open(int nseq, const int *w, const int *h, int me_mode) {
-    if (nseq <= 0 || !w || !h || me_mode < 0 || me_mode > 2) { amd_seterr("bad arguments"); return NULL; }
+    if (nseq <= 0 || !w || !h || me_mode < 0 || me_mode > 3) { amd_seterr("bad arguments"); return NULL; }
     fdvamd_encoder *e = calloc(1, sizeof *e);
     if (!e) return NULL;
     e->nseq = nseq; e->me_mode = me_mode;

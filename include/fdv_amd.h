@@ -51,7 +51,8 @@ typedef struct fdvamd_encoder fdvamd_encoder;
 
 /* An encoder for nseq sequences of the given sizes (multiples of 16).
  * me_mode: 0 the reference's diamond search (bit-exact with pframe_encode),
- * 1 exhaustive search, 2 learned proposer with exhaustive fallback. */
+ * 1 exhaustive search, 2 learned proposer with exhaustive fallback and
+ * block-parallel decisions, 3 exhaustive search with block-parallel decisions. */
 fdvamd_encoder *fdvamd_encoder_open(int nseq, const int *w, const int *h, int me_mode);
 void fdvamd_encoder_close(fdvamd_encoder *e);
 const char *fdvamd_last_error(void);

@@ -37,7 +37,9 @@ typedef struct fdvgpu fdvgpu;
 enum {
     FDVGPU_ME_DIAMOND = 0,     /* the reference's EPZS diamond (fdv.h:3375): bit-exact   */
     FDVGPU_ME_EXHAUSTIVE = 1,  /* full +-16 pel integer search + the same sub-pel step  */
-    FDVGPU_ME_LEARNED = 2      /* proposer candidates, exhaustive fallback below 0.7    */
+    FDVGPU_ME_LEARNED = 2,     /* proposer candidates, exhaustive fallback below 0.7,
+                                  block-parallel decisions                             */
+    FDVGPU_ME_PARALLEL = 3     /* exhaustive tables, block-parallel decisions           */
 };
 
 /* One MB decision (given-decisions input and fdvgpu_decisions output).

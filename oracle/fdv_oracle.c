@@ -1310,3 +1310,170 @@ size_t orc_pframe_encode(const uint8_t *cy, const uint8_t *cu, const uint8_t *cv
     free(mem); free(sb);
     return n;
 }
+
+
+/* ====================================================== parallel decisions ==
+ * The block-parallel analysis of this build's parallel and learned modes.  It
+ * is not in the reference.  Every MB's candidate costs are computed
+ * independently; a light sequential pass then replays the reference's decision
+ * logic (fdv.h:5847-6040: SKIP over the true merge list, the gates, the strict-<
+ * commit) with the true vector predictor and merge list; the decisions are then
+ * coded exactly (orc_pframe_analyse with `given`).
+ *   - costs: INTER16 on each reference and INTER8 from a motion table (the
+ *     residual SSD and bits, without the vector-difference bits, which depend on
+ *     the predictor), and INTRA 16x16 with its sub-mode screened and coded from
+ *     the *source* frame's neighbours -- the one approximation;
+ *   - repair: per MB in raster order, the true predictor and merge list, the
+ *     SKIP SSDs of the true candidates, then the reference's logic on the
+ *     stored costs plus the vector bits for the true predictor. */
+
+void orc_metab_exhaustive(const uint8_t *cy, int w, int h, const uint8_t *const RY[2], int ys,
+                          int navail, orc_metab *mt) {
+    const int mw = w / 16, mh = h / 16;
+    for (int my = 0; my < mh; ++my)
+        for (int mx = 0; mx < mw; ++mx) {
+            orc_metab *m = &mt[my * mw + mx];
+            memset(m, 0, sizeof *m);
+            const int bx = mx * 16, by = my * 16;
+            for (int r = 0; r < navail; ++r) {
+                int a, b;
+                m->sad16[r] = orc_me_search(cy, w, RY[r], ys, bx, by, 16, 16, 16, 0, 0, ORC_ME_EXHAUSTIVE, &a, &b);
+                m->mv16[r][0] = (int16_t)a; m->mv16[r][1] = (int16_t)b;
+            }
+            for (int q = 0; q < 4; ++q) {
+                int a, b;
+                m->sad8[q] = orc_me_search(cy, w, RY[0], ys, bx + (q & 1) * 8, by + (q >> 1) * 8, 8, 8, 16,
+                                           0, 0, ORC_ME_EXHAUSTIVE, &a, &b);
+                m->mv8[q][0] = (int16_t)a; m->mv8[q][1] = (int16_t)b;
+            }
+        }
+}
+
+void orc_pframe_costs_par(const uint8_t *cy, int w, int h, const uint8_t *const RY[2], int ys,
+                          int navail, int qp, double lambda, const orc_metab *mt, orc_pcost *pc) {
+    const int mw = w / 16, mh = h / 16;
+    for (int my = 0; my < mh; ++my)
+        for (int mx = 0; mx < mw; ++mx) {
+            const int bx = mx * 16, by = my * 16;
+            const orc_metab *T = &mt[my * mw + mx];
+            orc_pcost *P = &pc[my * mw + mx];
+            memset(P, 0, sizeof *P);
+            if (T->flags & 1) continue;                 /* a forced SKIP needs no costs */
+            for (int r = 0; r < navail; ++r) {
+                uint8_t pr[256], rr[256]; coded_t tc; double dd = 0.0; int bb = 6;
+                orc_mc_luma(RY[r], ys, bx, by, 16, 16, T->mv16[r][0], T->mv16[r][1], pr, 16);
+                residual(cy, w, bx, by, pr, 16, qp, lambda, rr, &tc, &dd, &bb);
+                P->d16[r] = (int32_t)dd; P->b16[r] = bb;
+            }
+            {
+                uint8_t ntop[16], nleft[16], ntl, pI[256], rn[256]; int nht, nhl;
+                neighbours(cy, w, bx, by, 16, 0, ntop, nleft, &ntl, &nht, &nhl);
+                int sel = -1, sbest = 0;
+                for (int m = 0; m < 4; ++m) {
+                    if (!nn_ok(m, 16, nht, nhl)) continue;
+                    orc_intra_nxn(m, ntop, nleft, ntl, 16, nht, nhl, pI);
+                    int sa = 0;
+                    for (int i = 0; i < 16; i += 4)
+                        for (int j = 0; j < 16; j += 4) {
+                            uint8_t pk[16];
+                            for (int k = 0; k < 4; ++k) memcpy(pk + 4 * k, pI + (i + k) * 16 + j, 4);
+                            sa += orc_satd4(cy + (by + i) * w + bx + j, w, pk);
+                        }
+                    if (sel < 0 || sa < sbest) { sbest = sa; sel = m; }
+                }
+                orc_intra_nxn(sel, ntop, nleft, ntl, 16, nht, nhl, pI);
+                coded_t cn; double dn = 0.0; int bn = 6;
+                residual(cy, w, bx, by, pI, 16, qp, lambda, rn, &cn, &dn, &bn);
+                P->dn = (int32_t)dn; P->bn = bn; P->nsub = (uint8_t)sel;
+            }
+            {
+                double d8 = 0.0; int b8 = 3;
+                for (int q = 0; q < 4; ++q) {
+                    const int qx = (q & 1) * 8, qy = (q >> 1) * 8;
+                    uint8_t p8[64], r8[64]; coded_t c8;
+                    orc_mc_luma(RY[0], ys, bx + qx, by + qy, 8, 8, T->mv8[q][0], T->mv8[q][1], p8, 8);
+                    residual(cy, w, bx + qx, by + qy, p8, 8, qp, lambda, r8, &c8, &d8, &b8);
+                }
+                P->d8 = (int32_t)d8; P->b8 = b8;
+            }
+        }
+}
+
+void orc_pframe_repair_par(const uint8_t *cy, int w, int h, const uint8_t *const RY[2], int ys,
+                           int navail, double lambda, const orc_metab *mt, const orc_pcost *pc,
+                           orc_mbinfo *given) {
+    const int mw = w / 16, mh = h / 16;
+    const int FL_INTRA = 2 * 3 + 4 * 2, FL_INTER16 = 2 * 3 + 2 + 4 * 2, FL_INTER8 = 3 + 4 * 2 + 4 * 2;
+    mvrec_t *mvm = calloc((size_t)mw * mh, sizeof *mvm);
+    if (!mvm) return;
+    for (int my = 0; my < mh; ++my) {
+        int pmx = 0, pmy = 0;
+        for (int mx = 0; mx < mw; ++mx) {
+            const int bx = mx * 16, by = my * 16;
+            const orc_metab *T = &mt[my * mw + mx];
+            const orc_pcost *P = &pc[my * mw + mx];
+            mvrec_t cand[3];
+            const int ncand = merge_list(mvm, mw, mx, my, pmx, pmy, cand);
+            int bmi = 0; int32_t Ds = 0;
+            for (int c = 0; c < ncand; ++c) {
+                uint8_t tp[256];
+                orc_mc_luma(RY[cand[c].ref], ys, bx, by, 16, 16, cand[c].x, cand[c].y, tp, 16);
+                int32_t e = 0;
+                for (int i = 0; i < 256; ++i) { int d = cy[(by + (i >> 4)) * w + bx + (i & 15)] - tp[i]; e += d * d; }
+                if (c == 0 || e < Ds) { Ds = e; bmi = c; }
+            }
+            const double Js = (double)Ds + lambda * (double)(3 + (ncand > 1 ? 3 : 0));
+            double Jbest = Js;
+            const int skip_wins = Js <= lambda * FL_INTRA || (T->flags & 1);
+            double Ji = 1e30; int iref = 0, sad16 = 0;
+            for (int r = 0; r < navail && !skip_wins; ++r) {
+                if (r > 0 && Jbest <= lambda * FL_INTER16) break;
+                const int bb = P->b16[r] + orc_bits_val(zmap(T->mv16[r][0] - pmx)) + orc_bits_val(zmap(T->mv16[r][1] - pmy));
+                const double J = (double)P->d16[r] + lambda * bb;
+                if (J < Jbest) Jbest = J;
+                if (J < Ji) { Ji = J; iref = r; sad16 = T->sad16[r]; }
+            }
+            double Jn = 1e30;
+            if (!skip_wins && Jbest > lambda * FL_INTRA) {
+                Jn = (double)P->dn + lambda * P->bn;
+                if (Jn < Jbest) Jbest = Jn;
+            }
+            double J8 = 1e30;
+            if (!skip_wins && Jbest > lambda * FL_INTER8) {
+                const int ssad = T->sad8[0] + T->sad8[1] + T->sad8[2] + T->sad8[3];
+                if (!(sad16 > 0 && ssad * 10 >= sad16 * 9)) {
+                    int b8 = P->b8, qpx = pmx, qpy = pmy;
+                    for (int q = 0; q < 4; ++q) {
+                        b8 += orc_bits_val(zmap(T->mv8[q][0] - qpx)) + orc_bits_val(zmap(T->mv8[q][1] - qpy));
+                        qpx = T->mv8[q][0]; qpy = T->mv8[q][1];
+                    }
+                    J8 = (double)P->d8 + lambda * b8;
+                }
+            }
+            int mode = 0; double Jm = Js;
+            if (Ji < Jm) { Jm = Ji; mode = 1; }
+            if (Jn < Jm) { Jm = Jn; mode = 2; }
+            if (J8 < Jm) { Jm = J8; mode = 3; }
+            orc_mbinfo *G = &given[my * mw + mx];
+            memset(G, 0, sizeof *G);
+            G->mode = (uint8_t)mode;
+            mvrec_t *M = &mvm[my * mw + mx];
+            if (mode == 0) {
+                G->merge_idx = (uint8_t)bmi; G->ref = cand[bmi].ref;
+                G->mv[0][0] = cand[bmi].x; G->mv[0][1] = cand[bmi].y;
+                *M = (mvrec_t){cand[bmi].x, cand[bmi].y, cand[bmi].ref, 1};
+            } else if (mode == 1) {
+                G->ref = (uint8_t)iref; G->mv[0][0] = T->mv16[iref][0]; G->mv[0][1] = T->mv16[iref][1];
+                *M = (mvrec_t){T->mv16[iref][0], T->mv16[iref][1], (uint8_t)iref, 1};
+            } else if (mode == 2) {
+                G->sub = P->nsub;
+                *M = (mvrec_t){0, 0, 0, 0};
+            } else {
+                for (int q = 0; q < 4; ++q) { G->mv[q][0] = T->mv8[q][0]; G->mv[q][1] = T->mv8[q][1]; }
+                *M = (mvrec_t){T->mv8[3][0], T->mv8[3][1], 0, 1};
+            }
+            if (mode != 2) { pmx = M->x; pmy = M->y; }
+        }
+    }
+    free(mvm);
+}

@@ -114,4 +114,36 @@ size_t orc_pframe_encode(const uint8_t *cy, const uint8_t *cu, const uint8_t *cv
                          orc_tabcache *tc, double lscale, int me_mode,
                          orc_mbinfo *mbinfo);
 
+/* ---- block-parallel decisions (this build's parallel / learned modes; not in
+ * the reference).  A per-MB motion table, as the GPU's table kernels write it. */
+typedef struct {
+    int16_t mv16[2][2];
+    int32_t sad16[2];
+    int16_t mv8[4][2];
+    int32_t sad8[4];
+    int32_t flags;           /* bit0: SKIP without evaluating the other modes */
+} orc_metab;
+
+/* Per-MB candidate costs: SSD and bits of each mode's residual coding, without
+ * the vector-difference bits (which depend on the predictor). */
+typedef struct {
+    int32_t d16[2], b16[2];  /* INTER16 on each reference (bits include the 6 mode/ref bits) */
+    int32_t dn, bn;          /* INTRA 16x16 from the source frame's neighbours (bits incl. 6) */
+    int32_t d8, b8;          /* INTER8 (bits include the 3 mode bits) */
+    uint8_t nsub, pad[3];
+} orc_pcost;
+
+/* Exhaustive-search table of a frame (16x16 per reference, 8x8 quadrants on 0). */
+void orc_metab_exhaustive(const uint8_t *cy, int w, int h, const uint8_t *const RY[2], int ys,
+                          int navail, orc_metab *mt);
+/* Every MB's candidate costs, independently of every other MB. */
+void orc_pframe_costs_par(const uint8_t *cy, int w, int h, const uint8_t *const RY[2], int ys,
+                          int navail, int qp, double lambda, const orc_metab *mt, orc_pcost *pc);
+/* The reference's decision logic in raster order on the stored costs, with the
+ * true vector predictor and merge list.  The result is a `given` for
+ * orc_pframe_analyse. */
+void orc_pframe_repair_par(const uint8_t *cy, int w, int h, const uint8_t *const RY[2], int ys,
+                           int navail, double lambda, const orc_metab *mt, const orc_pcost *pc,
+                           orc_mbinfo *given);
+
 #endif

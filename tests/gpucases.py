@@ -78,8 +78,12 @@ def run_parity(scenes, nframes, qps, me_mode=G.ME_DIAMOND, lscales=None, force_s
         orc = []
         for k in range(n):
             w, h = sizes[k]
-            st, rec, info = L.oracle_analyse(frames[k][f], near[k], far[k], w, h, navail[k], qps[k],
-                                             lscales[k], fs[k], me_mode if me_mode != G.ME_LEARNED else 1)
+            if me_mode in (G.ME_PARALLEL, G.ME_LEARNED) and not fs[k] and not given:
+                st, rec, info, _, _ = L.oracle_parallel(frames[k][f], near[k], far[k], w, h, navail[k], qps[k],
+                                                        lscales[k])
+            else:
+                st, rec, info = L.oracle_analyse(frames[k][f], near[k], far[k], w, h, navail[k], qps[k],
+                                                 lscales[k], fs[k], 0 if me_mode == G.ME_DIAMOND else 1)
             orc.append((st, rec, info))
             an.put_source(k, 0, frames[k][f])
             if given:

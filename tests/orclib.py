@@ -227,3 +227,69 @@ def oracle_pack(qp, streams, tc=None):
     n = lib.orc_pframe_pack(qp, C.addressof(sv), C.addressof(sn),
                             tc.buf.ctypes.data if tc is not None else None, out.ctypes.data, cap)
     return out[:n].tobytes()
+
+
+class Metab(C.Structure):
+    """orc_metab / fdvg_metab: per-MB motion table."""
+    _fields_ = [("mv16", (C.c_int16 * 2) * 2), ("sad16", C.c_int32 * 2), ("mv8", (C.c_int16 * 2) * 4),
+                ("sad8", C.c_int32 * 4), ("flags", C.c_int32)]
+
+
+def _padded_refs(r0, r1, w, h):
+    keep, ptrs = [], [[0, 0], [0, 0], [0, 0]]
+    refs = [r0, r1 if r1 is not None else r0]
+    ys = cs = 0
+    for r in range(2):
+        pl = split_i420(refs[r], w, h)
+        for k, (p, pw, ph) in enumerate(((pl[0], w, h), (pl[1], w // 2, h // 2), (pl[2], w // 2, h // 2))):
+            buf, off, st = padded(p, pw, ph)
+            keep.append(buf)
+            ptrs[k][r] = buf.ctypes.data + off
+            if k == 0:
+                ys = st
+            else:
+                cs = st
+    return keep, ptrs, ys, cs
+
+
+def oracle_metab_exhaustive(cur, r0, r1, w, h, navail):
+    lib = oracle()
+    keep, ptrs, ys, cs = _padded_refs(r0, r1, w, h)
+    RY = (C.c_void_p * 2)(*ptrs[0])
+    nmb = (w // 16) * (h // 16)
+    mt = (Metab * nmb)()
+    cy = split_i420(cur, w, h)[0]
+    lib.orc_metab_exhaustive.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_void_p]
+    lib.orc_metab_exhaustive(C.c_void_p(cy.ctypes.data), w, h, C.addressof(RY), ys, navail, C.addressof(mt))
+    return mt
+
+
+class PCost(C.Structure):
+    _fields_ = [("d16", C.c_int32 * 2), ("b16", C.c_int32 * 2), ("dn", C.c_int32), ("bn", C.c_int32),
+                ("d8", C.c_int32), ("b8", C.c_int32), ("nsub", C.c_uint8), ("pad", C.c_uint8 * 3)]
+
+
+def oracle_parallel(cur, r0, r1, w, h, navail, qp, lscale=1.0, metab=None):
+    """The block-parallel analysis: candidate costs from the motion table
+    (exhaustive when none is given), the reference's decision logic replayed on
+    them in raster order, then the decisions coded exactly.
+    Returns (streams, recon, info, decisions given to the coder, costs)."""
+    lib = oracle()
+    keep, ptrs, ys, cs = _padded_refs(r0, r1, w, h)
+    RY = (C.c_void_p * 2)(*ptrs[0])
+    nmb = (w // 16) * (h // 16)
+    mt = metab if metab is not None else oracle_metab_exhaustive(cur, r0, r1, w, h, navail)
+    pc = (PCost * nmb)()
+    given = (MbInfo * nmb)()
+    cy = split_i420(cur, w, h)[0]
+    lam = lib.orc_lambda(qp, lscale)
+    lib.orc_pframe_costs_par.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int,
+                                         C.c_int, C.c_double, C.c_void_p, C.c_void_p]
+    lib.orc_pframe_repair_par.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int,
+                                          C.c_double, C.c_void_p, C.c_void_p, C.c_void_p]
+    lib.orc_pframe_costs_par(C.c_void_p(cy.ctypes.data), w, h, C.addressof(RY), ys, navail, qp, lam,
+                             C.addressof(mt), C.addressof(pc))
+    lib.orc_pframe_repair_par(C.c_void_p(cy.ctypes.data), w, h, C.addressof(RY), ys, navail, lam,
+                              C.addressof(mt), C.addressof(pc), C.addressof(given))
+    st, rec, info = oracle_analyse(cur, r0, r1, w, h, navail, qp, lscale, 0, 0, given=given)
+    return st, rec, info, given, pc

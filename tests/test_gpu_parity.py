@@ -43,3 +43,18 @@ def test_exhaustive_vs_oracle():
     scenes = [content.scene_objects(96, 64, 13, nobj=3, smin=8, smax=32, vmax=12),
               content.scene_pan(64, 48, 14, nobj=2, smin=8, smax=24)]
     assert run_parity(scenes, 3, [26, 32], me_mode=G.ME_EXHAUSTIVE, label="exh") == 0
+
+
+def test_parallel_decisions_vs_oracle():
+    """Block-parallel decisions from exhaustive tables: candidate costs per MB,
+    the reference's decision logic replayed on them, then exact coding."""
+    scenes = [content.scene_objects(96, 64, 15, nobj=3, smin=8, smax=32, vmax=12),
+              content.scene_pan(128, 48, 16, nobj=2, smin=8, smax=24),
+              content.scene_blobs(64, 64, 17, nblob=20, vmax=10, grain=2.0)]
+    assert run_parity(scenes, 4, [24, 30, 36], lscales=[1.0, 0.8, 1.1], me_mode=G.ME_PARALLEL, label="par") == 0
+
+
+@pytest.mark.parametrize("qp", [0, 51])
+def test_parallel_decisions_qp_extremes(qp):
+    assert run_parity([content.scene_pan(80, 48, 30 + qp, grain=4.0, nobj=2, smin=8, smax=24)], 3, [qp],
+                      me_mode=G.ME_PARALLEL, label=f"par-qp{qp}") == 0
