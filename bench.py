@@ -153,6 +153,8 @@ def main():
             "parallel": G.ME_PARALLEL}[args.mode]
 
     an = G.Analyser([(W, bh) for _, bh in layout], nsrc=nframes)
+    if mode == G.ME_LEARNED:
+        an.set_proposer()
     per_band = [band_frames(f, W, H, layout) for f in frames]
     for b in range(len(layout)):
         an.put_ref(b, 0, per_band[0][b])

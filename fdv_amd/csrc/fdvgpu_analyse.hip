@@ -424,8 +424,8 @@ __device__ __forceinline__ void mb_decide(Smem &s, const fdvg_seq &q, MbCtx &c, 
     const double Js = (double)Ds + lambda * (double)(3 + (ncand > 1 ? 3 : 0));
     double Jbest = Js;
     bool skip_wins = Js <= lambda * (double)FL_INTRA;
-    if (mt && (mt->flags & 1)) skip_wins = true;
     if (gv) skip_wins = false;
+    const bool pruned = mt && (mt->flags & 1);       // the proposer expects SKIP: no INTRA, no INTER8
 
     // ---- the independent searches, one per wavefront
     const Nb nbL = nb_setup(&s.hbytes[1], s.lcol, s.hbytes[0], 16, c.ht, c.hl);
@@ -482,7 +482,7 @@ __device__ __forceinline__ void mb_decide(Smem &s, const fdvg_seq &q, MbCtx &c, 
 
     // ---- INTRA 16x16: SATD screen, one coding (fdv.h:5950-5988)
     double Jn = 1e30;
-    const bool want_n = gv ? gv->mode == 2 : (!skip_wins && Jbest > lambda * (double)FL_INTRA);
+    const bool want_n = gv ? gv->mode == 2 : (!skip_wins && !pruned && Jbest > lambda * (double)FL_INTRA);
     if (want_n) {
         int sel = 0;
         if (gv) sel = gv->sub;
@@ -505,7 +505,7 @@ __device__ __forceinline__ void mb_decide(Smem &s, const fdvg_seq &q, MbCtx &c, 
     // ---- INTER8: the quadrant vectors, the SAD split gate, four 8x8 residuals
     //      (fdv.h:5996-6033)
     double J8 = 1e30;
-    bool want_8 = gv ? gv->mode == 3 : (!skip_wins && Jbest > lambda * (double)FL_INTER8);
+    bool want_8 = gv ? gv->mode == 3 : (!skip_wins && !pruned && Jbest > lambda * (double)FL_INTER8);
     if (want_8) {
 #pragma unroll
         for (int qd = 0; qd < 4; ++qd) {
@@ -873,10 +873,6 @@ k_pcost(const fdvg_seq *__restrict__ seqs, fdvg_pcost *__restrict__ pcs) {
         P->sce[c] = c < ncs ? sred[c][0] + sred[c][1] + sred[c][2] + sred[c][3] : 0;
     }
     if (tid == 0) P->nsc = (uint8_t)ncs;
-    if (T->flags & 1) {                      // a forced SKIP needs no mode costs
-        if (tid == 0) { P->d16[0] = P->d16[1] = P->b16[0] = P->b16[1] = 0; P->dn = P->bn = P->d8 = P->b8 = 0; P->nsub = 0; }
-        return;
-    }
     const int qp = q.qp;
     const double lambda = q.lambda;
     // INTER16 on each reference
@@ -889,6 +885,10 @@ k_pcost(const fdvg_seq *__restrict__ seqs, fdvg_pcost *__restrict__ pcs) {
             P->b16[r] = 6 + s.ws.b[0] + s.ws.b[1] + s.ws.b[2] + s.ws.b[3];
         }
         __syncthreads();
+    }
+    if (T->flags & 1) {                      // the proposer expects SKIP: no INTRA, no INTER8
+        if (tid == 0) { P->dn = P->bn = P->d8 = P->b8 = 0; P->nsub = 0; }
+        return;
     }
     // INTRA 16x16 from the source neighbours: SATD screen, then one coding
     const Nb nbL = nb_setup(&s.hbytes[1], s.lcol, s.hbytes[0], 16, my > 0, mx > 0);
@@ -1146,7 +1146,8 @@ k_prepair(const fdvg_seq *__restrict__ seqs, int mwmax, unsigned *ctl,
             // the reference's gates and commit on the stored costs
             const double Js = (double)Ds + lambda * (double)(3 + (ncand > 1 ? 3 : 0));
             double Jbest = Js;
-            const bool skip_wins = Js <= lambda * (double)FL_INTRA || (tfl & 1);
+            const bool skip_wins = Js <= lambda * (double)FL_INTRA;
+            const bool pruned = tfl & 1;          // the proposer expects SKIP: no INTRA, no INTER8
             double Ji = 1e30;
             int iref = 0, sad16 = 0;
             if (!skip_wins) {
@@ -1162,12 +1163,12 @@ k_prepair(const fdvg_seq *__restrict__ seqs, int mwmax, unsigned *ctl,
                 }
             }
             double Jn = 1e30;
-            if (!skip_wins && Jbest > lambda * (double)FL_INTRA) {
+            if (!skip_wins && !pruned && Jbest > lambda * (double)FL_INTRA) {
                 Jn = (double)pdn + lambda * (double)pbn;
                 if (Jn < Jbest) Jbest = Jn;
             }
             double J8 = 1e30;
-            if (!skip_wins && Jbest > lambda * (double)FL_INTER8) {
+            if (!skip_wins && !pruned && Jbest > lambda * (double)FL_INTER8) {
                 if (!(sad16 > 0 && ssad * 10 >= sad16 * 9)) {
                     int b8 = pb8, qpx = pmx, qpy = pmy;
 #pragma unroll

@@ -25,7 +25,12 @@ extern "C" int fdvg_launch_pad(const void *, const uint8_t *const *, int, int, h
 extern "C" int fdvg_launch_zero(uint32_t *, int, hipStream_t);
 extern "C" int fdvg_launch_symbols(const void *, int, int, const fdvg_mbrec *, uint32_t *, uint32_t *,
                                    uint32_t *, uint8_t *, size_t, hipStream_t);
-extern "C" int fdvg_launch_metab(const fdvg_seq *, int, int, void *, hipStream_t);
+extern "C" int fdvg_launch_metab(const fdvg_seq *, int, int, void *, const void *, float, hipStream_t);
+extern "C" int fdvg_launch_nn_infer(const fdvg_seq *, int, int, const float *, fdvg_nnout *, hipStream_t);
+extern "C" int fdvg_nn_nparam(void);
+
+// The proposer's confidence threshold: below it an MB gets the exhaustive search.
+#define FDVG_NN_THRESHOLD 0.7f
 extern "C" int fdvg_launch_parallel(const fdvg_seq *, int, int, int, int, unsigned *, fdvg_pcost *,
                                     fdvg_mbinfo *, fdvg_pctx *, unsigned *, unsigned *, int,
                                     fdvg_mbrec *, hipStream_t);
@@ -77,6 +82,10 @@ struct fdvgpu {
     fdvg_mbinfo *pgiven_d;
     uint64_t *mrec_d;               // motion-record handoff of the decision wavefront
     unsigned *ilist_d, *done_d;     // intra MBs in dependency order (count first), per-MB done flags
+    // learned proposer
+    float *nnw_d;
+    fdvg_nnout *nn_d;
+    int have_nn;
     size_t compact_cap;
     uint32_t *flags_h;
     hipEvent_t ev[5];
@@ -114,6 +123,7 @@ extern "C" void fdvgpu_close(fdvgpu *g) {
     (void)hipFree(g->compact_d); (void)hipHostFree(g->compact_h); (void)hipHostFree(g->flags_h);
     (void)hipFree(g->pcost_d); (void)hipFree(g->pctx_d); (void)hipFree(g->pgiven_d); (void)hipFree(g->mrec_d);
     (void)hipFree(g->ilist_d); (void)hipFree(g->done_d);
+    (void)hipFree(g->nnw_d); (void)hipFree(g->nn_d);
     for (int i = 0; i < 5; ++i) if (g->ev[i]) (void)hipEventDestroy(g->ev[i]);
     for (int i = 0; i < 8; ++i) if (g->ring_ev[i]) (void)hipEventDestroy(g->ring_ev[i]);
     if (g->st) (void)hipStreamDestroy(g->st);
@@ -192,6 +202,8 @@ extern "C" fdvgpu *fdvgpu_open(int nseq, const int *w, const int *h, int nsrc) {
     CKN(hipMemset(g->mrec_d, 0, (size_t)g->total_mb * sizeof(uint64_t)));
     CKN(hipMalloc(&g->ilist_d, ((size_t)g->total_mb + 1) * sizeof(unsigned)));
     CKN(hipMalloc(&g->done_d, (size_t)g->total_mb * sizeof(unsigned)));
+    CKN(hipMalloc(&g->nnw_d, (size_t)fdvg_nn_nparam() * sizeof(float)));
+    CKN(hipMalloc(&g->nn_d, (size_t)g->total_mb * sizeof(fdvg_nnout)));
 
     // rows in (row, sequence) order: every sequence's row r before any row r+1
     rowtab_h = (int2 *)malloc(g->nrows * sizeof(int2));
@@ -280,6 +292,7 @@ extern "C" int fdvgpu_pframe(fdvgpu *g, const int *src_slot, const int *qp, cons
                              const int *navail, const int *force_skip, int me_mode) {
     if (!g || !src_slot || !qp || !lscale || !navail) { seterr("fdvgpu_pframe: bad arguments"); return -1; }
     if (me_mode < FDVGPU_ME_DIAMOND || me_mode > FDVGPU_ME_PARALLEL) { seterr("fdvgpu_pframe: bad me_mode"); return -1; }
+    if (me_mode == FDVGPU_ME_LEARNED && !g->have_nn) { seterr("fdvgpu_pframe: learned mode without proposer weights"); return -1; }
     // block-parallel decisions (learned / parallel modes) need neither held
     // frames nor given decisions; those frames take the wavefront, whose
     // table mode they share
@@ -330,8 +343,13 @@ extern "C" int fdvgpu_pframe(fdvgpu *g, const int *src_slot, const int *qp, cons
     CK(hipMemcpyAsync(g->seq_d, seq_h, g->nseq * sizeof(fdvg_seq), hipMemcpyHostToDevice, g->st));
     CK(hipMemcpyAsync(g->planes_d, planes_h, 3 * g->nseq * sizeof(PlaneRef), hipMemcpyHostToDevice, g->st));
     CK(hipEventRecord(g->ring_ev[slot], g->st));
+    if (me_mode == FDVGPU_ME_LEARNED &&
+        fdvg_launch_nn_infer(g->seq_d, g->nseq, g->maxmb, g->nnw_d, g->nn_d, g->st) != 0) {
+        seterr("proposer launch failed"); return -1;
+    }
     if (me_mode != FDVGPU_ME_DIAMOND &&
-        fdvg_launch_metab(g->seq_d, g->nseq, g->maxmb, g->metab_d, g->st) != 0) {
+        fdvg_launch_metab(g->seq_d, g->nseq, g->maxmb, g->metab_d, me_mode == FDVGPU_ME_LEARNED ? g->nn_d : NULL,
+                          FDVG_NN_THRESHOLD, g->st) != 0) {
         seterr("motion table launch failed"); return -1;
     }
     if (fdvg_launch_zero(g->ticket_d, 8, g->st) != 0) { seterr("ticket reset launch failed"); return -1; }
@@ -437,3 +455,40 @@ extern "C" int fdvgpu_timings(fdvgpu *g, float ms[4]) {
     return 0;
 }
 
+
+extern "C" int fdvgpu_set_proposer(fdvgpu *g, const float *w, int n) {
+    if (!g || !w || n != fdvg_nn_nparam()) { seterr("fdvgpu_set_proposer: expected %d weights", fdvg_nn_nparam()); return -1; }
+    CK(hipMemcpyAsync(g->nnw_d, w, (size_t)n * sizeof(float), hipMemcpyHostToDevice, g->st));
+    CK(hipStreamSynchronize(g->st));
+    g->have_nn = 1;
+    return 0;
+}
+
+extern "C" int fdvgpu_proposals(fdvgpu *g, int seq, fdvgpu_proposal *out) {
+    if (!seq_ok(g, seq) || !out) { seterr("fdvgpu_proposals: bad arguments"); return -1; }
+    static_assert(sizeof(fdvgpu_proposal) == sizeof(fdvg_nnout), "proposal layout");
+    CK(hipMemcpyAsync(out, g->nn_d + g->S[seq].mb0, (size_t)g->S[seq].nmb * sizeof(fdvg_nnout),
+                      hipMemcpyDeviceToHost, g->st));
+    CK(hipStreamSynchronize(g->st));
+    return 0;
+}
+
+extern "C" int fdvgpu_motion(fdvgpu *g, int seq, fdvgpu_motion_entry *out) {
+    if (!seq_ok(g, seq) || !out) { seterr("fdvgpu_motion: bad arguments"); return -1; }
+    static_assert(sizeof(fdvgpu_motion_entry) == sizeof(fdvg_metab), "motion table layout");
+    CK(hipMemcpyAsync(out, (const fdvg_metab *)g->metab_d + g->S[seq].mb0, (size_t)g->S[seq].nmb * sizeof(fdvg_metab),
+                      hipMemcpyDeviceToHost, g->st));
+    CK(hipStreamSynchronize(g->st));
+    return 0;
+}
+
+extern "C" int fdvgpu_planes(fdvgpu *g, int seq, int src_slot, const uint8_t **cur_y, int *cur_stride,
+                             const uint8_t **ref_y, int *ref_stride) {
+    if (!seq_ok(g, seq) || src_slot < 0 || src_slot >= g->nsrc || !cur_y || !cur_stride || !ref_y || !ref_stride) {
+        seterr("fdvgpu_planes: bad arguments"); return -1;
+    }
+    SeqMem &m = g->S[seq];
+    *cur_y = m.src + (size_t)src_slot * m.srcsz; *cur_stride = m.w;
+    *ref_y = fy(m, m.near_); *ref_stride = m.ys;
+    return 0;
+}

@@ -35,8 +35,15 @@ typedef struct {
     int32_t sad16[2];
     int16_t mv8[4][2];
     int32_t sad8[4];
-    int32_t flags;      // bit0: the proposer is confident the MB is SKIP
+    int32_t flags;      // bit0: the proposer expects SKIP -- INTRA and INTER8 are not evaluated
 } fdvg_metab;
+
+// The learned proposer's output for one MB: 8 whole-pel vectors on the nearest
+// reference (pels, best first), their total probability, the SKIP probability.
+typedef struct {
+    int8_t cx[8], cy[8];
+    float conf, pskip;
+} fdvg_nnout;
 
 // Block-parallel decisions (parallel and learned modes).  Per MB: the residual
 // SSD and bits of every candidate mode, without the vector-difference bits

@@ -62,8 +62,13 @@ __device__ void refine(MSmem &s, int &par, int b, int bs, int ox, int oy, const 
     if (threadIdx.x == 0) { s.res[b][0] = mx; s.res[b][1] = my; s.res[b][2] = best; }
 }
 
+// With a proposer output `nn` whose confidence is at least `thr`, the integer
+// search covers only the 3x3 whole-pel neighbourhoods of its 8 vectors (scaled
+// by 2 on the far reference), same tie order; below `thr` it is the full
+// search.  A SKIP probability of at least `thr` sets flags bit 0.
 extern "C" __global__ void __launch_bounds__(256)
-k_me_exhaustive(const fdvg_seq *__restrict__ seqs, fdvg_metab *__restrict__ out) {
+k_me_exhaustive(const fdvg_seq *__restrict__ seqs, fdvg_metab *__restrict__ out,
+                const fdvg_nnout *__restrict__ nn, float thr) {
     __shared__ MSmem s;
     const fdvg_seq q = seqs[blockIdx.y];
     const int mw = q.w >> 4, nmb = mw * (q.h >> 4);
@@ -86,12 +91,34 @@ k_me_exhaustive(const fdvg_seq *__restrict__ seqs, fdvg_metab *__restrict__ out)
     // (sad, |dx|+|dy|, raster index) so the minimum is the reference tie order.
     unsigned long long best[6];
     for (int b = 0; b < 6; ++b) best[b] = ~0ull;
-    for (int pidx = tid; pidx < 33 * 33; pidx += 256) {
-        const int dy = pidx / 33 - 16, dx = pidx % 33 - 16;
+    const fdvg_nnout *pr = nn ? &nn[q.mb0 + i] : nullptr;
+    const bool cand = pr && pr->conf >= thr;
+    // candidate mode: thread t < 72 takes point t%9 around vector t/9 on the near
+    // reference (and the quadrants), t = 72 the zero vector; 128 <= t <= 200 the
+    // same on the far one
+    const int npts = cand ? 1 : (33 * 33 + 255) / 256;
+    for (int it = 0; it < npts; ++it) {
+        int dx, dy, rlo = 0, rhi = nr;
+        if (cand) {
+            const int tt = tid & 127, c = tt / 9, o = tt % 9, far = tid >> 7;
+            if (tt > 72 || (far && nr < 2)) continue;
+            const int sc = far ? 2 : 1;
+            // point 72: the zero vector, so that every search has a point in range
+            dx = tt == 72 ? 0 : sc * pr->cx[c] + o % 3 - 1;
+            dy = tt == 72 ? 0 : sc * pr->cy[c] + o / 3 - 1;
+            if (dx < -16 || dx > 16 || dy < -16 || dy > 16) continue;
+            rlo = far; rhi = far + 1;
+        } else {
+            const int pidx = tid + 256 * it;
+            if (pidx >= 33 * 33) continue;
+            dy = pidx / 33 - 16; dx = pidx % 33 - 16;
+        }
+        const int pidx = (dy + 16) * 33 + dx + 16;
         const unsigned long long tie = ((unsigned long long)(abs(dx) + abs(dy)) << 16) | (unsigned)pidx;
 #pragma unroll
         for (int r = 0; r < 2; ++r) {
-            if (r >= nr) break;
+            if (r >= rhi) break;
+            if (r < rlo) continue;
             const uint8_t *w = s.win[r];
             uint32_t qs[4] = {0, 0, 0, 0};
 #pragma unroll
@@ -145,11 +172,13 @@ k_me_exhaustive(const fdvg_seq *__restrict__ seqs, fdvg_metab *__restrict__ out)
             m.mv8[qd][0] = (int16_t)s.res[2 + qd][0]; m.mv8[qd][1] = (int16_t)s.res[2 + qd][1];
             m.sad8[qd] = s.res[2 + qd][2];
         }
-        m.flags = 0;
+        m.flags = (pr && pr->pskip >= thr) ? 1 : 0;
     }
 }
 
-extern "C" int fdvg_launch_metab(const fdvg_seq *seqs_d, int nseq, int maxmb, void *metab, hipStream_t st) {
-    hipLaunchKernelGGL(k_me_exhaustive, dim3(maxmb, nseq), dim3(256), 0, st, seqs_d, (fdvg_metab *)metab);
+extern "C" int fdvg_launch_metab(const fdvg_seq *seqs_d, int nseq, int maxmb, void *metab, const void *nn,
+                                 float thr, hipStream_t st) {
+    hipLaunchKernelGGL(k_me_exhaustive, dim3(maxmb, nseq), dim3(256), 0, st, seqs_d, (fdvg_metab *)metab,
+                       (const fdvg_nnout *)nn, thr);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }

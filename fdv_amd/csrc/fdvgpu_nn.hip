@@ -1,0 +1,702 @@
+// fdvgpu_nn.hip -- the learned motion proposer: a small BF16 network on MFMA.
+//
+// Not part of the reference codec: it only chooses which whole-pel vectors the
+// block-parallel search evaluates, and flags MBs it is confident are SKIP.  All
+// codec arithmetic downstream stays integer and bit-exact.
+//
+// Input of one MB: its 16x16 luma block and the 64x64 co-located window of the
+// nearest reference, both downsampled 2x (8x8 and 32x32, rounded 2x2 means).  A
+// fixed correlation front end turns them into a 17x17 map over the half-res
+// displacements -8..8 (+-16 whole pels): channel 0 = SAD relative to the
+// quantiser step, SAD / (16 * 2^((qp - 4) / 6)) -- whether a residual survives
+// quantisation is what SKIP turns on -- and channel 1 = (SAD - min SAD) / 256.  Then three 3x3 convolutions with 32 channels and ReLU
+// (zero padding), and one linear layer applied at every position: output 0 is
+// the position's vector logit, output 1 averaged over the positions (plus its
+// bias) is the SKIP logit.
+//
+// Numerics: weights are rounded to bf16 for the forward pass; the inputs of the
+// three convolutions (the features, then the two 32-channel activations) are
+// bf16 -- the convolutions run on v_mfma_f32_32x32x8bf16_1k with fp32
+// accumulation -- and the linear layer runs on fp32 VALU with the bf16-rounded
+// weights; biases are fp32.  The backward pass feeds MFMA with bf16 gradients
+// and accumulates fp32.  Master weights and Adam state are fp32.
+//
+// Weight layout (floats): w1[32][18] (k = tap*2 + ci), b1[32], w2[32][288]
+// (k = tap*32 + ci, tap = ky*3 + kx), b2[32], w3[32][288], b3[32], wf[2][32], bf[2].
+#include <hip/hip_runtime.h>
+#include "fdvgpu_common.h"
+#include "fdvgpu_dev.h"
+
+#define NN_G 17
+#define NN_P 289
+#define NN_PT 320                  // positions padded to ten 32-row tiles
+#define NN_C 32
+#define NN_K 288
+#define NN_K1 18
+#define OFF_W1 0
+#define OFF_B1 (OFF_W1 + 32 * 18)
+#define OFF_W2 (OFF_B1 + 32)
+#define OFF_B2 (OFF_W2 + 32 * 288)
+#define OFF_W3 (OFF_B2 + 32)
+#define OFF_B3 (OFF_W3 + 32 * 288)
+#define OFF_WF (OFF_B3 + 32)
+#define OFF_BF (OFF_WF + 64)
+#define NN_NPARAM (OFF_BF + 2)
+
+typedef short bf4 __attribute__((ext_vector_type(4)));
+typedef float f16v __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ uint16_t f2bf(float f) {          // round to nearest even
+    const uint32_t u = __float_as_uint(f);
+    return (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+__device__ __forceinline__ float rbf(float f) { return bf2f(f2bf(f)); }
+
+// Position p = y*17 + x shifted by tap (ky-1, kx-1); -1 if outside the map.
+__device__ __forceinline__ int nn_shift(int p, int tap) {
+    if (p >= NN_P) return -1;
+    const int y = p / NN_G + tap / 3 - 1, x = p % NN_G + tap % 3 - 1;
+    return (y < 0 || y >= NN_G || x < 0 || x >= NN_G) ? -1 : y * NN_G + x;
+}
+// ... and by the opposite offset (the transposed convolution)
+__device__ __forceinline__ int nn_unshift(int p, int tap) {
+    if (p >= NN_P) return -1;
+    const int y = p / NN_G - (tap / 3 - 1), x = p % NN_G - (tap % 3 - 1);
+    return (y < 0 || y >= NN_G || x < 0 || x >= NN_G) ? -1 : y * NN_G + x;
+}
+
+struct __align__(16) NnLds {
+    uint16_t w2[NN_C * NN_K], w3[NN_C * NN_K];   // bf16, [co][k]
+    uint16_t w1[32 * 24];                        // bf16, [co][k], k = tap*2 + ci padded to 24
+    float b1[32], b2[32], b3[32], wf[64], bf[2];
+    float feat[NN_P * 2];                        // fp32 features, then ...
+    uint16_t fb[NN_P * 2];                       // ... their bf16 copy, [pos][ci]
+    uint16_t a1[NN_PT * NN_C], a2[NN_PT * NN_C]; // bf16 [pos][c], post-ReLU
+    float a3[NN_PT * NN_C];                      // fp32 [pos][c], post-ReLU (reused by the backward pass)
+    uint16_t dz[NN_PT * NN_C];                   // bf16 gradient buffer
+    float lg[NN_PT], sk[NN_PT];
+    float red[8][8];
+    int ired[8][8];
+};
+
+// ---- inputs: the correlation map of one MB (luma of cur at (bx, by), ref plane)
+__device__ void nn_features(const uint8_t *cur, int cw, const uint8_t *ref, int rs, int bx, int by, int qp,
+                            float *feat, float *redf, int *redi) {
+    __shared__ uint8_t c8[64], r32[32 * 32];
+    __shared__ int cost[NN_P];
+    const int t = threadIdx.x;
+    if (t < 64) {
+        const int y = t >> 3, x = t & 7;
+        const uint8_t *p = cur + (size_t)(by + 2 * y) * cw + bx + 2 * x;
+        c8[t] = (uint8_t)((p[0] + p[1] + p[cw] + p[cw + 1] + 2) >> 2);
+    }
+    for (int k = t; k < 1024; k += blockDim.x) {
+        const int y = k >> 5, x = k & 31;
+        const uint8_t *p = ref + (ptrdiff_t)(by - 24 + 2 * y) * rs + bx - 24 + 2 * x;
+        r32[k] = (uint8_t)((p[0] + p[1] + p[rs] + p[rs + 1] + 2) >> 2);
+    }
+    __syncthreads();
+    int mn = 1 << 30;
+    for (int p = t; p < NN_P; p += blockDim.x) {
+        const int dy = p / NN_G - 8, dx = p % NN_G - 8;
+        int s = 0;
+        for (int y = 0; y < 8; ++y)
+            for (int x = 0; x < 8; ++x) s += abs((int)c8[y * 8 + x] - (int)r32[(12 + dy + y) * 32 + 12 + dx + x]);
+        cost[p] = s;
+        mn = min(mn, s);
+    }
+    for (int o = 32; o > 0; o >>= 1) mn = min(mn, __shfl_xor(mn, o));
+    if ((t & 63) == 0) redi[t >> 6] = mn;
+    __syncthreads();
+    mn = redi[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) mn = min(mn, redi[w]);
+    const float s0 = 1.0f / (16.0f * exp2f((float)(qp - 4) * (1.0f / 6.0f)));
+    for (int p = t; p < NN_P; p += blockDim.x) {
+        feat[2 * p] = (float)cost[p] * s0;
+        feat[2 * p + 1] = (float)(cost[p] - mn) * (1.0f / 256.0f);
+    }
+    (void)redf;
+    __syncthreads();
+}
+
+// ---- parameters into LDS (bf16-rounded weights, fp32 biases)
+__device__ void nn_load(NnLds &S, const float *W) {
+    const int t = threadIdx.x;
+    for (int k = t; k < NN_C * NN_K; k += blockDim.x) { S.w2[k] = f2bf(W[OFF_W2 + k]); S.w3[k] = f2bf(W[OFF_W3 + k]); }
+    for (int k = t; k < 32 * 24; k += blockDim.x) S.w1[k] = (k % 24) < 18 ? f2bf(W[OFF_W1 + (k / 24) * 18 + k % 24]) : 0;
+    if (t < 32) { S.b1[t] = W[OFF_B1 + t]; S.b2[t] = W[OFF_B2 + t]; S.b3[t] = W[OFF_B3 + t]; }
+    if (t < 64) S.wf[t] = rbf(W[OFF_WF + t]);
+    if (t < 2) S.bf[t] = W[OFF_BF + t];
+    __syncthreads();
+}
+
+// One 32x32 output tile of a 32-channel 3x3 convolution: rows = positions
+// 32*tile .. +31, columns = output channels; K = 288 (tap*32 + ci).
+__device__ __forceinline__ f16v conv_tile(const uint16_t *in, const uint16_t *w, int tile) {
+    const int l = threadIdx.x & 63, i = l & 31, g = l >> 5;
+    const int p = tile * 32 + i;
+    f16v c = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    for (int tap = 0; tap < 9; ++tap) {
+        const int q = nn_shift(p, tap);
+#pragma unroll
+        for (int c0 = 0; c0 < 32; c0 += 8) {
+            const int ci = c0 + 4 * g;
+            bf4 a = {0, 0, 0, 0}, b;
+            if (q >= 0) a = *(const bf4 *)&in[q * NN_C + ci];
+            b = *(const bf4 *)&w[i * NN_K + tap * 32 + ci];          // B[k][co = i] = w[co][k]
+            c = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, c, 0, 0, 0);
+        }
+    }
+    return c;
+}
+
+// conv1 tile (2 -> 32 channels): K = 18 padded to 24, k = tap*2 + ci; the
+// four k of a lane are two taps x two channels.
+__device__ __forceinline__ f16v conv1_tile(const uint16_t *fb, const uint16_t *w1, int tile) {
+    const int l = threadIdx.x & 63, i = l & 31, g = l >> 5;
+    const int p = tile * 32 + i;
+    f16v c = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k0 = 0; k0 < 24; k0 += 8) {
+        const int ta = (k0 + 4 * g) >> 1;
+        const int qa = ta < 9 ? nn_shift(p, ta) : -1, qb = ta + 1 < 9 ? nn_shift(p, ta + 1) : -1;
+        bf4 a = {0, 0, 0, 0};
+        if (qa >= 0) { a[0] = (short)fb[2 * qa]; a[1] = (short)fb[2 * qa + 1]; }
+        if (qb >= 0) { a[2] = (short)fb[2 * qb]; a[3] = (short)fb[2 * qb + 1]; }
+        const bf4 b = *(const bf4 *)&w1[i * 24 + k0 + 4 * g];
+        c = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, c, 0, 0, 0);
+    }
+    return c;
+}
+
+// The forward pass of one MB from S.feat: S.a1, S.a2 (bf16), S.a3 (fp32),
+// S.lg (position logits), S.sk (per-position SKIP terms); returns the SKIP logit
+// (uniform).
+__device__ float nn_forward(NnLds &S) {
+    const int t = threadIdx.x, wv = t >> 6, l = t & 63, nw = blockDim.x >> 6;
+    for (int k = t; k < NN_P * 2; k += blockDim.x) S.fb[k] = f2bf(S.feat[k]);
+    __syncthreads();
+    // conv1 on MFMA: 2 -> 32 channels
+    for (int tile = wv; tile < NN_PT / 32; tile += nw) {
+        const f16v c = conv1_tile(S.fb, S.w1, tile);
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int i = (r >> 2) * 8 + (l >> 5) * 4 + (r & 3), co = l & 31, p = tile * 32 + i;
+            float v = p < NN_P ? c[r] + S.b1[co] : 0.f;
+            S.a1[p * NN_C + co] = f2bf(v > 0.f ? v : 0.f);
+        }
+    }
+    __syncthreads();
+    // conv2, conv3 on MFMA
+    for (int layer = 0; layer < 2; ++layer) {
+        const uint16_t *in = layer ? S.a2 : S.a1;
+        const uint16_t *w = layer ? S.w3 : S.w2;
+        const float *b = layer ? S.b3 : S.b2;
+        for (int tile = wv; tile < NN_PT / 32; tile += nw) {
+            const f16v c = conv_tile(in, w, tile);
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int i = (r >> 2) * 8 + (l >> 5) * 4 + (r & 3), co = l & 31, p = tile * 32 + i;
+                float v = p < NN_P ? c[r] + b[co] : 0.f;
+                v = v > 0.f ? v : 0.f;
+                if (layer) S.a3[p * NN_C + co] = v;
+                else S.a2[p * NN_C + co] = f2bf(v);
+            }
+        }
+        __syncthreads();
+    }
+    // the linear layer at every position
+    for (int p = t; p < NN_PT; p += blockDim.x) {
+        float o0 = 0.f, o1 = 0.f;
+        if (p < NN_P) {
+            o0 = S.bf[0];
+            for (int c = 0; c < NN_C; ++c) { o0 += S.a3[p * NN_C + c] * S.wf[c]; o1 += S.a3[p * NN_C + c] * S.wf[32 + c]; }
+        }
+        S.lg[p] = o0; S.sk[p] = o1;
+    }
+    __syncthreads();
+    float s = 0.f;
+    for (int p = t; p < NN_P; p += blockDim.x) s += S.sk[p];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (l == 0) S.red[0][wv] = s;
+    __syncthreads();
+    float tot = 0.f;
+    for (int w2 = 0; w2 < nw; ++w2) tot += S.red[0][w2];
+    __syncthreads();
+    return tot * (1.0f / NN_P) + S.bf[1];
+}
+
+// Softmax statistics of S.lg over the positions: max and sum of exp (uniform).
+__device__ void nn_softmax(NnLds &S, float *mx_out, float *sum_out) {
+    const int t = threadIdx.x, wv = t >> 6, l = t & 63, nw = blockDim.x >> 6;
+    float m = -3.0e38f;
+    for (int p = t; p < NN_P; p += blockDim.x) m = fmaxf(m, S.lg[p]);
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+    if (l == 0) S.red[1][wv] = m;
+    __syncthreads();
+    m = S.red[1][0];
+    for (int w = 1; w < nw; ++w) m = fmaxf(m, S.red[1][w]);
+    float s = 0.f;
+    for (int p = t; p < NN_P; p += blockDim.x) s += expf(S.lg[p] - m);
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (l == 0) S.red[2][wv] = s;
+    __syncthreads();
+    s = 0.f;
+    for (int w = 0; w < nw; ++w) s += S.red[2][w];
+    __syncthreads();
+    *mx_out = m; *sum_out = s;
+}
+
+// ---------------------------------------------------------------- training
+
+// Features of a batch of tuples: tuple j is the MB (bx, by) of a frame whose
+// luma and padded reference luma are given per tuple.
+extern "C" __global__ void __launch_bounds__(256)
+k_nn_feat(const uint8_t *const *__restrict__ curs, const int *__restrict__ cw, const uint8_t *const *__restrict__ refs,
+          const int *__restrict__ rs, const int2 *__restrict__ pos, const int *__restrict__ qps, float *__restrict__ feats) {
+    __shared__ float redf[8];
+    __shared__ int redi[8];
+    __shared__ float fe[NN_P * 2];
+    const int j = blockIdx.x;
+    nn_features(curs[j], cw[j], refs[j], rs[j], pos[j].x, pos[j].y, qps[j], fe, redf, redi);
+    for (int k = threadIdx.x; k < NN_P * 2; k += blockDim.x) feats[(size_t)j * NN_P * 2 + k] = fe[k];
+}
+
+// One training step's forward + backward over a batch: tuple j has features
+// feats[j], vector target tgt[j] (position index, or -1 for none) and SKIP label
+// skip[j].  The SKIP cross-entropy weighs false positives by wfp:
+// -y log p - wfp (1 - y) log(1 - p).  Gradients (sums over the batch)
+// accumulate into grad[]; into loss[]: [0] vector cross-entropy, [1] SKIP
+// cross-entropy, [2] top-1 hits, [3] top-8 hits, [4] tuples with p >= thr,
+// [5] of which labelled SKIP, [6] tuples whose top-8 probability >= thr,
+// [7] of which with the target in the top 8.
+extern "C" __global__ void __launch_bounds__(256)
+k_nn_train(const float *__restrict__ W, const float *__restrict__ feats, const int *__restrict__ tgt,
+           const uint8_t *__restrict__ skip, int n, float wfp, float thr, float *__restrict__ grad,
+           float *__restrict__ loss) {
+    extern __shared__ uint8_t dyn_nn[];
+    NnLds &S = *(NnLds *)dyn_nn;
+    const int t = threadIdx.x, wv = t >> 6, l = t & 63, nw = blockDim.x >> 6;
+    nn_load(S, W);
+    for (int j = blockIdx.x; j < n; j += gridDim.x) {
+        for (int k = t; k < NN_P * 2; k += blockDim.x) S.feat[k] = feats[(size_t)j * NN_P * 2 + k];
+        __syncthreads();
+        const float skl = nn_forward(S);
+        float m, se;
+        nn_softmax(S, &m, &se);
+        const int tg = tgt[j];
+        const float y = skip[j] ? 1.f : 0.f;
+        const float ps = 1.f / (1.f + expf(-skl));
+        // losses and the rank of the target
+        if (t == 0) {
+            float top[8];
+            for (int k = 0; k < 8; ++k) top[k] = -3.0e38f;
+            for (int p = 0; p < NN_P; ++p) {
+                float v = S.lg[p];
+                for (int k = 0; k < 8; ++k) if (v > top[k]) { const float u = top[k]; top[k] = v; v = u; }
+            }
+            float conf = 0.f;
+            for (int k = 0; k < 8; ++k) conf += expf(top[k] - m) / se;
+            if (tg >= 0) {
+                atomicAdd(&loss[0], -(S.lg[tg] - m - logf(se)));
+                int above = 0;
+                for (int p = 0; p < NN_P; ++p) above += S.lg[p] > S.lg[tg];
+                atomicAdd(&loss[2], above == 0 ? 1.f : 0.f);
+                atomicAdd(&loss[3], above < 8 ? 1.f : 0.f);
+                if (conf >= thr) { atomicAdd(&loss[6], 1.f); atomicAdd(&loss[7], above < 8 ? 1.f : 0.f); }
+            }
+            atomicAdd(&loss[1], -(y * logf(fmaxf(ps, 1e-12f)) + wfp * (1.f - y) * logf(fmaxf(1.f - ps, 1e-12f))));
+            if (ps >= thr) { atomicAdd(&loss[4], 1.f); atomicAdd(&loss[5], y); }
+        }
+        __syncthreads();                         // the logits are overwritten below
+        // d logits, d skip
+        const float dsk = (y > 0.5f ? ps - 1.f : wfp * ps) * (1.0f / NN_P);
+        for (int p = t; p < NN_PT; p += blockDim.x) {
+            float d = 0.f;
+            if (p < NN_P && tg >= 0) d = expf(S.lg[p] - m) / se - (p == tg ? 1.f : 0.f);
+            S.lg[p] = d;
+        }
+        __syncthreads();
+        // linear layer: dwf, dbf, and dZ3 = dA3 * [a3 > 0] into S.dz (bf16)
+        if (t < 64) {
+            const int o = t >> 5, c = t & 31;
+            float g = 0.f;
+            for (int p = 0; p < NN_P; ++p) g += (o ? dsk : S.lg[p]) * S.a3[p * NN_C + c];
+            atomicAdd(&grad[OFF_WF + t], g);
+        } else if (t == 64) {
+            float g = 0.f;
+            for (int p = 0; p < NN_P; ++p) g += S.lg[p];
+            atomicAdd(&grad[OFF_BF], g);
+            atomicAdd(&grad[OFF_BF + 1], dsk * NN_P);
+        }
+        for (int e = t; e < NN_PT * NN_C; e += blockDim.x) {
+            const int p = e >> 5, c = e & 31;
+            float d = 0.f;
+            if (p < NN_P && S.a3[e] > 0.f) d = S.lg[p] * S.wf[c] + dsk * S.wf[32 + c];
+            S.dz[e] = f2bf(d);
+        }
+        __syncthreads();
+        // conv3 then conv2: dW, db, and the input gradient (masked by ReLU)
+        for (int layer = 1; layer >= 0; --layer) {
+            const uint16_t *in = layer ? S.a2 : S.a1;      // this layer's input
+            const uint16_t *w = layer ? S.w3 : S.w2;
+            const int ow = layer ? OFF_W3 : OFF_W2, ob = layer ? OFF_B3 : OFF_B2;
+            // dW[co][tap*32 + ci] = sum_p dz[p][co] * in[shift(p, tap)][ci]: M = co, N = ci, K = p per tap
+            for (int tap = wv; tap < 9; tap += nw) {
+                f16v c = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+                const int i = l & 31, g = l >> 5;
+                for (int p0 = 0; p0 < NN_PT; p0 += 8) {
+                    bf4 a, b;
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int p = p0 + 4 * g + u;
+                        a[u] = (short)S.dz[p * NN_C + i];                       // A[co = i][p]
+                        const int q = nn_shift(p, tap);
+                        b[u] = q >= 0 ? (short)in[q * NN_C + i] : (short)0;      // B[p][ci = i]
+                    }
+                    c = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, c, 0, 0, 0);
+                }
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    const int co = (r >> 2) * 8 + (l >> 5) * 4 + (r & 3), ci = l & 31;
+                    atomicAdd(&grad[ow + co * NN_K + tap * 32 + ci], c[r]);
+                }
+            }
+            if (t < 32) {
+                float g = 0.f;
+                for (int p = 0; p < NN_P; ++p) g += bf2f(S.dz[p * NN_C + t]);
+                atomicAdd(&grad[ob + t], g);
+            }
+            // dIn[p][ci] = sum_tap,co dz[unshift(p, tap)][co] * w[co][tap*32 + ci]: M = p, N = ci, K = (tap, co)
+            for (int tile = wv; tile < NN_PT / 32; tile += nw) {
+                f16v c = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+                const int i = l & 31, g = l >> 5, p = tile * 32 + i;
+                for (int tap = 0; tap < 9; ++tap) {
+                    const int q = nn_unshift(p, tap);
+#pragma unroll
+                    for (int c0 = 0; c0 < 32; c0 += 8) {
+                        const int co = c0 + 4 * g;
+                        bf4 a = {0, 0, 0, 0}, b;
+                        if (q >= 0) a = *(const bf4 *)&S.dz[q * NN_C + co];
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) b[u] = (short)w[(co + u) * NN_K + tap * 32 + i];
+                        c = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, c, 0, 0, 0);
+                    }
+                }
+                // masked by this layer's input activation; stage in a3 (fp32) until dz is free
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    const int pi = (r >> 2) * 8 + (l >> 5) * 4 + (r & 3), ci = l & 31, pp = tile * 32 + pi;
+                    const float act = bf2f(in[pp * NN_C + ci]);
+                    S.a3[pp * NN_C + ci] = (pp < NN_P && act > 0.f) ? c[r] : 0.f;
+                }
+            }
+            __syncthreads();
+            for (int e = t; e < NN_PT * NN_C; e += blockDim.x) S.dz[e] = f2bf(S.a3[e]);
+            __syncthreads();
+        }
+        // conv1: dW1[co][tap*2 + ci] = sum_p dz[p][co] * feat[shift(p, tap)][ci], db1
+        for (int e = t; e < 32 * 18 + 32; e += blockDim.x) {
+            float g = 0.f;
+            if (e < 32 * 18) {
+                const int co = e / 18, k = e % 18, tap = k >> 1, ci = k & 1;
+                for (int p = 0; p < NN_P; ++p) {
+                    const int q = nn_shift(p, tap);
+                    if (q >= 0) g += bf2f(S.dz[p * NN_C + co]) * bf2f(S.fb[2 * q + ci]);
+                }
+                atomicAdd(&grad[OFF_W1 + e], g);
+            } else {
+                const int co = e - 32 * 18;
+                for (int p = 0; p < NN_P; ++p) g += bf2f(S.dz[p * NN_C + co]);
+                atomicAdd(&grad[OFF_B1 + co], g);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Adam on the fp32 master weights with the batch-mean gradient, then clear it.
+extern "C" __global__ void k_nn_adam(float *__restrict__ W, float *__restrict__ grad, float *__restrict__ m1,
+                                     float *__restrict__ m2, int n, float lr, float b1, float b2, float eps,
+                                     float inv_batch, float c1, float c2) {
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const float g = grad[k] * inv_batch;
+        const float a = b1 * m1[k] + (1.f - b1) * g, b = b2 * m2[k] + (1.f - b2) * g * g;
+        m1[k] = a; m2[k] = b;
+        W[k] -= lr * (a / c1) / (sqrtf(b / c2) + eps);
+        grad[k] = 0.f;
+    }
+}
+
+// --------------------------------------------------------------- inference
+
+#ifdef FDVG_PHASE_PROF
+__device__ unsigned long long g_nnph[8];
+#define NPH(k) do { if (t == 0) { long long t_ = clock64(); atomicAdd(&g_nnph[k], (unsigned long long)(t_ - nph_last)); nph_last = t_; } } while (0)
+extern "C" int fdvg_nnphase_read(unsigned long long *out) {
+    return hipMemcpyFromSymbol(out, HIP_SYMBOL(g_nnph), sizeof g_nnph) == hipSuccess ? 0 : -1;
+}
+#else
+#define NPH(k) do { } while (0)
+#endif
+
+// The inference path keeps each wavefront's B operands (all three layers'
+// weights, bf16) in registers and the activations in LDS rows of NN_RS bf16
+// (72 bytes: consecutive positions fall in different banks), so that two
+// workgroups fit a CU.
+#define NN_RS 36
+struct __align__(16) NnInfer {
+    float feat[NN_P * 2];
+    uint16_t fb[NN_P * 2];
+    uint16_t a2[NN_PT * NN_RS];
+    union {
+        uint16_t a1[NN_PT * NN_RS];
+        float a3[NN_PT * NN_C];           // conv3 output (a1 is dead by then)
+    };
+    float b1[32], b2[32], b3[32], wf[64], bf[2];
+    float lg[NN_PT], sk[NN_PT];
+    float cv[32];
+    int cp[32];
+    float red[8][8];
+    int ired[8][8];
+};
+
+__device__ __forceinline__ f16v conv_tile_r(const uint16_t *in, const bf4 *wr, int tile) {
+    const int l = threadIdx.x & 63, i = l & 31, g = l >> 5;
+    const int p = tile * 32 + i;
+    f16v c = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int tap = 0; tap < 9; ++tap) {
+        const int q = nn_shift(p, tap);
+#pragma unroll
+        for (int c0 = 0; c0 < 4; ++c0) {
+            bf4 a = {0, 0, 0, 0};
+            if (q >= 0) a = *(const bf4 *)&in[q * NN_RS + c0 * 8 + 4 * g];
+            c = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, wr[tap * 4 + c0], c, 0, 0, 0);
+        }
+    }
+    return c;
+}
+
+// Per MB of every sequence: the 8 most likely whole-pel vectors (nearest
+// reference), their total probability and the SKIP probability.
+extern "C" __global__ void __launch_bounds__(256, 1)
+k_nn_infer(const fdvg_seq *__restrict__ seqs, const float *__restrict__ W, fdvg_nnout *__restrict__ out) {
+    extern __shared__ uint8_t dyn_nn[];
+    NnInfer &S = *(NnInfer *)dyn_nn;
+    const fdvg_seq q = seqs[blockIdx.y];
+    const int mw = q.w >> 4, nmb = mw * (q.h >> 4);
+    const int t = threadIdx.x, wv = t >> 6, l = t & 63, i32 = l & 31, g = l >> 5, nw = blockDim.x >> 6;
+#ifdef FDVG_PHASE_PROF
+    long long nph_last = clock64();
+#endif
+    // this lane's B operands: B[k][co = i32] for k = 8 s + 4 g + 0..3
+    bf4 w1r[3], w2r[36], w3r[36];
+#pragma unroll
+    for (int s = 0; s < 3; ++s)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int k = 8 * s + 4 * g + u;
+            w1r[s][u] = k < 18 ? (short)f2bf(W[OFF_W1 + i32 * 18 + k]) : (short)0;
+        }
+#pragma unroll
+    for (int s = 0; s < 36; ++s)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int k = 8 * s + 4 * g + u;
+            w2r[s][u] = (short)f2bf(W[OFF_W2 + i32 * NN_K + k]);
+            w3r[s][u] = (short)f2bf(W[OFF_W3 + i32 * NN_K + k]);
+        }
+    if (t < 32) { S.b1[t] = W[OFF_B1 + t]; S.b2[t] = W[OFF_B2 + t]; S.b3[t] = W[OFF_B3 + t]; }
+    if (t < 64) S.wf[t] = rbf(W[OFF_WF + t]);
+    if (t < 2) S.bf[t] = W[OFF_BF + t];
+    __syncthreads();
+    NPH(0);
+    for (int i = blockIdx.x; i < nmb; i += gridDim.x) {
+        nn_features(q.cy, q.w, q.ry[0], q.ys, (i % mw) * 16, (i / mw) * 16, q.qp, S.feat, &S.red[7][0], &S.ired[7][0]);
+        for (int k = t; k < NN_P * 2; k += blockDim.x) S.fb[k] = f2bf(S.feat[k]);
+        __syncthreads();
+        NPH(1);
+        // conv1 (K = 18 padded to 24: two taps x two channels per lane)
+        for (int tile = wv; tile < NN_PT / 32; tile += nw) {
+            const int p = tile * 32 + i32;
+            f16v c = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int s = 0; s < 3; ++s) {
+                const int ta = (8 * s + 4 * g) >> 1;
+                const int qa = ta < 9 ? nn_shift(p, ta) : -1, qb = ta + 1 < 9 ? nn_shift(p, ta + 1) : -1;
+                bf4 a = {0, 0, 0, 0};
+                if (qa >= 0) { a[0] = (short)S.fb[2 * qa]; a[1] = (short)S.fb[2 * qa + 1]; }
+                if (qb >= 0) { a[2] = (short)S.fb[2 * qb]; a[3] = (short)S.fb[2 * qb + 1]; }
+                c = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, w1r[s], c, 0, 0, 0);
+            }
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int ii = (r >> 2) * 8 + g * 4 + (r & 3), pp = tile * 32 + ii;
+                const float v = pp < NN_P ? c[r] + S.b1[i32] : 0.f;
+                S.a1[pp * NN_RS + i32] = f2bf(v > 0.f ? v : 0.f);
+            }
+        }
+        __syncthreads();
+        for (int tile = wv; tile < NN_PT / 32; tile += nw) {
+            const f16v c = conv_tile_r(S.a1, w2r, tile);
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int ii = (r >> 2) * 8 + g * 4 + (r & 3), pp = tile * 32 + ii;
+                const float v = pp < NN_P ? c[r] + S.b2[i32] : 0.f;
+                S.a2[pp * NN_RS + i32] = f2bf(v > 0.f ? v : 0.f);
+            }
+        }
+        __syncthreads();
+        // conv3 reads a2 only: a1 is dead since the barrier above, so a3 (its alias) is free
+        for (int tile = wv; tile < NN_PT / 32; tile += nw) {
+            const f16v c = conv_tile_r(S.a2, w3r, tile);
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int ii = (r >> 2) * 8 + g * 4 + (r & 3), pp = tile * 32 + ii;
+                const float v = pp < NN_P ? c[r] + S.b3[i32] : 0.f;
+                S.a3[pp * NN_C + ((i32 + pp) & 31)] = v > 0.f ? v : 0.f;   // channel rotated by position
+            }
+        }
+        __syncthreads();
+        // the linear layer at every position
+        for (int p = t; p < NN_PT; p += blockDim.x) {
+            float o0 = 0.f, o1 = 0.f;
+            if (p < NN_P) {
+                o0 = S.bf[0];
+                for (int c = 0; c < NN_C; ++c) {
+                    const float a = S.a3[p * NN_C + ((c + p) & 31)];
+                    o0 += a * S.wf[c];
+                    o1 += a * S.wf[32 + c];
+                }
+            }
+            S.lg[p] = o0; S.sk[p] = o1;
+        }
+        __syncthreads();
+        float sacc = 0.f;
+        for (int p = t; p < NN_P; p += blockDim.x) sacc += S.sk[p];
+        for (int o = 32; o > 0; o >>= 1) sacc += __shfl_xor(sacc, o);
+        if (l == 0) S.red[0][wv] = sacc;
+        __syncthreads();
+        float skl = 0.f;
+        for (int w2 = 0; w2 < nw; ++w2) skl += S.red[0][w2];
+        skl = skl * (1.0f / NN_P) + S.bf[1];
+        NPH(2);
+        // softmax statistics
+        float m = -3.0e38f;
+        for (int p = t; p < NN_P; p += blockDim.x) m = fmaxf(m, S.lg[p]);
+        for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+        if (l == 0) S.red[1][wv] = m;
+        __syncthreads();
+        m = S.red[1][0];
+        for (int w = 1; w < nw; ++w) m = fmaxf(m, S.red[1][w]);
+        float se = 0.f;
+        for (int p = t; p < NN_P; p += blockDim.x) se += expf(S.lg[p] - m);
+        for (int o = 32; o > 0; o >>= 1) se += __shfl_xor(se, o);
+        if (l == 0) S.red[2][wv] = se;
+        __syncthreads();
+        se = 0.f;
+        for (int w = 0; w < nw; ++w) se += S.red[2][w];
+        NPH(3);
+        // top 8 (ties to the lower position): 8 arg-max rounds inside each
+        // wavefront over its share of positions, then the same over the 4 x 8
+        {
+            const int per = (NN_P + nw - 1) / nw, p0 = wv * per;
+            float v0 = -3.0e38f, v1 = -3.0e38f;
+            int q0 = NN_P, q1 = NN_P;
+            if (l < per && p0 + l < NN_P) { v0 = S.lg[p0 + l]; q0 = p0 + l; }
+            if (l + 64 < per && p0 + l + 64 < NN_P) { v1 = S.lg[p0 + l + 64]; q1 = p0 + l + 64; }
+            for (int k = 0; k < 8; ++k) {
+                float bv = v0; int bp = q0;
+                if (v1 > bv || (v1 == bv && q1 < bp)) { bv = v1; bp = q1; }
+                for (int o = 32; o > 0; o >>= 1) {
+                    const float ov = __shfl_xor(bv, o);
+                    const int op = __shfl_xor(bp, o);
+                    if (ov > bv || (ov == bv && op < bp)) { bv = ov; bp = op; }
+                }
+                if (l == 0) { S.cv[wv * 8 + k] = bv; S.cp[wv * 8 + k] = bp; }
+                if (q0 == bp) v0 = -3.0e38f;
+                if (q1 == bp) v1 = -3.0e38f;
+            }
+        }
+        __syncthreads();
+        NPH(4);
+        if (wv == 0) {
+            float v = l < 8 * nw ? S.cv[l] : -3.0e38f;
+            int pp = l < 8 * nw ? S.cp[l] : NN_P;
+            float conf = 0.f;
+            fdvg_nnout &o = out[q.mb0 + i];
+            for (int k = 0; k < 8; ++k) {
+                float bv = v; int bp = pp;
+                for (int o2 = 32; o2 > 0; o2 >>= 1) {
+                    const float ov = __shfl_xor(bv, o2);
+                    const int op = __shfl_xor(bp, o2);
+                    if (ov > bv || (ov == bv && op < bp)) { bv = ov; bp = op; }
+                }
+                conf += expf(bv - m) / se;
+                if (l == 0) { o.cx[k] = (int8_t)(2 * (bp % NN_G - 8)); o.cy[k] = (int8_t)(2 * (bp / NN_G - 8)); }
+                if (pp == bp) v = -3.0e38f;
+            }
+            if (l == 0) { o.conf = conf; o.pskip = 1.f / (1.f + expf(-skl)); }
+        }
+        __syncthreads();
+        NPH(5);
+    }
+}
+
+extern "C" size_t fdvg_nn_lds(void) { return sizeof(NnLds); }
+extern "C" int fdvg_nn_nparam(void) { return NN_NPARAM; }
+extern "C" int fdvgpu_nn_nparam(void) { return NN_NPARAM; }
+
+extern "C" int fdvg_launch_nn_infer(const fdvg_seq *seqs, int nseq, int maxmb, const float *W, fdvg_nnout *out,
+                                    hipStream_t st) {
+    static bool attr = false;
+    if (!attr) {
+        if (hipFuncSetAttribute((const void *)k_nn_infer, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)sizeof(NnInfer)) != hipSuccess) return -1;
+        attr = true;
+    }
+    static int ncu = 0;
+    if (!ncu) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return -1;
+    }
+    // one workgroup per CU in all (the weights load once per workgroup)
+    int gx = ncu / nseq;
+    gx = gx < 1 ? 1 : gx > maxmb ? maxmb : gx;
+    hipLaunchKernelGGL(k_nn_infer, dim3(gx, nseq), dim3(256), sizeof(NnInfer), st, seqs, W, out);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// ------------------------------------------------------------- C ABI (training)
+
+extern "C" int fdvgpu_nn_features(const uint8_t *const *curs, const int *cw, const uint8_t *const *refs,
+                                  const int *rs, const int *pos, const int *qps, int n, float *feats, void *stream) {
+    hipLaunchKernelGGL(k_nn_feat, dim3(n), dim3(256), 0, (hipStream_t)stream, curs, cw, refs, rs,
+                       (const int2 *)pos, qps, feats);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+extern "C" int fdvgpu_nn_train_step(const float *W, const float *feats, const int *tgt, const uint8_t *skip, int n,
+                                    float wfp, float thr, float *grad, float *loss, int nblocks, void *stream) {
+    static bool attr = false;
+    if (!attr) {
+        if (hipFuncSetAttribute((const void *)k_nn_train, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)sizeof(NnLds)) != hipSuccess) return -1;
+        attr = true;
+    }
+    hipLaunchKernelGGL(k_nn_train, dim3(nblocks), dim3(256), sizeof(NnLds), (hipStream_t)stream, W, feats, tgt, skip, n,
+                       wfp, thr, grad, loss);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+extern "C" int fdvgpu_nn_adam(float *W, float *grad, float *m1, float *m2, int n, float lr, int step, int batch,
+                              void *stream) {
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float c1 = 1.f - powf(b1, (float)step), c2 = 1.f - powf(b2, (float)step);
+    hipLaunchKernelGGL(k_nn_adam, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, W, grad, m1, m2, n, lr,
+                       b1, b2, 1e-8f, 1.f / (float)batch, c1, c2);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}

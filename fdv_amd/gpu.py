@@ -16,6 +16,10 @@ NS = 16
 
 MBINFO_DTYPE = np.dtype([("mode", "u1"), ("ref", "u1"), ("merge_idx", "u1"), ("sub", "u1"),
                          ("mv", "<i2", (4, 2))])
+PROPOSAL_DTYPE = np.dtype([("cx", "i1", 8), ("cy", "i1", 8), ("conf", "<f4"), ("pskip", "<f4")])
+MOTION_DTYPE = np.dtype([("mv16", "<i2", (2, 2)), ("sad16", "<i4", 2), ("mv8", "<i2", (4, 2)),
+                         ("sad8", "<i4", 4), ("flags", "<i4")])
+PROPOSER_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "learned", "proposer.f32")
 
 _lib = None
 
@@ -40,6 +44,15 @@ def lib():
         L.fdvgpu_recon.argtypes = [vp, C.c_int, vp]
         L.fdvgpu_decisions.argtypes = [vp, C.c_int, vp]
         L.fdvgpu_timings.argtypes = [vp, C.POINTER(C.c_float)]
+        L.fdvgpu_set_proposer.argtypes = [vp, vp, C.c_int]
+        L.fdvgpu_proposals.argtypes = [vp, C.c_int, vp]
+        L.fdvgpu_motion.argtypes = [vp, C.c_int, vp]
+        L.fdvgpu_rewind.argtypes = [vp]
+        L.fdvgpu_planes.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_int),
+                                    C.POINTER(C.c_void_p), C.POINTER(C.c_int)]
+        L.fdvgpu_nn_features.argtypes = [vp, vp, vp, vp, vp, vp, C.c_int, vp, vp]
+        L.fdvgpu_nn_train_step.argtypes = [vp, vp, vp, vp, C.c_int, C.c_float, C.c_float, vp, vp, C.c_int, vp]
+        L.fdvgpu_nn_adam.argtypes = [vp, vp, vp, vp, C.c_int, C.c_float, C.c_int, C.c_int, vp]
         _lib = L
     return _lib
 
@@ -130,6 +143,32 @@ class Analyser:
         self._ck(self.L.fdvgpu_decisions(self.h, seq, out.ctypes.data), "decisions")
         return out
 
+    def set_proposer(self, weights=None):
+        """Install the learned proposer's weights (default: the trained ones
+        shipped with the package)."""
+        w = load_proposer() if weights is None else np.ascontiguousarray(weights, dtype=np.float32)
+        self._ck(self.L.fdvgpu_set_proposer(self.h, w.ctypes.data, int(w.size)), "set_proposer")
+
+    def proposals(self, seq):
+        w, h = self.sizes[seq]
+        out = np.zeros((w // 16) * (h // 16), dtype=PROPOSAL_DTYPE)
+        self._ck(self.L.fdvgpu_proposals(self.h, seq, out.ctypes.data), "proposals")
+        return out
+
+    def motion(self, seq):
+        w, h = self.sizes[seq]
+        out = np.zeros((w // 16) * (h // 16), dtype=MOTION_DTYPE)
+        self._ck(self.L.fdvgpu_motion(self.h, seq, out.ctypes.data), "motion")
+        return out
+
+    def planes(self, seq, slot):
+        """Device pointers: (source luma, its stride, nearest reference luma, its stride)."""
+        cy, ry = C.c_void_p(), C.c_void_p()
+        cs, rs = C.c_int(), C.c_int()
+        self._ck(self.L.fdvgpu_planes(self.h, seq, slot, C.byref(cy), C.byref(cs), C.byref(ry), C.byref(rs)),
+                 "planes")
+        return cy.value, cs.value, ry.value, rs.value
+
     def timings(self):
         ms = (C.c_float * 4)()
         self._ck(self.L.fdvgpu_timings(self.h, ms), "timings")
@@ -204,6 +243,12 @@ class Encoder:
         if self.L.fdvamd_set_ref(self.h, seq, which, f.ctypes.data) != 0:
             raise GpuError("fdvamd_set_ref: " + self.L.fdvamd_last_error().decode())
 
+    def set_proposer(self, weights=None):
+        w = load_proposer() if weights is None else np.ascontiguousarray(weights, dtype=np.float32)
+        self.L.fdvamd_set_proposer.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
+        if self.L.fdvamd_set_proposer(self.h, w.ctypes.data, int(w.size)) != 0:
+            raise GpuError("fdvamd_set_proposer: " + self.L.fdvamd_last_error().decode())
+
     def set_ref_planes(self, seq, which, frame):
         """Install a reference from three separate tight planes (the form the
         reference's encoders hold their reconstruction in)."""
@@ -263,3 +308,16 @@ class Encoder:
         if rc != 0:
             raise GpuError("fdvamd_pframe_batch: " + self.L.fdvamd_last_error().decode())
         return [outs[s][:lens[s]].tobytes() for s in range(n)], recs
+
+
+def nn_nparam() -> int:
+    return int(lib().fdvgpu_nn_nparam())
+
+
+def load_proposer(path: str = PROPOSER_WEIGHTS) -> np.ndarray:
+    if not os.path.exists(path):
+        raise GpuError("the learned proposer's weights are missing: " + path)
+    w = np.fromfile(path, dtype="<f4")
+    if w.size != nn_nparam():
+        raise GpuError("proposer weights: expected %d floats, found %d" % (nn_nparam(), w.size))
+    return w

@@ -168,3 +168,10 @@ int fdvamd_rewind(fdvamd_encoder *e) {
     if (!e || fdvgpu_rewind(e->g) != 0) { amd_seterr("%s", e ? fdvgpu_last_error() : "bad arguments"); return -1; }
     return 0;
 }
+
+int fdvamd_set_proposer(fdvamd_encoder *e, const float *weights, int n) {
+    if (!e || fdvgpu_set_proposer(e->g, weights, n) != 0) {
+        amd_seterr("%s", e ? fdvgpu_last_error() : "bad arguments"); return -1;
+    }
+    return 0;
+}

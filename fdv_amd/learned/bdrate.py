@@ -1,0 +1,138 @@
+"""Rate-distortion comparison of two search modes (BD-rate), as the reference's
+tools/rd-bench.py:75-113 defines it:
+  - drop dominated operating points (keep a point only if it has more PSNR than
+    every cheaper one);
+  - fit log10(rate) as a polynomial of degree min(3, n - 1) in PSNR;
+  - integrate both fits over the overlapping PSNR range;
+  - BD-rate = 10^(mean difference) - 1, in percent (negative: the new mode
+    needs fewer bits for equal quality).
+
+evaluate() codes the same seeded sequences with two modes of the drop-in
+encoder at several QPs (payload bytes of the P-frames; PSNR-Y of their
+reconstruction against the source) and returns the BD-rate of the second
+against the first.
+
+    python -m fdv_amd.learned.bdrate [--frames 60] [--qps 20,24,28,32]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+
+def frontier(pts):
+    out, best = [], float("-inf")
+    for r, d in sorted(pts):
+        if d > best:
+            out.append((r, d))
+            best = d
+    return out
+
+
+def bd_rate(ref, new):
+    ref, new = frontier(ref), frontier(new)
+    if len(ref) < 3 or len(new) < 3:
+        return None
+
+    def fit(pts):
+        d = np.array([p[1] for p in pts])
+        r = np.log10([p[0] for p in pts])
+        return np.polyfit(d, r, min(3, len(pts) - 1)), d.min(), d.max()
+
+    pr, lr, hr = fit(ref)
+    pn, ln, hn = fit(new)
+    lo, hi = max(lr, ln), min(hr, hn)
+    if hi <= lo:
+        return None
+    ir = np.polyval(np.polyint(pr), [lo, hi])
+    inw = np.polyval(np.polyint(pn), [lo, hi])
+    return float((10 ** (((inw[1] - inw[0]) - (ir[1] - ir[0])) / (hi - lo)) - 1) * 100)
+
+
+def psnr_y(rec, src, n):
+    e = np.mean((rec[:n].astype(np.float64) - src[:n].astype(np.float64)) ** 2)
+    return 99.0 if e == 0 else 10 * np.log10(255.0 ** 2 / e)
+
+
+def bands_of(h, band_mbrows=6):
+    bh = band_mbrows * 16
+    return [(y, min(bh, h - y)) for y in range(0, h, bh)]
+
+
+def band_frame(frame, w, h, y0, bh):
+    ys, cw, ch = w * h, w // 2, h // 2
+    Y = frame[:ys].reshape(h, w)
+    U = frame[ys:ys + cw * ch].reshape(ch, cw)
+    V = frame[ys + cw * ch:].reshape(ch, cw)
+    return np.concatenate([Y[y0:y0 + bh].ravel(), U[y0 // 2:(y0 + bh) // 2].ravel(), V[y0 // 2:(y0 + bh) // 2].ravel()])
+
+
+def code_sequence(frames, w, h, qp, mode, weights=None):
+    """Bytes and mean PSNR-Y of P-frames 1.. of `frames` (frame 0 is installed as
+    the key frame's reconstruction), coded as 6-MB-row bands in one batch."""
+    from fdv_amd import gpu as G
+    layout = bands_of(h)
+    enc = G.Encoder([(w, bh) for _, bh in layout], me_mode=mode)
+    if mode == G.ME_LEARNED:
+        enc.set_proposer(weights)
+    for b, (y0, bh) in enumerate(layout):
+        enc.set_ref(b, 0, band_frame(frames[0], w, h, y0, bh))
+    tcs = [G.TabCache() for _ in layout]
+    total, ps = 0, []
+    for f in range(1, len(frames)):
+        fb = [band_frame(frames[f], w, h, y0, bh) for y0, bh in layout]
+        pays, recs = enc.pframe_batch(fb, [1 if f == 1 else 2] * len(layout), [qp] * len(layout), tcs)
+        total += sum(len(p) for p in pays)
+        se = sum(float(np.sum((r[:w * bh].astype(np.float64) - s[:w * bh]) ** 2))
+                 for r, s, (_, bh) in zip(recs, fb, layout))
+        mse = se / (w * h)
+        ps.append(99.0 if mse == 0 else 10 * np.log10(255.0 ** 2 / mse))
+    enc.close()
+    return total, float(np.mean(ps))
+
+
+def evaluate(scenes, qps=(20, 24, 28, 32), nframes=60, mode_a=None, mode_b=None, weights=None, log=print):
+    from fdv_amd import gpu as G
+    mode_a = G.ME_EXHAUSTIVE if mode_a is None else mode_a
+    mode_b = G.ME_LEARNED if mode_b is None else mode_b
+    out = {}
+    for name, sc in scenes:
+        frames = [sc.frame(i) for i in range(nframes)]
+        pa, pb = [], []
+        for qp in qps:
+            ba, da = code_sequence(frames, sc.w, sc.h, qp, mode_a, weights)
+            bb, db = code_sequence(frames, sc.w, sc.h, qp, mode_b, weights)
+            pa.append((ba, da))
+            pb.append((bb, db))
+            log("  %s qp %d: A %d B %.3f dB | B %d B %.3f dB" % (name, qp, ba, da, bb, db))
+        out[name] = {"a": pa, "b": pb, "bd_rate": bd_rate(pa, pb)}
+    return out
+
+
+def main():
+    from fdv_amd import content
+    from fdv_amd import gpu as G
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--frames", type=int, default=60)
+    ap.add_argument("--qps", default="20,24,28,32")
+    ap.add_argument("--b", default="learned", choices=["learned", "parallel", "exhaustive", "diamond"])
+    ap.add_argument("--a", default="exhaustive", choices=["learned", "parallel", "exhaustive", "diamond"])
+    args = ap.parse_args()
+    modes = {"learned": G.ME_LEARNED, "parallel": G.ME_PARALLEL, "exhaustive": G.ME_EXHAUSTIVE,
+             "diamond": G.ME_DIAMOND}
+    scenes = [("pan720", content.scene_pan(1280, 720, 2)),
+              ("objects720", content.scene_objects(1280, 720, 3, nobj=12, smin=32, smax=160, vmax=12, grain=2.0))]
+    res = evaluate(scenes, [int(q) for q in args.qps.split(",")], args.frames, modes[args.a], modes[args.b],
+                   log=lambda m: print(m, flush=True))
+    print(json.dumps({k: v["bd_rate"] for k, v in res.items()}), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -83,6 +83,10 @@ int fdvamd_pframe_batch(fdvamd_encoder *e, const uint8_t *const *frames, const i
                         fdvamd_tabcache *const *tc, uint8_t *const *out, const size_t *cap,
                         size_t *len, uint8_t *const *recon);
 
+/* Install the learned proposer's weights (fdvgpu_nn_nparam() floats) for
+ * me_mode 2.  No counterpart in the reference. */
+int fdvamd_set_proposer(fdvamd_encoder *e, const float *weights, int n);
+
 /* Undo the last frame's reference rotation (all sequences) so the frame can be
  * coded again, e.g. at another QP: the requantising retry of the streaming
  * encoder (fdv.h:8303-8334).  The caller restores its entropy state as the

@@ -101,6 +101,62 @@ int fdvgpu_recon(fdvgpu *g, int seq, uint8_t *i420);
 /* The per-MB decisions of the last frame (nmb records, raster order). */
 int fdvgpu_decisions(fdvgpu *g, int seq, fdvgpu_mbinfo *out);
 
+/* ---- the learned motion proposer (FDVGPU_ME_LEARNED).  It has no counterpart in
+ * the reference: it chooses which whole-pel vectors the block-parallel search
+ * evaluates, and which MBs go straight to SKIP; the codec arithmetic after it
+ * is the reference's. */
+
+/* One MB's proposal: 8 whole-pel vectors on the nearest reference (best first),
+ * their total probability, and the SKIP probability. */
+typedef struct {
+    int8_t cx[8], cy[8];
+    float conf, pskip;
+} fdvgpu_proposal;
+
+/* One MB's motion table entry (quarter-pel vectors): 16x16 per reference, the
+ * four 8x8 quadrants on the nearest, their SADs; flags bit 0 = the proposer
+ * expects SKIP, so the decision weighs SKIP against INTER16 only. */
+typedef struct {
+    int16_t mv16[2][2];
+    int32_t sad16[2];
+    int16_t mv8[4][2];
+    int32_t sad8[4];
+    int32_t flags;
+} fdvgpu_motion_entry;
+
+/* Number of proposer parameters (fp32), and installing them for the learned mode. */
+int fdvgpu_nn_nparam(void);
+int fdvgpu_set_proposer(fdvgpu *g, const float *weights, int n);
+
+/* The proposals and the motion table of the last frame (nmb entries each). */
+int fdvgpu_proposals(fdvgpu *g, int seq, fdvgpu_proposal *out);
+int fdvgpu_motion(fdvgpu *g, int seq, fdvgpu_motion_entry *out);
+
+/* Device pointers of a sequence's source slot luma (tight) and of the nearest
+ * reference's padded luma (at its sample (0,0)), for feature extraction. */
+int fdvgpu_planes(fdvgpu *g, int seq, int src_slot, const uint8_t **cur_y, int *cur_stride,
+                  const uint8_t **ref_y, int *ref_stride);
+
+/* Training on the GPU, on device buffers (stream: a hipStream_t, or NULL):
+ * - fdvgpu_nn_features: the proposer input (289 x 2 floats) of n MBs, MB j at
+ *   luma position pos[2j], pos[2j+1] of frame curs[j] (stride cw[j]) against the
+ *   padded reference luma refs[j] (pointer to its sample (0,0), stride rs[j]),
+ *   coded at QP qps[j];
+ * - fdvgpu_nn_train_step: forward and backward over n tuples (features, vector
+ *   target as a position 0..288 or -1, SKIP label), the SKIP loss weighing false
+ *   positives by wfp, adding the summed gradient into grad and into loss[8]:
+ *   vector loss, SKIP loss, top-1 hits, top-8 hits, tuples with SKIP
+ *   probability >= thr and how many of them are SKIP, tuples with top-8
+ *   probability >= thr and how many of them have the target in the top 8;
+ * - fdvgpu_nn_adam: one Adam step (beta 0.9 / 0.999) with grad / batch, then
+ *   grad cleared; step counts from 1. */
+int fdvgpu_nn_features(const uint8_t *const *curs, const int *cw, const uint8_t *const *refs, const int *rs,
+                       const int *pos, const int *qps, int n, float *feats, void *stream);
+int fdvgpu_nn_train_step(const float *w, const float *feats, const int *tgt, const uint8_t *skip, int n,
+                         float wfp, float thr, float *grad, float *loss, int nblocks, void *stream);
+int fdvgpu_nn_adam(float *w, float *grad, float *m1, float *m2, int n, float lr, int step, int batch,
+                   void *stream);
+
 /* Wall-clock milliseconds of the device work of the last fdvgpu_pframe, from
  * HIP events on the context's stream: [0] motion tables (exhaustive / learned
  * modes), [1] the MB wavefront, [2] deblock + borders, [3] serialisation. */

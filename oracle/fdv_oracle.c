@@ -1346,7 +1346,61 @@ void orc_metab_exhaustive(const uint8_t *cy, int w, int h, const uint8_t *const 
                                            0, 0, ORC_ME_EXHAUSTIVE, &a, &b);
                 m->mv8[q][0] = (int16_t)a; m->mv8[q][1] = (int16_t)b;
             }
+            if (navail < 2) {                   /* the unused far entry repeats the near one */
+                m->mv16[1][0] = m->mv16[0][0]; m->mv16[1][1] = m->mv16[0][1]; m->sad16[1] = m->sad16[0];
+            }
         }
+}
+
+/* Whole-pel search over a point set with the exhaustive search's tie order
+ * (SAD, then |dx|+|dy|, then raster index over the +-range grid), then the
+ * reference's sub-pel cross. */
+static int search_points(const uint8_t *cur, int cs, const uint8_t *ref, int rs, int bx, int by, int bs,
+                         const int (*pts)[2], int npts, int *mvx, int *mvy) {
+    long long bestk = -1;
+    int bxv = 0, byv = 0, best = 0;
+    for (int k = 0; k < npts; ++k) {
+        const int dx = pts[k][0], dy = pts[k][1];
+        const int s = sad_at(cur, cs, ref, rs, bx, by, bs, bs, dx * 4, dy * 4);
+        const long long key = ((long long)s << 32) | ((long long)(iabs(dx) + iabs(dy)) << 16) | ((dy + 16) * 33 + dx + 16);
+        if (bestk < 0 || key < bestk) { bestk = key; best = s; bxv = dx * 4; byv = dy * 4; }
+    }
+    subpel_refine(cur, cs, ref, rs, bx, by, bs, bs, &bxv, &byv, &best);
+    *mvx = bxv; *mvy = byv;
+    return best;
+}
+
+void orc_metab_candidates(const uint8_t *cy, int w, int h, const uint8_t *const RY[2], int ys, int navail,
+                          const orc_nnout *nn, float thr, orc_metab *mt) {
+    const int mw = w / 16, mh = h / 16;
+    orc_metab_exhaustive(cy, w, h, RY, ys, navail, mt);     /* the fallback for every MB */
+    for (int i = 0; i < mw * mh; ++i) {
+        const orc_nnout *P = &nn[i];
+        orc_metab *m = &mt[i];
+        m->flags = P->pskip >= thr ? 1 : 0;
+        if (!(P->conf >= thr)) continue;
+        const int bx = (i % mw) * 16, by = (i / mw) * 16;
+        for (int r = 0; r < navail; ++r) {
+            int pts[73][2], n = 0;
+            for (int c = 0; c < 8; ++c)
+                for (int o = 0; o < 9; ++o) {
+                    const int dx = (r + 1) * P->cx[c] + o % 3 - 1, dy = (r + 1) * P->cy[c] + o / 3 - 1;
+                    if (dx < -16 || dx > 16 || dy < -16 || dy > 16) continue;
+                    pts[n][0] = dx; pts[n][1] = dy; ++n;
+                }
+            pts[n][0] = 0; pts[n][1] = 0; ++n;
+            int a, b;
+            m->sad16[r] = search_points(cy, w, RY[r], ys, bx, by, 16, (const int (*)[2])pts, n, &a, &b);
+            m->mv16[r][0] = (int16_t)a; m->mv16[r][1] = (int16_t)b;
+            if (r == 0)
+                for (int q = 0; q < 4; ++q) {
+                    m->sad8[q] = search_points(cy, w, RY[0], ys, bx + (q & 1) * 8, by + (q >> 1) * 8, 8,
+                                               (const int (*)[2])pts, n, &a, &b);
+                    m->mv8[q][0] = (int16_t)a; m->mv8[q][1] = (int16_t)b;
+                }
+        }
+        if (navail < 2) { m->mv16[1][0] = m->mv16[0][0]; m->mv16[1][1] = m->mv16[0][1]; m->sad16[1] = m->sad16[0]; }
+    }
 }
 
 void orc_pframe_costs_par(const uint8_t *cy, int w, int h, const uint8_t *const RY[2], int ys,
@@ -1358,13 +1412,13 @@ void orc_pframe_costs_par(const uint8_t *cy, int w, int h, const uint8_t *const 
             const orc_metab *T = &mt[my * mw + mx];
             orc_pcost *P = &pc[my * mw + mx];
             memset(P, 0, sizeof *P);
-            if (T->flags & 1) continue;                 /* a forced SKIP needs no costs */
             for (int r = 0; r < navail; ++r) {
                 uint8_t pr[256], rr[256]; coded_t tc; double dd = 0.0; int bb = 6;
                 orc_mc_luma(RY[r], ys, bx, by, 16, 16, T->mv16[r][0], T->mv16[r][1], pr, 16);
                 residual(cy, w, bx, by, pr, 16, qp, lambda, rr, &tc, &dd, &bb);
                 P->d16[r] = (int32_t)dd; P->b16[r] = bb;
             }
+            if (T->flags & 1) continue;                 /* SKIP expected: no INTRA, no INTER8 costs */
             {
                 uint8_t ntop[16], nleft[16], ntl, pI[256], rn[256]; int nht, nhl;
                 neighbours(cy, w, bx, by, 16, 0, ntop, nleft, &ntl, &nht, &nhl);
@@ -1424,7 +1478,8 @@ void orc_pframe_repair_par(const uint8_t *cy, int w, int h, const uint8_t *const
             }
             const double Js = (double)Ds + lambda * (double)(3 + (ncand > 1 ? 3 : 0));
             double Jbest = Js;
-            const int skip_wins = Js <= lambda * FL_INTRA || (T->flags & 1);
+            const int skip_wins = Js <= lambda * FL_INTRA;
+            const int pruned = T->flags & 1;             /* SKIP expected: no INTRA, no INTER8 */
             double Ji = 1e30; int iref = 0, sad16 = 0;
             for (int r = 0; r < navail && !skip_wins; ++r) {
                 if (r > 0 && Jbest <= lambda * FL_INTER16) break;
@@ -1434,12 +1489,12 @@ void orc_pframe_repair_par(const uint8_t *cy, int w, int h, const uint8_t *const
                 if (J < Ji) { Ji = J; iref = r; sad16 = T->sad16[r]; }
             }
             double Jn = 1e30;
-            if (!skip_wins && Jbest > lambda * FL_INTRA) {
+            if (!skip_wins && !pruned && Jbest > lambda * FL_INTRA) {
                 Jn = (double)P->dn + lambda * P->bn;
                 if (Jn < Jbest) Jbest = Jn;
             }
             double J8 = 1e30;
-            if (!skip_wins && Jbest > lambda * FL_INTER8) {
+            if (!skip_wins && !pruned && Jbest > lambda * FL_INTER8) {
                 const int ssad = T->sad8[0] + T->sad8[1] + T->sad8[2] + T->sad8[3];
                 if (!(sad16 > 0 && ssad * 10 >= sad16 * 9)) {
                     int b8 = P->b8, qpx = pmx, qpy = pmy;

@@ -121,7 +121,7 @@ typedef struct {
     int32_t sad16[2];
     int16_t mv8[4][2];
     int32_t sad8[4];
-    int32_t flags;           /* bit0: SKIP without evaluating the other modes */
+    int32_t flags;           /* bit0: SKIP expected -- INTRA and INTER8 are not evaluated */
 } orc_metab;
 
 /* Per-MB candidate costs: SSD and bits of each mode's residual coding, without
@@ -133,9 +133,23 @@ typedef struct {
     uint8_t nsub, pad[3];
 } orc_pcost;
 
+/* The learned proposer's output for one MB (fdvg_nnout). */
+typedef struct {
+    int8_t cx[8], cy[8];     /* whole-pel vectors on the nearest reference */
+    float conf, pskip;
+} orc_nnout;
+
 /* Exhaustive-search table of a frame (16x16 per reference, 8x8 quadrants on 0). */
 void orc_metab_exhaustive(const uint8_t *cy, int w, int h, const uint8_t *const RY[2], int ys,
                           int navail, orc_metab *mt);
+/* The table of the learned mode: where the proposer's confidence is at least
+ * thr, the whole-pel search covers the 3x3 neighbourhoods of its 8 vectors
+ * (scaled by 2 on the far reference) and the zero vector, with the exhaustive
+ * search's tie order, then the sub-pel cross; elsewhere the exhaustive search.
+ * flags bit 0 = SKIP probability >= thr: the decision then weighs SKIP against
+ * INTER16 only. */
+void orc_metab_candidates(const uint8_t *cy, int w, int h, const uint8_t *const RY[2], int ys, int navail,
+                          const orc_nnout *nn, float thr, orc_metab *mt);
 /* Every MB's candidate costs, independently of every other MB. */
 void orc_pframe_costs_par(const uint8_t *cy, int w, int h, const uint8_t *const RY[2], int ys,
                           int navail, int qp, double lambda, const orc_metab *mt, orc_pcost *pc);

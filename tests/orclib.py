@@ -293,3 +293,28 @@ def oracle_parallel(cur, r0, r1, w, h, navail, qp, lscale=1.0, metab=None):
                               C.addressof(mt), C.addressof(pc), C.addressof(given))
     st, rec, info = oracle_analyse(cur, r0, r1, w, h, navail, qp, lscale, 0, 0, given=given)
     return st, rec, info, given, pc
+
+
+def oracle_metab_candidates(cur, r0, r1, w, h, navail, nnout, thr=0.7):
+    """The learned mode's motion table from the proposer outputs `nnout` (a numpy
+    array with the fdvgpu_proposal layout)."""
+    lib = oracle()
+    keep, ptrs, ys, cs = _padded_refs(r0, r1, w, h)
+    RY = (C.c_void_p * 2)(*ptrs[0])
+    nmb = (w // 16) * (h // 16)
+    mt = (Metab * nmb)()
+    cy = split_i420(cur, w, h)[0]
+    nn = np.ascontiguousarray(nnout)
+    assert nn.dtype.itemsize == 24 and nn.size == nmb
+    lib.orc_metab_candidates.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_void_p,
+                                         C.c_float, C.c_void_p]
+    lib.orc_metab_candidates(C.c_void_p(cy.ctypes.data), w, h, C.addressof(RY), ys, navail,
+                             C.c_void_p(nn.ctypes.data), float(thr), C.addressof(mt))
+    return mt
+
+
+def metab_to_np(mt):
+    import numpy as _np
+    dt = _np.dtype([("mv16", "<i2", (2, 2)), ("sad16", "<i4", 2), ("mv8", "<i2", (4, 2)),
+                    ("sad8", "<i4", 4), ("flags", "<i4")])
+    return _np.frombuffer(bytes(mt), dtype=dt).copy()
