@@ -11,6 +11,13 @@ decision, residual coding, reconstruction, deblocking and serialisation of the
 16 symbol streams, plus their copy back to the host.  Source frames are resident
 in HBM before timing starts.
 
+--mode picks the search: learned (default: the learned proposer with the
+exhaustive search below its confidence threshold, block-parallel decisions),
+exhaustive, parallel (exhaustive search, block-parallel decisions) or diamond
+(the reference's search and decision wavefront, bit-exact with pframe_encode).
+The exhaustive mode is timed on the same frames for the learned-vs-exhaustive
+comparison; launch shapes are autotuned on the first frame.
+
 Prints one JSON line (see the keys below).  --gpus above 1 is refused: this
 path runs on one GPU.
 """
@@ -118,7 +125,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--mode", default="diamond", choices=["diamond", "exhaustive", "learned", "parallel"])
+    ap.add_argument("--mode", default="learned", choices=["diamond", "exhaustive", "learned", "parallel"])
+    ap.add_argument("--baseline-steps", type=int, default=10,
+                    help="steps of the exhaustive search timed for comparison (0: skip)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -152,22 +161,30 @@ def main():
     mode = {"diamond": G.ME_DIAMOND, "exhaustive": G.ME_EXHAUSTIVE, "learned": G.ME_LEARNED,
             "parallel": G.ME_PARALLEL}[args.mode]
 
-    an = G.Analyser([(W, bh) for _, bh in layout], nsrc=nframes)
-    if mode == G.ME_LEARNED:
-        an.set_proposer()
     per_band = [band_frames(f, W, H, layout) for f in frames]
-    for b in range(len(layout)):
-        an.put_ref(b, 0, per_band[0][b])
-        for i in range(1, nframes):
-            an.put_source(b, i, per_band[i][b])
     nb = len(layout)
 
-    def step(i):
+    def open_analyser(m):
+        a = G.Analyser([(W, bh) for _, bh in layout], nsrc=nframes)
+        if m == G.ME_LEARNED:
+            a.set_proposer()
+        for b in range(nb):
+            a.put_ref(b, 0, per_band[0][b])
+            for i in range(1, nframes):
+                a.put_source(b, i, per_band[i][b])
+        # launch shapes for this batch, timed on its first frame (the reference pool is restored)
+        a.autotune([1] * nb, [QP] * nb, navail=[1] * nb, me_mode=m, reps=2)
+        return a
+
+    an = open_analyser(mode)
+
+    def step(i, a=None, m=None):
+        a = an if a is None else a
         src = 1 + (i % (nframes - 1))
         navail = 1 if i == 0 else 2
-        an.pframe([src] * nb, [QP] * nb, [1.0] * nb, [navail] * nb, [0] * nb, mode)
-        an.fetch()
-        return an.timings()
+        a.pframe([src] * nb, [QP] * nb, [1.0] * nb, [navail] * nb, [0] * nb, mode if m is None else m)
+        a.fetch()
+        return a.timings()
 
     for i in range(args.warmup):
         step(i)
@@ -191,17 +208,39 @@ def main():
             "bytes_per_launch": int(algo_bytes)}
     stage_ms = {"motion_tables": round(float(kt[:, 0].mean()), 4), "wavefront": round(k_ms, 4),
                 "deblock": round(float(kt[:, 2].mean()), 4), "serialise": round(float(kt[:, 3].mean()), 4)}
-    cpu = None if args.no_cpu else cpu_baseline(frames, W, layout, QP, args.cpu_seconds)
+    shapes = an.tune()
     an.close()
+    # the exhaustive search on the same frames, for the learned-vs-exhaustive comparison
+    base = None
+    if args.baseline_steps > 0 and mode != G.ME_EXHAUSTIVE:
+        ab = open_analyser(G.ME_EXHAUSTIVE)
+        for i in range(min(args.warmup, 2)):
+            step(i, ab, G.ME_EXHAUSTIVE)
+        tb = time.perf_counter()
+        for i in range(args.baseline_steps):
+            step(i + 2, ab, G.ME_EXHAUSTIVE)
+        base = args.baseline_steps * px_frame / (time.perf_counter() - tb) / 1e6
+        ab.close()
+    bd = None
+    bd_path = os.path.join(ROOT, "fdv_amd", "learned", "bdrate_720p.json")
+    if mode == G.ME_LEARNED and os.path.exists(bd_path):
+        with open(bd_path) as f:
+            bd = json.load(f)["bd_rate_percent"]
+    cpu = None if args.no_cpu else cpu_baseline(frames, W, layout, QP, args.cpu_seconds)
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "Mpix/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt / args.steps * 1e3, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u8",
         "data": "synthetic (seeded pan + objects + grain)",
         "config": {"workload": "1280x720 4:2:0 P-frames, QP 24, 8 tiles (6-MB-row bands), "
-                               "%s motion search" % args.mode,
+                               "%s motion search%s" % (args.mode, ", block-parallel decisions"
+                                                        if mode in (G.ME_LEARNED, G.ME_PARALLEL) else ""),
                    "frame": [W, H], "bands": len(layout), "qp": QP, "search": args.mode},
         "stage_ms": stage_ms,
+        "launch_shapes": {"serialiser_waves": shapes[0], "deblock_threads": shapes[1], "proposer_wg_per_cu": shapes[2]},
+        "exhaustive_mpix_s": None if base is None else round(base, 3),
+        "speedup_vs_exhaustive": None if base is None else round(value / base, 3),
+        "bd_rate_vs_exhaustive": bd,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

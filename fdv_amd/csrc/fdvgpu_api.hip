@@ -19,15 +19,23 @@
 extern "C" int fdvg_set_tables(const int32_t *c8, const int32_t *qs8);
 extern "C" int fdvg_launch_analyse(const fdvg_seq *, const int2 *, int, unsigned *, fdvg_mbrec *,
                                    uint64_t *, hipStream_t);
-extern "C" int fdvg_launch_deblock(const void *, int, int, hipStream_t);
+extern "C" int fdvg_launch_deblock(const void *, int, int, int, hipStream_t);
 extern "C" int fdvg_launch_border(const void *, int, int, hipStream_t);
 extern "C" int fdvg_launch_pad(const void *, const uint8_t *const *, int, int, hipStream_t);
 extern "C" int fdvg_launch_zero(uint32_t *, int, hipStream_t);
 extern "C" int fdvg_launch_symbols(const void *, int, int, const fdvg_mbrec *, uint32_t *, uint32_t *,
-                                   uint32_t *, uint8_t *, size_t, hipStream_t);
+                                   uint32_t *, uint8_t *, size_t, int, hipStream_t);
+
+// Launch shapes the autotuner chooses among (fdvgpu_autotune).
+struct Tune {
+    int sym_waves;      // serialiser MBs (wavefronts) per workgroup: 1, 2, 4, 8
+    int deblock_bs;     // deblock threads per workgroup: 64, 128, 256
+    int nn_wpc;         // proposer workgroups per CU: 1, 2
+};
 extern "C" int fdvg_launch_metab(const fdvg_seq *, int, int, void *, const void *, float, hipStream_t);
-extern "C" int fdvg_launch_nn_infer(const fdvg_seq *, int, int, const float *, fdvg_nnout *, hipStream_t);
+extern "C" int fdvg_launch_nn_infer(const fdvg_seq *, int, int, const float *, fdvg_nnout *, int, hipStream_t);
 extern "C" int fdvg_nn_nparam(void);
+extern "C" int fdvg_nn_setup(void);
 
 // The proposer's confidence threshold: below it an MB gets the exhaustive search.
 #define FDVG_NN_THRESHOLD 0.7f
@@ -82,6 +90,11 @@ struct fdvgpu {
     fdvg_mbinfo *pgiven_d;
     uint64_t *mrec_d;               // motion-record handoff of the decision wavefront
     unsigned *ilist_d, *done_d;     // intra MBs in dependency order (count first), per-MB done flags
+    // launch shapes, and one instantiated graph per (search mode, decision structure)
+    Tune tune;
+    int graphs;
+    hipGraphExec_t gexec[64];       // per (stage, search mode, decision structure)
+    int warm[64];                   // the key has run once eagerly (launch caches filled)
     // learned proposer
     float *nnw_d;
     fdvg_nnout *nn_d;
@@ -124,6 +137,7 @@ extern "C" void fdvgpu_close(fdvgpu *g) {
     (void)hipFree(g->pcost_d); (void)hipFree(g->pctx_d); (void)hipFree(g->pgiven_d); (void)hipFree(g->mrec_d);
     (void)hipFree(g->ilist_d); (void)hipFree(g->done_d);
     (void)hipFree(g->nnw_d); (void)hipFree(g->nn_d);
+    for (int k = 0; k < 64; ++k) if (g->gexec[k]) (void)hipGraphExecDestroy(g->gexec[k]);
     for (int i = 0; i < 5; ++i) if (g->ev[i]) (void)hipEventDestroy(g->ev[i]);
     for (int i = 0; i < 8; ++i) if (g->ring_ev[i]) (void)hipEventDestroy(g->ring_ev[i]);
     if (g->st) (void)hipStreamDestroy(g->st);
@@ -203,6 +217,9 @@ extern "C" fdvgpu *fdvgpu_open(int nseq, const int *w, const int *h, int nsrc) {
     CKN(hipMalloc(&g->ilist_d, ((size_t)g->total_mb + 1) * sizeof(unsigned)));
     CKN(hipMalloc(&g->done_d, (size_t)g->total_mb * sizeof(unsigned)));
     CKN(hipMalloc(&g->nnw_d, (size_t)fdvg_nn_nparam() * sizeof(float)));
+    if (fdvg_nn_setup() != 0) { seterr("proposer kernel setup failed"); goto fail; }
+    g->tune = Tune{4, 256, 1};
+    g->graphs = 1;
     CKN(hipMalloc(&g->nn_d, (size_t)g->total_mb * sizeof(fdvg_nnout)));
 
     // rows in (row, sequence) order: every sequence's row r before any row r+1
@@ -288,6 +305,72 @@ extern "C" int fdvgpu_set_given(fdvgpu *g, int seq, const fdvgpu_mbinfo *dec) {
     return 0;
 }
 
+// The device work of one frame step after its descriptor copies, in four
+// stages (motion tables, decisions, deblock + borders, serialisation): each a
+// fixed sequence of launches on the context's stream for a given search mode
+// and decision structure (par: block-parallel decisions).
+static int launch_stage(fdvgpu *g, int stage, int me_mode, bool par) {
+    if (stage == 0) {
+        if (me_mode == FDVGPU_ME_LEARNED &&
+            fdvg_launch_nn_infer(g->seq_d, g->nseq, g->maxmb, g->nnw_d, g->nn_d, g->tune.nn_wpc, g->st) != 0) {
+            seterr("proposer launch failed"); return -1;
+        }
+        if (me_mode != FDVGPU_ME_DIAMOND &&
+            fdvg_launch_metab(g->seq_d, g->nseq, g->maxmb, g->metab_d, me_mode == FDVGPU_ME_LEARNED ? g->nn_d : NULL,
+                              FDVG_NN_THRESHOLD, g->st) != 0) {
+            seterr("motion table launch failed"); return -1;
+        }
+        if (fdvg_launch_zero(g->ticket_d, 8, g->st) != 0) { seterr("ticket reset launch failed"); return -1; }
+    } else if (stage == 1) {
+        if (par) {
+            if (fdvg_launch_zero(g->ilist_d, 1, g->st) != 0 || fdvg_launch_zero(g->done_d, g->total_mb, g->st) != 0 ||
+                fdvg_launch_parallel(g->seq_d, g->nseq, g->maxmb, g->maxmh, g->maxmw, g->ticket_d, g->pcost_d,
+                                     g->pgiven_d, g->pctx_d, g->ilist_d, g->done_d, g->total_mb,
+                                     g->recs_d, g->st) != 0) {
+                seterr("parallel analysis launch failed: %s", hipGetErrorString(hipGetLastError())); return -1;
+            }
+        } else if (fdvg_launch_analyse(g->seq_d, g->rowtab_d, g->nrows, g->ticket_d, g->recs_d, g->hand_d, g->st) != 0) {
+            seterr("analysis launch failed: %s", hipGetErrorString(hipGetLastError())); return -1;
+        }
+    } else if (stage == 2) {
+        if (fdvg_launch_deblock(g->planes_d, 3 * g->nseq, g->maxpx, g->tune.deblock_bs, g->st) != 0 ||
+            fdvg_launch_border(g->planes_d, 3 * g->nseq, g->maxpx, g->st) != 0) {
+            seterr("deblock launch failed"); return -1;
+        }
+    } else if (fdvg_launch_symbols(g->sym_d, g->nseq, g->maxmb, g->recs_d, g->cnt_d, g->totals_d, g->bases_d,
+                                   g->compact_d, g->compact_cap, g->tune.sym_waves, g->st) != 0) {
+        seterr("serialiser launch failed"); return -1;
+    }
+    return 0;
+}
+
+// One stage: eagerly the first time its key is seen (which also fills the
+// launchers' one-time caches), from then on as a captured hipGraph.
+static int run_stage(fdvgpu *g, int stage, int me_mode, bool par) {
+    const int key = (stage * 4 + me_mode) * 2 + (par ? 1 : 0);
+    if (!g->graphs || !g->warm[key]) {
+        if (launch_stage(g, stage, me_mode, par) != 0) return -1;
+        g->warm[key] = 1;
+        return 0;
+    }
+    if (!g->gexec[key]) {
+        hipGraph_t gr = NULL;
+        CK(hipStreamBeginCapture(g->st, hipStreamCaptureModeThreadLocal));
+        const int rc = launch_stage(g, stage, me_mode, par);
+        const hipError_t e = hipStreamEndCapture(g->st, &gr);
+        if (rc != 0 || e != hipSuccess) {
+            if (gr) (void)hipGraphDestroy(gr);
+            seterr("frame graph capture failed: %s", rc != 0 ? fdvgpu_last_error() : hipGetErrorString(e));
+            return -1;
+        }
+        const hipError_t ei = hipGraphInstantiate(&g->gexec[key], gr, NULL, NULL, 0);
+        (void)hipGraphDestroy(gr);
+        if (ei != hipSuccess) { g->gexec[key] = NULL; seterr("frame graph: %s", hipGetErrorString(ei)); return -1; }
+    }
+    CK(hipGraphLaunch(g->gexec[key], g->st));
+    return 0;
+}
+
 extern "C" int fdvgpu_pframe(fdvgpu *g, const int *src_slot, const int *qp, const double *lscale,
                              const int *navail, const int *force_skip, int me_mode) {
     if (!g || !src_slot || !qp || !lscale || !navail) { seterr("fdvgpu_pframe: bad arguments"); return -1; }
@@ -343,38 +426,11 @@ extern "C" int fdvgpu_pframe(fdvgpu *g, const int *src_slot, const int *qp, cons
     CK(hipMemcpyAsync(g->seq_d, seq_h, g->nseq * sizeof(fdvg_seq), hipMemcpyHostToDevice, g->st));
     CK(hipMemcpyAsync(g->planes_d, planes_h, 3 * g->nseq * sizeof(PlaneRef), hipMemcpyHostToDevice, g->st));
     CK(hipEventRecord(g->ring_ev[slot], g->st));
-    if (me_mode == FDVGPU_ME_LEARNED &&
-        fdvg_launch_nn_infer(g->seq_d, g->nseq, g->maxmb, g->nnw_d, g->nn_d, g->st) != 0) {
-        seterr("proposer launch failed"); return -1;
+    // the stages, with the timing events between them (outside any graph)
+    for (int stage = 0; stage < 4; ++stage) {
+        if (run_stage(g, stage, me_mode, par) != 0) return -1;
+        CK(hipEventRecord(g->ev[stage + 1], g->st));
     }
-    if (me_mode != FDVGPU_ME_DIAMOND &&
-        fdvg_launch_metab(g->seq_d, g->nseq, g->maxmb, g->metab_d, me_mode == FDVGPU_ME_LEARNED ? g->nn_d : NULL,
-                          FDVG_NN_THRESHOLD, g->st) != 0) {
-        seterr("motion table launch failed"); return -1;
-    }
-    if (fdvg_launch_zero(g->ticket_d, 8, g->st) != 0) { seterr("ticket reset launch failed"); return -1; }
-    CK(hipEventRecord(g->ev[1], g->st));
-    if (par) {
-        if (fdvg_launch_zero(g->ilist_d, 1, g->st) != 0 || fdvg_launch_zero(g->done_d, g->total_mb, g->st) != 0 ||
-            fdvg_launch_parallel(g->seq_d, g->nseq, g->maxmb, g->maxmh, g->maxmw, g->ticket_d, g->pcost_d,
-                                 g->pgiven_d, g->pctx_d, g->ilist_d, g->done_d, g->total_mb,
-                                 g->recs_d, g->st) != 0) {
-            seterr("parallel analysis launch failed: %s", hipGetErrorString(hipGetLastError())); return -1;
-        }
-    } else if (fdvg_launch_analyse(g->seq_d, g->rowtab_d, g->nrows, g->ticket_d, g->recs_d, g->hand_d, g->st) != 0) {
-        seterr("analysis launch failed: %s", hipGetErrorString(hipGetLastError())); return -1;
-    }
-    CK(hipEventRecord(g->ev[2], g->st));
-    if (fdvg_launch_deblock(g->planes_d, 3 * g->nseq, g->maxpx, g->st) != 0 ||
-        fdvg_launch_border(g->planes_d, 3 * g->nseq, g->maxpx, g->st) != 0) {
-        seterr("deblock launch failed"); return -1;
-    }
-    CK(hipEventRecord(g->ev[3], g->st));
-    if (fdvg_launch_symbols(g->sym_d, g->nseq, g->maxmb, g->recs_d, g->cnt_d, g->totals_d, g->bases_d,
-                            g->compact_d, g->compact_cap, g->st) != 0) {
-        seterr("serialiser launch failed"); return -1;
-    }
-    CK(hipEventRecord(g->ev[4], g->st));
     for (int s = 0; s < g->nseq; ++s) {
         SeqMem &m = g->S[s];
         int old_far = m.far_;
@@ -490,5 +546,67 @@ extern "C" int fdvgpu_planes(fdvgpu *g, int seq, int src_slot, const uint8_t **c
     SeqMem &m = g->S[seq];
     *cur_y = m.src + (size_t)src_slot * m.srcsz; *cur_stride = m.w;
     *ref_y = fy(m, m.near_); *ref_stride = m.ys;
+    return 0;
+}
+
+extern "C" int fdvgpu_set_graphs(fdvgpu *g, int on) {
+    if (!g) { seterr("fdvgpu_set_graphs: bad context"); return -1; }
+    g->graphs = on ? 1 : 0;
+    return 0;
+}
+
+static void drop_graphs(fdvgpu *g) {
+    for (int k = 0; k < 64; ++k)
+        if (g->gexec[k]) { (void)hipGraphExecDestroy(g->gexec[k]); g->gexec[k] = NULL; }
+}
+
+extern "C" int fdvgpu_get_tune(fdvgpu *g, int out[3]) {
+    if (!g || !out) { seterr("fdvgpu_get_tune: bad arguments"); return -1; }
+    out[0] = g->tune.sym_waves; out[1] = g->tune.deblock_bs; out[2] = g->tune.nn_wpc;
+    return 0;
+}
+
+extern "C" int fdvgpu_set_tune(fdvgpu *g, const int in[3]) {
+    if (!g || !in) { seterr("fdvgpu_set_tune: bad arguments"); return -1; }
+    g->tune = Tune{in[0], in[1], in[2]};
+    drop_graphs(g);
+    return 0;
+}
+
+// Each candidate launch shape times `reps` frame steps of the given frame (the
+// reference pool is rewound after each), one parameter at a time, keeping the
+// fastest; the chosen shapes stay in the context.
+extern "C" int fdvgpu_autotune(fdvgpu *g, const int *src_slot, const int *qp, const double *lscale,
+                               const int *navail, int me_mode, int reps) {
+    if (!g || reps <= 0) { seterr("fdvgpu_autotune: bad arguments"); return -1; }
+    static const int opts[3][3] = {{2, 4, 8}, {64, 128, 256}, {1, 2, 0}};
+    for (int prm = 0; prm < 3; ++prm) {
+        if (prm == 2 && me_mode != FDVGPU_ME_LEARNED) continue;
+        float best = 1e30f;
+        int bestv = 0;
+        for (int k = 0; k < 3 && opts[prm][k]; ++k) {
+            Tune t = g->tune;
+            if (prm == 0) t.sym_waves = opts[0][k];
+            else if (prm == 1) t.deblock_bs = opts[1][k];
+            else t.nn_wpc = opts[2][k];
+            g->tune = t;
+            drop_graphs(g);
+            float tot = 0.f;
+            for (int r = 0; r <= reps; ++r) {           // the first run warms up (and captures)
+                if (fdvgpu_pframe(g, src_slot, qp, lscale, navail, NULL, me_mode) != 0) return -1;
+                CK(hipEventSynchronize(g->ev[4]));
+                float ms = 0.f;
+                CK(hipEventElapsedTime(&ms, g->ev[0], g->ev[4]));
+                if (r) tot += ms;
+                if (fdvgpu_rewind(g) != 0) return -1;
+            }
+            if (tot < best) { best = tot; bestv = opts[prm][k]; }
+        }
+        if (prm == 0) g->tune.sym_waves = bestv;
+        else if (prm == 1) g->tune.deblock_bs = bestv;
+        else g->tune.nn_wpc = bestv;
+        drop_graphs(g);
+    }
+    g->have_frame = 0;
     return 0;
 }

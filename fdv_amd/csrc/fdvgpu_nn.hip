@@ -441,19 +441,17 @@ extern "C" int fdvg_nnphase_read(unsigned long long *out) {
 #define NPH(k) do { } while (0)
 #endif
 
-// The inference path keeps each wavefront's B operands (all three layers'
-// weights, bf16) in registers and the activations in LDS rows of NN_RS bf16
-// (72 bytes: consecutive positions fall in different banks), so that two
-// workgroups fit a CU.
+// The inference path: activations in LDS rows of NN_RS bf16 (72 bytes, so
+// consecutive positions fall in different banks); each wavefront loads one
+// layer's B operands (weights, bf16) into registers at a time and runs up to
+// three 32-position tiles side by side (independent MFMA chains); the linear
+// layer is fused into the conv3 epilogue as cross-lane sums of the fp32
+// accumulators.  LDS is about 50 KB.
 #define NN_RS 36
 struct __align__(16) NnInfer {
     float feat[NN_P * 2];
     uint16_t fb[NN_P * 2];
-    uint16_t a2[NN_PT * NN_RS];
-    union {
-        uint16_t a1[NN_PT * NN_RS];
-        float a3[NN_PT * NN_C];           // conv3 output (a1 is dead by then)
-    };
+    uint16_t a1[NN_PT * NN_RS], a2[NN_PT * NN_RS];
     float b1[32], b2[32], b3[32], wf[64], bf[2];
     float lg[NN_PT], sk[NN_PT];
     float cv[32];
@@ -462,21 +460,41 @@ struct __align__(16) NnInfer {
     int ired[8][8];
 };
 
-__device__ __forceinline__ f16v conv_tile_r(const uint16_t *in, const bf4 *wr, int tile) {
+// B operands of one 32-channel layer for this lane: B[k][co = lane % 32],
+// k = 8 s + 4 (lane / 32) + 0..3.
+__device__ __forceinline__ void load_w(bf4 *wr, const float *W, int off) {
+    const int l = threadIdx.x & 63, i32 = l & 31, g = l >> 5;
+#pragma unroll
+    for (int s = 0; s < 36; ++s) {
+        const float4 f = *(const float4 *)&W[off + i32 * NN_K + 8 * s + 4 * g];
+        wr[s][0] = (short)f2bf(f.x); wr[s][1] = (short)f2bf(f.y);
+        wr[s][2] = (short)f2bf(f.z); wr[s][3] = (short)f2bf(f.w);
+    }
+}
+
+// Up to three output tiles (t0, t0 + nw, t0 + 2 nw) of a 32-channel 3x3 layer.
+__device__ __forceinline__ void conv3tiles(const uint16_t *in, const bf4 *wr, int t0, int nw, int nt, f16v *c) {
     const int l = threadIdx.x & 63, i = l & 31, g = l >> 5;
-    const int p = tile * 32 + i;
-    f16v c = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) c[k][r] = 0.f;
 #pragma unroll
     for (int tap = 0; tap < 9; ++tap) {
-        const int q = nn_shift(p, tap);
+        int q[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) q[k] = k < nt ? nn_shift((t0 + k * nw) * 32 + i, tap) : -1;
 #pragma unroll
         for (int c0 = 0; c0 < 4; ++c0) {
-            bf4 a = {0, 0, 0, 0};
-            if (q >= 0) a = *(const bf4 *)&in[q * NN_RS + c0 * 8 + 4 * g];
-            c = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, wr[tap * 4 + c0], c, 0, 0, 0);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                if (k >= nt) continue;
+                bf4 a = {0, 0, 0, 0};
+                if (q[k] >= 0) a = *(const bf4 *)&in[q[k] * NN_RS + c0 * 8 + 4 * g];
+                c[k] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, wr[tap * 4 + c0], c[k], 0, 0, 0);
+            }
         }
     }
-    return c;
 }
 
 // Per MB of every sequence: the 8 most likely whole-pel vectors (nearest
@@ -488,11 +506,11 @@ k_nn_infer(const fdvg_seq *__restrict__ seqs, const float *__restrict__ W, fdvg_
     const fdvg_seq q = seqs[blockIdx.y];
     const int mw = q.w >> 4, nmb = mw * (q.h >> 4);
     const int t = threadIdx.x, wv = t >> 6, l = t & 63, i32 = l & 31, g = l >> 5, nw = blockDim.x >> 6;
+    const int nt = (NN_PT / 32 - wv + nw - 1) / nw;      // this wave's tiles: wv, wv + nw, ...
 #ifdef FDVG_PHASE_PROF
     long long nph_last = clock64();
 #endif
-    // this lane's B operands: B[k][co = i32] for k = 8 s + 4 g + 0..3
-    bf4 w1r[3], w2r[36], w3r[36];
+    bf4 w1r[3];
 #pragma unroll
     for (int s = 0; s < 3; ++s)
 #pragma unroll
@@ -500,22 +518,16 @@ k_nn_infer(const fdvg_seq *__restrict__ seqs, const float *__restrict__ W, fdvg_
             const int k = 8 * s + 4 * g + u;
             w1r[s][u] = k < 18 ? (short)f2bf(W[OFF_W1 + i32 * 18 + k]) : (short)0;
         }
-#pragma unroll
-    for (int s = 0; s < 36; ++s)
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int k = 8 * s + 4 * g + u;
-            w2r[s][u] = (short)f2bf(W[OFF_W2 + i32 * NN_K + k]);
-            w3r[s][u] = (short)f2bf(W[OFF_W3 + i32 * NN_K + k]);
-        }
     if (t < 32) { S.b1[t] = W[OFF_B1 + t]; S.b2[t] = W[OFF_B2 + t]; S.b3[t] = W[OFF_B3 + t]; }
     if (t < 64) S.wf[t] = rbf(W[OFF_WF + t]);
     if (t < 2) S.bf[t] = W[OFF_BF + t];
     __syncthreads();
     NPH(0);
     for (int i = blockIdx.x; i < nmb; i += gridDim.x) {
+        bf4 wr[36];
         nn_features(q.cy, q.w, q.ry[0], q.ys, (i % mw) * 16, (i / mw) * 16, q.qp, S.feat, &S.red[7][0], &S.ired[7][0]);
         for (int k = t; k < NN_P * 2; k += blockDim.x) S.fb[k] = f2bf(S.feat[k]);
+        load_w(wr, W, OFF_W2);                         // in flight during conv1
         __syncthreads();
         NPH(1);
         // conv1 (K = 18 padded to 24: two taps x two channels per lane)
@@ -539,39 +551,39 @@ k_nn_infer(const fdvg_seq *__restrict__ seqs, const float *__restrict__ W, fdvg_
             }
         }
         __syncthreads();
-        for (int tile = wv; tile < NN_PT / 32; tile += nw) {
-            const f16v c = conv_tile_r(S.a1, w2r, tile);
+        NPH(6);
+        f16v c[3];
+        conv3tiles(S.a1, wr, wv, nw, nt, c);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            if (k >= nt) continue;
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
-                const int ii = (r >> 2) * 8 + g * 4 + (r & 3), pp = tile * 32 + ii;
-                const float v = pp < NN_P ? c[r] + S.b2[i32] : 0.f;
+                const int ii = (r >> 2) * 8 + g * 4 + (r & 3), pp = (wv + k * nw) * 32 + ii;
+                const float v = pp < NN_P ? c[k][r] + S.b2[i32] : 0.f;
                 S.a2[pp * NN_RS + i32] = f2bf(v > 0.f ? v : 0.f);
             }
         }
+        load_w(wr, W, OFF_W3);
         __syncthreads();
-        // conv3 reads a2 only: a1 is dead since the barrier above, so a3 (its alias) is free
-        for (int tile = wv; tile < NN_PT / 32; tile += nw) {
-            const f16v c = conv_tile_r(S.a2, w3r, tile);
+        NPH(7);
+        conv3tiles(S.a2, wr, wv, nw, nt, c);
+        // conv3 epilogue fused with the linear layer: per position, sums over the
+        // 32 channels held by the 32 lanes of a half-wave
+        const float wf0 = S.wf[i32], wf1 = S.wf[32 + i32], b3 = S.b3[i32];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            if (k >= nt) continue;
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
-                const int ii = (r >> 2) * 8 + g * 4 + (r & 3), pp = tile * 32 + ii;
-                const float v = pp < NN_P ? c[r] + S.b3[i32] : 0.f;
-                S.a3[pp * NN_C + ((i32 + pp) & 31)] = v > 0.f ? v : 0.f;   // channel rotated by position
+                const int ii = (r >> 2) * 8 + g * 4 + (r & 3), pp = (wv + k * nw) * 32 + ii;
+                float a = c[k][r] + b3;
+                a = (pp < NN_P && a > 0.f) ? a : 0.f;
+                float o0 = a * wf0, o1 = a * wf1;
+#pragma unroll
+                for (int o = 16; o > 0; o >>= 1) { o0 += __shfl_xor(o0, o, 32); o1 += __shfl_xor(o1, o, 32); }
+                if (i32 == 0) { S.lg[pp] = pp < NN_P ? o0 + S.bf[0] : 0.f; S.sk[pp] = o1; }
             }
-        }
-        __syncthreads();
-        // the linear layer at every position
-        for (int p = t; p < NN_PT; p += blockDim.x) {
-            float o0 = 0.f, o1 = 0.f;
-            if (p < NN_P) {
-                o0 = S.bf[0];
-                for (int c = 0; c < NN_C; ++c) {
-                    const float a = S.a3[p * NN_C + ((c + p) & 31)];
-                    o0 += a * S.wf[c];
-                    o1 += a * S.wf[32 + c];
-                }
-            }
-            S.lg[p] = o0; S.sk[p] = o1;
         }
         __syncthreads();
         float sacc = 0.f;
@@ -649,8 +661,16 @@ extern "C" size_t fdvg_nn_lds(void) { return sizeof(NnLds); }
 extern "C" int fdvg_nn_nparam(void) { return NN_NPARAM; }
 extern "C" int fdvgpu_nn_nparam(void) { return NN_NPARAM; }
 
+extern "C" int fdvg_nn_setup(void) {
+    return hipFuncSetAttribute((const void *)k_nn_infer, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)sizeof(NnInfer)) == hipSuccess &&
+           hipFuncSetAttribute((const void *)k_nn_train, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)sizeof(NnLds)) == hipSuccess ? 0 : -1;
+}
+
+// wpc: proposer workgroups per CU share (1 or 2)
 extern "C" int fdvg_launch_nn_infer(const fdvg_seq *seqs, int nseq, int maxmb, const float *W, fdvg_nnout *out,
-                                    hipStream_t st) {
+                                    int wpc, hipStream_t st) {
     static bool attr = false;
     if (!attr) {
         if (hipFuncSetAttribute((const void *)k_nn_infer, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -663,8 +683,8 @@ extern "C" int fdvg_launch_nn_infer(const fdvg_seq *seqs, int nseq, int maxmb, c
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return -1;
     }
-    // one workgroup per CU in all (the weights load once per workgroup)
-    int gx = ncu / nseq;
+    // wpc workgroups per CU in all (the weights load once per workgroup)
+    int gx = ncu * (wpc == 2 ? 2 : 1) / nseq;
     gx = gx < 1 ? 1 : gx > maxmb ? maxmb : gx;
     hipLaunchKernelGGL(k_nn_infer, dim3(gx, nseq), dim3(256), sizeof(NnInfer), st, seqs, W, out);
     return hipGetLastError() == hipSuccess ? 0 : -1;

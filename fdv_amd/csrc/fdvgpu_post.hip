@@ -191,14 +191,14 @@ __device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v) {
 // One wavefront per MB.  pass 0: per-MB byte counts into cnt[mb*16 + k];
 // pass 1: write stream k of sequence s at compact + bases[s*16 + k] + cnt[mb*16 + k]
 // (cnt holding the exclusive scan over the sequence's MBs by then).
-extern "C" __global__ void __launch_bounds__(256)
+extern "C" __global__ void __launch_bounds__(512)
 k_sym(const SymSeq *__restrict__ seqs, const fdvg_mbrec *__restrict__ recs,
       uint32_t *__restrict__ cnt, int pass, uint8_t *__restrict__ compact,
       const uint32_t *__restrict__ bases, uint32_t cap) {
     const SymSeq q = seqs[blockIdx.y];
     const int nmb = q.mw * q.mh;
     const int L = threadIdx.x & 63;
-    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int i = blockIdx.x * (int)(blockDim.x >> 6) + (threadIdx.x >> 6);
     if (i >= nmb) return;
     const int mb = q.mb0 + i;
     const fdvg_mbrec *R = &recs[mb];
@@ -299,12 +299,13 @@ extern "C" __global__ void k_bases(const uint32_t *__restrict__ totals, uint32_t
 
 // ------------------------------------------------------------ host launchers
 
-extern "C" int fdvg_launch_deblock(const void *planes, int nplanes, int maxpx, hipStream_t st) {
+extern "C" int fdvg_launch_deblock(const void *planes, int nplanes, int maxpx, int bs, hipStream_t st) {
     const PlaneRef *P = (const PlaneRef *)planes;
-    int gx = (maxpx / 8 + 255) / 256;
-    gx = gx < 1 ? 1 : gx > 1024 ? 1024 : gx;
-    hipLaunchKernelGGL(k_deblock, dim3(gx, nplanes), dim3(256), 0, st, P, 0);
-    hipLaunchKernelGGL(k_deblock, dim3(gx, nplanes), dim3(256), 0, st, P, 1);
+    bs = bs == 64 || bs == 128 ? bs : 256;
+    int gx = (maxpx / 8 + bs - 1) / bs;
+    gx = gx < 1 ? 1 : gx > 4096 ? 4096 : gx;
+    hipLaunchKernelGGL(k_deblock, dim3(gx, nplanes), dim3(bs), 0, st, P, 0);
+    hipLaunchKernelGGL(k_deblock, dim3(gx, nplanes), dim3(bs), 0, st, P, 1);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -338,13 +339,14 @@ extern "C" int fdvg_launch_zero(uint32_t *p, int n, hipStream_t st) {
 
 extern "C" int fdvg_launch_symbols(const void *symseqs, int nseq, int maxmb, const fdvg_mbrec *recs,
                                    uint32_t *cnt, uint32_t *totals, uint32_t *bases, uint8_t *compact,
-                                   size_t cap, hipStream_t st) {
+                                   size_t cap, int waves, hipStream_t st) {
     const uint32_t cap32 = cap > 0xffffffffu ? 0xffffffffu : (uint32_t)cap;
     const SymSeq *S = (const SymSeq *)symseqs;
-    int gx = (maxmb + 3) / 4;             // four MBs (wavefronts) per workgroup
-    hipLaunchKernelGGL(k_sym, dim3(gx, nseq), dim3(256), 0, st, S, recs, cnt, 0, compact, (const uint32_t *)bases, cap32);
+    waves = waves == 1 || waves == 2 || waves == 8 ? waves : 4;    // MBs (wavefronts) per workgroup
+    int gx = (maxmb + waves - 1) / waves;
+    hipLaunchKernelGGL(k_sym, dim3(gx, nseq), dim3(64 * waves), 0, st, S, recs, cnt, 0, compact, (const uint32_t *)bases, cap32);
     hipLaunchKernelGGL(k_scan, dim3(FDVG_NS, nseq), dim3(256), 0, st, S, cnt, totals);
     hipLaunchKernelGGL(k_bases, dim3(1), dim3(64), 0, st, (const uint32_t *)totals, bases, nseq * FDVG_NS);
-    hipLaunchKernelGGL(k_sym, dim3(gx, nseq), dim3(256), 0, st, S, recs, cnt, 1, compact, (const uint32_t *)bases, cap32);
+    hipLaunchKernelGGL(k_sym, dim3(gx, nseq), dim3(64 * waves), 0, st, S, recs, cnt, 1, compact, (const uint32_t *)bases, cap32);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }

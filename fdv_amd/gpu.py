@@ -45,6 +45,10 @@ def lib():
         L.fdvgpu_decisions.argtypes = [vp, C.c_int, vp]
         L.fdvgpu_timings.argtypes = [vp, C.POINTER(C.c_float)]
         L.fdvgpu_set_proposer.argtypes = [vp, vp, C.c_int]
+        L.fdvgpu_set_graphs.argtypes = [vp, C.c_int]
+        L.fdvgpu_get_tune.argtypes = [vp, vp]
+        L.fdvgpu_set_tune.argtypes = [vp, vp]
+        L.fdvgpu_autotune.argtypes = [vp, ip, ip, dp, ip, C.c_int, C.c_int]
         L.fdvgpu_proposals.argtypes = [vp, C.c_int, vp]
         L.fdvgpu_motion.argtypes = [vp, C.c_int, vp]
         L.fdvgpu_rewind.argtypes = [vp]
@@ -148,6 +152,26 @@ class Analyser:
         shipped with the package)."""
         w = load_proposer() if weights is None else np.ascontiguousarray(weights, dtype=np.float32)
         self._ck(self.L.fdvgpu_set_proposer(self.h, w.ctypes.data, int(w.size)), "set_proposer")
+
+    def set_graphs(self, on: bool):
+        self._ck(self.L.fdvgpu_set_graphs(self.h, 1 if on else 0), "set_graphs")
+
+    def tune(self, shapes=None):
+        """Get (or set) the launch shapes: [serialiser waves per workgroup,
+        deblock workgroup size, proposer workgroups per CU]."""
+        if shapes is not None:
+            self._ck(self.L.fdvgpu_set_tune(self.h, _ints(shapes)), "set_tune")
+        out = (C.c_int * 3)()
+        self._ck(self.L.fdvgpu_get_tune(self.h, out), "get_tune")
+        return list(out)
+
+    def autotune(self, src_slot, qp, lscale=None, navail=None, me_mode=ME_DIAMOND, reps=3):
+        n = self.nseq
+        ls = (C.c_double * n)(*([1.0] * n if lscale is None else lscale))
+        self._ck(self.L.fdvgpu_autotune(self.h, _ints(src_slot), _ints(qp), ls,
+                                        _ints([2] * n if navail is None else navail), int(me_mode), int(reps)),
+                 "autotune")
+        return self.tune()
 
     def proposals(self, seq):
         w, h = self.sizes[seq]

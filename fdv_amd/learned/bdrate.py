@@ -124,6 +124,7 @@ def main():
     ap.add_argument("--qps", default="20,24,28,32")
     ap.add_argument("--b", default="learned", choices=["learned", "parallel", "exhaustive", "diamond"])
     ap.add_argument("--a", default="exhaustive", choices=["learned", "parallel", "exhaustive", "diamond"])
+    ap.add_argument("--out", default=None, help="write the points and BD-rates as JSON")
     args = ap.parse_args()
     modes = {"learned": G.ME_LEARNED, "parallel": G.ME_PARALLEL, "exhaustive": G.ME_EXHAUSTIVE,
              "diamond": G.ME_DIAMOND}
@@ -131,7 +132,13 @@ def main():
               ("objects720", content.scene_objects(1280, 720, 3, nobj=12, smin=32, smax=160, vmax=12, grain=2.0))]
     res = evaluate(scenes, [int(q) for q in args.qps.split(",")], args.frames, modes[args.a], modes[args.b],
                    log=lambda m: print(m, flush=True))
-    print(json.dumps({k: v["bd_rate"] for k, v in res.items()}), flush=True)
+    summary = {k: v["bd_rate"] for k, v in res.items()}
+    print(json.dumps(summary), flush=True)
+    if args.out:
+        with open(args.out, "w") as f:
+            json.dump({"a": args.a, "b": args.b, "frames": args.frames, "qps": args.qps,
+                       "bd_rate_percent": summary,
+                       "points": {k: {"a": v["a"], "b": v["b"]} for k, v in res.items()}}, f, indent=1)
 
 
 if __name__ == "__main__":

@@ -101,6 +101,19 @@ int fdvgpu_recon(fdvgpu *g, int seq, uint8_t *i420);
 /* The per-MB decisions of the last frame (nmb records, raster order). */
 int fdvgpu_decisions(fdvgpu *g, int seq, fdvgpu_mbinfo *out);
 
+/* ---- launch machinery.  Each frame step's device work after its descriptor
+ * copies is a fixed launch sequence per (search mode, decision structure); from
+ * its second use it is captured into a hipGraph and replayed (on by default).
+ * Launch shapes (serialiser wavefronts per workgroup, deblock workgroup size,
+ * proposer workgroups per CU) are chosen by fdvgpu_autotune, which times each
+ * candidate on a real frame step (rewinding the reference pool after each) and
+ * keeps the fastest.  None of this has a counterpart in the reference. */
+int fdvgpu_set_graphs(fdvgpu *g, int on);
+int fdvgpu_get_tune(fdvgpu *g, int shapes[3]);
+int fdvgpu_set_tune(fdvgpu *g, const int shapes[3]);
+int fdvgpu_autotune(fdvgpu *g, const int *src_slot, const int *qp, const double *lscale,
+                    const int *navail, int me_mode, int reps);
+
 /* ---- the learned motion proposer (FDVGPU_ME_LEARNED).  It has no counterpart in
  * the reference: it chooses which whole-pel vectors the block-parallel search
  * evaluates, and which MBs go straight to SKIP; the codec arithmetic after it

@@ -1,0 +1,72 @@
+"""Frame steps replayed from captured hipGraphs, and autotuned launch shapes,
+give exactly the results of eager launches: every replay (with eager work and
+synchronisation between replays) against a context that never captures."""
+import numpy as np
+import pytest
+
+from fdv_amd import content
+from fdv_amd import gpu as G
+from gpucases import say
+
+pytestmark = pytest.mark.gpu
+
+
+def _scenes():
+    return [content.scene_objects(96, 64, 61, nobj=3, smin=8, smax=32, vmax=6),
+            content.scene_pan(128, 48, 62, nobj=2, smin=8, smax=24, pan=(3, -1), grain=2.0)]
+
+
+@pytest.mark.parametrize("mode", [G.ME_DIAMOND, G.ME_PARALLEL, G.ME_LEARNED])
+def test_graph_replays_match_eager(mode):
+    scenes = _scenes()
+    sizes = [(s.w, s.h) for s in scenes]
+    nfr = 6
+    frames = [[s.frame(i) for i in range(nfr)] for s in scenes]
+    ctx = []
+    for graphs in (False, True):
+        an = G.Analyser(sizes, nsrc=nfr)
+        an.set_graphs(graphs)
+        if mode == G.ME_LEARNED:
+            an.set_proposer()
+        for k in range(2):
+            an.put_ref(k, 0, frames[k][0])
+            for f in range(1, nfr):
+                an.put_source(k, f, frames[k][f])
+        ctx.append(an)
+    for f in range(1, nfr):
+        say(f"  graph replay mode {mode} frame {f}")
+        outs = []
+        for an in ctx:
+            an.pframe([f, f], [24, 30], navail=[1 if f == 1 else 2] * 2, me_mode=mode)
+            an.fetch()
+            outs.append([(an.streams(k), an.recon(k)) for k in range(2)])
+            np.sort(np.random.default_rng(f).random(10000))      # eager host work between replays
+        for k in range(2):
+            assert outs[0][k][0] == outs[1][k][0], f"frame {f} seq {k}: streams"
+            assert np.array_equal(outs[0][k][1], outs[1][k][1]), f"frame {f} seq {k}: recon"
+    for an in ctx:
+        an.close()
+
+
+def test_autotune_changes_no_result():
+    scenes = _scenes()
+    sizes = [(s.w, s.h) for s in scenes]
+    frames = [[s.frame(i) for i in range(4)] for s in scenes]
+    a, b = G.Analyser(sizes, nsrc=4), G.Analyser(sizes, nsrc=4)
+    for an in (a, b):
+        an.set_proposer()
+        for k in range(2):
+            an.put_ref(k, 0, frames[k][0])
+            for f in range(1, 4):
+                an.put_source(k, f, frames[k][f])
+    say("  autotune")
+    shapes = b.autotune([1, 1], [24, 30], navail=[1, 1], me_mode=G.ME_LEARNED, reps=1)
+    assert shapes[0] in (2, 4, 8) and shapes[1] in (64, 128, 256) and shapes[2] in (1, 2)
+    for f in range(1, 4):
+        for an in (a, b):
+            an.pframe([f, f], [24, 30], navail=[1 if f == 1 else 2] * 2, me_mode=G.ME_LEARNED)
+            an.fetch()
+        for k in range(2):
+            assert a.streams(k) == b.streams(k) and np.array_equal(a.recon(k), b.recon(k)), f"frame {f} seq {k}"
+    a.close()
+    b.close()
