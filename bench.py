@@ -120,12 +120,72 @@ def profiled_traffic(mode: str):
     return None, None
 
 
+def _job_720p(content, nframes):
+    """The headline workload: one 1280x720 stream (pan over a blurred-noise
+    texture, 12 textured rectangles, grain sigma 3), QP 24, as 8 bands of 6 MB
+    rows coded in one batch."""
+    W, H = 1280, 720
+    layout = bands(H, 6)
+    scene = content.scene_pan(W, H, seed=2)
+    frames = [scene.frame(i) for i in range(nframes)]
+    per_band = [band_frames(f, W, H, layout) for f in frames]
+    return {"sizes": [(W, bh) for _, bh in layout], "seq_frames": [[pb[b] for pb in per_band] for b in range(len(layout))],
+            "qps": [24] * len(layout), "frames": frames,
+            "desc": "1280x720 4:2:0 P-frames, QP 24, 8 tiles (6-MB-row bands)",
+            "data": "synthetic (seeded pan + objects + grain)",
+            "extra": {"frame": [W, H], "bands": len(layout), "qp": 24}}
+
+
+def _job_batched(content, nframes):
+    """16 independent streams in one batch: 8 x 640x368, 4 x 1280x720, 4 x
+    1920x1088; moving textured rectangles over blurred noise with per-stream
+    seeds; QP drawn from {20, 24, 28, 32} per stream (seeded)."""
+    rng = np.random.default_rng(2024)
+    sizes = [(640, 368)] * 8 + [(1280, 720)] * 4 + [(1920, 1088)] * 4
+    qps = [int(q) for q in rng.choice([20, 24, 28, 32], size=len(sizes))]
+    seq_frames = []
+    for k, (w, h) in enumerate(sizes):
+        sc = content.scene_objects(w, h, 300 + k, nobj=4 + w // 320, smin=24, smax=96, vmax=8, grain=1.0)
+        seq_frames.append([sc.frame(i) for i in range(nframes)])
+    return {"sizes": sizes, "seq_frames": seq_frames, "qps": qps, "frames": None,
+            "desc": "16 streams (8 x 640x368, 4 x 1280x720, 4 x 1920x1088) in one batch, QP per stream "
+                    "from {20,24,28,32}",
+            "data": "synthetic (seeded moving rectangles over blurred noise, grain sigma 1)",
+            "extra": {"qps": qps}}
+
+
+def _job_stress(w, h):
+    def make(content, nframes):
+        """500 blobs of 4-12 px at velocities up to 48 px/frame, a new background
+        every 40 frames, grain sigma 5; QP 24; 6-MB-row bands in one batch."""
+        layout = bands(h, 6)
+        sc = content.scene_blobs(w, h, 5)
+        frames = [sc.frame(i) for i in range(nframes)]
+        per_band = [band_frames(f, w, h, layout) for f in frames]
+        return {"sizes": [(w, bh) for _, bh in layout],
+                "seq_frames": [[pb[b] for pb in per_band] for b in range(len(layout))],
+                "qps": [24] * len(layout), "frames": None,
+                "desc": "%dx%d 4:2:0 P-frames, QP 24, %d tiles (6-MB-row bands), 500 blobs, scene cut every "
+                        "40 frames" % (w, h, len(layout)),
+                "data": "synthetic (seeded blobs, grain sigma 5)",
+                "extra": {"frame": [w, h], "bands": len(layout), "qp": 24}}
+    return make
+
+
+WORKLOADS = {"720p": _job_720p, "batched": _job_batched, "stress1080": _job_stress(1920, 1088),
+             "stress4k": _job_stress(3840, 2160)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--mode", default="learned", choices=["diamond", "exhaustive", "learned", "parallel"])
+    ap.add_argument("--workload", default="720p", choices=list(WORKLOADS),
+                    help="720p: the headline workload (8 bands of one 1280x720 stream); batched: 16 streams "
+                         "of mixed sizes; stress1080 / stress4k: 500 small fast blobs with scene cuts, "
+                         "as 6-MB-row bands")
     ap.add_argument("--baseline-steps", type=int, default=10,
                     help="steps of the exhaustive search timed for comparison (0: skip)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -152,28 +212,27 @@ def main():
     from fdv_amd import content
     from fdv_amd import gpu as G
 
-    W, H, QP, BAND_MBROWS = 1280, 720, 24, 6
-    layout = bands(H, BAND_MBROWS)
     nsteps = args.warmup + args.steps
     nframes = min(nsteps + 1, 60)
-    scene = content.scene_pan(W, H, seed=2)
-    frames = [scene.frame(i) for i in range(nframes)]
+    job = WORKLOADS[args.workload](content, nframes)
+    W, H, QP, layout = 1280, 720, 24, bands(720, 6)
+    frames = job.get("frames")
     mode = {"diamond": G.ME_DIAMOND, "exhaustive": G.ME_EXHAUSTIVE, "learned": G.ME_LEARNED,
             "parallel": G.ME_PARALLEL}[args.mode]
 
-    per_band = [band_frames(f, W, H, layout) for f in frames]
-    nb = len(layout)
+    sizes, per_seq, qps = job["sizes"], job["seq_frames"], job["qps"]
+    nb = len(sizes)
 
     def open_analyser(m):
-        a = G.Analyser([(W, bh) for _, bh in layout], nsrc=nframes)
+        a = G.Analyser(sizes, nsrc=nframes)
         if m == G.ME_LEARNED:
             a.set_proposer()
         for b in range(nb):
-            a.put_ref(b, 0, per_band[0][b])
+            a.put_ref(b, 0, per_seq[b][0])
             for i in range(1, nframes):
-                a.put_source(b, i, per_band[i][b])
+                a.put_source(b, i, per_seq[b][i])
         # launch shapes for this batch, timed on its first frame (the reference pool is restored)
-        a.autotune([1] * nb, [QP] * nb, navail=[1] * nb, me_mode=m, reps=2)
+        a.autotune([1] * nb, qps, navail=[1] * nb, me_mode=m, reps=2)
         return a
 
     an = open_analyser(mode)
@@ -182,7 +241,7 @@ def main():
         a = an if a is None else a
         src = 1 + (i % (nframes - 1))
         navail = 1 if i == 0 else 2
-        a.pframe([src] * nb, [QP] * nb, [1.0] * nb, [navail] * nb, [0] * nb, mode if m is None else m)
+        a.pframe([src] * nb, qps, [1.0] * nb, [navail] * nb, [0] * nb, mode if m is None else m)
         a.fetch()
         return a.timings()
 
@@ -195,7 +254,7 @@ def main():
     dt = time.perf_counter() - t0
     kt = np.array(kt)
 
-    px_frame = W * H
+    px_frame = sum(w * h for w, h in sizes)
     value = args.steps * px_frame / dt / 1e6
     k_ms = float(kt[:, 1].mean())                  # the MB wavefront kernel
     bytes_per_px = 6.0                              # 1.5 B x (source + 2 references + recon)
@@ -223,19 +282,19 @@ def main():
         ab.close()
     bd = None
     bd_path = os.path.join(ROOT, "fdv_amd", "learned", "bdrate_720p.json")
-    if mode == G.ME_LEARNED and os.path.exists(bd_path):
+    if mode == G.ME_LEARNED and args.workload == "720p" and os.path.exists(bd_path):
         with open(bd_path) as f:
             bd = json.load(f)["bd_rate_percent"]
-    cpu = None if args.no_cpu else cpu_baseline(frames, W, layout, QP, args.cpu_seconds)
+    # the CPU leg runs on the headline workload only
+    cpu = None if (args.no_cpu or args.workload != "720p") else cpu_baseline(frames, W, layout, QP, args.cpu_seconds)
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "Mpix/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt / args.steps * 1e3, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u8",
-        "data": "synthetic (seeded pan + objects + grain)",
-        "config": {"workload": "1280x720 4:2:0 P-frames, QP 24, 8 tiles (6-MB-row bands), "
-                               "%s motion search%s" % (args.mode, ", block-parallel decisions"
-                                                        if mode in (G.ME_LEARNED, G.ME_PARALLEL) else ""),
-                   "frame": [W, H], "bands": len(layout), "qp": QP, "search": args.mode},
+        "data": job["data"],
+        "config": {"workload": job["desc"] + ", %s motion search%s" % (
+                       args.mode, ", block-parallel decisions" if mode in (G.ME_LEARNED, G.ME_PARALLEL) else ""),
+                   "sequences": nb, "coded_px_per_step": px_frame, "search": args.mode, **job["extra"]},
         "stage_ms": stage_ms,
         "launch_shapes": {"serialiser_waves": shapes[0], "deblock_threads": shapes[1], "proposer_wg_per_cu": shapes[2]},
         "exhaustive_mpix_s": None if base is None else round(base, 3),

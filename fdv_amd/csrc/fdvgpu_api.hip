@@ -312,7 +312,7 @@ extern "C" int fdvgpu_set_given(fdvgpu *g, int seq, const fdvgpu_mbinfo *dec) {
 static int launch_stage(fdvgpu *g, int stage, int me_mode, bool par) {
     if (stage == 0) {
         if (me_mode == FDVGPU_ME_LEARNED &&
-            fdvg_launch_nn_infer(g->seq_d, g->nseq, g->maxmb, g->nnw_d, g->nn_d, g->tune.nn_wpc, g->st) != 0) {
+            fdvg_launch_nn_infer(g->seq_d, g->nseq, g->total_mb, g->nnw_d, g->nn_d, g->tune.nn_wpc, g->st) != 0) {
             seterr("proposer launch failed"); return -1;
         }
         if (me_mode != FDVGPU_ME_DIAMOND &&

@@ -441,58 +441,117 @@ extern "C" int fdvg_nnphase_read(unsigned long long *out) {
 #define NPH(k) do { } while (0)
 #endif
 
-// The inference path: activations in LDS rows of NN_RS bf16 (72 bytes, so
-// consecutive positions fall in different banks); each wavefront loads one
-// layer's B operands (weights, bf16) into registers at a time and runs up to
-// three 32-position tiles side by side (independent MFMA chains); the linear
-// layer is fused into the conv3 epilogue as cross-lane sums of the fp32
-// accumulators.  LDS is about 50 KB.
-#define NN_RS 36
-struct __align__(16) NnInfer {
-    float feat[NN_P * 2];
-    uint16_t fb[NN_P * 2];
-    uint16_t a1[NN_PT * NN_RS], a2[NN_PT * NN_RS];
+// The inference path: one wavefront per MB, four per workgroup, no
+// workgroup barriers after the weights are staged.
+//  - conv2 and conv3 run on v_mfma_f32_32x32x16_bf16 (K = 16: half a tap's 32
+//    input channels per step, 18 steps).  All ten 32-position tiles accumulate
+//    at once (160 accumulator registers), so every B operand is read once per
+//    step and the wave's single activation buffer is rewritten in place after
+//    the last step.
+//  - B operands: both layers' weights in bf16, stored in LDS in lane order
+//    (step, lane) -> 16 bytes, shared by the workgroup's four waves.
+//  - A operands: rows of NI_RS bf16 (80 bytes: eight consecutive rows cover
+//    the 32 banks once for 16-byte reads); row NN_PT is zero (the padding of
+//    the 3x3 taps).
+//  - The linear layer is fused into the conv3 epilogue: per tile a
+//    reduce-scatter over the 32 lanes of each half-wave leaves every position's
+//    logit on two lanes (fp32 throughout).
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef unsigned int u4v __attribute__((ext_vector_type(4)));
+#define NI_RS 40
+#define NI_WAVES 4
+struct __align__(16) NiWave {
+    uint16_t act[(NN_PT + 1) * NI_RS];     // [pos][channel], row NN_PT zero; also the features' scratch
+    uint32_t fb[NN_PT + 1];                // the two bf16 features of each position; entry NN_PT zero
+    int pad_[3];
+};
+struct __align__(16) NiShared {
+    u4v wl[2][18][64];                     // conv2 / conv3 B operands, bf16 x 8 per (step, lane)
     float b1[32], b2[32], b3[32], wf[64], bf[2];
-    float lg[NN_PT], sk[NN_PT];
-    float cv[32];
-    int cp[32];
-    float red[8][8];
-    int ired[8][8];
+    NiWave wave[NI_WAVES];
 };
 
-// B operands of one 32-channel layer for this lane: B[k][co = lane % 32],
-// k = 8 s + 4 (lane / 32) + 0..3.
-__device__ __forceinline__ void load_w(bf4 *wr, const float *W, int off) {
-    const int l = threadIdx.x & 63, i32 = l & 31, g = l >> 5;
+__device__ __forceinline__ u4v ni_pack8(const float *f) {
+    u4v r;
 #pragma unroll
-    for (int s = 0; s < 36; ++s) {
-        const float4 f = *(const float4 *)&W[off + i32 * NN_K + 8 * s + 4 * g];
-        wr[s][0] = (short)f2bf(f.x); wr[s][1] = (short)f2bf(f.y);
-        wr[s][2] = (short)f2bf(f.z); wr[s][3] = (short)f2bf(f.w);
-    }
+    for (int j = 0; j < 4; ++j) r[j] = (uint32_t)f2bf(f[2 * j]) | ((uint32_t)f2bf(f[2 * j + 1]) << 16);
+    return r;
 }
 
-// Up to three output tiles (t0, t0 + nw, t0 + 2 nw) of a 32-channel 3x3 layer.
-__device__ __forceinline__ void conv3tiles(const uint16_t *in, const bf4 *wr, int t0, int nw, int nt, f16v *c) {
-    const int l = threadIdx.x & 63, i = l & 31, g = l >> 5;
+// The correlation map of one MB into w.fb (bf16 pairs), by one wavefront.
+// Integer SADs and the fp32 feature formulas are those of nn_features.
+__device__ __forceinline__ void ni_features(NiWave &w, const uint8_t *cur, int cw, const uint8_t *ref, int rs,
+                                            int bx, int by, int qp) {
+    const int l = threadIdx.x & 63;
+    uint8_t *c8 = (uint8_t *)w.act;
+    uint8_t *r32 = c8 + 64;
+    int *cost = (int *)(c8 + 1088);
+    {
+        const int y = l >> 3, x = l & 7;
+        const uint8_t *p = cur + (size_t)(by + 2 * y) * cw + bx + 2 * x;
+        c8[l] = (uint8_t)((p[0] + p[1] + p[cw] + p[cw + 1] + 2) >> 2);
+    }
+    {
+        // lane: r32 row l/2, columns 16 (l&1) .. +15, from two reference rows of 32 bytes (8-byte aligned)
+        const int y = l >> 1, h = l & 1;
+        const uint8_t *p0 = ref + (ptrdiff_t)(by - 24 + 2 * y) * rs + bx - 24 + 32 * h;
+        const uint8_t *p1 = p0 + rs;
+        uint32_t o[4];
 #pragma unroll
-    for (int k = 0; k < 3; ++k)
+        for (int k = 0; k < 4; ++k) {
+            const uint2 a = *(const uint2 *)(p0 + 8 * k), b = *(const uint2 *)(p1 + 8 * k);
+            uint32_t v = 0;
 #pragma unroll
-        for (int r = 0; r < 16; ++r) c[k][r] = 0.f;
-#pragma unroll
-    for (int tap = 0; tap < 9; ++tap) {
-        int q[3];
-#pragma unroll
-        for (int k = 0; k < 3; ++k) q[k] = k < nt ? nn_shift((t0 + k * nw) * 32 + i, tap) : -1;
-#pragma unroll
-        for (int c0 = 0; c0 < 4; ++c0) {
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                if (k >= nt) continue;
-                bf4 a = {0, 0, 0, 0};
-                if (q[k] >= 0) a = *(const bf4 *)&in[q[k] * NN_RS + c0 * 8 + 4 * g];
-                c[k] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, wr[tap * 4 + c0], c[k], 0, 0, 0);
+            for (int e = 0; e < 4; ++e) {
+                const uint32_t wa = e < 2 ? a.x : a.y, wb = e < 2 ? b.x : b.y;
+                const int sh = (e & 1) * 16;
+                const uint32_t s4 = ((wa >> sh) & 255) + ((wa >> (sh + 8)) & 255) + ((wb >> sh) & 255) +
+                                    ((wb >> (sh + 8)) & 255) + 2;
+                v |= ((s4 >> 2) & 255) << (8 * e);
             }
+            o[k] = v;
+        }
+        *(uint4 *)(r32 + y * 32 + 16 * h) = make_uint4(o[0], o[1], o[2], o[3]);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    uint32_t cr[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) cr[k] = *(const uint32_t *)(c8 + 4 * k);
+    int mn = 1 << 30;
+    int cv[5];
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+        const int p = l + 64 * m;
+        int sad = 1 << 30;
+        if (p < NN_P) {
+            const int dy = p / NN_G - 8, dx = p % NN_G - 8, c0 = 12 + dx, sh = c0 & 3;
+            const uint8_t *row = r32 + (12 + dy) * 32 + (c0 & ~3);
+            uint32_t acc = 0;
+#pragma unroll
+            for (int y = 0; y < 8; ++y) {
+                const uint32_t d0 = *(const uint32_t *)(row + 32 * y), d1 = *(const uint32_t *)(row + 32 * y + 4),
+                               d2 = *(const uint32_t *)(row + 32 * y + 8);
+                acc = __builtin_amdgcn_sad_u8(__builtin_amdgcn_alignbyte(d1, d0, sh), cr[2 * y], acc);
+                acc = __builtin_amdgcn_sad_u8(__builtin_amdgcn_alignbyte(d2, d1, sh), cr[2 * y + 1], acc);
+            }
+            sad = (int)acc;
+            mn = min(mn, sad);
+        }
+        cv[m] = sad;
+    }
+    for (int o = 32; o > 0; o >>= 1) mn = min(mn, __shfl_xor(mn, o));
+    (void)cost;
+    const float s0 = 1.0f / (16.0f * exp2f((float)(qp - 4) * (1.0f / 6.0f)));
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+        const int p = l + 64 * m;
+        if (p < NN_PT) {
+            uint32_t v = 0;
+            if (p < NN_P)
+                v = (uint32_t)f2bf((float)cv[m] * s0) | ((uint32_t)f2bf((float)(cv[m] - mn) * (1.0f / 256.0f)) << 16);
+            w.fb[p] = v;
         }
     }
 }
@@ -500,160 +559,201 @@ __device__ __forceinline__ void conv3tiles(const uint16_t *in, const bf4 *wr, in
 // Per MB of every sequence: the 8 most likely whole-pel vectors (nearest
 // reference), their total probability and the SKIP probability.
 extern "C" __global__ void __launch_bounds__(256, 1)
-k_nn_infer(const fdvg_seq *__restrict__ seqs, const float *__restrict__ W, fdvg_nnout *__restrict__ out) {
+k_nn_infer(const fdvg_seq *__restrict__ seqs, int nseq, int total, const float *__restrict__ W,
+           fdvg_nnout *__restrict__ out) {
     extern __shared__ uint8_t dyn_nn[];
-    NnInfer &S = *(NnInfer *)dyn_nn;
-    const fdvg_seq q = seqs[blockIdx.y];
-    const int mw = q.w >> 4, nmb = mw * (q.h >> 4);
-    const int t = threadIdx.x, wv = t >> 6, l = t & 63, i32 = l & 31, g = l >> 5, nw = blockDim.x >> 6;
-    const int nt = (NN_PT / 32 - wv + nw - 1) / nw;      // this wave's tiles: wv, wv + nw, ...
+    NiShared &S = *(NiShared *)dyn_nn;
+    const int t = threadIdx.x, wv = t >> 6, l = t & 63, i32 = l & 31, g = l >> 5;
+    NiWave &Wv = S.wave[wv];
 #ifdef FDVG_PHASE_PROF
     long long nph_last = clock64();
 #endif
-    bf4 w1r[3];
-#pragma unroll
-    for (int s = 0; s < 3; ++s)
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int k = 8 * s + 4 * g + u;
-            w1r[s][u] = k < 18 ? (short)f2bf(W[OFF_W1 + i32 * 18 + k]) : (short)0;
-        }
+    // stage the weights: B[k][co] for k = 16 s + 8 (lane / 32) + j, co = lane % 32
+    for (int e = t; e < 2 * 18 * 64; e += blockDim.x) {
+        const int layer = e / (18 * 64), st = (e / 64) % 18, ln = e % 64;
+        const float *src = W + (layer ? OFF_W3 : OFF_W2) + (ln & 31) * NN_K + 16 * st + 8 * (ln >> 5);
+        float f[8];
+        const float4 x0 = *(const float4 *)src, x1 = *(const float4 *)(src + 4);
+        f[0] = x0.x; f[1] = x0.y; f[2] = x0.z; f[3] = x0.w; f[4] = x1.x; f[5] = x1.y; f[6] = x1.z; f[7] = x1.w;
+        S.wl[layer][st][ln] = ni_pack8(f);
+    }
     if (t < 32) { S.b1[t] = W[OFF_B1 + t]; S.b2[t] = W[OFF_B2 + t]; S.b3[t] = W[OFF_B3 + t]; }
     if (t < 64) S.wf[t] = rbf(W[OFF_WF + t]);
     if (t < 2) S.bf[t] = W[OFF_BF + t];
+    // this wave's zero rows
+    if (l < NI_RS / 2) ((uint32_t *)&Wv.act[NN_PT * NI_RS])[l] = 0;
+    if (l == 0) Wv.fb[NN_PT] = 0;
+    // conv1's B operands in registers: k = 16 s + 8 g + j (k = tap*2 + ci, zero beyond 18)
+    u4v w1r[2];
+#pragma unroll
+    for (int s2 = 0; s2 < 2; ++s2) {
+        float f[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int k = 16 * s2 + 8 * g + j;
+            f[j] = k < NN_K1 ? W[OFF_W1 + i32 * NN_K1 + k] : 0.f;
+        }
+        w1r[s2] = ni_pack8(f);
+    }
+    // per tile: this lane's position and which of the 9 taps stay inside the map
+    int tp[10], tmask[10];
+#pragma unroll
+    for (int tl = 0; tl < 10; ++tl) {
+        const int p = tl * 32 + i32;
+        int m = 0;
+#pragma unroll
+        for (int tap = 0; tap < 9; ++tap) m |= (nn_shift(p, tap) >= 0 ? 1 : 0) << tap;
+        tp[tl] = p;
+        tmask[tl] = m;
+    }
     __syncthreads();
     NPH(0);
-    for (int i = blockIdx.x; i < nmb; i += gridDim.x) {
-        bf4 wr[36];
-        nn_features(q.cy, q.w, q.ry[0], q.ys, (i % mw) * 16, (i / mw) * 16, q.qp, S.feat, &S.red[7][0], &S.ired[7][0]);
-        for (int k = t; k < NN_P * 2; k += blockDim.x) S.fb[k] = f2bf(S.feat[k]);
-        load_w(wr, W, OFF_W2);                         // in flight during conv1
-        __syncthreads();
+    const int zrow = NN_PT * NI_RS * 2;    // byte offset of the zero row
+    const uint8_t *actb = (const uint8_t *)Wv.act;
+    for (int gi = blockIdx.x * NI_WAVES + wv; gi < total; gi += gridDim.x * NI_WAVES) {
+        int k = 0;
+        while (k + 1 < nseq && seqs[k + 1].mb0 <= gi) ++k;
+        const fdvg_seq &q = seqs[k];
+        const int i = gi - q.mb0, mw = q.w >> 4;
+        ni_features(Wv, q.cy, q.w, q.ry[0], q.ys, (i % mw) * 16, (i / mw) * 16, q.qp);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         NPH(1);
-        // conv1 (K = 18 padded to 24: two taps x two channels per lane)
-        for (int tile = wv; tile < NN_PT / 32; tile += nw) {
-            const int p = tile * 32 + i32;
+        // conv1: two K steps (taps 0-3 | 4-7 by lane half, then tap 8)
+#pragma unroll
+        for (int tl = 0; tl < 10; ++tl) {
             f16v c = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+            u4v a0, a1 = {0u, 0u, 0u, 0u};
 #pragma unroll
-            for (int s = 0; s < 3; ++s) {
-                const int ta = (8 * s + 4 * g) >> 1;
-                const int qa = ta < 9 ? nn_shift(p, ta) : -1, qb = ta + 1 < 9 ? nn_shift(p, ta + 1) : -1;
-                bf4 a = {0, 0, 0, 0};
-                if (qa >= 0) { a[0] = (short)S.fb[2 * qa]; a[1] = (short)S.fb[2 * qa + 1]; }
-                if (qb >= 0) { a[2] = (short)S.fb[2 * qb]; a[3] = (short)S.fb[2 * qb + 1]; }
-                c = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, w1r[s], c, 0, 0, 0);
+            for (int j = 0; j < 4; ++j) {
+                const int tap = 4 * g + j;
+                const int qq = (tmask[tl] >> tap) & 1 ? tp[tl] + (tap / 3 - 1) * NN_G + tap % 3 - 1 : NN_PT;
+                a0[j] = Wv.fb[qq];
             }
+            if (g == 0) a1[0] = (tmask[tl] >> 8) & 1 ? Wv.fb[tp[tl] + NN_G + 1] : 0u;
+            c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf8, a0), __builtin_bit_cast(bf8, w1r[0]), c, 0, 0, 0);
+            c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf8, a1), __builtin_bit_cast(bf8, w1r[1]), c, 0, 0, 0);
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
-                const int ii = (r >> 2) * 8 + g * 4 + (r & 3), pp = tile * 32 + ii;
+                const int pp = tl * 32 + (r >> 2) * 8 + g * 4 + (r & 3);
                 const float v = pp < NN_P ? c[r] + S.b1[i32] : 0.f;
-                S.a1[pp * NN_RS + i32] = f2bf(v > 0.f ? v : 0.f);
+                Wv.act[pp * NI_RS + i32] = f2bf(v > 0.f ? v : 0.f);
             }
         }
-        __syncthreads();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         NPH(6);
-        f16v c[3];
-        conv3tiles(S.a1, wr, wv, nw, nt, c);
+        f16v acc[10];
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            if (k >= nt) continue;
+        for (int layer = 0; layer < 2; ++layer) {
 #pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int ii = (r >> 2) * 8 + g * 4 + (r & 3), pp = (wv + k * nw) * 32 + ii;
-                const float v = pp < NN_P ? c[k][r] + S.b2[i32] : 0.f;
-                S.a2[pp * NN_RS + i32] = f2bf(v > 0.f ? v : 0.f);
+            for (int tl = 0; tl < 10; ++tl)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) acc[tl][r] = 0.f;
+#pragma unroll
+            for (int st = 0; st < 18; ++st) {
+                const int tap = st >> 1, coff = (16 * (st & 1) + 8 * g) * 2;
+                const int sh = (tap / 3 - 1) * NN_G + tap % 3 - 1;
+                const bf8 b = __builtin_bit_cast(bf8, S.wl[layer][st][l]);
+#pragma unroll
+                for (int tl = 0; tl < 10; ++tl) {
+                    const int off = (tmask[tl] >> tap) & 1 ? (tp[tl] + sh) * (NI_RS * 2) : zrow;
+                    const bf8 a = __builtin_bit_cast(bf8, *(const u4v *)(actb + off + coff));
+                    acc[tl] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[tl], 0, 0, 0);
+                }
+            }
+            if (layer == 0) {
+                // every read of the buffer has completed (its values fed the MFMAs): rewrite it with conv2
+#pragma unroll
+                for (int tl = 0; tl < 10; ++tl)
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) {
+                        const int pp = tl * 32 + (r >> 2) * 8 + g * 4 + (r & 3);
+                        const float v = pp < NN_P ? acc[tl][r] + S.b2[i32] : 0.f;
+                        Wv.act[pp * NI_RS + i32] = f2bf(v > 0.f ? v : 0.f);
+                    }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                NPH(7);
             }
         }
-        load_w(wr, W, OFF_W3);
-        __syncthreads();
-        NPH(7);
-        conv3tiles(S.a2, wr, wv, nw, nt, c);
-        // conv3 epilogue fused with the linear layer: per position, sums over the
-        // 32 channels held by the 32 lanes of a half-wave
+        // conv3 epilogue + linear layer.  Reduce-scatter of the 16 per-lane values
+        // of a tile over the half-wave: after the steps xor 16, 8, 4, 2 (each
+        // halving the values a lane keeps) and xor 1, lanes i and i^1 hold the
+        // sum for r = (i >> 1) & 15.
         const float wf0 = S.wf[i32], wf1 = S.wf[32 + i32], b3 = S.b3[i32];
+        float skacc = 0.f;
+        float lg[10];
+        const int rr = (i32 >> 1) & 15;
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            if (k >= nt) continue;
+        for (int tl = 0; tl < 10; ++tl) {
+            float v[16];
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
-                const int ii = (r >> 2) * 8 + g * 4 + (r & 3), pp = (wv + k * nw) * 32 + ii;
-                float a = c[k][r] + b3;
+                const int pp = tl * 32 + (r >> 2) * 8 + g * 4 + (r & 3);
+                float a = acc[tl][r] + b3;
                 a = (pp < NN_P && a > 0.f) ? a : 0.f;
-                float o0 = a * wf0, o1 = a * wf1;
+                v[r] = a * wf0;
+                skacc += a * wf1;
+            }
 #pragma unroll
-                for (int o = 16; o > 0; o >>= 1) { o0 += __shfl_xor(o0, o, 32); o1 += __shfl_xor(o1, o, 32); }
-                if (i32 == 0) { S.lg[pp] = pp < NN_P ? o0 + S.bf[0] : 0.f; S.sk[pp] = o1; }
+            for (int lev = 0; lev < 4; ++lev) {
+                const int half = 8 >> lev, bit = (i32 >> (4 - lev)) & 1;
+#pragma unroll
+                for (int j = 0; j < half; ++j) {
+                    const float keep = bit ? v[j + half] : v[j], send = bit ? v[j] : v[j + half];
+                    v[j] = keep + __shfl_xor(send, 16 >> lev, 32);
+                }
             }
+            const float sum = v[0] + __shfl_xor(v[0], 1, 32);
+            const int pp = tl * 32 + (rr >> 2) * 8 + g * 4 + (rr & 3);
+            lg[tl] = pp < NN_P ? sum + S.bf[0] : -3.0e38f;
         }
-        __syncthreads();
-        float sacc = 0.f;
-        for (int p = t; p < NN_P; p += blockDim.x) sacc += S.sk[p];
-        for (int o = 32; o > 0; o >>= 1) sacc += __shfl_xor(sacc, o);
-        if (l == 0) S.red[0][wv] = sacc;
-        __syncthreads();
-        float skl = 0.f;
-        for (int w2 = 0; w2 < nw; ++w2) skl += S.red[0][w2];
-        skl = skl * (1.0f / NN_P) + S.bf[1];
+        for (int o = 32; o > 0; o >>= 1) skacc += __shfl_xor(skacc, o);
+        const float skl = skacc * (1.0f / NN_P) + S.bf[1];
         NPH(2);
-        // softmax statistics
+        // softmax statistics (each position counted once: even lanes)
         float m = -3.0e38f;
-        for (int p = t; p < NN_P; p += blockDim.x) m = fmaxf(m, S.lg[p]);
+#pragma unroll
+        for (int tl = 0; tl < 10; ++tl) m = fmaxf(m, lg[tl]);
         for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
-        if (l == 0) S.red[1][wv] = m;
-        __syncthreads();
-        m = S.red[1][0];
-        for (int w = 1; w < nw; ++w) m = fmaxf(m, S.red[1][w]);
         float se = 0.f;
-        for (int p = t; p < NN_P; p += blockDim.x) se += expf(S.lg[p] - m);
+        if (!(i32 & 1)) {
+#pragma unroll
+            for (int tl = 0; tl < 10; ++tl) se += lg[tl] > -1.0e38f ? expf(lg[tl] - m) : 0.f;
+        }
         for (int o = 32; o > 0; o >>= 1) se += __shfl_xor(se, o);
-        if (l == 0) S.red[2][wv] = se;
-        __syncthreads();
-        se = 0.f;
-        for (int w = 0; w < nw; ++w) se += S.red[2][w];
         NPH(3);
-        // top 8 (ties to the lower position): 8 arg-max rounds inside each
-        // wavefront over its share of positions, then the same over the 4 x 8
-        {
-            const int per = (NN_P + nw - 1) / nw, p0 = wv * per;
-            float v0 = -3.0e38f, v1 = -3.0e38f;
-            int q0 = NN_P, q1 = NN_P;
-            if (l < per && p0 + l < NN_P) { v0 = S.lg[p0 + l]; q0 = p0 + l; }
-            if (l + 64 < per && p0 + l + 64 < NN_P) { v1 = S.lg[p0 + l + 64]; q1 = p0 + l + 64; }
-            for (int k = 0; k < 8; ++k) {
-                float bv = v0; int bp = q0;
-                if (v1 > bv || (v1 == bv && q1 < bp)) { bv = v1; bp = q1; }
-                for (int o = 32; o > 0; o >>= 1) {
-                    const float ov = __shfl_xor(bv, o);
-                    const int op = __shfl_xor(bp, o);
-                    if (ov > bv || (ov == bv && op < bp)) { bv = ov; bp = op; }
-                }
-                if (l == 0) { S.cv[wv * 8 + k] = bv; S.cp[wv * 8 + k] = bp; }
-                if (q0 == bp) v0 = -3.0e38f;
-                if (q1 == bp) v1 = -3.0e38f;
+        // top 8, ties to the lower position
+        fdvg_nnout &o = out[gi];
+        float conf = 0.f;
+        int taken = 0;
+        for (int kk = 0; kk < 8; ++kk) {
+            float bv = -3.0e38f;
+            int bp = NN_PT;
+#pragma unroll
+            for (int tl = 0; tl < 10; ++tl) {
+                const int pp = tl * 32 + (rr >> 2) * 8 + g * 4 + (rr & 3);
+                if (!((taken >> tl) & 1) && pp < NN_P && (lg[tl] > bv || (lg[tl] == bv && pp < bp))) { bv = lg[tl]; bp = pp; }
             }
+            for (int o2 = 32; o2 > 0; o2 >>= 1) {
+                const float ov = __shfl_xor(bv, o2);
+                const int op = __shfl_xor(bp, o2);
+                if (ov > bv || (ov == bv && op < bp)) { bv = ov; bp = op; }
+            }
+            conf += expf(bv - m) / se;
+#pragma unroll
+            for (int tl = 0; tl < 10; ++tl) {
+                const int pp = tl * 32 + (rr >> 2) * 8 + g * 4 + (rr & 3);
+                if (pp == bp) taken |= 1 << tl;
+            }
+            if (l == 0) { o.cx[kk] = (int8_t)(2 * (bp % NN_G - 8)); o.cy[kk] = (int8_t)(2 * (bp / NN_G - 8)); }
         }
-        __syncthreads();
+        if (l == 0) { o.conf = conf; o.pskip = 1.f / (1.f + expf(-skl)); }
         NPH(4);
-        if (wv == 0) {
-            float v = l < 8 * nw ? S.cv[l] : -3.0e38f;
-            int pp = l < 8 * nw ? S.cp[l] : NN_P;
-            float conf = 0.f;
-            fdvg_nnout &o = out[q.mb0 + i];
-            for (int k = 0; k < 8; ++k) {
-                float bv = v; int bp = pp;
-                for (int o2 = 32; o2 > 0; o2 >>= 1) {
-                    const float ov = __shfl_xor(bv, o2);
-                    const int op = __shfl_xor(bp, o2);
-                    if (ov > bv || (ov == bv && op < bp)) { bv = ov; bp = op; }
-                }
-                conf += expf(bv - m) / se;
-                if (l == 0) { o.cx[k] = (int8_t)(2 * (bp % NN_G - 8)); o.cy[k] = (int8_t)(2 * (bp / NN_G - 8)); }
-                if (pp == bp) v = -3.0e38f;
-            }
-            if (l == 0) { o.conf = conf; o.pskip = 1.f / (1.f + expf(-skl)); }
-        }
-        __syncthreads();
-        NPH(5);
     }
 }
 
@@ -663,18 +763,20 @@ extern "C" int fdvgpu_nn_nparam(void) { return NN_NPARAM; }
 
 extern "C" int fdvg_nn_setup(void) {
     return hipFuncSetAttribute((const void *)k_nn_infer, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)sizeof(NnInfer)) == hipSuccess &&
+                               (int)sizeof(NiShared)) == hipSuccess &&
            hipFuncSetAttribute((const void *)k_nn_train, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)sizeof(NnLds)) == hipSuccess ? 0 : -1;
 }
 
-// wpc: proposer workgroups per CU share (1 or 2)
-extern "C" int fdvg_launch_nn_infer(const fdvg_seq *seqs, int nseq, int maxmb, const float *W, fdvg_nnout *out,
+// wpc: proposer workgroups per CU (1 or 2; one fits a CU at a time, so 2 only
+// shortens each workgroup's share).  Waves take the MBs of all sequences in
+// turn (sequence k owns MBs mb0 .. mb0 + its MB count - 1, in order).
+extern "C" int fdvg_launch_nn_infer(const fdvg_seq *seqs, int nseq, int total, const float *W, fdvg_nnout *out,
                                     int wpc, hipStream_t st) {
     static bool attr = false;
     if (!attr) {
         if (hipFuncSetAttribute((const void *)k_nn_infer, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)sizeof(NnInfer)) != hipSuccess) return -1;
+                                (int)sizeof(NiShared)) != hipSuccess) return -1;
         attr = true;
     }
     static int ncu = 0;
@@ -683,10 +785,11 @@ extern "C" int fdvg_launch_nn_infer(const fdvg_seq *seqs, int nseq, int maxmb, c
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return -1;
     }
-    // wpc workgroups per CU in all (the weights load once per workgroup)
-    int gx = ncu * (wpc == 2 ? 2 : 1) / nseq;
-    gx = gx < 1 ? 1 : gx > maxmb ? maxmb : gx;
-    hipLaunchKernelGGL(k_nn_infer, dim3(gx, nseq), dim3(256), sizeof(NnInfer), st, seqs, W, out);
+    if (total <= 0) return 0;
+    int gx = ncu * (wpc == 2 ? 2 : 1);
+    const int need = (total + NI_WAVES - 1) / NI_WAVES;
+    gx = gx > need ? need : gx;
+    hipLaunchKernelGGL(k_nn_infer, dim3(gx), dim3(64 * NI_WAVES), sizeof(NiShared), st, seqs, nseq, total, W, out);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
