@@ -285,7 +285,7 @@ __device__ __forceinline__ int ipred(int m, const Nb &q, const uint8_t *top, con
 
 // SATD of the four 16x16 intra sub-modes (mb_satd, fdv.h:4411): one wavefront,
 // lane = (mode, 4x4 block).  Returns mode m's total in sat[m] (uniform).
-__device__ __noinline__ void satd_modes(const uint8_t *cur, const Nb &nb, const uint8_t *top,
+__device__ __forceinline__ void satd_modes(const uint8_t *cur, const Nb &nb, const uint8_t *top,
                                            const uint8_t *left, int *sat) {
     const int L = threadIdx.x & 63, m = L >> 4, b = L & 15, oy = (b >> 2) * 4, ox = (b & 3) * 4;
     int d[16], t[16], sa = 0;
@@ -429,13 +429,15 @@ __device__ __forceinline__ void mb_decide(Smem &s, const fdvg_seq &q, MbCtx &c, 
 
     // ---- the independent searches, one per wavefront
     const Nb nbL = nb_setup(&s.hbytes[1], s.lcol, s.hbytes[0], 16, c.ht, c.hl);
-    if (!gv && !skip_wins) {
-        if (wave == 0 || (wave == 1 && q.navail == 2)) {
+    const bool screen = gv && gv->mode == 2 && gv->sub == FDVG_SUB_SCREEN;
+    if ((!gv && !skip_wins) || screen) {
+        // a given INTRA MB runs only the sub-mode screen (wave 3)
+        if (!screen && (wave == 0 || (wave == 1 && q.navail == 2))) {
             int a, b, sd;
             if (mt) { a = mt->mv16[wave][0]; b = mt->mv16[wave][1]; sd = mt->sad16[wave]; }
             else sd = me_diamond(s.win[wave], s.cur, s.sp[wave], 16, 0, 0, pmx, pmy, &a, &b);
             if ((tid & 63) == 0) { s.me[wave][0] = a; s.me[wave][1] = b; s.me[wave][2] = sd; }
-        } else if (wave == 2) {
+        } else if (!screen && wave == 2) {
             int sx = pmx, sy = pmy, ssad = 0;
             for (int qd = 0; qd < 4; ++qd) {
                 int a, b;
@@ -485,7 +487,7 @@ __device__ __forceinline__ void mb_decide(Smem &s, const fdvg_seq &q, MbCtx &c, 
     const bool want_n = gv ? gv->mode == 2 : (!skip_wins && !pruned && Jbest > lambda * (double)FL_INTRA);
     if (want_n) {
         int sel = 0;
-        if (gv) sel = gv->sub;
+        if (gv && !screen) sel = gv->sub;
         else {
             int sbest = 0; sel = -1;
             for (int m = 0; m < 4; ++m) {
@@ -1068,7 +1070,7 @@ k_prepair(const fdvg_seq *__restrict__ seqs, int mwmax, unsigned *ctl,
             const int sx_l = L < FDVG_NSC ? P.scx[L] : 0, sy_l = L < FDVG_NSC ? P.scy[L] : 0;
             const int sr_l = L < FDVG_NSC ? P.scr[L] : 0, se_l = L < FDVG_NSC ? P.sce[L] : 0;
             const int d16_0 = P.d16[0], d16_1 = P.d16[1], b16_0 = P.b16[0], b16_1 = P.b16[1];
-            const int pdn = P.dn, pbn = P.bn, pd8 = P.d8, pb8 = P.b8, psub = P.nsub;
+            const int pdn = P.dn, pbn = P.bn, pd8 = P.d8, pb8 = P.b8;
             const int m0x = T.mv16[0][0], m0y = T.mv16[0][1], m1x = T.mv16[1][0], m1y = T.mv16[1][1];
             const int s16_0 = T.sad16[0], s16_1 = T.sad16[1], tfl = T.flags;
             int q8x[4], q8y[4];
@@ -1198,7 +1200,7 @@ k_prepair(const fdvg_seq *__restrict__ seqs, int mwmax, unsigned *ctl,
                 __hip_atomic_store(&prog[wv], my * 65536 + mx + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
                 fdvg_mbinfo G;
                 G.mode = (uint8_t)mode; G.ref = (uint8_t)(mode == 0 || mode == 1 ? nr : 0);
-                G.merge_idx = (uint8_t)(mode == 0 ? bmi : 0); G.sub = (uint8_t)(mode == 2 ? psub : 0);
+                G.merge_idx = (uint8_t)(mode == 0 ? bmi : 0); G.sub = (uint8_t)(mode == 2 ? FDVG_SUB_SCREEN : 0);
                 for (int k = 0; k < 4; ++k) { G.mv[k][0] = 0; G.mv[k][1] = 0; }
                 if (mode == 0 || mode == 1) { G.mv[0][0] = (int16_t)nx; G.mv[0][1] = (int16_t)ny; }
                 if (mode == 3) for (int k = 0; k < 4; ++k) { G.mv[k][0] = (int16_t)q8x[k]; G.mv[k][1] = (int16_t)q8y[k]; }

@@ -32,13 +32,13 @@ struct Tune {
     int deblock_bs;     // deblock threads per workgroup: 64, 128, 256
     int nn_wpc;         // proposer workgroups per CU: 1, 2
 };
-extern "C" int fdvg_launch_metab(const fdvg_seq *, int, int, void *, const void *, float, hipStream_t);
+extern "C" int fdvg_launch_metab(const fdvg_seq *, int, int, void *, const void *, float, float, hipStream_t);
 extern "C" int fdvg_launch_nn_infer(const fdvg_seq *, int, int, const float *, fdvg_nnout *, int, hipStream_t);
 extern "C" int fdvg_nn_nparam(void);
 extern "C" int fdvg_nn_setup(void);
 
 // The proposer's confidence threshold: below it an MB gets the exhaustive search.
-#define FDVG_NN_THRESHOLD 0.7f
+#define FDVG_NN_THRESHOLD 0.7f     // default confidence and SKIP thresholds of the proposer
 extern "C" int fdvg_launch_parallel(const fdvg_seq *, int, int, int, int, unsigned *, fdvg_pcost *,
                                     fdvg_mbinfo *, fdvg_pctx *, unsigned *, unsigned *, int,
                                     fdvg_mbrec *, hipStream_t);
@@ -99,6 +99,7 @@ struct fdvgpu {
     float *nnw_d;
     fdvg_nnout *nn_d;
     int have_nn;
+    float nn_thr, nn_sthr;          // proposer thresholds: candidate search, SKIP pruning
     size_t compact_cap;
     uint32_t *flags_h;
     hipEvent_t ev[5];
@@ -220,6 +221,7 @@ extern "C" fdvgpu *fdvgpu_open(int nseq, const int *w, const int *h, int nsrc) {
     if (fdvg_nn_setup() != 0) { seterr("proposer kernel setup failed"); goto fail; }
     g->tune = Tune{4, 256, 1};
     g->graphs = 1;
+    g->nn_thr = g->nn_sthr = FDVG_NN_THRESHOLD;
     CKN(hipMalloc(&g->nn_d, (size_t)g->total_mb * sizeof(fdvg_nnout)));
 
     // rows in (row, sequence) order: every sequence's row r before any row r+1
@@ -317,7 +319,7 @@ static int launch_stage(fdvgpu *g, int stage, int me_mode, bool par) {
         }
         if (me_mode != FDVGPU_ME_DIAMOND &&
             fdvg_launch_metab(g->seq_d, g->nseq, g->maxmb, g->metab_d, me_mode == FDVGPU_ME_LEARNED ? g->nn_d : NULL,
-                              FDVG_NN_THRESHOLD, g->st) != 0) {
+                              g->nn_thr, g->nn_sthr, g->st) != 0) {
             seterr("motion table launch failed"); return -1;
         }
         if (fdvg_launch_zero(g->ticket_d, 8, g->st) != 0) { seterr("ticket reset launch failed"); return -1; }
@@ -517,6 +519,16 @@ extern "C" int fdvgpu_set_proposer(fdvgpu *g, const float *w, int n) {
     CK(hipMemcpyAsync(g->nnw_d, w, (size_t)n * sizeof(float), hipMemcpyHostToDevice, g->st));
     CK(hipStreamSynchronize(g->st));
     g->have_nn = 1;
+    return 0;
+}
+
+static void drop_graphs(fdvgpu *g);
+
+extern "C" int fdvgpu_set_proposer_thresholds(fdvgpu *g, float conf, float skip) {
+    if (!g || !(conf >= 0.f) || !(skip >= 0.f)) { seterr("fdvgpu_set_proposer_thresholds: bad arguments"); return -1; }
+    g->nn_thr = conf;
+    g->nn_sthr = skip;
+    drop_graphs(g);                 // the thresholds are launch arguments of captured graphs
     return 0;
 }
 

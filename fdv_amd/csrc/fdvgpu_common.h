@@ -50,6 +50,9 @@ typedef struct {
 // (which depend on the predictor), and the SKIP SSDs of a set of likely merge
 // candidates (the MB's own and its neighbours' table vectors).
 #define FDVG_NSC 12
+// A given INTRA decision whose sub-mode is chosen when the MB is coded, by the
+// SATD screen over its reconstructed neighbours (fdv.h:5950-5970).
+#define FDVG_SUB_SCREEN 0xFF
 typedef struct {
     int32_t d16[2], b16[2];           // INTER16 on each reference (bits incl. the 6 mode/ref bits)
     int32_t dn, bn;                   // INTRA 16x16 from the source frame's neighbours

@@ -65,10 +65,10 @@ __device__ void refine(MSmem &s, int &par, int b, int bs, int ox, int oy, const 
 // With a proposer output `nn` whose confidence is at least `thr`, the integer
 // search covers only the 3x3 whole-pel neighbourhoods of its 8 vectors (scaled
 // by 2 on the far reference), same tie order; below `thr` it is the full
-// search.  A SKIP probability of at least `thr` sets flags bit 0.
+// search.  A SKIP probability of at least `sthr` sets flags bit 0.
 extern "C" __global__ void __launch_bounds__(256)
 k_me_exhaustive(const fdvg_seq *__restrict__ seqs, fdvg_metab *__restrict__ out,
-                const fdvg_nnout *__restrict__ nn, float thr) {
+                const fdvg_nnout *__restrict__ nn, float thr, float sthr) {
     __shared__ MSmem s;
     const fdvg_seq q = seqs[blockIdx.y];
     const int mw = q.w >> 4, nmb = mw * (q.h >> 4);
@@ -172,13 +172,13 @@ k_me_exhaustive(const fdvg_seq *__restrict__ seqs, fdvg_metab *__restrict__ out,
             m.mv8[qd][0] = (int16_t)s.res[2 + qd][0]; m.mv8[qd][1] = (int16_t)s.res[2 + qd][1];
             m.sad8[qd] = s.res[2 + qd][2];
         }
-        m.flags = (pr && pr->pskip >= thr) ? 1 : 0;
+        m.flags = (pr && pr->pskip >= sthr) ? 1 : 0;
     }
 }
 
 extern "C" int fdvg_launch_metab(const fdvg_seq *seqs_d, int nseq, int maxmb, void *metab, const void *nn,
-                                 float thr, hipStream_t st) {
+                                 float thr, float sthr, hipStream_t st) {
     hipLaunchKernelGGL(k_me_exhaustive, dim3(maxmb, nseq), dim3(256), 0, st, seqs_d, (fdvg_metab *)metab,
-                       (const fdvg_nnout *)nn, thr);
+                       (const fdvg_nnout *)nn, thr, sthr);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }

@@ -79,7 +79,7 @@ __device__ __forceinline__ uint32_t lds_word(const uint8_t *p) { return *(const 
 // 2: A with RDOQ, no flag.  cb/pb/rb: LDS bases of the region's source,
 // prediction and reconstruction (4-byte aligned, strides multiples of 8).
 template <int KIND>
-__device__ __noinline__ void region_code(RegionWs &ws, int r, const uint8_t *cb, int cst,
+__device__ __forceinline__ void region_code(RegionWs &ws, int r, const uint8_t *cb, int cst,
                                             const uint8_t *pb, int pst, uint8_t *rb, int rst,
                                             int16_t *lev, uint8_t *flag, int qp, double lambda) {
     const int L = threadIdx.x & 63, blk = L >> 4, c = L & 15;

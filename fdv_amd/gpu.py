@@ -153,6 +153,12 @@ class Analyser:
         w = load_proposer() if weights is None else np.ascontiguousarray(weights, dtype=np.float32)
         self._ck(self.L.fdvgpu_set_proposer(self.h, w.ctypes.data, int(w.size)), "set_proposer")
 
+    def set_proposer_thresholds(self, conf=0.7, skip=0.7):
+        """Confidence threshold of the candidate search and SKIP threshold of the
+        pruning (above 1: never)."""
+        self.L.fdvgpu_set_proposer_thresholds.argtypes = [C.c_void_p, C.c_float, C.c_float]
+        self._ck(self.L.fdvgpu_set_proposer_thresholds(self.h, float(conf), float(skip)), "set_proposer_thresholds")
+
     def set_graphs(self, on: bool):
         self._ck(self.L.fdvgpu_set_graphs(self.h, 1 if on else 0), "set_graphs")
 
@@ -272,6 +278,11 @@ class Encoder:
         self.L.fdvamd_set_proposer.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
         if self.L.fdvamd_set_proposer(self.h, w.ctypes.data, int(w.size)) != 0:
             raise GpuError("fdvamd_set_proposer: " + self.L.fdvamd_last_error().decode())
+
+    def set_proposer_thresholds(self, conf=0.7, skip=0.7):
+        self.L.fdvamd_set_proposer_thresholds.argtypes = [C.c_void_p, C.c_float, C.c_float]
+        if self.L.fdvamd_set_proposer_thresholds(self.h, float(conf), float(skip)) != 0:
+            raise GpuError("fdvamd_set_proposer_thresholds: " + self.L.fdvamd_last_error().decode())
 
     def set_ref_planes(self, seq, which, frame):
         """Install a reference from three separate tight planes (the form the

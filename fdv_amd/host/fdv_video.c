@@ -175,3 +175,10 @@ int fdvamd_set_proposer(fdvamd_encoder *e, const float *weights, int n) {
     }
     return 0;
 }
+
+int fdvamd_set_proposer_thresholds(fdvamd_encoder *e, float conf, float skip) {
+    if (!e || fdvgpu_set_proposer_thresholds(e->g, conf, skip) != 0) {
+        amd_seterr("%s", e ? fdvgpu_last_error() : "bad arguments"); return -1;
+    }
+    return 0;
+}

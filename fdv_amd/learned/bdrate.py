@@ -74,7 +74,7 @@ def band_frame(frame, w, h, y0, bh):
     return np.concatenate([Y[y0:y0 + bh].ravel(), U[y0 // 2:(y0 + bh) // 2].ravel(), V[y0 // 2:(y0 + bh) // 2].ravel()])
 
 
-def code_sequence(frames, w, h, qp, mode, weights=None):
+def code_sequence(frames, w, h, qp, mode, weights=None, thresholds=None):
     """Bytes and mean PSNR-Y of P-frames 1.. of `frames` (frame 0 is installed as
     the key frame's reconstruction), coded as 6-MB-row bands in one batch."""
     from fdv_amd import gpu as G
@@ -82,6 +82,8 @@ def code_sequence(frames, w, h, qp, mode, weights=None):
     enc = G.Encoder([(w, bh) for _, bh in layout], me_mode=mode)
     if mode == G.ME_LEARNED:
         enc.set_proposer(weights)
+        if thresholds is not None:
+            enc.set_proposer_thresholds(*thresholds)
     for b, (y0, bh) in enumerate(layout):
         enc.set_ref(b, 0, band_frame(frames[0], w, h, y0, bh))
     tcs = [G.TabCache() for _ in layout]
@@ -98,7 +100,8 @@ def code_sequence(frames, w, h, qp, mode, weights=None):
     return total, float(np.mean(ps))
 
 
-def evaluate(scenes, qps=(20, 24, 28, 32), nframes=60, mode_a=None, mode_b=None, weights=None, log=print):
+def evaluate(scenes, qps=(20, 24, 28, 32), nframes=60, mode_a=None, mode_b=None, weights=None, log=print,
+             thresholds=None):
     from fdv_amd import gpu as G
     mode_a = G.ME_EXHAUSTIVE if mode_a is None else mode_a
     mode_b = G.ME_LEARNED if mode_b is None else mode_b
@@ -107,8 +110,8 @@ def evaluate(scenes, qps=(20, 24, 28, 32), nframes=60, mode_a=None, mode_b=None,
         frames = [sc.frame(i) for i in range(nframes)]
         pa, pb = [], []
         for qp in qps:
-            ba, da = code_sequence(frames, sc.w, sc.h, qp, mode_a, weights)
-            bb, db = code_sequence(frames, sc.w, sc.h, qp, mode_b, weights)
+            ba, da = code_sequence(frames, sc.w, sc.h, qp, mode_a, weights, thresholds)
+            bb, db = code_sequence(frames, sc.w, sc.h, qp, mode_b, weights, thresholds)
             pa.append((ba, da))
             pb.append((bb, db))
             log("  %s qp %d: A %d B %.3f dB | B %d B %.3f dB" % (name, qp, ba, da, bb, db))
@@ -125,18 +128,21 @@ def main():
     ap.add_argument("--b", default="learned", choices=["learned", "parallel", "exhaustive", "diamond"])
     ap.add_argument("--a", default="exhaustive", choices=["learned", "parallel", "exhaustive", "diamond"])
     ap.add_argument("--out", default=None, help="write the points and BD-rates as JSON")
+    ap.add_argument("--conf", type=float, default=0.7, help="proposer confidence threshold")
+    ap.add_argument("--skip", type=float, default=0.7, help="proposer SKIP threshold (above 1: no pruning)")
     args = ap.parse_args()
     modes = {"learned": G.ME_LEARNED, "parallel": G.ME_PARALLEL, "exhaustive": G.ME_EXHAUSTIVE,
              "diamond": G.ME_DIAMOND}
     scenes = [("pan720", content.scene_pan(1280, 720, 2)),
               ("objects720", content.scene_objects(1280, 720, 3, nobj=12, smin=32, smax=160, vmax=12, grain=2.0))]
     res = evaluate(scenes, [int(q) for q in args.qps.split(",")], args.frames, modes[args.a], modes[args.b],
-                   log=lambda m: print(m, flush=True))
+                   log=lambda m: print(m, flush=True), thresholds=(args.conf, args.skip))
     summary = {k: v["bd_rate"] for k, v in res.items()}
     print(json.dumps(summary), flush=True)
     if args.out:
         with open(args.out, "w") as f:
             json.dump({"a": args.a, "b": args.b, "frames": args.frames, "qps": args.qps,
+                       "thresholds": {"conf": args.conf, "skip": args.skip},
                        "bd_rate_percent": summary,
                        "points": {k: {"a": v["a"], "b": v["b"]} for k, v in res.items()}}, f, indent=1)
 

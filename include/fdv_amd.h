@@ -87,6 +87,10 @@ int fdvamd_pframe_batch(fdvamd_encoder *e, const uint8_t *const *frames, const i
  * me_mode 2.  No counterpart in the reference. */
 int fdvamd_set_proposer(fdvamd_encoder *e, const float *weights, int n);
 
+/* The proposer's confidence and SKIP thresholds (fdvgpu_set_proposer_thresholds).
+ * No counterpart in the reference. */
+int fdvamd_set_proposer_thresholds(fdvamd_encoder *e, float conf, float skip);
+
 /* Undo the last frame's reference rotation (all sequences) so the frame can be
  * coded again, e.g. at another QP: the requantising retry of the streaming
  * encoder (fdv.h:8303-8334).  The caller restores its entropy state as the

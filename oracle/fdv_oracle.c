@@ -836,7 +836,7 @@ int orc_pframe_analyse(const uint8_t *cy, const uint8_t *cu, const uint8_t *cv,
             if (want_n) {
                 uint8_t pI[256];
                 int sel = -1, sbest = 0;
-                if (gv) sel = gv->sub;
+                if (gv && gv->sub != ORC_SUB_SCREEN) sel = gv->sub;
                 else
                     for (int m = 0; m < 4; ++m) {
                         if (!nn_ok(m, 16, nht, nhl)) continue;
@@ -1521,7 +1521,7 @@ void orc_pframe_repair_par(const uint8_t *cy, int w, int h, const uint8_t *const
                 G->ref = (uint8_t)iref; G->mv[0][0] = T->mv16[iref][0]; G->mv[0][1] = T->mv16[iref][1];
                 *M = (mvrec_t){T->mv16[iref][0], T->mv16[iref][1], (uint8_t)iref, 1};
             } else if (mode == 2) {
-                G->sub = P->nsub;
+                G->sub = ORC_SUB_SCREEN;          /* chosen when coded, from the reconstruction */
                 *M = (mvrec_t){0, 0, 0, 0};
             } else {
                 for (int q = 0; q < 4; ++q) { G->mv[q][0] = T->mv8[q][0]; G->mv[q][1] = T->mv8[q][1]; }

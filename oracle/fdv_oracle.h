@@ -29,7 +29,10 @@ enum { ORC_S_MODE0 = 0, ORC_S_REF = 4, ORC_S_MVX = 5, ORC_S_MVY = 6, ORC_S_SUB =
 enum { ORC_ME_DIAMOND = 0,   /* the reference's EPZS diamond (fdv.h:3375)      */
        ORC_ME_EXHAUSTIVE = 1 /* full +-range integer search, same sub-pel step */ };
 
-/* One macroblock's decision.  mode: 0 SKIP, 1 INTER16, 2 INTRA, 3 INTER8. */
+/* One macroblock's decision.  mode: 0 SKIP, 1 INTER16, 2 INTRA, 3 INTER8.
+ * A given INTRA decision with sub == ORC_SUB_SCREEN has its sub-mode chosen when
+ * the MB is coded, by the SATD screen over its reconstructed neighbours. */
+#define ORC_SUB_SCREEN 0xFF
 typedef struct {
     uint8_t mode, ref, merge_idx, sub;
     int16_t mv[4][2];        /* [0] for SKIP / INTER16, all four for INTER8 */
