@@ -107,10 +107,13 @@ def cpu_baseline(frames, w, layout, qp, seconds=12.0):
 
 
 def profiled_traffic(mode: str):
-    """HBM bytes per k_analyse launch from the committed rocprofv3 counter
-    summary of this workload (FETCH_SIZE and WRITE_SIZE, each calibrated
-    against a plain copy), or None when no summary exists for this mode."""
-    path = os.path.join(ROOT, "profiles", "k_analyse_720p_%s.md" % mode)
+    """HBM bytes per launch of the dominant kernel (diamond / exhaustive: the
+    k_analyse wavefront; learned / parallel: the decision stage's five kernels)
+    from the committed rocprofv3 counter summary of the 720p workload
+    (FETCH_SIZE and WRITE_SIZE, each calibrated against a plain copy), or None
+    when no summary exists for this mode."""
+    name = ("decisions_720p_%s.md" if mode in ("learned", "parallel") else "k_analyse_720p_%s.md") % mode
+    path = os.path.join(ROOT, "profiles", name)
     try:
         for line in open(path):
             if line.startswith("traffic_bytes_per_launch:"):
@@ -261,16 +264,22 @@ def main():
 
     px_frame = sum(w * h for w, h in sizes)
     value = args.steps * px_frame / dt / 1e6
-    k_ms = float(kt[:, 1].mean())                  # the MB wavefront kernel
+    k_ms = float(kt[:, 1].mean())                  # the decision stage (wavefront or block-parallel)
     bytes_per_px = 6.0                              # 1.5 B x (source + 2 references + recon)
     algo_bytes = bytes_per_px * px_frame
     achieved = algo_bytes / (k_ms * 1e-3) / 1e9
-    traffic, traffic_src = profiled_traffic(args.mode)
+    traffic, traffic_src = profiled_traffic(args.mode) if args.workload == "720p" else (None, None)
+    par = mode in (G.ME_LEARNED, G.ME_PARALLEL)
+    # the dominant launch: the decision stage (one graph launch of five kernels)
+    # in the block-parallel modes, the k_analyse wavefront otherwise; timed with
+    # HIP events on the stream it runs on
     roof = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic, "traffic_source": traffic_src,
-            "kernel": "k_analyse", "kernel_ms": round(k_ms, 4),
-            "bytes_per_launch": int(algo_bytes)}
-    stage_ms = {"motion_tables": round(float(kt[:, 0].mean()), 4), "wavefront": round(k_ms, 4),
+            "kernel": "decision stage (k_pcost, k_prepair, k_plist, k_pencode, k_pintra)" if par else "k_analyse",
+            "kernel_ms": round(k_ms, 4), "bytes_per_launch": int(algo_bytes),
+            "path_achieved": round(value * bytes_per_px / 1000.0, 3),
+            "path_frac": round(value * bytes_per_px / 1000.0 / HBM_PEAK_GBS, 6)}
+    stage_ms = {"motion_tables": round(float(kt[:, 0].mean()), 4), "decisions": round(k_ms, 4),
                 "deblock": round(float(kt[:, 2].mean()), 4), "serialise": round(float(kt[:, 3].mean()), 4)}
     shapes = an.tune()
     an.close()

@@ -70,14 +70,20 @@ struct __align__(16) Smem {
 // vector predictor, the merge candidates (fdv.h:5711-5732) and whether the
 // top / left neighbours exist.  Luma intra neighbours are in s.hbytes (corner,
 // then 16 above) and s.lcol; chroma ones in s.ctop and s.lcol.
+// The small arrays are vector values (elements 0..2 / x,y pairs): indexing
+// them with a run-time value then selects among registers, where an array
+// member would put the whole struct in scratch memory.
+typedef int i4v __attribute__((ext_vector_type(4)));
+typedef int i8v __attribute__((ext_vector_type(8)));
 struct MbCtx {
     int pmx, pmy, ncand, ht, hl;
-    int cx[3], cy[3], cr[3];
+    i4v cx, cy, cr;
 };
-// The decision and what the finishing stage needs of it.
+// The decision and what the finishing stage needs of it; qmv[2 qd + 0/1] is
+// the vector of quadrant qd.
 struct MbDec {
     int mode, bmi, iref, imx, imy, nsub;
-    int qmv[4][2];
+    i8v qmv;
 };
 
 // ---- wave-local motion search ------------------------------------------------
@@ -385,7 +391,7 @@ __device__ __forceinline__ void mb_decide(Smem &s, const fdvg_seq &q, MbCtx &c, 
     const int qp = q.qp, pmx = c.pmx, pmy = c.pmy;
     D.bmi = 0; D.iref = 0; D.imx = 0; D.imy = 0; D.nsub = 0;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) { D.qmv[k][0] = 0; D.qmv[k][1] = 0; }
+    for (int k = 0; k < 4; ++k) { D.qmv[2 * k + 0] = 0; D.qmv[2 * k + 1] = 0; }
 
     if (q.force_skip) {
         // a held frame: SKIP with the zero vector everywhere (fdv.h:5820-5836)
@@ -397,7 +403,7 @@ __device__ __forceinline__ void mb_decide(Smem &s, const fdvg_seq &q, MbCtx &c, 
     // ---- SKIP: best merge candidate by SSD, first wins (fdv.h:5847-5866)
     const int ncand = c.ncand;
     const int x = tid & 15, y = tid >> 4;
-    int pv[3] = {0, 0, 0}, e[3] = {0, 0, 0};
+    i4v pv = {0, 0, 0, 0}, e = {0, 0, 0, 0};
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         if (k >= ncand || (gv && !(gv->mode == 0 && gv->merge_idx == k))) continue;
@@ -511,15 +517,15 @@ __device__ __forceinline__ void mb_decide(Smem &s, const fdvg_seq &q, MbCtx &c, 
     if (want_8) {
 #pragma unroll
         for (int qd = 0; qd < 4; ++qd) {
-            D.qmv[qd][0] = gv ? gv->mv[qd][0] : s.me[2 + qd][0];
-            D.qmv[qd][1] = gv ? gv->mv[qd][1] : s.me[2 + qd][1];
+            D.qmv[2 * qd + 0] = gv ? gv->mv[qd][0] : s.me[2 + qd][0];
+            D.qmv[2 * qd + 1] = gv ? gv->mv[qd][1] : s.me[2 + qd][1];
         }
         const int ssad = s.me[7][0];
         if (!gv && sad16 > 0 && ssad * 10 >= sad16 * 9) want_8 = false;
         if (want_8) {
             const int qd = (y >> 3) * 2 + (x >> 3);
-            const int vx = qd == 0 ? D.qmv[0][0] : qd == 1 ? D.qmv[1][0] : qd == 2 ? D.qmv[2][0] : D.qmv[3][0];
-            const int vy = qd == 0 ? D.qmv[0][1] : qd == 1 ? D.qmv[1][1] : qd == 2 ? D.qmv[2][1] : D.qmv[3][1];
+            const int vx = qd == 0 ? D.qmv[0] : qd == 1 ? D.qmv[2] : qd == 2 ? D.qmv[4] : D.qmv[6];
+            const int vy = qd == 0 ? D.qmv[1] : qd == 1 ? D.qmv[3] : qd == 2 ? D.qmv[5] : D.qmv[7];
             s.pred[tid] = (uint8_t)lumaq(s.win[0], x, y, vx, vy);
             __syncthreads();
             code_luma(s, s.pred, s.rec8, s.lev8, s.flag8, qp, lambda);
@@ -527,8 +533,8 @@ __device__ __forceinline__ void mb_decide(Smem &s, const fdvg_seq &q, MbCtx &c, 
             int qpx = pmx, qpy = pmy;
 #pragma unroll
             for (int qd2 = 0; qd2 < 4; ++qd2) {
-                b8 += egbits(zmap(D.qmv[qd2][0] - qpx)) + egbits(zmap(D.qmv[qd2][1] - qpy));
-                qpx = D.qmv[qd2][0]; qpy = D.qmv[qd2][1];
+                b8 += egbits(zmap(D.qmv[2 * qd2 + 0] - qpx)) + egbits(zmap(D.qmv[2 * qd2 + 1] - qpy));
+                qpx = D.qmv[2 * qd2 + 0]; qpy = D.qmv[2 * qd2 + 1];
             }
             J8 = (double)(s.ws.e[0] + s.ws.e[1] + s.ws.e[2] + s.ws.e[3]) + lambda * (double)b8;
         }
@@ -556,7 +562,7 @@ __device__ __forceinline__ void mb_motion(const MbCtx &c, const MbDec &D, int *n
         *ninter = 1;
     } else if (D.mode == 1) { *nmx = D.imx; *nmy = D.imy; *nref = D.iref; *ninter = 1; }
     else if (D.mode == 2)   { *nmx = 0; *nmy = 0; *nref = 0; *ninter = 0; }
-    else                    { *nmx = D.qmv[3][0]; *nmy = D.qmv[3][1]; *nref = 0; *ninter = 1; }
+    else                    { *nmx = D.qmv[6]; *nmy = D.qmv[7]; *nref = 0; *ninter = 1; }
 }
 
 // Chroma of the chosen mode, the reconstruction into the output planes, the
@@ -598,8 +604,8 @@ __device__ __forceinline__ const uint8_t *mb_finish(Smem &s, const fdvg_seq &q, 
     } else {
         if (tid < 128) {
             const int pl = tid >> 6, p = tid & 63, x = p & 7, y = p >> 3, qd = (y >> 2) * 2 + (x >> 2);
-            const int vx = qd == 0 ? D.qmv[0][0] : qd == 1 ? D.qmv[1][0] : qd == 2 ? D.qmv[2][0] : D.qmv[3][0];
-            const int vy = qd == 0 ? D.qmv[0][1] : qd == 1 ? D.qmv[1][1] : qd == 2 ? D.qmv[2][1] : D.qmv[3][1];
+            const int vx = qd == 0 ? D.qmv[0] : qd == 1 ? D.qmv[2] : qd == 2 ? D.qmv[4] : D.qmv[6];
+            const int vy = qd == 0 ? D.qmv[1] : qd == 1 ? D.qmv[3] : qd == 2 ? D.qmv[5] : D.qmv[7];
             s.cpred[tid] = (uint8_t)chromaq(s.cwin[0][pl], x, y, vx / 2, vy / 2);
         }
         __syncthreads();
@@ -633,10 +639,11 @@ __device__ __forceinline__ const uint8_t *mb_finish(Smem &s, const fdvg_seq &q, 
         }
         if (mode == 3) {
             int qpx = pmx, qpy = pmy;
+#pragma unroll
             for (int k = 0; k < 4; ++k) {
-                R->mv[k][0] = (int16_t)D.qmv[k][0]; R->mv[k][1] = (int16_t)D.qmv[k][1];
-                R->mvd[k][0] = (int16_t)(D.qmv[k][0] - qpx); R->mvd[k][1] = (int16_t)(D.qmv[k][1] - qpy);
-                qpx = D.qmv[k][0]; qpy = D.qmv[k][1];
+                R->mv[k][0] = (int16_t)D.qmv[2 * k + 0]; R->mv[k][1] = (int16_t)D.qmv[2 * k + 1];
+                R->mvd[k][0] = (int16_t)(D.qmv[2 * k + 0] - qpx); R->mvd[k][1] = (int16_t)(D.qmv[2 * k + 1] - qpy);
+                qpx = D.qmv[2 * k + 0]; qpy = D.qmv[2 * k + 1];
             }
         }
         if (q.info) {
@@ -844,7 +851,10 @@ k_pcost(const fdvg_seq *__restrict__ seqs, fdvg_pcost *__restrict__ pcs) {
     // SKIP SSDs of likely merge candidates: (0,0), and the table vectors of the
     // left, above, above-right (16x16 on both references, last quadrant), above-
     // left and left-left MBs; deduplicated, references that exist only
-    int ncs = 0, csx[FDVG_NSC], csy[FDVG_NSC], csr[FDVG_NSC];
+    typedef int i16v __attribute__((ext_vector_type(16)));
+    static_assert(FDVG_NSC <= 16, "candidate vectors");
+    int ncs = 0;
+    i16v csx = {}, csy = {}, csr = {};
     auto add = [&](int x, int y, int r) {
         if (r >= q.navail || ncs >= FDVG_NSC) return;
         for (int k = 0; k < ncs; ++k) if (csx[k] == x && csy[k] == y && csr[k] == r) return;
@@ -1252,6 +1262,7 @@ static size_t prepair_lds(int nw, int mwmax) {
 
 __device__ __forceinline__ void ctx_from(MbCtx &c, const fdvg_pctx &X, int ht, int hl) {
     c.pmx = X.pmx; c.pmy = X.pmy; c.ncand = X.ncand; c.ht = ht; c.hl = hl;
+#pragma unroll
     for (int k = 0; k < 3; ++k) { c.cx[k] = X.cx[k]; c.cy[k] = X.cy[k]; c.cr[k] = X.cr[k]; }
 }
 

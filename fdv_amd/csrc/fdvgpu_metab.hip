@@ -37,7 +37,7 @@ __device__ __forceinline__ uint32_t win_word(const uint8_t *w, int y, int x) {
 __device__ __forceinline__ unsigned long long umin64(unsigned long long a, unsigned long long b) { return a < b ? a : b; }
 
 // Refine block b (0,1: 16x16 on ref 0,1; 2..5: 8x8 quadrants on ref 0).
-__device__ void refine(MSmem &s, int &par, int b, int bs, int ox, int oy, const uint8_t *w,
+__device__ __forceinline__ void refine(MSmem &s, int &par, int b, int bs, int ox, int oy, const uint8_t *w,
                        int mx, int my, int best) {
     for (int st = 2; st >= 1 && best > 0; st >>= 1) {
         const int t = threadIdx.x, k = t >> 6, tt = t & 63;
@@ -79,7 +79,7 @@ k_me_exhaustive(const fdvg_seq *__restrict__ seqs, fdvg_metab *__restrict__ out,
     s.cur[tid] = q.cy[(size_t)(by + (tid >> 4)) * q.w + bx + (tid & 15)];
     for (int r = 0; r < nr; ++r) {
         const int wy = tid >> 2, wx = (tid & 3) * 16;
-        const uint8_t *src = q.ry[r] + (ptrdiff_t)(by - WOFF + wy) * q.ys + bx - WOFF + wx;
+        const uint8_t *src = (r ? q.ry[1] : q.ry[0]) + (ptrdiff_t)(by - WOFF + wy) * q.ys + bx - WOFF + wx;
         *(uint64_t *)&s.win[r][wy * WIN + wx] = *(const uint64_t *)src;
         *(uint64_t *)&s.win[r][wy * WIN + wx + 8] = *(const uint64_t *)(src + 8);
     }

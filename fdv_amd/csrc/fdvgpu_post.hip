@@ -214,8 +214,11 @@ k_sym(const SymSeq *__restrict__ seqs, const fdvg_mbrec *__restrict__ recs,
         for (int k = 0; k <= last; ++k) lb += leb_n(zz(lev[scan[k]]));
     }
     // per coefficient-stream pair: lanes' byte counts and their in-MB offsets
+    // (vector values: a lane indexes them with its own pair)
+    typedef uint32_t u4v __attribute__((ext_vector_type(4)));
     const int pr = !on ? -1 : cn == SG_CN ? 0 : cn == SG_CN8 ? 1 : 2;
-    uint32_t offn[3], offl[3], totn[3], totl[3];
+    u4v offn = {0u, 0u, 0u, 0u}, offl = {0u, 0u, 0u, 0u}, totn = {0u, 0u, 0u, 0u}, totl = {0u, 0u, 0u, 0u};
+#pragma unroll
     for (int c = 0; c < 3; ++c) {
         const uint32_t vn = pr == c ? 1u : 0u, vl = pr == c ? (uint32_t)lb : 0u;
         offn[c] = wave_excl_scan(vn);
@@ -224,30 +227,38 @@ k_sym(const SymSeq *__restrict__ seqs, const fdvg_mbrec *__restrict__ recs,
         totl[c] = __shfl(offl[c] + vl, 63, 64);
     }
 
-    Sink S;
-    for (int k = 0; k < FDVG_NS; ++k) {
-        if (pass == 0) { S.base[k] = nullptr; S.n[k] = 0; S.lim[k] = 0; }
-        else {
-            const uint32_t b0 = bases[blockIdx.y * FDVG_NS + k];
-            S.base[k] = compact + b0;
-            S.n[k] = cnt[(size_t)mb * FDVG_NS + k];
-            S.lim[k] = b0 < cap ? cap - b0 : 0;
+    if (pass == 1 && on) {
+        // lanes write their blocks at their offsets inside the MB's run of the
+        // two streams they feed (capacity-guarded like Sink::byte)
+        const uint32_t bn = bases[blockIdx.y * FDVG_NS + cn], bl = bases[blockIdx.y * FDVG_NS + cl];
+        const uint32_t limn = bn < cap ? cap - bn : 0, liml = bl < cap ? cap - bl : 0;
+        uint32_t nn = cnt[(size_t)mb * FDVG_NS + cn] + offn[pr], nl = cnt[(size_t)mb * FDVG_NS + cl] + offl[pr];
+        if (nn < limn) compact[bn + nn] = (uint8_t)(last + 1);
+        const uint8_t *scan = n == 16 ? p_SCAN4 : p_SCAN8;
+        for (int k = 0; k <= last; ++k) {
+            uint32_t u = zz(lev[scan[k]]);
+            while (u >= 0x80) { if (nl < liml) compact[bl + nl] = (uint8_t)(0x80 | (u & 0x7f)); ++nl; u >>= 7; }
+            if (nl < liml) compact[bl + nl] = (uint8_t)u;
+            ++nl;
         }
     }
-    if (pass == 1 && on) {
-        // lanes write their blocks at their offsets inside the MB's run
-        Sink W = S;
-        W.n[cn] += offn[pr];
-        W.n[cl] += offl[pr];
-        const uint8_t *scan = n == 16 ? p_SCAN4 : p_SCAN8;
-        W.byte(cn, (unsigned)(last + 1));
-        for (int k = 0; k <= last; ++k) W.leb(cl, zz(lev[scan[k]]));
-    }
     if (L == 0) {
+        // the structure symbols, by one lane
+        Sink S;
+        for (int k = 0; k < FDVG_NS; ++k) {
+            if (pass == 0) { S.base[k] = nullptr; S.n[k] = 0; S.lim[k] = 0; }
+            else {
+                const uint32_t b0 = bases[blockIdx.y * FDVG_NS + k];
+                S.base[k] = compact + b0;
+                S.n[k] = cnt[(size_t)mb * FDVG_NS + k];
+                S.lim[k] = b0 < cap ? cap - b0 : 0;
+            }
+        }
         emit_structure(S, R, mb_ctx(recs, q, i % q.mw, i / q.mw));
         if (pass == 0) {
-            const int sn[3] = {SG_CN, SG_CN8, SG_CNC}, sl[3] = {SG_CL, SG_CL8, SG_CLC};
-            for (int c = 0; c < 3; ++c) { S.n[sn[c]] += totn[c]; S.n[sl[c]] += totl[c]; }
+            S.n[SG_CN] += totn[0]; S.n[SG_CL] += totl[0];
+            S.n[SG_CN8] += totn[1]; S.n[SG_CL8] += totl[1];
+            S.n[SG_CNC] += totn[2]; S.n[SG_CLC] += totl[2];
             for (int k = 0; k < FDVG_NS; ++k) cnt[(size_t)mb * FDVG_NS + k] = S.n[k];
         }
     }

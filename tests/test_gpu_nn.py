@@ -188,3 +188,31 @@ def test_learned_mode_bit_exact_with_oracle(weights_seed):
             far[k], near[k] = near[k], rec
     an.close()
     assert ncand > 0
+
+
+def test_thresholds_above_one_give_the_exhaustive_table():
+    """With both proposer thresholds above 1 no MB takes the candidate search or
+    the SKIP pruning: the learned mode's motion table is the exhaustive one."""
+    sc = content.scene_objects(96, 64, 61, nobj=3, smin=8, smax=32, vmax=6)
+    w, h = sc.w, sc.h
+    an = G.Analyser([(w, h)], nsrc=2)
+    an.set_proposer(_weights(3))
+    an.set_proposer_thresholds(2.0, 2.0)
+    f0, f1, f2 = sc.frame(0), sc.frame(1), sc.frame(2)
+    an.put_ref(0, 0, f0)
+    an.put_source(0, 0, f1)
+    an.put_source(0, 1, f2)
+    say("  thresholds above 1: frame 1")
+    an.pframe([0], [24], navail=[1], me_mode=G.ME_LEARNED)
+    an.fetch()
+    rec1 = an.recon(0)
+    om = L.metab_to_np(L.oracle_metab_exhaustive(f1, f0, None, w, h, 1))
+    gm = an.motion(0)
+    assert all(om[i].tobytes() == gm[i].tobytes() for i in range(len(om)))
+    say("  thresholds above 1: frame 2")
+    an.pframe([1], [24], navail=[2], me_mode=G.ME_LEARNED)
+    an.fetch()
+    om = L.metab_to_np(L.oracle_metab_exhaustive(f2, rec1, f0, w, h, 2))
+    gm = an.motion(0)
+    assert all(om[i].tobytes() == gm[i].tobytes() for i in range(len(om)))
+    an.close()
