@@ -836,8 +836,8 @@ k_pcost(const fdvg_seq *__restrict__ seqs, fdvg_pcost *__restrict__ pcs) {
     __shared__ Smem s;
     __shared__ int32_t sred[FDVG_NSC][4];
     fdvg_seq q = seqs[blockIdx.y];
-    const int i = blockIdx.x;
-    if (i >= mb_count(q)) return;
+    int i;
+    if (!xcd_mb(blockIdx.x, q.w >> 4, q.h >> 4, i)) return;
     q.given = nullptr;
     const int tid = threadIdx.x, wave = tid >> 6;
     const int mw = q.w >> 4, mx = i % mw, my = i / mw, bx = mx * 16, by = my * 16;
@@ -1272,8 +1272,8 @@ k_pencode(const fdvg_seq *__restrict__ seqs, const fdvg_mbinfo *__restrict__ giv
           const fdvg_pctx *__restrict__ pctx, fdvg_mbrec *__restrict__ recs) {
     __shared__ Smem s;
     fdvg_seq q = seqs[blockIdx.y];
-    const int i = blockIdx.x;
-    if (i >= mb_count(q)) return;
+    int i;
+    if (!xcd_mb(blockIdx.x, q.w >> 4, q.h >> 4, i)) return;
     const int mb = q.mb0 + i;
     const fdvg_mbinfo *gv = &given[mb];
     if (gv->mode == 2) return;
@@ -1342,7 +1342,7 @@ k_pintra(const fdvg_seq *__restrict__ seqs, int nseq, const fdvg_mbinfo *__restr
     }
 }
 
-extern "C" int fdvg_launch_parallel(const fdvg_seq *seqs, int nseq, int maxmb, int maxmh, int mwmax,
+extern "C" int fdvg_launch_parallel(const fdvg_seq *seqs, int nseq, int xgrid, int maxmh, int mwmax,
                                     unsigned *ctl, fdvg_pcost *pcs, fdvg_mbinfo *given, fdvg_pctx *pctx,
                                     unsigned *ilist, unsigned *done, int total_mb,
                                     fdvg_mbrec *recs, hipStream_t st) {
@@ -1358,11 +1358,11 @@ extern "C" int fdvg_launch_parallel(const fdvg_seq *seqs, int nseq, int maxmb, i
     size_t lds = prepair_lds(nw, mwmax);
     while (nw > 1 && lds > 64 * 1024) lds = prepair_lds(--nw, mwmax);
     if (lds > 64 * 1024) return -1;
-    hipLaunchKernelGGL(k_pcost, dim3(maxmb, nseq), dim3(256), 0, st, seqs, pcs);
+    hipLaunchKernelGGL(k_pcost, dim3(xgrid, nseq), dim3(256), 0, st, seqs, pcs);
     hipLaunchKernelGGL(k_prepair, dim3(nseq), dim3(64 * nw), lds, st, seqs, mwmax, ctl, (const fdvg_pcost *)pcs,
                        given, pctx, ilist);
     hipLaunchKernelGGL(k_plist, dim3(1), dim3(1024), 0, st, (const fdvg_mbinfo *)given, total_mb, ilist);
-    hipLaunchKernelGGL(k_pencode, dim3(maxmb, nseq), dim3(256), 0, st, seqs, (const fdvg_mbinfo *)given,
+    hipLaunchKernelGGL(k_pencode, dim3(xgrid, nseq), dim3(256), 0, st, seqs, (const fdvg_mbinfo *)given,
                        (const fdvg_pctx *)pctx, recs);
     hipLaunchKernelGGL(k_pintra, dim3(total_mb < cap_i ? total_mb : cap_i), dim3(256), 0, st, seqs, nseq,
                        (const fdvg_mbinfo *)given, (const fdvg_pctx *)pctx, recs, ctl, (const unsigned *)ilist, done);

@@ -69,6 +69,7 @@ struct SeqMem {
 
 struct fdvgpu {
     int nseq, nsrc, nrows, total_mb, maxmb, maxpx, maxmh, maxmw;
+    int xgrid;                      // grid width of the one-workgroup-per-MB kernels (XCD strips)
     SeqMem *S;
     hipStream_t st;
     fdvg_seq *seq_d, *seq_h;        // seq_h: RING staging copies, one per frame in flight
@@ -179,6 +180,8 @@ extern "C" fdvgpu *fdvgpu_open(int nseq, const int *w, const int *h, int nsrc) {
         if (m.mh > g->maxmh) g->maxmh = m.mh;
         if (m.mw > g->maxmw) g->maxmw = m.mw;
         if (m.nmb > g->maxmb) g->maxmb = m.nmb;
+        // one-workgroup-per-MB kernels: 8 column strips of ceil(mw / 8), one per XCD
+        if (m.mh * ((m.mw + 7) >> 3) * 8 > g->xgrid) g->xgrid = m.mh * ((m.mw + 7) >> 3) * 8;
         if ((int)((size_t)m.ys * (m.h + 2 * FDVG_BORDER)) > g->maxpx) g->maxpx = (int)((size_t)m.ys * (m.h + 2 * FDVG_BORDER));
         m.near_ = 0; m.far_ = 1; m.out_ = 2;
         CKN(hipMalloc(&m.frames, 3 * m.fsz));
@@ -318,7 +321,7 @@ static int launch_stage(fdvgpu *g, int stage, int me_mode, bool par) {
             seterr("proposer launch failed"); return -1;
         }
         if (me_mode != FDVGPU_ME_DIAMOND &&
-            fdvg_launch_metab(g->seq_d, g->nseq, g->maxmb, g->metab_d, me_mode == FDVGPU_ME_LEARNED ? g->nn_d : NULL,
+            fdvg_launch_metab(g->seq_d, g->nseq, g->xgrid, g->metab_d, me_mode == FDVGPU_ME_LEARNED ? g->nn_d : NULL,
                               g->nn_thr, g->nn_sthr, g->st) != 0) {
             seterr("motion table launch failed"); return -1;
         }
@@ -326,7 +329,7 @@ static int launch_stage(fdvgpu *g, int stage, int me_mode, bool par) {
     } else if (stage == 1) {
         if (par) {
             if (fdvg_launch_zero(g->ilist_d, 1, g->st) != 0 || fdvg_launch_zero(g->done_d, g->total_mb, g->st) != 0 ||
-                fdvg_launch_parallel(g->seq_d, g->nseq, g->maxmb, g->maxmh, g->maxmw, g->ticket_d, g->pcost_d,
+                fdvg_launch_parallel(g->seq_d, g->nseq, g->xgrid, g->maxmh, g->maxmw, g->ticket_d, g->pcost_d,
                                      g->pgiven_d, g->pctx_d, g->ilist_d, g->done_d, g->total_mb,
                                      g->recs_d, g->st) != 0) {
                 seterr("parallel analysis launch failed: %s", hipGetErrorString(hipGetLastError())); return -1;

@@ -5,6 +5,21 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+// Workgroups go to the 8 XCDs round-robin by linear id, so with a grid width
+// that is a multiple of 8, blockIdx.x % 8 names the XCD.  A one-workgroup-per-MB
+// kernel gives each XCD one vertical strip of ceil(mw / 8) MB columns (MBs k =
+// blockIdx.x / 8 of the strip in raster order): the reference windows that
+// neighbouring MBs share are then fetched into one L2.  A grid width of
+// xcd_grid(mw, mh) covers every MB; false for the workgroups beyond it.
+__host__ __device__ __forceinline__ int xcd_grid(int mw, int mh) { return mh * ((mw + 7) >> 3) * 8; }
+__device__ __forceinline__ bool xcd_mb(int bx, int mw, int mh, int &i) {
+    const int sw = (mw + 7) >> 3, k = bx >> 3;
+    const int row = k / sw, col = (bx & 7) * sw + k % sw;
+    if (row >= mh || col >= mw) return false;
+    i = row * mw + col;
+    return true;
+}
+
 static __device__ __constant__ uint8_t k_SCAN4[16] = {0, 1, 4, 8, 5, 2, 3, 6, 9, 12, 13, 10, 7, 11, 14, 15};
 static __device__ __constant__ uint8_t k_INV4[16] = {0, 1, 5, 6, 2, 4, 7, 12, 3, 8, 11, 13, 9, 10, 14, 15};
 static __device__ __constant__ uint8_t k_INV8[64] = {

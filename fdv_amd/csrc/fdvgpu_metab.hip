@@ -72,8 +72,9 @@ k_me_exhaustive(const fdvg_seq *__restrict__ seqs, fdvg_metab *__restrict__ out,
     __shared__ MSmem s;
     const fdvg_seq q = seqs[blockIdx.y];
     const int mw = q.w >> 4, nmb = mw * (q.h >> 4);
-    const int i = blockIdx.x;
-    if (i >= nmb) return;
+    int i;
+    if (!xcd_mb(blockIdx.x, mw, q.h >> 4, i)) return;
+    (void)nmb;
     const int tid = threadIdx.x, bx = (i % mw) * 16, by = (i / mw) * 16;
     const int nr = q.navail;
     s.cur[tid] = q.cy[(size_t)(by + (tid >> 4)) * q.w + bx + (tid & 15)];
@@ -176,9 +177,10 @@ k_me_exhaustive(const fdvg_seq *__restrict__ seqs, fdvg_metab *__restrict__ out,
     }
 }
 
-extern "C" int fdvg_launch_metab(const fdvg_seq *seqs_d, int nseq, int maxmb, void *metab, const void *nn,
+// xgrid: the XCD-strip grid width (xcd_grid) of the largest sequence
+extern "C" int fdvg_launch_metab(const fdvg_seq *seqs_d, int nseq, int xgrid, void *metab, const void *nn,
                                  float thr, float sthr, hipStream_t st) {
-    hipLaunchKernelGGL(k_me_exhaustive, dim3(maxmb, nseq), dim3(256), 0, st, seqs_d, (fdvg_metab *)metab,
+    hipLaunchKernelGGL(k_me_exhaustive, dim3(xgrid, nseq), dim3(256), 0, st, seqs_d, (fdvg_metab *)metab,
                        (const fdvg_nnout *)nn, thr, sthr);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
