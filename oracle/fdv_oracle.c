@@ -742,6 +742,11 @@ static int merge_list(const mvrec_t *map, int mw, int mx, int my, int pmx, int p
     return n;
 }
 
+/* Test hook: with a nonzero value orc_pframe_analyse returns the reconstruction
+ * before the deblocking pass (the planes the MB loop's intra neighbours read). */
+static int orc_no_deblock = 0;
+void orc_set_no_deblock(int on) { orc_no_deblock = on; }
+
 int orc_pframe_analyse(const uint8_t *cy, const uint8_t *cu, const uint8_t *cv,
                        int w, int h,
                        const uint8_t *const RY[2], const uint8_t *const RU[2],
@@ -976,9 +981,11 @@ int orc_pframe_analyse(const uint8_t *cy, const uint8_t *cu, const uint8_t *cv,
         }
     }
 #undef STRUCT
-    orc_deblock_plane(ry, w, h, w, qp);                    /* fdv.h:6168-6170 */
-    orc_deblock_plane(ru, cw, h / 2, cw, qp);
-    orc_deblock_plane(rv, cw, h / 2, cw, qp);
+    if (!orc_no_deblock) {
+        orc_deblock_plane(ry, w, h, w, qp);                /* fdv.h:6168-6170 */
+        orc_deblock_plane(ru, cw, h / 2, cw, qp);
+        orc_deblock_plane(rv, cw, h / 2, cw, qp);
+    }
 
     /* Structure symbols partitioned by model, raster order kept (fdv.h:6180-6193). */
     for (int c = 0; c < ORC_NSTRUCT; ++c) {
@@ -1403,8 +1410,17 @@ void orc_metab_candidates(const uint8_t *cy, int w, int h, const uint8_t *const 
     }
 }
 
+/* The block-parallel costs with the intra neighbours read from plane `nb` (tight,
+ * stride w): the product computes them from the source (nb == cy); the
+ * parity tests also pass the exact path's pre-deblock reconstruction. */
+void orc_pframe_costs_par_nb(const uint8_t *cy, const uint8_t *nb, int w, int h, const uint8_t *const RY[2],
+                             int ys, int navail, int qp, double lambda, const orc_metab *mt, orc_pcost *pc);
 void orc_pframe_costs_par(const uint8_t *cy, int w, int h, const uint8_t *const RY[2], int ys,
                           int navail, int qp, double lambda, const orc_metab *mt, orc_pcost *pc) {
+    orc_pframe_costs_par_nb(cy, cy, w, h, RY, ys, navail, qp, lambda, mt, pc);
+}
+void orc_pframe_costs_par_nb(const uint8_t *cy, const uint8_t *nb, int w, int h, const uint8_t *const RY[2],
+                             int ys, int navail, int qp, double lambda, const orc_metab *mt, orc_pcost *pc) {
     const int mw = w / 16, mh = h / 16;
     for (int my = 0; my < mh; ++my)
         for (int mx = 0; mx < mw; ++mx) {
@@ -1421,7 +1437,7 @@ void orc_pframe_costs_par(const uint8_t *cy, int w, int h, const uint8_t *const 
             if (T->flags & 1) continue;                 /* SKIP expected: no INTRA, no INTER8 costs */
             {
                 uint8_t ntop[16], nleft[16], ntl, pI[256], rn[256]; int nht, nhl;
-                neighbours(cy, w, bx, by, 16, 0, ntop, nleft, &ntl, &nht, &nhl);
+                neighbours(nb, w, bx, by, 16, 0, ntop, nleft, &ntl, &nht, &nhl);
                 int sel = -1, sbest = 0;
                 for (int m = 0; m < 4; ++m) {
                     if (!nn_ok(m, 16, nht, nhl)) continue;

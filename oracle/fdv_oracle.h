@@ -156,6 +156,9 @@ void orc_metab_candidates(const uint8_t *cy, int w, int h, const uint8_t *const 
 /* Every MB's candidate costs, independently of every other MB. */
 void orc_pframe_costs_par(const uint8_t *cy, int w, int h, const uint8_t *const RY[2], int ys,
                           int navail, int qp, double lambda, const orc_metab *mt, orc_pcost *pc);
+void orc_pframe_costs_par_nb(const uint8_t *cy, const uint8_t *nb, int w, int h, const uint8_t *const RY[2],
+                             int ys, int navail, int qp, double lambda, const orc_metab *mt, orc_pcost *pc);
+void orc_set_no_deblock(int on);
 /* The reference's decision logic in raster order on the stored costs, with the
  * true vector predictor and merge list.  The result is a `given` for
  * orc_pframe_analyse. */

@@ -1,0 +1,132 @@
+"""Pinning of the oracle's block-parallel and learned-search functions
+(oracle/fdv_oracle.c: orc_metab_exhaustive, orc_metab_candidates,
+orc_pframe_costs_par_nb, orc_pframe_repair_par) against functions the
+reference's vectors and brute force already pin:
+
+- the motion table is orc_me_search's exhaustive search of every block;
+- the candidate search equals it wherever the proposals cover the whole-pel
+  minimum, falls back to it below the confidence threshold, and flags SKIP by
+  its own threshold;
+- the decision replay, given intra costs read from the exact wavefront's own
+  pre-deblock reconstruction, reproduces the exact wavefront's decisions (modes,
+  references, merge indices and vectors) on every MB.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import orclib as L
+from fdv_amd import content
+
+PROPOSAL = np.dtype([("cx", "i1", 8), ("cy", "i1", 8), ("conf", "<f4"), ("pskip", "<f4")])
+
+
+def _luma(f, w, h):
+    return np.ascontiguousarray(f[:w * h]).reshape(h, w)
+
+
+def _search(cur, ref, w, h, bx, by, bs):
+    lib = L.oracle()
+    buf, off, st = L.padded(_luma(ref, w, h), w, h)
+    mx, my = C.c_int(), C.c_int()
+    s = lib.orc_me_search(C.c_void_p(np.ascontiguousarray(cur[:w * h]).ctypes.data), w,
+                          C.c_void_p(buf.ctypes.data + off), st, bx, by, bs, bs, 16, 0, 0, 1,
+                          C.byref(mx), C.byref(my))
+    return mx.value, my.value, s
+
+
+def _wholepel_min(cur, ref, w, h, bx, by):
+    """Brute-force whole-pel minimum over +-16 (ties: |dx|+|dy|, then raster)."""
+    P = np.pad(_luma(ref, w, h), 32, mode="edge").astype(int)
+    blk = _luma(cur, w, h)[by:by + 16, bx:bx + 16].astype(int)
+    best = None
+    for dy in range(-16, 17):
+        for dx in range(-16, 17):
+            s = int(np.abs(blk - P[32 + by + dy:48 + by + dy, 32 + bx + dx:48 + bx + dx]).sum())
+            key = (s, abs(dx) + abs(dy), (dy + 16) * 33 + dx + 16)
+            if best is None or key < best[0]:
+                best = (key, dx, dy)
+    return best[1], best[2]
+
+
+def _frames():
+    sc = content.scene_objects(96, 64, 71, nobj=3, smin=8, smax=32, vmax=6, grain=1.0)
+    return sc.w, sc.h, [sc.frame(i) for i in range(3)]
+
+
+def test_table_is_the_exhaustive_search_of_every_block():
+    w, h, (f0, f1, f2) = _frames()
+    for navail, r0, r1 in ((2, f1, f0), (1, f1, None)):
+        mt = L.metab_to_np(L.oracle_metab_exhaustive(f2, r0, r1, w, h, navail))
+        for i in range(len(mt)):
+            bx, by = (i % (w // 16)) * 16, (i // (w // 16)) * 16
+            for r, ref in enumerate((r0, r1 if navail == 2 else r0)):
+                mx, my, s = _search(f2, ref, w, h, bx, by, 16)
+                assert (int(mt[i]["mv16"][r][0]), int(mt[i]["mv16"][r][1]), int(mt[i]["sad16"][r])) == (mx, my, s)
+            for q in range(4):
+                mx, my, s = _search(f2, r0, w, h, bx + (q & 1) * 8, by + (q >> 1) * 8, 8)
+                assert (int(mt[i]["mv8"][q][0]), int(mt[i]["mv8"][q][1]), int(mt[i]["sad8"][q])) == (mx, my, s)
+            assert mt[i]["flags"] == 0
+
+
+def test_candidate_search_covers_the_exhaustive_minimum_and_thresholds():
+    w, h, (f0, f1, f2) = _frames()
+    nmb = (w // 16) * (h // 16)
+    full = L.metab_to_np(L.oracle_metab_exhaustive(f2, f1, None, w, h, 1))
+    # below the confidence threshold: the exhaustive table, flags by the SKIP threshold
+    low = np.zeros(nmb, PROPOSAL)
+    low["conf"] = 0.5
+    low["pskip"] = np.where(np.arange(nmb) % 2 == 0, 0.9, 0.1)
+    mt = L.metab_to_np(L.oracle_metab_candidates(f2, f1, None, w, h, 1, low, thr=0.7))
+    assert np.array_equal(mt["mv16"], full["mv16"]) and np.array_equal(mt["sad16"], full["sad16"])
+    assert np.array_equal(mt["mv8"], full["mv8"])
+    assert list(mt["flags"]) == [1 if i % 2 == 0 else 0 for i in range(nmb)]
+    # confident proposals whose first vector's 3x3 neighbourhood holds the whole-pel
+    # minimum: the same 16x16 vector and SAD as the full search
+    hi = np.zeros(nmb, PROPOSAL)
+    hi["conf"] = 0.9
+    for i in range(nmb):
+        bx, by = (i % (w // 16)) * 16, (i // (w // 16)) * 16
+        dx, dy = _wholepel_min(f2, f1, w, h, bx, by)
+        hi["cx"][i] = dx - (dx & 1)              # even, as the proposer's grid; +-1 covers dx
+        hi["cy"][i] = dy - (dy & 1)
+        hi["cx"][i][1:] = 8                      # far-away decoys
+        hi["cy"][i][1:] = -8
+    mt = L.metab_to_np(L.oracle_metab_candidates(f2, f1, None, w, h, 1, hi, thr=0.7))
+    assert np.array_equal(mt["mv16"][:, 0], full["mv16"][:, 0])
+    assert np.array_equal(mt["sad16"][:, 0], full["sad16"][:, 0])
+    assert not mt["flags"].any()
+
+
+@pytest.mark.parametrize("qp", [20, 28, 36])
+def test_replay_with_exact_intra_costs_reproduces_the_wavefront(qp):
+    sc = content.scene_objects(128, 64, 73, nobj=4, smin=8, smax=40, vmax=8, grain=2.0)
+    w, h = sc.w, sc.h
+    frames = [sc.frame(i) for i in range(4)]
+    near, far = frames[0], None
+    seen = set()
+    for f in range(1, 4):
+        navail = 1 if f == 1 else 2
+        L.set_no_deblock(True)
+        try:
+            _, rec_nd, info = L.oracle_analyse(frames[f], near, far, w, h, navail, qp, me_mode=1)
+        finally:
+            L.set_no_deblock(False)
+        _, rec, info_db = L.oracle_analyse(frames[f], near, far, w, h, navail, qp, me_mode=1)
+        assert [m.mode for m in info] == [m.mode for m in info_db]
+        given, _ = L.oracle_parallel_decisions(frames[f], near, far, w, h, navail, qp, nb=rec_nd)
+        for i in range(len(info)):
+            a, g = info[i], given[i]
+            assert a.mode == g.mode, f"frame {f} MB {i}: mode {a.mode} vs replay {g.mode}"
+            seen.add(a.mode)
+            if a.mode in (0, 1):
+                assert (a.ref, a.mv[0][0], a.mv[0][1]) == (g.ref, g.mv[0][0], g.mv[0][1]), f"frame {f} MB {i}"
+            if a.mode == 0:
+                assert a.merge_idx == g.merge_idx, f"frame {f} MB {i}"
+            if a.mode == 3:
+                assert [tuple(a.mv[k]) for k in range(4)] == [tuple(g.mv[k]) for k in range(4)], f"frame {f} MB {i}"
+            if a.mode == 2:
+                assert g.sub == 0xFF                 # screened when coded
+        far, near = near, rec
+    assert {0, 1, 2}.issubset(seen)
