@@ -190,7 +190,7 @@ def main():
                          "of mixed sizes; stress1080 / stress4k: 500 small fast blobs with scene cuts, "
                          "as 6-MB-row bands")
     ap.add_argument("--conf", type=float, default=None, help="proposer confidence threshold (default 0.7)")
-    ap.add_argument("--skip", type=float, default=None, help="proposer SKIP threshold (default 0.7; above 1: off)")
+    ap.add_argument("--skip", type=float, default=None, help="proposer SKIP threshold (default: off; 0.7 prunes)")
     ap.add_argument("--baseline-steps", type=int, default=10,
                     help="steps of the exhaustive search timed for comparison (0: skip)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -234,7 +234,7 @@ def main():
             a.set_proposer()
             if args.conf is not None or args.skip is not None:
                 a.set_proposer_thresholds(0.7 if args.conf is None else args.conf,
-                                          0.7 if args.skip is None else args.skip)
+                                          2.0 if args.skip is None else args.skip)
         for b in range(nb):
             a.put_ref(b, 0, per_seq[b][0])
             for i in range(1, nframes):

@@ -38,7 +38,8 @@ extern "C" int fdvg_nn_nparam(void);
 extern "C" int fdvg_nn_setup(void);
 
 // The proposer's confidence threshold: below it an MB gets the exhaustive search.
-#define FDVG_NN_THRESHOLD 0.7f     // default confidence and SKIP thresholds of the proposer
+#define FDVG_NN_THRESHOLD 0.7f     // default confidence threshold of the proposer
+#define FDVG_NN_SKIP_OFF 2.0f      // default SKIP threshold: no pruning
 extern "C" int fdvg_launch_parallel(const fdvg_seq *, int, int, int, int, unsigned *, fdvg_pcost *,
                                     fdvg_mbinfo *, fdvg_pctx *, unsigned *, unsigned *, int,
                                     fdvg_mbrec *, hipStream_t);
@@ -224,7 +225,8 @@ extern "C" fdvgpu *fdvgpu_open(int nseq, const int *w, const int *h, int nsrc) {
     if (fdvg_nn_setup() != 0) { seterr("proposer kernel setup failed"); goto fail; }
     g->tune = Tune{4, 256, 1};
     g->graphs = 1;
-    g->nn_thr = g->nn_sthr = FDVG_NN_THRESHOLD;
+    g->nn_thr = FDVG_NN_THRESHOLD;
+    g->nn_sthr = FDVG_NN_SKIP_OFF;
     CKN(hipMalloc(&g->nn_d, (size_t)g->total_mb * sizeof(fdvg_nnout)));
 
     // rows in (row, sequence) order: every sequence's row r before any row r+1

@@ -153,9 +153,9 @@ class Analyser:
         w = load_proposer() if weights is None else np.ascontiguousarray(weights, dtype=np.float32)
         self._ck(self.L.fdvgpu_set_proposer(self.h, w.ctypes.data, int(w.size)), "set_proposer")
 
-    def set_proposer_thresholds(self, conf=0.7, skip=0.7):
+    def set_proposer_thresholds(self, conf=0.7, skip=2.0):
         """Confidence threshold of the candidate search and SKIP threshold of the
-        pruning (above 1: never)."""
+        pruning (above 1: never; the default)."""
         self.L.fdvgpu_set_proposer_thresholds.argtypes = [C.c_void_p, C.c_float, C.c_float]
         self._ck(self.L.fdvgpu_set_proposer_thresholds(self.h, float(conf), float(skip)), "set_proposer_thresholds")
 
@@ -279,7 +279,7 @@ class Encoder:
         if self.L.fdvamd_set_proposer(self.h, w.ctypes.data, int(w.size)) != 0:
             raise GpuError("fdvamd_set_proposer: " + self.L.fdvamd_last_error().decode())
 
-    def set_proposer_thresholds(self, conf=0.7, skip=0.7):
+    def set_proposer_thresholds(self, conf=0.7, skip=2.0):
         self.L.fdvamd_set_proposer_thresholds.argtypes = [C.c_void_p, C.c_float, C.c_float]
         if self.L.fdvamd_set_proposer_thresholds(self.h, float(conf), float(skip)) != 0:
             raise GpuError("fdvamd_set_proposer_thresholds: " + self.L.fdvamd_last_error().decode())

@@ -129,7 +129,7 @@ def main():
     ap.add_argument("--a", default="exhaustive", choices=["learned", "parallel", "exhaustive", "diamond"])
     ap.add_argument("--out", default=None, help="write the points and BD-rates as JSON")
     ap.add_argument("--conf", type=float, default=0.7, help="proposer confidence threshold")
-    ap.add_argument("--skip", type=float, default=0.7, help="proposer SKIP threshold (above 1: no pruning)")
+    ap.add_argument("--skip", type=float, default=2.0, help="proposer SKIP threshold (above 1: no pruning)")
     args = ap.parse_args()
     modes = {"learned": G.ME_LEARNED, "parallel": G.ME_PARALLEL, "exhaustive": G.ME_EXHAUSTIVE,
              "diamond": G.ME_DIAMOND}

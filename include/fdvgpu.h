@@ -140,11 +140,11 @@ typedef struct {
 /* Number of proposer parameters (fp32), and installing them for the learned mode. */
 int fdvgpu_nn_nparam(void);
 int fdvgpu_set_proposer(fdvgpu *g, const float *weights, int n);
-/* Proposer thresholds (both 0.7 by default): an MB whose top-8 probability is
- * at least `conf` gets the candidate search, below it the exhaustive search; an
- * MB whose SKIP probability is at least `skip` is weighed SKIP against INTER16
- * only.  A value above 1 disables that use.  No counterpart in the reference
- * (the proposer is this library's own). */
+/* Proposer thresholds: an MB whose top-8 probability is at least `conf` (0.7 by
+ * default) gets the candidate search, below it the exhaustive search; an MB
+ * whose SKIP probability is at least `skip` is weighed SKIP against INTER16 only
+ * (off by default: 2.0).  A value above 1 disables that use.  No counterpart in
+ * the reference (the proposer is this library's own). */
 int fdvgpu_set_proposer_thresholds(fdvgpu *g, float conf, float skip);
 
 /* The proposals and the motion table of the last frame (nmb entries each). */

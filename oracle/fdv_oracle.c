@@ -1378,13 +1378,13 @@ static int search_points(const uint8_t *cur, int cs, const uint8_t *ref, int rs,
 }
 
 void orc_metab_candidates(const uint8_t *cy, int w, int h, const uint8_t *const RY[2], int ys, int navail,
-                          const orc_nnout *nn, float thr, orc_metab *mt) {
+                          const orc_nnout *nn, float thr, float sthr, orc_metab *mt) {
     const int mw = w / 16, mh = h / 16;
     orc_metab_exhaustive(cy, w, h, RY, ys, navail, mt);     /* the fallback for every MB */
     for (int i = 0; i < mw * mh; ++i) {
         const orc_nnout *P = &nn[i];
         orc_metab *m = &mt[i];
-        m->flags = P->pskip >= thr ? 1 : 0;
+        m->flags = P->pskip >= sthr ? 1 : 0;
         if (!(P->conf >= thr)) continue;
         const int bx = (i % mw) * 16, by = (i / mw) * 16;
         for (int r = 0; r < navail; ++r) {

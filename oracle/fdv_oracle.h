@@ -149,10 +149,10 @@ void orc_metab_exhaustive(const uint8_t *cy, int w, int h, const uint8_t *const 
  * thr, the whole-pel search covers the 3x3 neighbourhoods of its 8 vectors
  * (scaled by 2 on the far reference) and the zero vector, with the exhaustive
  * search's tie order, then the sub-pel cross; elsewhere the exhaustive search.
- * flags bit 0 = SKIP probability >= thr: the decision then weighs SKIP against
+ * flags bit 0 = SKIP probability >= sthr: the decision then weighs SKIP against
  * INTER16 only. */
 void orc_metab_candidates(const uint8_t *cy, int w, int h, const uint8_t *const RY[2], int ys, int navail,
-                          const orc_nnout *nn, float thr, orc_metab *mt);
+                          const orc_nnout *nn, float thr, float sthr, orc_metab *mt);
 /* Every MB's candidate costs, independently of every other MB. */
 void orc_pframe_costs_par(const uint8_t *cy, int w, int h, const uint8_t *const RY[2], int ys,
                           int navail, int qp, double lambda, const orc_metab *mt, orc_pcost *pc);

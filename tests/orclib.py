@@ -324,7 +324,7 @@ def set_no_deblock(on: bool):
     oracle().orc_set_no_deblock(1 if on else 0)
 
 
-def oracle_metab_candidates(cur, r0, r1, w, h, navail, nnout, thr=0.7):
+def oracle_metab_candidates(cur, r0, r1, w, h, navail, nnout, thr=0.7, sthr=2.0):
     """The learned mode's motion table from the proposer outputs `nnout` (a numpy
     array with the fdvgpu_proposal layout)."""
     lib = oracle()
@@ -336,9 +336,9 @@ def oracle_metab_candidates(cur, r0, r1, w, h, navail, nnout, thr=0.7):
     nn = np.ascontiguousarray(nnout)
     assert nn.dtype.itemsize == 24 and nn.size == nmb
     lib.orc_metab_candidates.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_void_p,
-                                         C.c_float, C.c_void_p]
+                                         C.c_float, C.c_float, C.c_void_p]
     lib.orc_metab_candidates(C.c_void_p(cy.ctypes.data), w, h, C.addressof(RY), ys, navail,
-                             C.c_void_p(nn.ctypes.data), float(thr), C.addressof(mt))
+                             C.c_void_p(nn.ctypes.data), float(thr), float(sthr), C.addressof(mt))
     return mt
 
 

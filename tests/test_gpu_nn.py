@@ -158,6 +158,9 @@ def test_learned_mode_bit_exact_with_oracle(weights_seed):
     W = _weights(weights_seed)
     W[N.OFF_WF:N.OFF_BF] *= 4.0                      # sharper logits: both search branches occur
     an.set_proposer(W)
+    sthr = 0.7 if weights_seed == 1 else 2.0         # SKIP pruning on, then off (the default)
+    if weights_seed == 1:
+        an.set_proposer_thresholds(0.7, sthr)
     frames = [[s.frame(i) for i in range(4)] for s in scenes]
     near = [fr[0] for fr in frames]
     far = [None, None]
@@ -175,7 +178,7 @@ def test_learned_mode_bit_exact_with_oracle(weights_seed):
             props = an.proposals(k)
             ncand += int((props["conf"] >= 0.7).sum())
             nfull += int((props["conf"] < 0.7).sum())
-            mt = L.oracle_metab_candidates(frames[k][f], near[k], far[k], w, h, navail[k], props)
+            mt = L.oracle_metab_candidates(frames[k][f], near[k], far[k], w, h, navail[k], props, sthr=sthr)
             om, gm = L.metab_to_np(mt), an.motion(k)
             bad = [i for i in range(len(om)) if om[i].tobytes() != gm[i].tobytes()]
             assert not bad, (f"seq {k} frame {f}: motion table differs at {len(bad)} MBs, first {bad[0]}: "

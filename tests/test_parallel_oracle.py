@@ -78,7 +78,7 @@ def test_candidate_search_covers_the_exhaustive_minimum_and_thresholds():
     low = np.zeros(nmb, PROPOSAL)
     low["conf"] = 0.5
     low["pskip"] = np.where(np.arange(nmb) % 2 == 0, 0.9, 0.1)
-    mt = L.metab_to_np(L.oracle_metab_candidates(f2, f1, None, w, h, 1, low, thr=0.7))
+    mt = L.metab_to_np(L.oracle_metab_candidates(f2, f1, None, w, h, 1, low, thr=0.7, sthr=0.7))
     assert np.array_equal(mt["mv16"], full["mv16"]) and np.array_equal(mt["sad16"], full["sad16"])
     assert np.array_equal(mt["mv8"], full["mv8"])
     assert list(mt["flags"]) == [1 if i % 2 == 0 else 0 for i in range(nmb)]
@@ -93,7 +93,7 @@ def test_candidate_search_covers_the_exhaustive_minimum_and_thresholds():
         hi["cy"][i] = dy - (dy & 1)
         hi["cx"][i][1:] = 8                      # far-away decoys
         hi["cy"][i][1:] = -8
-    mt = L.metab_to_np(L.oracle_metab_candidates(f2, f1, None, w, h, 1, hi, thr=0.7))
+    mt = L.metab_to_np(L.oracle_metab_candidates(f2, f1, None, w, h, 1, hi, thr=0.7, sthr=0.7))
     assert np.array_equal(mt["mv16"][:, 0], full["mv16"][:, 0])
     assert np.array_equal(mt["sad16"][:, 0], full["sad16"][:, 0])
     assert not mt["flags"].any()
