@@ -189,6 +189,8 @@ def main():
                     help="720p: the headline workload (8 bands of one 1280x720 stream); batched: 16 streams "
                          "of mixed sizes; stress1080 / stress4k: 500 small fast blobs with scene cuts, "
                          "as 6-MB-row bands")
+    ap.add_argument("--no-overlap", action="store_true",
+                    help="run the block-parallel decision stage's costs and replay one after the other")
     ap.add_argument("--conf", type=float, default=None, help="proposer confidence threshold (default 0.7)")
     ap.add_argument("--skip", type=float, default=None, help="proposer SKIP threshold (default: off; 0.7 prunes)")
     ap.add_argument("--baseline-steps", type=int, default=10,
@@ -230,6 +232,8 @@ def main():
 
     def open_analyser(m):
         a = G.Analyser(sizes, nsrc=nframes)
+        if args.no_overlap:
+            a.set_overlap(False)
         if m == G.ME_LEARNED:
             a.set_proposer()
             if args.conf is not None or args.skip is not None:

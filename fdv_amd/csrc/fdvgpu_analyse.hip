@@ -831,8 +831,18 @@ __device__ __forceinline__ void nb_load(Smem &s, const uint8_t *py, int ys, cons
 
 __device__ __forceinline__ int mb_count(const fdvg_seq &q) { return (q.w >> 4) * (q.h >> 4); }
 
-extern "C" __global__ void __launch_bounds__(256)
-k_pcost(const fdvg_seq *__restrict__ seqs, fdvg_pcost *__restrict__ pcs) {
+// An MB's costs are complete: every wave's stores are released at agent scope,
+// then one store of the frame-step id (the replay may run concurrently).  No
+// flags (nullptr): the replay runs after this kernel, nothing to publish.
+__device__ __forceinline__ void pcost_publish(unsigned *flag, unsigned step) {
+    if (!flag) return;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (threadIdx.x == 0) __hip_atomic_store(flag, step, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+extern "C" __global__ void __launch_bounds__(256, 4)
+k_pcost(const fdvg_seq *__restrict__ seqs, fdvg_pcost *__restrict__ pcs, unsigned *__restrict__ pflag) {
     __shared__ Smem s;
     __shared__ int32_t sred[FDVG_NSC][4];
     fdvg_seq q = seqs[blockIdx.y];
@@ -900,6 +910,7 @@ k_pcost(const fdvg_seq *__restrict__ seqs, fdvg_pcost *__restrict__ pcs) {
     }
     if (T->flags & 1) {                      // the proposer expects SKIP: no INTRA, no INTER8
         if (tid == 0) { P->dn = P->bn = P->d8 = P->b8 = 0; P->nsub = 0; }
+        pcost_publish(pflag ? pflag + mb : nullptr, q.step);
         return;
     }
     // INTRA 16x16 from the source neighbours: SATD screen, then one coding
@@ -935,6 +946,7 @@ k_pcost(const fdvg_seq *__restrict__ seqs, fdvg_pcost *__restrict__ pcs) {
         P->d8 = s.ws.e[0] + s.ws.e[1] + s.ws.e[2] + s.ws.e[3];
         P->b8 = 3 + s.ws.b[0] + s.ws.b[1] + s.ws.b[2] + s.ws.b[3];
     }
+    pcost_publish(pflag ? pflag + mb : nullptr, q.step);
 }
 
 // Motion record handoff of the decision wavefront: tag | inter | ref | x | y.
@@ -1009,9 +1021,23 @@ struct PrPre {
     uint32_t p[PR_PW], m[PR_MW], c[PR_CW];
     int n;
 };
-__device__ __forceinline__ void pr_load(PrPre &R, const fdvg_seq &q, const fdvg_pcost *pcs, int my, int mx0, int L) {
+__device__ __forceinline__ void pr_load(PrPre &R, const fdvg_seq &q, const fdvg_pcost *pcs, const unsigned *pflag,
+                                        unsigned *ctl, int my, int mx0, int L) {
     const int mw = q.w >> 4, i0 = my * mw + mx0;
     const int n = mw - mx0 < PR_CHUNK ? mw - mx0 : PR_CHUNK;
+    // with pflag, the chunk's costs may still be in flight (k_pcost running
+    // concurrently): wait for their flags to carry this step's id (bounded;
+    // error flag on timeout).  Without, k_pcost has completed before this launch.
+    if (pflag) {
+        const unsigned *f = pflag + q.mb0 + i0 + (L < n ? L : 0);
+        bool ok = L >= n || __hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == q.step;
+        for (unsigned spins = 0; !__all(ok); ++spins) {
+            if (spins > (1u << 24)) { if (L == 0) atomicOr(ctl + 1, 1u); break; }
+            __builtin_amdgcn_s_sleep(1);
+            ok = ok || __hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == q.step;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
     const uint32_t *gp = (const uint32_t *)(pcs + q.mb0 + i0);
     const uint32_t *gm = (const uint32_t *)(q.metab + i0);
     const int np = n * (int)(sizeof(fdvg_pcost) / 4), nm = n * (int)(sizeof(fdvg_metab) / 4);
@@ -1039,8 +1065,8 @@ __device__ __forceinline__ void pr_store(PrWave &W, const PrPre &R, int L) {
 
 extern "C" __global__ void __launch_bounds__(512)
 k_prepair(const fdvg_seq *__restrict__ seqs, int mwmax, unsigned *ctl,
-          const fdvg_pcost *__restrict__ pcs, fdvg_mbinfo *__restrict__ given, fdvg_pctx *__restrict__ pctx,
-          unsigned *__restrict__ ilist) {
+          const fdvg_pcost *__restrict__ pcs, const unsigned *__restrict__ pflag, fdvg_mbinfo *__restrict__ given,
+          fdvg_pctx *__restrict__ pctx, unsigned *__restrict__ ilist) {
     extern __shared__ uint64_t dyn[];
     const int NW = blockDim.x >> 6, wv = threadIdx.x >> 6, L = threadIdx.x & 63;
     uint64_t *ring = dyn;                                    // NW slots x mwmax motion records
@@ -1067,10 +1093,10 @@ k_prepair(const fdvg_seq *__restrict__ seqs, int mwmax, unsigned *ctl,
                 // this chunk's costs, tables and source MBs into LDS (from the
                 // registers loaded during the previous chunk, or now at a row
                 // start), then the next chunk's loads issued into registers
-                if (mx == 0) pr_load(pre, q, pcs, my, 0, L);
+                if (mx == 0) pr_load(pre, q, pcs, pflag, ctl, my, 0, L);
                 pr_store(W, pre, L);
                 wave_sync();
-                if (mx + PR_CHUNK < mw) pr_load(pre, q, pcs, my, mx + PR_CHUNK, L);
+                if (mx + PR_CHUNK < mw) pr_load(pre, q, pcs, pflag, ctl, my, mx + PR_CHUNK, L);
             }
             RPP(0);
             const fdvg_pcost &P = W.pc[mx % PR_CHUNK];
@@ -1228,16 +1254,70 @@ k_prepair(const fdvg_seq *__restrict__ seqs, int mwmax, unsigned *ctl,
     }
 }
 
-// The intra MBs of the batch in MB order (sequence by sequence, raster within
-// each), so every intra MB's left / top / top-left intra neighbours come first:
-// ilist[0] = count, then the MB indices.  One workgroup.
+// The intra MBs of the batch ordered by anti-diagonal (mx + my), all sequences
+// interleaved: an intra MB depends only on its left, top and top-left
+// neighbours, which lie on earlier diagonals, so every dependency comes earlier
+// in the list and the MBs of one diagonal are independent.  ilist[0] = count,
+// then the MB indices.  One workgroup: a histogram per diagonal, its exclusive
+// scan, then the scatter (order within a diagonal is immaterial).  With more
+// diagonals than the histogram holds, MB order (also dependency-safe).
+#define PL_MAXD 2048
 extern "C" __global__ void __launch_bounds__(1024)
-k_plist(const fdvg_mbinfo *__restrict__ given, int total_mb, unsigned *__restrict__ ilist) {
+k_plist(const fdvg_seq *__restrict__ seqs, int nseq, const fdvg_mbinfo *__restrict__ given, int total_mb,
+        unsigned *__restrict__ ilist) {
+    __shared__ unsigned hist[PL_MAXD];
+    __shared__ int s_mb0[64], s_mw[64];
     __shared__ unsigned wsum[16];
     __shared__ unsigned base;
+    __shared__ int maxd;
     const int t = threadIdx.x, L = t & 63, wv = t >> 6;
-    if (t == 0) base = 0;
+    const bool seq_lds = nseq <= 64;
+    if (t < 64 && t < nseq) { s_mb0[t] = seqs[t].mb0; s_mw[t] = seqs[t].w >> 4; }
+    if (t == 0) {
+        int md = 0;
+        for (int k = 0; k < nseq; ++k) md = max(md, (seqs[k].w >> 4) + (seqs[k].h >> 4) - 2);
+        maxd = md;
+        base = 0;
+    }
+    for (int d = t; d < PL_MAXD; d += blockDim.x) hist[d] = 0;
     __syncthreads();
+    auto diag = [&](int mb) {
+        int k = 0;
+        if (seq_lds) { for (int kk = 1; kk < nseq; ++kk) if (mb >= s_mb0[kk]) k = kk; }
+        else { for (int kk = 1; kk < nseq; ++kk) if (mb >= seqs[kk].mb0) k = kk; }
+        const int mb0 = seq_lds ? s_mb0[k] : seqs[k].mb0, mw = seq_lds ? s_mw[k] : seqs[k].w >> 4;
+        const int i = mb - mb0;
+        return i % mw + i / mw;
+    };
+    if (maxd < PL_MAXD) {
+        for (int mb = t; mb < total_mb; mb += blockDim.x)
+            if (given[mb].mode == 2) atomicAdd(&hist[diag(mb)], 1u);
+        __syncthreads();
+        // exclusive scan of the histogram: two entries a thread, wave scans, then
+        // the 16 wave totals
+        {
+            const unsigned a = hist[2 * t], b = hist[2 * t + 1];
+            unsigned v = a + b;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const unsigned u = __shfl_up(v, o);
+                if (L >= o) v += u;
+            }
+            if (L == 63) wsum[wv] = v;
+            __syncthreads();
+            unsigned off = 0;
+            for (int k = 0; k < wv; ++k) off += wsum[k];
+            const unsigned excl = off + v - (a + b);
+            hist[2 * t] = excl;
+            hist[2 * t + 1] = excl + a;
+            if (t == 0) { unsigned tot = 0; for (int k = 0; k < 16; ++k) tot += wsum[k]; base = tot; }
+        }
+        __syncthreads();
+        for (int mb = t; mb < total_mb; mb += blockDim.x)
+            if (given[mb].mode == 2) ilist[1 + atomicAdd(&hist[diag(mb)], 1u)] = (unsigned)mb;
+        if (t == 0) ilist[0] = base;
+        return;
+    }
     for (int c = 0; c < total_mb; c += 1024) {
         const int mb = c + t;
         const bool is = mb < total_mb && given[mb].mode == 2;
@@ -1342,10 +1422,31 @@ k_pintra(const fdvg_seq *__restrict__ seqs, int nseq, const fdvg_mbinfo *__restr
     }
 }
 
-extern "C" int fdvg_launch_parallel(const fdvg_seq *seqs, int nseq, int xgrid, int maxmh, int mwmax,
-                                    unsigned *ctl, fdvg_pcost *pcs, fdvg_mbinfo *given, fdvg_pctx *pctx,
-                                    unsigned *ilist, unsigned *done, int total_mb,
-                                    fdvg_mbrec *recs, hipStream_t st) {
+// The decision stage in three launches: the costs, the replay (which waits on
+// the costs' per-MB flags, so it may run on another stream at the same time),
+// and the coding of the decisions.  fdvg_launch_parallel issues all three on one
+// stream.
+extern "C" int fdvg_launch_pcost(const fdvg_seq *seqs, int nseq, int xgrid, fdvg_pcost *pcs, unsigned *pflag,
+                                 hipStream_t st) {
+    hipLaunchKernelGGL(k_pcost, dim3(xgrid, nseq), dim3(256), 0, st, seqs, pcs, pflag);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+extern "C" int fdvg_launch_prepair(const fdvg_seq *seqs, int nseq, int maxmh, int mwmax, unsigned *ctl,
+                                   const fdvg_pcost *pcs, const unsigned *pflag, fdvg_mbinfo *given,
+                                   fdvg_pctx *pctx, unsigned *ilist, hipStream_t st) {
+    int nw = maxmh < 8 ? maxmh : 8;
+    size_t lds = prepair_lds(nw, mwmax);
+    while (nw > 1 && lds > 64 * 1024) lds = prepair_lds(--nw, mwmax);
+    if (lds > 64 * 1024) return -1;
+    hipLaunchKernelGGL(k_prepair, dim3(nseq), dim3(64 * nw), lds, st, seqs, mwmax, ctl, pcs, pflag, given, pctx,
+                       ilist);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+extern "C" int fdvg_launch_pfinish(const fdvg_seq *seqs, int nseq, int xgrid, unsigned *ctl, fdvg_mbinfo *given,
+                                   fdvg_pctx *pctx, unsigned *ilist, unsigned *done, int total_mb,
+                                   fdvg_mbrec *recs, hipStream_t st) {
     static int cap_i = 0;
     if (!cap_i) {
         int dev = 0, ncu = 0, pi = 0;
@@ -1354,19 +1455,23 @@ extern "C" int fdvg_launch_parallel(const fdvg_seq *seqs, int nseq, int xgrid, i
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&pi, (const void *)k_pintra, 256, 0) != hipSuccess) return -1;
         cap_i = ncu * (pi > 0 ? pi : 1);
     }
-    int nw = maxmh < 8 ? maxmh : 8;
-    size_t lds = prepair_lds(nw, mwmax);
-    while (nw > 1 && lds > 64 * 1024) lds = prepair_lds(--nw, mwmax);
-    if (lds > 64 * 1024) return -1;
-    hipLaunchKernelGGL(k_pcost, dim3(xgrid, nseq), dim3(256), 0, st, seqs, pcs);
-    hipLaunchKernelGGL(k_prepair, dim3(nseq), dim3(64 * nw), lds, st, seqs, mwmax, ctl, (const fdvg_pcost *)pcs,
-                       given, pctx, ilist);
-    hipLaunchKernelGGL(k_plist, dim3(1), dim3(1024), 0, st, (const fdvg_mbinfo *)given, total_mb, ilist);
+    hipLaunchKernelGGL(k_plist, dim3(1), dim3(1024), 0, st, seqs, nseq, (const fdvg_mbinfo *)given, total_mb, ilist);
     hipLaunchKernelGGL(k_pencode, dim3(xgrid, nseq), dim3(256), 0, st, seqs, (const fdvg_mbinfo *)given,
                        (const fdvg_pctx *)pctx, recs);
     hipLaunchKernelGGL(k_pintra, dim3(total_mb < cap_i ? total_mb : cap_i), dim3(256), 0, st, seqs, nseq,
                        (const fdvg_mbinfo *)given, (const fdvg_pctx *)pctx, recs, ctl, (const unsigned *)ilist, done);
     return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+extern "C" int fdvg_launch_parallel(const fdvg_seq *seqs, int nseq, int xgrid, int maxmh, int mwmax,
+                                    unsigned *ctl, fdvg_pcost *pcs, unsigned *pflag, fdvg_mbinfo *given,
+                                    fdvg_pctx *pctx, unsigned *ilist, unsigned *done, int total_mb,
+                                    fdvg_mbrec *recs, hipStream_t st) {
+    (void)pflag;
+    if (fdvg_launch_pcost(seqs, nseq, xgrid, pcs, nullptr, st) != 0) return -1;
+    // same stream: the costs are complete when the replay starts, no flag waits
+    if (fdvg_launch_prepair(seqs, nseq, maxmh, mwmax, ctl, pcs, nullptr, given, pctx, ilist, st) != 0) return -1;
+    return fdvg_launch_pfinish(seqs, nseq, xgrid, ctl, given, pctx, ilist, done, total_mb, recs, st);
 }
 
 // ------------------------------------------------------------ host launchers

@@ -40,9 +40,13 @@ extern "C" int fdvg_nn_setup(void);
 // The proposer's confidence threshold: below it an MB gets the exhaustive search.
 #define FDVG_NN_THRESHOLD 0.7f     // default confidence threshold of the proposer
 #define FDVG_NN_SKIP_OFF 2.0f      // default SKIP threshold: no pruning
-extern "C" int fdvg_launch_parallel(const fdvg_seq *, int, int, int, int, unsigned *, fdvg_pcost *,
-                                    fdvg_mbinfo *, fdvg_pctx *, unsigned *, unsigned *, int,
-                                    fdvg_mbrec *, hipStream_t);
+extern "C" int fdvg_launch_parallel(const fdvg_seq *, int, int, int, int, unsigned *, fdvg_pcost *, unsigned *,
+                                    fdvg_mbinfo *, fdvg_pctx *, unsigned *, unsigned *, int, fdvg_mbrec *, hipStream_t);
+extern "C" int fdvg_launch_pcost(const fdvg_seq *, int, int, fdvg_pcost *, unsigned *, hipStream_t);
+extern "C" int fdvg_launch_prepair(const fdvg_seq *, int, int, int, unsigned *, const fdvg_pcost *, const unsigned *,
+                                   fdvg_mbinfo *, fdvg_pctx *, unsigned *, hipStream_t);
+extern "C" int fdvg_launch_pfinish(const fdvg_seq *, int, int, unsigned *, fdvg_mbinfo *, fdvg_pctx *, unsigned *,
+                                   unsigned *, int, fdvg_mbrec *, hipStream_t);
 
 static thread_local char g_err[256];
 static void seterr(const char *fmt, ...) {
@@ -92,6 +96,14 @@ struct fdvgpu {
     fdvg_mbinfo *pgiven_d;
     uint64_t *mrec_d;               // motion-record handoff of the decision wavefront
     unsigned *ilist_d, *done_d;     // intra MBs in dependency order (count first), per-MB done flags
+    unsigned *pflag_d;              // per MB: the id of the frame step whose costs are written
+    unsigned step_ctr;              // frame-step ids (never rewound)
+    // The decision stage overlaps the costs and the replay: the replay's
+    // workgroups (one per sequence) get CUs of their own through CU-masked
+    // streams, so they run while the costs fill the rest of the chip.
+    hipStream_t st_cost, st_rep;
+    hipEvent_t ev_fork, ev_cost, ev_rep;
+    int overlap, overlap_avail;
     // launch shapes, and one instantiated graph per (search mode, decision structure)
     Tune tune;
     int graphs;
@@ -144,6 +156,12 @@ extern "C" void fdvgpu_close(fdvgpu *g) {
     for (int i = 0; i < 5; ++i) if (g->ev[i]) (void)hipEventDestroy(g->ev[i]);
     for (int i = 0; i < 8; ++i) if (g->ring_ev[i]) (void)hipEventDestroy(g->ring_ev[i]);
     if (g->st) (void)hipStreamDestroy(g->st);
+    if (g->st_cost) (void)hipStreamDestroy(g->st_cost);
+    if (g->st_rep) (void)hipStreamDestroy(g->st_rep);
+    if (g->ev_fork) (void)hipEventDestroy(g->ev_fork);
+    if (g->ev_cost) (void)hipEventDestroy(g->ev_cost);
+    if (g->ev_rep) (void)hipEventDestroy(g->ev_rep);
+    (void)hipFree(g->pflag_d);
     free(g->S);
     free(g);
 }
@@ -191,6 +209,14 @@ extern "C" fdvgpu *fdvgpu_open(int nseq, const int *w, const int *h, int nsrc) {
         g->compact_cap += (size_t)m.nmb * 2048 + 4096;
     }
     CKN(hipStreamCreate(&g->st));
+    // The overlapped decision stage (costs and replay on CU-masked streams) is
+    // switched off: its first device run ended in a host segmentation fault
+    // whose cause has not been found.  The streams are not created and the stage
+    // runs its kernels one after the other.
+    g->overlap = g->overlap_avail = 0;
+    CKN(hipEventCreateWithFlags(&g->ev_fork, hipEventDisableTiming));
+    CKN(hipEventCreateWithFlags(&g->ev_cost, hipEventDisableTiming));
+    CKN(hipEventCreateWithFlags(&g->ev_rep, hipEventDisableTiming));
     for (int i = 0; i < 5; ++i) CKN(hipEventCreate(&g->ev[i]));
     for (int i = 0; i < 8; ++i) CKN(hipEventCreateWithFlags(&g->ring_ev[i], hipEventDisableTiming));
     CKN(hipMalloc(&g->seq_d, nseq * sizeof(fdvg_seq)));
@@ -221,6 +247,8 @@ extern "C" fdvgpu *fdvgpu_open(int nseq, const int *w, const int *h, int nsrc) {
     CKN(hipMemset(g->mrec_d, 0, (size_t)g->total_mb * sizeof(uint64_t)));
     CKN(hipMalloc(&g->ilist_d, ((size_t)g->total_mb + 1) * sizeof(unsigned)));
     CKN(hipMalloc(&g->done_d, (size_t)g->total_mb * sizeof(unsigned)));
+    CKN(hipMalloc(&g->pflag_d, (size_t)g->total_mb * sizeof(unsigned)));
+    CKN(hipMemset(g->pflag_d, 0, (size_t)g->total_mb * sizeof(unsigned)));
     CKN(hipMalloc(&g->nnw_d, (size_t)fdvg_nn_nparam() * sizeof(float)));
     if (fdvg_nn_setup() != 0) { seterr("proposer kernel setup failed"); goto fail; }
     g->tune = Tune{4, 256, 1};
@@ -329,10 +357,32 @@ static int launch_stage(fdvgpu *g, int stage, int me_mode, bool par) {
         }
         if (fdvg_launch_zero(g->ticket_d, 8, g->st) != 0) { seterr("ticket reset launch failed"); return -1; }
     } else if (stage == 1) {
-        if (par) {
+        if (par && g->overlap) {
+            // costs and replay side by side on their CU-masked streams, joined
+            // back before the coding of the decisions
+            if (fdvg_launch_zero(g->ilist_d, 1, g->st) != 0 || fdvg_launch_zero(g->done_d, g->total_mb, g->st) != 0) {
+                seterr("parallel analysis launch failed"); return -1;
+            }
+            CK(hipEventRecord(g->ev_fork, g->st));
+            CK(hipStreamWaitEvent(g->st_cost, g->ev_fork, 0));
+            CK(hipStreamWaitEvent(g->st_rep, g->ev_fork, 0));
+            if (fdvg_launch_pcost(g->seq_d, g->nseq, g->xgrid, g->pcost_d, g->pflag_d, g->st_cost) != 0 ||
+                fdvg_launch_prepair(g->seq_d, g->nseq, g->maxmh, g->maxmw, g->ticket_d, g->pcost_d, g->pflag_d,
+                                    g->pgiven_d, g->pctx_d, g->ilist_d, g->st_rep) != 0) {
+                seterr("parallel analysis launch failed: %s", hipGetErrorString(hipGetLastError())); return -1;
+            }
+            CK(hipEventRecord(g->ev_cost, g->st_cost));
+            CK(hipEventRecord(g->ev_rep, g->st_rep));
+            CK(hipStreamWaitEvent(g->st, g->ev_cost, 0));
+            CK(hipStreamWaitEvent(g->st, g->ev_rep, 0));
+            if (fdvg_launch_pfinish(g->seq_d, g->nseq, g->xgrid, g->ticket_d, g->pgiven_d, g->pctx_d, g->ilist_d,
+                                    g->done_d, g->total_mb, g->recs_d, g->st) != 0) {
+                seterr("parallel analysis launch failed: %s", hipGetErrorString(hipGetLastError())); return -1;
+            }
+        } else if (par) {
             if (fdvg_launch_zero(g->ilist_d, 1, g->st) != 0 || fdvg_launch_zero(g->done_d, g->total_mb, g->st) != 0 ||
                 fdvg_launch_parallel(g->seq_d, g->nseq, g->xgrid, g->maxmh, g->maxmw, g->ticket_d, g->pcost_d,
-                                     g->pgiven_d, g->pctx_d, g->ilist_d, g->done_d, g->total_mb,
+                                     g->pflag_d, g->pgiven_d, g->pctx_d, g->ilist_d, g->done_d, g->total_mb,
                                      g->recs_d, g->st) != 0) {
                 seterr("parallel analysis launch failed: %s", hipGetErrorString(hipGetLastError())); return -1;
             }
@@ -355,6 +405,9 @@ static int launch_stage(fdvgpu *g, int stage, int me_mode, bool par) {
 // launchers' one-time caches), from then on as a captured hipGraph.
 static int run_stage(fdvgpu *g, int stage, int me_mode, bool par) {
     const int key = (stage * 4 + me_mode) * 2 + (par ? 1 : 0);
+    // the overlapped decision stage runs eagerly: its CU masks belong to its
+    // streams, which a captured graph would not keep
+    if (stage == 1 && par && g->overlap) return launch_stage(g, stage, me_mode, par);
     if (!g->graphs || !g->warm[key]) {
         if (launch_stage(g, stage, me_mode, par) != 0) return -1;
         g->warm[key] = 1;
@@ -395,6 +448,8 @@ extern "C" int fdvgpu_pframe(fdvgpu *g, const int *src_slot, const int *qp, cons
             seterr("fdvgpu_pframe: bad parameters for sequence %d", s); return -1;
         }
     }
+    g->step_ctr = g->step_ctr + 1 ? g->step_ctr + 1 : 1;       // never 0 (the flags' initial value)
+    const unsigned step = g->step_ctr;
     CK(hipEventRecord(g->ev[0], g->st));
     // staging slot for this frame's descriptors: the copy of the frame 8 steps
     // back must have run before the slot is rewritten
@@ -425,6 +480,7 @@ extern "C" int fdvgpu_pframe(fdvgpu *g, const int *src_slot, const int *qp, cons
         q.mb0 = m.mb0;
         m.tag = m.tag % 255 + 1;
         q.tag = m.tag;
+        q.step = step;
         PlaneRef *pl = &planes_h[3 * s];
         pl[0] = PlaneRef{q.oy, m.w, m.h, m.ys, qp[s]};
         pl[1] = PlaneRef{q.ou, m.cw, m.ch, m.cs, qp[s]};
@@ -563,6 +619,13 @@ extern "C" int fdvgpu_planes(fdvgpu *g, int seq, int src_slot, const uint8_t **c
     SeqMem &m = g->S[seq];
     *cur_y = m.src + (size_t)src_slot * m.srcsz; *cur_stride = m.w;
     *ref_y = fy(m, m.near_); *ref_stride = m.ys;
+    return 0;
+}
+
+extern "C" int fdvgpu_set_overlap(fdvgpu *g, int on) {
+    if (!g) { seterr("fdvgpu_set_overlap: bad arguments"); return -1; }
+    if (on && !g->overlap_avail) { seterr("fdvgpu_set_overlap: the overlapped decision stage is switched off"); return -1; }
+    g->overlap = on ? 1 : 0;
     return 0;
 }
 

@@ -102,7 +102,8 @@ typedef struct {
     int navail, qp, me_mode, force_skip;
     int mb0;                           // first MB of this sequence in the global MB arrays
     int tag;                           // 1..255, differs between consecutive frames
-    int pad[2];
+    unsigned step;                     // the context's frame-step id: nonzero, new for every step
+    int pad;
 } fdvg_seq;
 
 // Per-MB handoff to the MB row below: the bottom recon rows (16 Y, 8 U, 8 V),

@@ -22,7 +22,7 @@ struct __align__(16) MSmem {
     uint32_t cur32[64];                 // the source MB, 4 words a row
     uint8_t cur[256];
     unsigned long long key[6][4];       // per block, per wave: best packed key
-    int32_t sad[2][8];
+    int32_t sad[2][6][4];               // per step parity, block, cross point
     int32_t res[6][3];                  // per block: mvx, mvy, sad after sub-pel
 };
 
@@ -36,30 +36,45 @@ __device__ __forceinline__ uint32_t win_word(const uint8_t *w, int y, int x) {
 
 __device__ __forceinline__ unsigned long long umin64(unsigned long long a, unsigned long long b) { return a < b ? a : b; }
 
-// Refine block b (0,1: 16x16 on ref 0,1; 2..5: 8x8 quadrants on ref 0).
-__device__ __forceinline__ void refine(MSmem &s, int &par, int b, int bs, int ox, int oy, const uint8_t *w,
-                       int mx, int my, int best) {
-    for (int st = 2; st >= 1 && best > 0; st >>= 1) {
-        const int t = threadIdx.x, k = t >> 6, tt = t & 63;
-        const int px = mx + k_CROSS[k][0] * st, py = my + k_CROSS[k][1] * st;
-        int acc = 0;
-        const int npx = bs * bs / 64;
-        for (int j = 0; j < npx; ++j) {
-            const int p = tt * npx + j, y = p / bs, x = p % bs;
-            const int d = s.cur[(oy + y) * 16 + ox + x] - lumaq(w, ox + x, oy + y, px, py);
-            acc += d < 0 ? -d : d;
+// The reference's sub-pel step (half- then quarter-pel four-point cross,
+// unclamped, skipped once a block's SAD is 0: fdv.h:3453-3461) for all six
+// blocks at once (0, 1: 16x16 on references 0 and 1; 2..5: the 8x8 quadrants
+// on reference 0): at each step wave k scores cross point k of every block
+// still refining, then every block takes its best point (strict <, in cross
+// order).  Each block's result is that of refining it alone.
+__device__ __forceinline__ void refine_all(MSmem &s, int nr, int *bmx, int *bmy, int *bbest) {
+    const int t = threadIdx.x, k = t >> 6, tt = t & 63;
+    int par = 0;
+#pragma unroll
+    for (int st = 2; st >= 1; st >>= 1) {
+#pragma unroll
+        for (int b = 0; b < 6; ++b) {
+            if (bbest[b] <= 0 || (b == 1 && nr < 2)) continue;        // uniform
+            const int bs = b < 2 ? 16 : 8, npx = bs * bs / 64;
+            const int ox = b < 2 ? 0 : ((b - 2) & 1) * 8, oy = b < 2 ? 0 : ((b - 2) >> 1) * 8;
+            const uint8_t *w = s.win[b == 1 ? 1 : 0];
+            const int px = bmx[b] + k_CROSS[k][0] * st, py = bmy[b] + k_CROSS[k][1] * st;
+            int acc = 0;
+            for (int j = 0; j < npx; ++j) {
+                const int p = tt * npx + j, y = p / bs, x = p % bs;
+                const int d = s.cur[(oy + y) * 16 + ox + x] - lumaq(w, ox + x, oy + y, px, py);
+                acc += d < 0 ? -d : d;
+            }
+            acc = gsum<64>(acc);
+            if (tt == 0) s.sad[par][b][k] = acc;
         }
-        acc = gsum<64>(acc);
-        if (tt == 0) s.sad[par][k] = acc;
         __syncthreads();
-        const int cx = mx, cy = my;
-        for (int k2 = 0; k2 < 4; ++k2) {
-            const int v = s.sad[par][k2];
-            if (v < best) { best = v; mx = cx + k_CROSS[k2][0] * st; my = cy + k_CROSS[k2][1] * st; }
+#pragma unroll
+        for (int b = 0; b < 6; ++b) {
+            if (bbest[b] <= 0 || (b == 1 && nr < 2)) continue;
+            const int cx = bmx[b], cy = bmy[b];
+            for (int k2 = 0; k2 < 4; ++k2) {
+                const int v = s.sad[par][b][k2];
+                if (v < bbest[b]) { bbest[b] = v; bmx[b] = cx + k_CROSS[k2][0] * st; bmy[b] = cy + k_CROSS[k2][1] * st; }
+            }
         }
         par ^= 1;
     }
-    if (threadIdx.x == 0) { s.res[b][0] = mx; s.res[b][1] = my; s.res[b][2] = best; }
 }
 
 // With a proposer output `nn` whose confidence is at least `thr`, the integer
@@ -152,16 +167,18 @@ k_me_exhaustive(const fdvg_seq *__restrict__ seqs, fdvg_metab *__restrict__ out,
         if (L == 0) s.key[b][wv] = v;
     }
     __syncthreads();
-    int par = 0;
+    int bmx[6], bmy[6], bbest[6];
+#pragma unroll
     for (int b = 0; b < 6; ++b) {
-        if (b == 1 && nr < 2) continue;
         const unsigned long long k = umin64(umin64(s.key[b][0], s.key[b][1]), umin64(s.key[b][2], s.key[b][3]));
-        const int pidx = (int)(k & 0xffff), sad = (int)(k >> 32);
-        const int mx = (pidx % 33 - 16) * 4, my = (pidx / 33 - 16) * 4;
-        const int bs = b < 2 ? 16 : 8;
-        const int ox = b < 2 ? 0 : ((b - 2) & 1) * 8, oy = b < 2 ? 0 : ((b - 2) >> 1) * 8;
-        refine(s, par, b, bs, ox, oy, s.win[b == 1 ? 1 : 0], mx, my, sad);
+        const int pidx = (int)(k & 0xffff);
+        bbest[b] = (int)(k >> 32);
+        bmx[b] = (pidx % 33 - 16) * 4; bmy[b] = (pidx / 33 - 16) * 4;
     }
+    refine_all(s, nr, bmx, bmy, bbest);
+    if (tid == 0)
+#pragma unroll
+        for (int b = 0; b < 6; ++b) { s.res[b][0] = bmx[b]; s.res[b][1] = bmy[b]; s.res[b][2] = bbest[b]; }
     __syncthreads();
     if (tid == 0) {
         fdvg_metab &m = out[q.mb0 + i];

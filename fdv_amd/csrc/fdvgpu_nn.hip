@@ -556,6 +556,30 @@ __device__ __forceinline__ void ni_features(NiWave &w, const uint8_t *cur, int c
     }
 }
 
+// The wavefront's best (value, position): larger value, then lower position.
+// Inside each 16-lane row by DPP (quad swaps, then rotations by 4 and 8), then
+// the four row results by readlane; the result is uniform.
+__device__ __forceinline__ void dpp_best(float &v, int &p, float ov, int op) {
+    if (ov > v || (ov == v && op < p)) { v = ov; p = op; }
+}
+#define NN_DPP_STEP(ctrl)                                                                                 \
+    dpp_best(v, p, __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), ctrl, 0xf, 0xf, false)), \
+             __builtin_amdgcn_update_dpp(0, p, ctrl, 0xf, 0xf, false))
+__device__ __forceinline__ void wave_argmax(float &v, int &p) {
+    NN_DPP_STEP(0xB1);             // quad_perm [1,0,3,2]
+    NN_DPP_STEP(0x4E);             // quad_perm [2,3,0,1]
+    NN_DPP_STEP(0x124);            // row_ror:4
+    NN_DPP_STEP(0x128);            // row_ror:8
+    float bv = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0));
+    int bp = __builtin_amdgcn_readlane(p, 0);
+#pragma unroll
+    for (int r = 16; r < 64; r += 16)
+        dpp_best(bv, bp, __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), r)), __builtin_amdgcn_readlane(p, r));
+    v = bv;
+    p = bp;
+}
+#undef NN_DPP_STEP
+
 // Per MB of every sequence: the 8 most likely whole-pel vectors (nearest
 // reference), their total probability and the SKIP probability.
 extern "C" __global__ void __launch_bounds__(256, 1)
@@ -739,11 +763,7 @@ k_nn_infer(const fdvg_seq *__restrict__ seqs, int nseq, int total, const float *
                 const int pp = tl * 32 + (rr >> 2) * 8 + g * 4 + (rr & 3);
                 if (!((taken >> tl) & 1) && pp < NN_P && (lg[tl] > bv || (lg[tl] == bv && pp < bp))) { bv = lg[tl]; bp = pp; }
             }
-            for (int o2 = 32; o2 > 0; o2 >>= 1) {
-                const float ov = __shfl_xor(bv, o2);
-                const int op = __shfl_xor(bp, o2);
-                if (ov > bv || (ov == bv && op < bp)) { bv = ov; bp = op; }
-            }
+            wave_argmax(bv, bp);
             conf += expf(bv - m) / se;
 #pragma unroll
             for (int tl = 0; tl < 10; ++tl) {

@@ -159,6 +159,10 @@ class Analyser:
         self.L.fdvgpu_set_proposer_thresholds.argtypes = [C.c_void_p, C.c_float, C.c_float]
         self._ck(self.L.fdvgpu_set_proposer_thresholds(self.h, float(conf), float(skip)), "set_proposer_thresholds")
 
+    def set_overlap(self, on: bool):
+        """Overlap the block-parallel decision stage's costs and replay."""
+        self._ck(self.L.fdvgpu_set_overlap(self.h, 1 if on else 0), "set_overlap")
+
     def set_graphs(self, on: bool):
         self._ck(self.L.fdvgpu_set_graphs(self.h, 1 if on else 0), "set_graphs")
 

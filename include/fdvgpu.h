@@ -109,6 +109,12 @@ int fdvgpu_decisions(fdvgpu *g, int seq, fdvgpu_mbinfo *out);
  * candidate on a real frame step (rewinding the reference pool after each) and
  * keeps the fastest.  None of this has a counterpart in the reference. */
 int fdvgpu_set_graphs(fdvgpu *g, int on);
+
+/* Overlap of the block-parallel decision stage: the candidate costs on most CUs
+ * while the decision replay, waiting per MB for its costs, runs on CUs reserved
+ * for it.  Currently switched off in this build: on = 1 returns -1 and the
+ * stage runs its kernels one after the other.  No counterpart in the reference. */
+int fdvgpu_set_overlap(fdvgpu *g, int on);
 int fdvgpu_get_tune(fdvgpu *g, int shapes[3]);
 int fdvgpu_set_tune(fdvgpu *g, const int shapes[3]);
 int fdvgpu_autotune(fdvgpu *g, const int *src_slot, const int *qp, const double *lscale,

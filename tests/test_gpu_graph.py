@@ -70,3 +70,42 @@ def test_autotune_changes_no_result():
             assert a.streams(k) == b.streams(k) and np.array_equal(a.recon(k), b.recon(k)), f"frame {f} seq {k}"
     a.close()
     b.close()
+
+
+@pytest.mark.skip(reason="overlapped decision stage switched off: its first device run ended in a host "
+                         "segmentation fault whose cause has not been found")
+@pytest.mark.parametrize("mode", [G.ME_PARALLEL, G.ME_LEARNED])
+def test_overlapped_decision_stage_matches_sequential(mode):
+    """The decision stage with the costs and the replay running side by side
+    (CU-masked streams, per-MB flags) gives the results of running them one
+    after the other."""
+    scenes = _scenes()
+    sizes = [(s.w, s.h) for s in scenes]
+    nfr = 5
+    frames = [[s.frame(i) for i in range(nfr)] for s in scenes]
+    outs = []
+    for overlap in (False, True):
+        an = G.Analyser(sizes, nsrc=nfr)
+        try:
+            an.set_overlap(overlap)
+        except G.GpuError:
+            an.close()
+            pytest.skip("CU-masked streams unavailable on this device")
+        if mode == G.ME_LEARNED:
+            an.set_proposer()
+        for k in range(2):
+            an.put_ref(k, 0, frames[k][0])
+            for f in range(1, nfr):
+                an.put_source(k, f, frames[k][f])
+        res = []
+        for f in range(1, nfr):
+            say(f"  overlap {overlap} mode {mode} frame {f}")
+            an.pframe([f, f], [24, 30], navail=[1 if f == 1 else 2] * 2, me_mode=mode)
+            an.fetch()
+            res.append([(an.streams(k), an.recon(k).copy()) for k in range(2)])
+        an.close()
+        outs.append(res)
+    for f in range(nfr - 1):
+        for k in range(2):
+            assert outs[0][f][k][0] == outs[1][f][k][0], f"frame {f + 1} seq {k}: streams"
+            assert np.array_equal(outs[0][f][k][1], outs[1][f][k][1]), f"frame {f + 1} seq {k}: recon"
