@@ -1372,10 +1372,85 @@ k_pencode(const fdvg_seq *__restrict__ seqs, const fdvg_mbinfo *__restrict__ giv
 // Decided intra MBs, in the order of ilist (every intra neighbour first): each
 // waits for its left, top and top-left neighbours when those are intra, then
 // reads the unfiltered reconstruction around it.
+// Intra handoff: what an intra MB's intra neighbours read of it, its bottom
+// row and right column (luma 16 + 16, U and V 8 + 8 each), as 16 64-bit words
+// of 4 payload bytes and the 32-bit frame-step id.  Relaxed agent-scope atomics
+// keep them coherent across XCDs without cache-wide fences; the reader spins on
+// the words it needs until their id is this step's (bounded, error flag).
+//   words 0-3 luma bottom row, 4-5 U bottom row, 6-7 V bottom row,
+//   8-11 luma right column, 12-13 U right column, 14-15 V right column.
+#define IH_WORDS 16
+__device__ __forceinline__ void ih_publish(uint64_t *ih, const uint8_t *yrec, const uint8_t *crec, unsigned step) {
+    const int w = threadIdx.x;
+    if (w >= IH_WORDS) return;
+    uint32_t v = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        int b;
+        if (w < 4) b = yrec[15 * 16 + 4 * w + j];
+        else if (w < 8) b = crec[(w >= 6 ? 64 : 0) + 7 * 8 + 4 * ((w - 4) & 1) + j];
+        else if (w < 12) b = yrec[(4 * (w - 8) + j) * 16 + 15];
+        else b = crec[(w >= 14 ? 64 : 0) + (4 * ((w - 12) & 1) + j) * 8 + 7];
+        v |= (uint32_t)b << (8 * j);
+    }
+    __hip_atomic_store(ih + w, ((uint64_t)step << 32) | v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int ih_byte(const uint64_t *ih, int w, int byte, unsigned step, unsigned *ctl) {
+    uint64_t v = __hip_atomic_load(ih + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (unsigned spins = 0; (unsigned)(v >> 32) != step; ++spins) {
+        if (spins > (1u << 24)) { atomicOr(ctl + 1, 1u); break; }
+        __builtin_amdgcn_s_sleep(1);
+        v = __hip_atomic_load(ih + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    return (int)((v >> (8 * byte)) & 0xff);
+}
+
+// The neighbours of an intra MB in the block-parallel pass (the layout of
+// nb_load): intra neighbours from their handoff words, the others (coded by
+// an earlier kernel) from the reconstruction planes.
+__device__ __forceinline__ void nb_load_ih(Smem &s, const fdvg_seq &q, const fdvg_mbinfo *gq, const uint64_t *ih,
+                                           unsigned *ctl, int mx, int my, int mw, int bx, int by) {
+    const int tid = threadIdx.x, cbx = bx >> 1, cby = by >> 1, ht = my > 0, hl = mx > 0;
+    const int i_top = (my - 1) * mw + mx, i_left = my * mw + mx - 1, i_tl = (my - 1) * mw + mx - 1;
+    const bool top_i = ht && gq[i_top].mode == 2, left_i = hl && gq[i_left].mode == 2;
+    const bool tl_i = ht && hl && gq[i_tl].mode == 2;
+    const uint64_t *ih0 = ih + (size_t)q.mb0 * IH_WORDS;
+    const unsigned step = q.step;
+    const uint8_t *py = q.oy, *pu = q.ou, *pv = q.ov;
+    const int ys = q.ys, cs = q.cs;
+    if (tid < 16) {
+        s.hbytes[1 + tid] = !ht ? 0 : top_i ? ih_byte(ih0 + (size_t)i_top * IH_WORDS, tid >> 2, tid & 3, step, ctl)
+                                           : py[(ptrdiff_t)(by - 1) * ys + bx + tid];
+    } else if (tid == 16) {
+        s.hbytes[0] = !(ht && hl) ? 128 : tl_i ? ih_byte(ih0 + (size_t)i_tl * IH_WORDS, 3, 3, step, ctl)
+                                               : py[(ptrdiff_t)(by - 1) * ys + bx - 1];
+    } else if (tid >= 32 && tid < 48) {
+        const int k = tid - 32;
+        s.lcol[k] = !hl ? 0 : left_i ? ih_byte(ih0 + (size_t)i_left * IH_WORDS, 8 + (k >> 2), k & 3, step, ctl)
+                                     : py[(ptrdiff_t)(by + k) * ys + bx - 1];
+    } else if (tid >= 64 && tid < 82) {
+        const int pl = (tid - 64) / 9, k = (tid - 64) % 9;
+        const uint8_t *p = pl ? pv : pu;
+        if (k == 0)
+            s.ctop[pl][0] = !(ht && hl) ? 128 : tl_i ? ih_byte(ih0 + (size_t)i_tl * IH_WORDS, pl ? 7 : 5, 3, step, ctl)
+                                                     : p[(ptrdiff_t)(cby - 1) * cs + cbx - 1];
+        else
+            s.ctop[pl][k] = !ht ? 0 : top_i ? ih_byte(ih0 + (size_t)i_top * IH_WORDS, (pl ? 6 : 4) + ((k - 1) >> 2),
+                                                      (k - 1) & 3, step, ctl)
+                                            : p[(ptrdiff_t)(cby - 1) * cs + cbx + k - 1];
+    } else if (tid >= 96 && tid < 112) {
+        const int pl = (tid - 96) >> 3, k = (tid - 96) & 7;
+        const uint8_t *p = pl ? pv : pu;
+        s.lcol[16 + 8 * pl + k] = !hl ? 0 : left_i ? ih_byte(ih0 + (size_t)i_left * IH_WORDS, (pl ? 14 : 12) + (k >> 2),
+                                                             k & 3, step, ctl)
+                                                   : p[(ptrdiff_t)(cby + k) * cs + cbx - 1];
+    }
+}
+
 extern "C" __global__ void __launch_bounds__(256)
 k_pintra(const fdvg_seq *__restrict__ seqs, int nseq, const fdvg_mbinfo *__restrict__ given,
          const fdvg_pctx *__restrict__ pctx, fdvg_mbrec *__restrict__ recs, unsigned *ctl,
-         const unsigned *__restrict__ ilist, unsigned *__restrict__ done) {
+         const unsigned *__restrict__ ilist, uint64_t *__restrict__ ih) {
     __shared__ Smem s;
     __shared__ int job_s;
     const int tid = threadIdx.x;
@@ -1392,33 +1467,17 @@ k_pintra(const fdvg_seq *__restrict__ seqs, int nseq, const fdvg_mbinfo *__restr
         fdvg_seq q = seqs[sq];
         q.given = given + q.mb0;
         const int i = mb - q.mb0, mw = q.w >> 4, mx = i % mw, my = i / mw, bx = mx * 16, by = my * 16;
-        if (tid < 3) {
-            // left, top, top-left
-            const int ax = tid == 0 ? mx - 1 : tid == 1 ? mx : mx - 1, ay = tid == 0 ? my : my - 1;
-            if (ax >= 0 && ay >= 0 && given[q.mb0 + ay * mw + ax].mode == 2) {
-                const unsigned *p = done + q.mb0 + ay * mw + ax;
-                unsigned v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                for (unsigned spins = 0; v == 0; ++spins) {
-                    if (spins > (1u << 24)) { atomicOr(ctl + 1, 1u); break; }
-                    __builtin_amdgcn_s_sleep(1);
-                    v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                }
-            }
-        }
-        __syncthreads();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");   // every wave reads the neighbours
         mb_stage(s, q, mb, bx, by);
-        nb_load(s, q.oy, q.ys, q.ou, q.ov, q.cs, bx, by, my > 0, mx > 0, true);
+        nb_load_ih(s, q, given + q.mb0, ih, ctl, mx, my, mw, bx, by);
         MbCtx c;
         ctx_from(c, pctx[mb], my > 0, mx > 0);
         __syncthreads();
         int par = 0;
         MbDec D;
         mb_decide(s, q, c, &given[mb], nullptr, par, D);
-        mb_finish(s, q, mb, bx, by, c, D, &recs[mb]);
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");   // every wave's reconstruction stores
-        __syncthreads();
-        if (tid == 0) __hip_atomic_store(done + mb, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint8_t *yrec = mb_finish(s, q, mb, bx, by, c, D, &recs[mb]);
+        __syncthreads();                       // the chroma reconstruction is complete
+        ih_publish(ih + (size_t)mb * IH_WORDS, yrec, s.crec, q.step);
     }
 }
 
@@ -1445,7 +1504,7 @@ extern "C" int fdvg_launch_prepair(const fdvg_seq *seqs, int nseq, int maxmh, in
 }
 
 extern "C" int fdvg_launch_pfinish(const fdvg_seq *seqs, int nseq, int xgrid, unsigned *ctl, fdvg_mbinfo *given,
-                                   fdvg_pctx *pctx, unsigned *ilist, unsigned *done, int total_mb,
+                                   fdvg_pctx *pctx, unsigned *ilist, uint64_t *ih, int total_mb,
                                    fdvg_mbrec *recs, hipStream_t st) {
     static int cap_i = 0;
     if (!cap_i) {
@@ -1459,19 +1518,19 @@ extern "C" int fdvg_launch_pfinish(const fdvg_seq *seqs, int nseq, int xgrid, un
     hipLaunchKernelGGL(k_pencode, dim3(xgrid, nseq), dim3(256), 0, st, seqs, (const fdvg_mbinfo *)given,
                        (const fdvg_pctx *)pctx, recs);
     hipLaunchKernelGGL(k_pintra, dim3(total_mb < cap_i ? total_mb : cap_i), dim3(256), 0, st, seqs, nseq,
-                       (const fdvg_mbinfo *)given, (const fdvg_pctx *)pctx, recs, ctl, (const unsigned *)ilist, done);
+                       (const fdvg_mbinfo *)given, (const fdvg_pctx *)pctx, recs, ctl, (const unsigned *)ilist, ih);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 extern "C" int fdvg_launch_parallel(const fdvg_seq *seqs, int nseq, int xgrid, int maxmh, int mwmax,
                                     unsigned *ctl, fdvg_pcost *pcs, unsigned *pflag, fdvg_mbinfo *given,
-                                    fdvg_pctx *pctx, unsigned *ilist, unsigned *done, int total_mb,
+                                    fdvg_pctx *pctx, unsigned *ilist, uint64_t *ih, int total_mb,
                                     fdvg_mbrec *recs, hipStream_t st) {
     (void)pflag;
     if (fdvg_launch_pcost(seqs, nseq, xgrid, pcs, nullptr, st) != 0) return -1;
     // same stream: the costs are complete when the replay starts, no flag waits
     if (fdvg_launch_prepair(seqs, nseq, maxmh, mwmax, ctl, pcs, nullptr, given, pctx, ilist, st) != 0) return -1;
-    return fdvg_launch_pfinish(seqs, nseq, xgrid, ctl, given, pctx, ilist, done, total_mb, recs, st);
+    return fdvg_launch_pfinish(seqs, nseq, xgrid, ctl, given, pctx, ilist, ih, total_mb, recs, st);
 }
 
 // ------------------------------------------------------------ host launchers

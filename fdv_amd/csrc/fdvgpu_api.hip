@@ -41,12 +41,12 @@ extern "C" int fdvg_nn_setup(void);
 #define FDVG_NN_THRESHOLD 0.7f     // default confidence threshold of the proposer
 #define FDVG_NN_SKIP_OFF 2.0f      // default SKIP threshold: no pruning
 extern "C" int fdvg_launch_parallel(const fdvg_seq *, int, int, int, int, unsigned *, fdvg_pcost *, unsigned *,
-                                    fdvg_mbinfo *, fdvg_pctx *, unsigned *, unsigned *, int, fdvg_mbrec *, hipStream_t);
+                                    fdvg_mbinfo *, fdvg_pctx *, unsigned *, uint64_t *, int, fdvg_mbrec *, hipStream_t);
 extern "C" int fdvg_launch_pcost(const fdvg_seq *, int, int, fdvg_pcost *, unsigned *, hipStream_t);
 extern "C" int fdvg_launch_prepair(const fdvg_seq *, int, int, int, unsigned *, const fdvg_pcost *, const unsigned *,
                                    fdvg_mbinfo *, fdvg_pctx *, unsigned *, hipStream_t);
 extern "C" int fdvg_launch_pfinish(const fdvg_seq *, int, int, unsigned *, fdvg_mbinfo *, fdvg_pctx *, unsigned *,
-                                   unsigned *, int, fdvg_mbrec *, hipStream_t);
+                                   uint64_t *, int, fdvg_mbrec *, hipStream_t);
 
 static thread_local char g_err[256];
 static void seterr(const char *fmt, ...) {
@@ -95,7 +95,8 @@ struct fdvgpu {
     fdvg_pctx *pctx_d;
     fdvg_mbinfo *pgiven_d;
     uint64_t *mrec_d;               // motion-record handoff of the decision wavefront
-    unsigned *ilist_d, *done_d;     // intra MBs in dependency order (count first), per-MB done flags
+    unsigned *ilist_d;              // intra MBs in dependency order (count first)
+    uint64_t *ih_d;                 // intra handoff words, 16 per MB (bottom row, right column)
     unsigned *pflag_d;              // per MB: the id of the frame step whose costs are written
     unsigned step_ctr;              // frame-step ids (never rewound)
     // The decision stage overlaps the costs and the replay: the replay's
@@ -150,7 +151,7 @@ extern "C" void fdvgpu_close(fdvgpu *g) {
     (void)hipFree(g->cnt_d); (void)hipFree(g->totals_d); (void)hipFree(g->bases_d); (void)hipHostFree(g->bases_h);
     (void)hipFree(g->compact_d); (void)hipHostFree(g->compact_h); (void)hipHostFree(g->flags_h);
     (void)hipFree(g->pcost_d); (void)hipFree(g->pctx_d); (void)hipFree(g->pgiven_d); (void)hipFree(g->mrec_d);
-    (void)hipFree(g->ilist_d); (void)hipFree(g->done_d);
+    (void)hipFree(g->ilist_d); (void)hipFree(g->ih_d);
     (void)hipFree(g->nnw_d); (void)hipFree(g->nn_d);
     for (int k = 0; k < 64; ++k) if (g->gexec[k]) (void)hipGraphExecDestroy(g->gexec[k]);
     for (int i = 0; i < 5; ++i) if (g->ev[i]) (void)hipEventDestroy(g->ev[i]);
@@ -246,7 +247,8 @@ extern "C" fdvgpu *fdvgpu_open(int nseq, const int *w, const int *h, int nsrc) {
     CKN(hipMalloc(&g->mrec_d, (size_t)g->total_mb * sizeof(uint64_t)));
     CKN(hipMemset(g->mrec_d, 0, (size_t)g->total_mb * sizeof(uint64_t)));
     CKN(hipMalloc(&g->ilist_d, ((size_t)g->total_mb + 1) * sizeof(unsigned)));
-    CKN(hipMalloc(&g->done_d, (size_t)g->total_mb * sizeof(unsigned)));
+    CKN(hipMalloc(&g->ih_d, (size_t)g->total_mb * 16 * sizeof(uint64_t)));
+    CKN(hipMemset(g->ih_d, 0, (size_t)g->total_mb * 16 * sizeof(uint64_t)));   // id 0: never a step
     CKN(hipMalloc(&g->pflag_d, (size_t)g->total_mb * sizeof(unsigned)));
     CKN(hipMemset(g->pflag_d, 0, (size_t)g->total_mb * sizeof(unsigned)));
     CKN(hipMalloc(&g->nnw_d, (size_t)fdvg_nn_nparam() * sizeof(float)));
@@ -360,7 +362,7 @@ static int launch_stage(fdvgpu *g, int stage, int me_mode, bool par) {
         if (par && g->overlap) {
             // costs and replay side by side on their CU-masked streams, joined
             // back before the coding of the decisions
-            if (fdvg_launch_zero(g->ilist_d, 1, g->st) != 0 || fdvg_launch_zero(g->done_d, g->total_mb, g->st) != 0) {
+            if (fdvg_launch_zero(g->ilist_d, 1, g->st) != 0) {
                 seterr("parallel analysis launch failed"); return -1;
             }
             CK(hipEventRecord(g->ev_fork, g->st));
@@ -376,13 +378,13 @@ static int launch_stage(fdvgpu *g, int stage, int me_mode, bool par) {
             CK(hipStreamWaitEvent(g->st, g->ev_cost, 0));
             CK(hipStreamWaitEvent(g->st, g->ev_rep, 0));
             if (fdvg_launch_pfinish(g->seq_d, g->nseq, g->xgrid, g->ticket_d, g->pgiven_d, g->pctx_d, g->ilist_d,
-                                    g->done_d, g->total_mb, g->recs_d, g->st) != 0) {
+                                    g->ih_d, g->total_mb, g->recs_d, g->st) != 0) {
                 seterr("parallel analysis launch failed: %s", hipGetErrorString(hipGetLastError())); return -1;
             }
         } else if (par) {
-            if (fdvg_launch_zero(g->ilist_d, 1, g->st) != 0 || fdvg_launch_zero(g->done_d, g->total_mb, g->st) != 0 ||
+            if (fdvg_launch_zero(g->ilist_d, 1, g->st) != 0 ||
                 fdvg_launch_parallel(g->seq_d, g->nseq, g->xgrid, g->maxmh, g->maxmw, g->ticket_d, g->pcost_d,
-                                     g->pflag_d, g->pgiven_d, g->pctx_d, g->ilist_d, g->done_d, g->total_mb,
+                                     g->pflag_d, g->pgiven_d, g->pctx_d, g->ilist_d, g->ih_d, g->total_mb,
                                      g->recs_d, g->st) != 0) {
                 seterr("parallel analysis launch failed: %s", hipGetErrorString(hipGetLastError())); return -1;
             }
