@@ -122,8 +122,9 @@ struct Sink {
 // Structure symbols and region / any-flags of one MB, in emission order
 // (pframe_encode, fdv.h:6036-6160; code_residual flags 5385-5410; chroma_block
 // any-flags 5599-5606).  Coefficient blocks are emitted by the lanes (below).
-__device__ void emit_structure(Sink &S, const fdvg_mbrec *R, int ctx) {
-    S.byte(ctx, R->mode);
+// The mode byte itself (to stream ctx, 0-3) is emitted by the caller, so that
+// every stream index here is a constant and the sink stays in registers.
+__device__ __forceinline__ void emit_structure(Sink &S, const fdvg_mbrec *R) {
     if (R->mode == 0) {
         if (R->ncand > 1) S.byte(SG_MERGE, R->merge_idx);
         return;
@@ -201,7 +202,17 @@ k_sym(const SymSeq *__restrict__ seqs, const fdvg_mbrec *__restrict__ recs,
     const int i = blockIdx.x * (int)(blockDim.x >> 6) + (threadIdx.x >> 6);
     if (i >= nmb) return;
     const int mb = q.mb0 + i;
-    const fdvg_mbrec *R = &recs[mb];
+    // the MB's record into LDS first (16 bytes a lane): every later read of
+    // levels and flags is then an LDS read, not a dependent global load
+    __shared__ __align__(16) fdvg_mbrec rec_s[8];
+    static_assert(sizeof(fdvg_mbrec) % 16 == 0 && sizeof(fdvg_mbrec) / 16 <= 64, "record staging");
+    fdvg_mbrec *Rs = &rec_s[threadIdx.x >> 6];
+    if (L < (int)(sizeof(fdvg_mbrec) / 16))
+        ((uint4 *)Rs)[L] = ((const uint4 *)&recs[mb])[L];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const fdvg_mbrec *R = Rs;
 
     // this lane's coefficient unit: EOB count and LEB bytes of its levels
     const int16_t *lev = nullptr;
@@ -245,6 +256,7 @@ k_sym(const SymSeq *__restrict__ seqs, const fdvg_mbrec *__restrict__ recs,
     if (L == 0) {
         // the structure symbols, by one lane
         Sink S;
+#pragma unroll
         for (int k = 0; k < FDVG_NS; ++k) {
             if (pass == 0) { S.base[k] = nullptr; S.n[k] = 0; S.lim[k] = 0; }
             else {
@@ -254,12 +266,19 @@ k_sym(const SymSeq *__restrict__ seqs, const fdvg_mbrec *__restrict__ recs,
                 S.lim[k] = b0 < cap ? cap - b0 : 0;
             }
         }
-        emit_structure(S, R, mb_ctx(recs, q, i % q.mw, i / q.mw));
+        const int ctx = mb_ctx(recs, q, i % q.mw, i / q.mw);
+        emit_structure(S, R);
+        if (pass == 1) {
+            // the mode byte, first in its MB's run of stream ctx (0-3)
+            const uint32_t b0 = bases[blockIdx.y * FDVG_NS + ctx], n0 = cnt[(size_t)mb * FDVG_NS + ctx];
+            if (b0 < cap && n0 < cap - b0) compact[b0 + n0] = R->mode;
+        }
         if (pass == 0) {
             S.n[SG_CN] += totn[0]; S.n[SG_CL] += totl[0];
             S.n[SG_CN8] += totn[1]; S.n[SG_CL8] += totl[1];
             S.n[SG_CNC] += totn[2]; S.n[SG_CLC] += totl[2];
-            for (int k = 0; k < FDVG_NS; ++k) cnt[(size_t)mb * FDVG_NS + k] = S.n[k];
+#pragma unroll
+            for (int k = 0; k < FDVG_NS; ++k) cnt[(size_t)mb * FDVG_NS + k] = S.n[k] + (k == ctx ? 1u : 0u);
         }
     }
 }
