@@ -1113,6 +1113,19 @@ k_prepair(const fdvg_seq *__restrict__ seqs, int mwmax, unsigned *ctl,
 #pragma unroll
             for (int qd = 0; qd < 4; ++qd) { q8x[qd] = T.mv8[qd][0]; q8y[qd] = T.mv8[qd][1]; }
             const int ssad = T.sad8[0] + T.sad8[1] + T.sad8[2] + T.sad8[3];
+            // merge candidate 0 is the predictor on reference 0, known already:
+            // outside the precomputed set, its 21x21 tile loads are issued now so
+            // that they complete during the wait below
+            const bool hit0 = __ballot(L < nsc && sx_l == pmx && sy_l == pmy && sr_l == 0) != 0;
+            uint8_t tv0[7];
+            if (!hit0) {
+                const uint8_t *rp = q.ry[0] + (ptrdiff_t)(by + (pmy >> 2) - 2) * q.ys + bx + (pmx >> 2) - 2;
+#pragma unroll
+                for (int j = 0; j < 7; ++j) {
+                    const int k = L + 64 * j;
+                    tv0[j] = k < 21 * 21 ? rp[(ptrdiff_t)(k / 21) * q.ys + k % 21] : 0;
+                }
+            }
             // the row above must have finished the above-right MB (or the last one)
             if (my > 0) {
                 const int need = (my - 1) * 65536 + (mx + 2 < mw ? mx + 2 : mw);
@@ -1152,13 +1165,18 @@ k_prepair(const fdvg_seq *__restrict__ seqs, int mwmax, unsigned *ctl,
                     ssd[c] = __builtin_amdgcn_readlane(se_l, __builtin_ctzll(hm));
                     RPP(2);
                 } else {
-                    const uint8_t *rp = (r ? q.ry[1] : q.ry[0]) +
-                                        (ptrdiff_t)(by + (y >> 2) - 2) * q.ys + bx + (x >> 2) - 2;
                     uint8_t tv[7];
+                    if (c == 0) {
 #pragma unroll
-                    for (int j = 0; j < 7; ++j) {
-                        const int k = L + 64 * j;
-                        tv[j] = k < 21 * 21 ? rp[(ptrdiff_t)(k / 21) * q.ys + k % 21] : 0;
+                        for (int j = 0; j < 7; ++j) tv[j] = tv0[j];
+                    } else {
+                        const uint8_t *rp = (r ? q.ry[1] : q.ry[0]) +
+                                            (ptrdiff_t)(by + (y >> 2) - 2) * q.ys + bx + (x >> 2) - 2;
+#pragma unroll
+                        for (int j = 0; j < 7; ++j) {
+                            const int k = L + 64 * j;
+                            tv[j] = k < 21 * 21 ? rp[(ptrdiff_t)(k / 21) * q.ys + k % 21] : 0;
+                        }
                     }
 #pragma unroll
                     for (int j = 0; j < 7; ++j) {
