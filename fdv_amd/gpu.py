@@ -321,13 +321,14 @@ class Encoder:
             raise GpuError("fdvamd_pframe_encode: " + self.L.fdvamd_last_error().decode())
         return out[:n].tobytes(), rec
 
-    def pframe_batch(self, frames, navail, qp, tcs=None, lscale=None, force_skip=None):
-        """One P-frame of every sequence in one device pass: lists of payloads and recons."""
+    def pframe_batch(self, frames, navail, qp, tcs=None, lscale=None, force_skip=None, recon=True):
+        """One P-frame of every sequence in one device pass: lists of payloads and
+        recons (None without `recon`: the reconstruction then stays on the device)."""
         n = len(self.sizes)
         fr = [np.ascontiguousarray(f, dtype=np.uint8) for f in frames]
         caps = [w * h * 3 + 65536 for w, h in self.sizes]
-        outs = [np.zeros(c, np.uint8) for c in caps]
-        recs = [np.zeros(w * h + 2 * (w // 2) * (h // 2), np.uint8) for w, h in self.sizes]
+        outs = [np.empty(c, np.uint8) for c in caps]
+        recs = [np.empty(w * h + 2 * (w // 2) * (h // 2), np.uint8) for w, h in self.sizes] if recon else None
         P = C.c_void_p * n
         lens = (C.c_size_t * n)()
         ls = (C.c_double * n)(*([1.0] * n if lscale is None else lscale))
@@ -339,11 +340,11 @@ class Encoder:
         fsk = _ints(force_skip if force_skip is not None else [0] * n)
         op = P(*[o.ctypes.data for o in outs])
         cp = (C.c_size_t * n)(*caps)
-        rp = P(*[r.ctypes.data for r in recs])
+        rp = P(*[r.ctypes.data for r in recs]) if recs is not None else None
         rc = self.L.fdvamd_pframe_batch(
             self.h, C.addressof(fp), C.addressof(nav), C.addressof(qps), C.addressof(ls),
             C.addressof(fsk), C.addressof(tcp) if tcp is not None else None, C.addressof(op),
-            C.addressof(cp), C.addressof(lens), C.addressof(rp))
+            C.addressof(cp), C.addressof(lens), C.addressof(rp) if rp is not None else None)
         if rc != 0:
             raise GpuError("fdvamd_pframe_batch: " + self.L.fdvamd_last_error().decode())
         return [outs[s][:lens[s]].tobytes() for s in range(n)], recs
