@@ -795,15 +795,19 @@ k_analyse(const fdvg_seq *__restrict__ seqs, const int2 *__restrict__ rowtab, in
 // ---------------------------------------------------- block-parallel decisions
 //
 // The parallel and learned modes replace the decision wavefront by
-//   k_pcost    every MB's candidate costs, all MBs at once;
+//   k_pcost    every MB's candidate costs except INTRA, and its stand-in
+//              reconstruction edges, all MBs at once;
+//   k_picost   every MB's INTRA cost from its neighbours' stand-in edges;
 //   k_prepair  the reference's decision logic (fdv.h:5847-6040) replayed in a
 //              light wavefront (one wavefront per MB row, motion records only)
 //              on those costs, with the true vector predictor and merge list;
 //   k_pencode  the decided non-intra MBs coded exactly, all at once;
 //   k_pintra   the intra MBs coded in dependency order (an intra MB waits only
 //              for intra neighbours; inter ones are already reconstructed).
-// The one approximation is that intra costs and sub-modes are computed from
-// the source frame's neighbours instead of the reconstruction's.
+// The one approximation is that intra costs are computed from stand-in
+// neighbours (k_pcost) instead of the reconstruction, which exists only once
+// every decision is made; the intra sub-mode is screened again at coding time on
+// the true reconstruction.
 
 // Neighbours of an MB from a plane (gather_nb, fdv.h:5201-5212): luma corner +
 // top row into s.hbytes, left column into s.lcol[0..15]; with `chroma`, also
@@ -841,10 +845,22 @@ __device__ __forceinline__ void pcost_publish(unsigned *flag, unsigned step) {
     if (threadIdx.x == 0) __hip_atomic_store(flag, step, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// The costs of every MB's candidates except INTRA, and the MB's stand-in
+// reconstruction edges for its neighbours' intra costs (k_picost): the true
+// reconstruction exists only once every decision is made, so each MB stands in
+// with what its own candidates say it will most likely be:
+//   - INTER16 on each reference with the table vector, coded, costs
+//     J = SSD + lambda (bits + vector bits against (0, 0)); the best reference is
+//     the first with the smallest J;
+//   - SKIP at (0, 0) on reference 0 costs Js = SSD + 3 lambda;
+//   - the stand-in is the co-located reference-0 block if Js <= 14 lambda (the
+//     INTRA floor) or Js <= the best INTER16 J, else that INTER16 reconstruction.
+// edge[mb * 32 ..]: the stand-in's bottom row (16), then its right column (16).
 extern "C" __global__ void __launch_bounds__(256, 4)
-k_pcost(const fdvg_seq *__restrict__ seqs, fdvg_pcost *__restrict__ pcs, unsigned *__restrict__ pflag) {
+k_pcost(const fdvg_seq *__restrict__ seqs, fdvg_pcost *__restrict__ pcs, uint8_t *__restrict__ edge) {
     __shared__ Smem s;
     __shared__ int32_t sred[FDVG_NSC][4];
+    __shared__ uint8_t sedge[32];
     fdvg_seq q = seqs[blockIdx.y];
     int i;
     if (!xcd_mb(blockIdx.x, q.w >> 4, q.h >> 4, i)) return;
@@ -855,7 +871,6 @@ k_pcost(const fdvg_seq *__restrict__ seqs, fdvg_pcost *__restrict__ pcs, unsigne
     const fdvg_metab *T = &q.metab[i];
     fdvg_pcost *P = &pcs[mb];
     mb_stage(s, q, mb, bx, by);
-    nb_load(s, q.cy, q.w, nullptr, nullptr, 0, bx, by, my > 0, mx > 0, false);
     __syncthreads();
 
     // SKIP SSDs of likely merge candidates: (0,0), and the table vectors of the
@@ -895,25 +910,78 @@ k_pcost(const fdvg_seq *__restrict__ seqs, fdvg_pcost *__restrict__ pcs, unsigne
         P->sce[c] = c < ncs ? sred[c][0] + sred[c][1] + sred[c][2] + sred[c][3] : 0;
     }
     if (tid == 0) P->nsc = (uint8_t)ncs;
+    const int e00 = sred[0][0] + sred[0][1] + sred[0][2] + sred[0][3];     // candidate 0 is (0, 0) on ref 0
     const int qp = q.qp;
     const double lambda = q.lambda;
     // INTER16 on each reference
+    double bj = 0.0;
     for (int r = 0; r < q.navail; ++r) {
         s.pred[tid] = (uint8_t)lumaq(s.win[r], tid & 15, tid >> 4, T->mv16[r][0], T->mv16[r][1]);
         __syncthreads();
         code_luma(s, s.pred, s.recT, s.levT, s.flagT, qp, lambda);
-        if (tid == 0) {
-            P->d16[r] = s.ws.e[0] + s.ws.e[1] + s.ws.e[2] + s.ws.e[3];
-            P->b16[r] = 6 + s.ws.b[0] + s.ws.b[1] + s.ws.b[2] + s.ws.b[3];
+        const int d = s.ws.e[0] + s.ws.e[1] + s.ws.e[2] + s.ws.e[3];
+        const int b = 6 + s.ws.b[0] + s.ws.b[1] + s.ws.b[2] + s.ws.b[3];
+        if (tid == 0) { P->d16[r] = d; P->b16[r] = b; }
+        const double j = (double)d + lambda * (double)(b + egbits(zmap(T->mv16[r][0])) + egbits(zmap(T->mv16[r][1])));
+        if (r == 0 || j < bj) {                  // uniform
+            bj = j;
+            if (tid < 16) sedge[tid] = s.recT[15 * 16 + tid];
+            else if (tid < 32) sedge[tid] = s.recT[(tid - 16) * 16 + 15];
         }
         __syncthreads();
     }
-    if (T->flags & 1) {                      // the proposer expects SKIP: no INTRA, no INTER8
-        if (tid == 0) { P->dn = P->bn = P->d8 = P->b8 = 0; P->nsub = 0; }
+    {
+        const double js = (double)e00 + lambda * 3.0;
+        const bool skip = js <= lambda * 14.0 || js <= bj;
+        if (tid < 32) {
+            const int y = tid < 16 ? 15 : tid - 16, x = tid < 16 ? tid : 15;
+            edge[(size_t)mb * 32 + tid] = skip ? s.win[0][(WOFF + y) * WIN + WOFF + x] : sedge[tid];
+        }
+    }
+    if (T->flags & 1) {                      // the proposer expects SKIP: no INTER8 (and no INTRA)
+        if (tid == 0) P->d8 = P->b8 = 0;
+        return;
+    }
+    // INTER8 from the quadrant vectors
+    {
+        const int x = tid & 15, y = tid >> 4, qd = (y >> 3) * 2 + (x >> 3);
+        s.pred[tid] = (uint8_t)lumaq(s.win[0], x, y, T->mv8[qd][0], T->mv8[qd][1]);
+    }
+    __syncthreads();
+    code_luma(s, s.pred, s.rec8, s.lev8, s.flag8, qp, lambda);
+    if (tid == 0) {
+        P->d8 = s.ws.e[0] + s.ws.e[1] + s.ws.e[2] + s.ws.e[3];
+        P->b8 = 3 + s.ws.b[0] + s.ws.b[1] + s.ws.b[2] + s.ws.b[3];
+    }
+}
+
+// INTRA 16x16 costs of every MB from its neighbours' stand-in edges (k_pcost):
+// SATD screen of the four modes, then one coding of the best.  Publishes the
+// MB's costs as complete (the replay may run concurrently).
+extern "C" __global__ void __launch_bounds__(256, 4)
+k_picost(const fdvg_seq *__restrict__ seqs, fdvg_pcost *__restrict__ pcs, const uint8_t *__restrict__ edge,
+         unsigned *__restrict__ pflag) {
+    __shared__ Smem s;
+    const fdvg_seq q = seqs[blockIdx.y];
+    int i;
+    if (!xcd_mb(blockIdx.x, q.w >> 4, q.h >> 4, i)) return;
+    const int tid = threadIdx.x, wave = tid >> 6;
+    const int mw = q.w >> 4, mx = i % mw, my = i / mw, bx = mx * 16, by = my * 16;
+    const int mb = q.mb0 + i;
+    fdvg_pcost *P = &pcs[mb];
+    if (q.metab[i].flags & 1) {              // the proposer expects SKIP: no INTRA
+        if (tid == 0) { P->dn = P->bn = 0; P->nsub = 0; }
         pcost_publish(pflag ? pflag + mb : nullptr, q.step);
         return;
     }
-    // INTRA 16x16 from the source neighbours: SATD screen, then one coding
+    s.cur[tid] = q.cy[(size_t)(by + (tid >> 4)) * q.w + bx + (tid & 15)];
+    const uint8_t *E = edge + (size_t)q.mb0 * 32;
+    if (tid < 16) s.hbytes[1 + tid] = my > 0 ? E[(size_t)(i - mw) * 32 + tid] : 0;
+    else if (tid == 16) s.hbytes[0] = (my > 0 && mx > 0) ? E[(size_t)(i - mw - 1) * 32 + 15] : 128;
+    else if (tid >= 32 && tid < 48) s.lcol[tid - 32] = mx > 0 ? E[(size_t)(i - 1) * 32 + 16 + tid - 32] : 0;
+    __syncthreads();
+    const int qp = q.qp;
+    const double lambda = q.lambda;
     const Nb nbL = nb_setup(&s.hbytes[1], s.lcol, s.hbytes[0], 16, my > 0, mx > 0);
     if (wave == 0) {
         int sat[4];
@@ -933,18 +1001,6 @@ k_pcost(const fdvg_seq *__restrict__ seqs, fdvg_pcost *__restrict__ pcs, unsigne
         P->dn = s.ws.e[0] + s.ws.e[1] + s.ws.e[2] + s.ws.e[3];
         P->bn = 6 + s.ws.b[0] + s.ws.b[1] + s.ws.b[2] + s.ws.b[3];
         P->nsub = (uint8_t)sel;
-    }
-    __syncthreads();
-    // INTER8 from the quadrant vectors
-    {
-        const int x = tid & 15, y = tid >> 4, qd = (y >> 3) * 2 + (x >> 3);
-        s.pred[tid] = (uint8_t)lumaq(s.win[0], x, y, T->mv8[qd][0], T->mv8[qd][1]);
-    }
-    __syncthreads();
-    code_luma(s, s.pred, s.rec8, s.lev8, s.flag8, qp, lambda);
-    if (tid == 0) {
-        P->d8 = s.ws.e[0] + s.ws.e[1] + s.ws.e[2] + s.ws.e[3];
-        P->b8 = 3 + s.ws.b[0] + s.ws.b[1] + s.ws.b[2] + s.ws.b[3];
     }
     pcost_publish(pflag ? pflag + mb : nullptr, q.step);
 }
@@ -1025,9 +1081,9 @@ __device__ __forceinline__ void pr_load(PrPre &R, const fdvg_seq &q, const fdvg_
                                         unsigned *ctl, int my, int mx0, int L) {
     const int mw = q.w >> 4, i0 = my * mw + mx0;
     const int n = mw - mx0 < PR_CHUNK ? mw - mx0 : PR_CHUNK;
-    // with pflag, the chunk's costs may still be in flight (k_pcost running
+    // with pflag, the chunk's costs may still be in flight (k_picost running
     // concurrently): wait for their flags to carry this step's id (bounded;
-    // error flag on timeout).  Without, k_pcost has completed before this launch.
+    // error flag on timeout).  Without, the costs are complete before this launch.
     if (pflag) {
         const unsigned *f = pflag + q.mb0 + i0 + (L < n ? L : 0);
         bool ok = L >= n || __hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == q.step;
@@ -1503,9 +1559,10 @@ k_pintra(const fdvg_seq *__restrict__ seqs, int nseq, const fdvg_mbinfo *__restr
 // the costs' per-MB flags, so it may run on another stream at the same time),
 // and the coding of the decisions.  fdvg_launch_parallel issues all three on one
 // stream.
-extern "C" int fdvg_launch_pcost(const fdvg_seq *seqs, int nseq, int xgrid, fdvg_pcost *pcs, unsigned *pflag,
-                                 hipStream_t st) {
-    hipLaunchKernelGGL(k_pcost, dim3(xgrid, nseq), dim3(256), 0, st, seqs, pcs, pflag);
+extern "C" int fdvg_launch_pcost(const fdvg_seq *seqs, int nseq, int xgrid, fdvg_pcost *pcs, uint8_t *edge,
+                                 unsigned *pflag, hipStream_t st) {
+    hipLaunchKernelGGL(k_pcost, dim3(xgrid, nseq), dim3(256), 0, st, seqs, pcs, edge);
+    hipLaunchKernelGGL(k_picost, dim3(xgrid, nseq), dim3(256), 0, st, seqs, pcs, (const uint8_t *)edge, pflag);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -1541,11 +1598,11 @@ extern "C" int fdvg_launch_pfinish(const fdvg_seq *seqs, int nseq, int xgrid, un
 }
 
 extern "C" int fdvg_launch_parallel(const fdvg_seq *seqs, int nseq, int xgrid, int maxmh, int mwmax,
-                                    unsigned *ctl, fdvg_pcost *pcs, unsigned *pflag, fdvg_mbinfo *given,
-                                    fdvg_pctx *pctx, unsigned *ilist, uint64_t *ih, int total_mb,
+                                    unsigned *ctl, fdvg_pcost *pcs, uint8_t *edge, unsigned *pflag,
+                                    fdvg_mbinfo *given, fdvg_pctx *pctx, unsigned *ilist, uint64_t *ih, int total_mb,
                                     fdvg_mbrec *recs, hipStream_t st) {
     (void)pflag;
-    if (fdvg_launch_pcost(seqs, nseq, xgrid, pcs, nullptr, st) != 0) return -1;
+    if (fdvg_launch_pcost(seqs, nseq, xgrid, pcs, edge, nullptr, st) != 0) return -1;
     // same stream: the costs are complete when the replay starts, no flag waits
     if (fdvg_launch_prepair(seqs, nseq, maxmh, mwmax, ctl, pcs, nullptr, given, pctx, ilist, st) != 0) return -1;
     return fdvg_launch_pfinish(seqs, nseq, xgrid, ctl, given, pctx, ilist, ih, total_mb, recs, st);

@@ -40,9 +40,9 @@ extern "C" int fdvg_nn_setup(void);
 // The proposer's confidence threshold: below it an MB gets the exhaustive search.
 #define FDVG_NN_THRESHOLD 0.7f     // default confidence threshold of the proposer
 #define FDVG_NN_SKIP_OFF 2.0f      // default SKIP threshold: no pruning
-extern "C" int fdvg_launch_parallel(const fdvg_seq *, int, int, int, int, unsigned *, fdvg_pcost *, unsigned *,
+extern "C" int fdvg_launch_parallel(const fdvg_seq *, int, int, int, int, unsigned *, fdvg_pcost *, uint8_t *, unsigned *,
                                     fdvg_mbinfo *, fdvg_pctx *, unsigned *, uint64_t *, int, fdvg_mbrec *, hipStream_t);
-extern "C" int fdvg_launch_pcost(const fdvg_seq *, int, int, fdvg_pcost *, unsigned *, hipStream_t);
+extern "C" int fdvg_launch_pcost(const fdvg_seq *, int, int, fdvg_pcost *, uint8_t *, unsigned *, hipStream_t);
 extern "C" int fdvg_launch_prepair(const fdvg_seq *, int, int, int, unsigned *, const fdvg_pcost *, const unsigned *,
                                    fdvg_mbinfo *, fdvg_pctx *, unsigned *, hipStream_t);
 extern "C" int fdvg_launch_pfinish(const fdvg_seq *, int, int, unsigned *, fdvg_mbinfo *, fdvg_pctx *, unsigned *,
@@ -92,6 +92,7 @@ struct fdvgpu {
     uint8_t *compact_d, *compact_h;
     // block-parallel decisions
     fdvg_pcost *pcost_d;
+    uint8_t *edge_d;                // per MB: the stand-in reconstruction's bottom row and right column
     fdvg_pctx *pctx_d;
     fdvg_mbinfo *pgiven_d;
     uint64_t *mrec_d;               // motion-record handoff of the decision wavefront
@@ -163,6 +164,7 @@ extern "C" void fdvgpu_close(fdvgpu *g) {
     if (g->ev_cost) (void)hipEventDestroy(g->ev_cost);
     if (g->ev_rep) (void)hipEventDestroy(g->ev_rep);
     (void)hipFree(g->pflag_d);
+    (void)hipFree(g->edge_d);
     free(g->S);
     free(g);
 }
@@ -242,6 +244,7 @@ extern "C" fdvgpu *fdvgpu_open(int nseq, const int *w, const int *h, int nsrc) {
     CKN(hipHostMalloc(&g->compact_h, g->compact_cap, hipHostMallocDefault));
     CKN(hipHostMalloc(&g->flags_h, 64, hipHostMallocDefault));
     CKN(hipMalloc(&g->pcost_d, (size_t)g->total_mb * sizeof(fdvg_pcost)));
+    CKN(hipMalloc(&g->edge_d, (size_t)g->total_mb * 32));
     CKN(hipMalloc(&g->pctx_d, (size_t)g->total_mb * sizeof(fdvg_pctx)));
     CKN(hipMalloc(&g->pgiven_d, (size_t)g->total_mb * sizeof(fdvg_mbinfo)));
     CKN(hipMalloc(&g->mrec_d, (size_t)g->total_mb * sizeof(uint64_t)));
@@ -368,7 +371,7 @@ static int launch_stage(fdvgpu *g, int stage, int me_mode, bool par) {
             CK(hipEventRecord(g->ev_fork, g->st));
             CK(hipStreamWaitEvent(g->st_cost, g->ev_fork, 0));
             CK(hipStreamWaitEvent(g->st_rep, g->ev_fork, 0));
-            if (fdvg_launch_pcost(g->seq_d, g->nseq, g->xgrid, g->pcost_d, g->pflag_d, g->st_cost) != 0 ||
+            if (fdvg_launch_pcost(g->seq_d, g->nseq, g->xgrid, g->pcost_d, g->edge_d, g->pflag_d, g->st_cost) != 0 ||
                 fdvg_launch_prepair(g->seq_d, g->nseq, g->maxmh, g->maxmw, g->ticket_d, g->pcost_d, g->pflag_d,
                                     g->pgiven_d, g->pctx_d, g->ilist_d, g->st_rep) != 0) {
                 seterr("parallel analysis launch failed: %s", hipGetErrorString(hipGetLastError())); return -1;
@@ -384,7 +387,7 @@ static int launch_stage(fdvgpu *g, int stage, int me_mode, bool par) {
         } else if (par) {
             if (fdvg_launch_zero(g->ilist_d, 1, g->st) != 0 ||
                 fdvg_launch_parallel(g->seq_d, g->nseq, g->xgrid, g->maxmh, g->maxmw, g->ticket_d, g->pcost_d,
-                                     g->pflag_d, g->pgiven_d, g->pctx_d, g->ilist_d, g->ih_d, g->total_mb,
+                                     g->edge_d, g->pflag_d, g->pgiven_d, g->pctx_d, g->ilist_d, g->ih_d, g->total_mb,
                                      g->recs_d, g->st) != 0) {
                 seterr("parallel analysis launch failed: %s", hipGetErrorString(hipGetLastError())); return -1;
             }

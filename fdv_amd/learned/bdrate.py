@@ -7,12 +7,18 @@ tools/rd-bench.py:75-113 defines it:
   - BD-rate = 10^(mean difference) - 1, in percent (negative: the new mode
     needs fewer bits for equal quality).
 
-evaluate() codes the same seeded sequences with two modes of the drop-in
-encoder at several QPs (payload bytes of the P-frames; PSNR-Y of their
-reconstruction against the source) and returns the BD-rate of the second
-against the first.
+evaluate() codes the same sequences with two modes of the drop-in encoder at
+several QPs (payload bytes of the P-frames; PSNR-Y of their reconstruction
+against the source) and returns the BD-rate of the second against the first.
+
+Content: either two seeded sequences of fdv_amd/content.py, or (--library) the
+reference's scene library (scenes/*.scn, rendered by its include/fdv_scene.h
+through scene_render.c; `make -C fdv_amd/learned scenes` builds both into
+_scenes/) at rd-bench.py's QPs (16, 22, 28, 34: tools/rd-bench.py:19), with its
+per-scene summary (mean, median, best, worst: tools/rd-bench.py:116-138).
 
     python -m fdv_amd.learned.bdrate [--frames 60] [--qps 20,24,28,32]
+    python -m fdv_amd.learned.bdrate --library [--frames 30] [--size 1280x720]
 """
 from __future__ import annotations
 
@@ -101,17 +107,27 @@ def code_sequence(frames, w, h, qp, mode, weights=None, thresholds=None):
 
 
 def evaluate(scenes, qps=(20, 24, 28, 32), nframes=60, mode_a=None, mode_b=None, weights=None, log=print,
-             thresholds=None):
+             thresholds=None, cache=None):
+    """`cache` (a dict, optional) keeps each (scene, QP, mode) result, so that one
+    first mode can be compared with several second ones without recoding it."""
     from fdv_amd import gpu as G
     mode_a = G.ME_EXHAUSTIVE if mode_a is None else mode_a
     mode_b = G.ME_LEARNED if mode_b is None else mode_b
+    cache = {} if cache is None else cache
+
+    def coded(name, frames, sc, qp, mode):
+        key = (name, qp, mode)
+        if key not in cache:
+            cache[key] = code_sequence(frames, sc.w, sc.h, qp, mode, weights, thresholds)
+        return cache[key]
+
     out = {}
     for name, sc in scenes:
         frames = [sc.frame(i) for i in range(nframes)]
         pa, pb = [], []
         for qp in qps:
-            ba, da = code_sequence(frames, sc.w, sc.h, qp, mode_a, weights, thresholds)
-            bb, db = code_sequence(frames, sc.w, sc.h, qp, mode_b, weights, thresholds)
+            ba, da = coded(name, frames, sc, qp, mode_a)
+            bb, db = coded(name, frames, sc, qp, mode_b)
             pa.append((ba, da))
             pb.append((bb, db))
             log("  %s qp %d: A %d B %.3f dB | B %d B %.3f dB" % (name, qp, ba, da, bb, db))
@@ -119,13 +135,62 @@ def evaluate(scenes, qps=(20, 24, 28, 32), nframes=60, mode_a=None, mode_b=None,
     return out
 
 
+SCENE_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_scenes")
+
+
+class LibraryScene:
+    """One scene of the reference's library, rendered once to a temporary file."""
+
+    def __init__(self, path, w, h, nframes):
+        import subprocess
+        import tempfile
+        exe = os.path.join(SCENE_DIR, "scene_render")
+        if not os.path.exists(exe):
+            raise RuntimeError("scene library not built: make -C fdv_amd/learned scenes")
+        self.w, self.h = w, h
+        fs = w * h * 3 // 2
+        with tempfile.NamedTemporaryFile(suffix=".yuv") as t:
+            subprocess.run([exe, path, str(w), str(h), str(nframes), t.name], check=True)
+            self.data = np.fromfile(t.name, dtype=np.uint8).reshape(nframes, fs)
+
+    def frame(self, i):
+        return self.data[i]
+
+
+def library_scenes(w, h, nframes, names=None):
+    """(name, scene) for every .scn of the library, in name order."""
+    import glob
+    out = []
+    for path in sorted(glob.glob(os.path.join(SCENE_DIR, "*.scn"))):
+        name = os.path.basename(path)[:-4]
+        if names and name not in names:
+            continue
+        out.append((name, LibraryScene(path, w, h, nframes)))
+    return out
+
+
+def summarise(res):
+    """rd-bench.py's per-scene summary (tools/rd-bench.py:116-138)."""
+    rows = sorted(((k, v["bd_rate"]) for k, v in res.items() if v["bd_rate"] is not None), key=lambda r: r[1])
+    if not rows:
+        return None
+    v = [r[1] for r in rows]
+    return {"scenes": len(rows), "mean": float(np.mean(v)), "median": float(np.median(v)),
+            "best": list(rows[0]), "worst": list(rows[-1]),
+            "improved": sum(1 for x in v if x < 0), "regressed": sum(1 for x in v if x > 0)}
+
+
 def main():
     from fdv_amd import content
     from fdv_amd import gpu as G
     ap = argparse.ArgumentParser()
-    ap.add_argument("--frames", type=int, default=60)
-    ap.add_argument("--qps", default="20,24,28,32")
-    ap.add_argument("--b", default="learned", choices=["learned", "parallel", "exhaustive", "diamond"])
+    ap.add_argument("--frames", type=int, default=None, help="frames per sequence (60; 30 with --library)")
+    ap.add_argument("--qps", default=None, help="QPs (20,24,28,32; rd-bench's 16,22,28,34 with --library)")
+    ap.add_argument("--library", action="store_true", help="the reference's scene library as content")
+    ap.add_argument("--size", default="1280x720", help="library render size")
+    ap.add_argument("--scenes", default=None, help="comma-separated subset of the library")
+    ap.add_argument("--b", default="learned",
+                    help="second mode(s), comma-separated: learned, parallel, exhaustive, diamond")
     ap.add_argument("--a", default="exhaustive", choices=["learned", "parallel", "exhaustive", "diamond"])
     ap.add_argument("--out", default=None, help="write the points and BD-rates as JSON")
     ap.add_argument("--conf", type=float, default=0.7, help="proposer confidence threshold")
@@ -133,18 +198,38 @@ def main():
     args = ap.parse_args()
     modes = {"learned": G.ME_LEARNED, "parallel": G.ME_PARALLEL, "exhaustive": G.ME_EXHAUSTIVE,
              "diamond": G.ME_DIAMOND}
-    scenes = [("pan720", content.scene_pan(1280, 720, 2)),
-              ("objects720", content.scene_objects(1280, 720, 3, nobj=12, smin=32, smax=160, vmax=12, grain=2.0))]
-    res = evaluate(scenes, [int(q) for q in args.qps.split(",")], args.frames, modes[args.a], modes[args.b],
-                   log=lambda m: print(m, flush=True), thresholds=(args.conf, args.skip))
-    summary = {k: v["bd_rate"] for k, v in res.items()}
-    print(json.dumps(summary), flush=True)
+    if args.library:
+        frames = args.frames or 30
+        qps = args.qps or "16,22,28,34"
+        w, h = (int(x) for x in args.size.split("x"))
+        scenes = library_scenes(w, h, frames, args.scenes.split(",") if args.scenes else None)
+    else:
+        frames = args.frames or 60
+        qps = args.qps or "20,24,28,32"
+        scenes = [("pan720", content.scene_pan(1280, 720, 2)),
+                  ("objects720", content.scene_objects(1280, 720, 3, nobj=12, smin=32, smax=160, vmax=12,
+                                                       grain=2.0))]
+    cache, recs = {}, {}
+    for bname in args.b.split(","):
+        print("== %s vs %s" % (bname, args.a), flush=True)
+        res = evaluate(scenes, [int(q) for q in qps.split(",")], frames, modes[args.a], modes[bname],
+                       log=lambda m: print(m, flush=True), thresholds=(args.conf, args.skip), cache=cache)
+        summary = {k: v["bd_rate"] for k, v in res.items()}
+        print(json.dumps(summary), flush=True)
+        stats = summarise(res)
+        if stats:
+            print("%s vs %s: mean %+.2f%%, median %+.2f%%, best %s %+.2f%%, worst %s %+.2f%%, improved %d/%d" % (
+                bname, args.a, stats["mean"], stats["median"], stats["best"][0], stats["best"][1],
+                stats["worst"][0], stats["worst"][1], stats["improved"], stats["scenes"]), flush=True)
+        recs[bname] = {"a": args.a, "b": bname, "frames": frames, "qps": qps,
+                       "content": ("reference scene library at " + args.size) if args.library
+                       else "content.py seeds",
+                       "thresholds": {"conf": args.conf, "skip": args.skip},
+                       "bd_rate_percent": summary, "summary": stats,
+                       "points": {k: {"a": v["a"], "b": v["b"]} for k, v in res.items()}}
     if args.out:
         with open(args.out, "w") as f:
-            json.dump({"a": args.a, "b": args.b, "frames": args.frames, "qps": args.qps,
-                       "thresholds": {"conf": args.conf, "skip": args.skip},
-                       "bd_rate_percent": summary,
-                       "points": {k: {"a": v["a"], "b": v["b"]} for k, v in res.items()}}, f, indent=1)
+            json.dump(recs[args.b] if "," not in args.b else recs, f, indent=1)
 
 
 if __name__ == "__main__":

@@ -1410,18 +1410,67 @@ void orc_metab_candidates(const uint8_t *cy, int w, int h, const uint8_t *const 
     }
 }
 
+/* The stand-in reconstruction of every MB, from which the block-parallel intra
+ * costs read their neighbours (the true reconstruction exists only once every
+ * decision is made).  Per MB, from its own candidates:
+ *   - INTER16 on each reference with the table vector, coded; its cost
+ *     J = SSD + lambda * (bits + vector bits against (0, 0)); the best
+ *     reference is the first with the smallest J;
+ *   - SKIP at (0, 0) on reference 0: Js = SSD + 3 lambda;
+ *   - the stand-in is the co-located reference-0 block if Js <= 14 lambda (the
+ *     INTRA floor) or Js <= the best INTER16 J, else that INTER16 reconstruction.
+ * `sb` is a tight w x h plane. */
+void orc_standin_plane(const uint8_t *cy, int w, int h, const uint8_t *const RY[2], int ys, int navail, int qp,
+                       double lambda, const orc_metab *mt, uint8_t *sb) {
+    const int mw = w / 16, mh = h / 16;
+    for (int my = 0; my < mh; ++my)
+        for (int mx = 0; mx < mw; ++mx) {
+            const int bx = mx * 16, by = my * 16;
+            const orc_metab *T = &mt[my * mw + mx];
+            uint8_t best[256];
+            double bj = 0.0;
+            for (int r = 0; r < navail; ++r) {
+                uint8_t pr[256], rr[256]; coded_t tc; double dd = 0.0; int bb = 6;
+                orc_mc_luma(RY[r], ys, bx, by, 16, 16, T->mv16[r][0], T->mv16[r][1], pr, 16);
+                residual(cy, w, bx, by, pr, 16, qp, lambda, rr, &tc, &dd, &bb);
+                bb += orc_bits_val(zmap(T->mv16[r][0])) + orc_bits_val(zmap(T->mv16[r][1]));
+                const double j = dd + lambda * (double)bb;
+                if (r == 0 || j < bj) { bj = j; memcpy(best, rr, 256); }
+            }
+            int e0 = 0;
+            for (int y = 0; y < 16; ++y)
+                for (int x = 0; x < 16; ++x) {
+                    const int d = cy[(by + y) * w + bx + x] - RY[0][(ptrdiff_t)(by + y) * ys + bx + x];
+                    e0 += d * d;
+                }
+            const double js = (double)e0 + lambda * 3.0;
+            const int skip = js <= lambda * 14.0 || js <= bj;
+            for (int y = 0; y < 16; ++y)
+                for (int x = 0; x < 16; ++x)
+                    sb[(by + y) * w + bx + x] = skip ? RY[0][(ptrdiff_t)(by + y) * ys + bx + x] : best[y * 16 + x];
+        }
+}
+
 /* The block-parallel costs with the intra neighbours read from plane `nb` (tight,
- * stride w): the product computes them from the source (nb == cy); the
- * parity tests also pass the exact path's pre-deblock reconstruction. */
+ * stride w), or, with nb == NULL, from the stand-in reconstruction above (what
+ * the product does); the parity tests also pass the exact path's pre-deblock
+ * reconstruction. */
 void orc_pframe_costs_par_nb(const uint8_t *cy, const uint8_t *nb, int w, int h, const uint8_t *const RY[2],
                              int ys, int navail, int qp, double lambda, const orc_metab *mt, orc_pcost *pc);
 void orc_pframe_costs_par(const uint8_t *cy, int w, int h, const uint8_t *const RY[2], int ys,
                           int navail, int qp, double lambda, const orc_metab *mt, orc_pcost *pc) {
-    orc_pframe_costs_par_nb(cy, cy, w, h, RY, ys, navail, qp, lambda, mt, pc);
+    orc_pframe_costs_par_nb(cy, NULL, w, h, RY, ys, navail, qp, lambda, mt, pc);
 }
 void orc_pframe_costs_par_nb(const uint8_t *cy, const uint8_t *nb, int w, int h, const uint8_t *const RY[2],
                              int ys, int navail, int qp, double lambda, const orc_metab *mt, orc_pcost *pc) {
     const int mw = w / 16, mh = h / 16;
+    uint8_t *own = NULL;
+    if (!nb) {
+        own = malloc((size_t)w * h);
+        if (!own) return;
+        orc_standin_plane(cy, w, h, RY, ys, navail, qp, lambda, mt, own);
+        nb = own;
+    }
     for (int my = 0; my < mh; ++my)
         for (int mx = 0; mx < mw; ++mx) {
             const int bx = mx * 16, by = my * 16;
@@ -1467,6 +1516,7 @@ void orc_pframe_costs_par_nb(const uint8_t *cy, const uint8_t *nb, int w, int h,
                 P->d8 = (int32_t)d8; P->b8 = b8;
             }
         }
+    free(own);
 }
 
 void orc_pframe_repair_par(const uint8_t *cy, int w, int h, const uint8_t *const RY[2], int ys,

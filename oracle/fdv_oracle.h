@@ -131,7 +131,7 @@ typedef struct {
  * the vector-difference bits (which depend on the predictor). */
 typedef struct {
     int32_t d16[2], b16[2];  /* INTER16 on each reference (bits include the 6 mode/ref bits) */
-    int32_t dn, bn;          /* INTRA 16x16 from the source frame's neighbours (bits incl. 6) */
+    int32_t dn, bn;          /* INTRA 16x16 from the stand-in neighbours (bits incl. 6) */
     int32_t d8, b8;          /* INTER8 (bits include the 3 mode bits) */
     uint8_t nsub, pad[3];
 } orc_pcost;
@@ -156,6 +156,8 @@ void orc_metab_candidates(const uint8_t *cy, int w, int h, const uint8_t *const 
 /* Every MB's candidate costs, independently of every other MB. */
 void orc_pframe_costs_par(const uint8_t *cy, int w, int h, const uint8_t *const RY[2], int ys,
                           int navail, int qp, double lambda, const orc_metab *mt, orc_pcost *pc);
+void orc_standin_plane(const uint8_t *cy, int w, int h, const uint8_t *const RY[2], int ys, int navail, int qp,
+                       double lambda, const orc_metab *mt, uint8_t *sb);
 void orc_pframe_costs_par_nb(const uint8_t *cy, const uint8_t *nb, int w, int h, const uint8_t *const RY[2],
                              int ys, int navail, int qp, double lambda, const orc_metab *mt, orc_pcost *pc);
 void orc_set_no_deblock(int on);

@@ -298,7 +298,7 @@ def oracle_parallel(cur, r0, r1, w, h, navail, qp, lscale=1.0, metab=None):
 def oracle_parallel_decisions(cur, r0, r1, w, h, navail, qp, lscale=1.0, metab=None, nb=None):
     """The block-parallel decisions alone: (given decisions, costs).  `nb` is the
     tight luma plane the intra costs read their neighbours from (default: the
-    source, as the product does)."""
+    stand-in reconstruction, as the product does)."""
     lib = oracle()
     keep, ptrs, ys, cs = _padded_refs(r0, r1, w, h)
     RY = (C.c_void_p * 2)(*ptrs[0])
@@ -307,13 +307,14 @@ def oracle_parallel_decisions(cur, r0, r1, w, h, navail, qp, lscale=1.0, metab=N
     pc = (PCost * nmb)()
     given = (MbInfo * nmb)()
     cy = split_i420(cur, w, h)[0]
-    nbp = cy if nb is None else np.ascontiguousarray(nb[:w * h], dtype=np.uint8)
+    nbp = None if nb is None else np.ascontiguousarray(nb[:w * h], dtype=np.uint8)
     lam = lib.orc_lambda(qp, lscale)
     lib.orc_pframe_costs_par_nb.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int,
                                             C.c_int, C.c_double, C.c_void_p, C.c_void_p]
     lib.orc_pframe_repair_par.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int,
                                           C.c_double, C.c_void_p, C.c_void_p, C.c_void_p]
-    lib.orc_pframe_costs_par_nb(C.c_void_p(cy.ctypes.data), C.c_void_p(nbp.ctypes.data), w, h, C.addressof(RY), ys,
+    lib.orc_pframe_costs_par_nb(C.c_void_p(cy.ctypes.data), C.c_void_p(None if nbp is None else nbp.ctypes.data),
+                                w, h, C.addressof(RY), ys,
                                 navail, qp, lam, C.addressof(mt), C.addressof(pc))
     lib.orc_pframe_repair_par(C.c_void_p(cy.ctypes.data), w, h, C.addressof(RY), ys, navail, lam,
                               C.addressof(mt), C.addressof(pc), C.addressof(given))

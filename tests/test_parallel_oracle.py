@@ -1,12 +1,15 @@
 """Pinning of the oracle's block-parallel and learned-search functions
 (oracle/fdv_oracle.c: orc_metab_exhaustive, orc_metab_candidates,
-orc_pframe_costs_par_nb, orc_pframe_repair_par) against functions the
-reference's vectors and brute force already pin:
+orc_standin_plane, orc_pframe_costs_par_nb, orc_pframe_repair_par) against
+functions the reference's vectors and brute force already pin:
 
 - the motion table is orc_me_search's exhaustive search of every block;
 - the candidate search equals it wherever the proposals cover the whole-pel
   minimum, falls back to it below the confidence threshold, and flags SKIP by
   its own threshold;
+- the stand-in reconstruction is, block by block, the co-located reference or
+  the exact coder's INTER16 reconstruction (the reference itself on a static
+  frame), and the intra costs read their neighbours from it;
 - the decision replay, given intra costs read from the exact wavefront's own
   pre-deblock reconstruction, reproduces the exact wavefront's decisions (modes,
   references, merge indices and vectors) on every MB.
@@ -130,3 +133,64 @@ def test_replay_with_exact_intra_costs_reproduces_the_wavefront(qp):
                 assert g.sub == 0xFF                 # screened when coded
         far, near = near, rec
     assert {0, 1, 2}.issubset(seen)
+
+
+def _standin(cur, r0, r1, w, h, navail, qp, mt):
+    lib = L.oracle()
+    keep, ptrs, ys, cs = L._padded_refs(r0, r1, w, h)
+    RY = (C.c_void_p * 2)(*ptrs[0])
+    sb = np.zeros(w * h, np.uint8)
+    cy = L.split_i420(cur, w, h)[0]
+    lib.orc_standin_plane.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_int,
+                                      C.c_double, C.c_void_p, C.c_void_p]
+    lib.orc_standin_plane(C.c_void_p(cy.ctypes.data), w, h, C.addressof(RY), ys, navail, qp,
+                          lib.orc_lambda(qp, C.c_double(1.0)), C.addressof(mt), C.c_void_p(sb.ctypes.data))
+    return sb.reshape(h, w)
+
+
+def test_standin_of_a_static_frame_is_the_reference():
+    w, h, (f0, f1, f2) = _frames()
+    for qp in (20, 36):
+        mt = L.oracle_metab_exhaustive(f1, f1, f0, w, h, 2)
+        sb = _standin(f1, f1, f0, w, h, 2, qp, mt)
+        assert np.array_equal(sb, _luma(f1, w, h))
+
+
+def test_standin_blocks_are_the_reference_or_the_inter16_reconstruction():
+    """Every stand-in block is the co-located reference-0 block or the exact
+    coder's INTER16 reconstruction of the MB with its table vector (both kinds
+    occur over the QPs), and the block-parallel intra costs are those read from
+    this plane."""
+    sc = content.scene_objects(128, 64, 79, nobj=4, smin=8, smax=40, vmax=8, grain=2.0)
+    w, h = sc.w, sc.h
+    f0, f1, f2 = (sc.frame(i) for i in range(3))
+    nmb = (w // 16) * (h // 16)
+    mt = L.oracle_metab_exhaustive(f2, f1, f0, w, h, 2)
+    ref0 = _luma(f1, w, h)
+    kinds = set()
+    for qp in (16, 28, 40):
+        sb = _standin(f2, f1, f0, w, h, 2, qp, mt)
+        recs = []
+        for r in range(2):
+            given = (L.MbInfo * nmb)()
+            for i in range(nmb):
+                given[i].mode, given[i].ref = 1, r
+                given[i].mv[0][0], given[i].mv[0][1] = mt[i].mv16[r][0], mt[i].mv16[r][1]
+            L.set_no_deblock(True)
+            try:
+                _, rec, info = L.oracle_analyse(f2, f1, f0, w, h, 2, qp, given=given)
+            finally:
+                L.set_no_deblock(False)
+            assert all(info[i].mode == 1 for i in range(nmb))
+            recs.append(_luma(rec, w, h))
+        for i in range(nmb):
+            bx, by = (i % (w // 16)) * 16, (i // (w // 16)) * 16
+            blk = sb[by:by + 16, bx:bx + 16]
+            k = [n for n, src in (("ref", ref0), ("inter", recs[0]), ("inter", recs[1]))
+                 if np.array_equal(blk, src[by:by + 16, bx:bx + 16])]
+            assert k, f"qp {qp} MB {i}: stand-in block is neither the reference nor an INTER16 reconstruction"
+            kinds.add(k[0])
+        _, pc_default = L.oracle_parallel_decisions(f2, f1, f0, w, h, 2, qp, metab=mt)
+        _, pc_plane = L.oracle_parallel_decisions(f2, f1, f0, w, h, 2, qp, metab=mt, nb=sb.ravel())
+        assert bytes(pc_default) == bytes(pc_plane)
+    assert kinds == {"ref", "inter"}
