@@ -39,9 +39,12 @@ typedef struct {
 } fdvg_metab;
 
 // The learned proposer's output for one MB: 8 whole-pel vectors on the nearest
-// reference (pels, best first), their total probability, the SKIP probability.
+// reference (pels, best first), the 4 lowest-cost positions of its half-res
+// correlation map (pels; ties to the smaller |dx| + |dy|, then the lower
+// position), the 8 vectors' total probability, the SKIP probability.
 typedef struct {
     int8_t cx[8], cy[8];
+    int8_t kx[4], ky[4];
     float conf, pskip;
 } fdvg_nnout;
 

@@ -479,9 +479,12 @@ __device__ __forceinline__ u4v ni_pack8(const float *f) {
 }
 
 // The correlation map of one MB into w.fb (bf16 pairs), by one wavefront.
-// Integer SADs and the fp32 feature formulas are those of nn_features.
+// Integer SADs and the fp32 feature formulas are those of nn_features.  Also
+// the map's 4 lowest-cost positions into o->kx, o->ky (whole pels; ties to the
+// smaller |dx| + |dy|, then the lower position): classical coarse search
+// candidates beside the network's.
 __device__ __forceinline__ void ni_features(NiWave &w, const uint8_t *cur, int cw, const uint8_t *ref, int rs,
-                                            int bx, int by, int qp) {
+                                            int bx, int by, int qp, fdvg_nnout *o) {
     const int l = threadIdx.x & 63;
     uint8_t *c8 = (uint8_t *)w.act;
     uint8_t *r32 = c8 + 64;
@@ -540,6 +543,27 @@ __device__ __forceinline__ void ni_features(NiWave &w, const uint8_t *cur, int c
             mn = min(mn, sad);
         }
         cv[m] = sad;
+    }
+    {
+        // keys (cost, |dx| + |dy|, position): cost < 2^14, distance < 2^5, position < 2^9
+        uint32_t key[5];
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
+            const int p = l + 64 * m, dy = p / NN_G - 8, dx = p % NN_G - 8;
+            key[m] = p < NN_P ? ((uint32_t)cv[m] << 14) | ((uint32_t)(abs(dx) + abs(dy)) << 9) | (uint32_t)p
+                              : 0xFFFFFFFFu;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            uint32_t b = key[0];
+#pragma unroll
+            for (int m = 1; m < 5; ++m) b = min(b, key[m]);
+            for (int s = 32; s > 0; s >>= 1) b = min(b, (uint32_t)__shfl_xor((int)b, s));
+#pragma unroll
+            for (int m = 0; m < 5; ++m) if (key[m] == b) key[m] = 0xFFFFFFFFu;
+            const int p = (int)(b & 511u);
+            if (l == 0) { o->kx[k] = (int8_t)(2 * (p % NN_G - 8)); o->ky[k] = (int8_t)(2 * (p / NN_G - 8)); }
+        }
     }
     for (int o = 32; o > 0; o >>= 1) mn = min(mn, __shfl_xor(mn, o));
     (void)cost;
@@ -639,7 +663,7 @@ k_nn_infer(const fdvg_seq *__restrict__ seqs, int nseq, int total, const float *
         while (k + 1 < nseq && seqs[k + 1].mb0 <= gi) ++k;
         const fdvg_seq &q = seqs[k];
         const int i = gi - q.mb0, mw = q.w >> 4;
-        ni_features(Wv, q.cy, q.w, q.ry[0], q.ys, (i % mw) * 16, (i / mw) * 16, q.qp);
+        ni_features(Wv, q.cy, q.w, q.ry[0], q.ys, (i % mw) * 16, (i / mw) * 16, q.qp, &out[gi]);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");

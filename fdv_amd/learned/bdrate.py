@@ -195,7 +195,9 @@ def main():
     ap.add_argument("--out", default=None, help="write the points and BD-rates as JSON")
     ap.add_argument("--conf", type=float, default=0.7, help="proposer confidence threshold")
     ap.add_argument("--skip", type=float, default=2.0, help="proposer SKIP threshold (above 1: no pruning)")
+    ap.add_argument("--weights", default=None, help="proposer weights file (default: the shipped ones)")
     args = ap.parse_args()
+    weights = np.fromfile(args.weights, dtype="<f4") if args.weights else None
     modes = {"learned": G.ME_LEARNED, "parallel": G.ME_PARALLEL, "exhaustive": G.ME_EXHAUSTIVE,
              "diamond": G.ME_DIAMOND}
     if args.library:
@@ -212,7 +214,7 @@ def main():
     cache, recs = {}, {}
     for bname in args.b.split(","):
         print("== %s vs %s" % (bname, args.a), flush=True)
-        res = evaluate(scenes, [int(q) for q in qps.split(",")], frames, modes[args.a], modes[bname],
+        res = evaluate(scenes, [int(q) for q in qps.split(",")], frames, modes[args.a], modes[bname], weights=weights,
                        log=lambda m: print(m, flush=True), thresholds=(args.conf, args.skip), cache=cache)
         summary = {k: v["bd_rate"] for k, v in res.items()}
         print(json.dumps(summary), flush=True)

@@ -8,7 +8,13 @@ wavefront (the reference's decision logic); the tuple is (the MB's proposer
 input, the exhaustive 16x16 vector on the nearest reference as a position of
 the half-res displacement grid, whether the MB was coded SKIP).
 
+With --library N, the reference's scene library (bdrate.py --library) adds N
+P-frames of every scene at every rd-bench QP, from frame --library-first on, so
+that the frames the rate-distortion study codes (0 .. 29) are not trained on;
+--holdout leaves whole scenes out.
+
     python -m fdv_amd.learned.train [--frames 2000] [--epochs 3] [--out fdv_amd/learned/proposer.f32]
+    python -m fdv_amd.learned.train --library 10 --holdout chroma,detail [--out ...]
 """
 from __future__ import annotations
 
@@ -68,21 +74,39 @@ def make_scene(k: int, w: int, h: int):
 def generate(nseq: int, frames_per_seq: int, w: int, h: int, qps=(20, 24, 28, 32), first_seq: int = 0,
              log=print):
     """(features [N, 578] on the device, targets [N], SKIP labels [N]) as torch tensors."""
+    items = [(make_scene(k, w, h), int(qps[k % len(qps)])) for k in range(first_seq, first_seq + nseq)]
+    return generate_from(items, frames_per_seq, log)
+
+
+def library_items(names, frames_per_seq: int, first: int, w: int, h: int, qps=(16, 22, 28, 34)):
+    """(scene, qp) pairs of the reference's scene library: every scene at every
+    QP, frames first .. first + frames_per_seq (frames 0 .. first - 1 are left to
+    the rate-distortion study)."""
+    from fdv_amd.learned import bdrate
+    out = []
+    for name, sc in bdrate.library_scenes(w, h, first + frames_per_seq + 1, names):
+        sc.data = sc.data[first:]
+        for qp in qps:
+            out.append((sc, qp))
+    return out
+
+
+def generate_from(items, frames_per_seq: int, log=print):
+    """Training tuples of (scene, qp) items, frames_per_seq P-frames each."""
     import torch
     L = G.lib()
-    nmb = (w // 16) * (h // 16)
-    N = nseq * frames_per_seq * nmb
+    N = sum(frames_per_seq * (sc.w // 16) * (sc.h // 16) for sc, _ in items)
     feats = torch.empty((N, NFEAT), dtype=torch.float32, device="cuda")
     tgt = np.zeros(N, dtype=np.int32)
     skip = np.zeros(N, dtype=np.uint8)
-    pos = torch.tensor([[(i % (w // 16)) * 16, (i // (w // 16)) * 16] for i in range(nmb)], dtype=torch.int32,
-                       device="cuda")
-    cw = torch.full((nmb,), w, dtype=torch.int32, device="cuda")
     off = 0
     t0 = time.time()
-    for k in range(first_seq, first_seq + nseq):
-        sc = make_scene(k, w, h)
-        qp = int(qps[k % len(qps)])
+    for k, (sc, qp) in enumerate(items):
+        w, h = sc.w, sc.h
+        nmb = (w // 16) * (h // 16)
+        pos = torch.tensor([[(i % (w // 16)) * 16, (i // (w // 16)) * 16] for i in range(nmb)], dtype=torch.int32,
+                           device="cuda")
+        cw = torch.full((nmb,), w, dtype=torch.int32, device="cuda")
         an = G.Analyser([(sc.w, sc.h)], nsrc=1)
         an.put_ref(0, 0, sc.frame(0))
         for f in range(1, frames_per_seq + 1):
@@ -105,8 +129,8 @@ def generate(nseq: int, frames_per_seq: int, w: int, h: int, qps=(20, 24, 28, 32
             skip[off:off + nmb] = (dec["mode"] == 0).astype(np.uint8)
             off += nmb
         an.close()
-        if (k - first_seq + 1) % 10 == 0:
-            log("  data: %d/%d sequences, %d tuples, %.1f s" % (k - first_seq + 1, nseq, off, time.time() - t0))
+        if (k + 1) % 10 == 0:
+            log("  data: %d/%d sequences, %d tuples, %.1f s" % (k + 1, len(items), off, time.time() - t0))
     return feats, torch.tensor(tgt, device="cuda"), torch.tensor(skip, device="cuda")
 
 
@@ -195,11 +219,31 @@ def main():
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--wfp", type=float, default=3.0, help="weight of false SKIP positives in the loss")
     ap.add_argument("--out", default=G.PROPOSER_WEIGHTS)
+    ap.add_argument("--library", type=int, default=0,
+                    help="P-frames per scene of the reference's scene library to add (0: none)")
+    ap.add_argument("--library-first", type=int, default=30, help="first library frame used for training")
+    ap.add_argument("--library-size", default="1280x720")
+    ap.add_argument("--holdout", default="", help="comma-separated library scenes left out of training")
     args = ap.parse_args()
+    import torch
     w, h = (int(x) for x in args.size.split("x"))
     nseq = max(1, args.frames // args.frames_per_seq)
     print("generating %d sequences x %d P-frames at %dx%d" % (nseq, args.frames_per_seq, w, h), flush=True)
     feats, tgt, skip = generate(nseq, args.frames_per_seq, w, h, log=lambda m: print(m, flush=True))
+    lib_names = []
+    if args.library > 0:
+        from fdv_amd.learned import bdrate
+        import glob
+        hold = set(x for x in args.holdout.split(",") if x)
+        lib_names = [os.path.basename(p)[:-4] for p in sorted(glob.glob(os.path.join(bdrate.SCENE_DIR, "*.scn")))
+                     if os.path.basename(p)[:-4] not in hold]
+        lw, lh = (int(x) for x in args.library_size.split("x"))
+        items = library_items(lib_names, args.library, args.library_first, lw, lh)
+        print("generating %d library sequences x %d P-frames at %dx%d" % (len(items), args.library, lw, lh),
+              flush=True)
+        lf, lt, ls = generate_from(items, args.library, log=lambda m: print(m, flush=True))
+        feats, tgt, skip = torch.cat([feats, lf]), torch.cat([tgt, lt]), torch.cat([skip, ls])
+        del lf, lt, ls
     vf, vt, vs = generate(8, 10, w, h, first_seq=100000, log=lambda m: print(m, flush=True))
     print("training on %d tuples, validating on %d" % (feats.shape[0], vf.shape[0]), flush=True)
     W, hist = train(feats, tgt, skip, epochs=args.epochs, batch=args.batch, lr=args.lr, vfeats=vf, vtgt=vt,
@@ -208,6 +252,8 @@ def main():
     W.astype("<f4").tofile(args.out)
     meta = {"tuples": int(feats.shape[0]), "frames": nseq * args.frames_per_seq, "size": [w, h],
             "qps": [20, 24, 28, 32], "epochs": args.epochs, "batch": args.batch, "lr": args.lr, "wfp": args.wfp,
+            "library": {"scenes": lib_names, "frames_per_scene": args.library, "first_frame": args.library_first,
+                        "size": args.library_size, "qps": [16, 22, 28, 34]} if lib_names else None,
             "history": hist}
     with open(os.path.splitext(args.out)[0] + ".json", "w") as f:
         json.dump(meta, f, indent=1)

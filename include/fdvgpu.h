@@ -126,9 +126,12 @@ int fdvgpu_autotune(fdvgpu *g, const int *src_slot, const int *qp, const double 
  * is the reference's. */
 
 /* One MB's proposal: 8 whole-pel vectors on the nearest reference (best first),
- * their total probability, and the SKIP probability. */
+ * the 4 lowest-cost positions of the proposer's half-res correlation map
+ * (whole pels; ties to the smaller |dx| + |dy|, then the lower position), the 8
+ * vectors' total probability, and the SKIP probability. */
 typedef struct {
     int8_t cx[8], cy[8];
+    int8_t kx[4], ky[4];
     float conf, pskip;
 } fdvgpu_proposal;
 

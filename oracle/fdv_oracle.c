@@ -1388,10 +1388,11 @@ void orc_metab_candidates(const uint8_t *cy, int w, int h, const uint8_t *const 
         if (!(P->conf >= thr)) continue;
         const int bx = (i % mw) * 16, by = (i / mw) * 16;
         for (int r = 0; r < navail; ++r) {
-            int pts[73][2], n = 0;
-            for (int c = 0; c < 8; ++c)
+            int pts[109][2], n = 0;
+            for (int c = 0; c < 12; ++c)
                 for (int o = 0; o < 9; ++o) {
-                    const int dx = (r + 1) * P->cx[c] + o % 3 - 1, dy = (r + 1) * P->cy[c] + o / 3 - 1;
+                    const int vx = c < 8 ? P->cx[c] : P->kx[c - 8], vy = c < 8 ? P->cy[c] : P->ky[c - 8];
+                    const int dx = (r + 1) * vx + o % 3 - 1, dy = (r + 1) * vy + o / 3 - 1;
                     if (dx < -16 || dx > 16 || dy < -16 || dy > 16) continue;
                     pts[n][0] = dx; pts[n][1] = dy; ++n;
                 }

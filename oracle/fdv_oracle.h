@@ -139,6 +139,7 @@ typedef struct {
 /* The learned proposer's output for one MB (fdvg_nnout). */
 typedef struct {
     int8_t cx[8], cy[8];     /* whole-pel vectors on the nearest reference */
+    int8_t kx[4], ky[4];     /* the correlation map's 4 lowest-cost positions */
     float conf, pskip;
 } orc_nnout;
 
@@ -146,8 +147,9 @@ typedef struct {
 void orc_metab_exhaustive(const uint8_t *cy, int w, int h, const uint8_t *const RY[2], int ys,
                           int navail, orc_metab *mt);
 /* The table of the learned mode: where the proposer's confidence is at least
- * thr, the whole-pel search covers the 3x3 neighbourhoods of its 8 vectors
- * (scaled by 2 on the far reference) and the zero vector, with the exhaustive
+ * thr, the whole-pel search covers the 3x3 neighbourhoods of its 12 vectors
+ * (the network's 8, then the correlation map's 4; scaled by 2 on the far
+ * reference) and the zero vector, with the exhaustive
  * search's tie order, then the sub-pel cross; elsewhere the exhaustive search.
  * flags bit 0 = SKIP probability >= sthr: the decision then weighs SKIP against
  * INTER16 only. */

@@ -144,6 +144,13 @@ def test_features_and_inference_match_oracle():
         e = np.exp(lg - lg.max())
         assert props[i]["conf"] == pytest.approx(e[order[:8]].sum() / e.sum(), abs=2e-3)
         assert props[i]["pskip"] == pytest.approx(1 / (1 + np.exp(-fw["skl"])), abs=2e-3)
+        # the correlation map's 4 lowest costs: ties to the smaller |dx| + |dy|, then the lower position
+        rel = np.rint(want[i][:, 1] * 256).astype(np.int64)
+        pos = np.arange(289)
+        key = [(int(rel[p]), abs(p % 17 - 8) + abs(p // 17 - 8), int(p)) for p in pos]
+        low = [k[2] for k in sorted(key)[:4]]
+        assert list(zip(props[i]["kx"].tolist(), props[i]["ky"].tolist())) == \
+            [(2 * (p % 17 - 8), 2 * (p // 17 - 8)) for p in low]
     an.close()
 
 

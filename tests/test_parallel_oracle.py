@@ -22,7 +22,8 @@ import pytest
 import orclib as L
 from fdv_amd import content
 
-PROPOSAL = np.dtype([("cx", "i1", 8), ("cy", "i1", 8), ("conf", "<f4"), ("pskip", "<f4")])
+PROPOSAL = np.dtype([("cx", "i1", 8), ("cy", "i1", 8), ("kx", "i1", 4), ("ky", "i1", 4),
+                     ("conf", "<f4"), ("pskip", "<f4")])
 
 
 def _luma(f, w, h):
@@ -100,6 +101,19 @@ def test_candidate_search_covers_the_exhaustive_minimum_and_thresholds():
     assert np.array_equal(mt["mv16"][:, 0], full["mv16"][:, 0])
     assert np.array_equal(mt["sad16"][:, 0], full["sad16"][:, 0])
     assert not mt["flags"].any()
+    # the same with the minimum held only by the third correlation-map position
+    co = np.zeros(nmb, PROPOSAL)
+    co["conf"] = 0.9
+    co["cx"], co["cy"] = 8, -8
+    co["kx"], co["ky"] = -10, 10
+    co["kx"][:, 2], co["ky"][:, 2] = hi["cx"][:, 0], hi["cy"][:, 0]
+    mt = L.metab_to_np(L.oracle_metab_candidates(f2, f1, None, w, h, 1, co, thr=0.7, sthr=0.7))
+    assert np.array_equal(mt["mv16"][:, 0], full["mv16"][:, 0])
+    assert np.array_equal(mt["sad16"][:, 0], full["sad16"][:, 0])
+    # and with every candidate a decoy, the table is a different one
+    co["kx"][:, 2], co["ky"][:, 2] = -10, 10
+    mt = L.metab_to_np(L.oracle_metab_candidates(f2, f1, None, w, h, 1, co, thr=0.7, sthr=0.7))
+    assert not np.array_equal(mt["mv16"][:, 0], full["mv16"][:, 0])
 
 
 @pytest.mark.parametrize("qp", [20, 28, 36])
