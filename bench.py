@@ -282,7 +282,7 @@ def main():
     # HIP events on the stream it runs on
     roof = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic, "traffic_source": traffic_src,
-            "kernel": "decision stage (k_pcost, k_prepair, k_plist, k_pencode, k_pintra)" if par else "k_analyse",
+            "kernel": "decision stage (k_pcost, k_picost, k_prepair, k_plist, k_pencode, k_pintra)" if par else "k_analyse",
             "kernel_ms": round(k_ms, 4), "bytes_per_launch": int(algo_bytes),
             "path_achieved": round(value * bytes_per_px / 1000.0, 3),
             "path_frac": round(value * bytes_per_px / 1000.0 / HBM_PEAK_GBS, 6)}
@@ -331,11 +331,16 @@ def main():
                "payload_bytes_per_step": int(nbytes / args.steps),
                "includes": "source upload, device analysis, stream fetch, host entropy coding (one thread per "
                            "sequence, up to 16)"}
+    # BD-rate of this search mode against the exhaustive one, from the committed
+    # rate-distortion study over the reference's scene library (bdrate.py --library)
     bd = None
-    bd_path = os.path.join(ROOT, "fdv_amd", "learned", "bdrate_720p.json")
-    if mode == G.ME_LEARNED and args.workload == "720p" and os.path.exists(bd_path):
+    bd_path = os.path.join(ROOT, "fdv_amd", "learned", "bdrate_library_720p.json")
+    if mode in (G.ME_LEARNED, G.ME_PARALLEL) and os.path.exists(bd_path):
         with open(bd_path) as f:
-            bd = json.load(f)["bd_rate_percent"]
+            rec = json.load(f)[args.mode]
+        bd = {"median_percent": round(rec["summary"]["median"], 2), "mean_percent": round(rec["summary"]["mean"], 2),
+              "scenes": rec["summary"]["scenes"], "worst": rec["summary"]["worst"],
+              "source": "fdv_amd/learned/bdrate_library_720p.json"}
     # the CPU leg runs on the headline workload only
     cpu = None if (args.no_cpu or args.workload != "720p") else cpu_baseline(frames, W, layout, QP, args.cpu_seconds)
     out = {

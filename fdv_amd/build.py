@@ -4,6 +4,8 @@
   (include/fdvgpu.h), plus the C11 host library (include/fdv_amd.h): entropy
   packing of P-frame payloads and the sequence drivers.
 * oracle/ -- the parity oracle (test infrastructure), via oracle/Makefile.
+* fdv_amd/learned/_scenes/ -- the scene library of the rate-distortion study,
+  via fdv_amd/learned/Makefile (where the reference tree is present).
 
 Run as `python -m fdv_amd.build` or through __graft_entry__.build().
 """
@@ -68,8 +70,18 @@ def build_oracle() -> None:
         raise RuntimeError("oracle build failed:\n" + r.stdout + r.stderr)
 
 
+def build_scenes() -> None:
+    """The scene library of the rate-distortion study (fdv_amd/learned/Makefile;
+    a no-op where the reference tree is absent)."""
+    r = subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "fdv_amd", "learned"), "scenes"],
+                       capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("scene library build failed:\n" + r.stdout + r.stderr)
+
+
 def build_all(verbose: bool = False) -> None:
     build_oracle()
+    build_scenes()
     build_gpu(verbose)
 
 
