@@ -1050,18 +1050,75 @@ template <int S> __device__ __forceinline__ int lumaq_t(const uint8_t *w, int X,
     }
 }
 
+// SSD of the 16x16 block `curb` against its prediction from a tile (row stride
+// S, 4-byte aligned, the block's (0, 0) at tile (2, 2)) at quarter-pel phase ph,
+// by one wavefront: lane L takes column L & 15, rows 4 (L >> 4) .. + 3, loads
+// the 9 x 6 tile samples those need once (27 dword reads) and forms every
+// sample of lumaq_t's definition from registers.
+template <int S>
+__device__ __forceinline__ int ssd_tile_q(const uint8_t *tile, const uint8_t *curb, int ph, int L) {
+    const int x = L & 15, y0 = (L >> 4) * 4, sh = x & 3;
+    int V[9][6];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        const uint32_t *row = (const uint32_t *)(tile + (y0 + i) * S + (x & ~3));
+        const uint32_t w0 = row[0], w1 = row[1], w2 = row[2];
+        const uint32_t a = __builtin_amdgcn_alignbyte(w1, w0, sh), b = __builtin_amdgcn_alignbyte(w2, w1, sh);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) V[i][j] = (int)((a >> (8 * j)) & 255u);
+        V[i][4] = (int)(b & 255u);
+        V[i][5] = (int)((b >> 8) & 255u);
+    }
+    int Hr[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Hr[i] = tap6(V[i][0], V[i][1], V[i][2], V[i][3], V[i][4], V[i][5]);
+    int e = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int G = V[k + 2][2], Gr = V[k + 2][3], Gd = V[k + 3][2];
+        const int b = clip8((Hr[k + 2] + 16) >> 5), bd = clip8((Hr[k + 3] + 16) >> 5);
+        const int hh = clip8((tap6(V[k][2], V[k + 1][2], V[k + 2][2], V[k + 3][2], V[k + 4][2], V[k + 5][2]) + 16) >> 5);
+        const int hhr = clip8((tap6(V[k][3], V[k + 1][3], V[k + 2][3], V[k + 3][3], V[k + 4][3], V[k + 5][3]) + 16) >> 5);
+        const int jj = clip8((tap6(Hr[k], Hr[k + 1], Hr[k + 2], Hr[k + 3], Hr[k + 4], Hr[k + 5]) + 512) >> 10);
+        int p;
+        switch (ph) {
+        case 0:  p = G; break;
+        case 1:  p = av2(G, b); break;
+        case 2:  p = b; break;
+        case 3:  p = av2(b, Gr); break;
+        case 4:  p = av2(G, hh); break;
+        case 5:  p = av2(b, hh); break;
+        case 6:  p = av2(b, jj); break;
+        case 7:  p = av2(b, hhr); break;
+        case 8:  p = hh; break;
+        case 9:  p = av2(hh, jj); break;
+        case 10: p = jj; break;
+        case 11: p = av2(jj, hhr); break;
+        case 12: p = av2(hh, Gd); break;
+        case 13: p = av2(hh, bd); break;
+        case 14: p = av2(jj, bd); break;
+        default: p = av2(hhr, bd); break;
+        }
+        const int d = curb[(y0 + k) * 16 + x] - p;
+        e += d * d;
+    }
+    return e;
+}
+
 // The decision wavefront of one sequence inside one workgroup: wave w replays
 // rows w, w + NW, ...; a row's motion records go to an LDS ring slot per row
 // and its progress to prog[w], which the wave of the row below polls.  Each
 // wave stages its row's costs and tables PR_CHUNK MBs at a time.
 #ifdef FDVG_PHASE_PROF
 #define RPP(k) do { if (L == 0) { long long t_ = clock64(); atomicAdd(&g_phase[8 + (k)], (unsigned long long)(t_ - rp_last)); rp_last = t_; } } while (0)
+#define RPC(k) do { if (L == 0) atomicAdd(&g_phase[(k)], 1ull); } while (0)
 #else
 #define RPP(k) do { } while (0)
+#define RPC(k) do { } while (0)
 #endif
 #define PR_CHUNK 8
 #define PR_TS 24                         // tile stride: 21 samples a row
-struct PrWave {
+struct __align__(16) PrWave {
     fdvg_pcost pc[PR_CHUNK];
     fdvg_metab mt[PR_CHUNK];
     uint8_t cur[PR_CHUNK][256];          // the source MBs, for SSDs outside the precomputed set
@@ -1220,6 +1277,7 @@ k_prepair(const fdvg_seq *__restrict__ seqs, int mwmax, unsigned *ctl,
                 if (hm) {
                     ssd[c] = __builtin_amdgcn_readlane(se_l, __builtin_ctzll(hm));
                     RPP(2);
+                    RPC(0);
                 } else {
                     uint8_t tv[7];
                     if (c == 0) {
@@ -1240,15 +1298,20 @@ k_prepair(const fdvg_seq *__restrict__ seqs, int mwmax, unsigned *ctl,
                         if (k < 21 * 21) W.tile[(k / 21) * PR_TS + k % 21] = tv[j];
                     }
                     wave_sync();
-                    const int ph = ((y & 3) << 2) | (x & 3);
-                    int e = 0;
-                    for (int k = L; k < 256; k += 64) {
-                        const int d = curb[k] - lumaq_t<PR_TS>(W.tile, (k & 15) + 2, (k >> 4) + 2, ph);
-                        e += d * d;
+                    ssd[c] = wave_sum(ssd_tile_q<PR_TS>(W.tile, curb, ((y & 3) << 2) | (x & 3), L));
+#ifdef FDVG_PHASE_PROF
+                    {   // the per-sample definition, for the profiling build's self-check
+                        const int ph = ((y & 3) << 2) | (x & 3);
+                        int e = 0;
+                        for (int k = L; k < 256; k += 64) {
+                            const int d = curb[k] - lumaq_t<PR_TS>(W.tile, (k & 15) + 2, (k >> 4) + 2, ph);
+                            e += d * d;
+                        }
+                        if (wave_sum(e) != ssd[c]) RPC(3);
                     }
-                    ssd[c] = wave_sum(e);
+#endif
                     wave_sync();
-                    RPP(3);
+                    if (c == 0) { RPP(3); RPC(1); } else { RPP(7); RPC(2); }
                 }
             }
             int Ds = ssd[0], bmi = 0;
