@@ -36,32 +36,87 @@ __device__ __forceinline__ uint32_t win_word(const uint8_t *w, int y, int x) {
 
 __device__ __forceinline__ unsigned long long umin64(unsigned long long a, unsigned long long b) { return a < b ? a : b; }
 
+// SAD of 4 predicted samples against the source MB: block-relative column x,
+// rows y0 .. y0 + 3 of the block at (ox, oy) in the MB, window w (stride WIN),
+// quarter-pel vector (px, py).  The 9 x 6 window samples these need are read
+// once (27 dword reads) and every quarter-pel sample of lumaq's definition is
+// formed in registers; the phase may differ between lanes, so the sample is
+// selected without branches: av2 of two of the eight candidate values.
+__device__ __forceinline__ int pick8(int i, int v0, int v1, int v2, int v3, int v4, int v5, int v6, int v7) {
+    int r = v0;
+    r = i == 1 ? v1 : r;
+    r = i == 2 ? v2 : r;
+    r = i == 3 ? v3 : r;
+    r = i == 4 ? v4 : r;
+    r = i == 5 ? v5 : r;
+    r = i == 6 ? v6 : r;
+    return i == 7 ? v7 : r;
+}
+__device__ __forceinline__ int sad4_q(const uint8_t *w, const uint8_t *cur, int ox, int oy, int x, int y0, int px,
+                                      int py) {
+    const int X = WOFF + ox + x + (px >> 2), Y = WOFF + oy + y0 + (py >> 2), ph = ((py & 3) << 2) | (px & 3);
+    const int c0 = X - 2, sh = c0 & 3;
+    int V[9][6];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        const uint32_t *row = (const uint32_t *)(w + (Y - 2 + i) * WIN + (c0 & ~3));
+        const uint32_t w0 = row[0], w1 = row[1], w2 = row[2];
+        const uint32_t a = __builtin_amdgcn_alignbyte(w1, w0, sh), b = __builtin_amdgcn_alignbyte(w2, w1, sh);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) V[i][j] = (int)((a >> (8 * j)) & 255u);
+        V[i][4] = (int)(b & 255u);
+        V[i][5] = (int)((b >> 8) & 255u);
+    }
+    int Hr[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Hr[i] = tap6(V[i][0], V[i][1], V[i][2], V[i][3], V[i][4], V[i][5]);
+    // per phase: the two values averaged (indices into G, b, hh, j, Gr, Gd, bd, hhr)
+    const uint64_t A = 0x7322332211101100ull, B = 0x6665733273224110ull;
+    // A, B: 4 bits per phase, phase 0 in the low nibble
+    const int sa = (int)((A >> (4 * ph)) & 15u), sb = (int)((B >> (4 * ph)) & 15u);
+    int acc = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int G = V[k + 2][2], Gr = V[k + 2][3], Gd = V[k + 3][2];
+        const int b = clip8((Hr[k + 2] + 16) >> 5), bd = clip8((Hr[k + 3] + 16) >> 5);
+        const int hh = clip8((tap6(V[k][2], V[k + 1][2], V[k + 2][2], V[k + 3][2], V[k + 4][2], V[k + 5][2]) + 16) >> 5);
+        const int hhr = clip8((tap6(V[k][3], V[k + 1][3], V[k + 2][3], V[k + 3][3], V[k + 4][3], V[k + 5][3]) + 16) >> 5);
+        const int jj = clip8((tap6(Hr[k], Hr[k + 1], Hr[k + 2], Hr[k + 3], Hr[k + 4], Hr[k + 5]) + 512) >> 10);
+        const int d = cur[(oy + y0 + k) * 16 + ox + x] -
+                      av2(pick8(sa, G, b, hh, jj, Gr, Gd, bd, hhr), pick8(sb, G, b, hh, jj, Gr, Gd, bd, hhr));
+        acc += d < 0 ? -d : d;
+    }
+    return acc;
+}
+
 // The reference's sub-pel step (half- then quarter-pel four-point cross,
 // unclamped, skipped once a block's SAD is 0: fdv.h:3453-3461) for all six
 // blocks at once (0, 1: 16x16 on references 0 and 1; 2..5: the 8x8 quadrants
 // on reference 0): at each step wave k scores cross point k of every block
-// still refining, then every block takes its best point (strict <, in cross
-// order).  Each block's result is that of refining it alone.
+// still refining (16x16: a lane per column and 4 rows; quadrants: all four in
+// one pass, 16 lanes each), then every block takes its best point (strict <,
+// in cross order).  Each block's result is that of refining it alone.
 __device__ __forceinline__ void refine_all(MSmem &s, int nr, int *bmx, int *bmy, int *bbest) {
     const int t = threadIdx.x, k = t >> 6, tt = t & 63;
     int par = 0;
 #pragma unroll
     for (int st = 2; st >= 1; st >>= 1) {
 #pragma unroll
-        for (int b = 0; b < 6; ++b) {
+        for (int b = 0; b < 2; ++b) {
             if (bbest[b] <= 0 || (b == 1 && nr < 2)) continue;        // uniform
-            const int bs = b < 2 ? 16 : 8, npx = bs * bs / 64;
-            const int ox = b < 2 ? 0 : ((b - 2) & 1) * 8, oy = b < 2 ? 0 : ((b - 2) >> 1) * 8;
-            const uint8_t *w = s.win[b == 1 ? 1 : 0];
             const int px = bmx[b] + k_CROSS[k][0] * st, py = bmy[b] + k_CROSS[k][1] * st;
-            int acc = 0;
-            for (int j = 0; j < npx; ++j) {
-                const int p = tt * npx + j, y = p / bs, x = p % bs;
-                const int d = s.cur[(oy + y) * 16 + ox + x] - lumaq(w, ox + x, oy + y, px, py);
-                acc += d < 0 ? -d : d;
-            }
+            int acc = sad4_q(s.win[b], s.cur, 0, 0, tt & 15, (tt >> 4) * 4, px, py);
             acc = gsum<64>(acc);
             if (tt == 0) s.sad[par][b][k] = acc;
+        }
+        if (bbest[2] > 0 || bbest[3] > 0 || bbest[4] > 0 || bbest[5] > 0) {   // uniform
+            const int qd = tt >> 4;
+            const int mx = qd == 0 ? bmx[2] : qd == 1 ? bmx[3] : qd == 2 ? bmx[4] : bmx[5];
+            const int my = qd == 0 ? bmy[2] : qd == 1 ? bmy[3] : qd == 2 ? bmy[4] : bmy[5];
+            int acc = sad4_q(s.win[0], s.cur, (qd & 1) * 8, (qd >> 1) * 8, tt & 7, ((tt >> 3) & 1) * 4,
+                             mx + k_CROSS[k][0] * st, my + k_CROSS[k][1] * st);
+            acc = gsum<16>(acc);
+            if ((tt & 15) == 0) s.sad[par][2 + qd][k] = acc;
         }
         __syncthreads();
 #pragma unroll
