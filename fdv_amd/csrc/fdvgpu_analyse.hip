@@ -894,11 +894,14 @@ k_pcost(const fdvg_seq *__restrict__ seqs, fdvg_pcost *__restrict__ pcs, uint8_t
         if (k < 3) { add(A->mv16[1][0], A->mv16[1][1], 1); add(A->mv8[3][0], A->mv8[3][1], 0); }
     }
     {
-        const int x = tid & 15, y = tid >> 4;
-        for (int c = 0; c < ncs; ++c) {
-            const int d = s.cur[tid] - lumaq(s.win[csr[c]], x, y, csx[c], csy[c]);
-            const int e = wave_sum(d * d);
-            if ((tid & 63) == 0) sred[c][wave] = e;
+        // wave w takes candidates w, w + 4, w + 8; lane L column L & 15, rows 4 (L >> 4) .. + 3
+        const int L = tid & 63, x = L & 15, y0 = (L >> 4) * 4;
+        for (int c = wave; c < ncs; c += 4) {                 // uniform in the wave
+            const int vx = csx[c], vy = csy[c];
+            const int e = qerr4<WIN, true>(s.win[csr[c]], WOFF + x + (vx >> 2), WOFF + y0 + (vy >> 2),
+                                           ((vy & 3) << 2) | (vx & 3), s.cur + y0 * 16 + x, 16);
+            const int t = wave_sum(e);
+            if (L == 0) sred[c][0] = t;
         }
     }
     __syncthreads();
@@ -907,10 +910,10 @@ k_pcost(const fdvg_seq *__restrict__ seqs, fdvg_pcost *__restrict__ pcs, uint8_t
         P->scx[c] = (int16_t)(c < ncs ? csx[c] : 0);
         P->scy[c] = (int16_t)(c < ncs ? csy[c] : 0);
         P->scr[c] = (uint8_t)(c < ncs ? csr[c] : 0);
-        P->sce[c] = c < ncs ? sred[c][0] + sred[c][1] + sred[c][2] + sred[c][3] : 0;
+        P->sce[c] = c < ncs ? sred[c][0] : 0;
     }
     if (tid == 0) P->nsc = (uint8_t)ncs;
-    const int e00 = sred[0][0] + sred[0][1] + sred[0][2] + sred[0][3];     // candidate 0 is (0, 0) on ref 0
+    const int e00 = sred[0][0];                      // candidate 0 is (0, 0) on ref 0
     const int qp = q.qp;
     const double lambda = q.lambda;
     // INTER16 on each reference
@@ -1051,58 +1054,12 @@ template <int S> __device__ __forceinline__ int lumaq_t(const uint8_t *w, int X,
 }
 
 // SSD of the 16x16 block `curb` against its prediction from a tile (row stride
-// S, 4-byte aligned, the block's (0, 0) at tile (2, 2)) at quarter-pel phase ph,
-// by one wavefront: lane L takes column L & 15, rows 4 (L >> 4) .. + 3, loads
-// the 9 x 6 tile samples those need once (27 dword reads) and forms every
-// sample of lumaq_t's definition from registers.
+// S, the block's (0, 0) at tile (2, 2)) at quarter-pel phase ph, by one
+// wavefront: lane L takes column L & 15, rows 4 (L >> 4) .. + 3 (qerr4).
 template <int S>
 __device__ __forceinline__ int ssd_tile_q(const uint8_t *tile, const uint8_t *curb, int ph, int L) {
-    const int x = L & 15, y0 = (L >> 4) * 4, sh = x & 3;
-    int V[9][6];
-#pragma unroll
-    for (int i = 0; i < 9; ++i) {
-        const uint32_t *row = (const uint32_t *)(tile + (y0 + i) * S + (x & ~3));
-        const uint32_t w0 = row[0], w1 = row[1], w2 = row[2];
-        const uint32_t a = __builtin_amdgcn_alignbyte(w1, w0, sh), b = __builtin_amdgcn_alignbyte(w2, w1, sh);
-#pragma unroll
-        for (int j = 0; j < 4; ++j) V[i][j] = (int)((a >> (8 * j)) & 255u);
-        V[i][4] = (int)(b & 255u);
-        V[i][5] = (int)((b >> 8) & 255u);
-    }
-    int Hr[9];
-#pragma unroll
-    for (int i = 0; i < 9; ++i) Hr[i] = tap6(V[i][0], V[i][1], V[i][2], V[i][3], V[i][4], V[i][5]);
-    int e = 0;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const int G = V[k + 2][2], Gr = V[k + 2][3], Gd = V[k + 3][2];
-        const int b = clip8((Hr[k + 2] + 16) >> 5), bd = clip8((Hr[k + 3] + 16) >> 5);
-        const int hh = clip8((tap6(V[k][2], V[k + 1][2], V[k + 2][2], V[k + 3][2], V[k + 4][2], V[k + 5][2]) + 16) >> 5);
-        const int hhr = clip8((tap6(V[k][3], V[k + 1][3], V[k + 2][3], V[k + 3][3], V[k + 4][3], V[k + 5][3]) + 16) >> 5);
-        const int jj = clip8((tap6(Hr[k], Hr[k + 1], Hr[k + 2], Hr[k + 3], Hr[k + 4], Hr[k + 5]) + 512) >> 10);
-        int p;
-        switch (ph) {
-        case 0:  p = G; break;
-        case 1:  p = av2(G, b); break;
-        case 2:  p = b; break;
-        case 3:  p = av2(b, Gr); break;
-        case 4:  p = av2(G, hh); break;
-        case 5:  p = av2(b, hh); break;
-        case 6:  p = av2(b, jj); break;
-        case 7:  p = av2(b, hhr); break;
-        case 8:  p = hh; break;
-        case 9:  p = av2(hh, jj); break;
-        case 10: p = jj; break;
-        case 11: p = av2(jj, hhr); break;
-        case 12: p = av2(hh, Gd); break;
-        case 13: p = av2(hh, bd); break;
-        case 14: p = av2(jj, bd); break;
-        default: p = av2(hhr, bd); break;
-        }
-        const int d = curb[(y0 + k) * 16 + x] - p;
-        e += d * d;
-    }
-    return e;
+    const int x = L & 15, y0 = (L >> 4) * 4;
+    return qerr4<S, true>(tile, x + 2, y0 + 2, ph, curb + y0 * 16 + x, 16);
 }
 
 // The decision wavefront of one sequence inside one workgroup: wave w replays

@@ -93,6 +93,59 @@ __device__ __forceinline__ int wj(const uint8_t *w, int x, int y) {
     return clip8((v + 512) >> 10);
 }
 __device__ __forceinline__ int av2(int a, int b) { return (a + b + 1) >> 1; }
+__device__ __forceinline__ int pick8(int i, int v0, int v1, int v2, int v3, int v4, int v5, int v6, int v7) {
+    int r = v0;
+    r = i == 1 ? v1 : r;
+    r = i == 2 ? v2 : r;
+    r = i == 3 ? v3 : r;
+    r = i == 4 ? v4 : r;
+    r = i == 5 ? v5 : r;
+    r = i == 6 ? v6 : r;
+    return i == 7 ? v7 : r;
+}
+// Error of 4 vertically adjacent predicted luma samples: the samples at
+// (X, Y .. Y + 3) of plane w (row stride S, 4-byte aligned, S a multiple of 4)
+// at quarter-pel phase ph, against cur[0], cur[cs], cur[2 cs], cur[3 cs]; the
+// sum of |d| (SQ false) or d^2 (SQ true).  The 9 x 6 samples those need
+// (rows Y - 2 .. Y + 6, columns X - 2 .. X + 3) are read once as 27 dwords, and
+// every quarter-pel sample of lumaq's definition is formed in registers; the
+// phase may differ between lanes, so the sample is picked without branches as
+// av2 of two of the eight candidate values.  Reads dwords up to 9 bytes past
+// column X + 3 of each row.
+template <int S, bool SQ>
+__device__ __forceinline__ int qerr4(const uint8_t *w, int X, int Y, int ph, const uint8_t *cur, int cs) {
+    const int c0 = X - 2, sh = c0 & 3;
+    int V[9][6];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        const uint32_t *row = (const uint32_t *)(w + (Y - 2 + i) * S + (c0 & ~3));
+        const uint32_t w0 = row[0], w1 = row[1], w2 = row[2];
+        const uint32_t a = __builtin_amdgcn_alignbyte(w1, w0, sh), b = __builtin_amdgcn_alignbyte(w2, w1, sh);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) V[i][j] = (int)((a >> (8 * j)) & 255u);
+        V[i][4] = (int)(b & 255u);
+        V[i][5] = (int)((b >> 8) & 255u);
+    }
+    int Hr[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Hr[i] = tap6(V[i][0], V[i][1], V[i][2], V[i][3], V[i][4], V[i][5]);
+    // per phase (4 bits each, phase 0 lowest): the two values averaged, as
+    // indices into G, b, hh, j, G(x+1), G(y+1), b(y+1), hh(x+1)
+    const uint64_t A = 0x7322332211101100ull, B = 0x6665733273224110ull;
+    const int sa = (int)((A >> (4 * ph)) & 15u), sb = (int)((B >> (4 * ph)) & 15u);
+    int acc = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int G = V[k + 2][2], Gr = V[k + 2][3], Gd = V[k + 3][2];
+        const int b = clip8((Hr[k + 2] + 16) >> 5), bd = clip8((Hr[k + 3] + 16) >> 5);
+        const int hh = clip8((tap6(V[k][2], V[k + 1][2], V[k + 2][2], V[k + 3][2], V[k + 4][2], V[k + 5][2]) + 16) >> 5);
+        const int hhr = clip8((tap6(V[k][3], V[k + 1][3], V[k + 2][3], V[k + 3][3], V[k + 4][3], V[k + 5][3]) + 16) >> 5);
+        const int jj = clip8((tap6(Hr[k], Hr[k + 1], Hr[k + 2], Hr[k + 3], Hr[k + 4], Hr[k + 5]) + 512) >> 10);
+        const int d = cur[k * cs] - av2(pick8(sa, G, b, hh, jj, Gr, Gd, bd, hhr), pick8(sb, G, b, hh, jj, Gr, Gd, bd, hhr));
+        acc += SQ ? d * d : (d < 0 ? -d : d);
+    }
+    return acc;
+}
 // Predicted luma sample for the MB-relative pixel (x, y) displaced by (mvx, mvy).
 __device__ __forceinline__ int lumaq(const uint8_t *w, int x, int y, int mvx, int mvy) {
     int X = WOFF + x + (mvx >> 2), Y = WOFF + y + (mvy >> 2);

@@ -37,56 +37,12 @@ __device__ __forceinline__ uint32_t win_word(const uint8_t *w, int y, int x) {
 __device__ __forceinline__ unsigned long long umin64(unsigned long long a, unsigned long long b) { return a < b ? a : b; }
 
 // SAD of 4 predicted samples against the source MB: block-relative column x,
-// rows y0 .. y0 + 3 of the block at (ox, oy) in the MB, window w (stride WIN),
-// quarter-pel vector (px, py).  The 9 x 6 window samples these need are read
-// once (27 dword reads) and every quarter-pel sample of lumaq's definition is
-// formed in registers; the phase may differ between lanes, so the sample is
-// selected without branches: av2 of two of the eight candidate values.
-__device__ __forceinline__ int pick8(int i, int v0, int v1, int v2, int v3, int v4, int v5, int v6, int v7) {
-    int r = v0;
-    r = i == 1 ? v1 : r;
-    r = i == 2 ? v2 : r;
-    r = i == 3 ? v3 : r;
-    r = i == 4 ? v4 : r;
-    r = i == 5 ? v5 : r;
-    r = i == 6 ? v6 : r;
-    return i == 7 ? v7 : r;
-}
+// rows y0 .. y0 + 3 of the block at (ox, oy) in the MB, window w, quarter-pel
+// vector (px, py).
 __device__ __forceinline__ int sad4_q(const uint8_t *w, const uint8_t *cur, int ox, int oy, int x, int y0, int px,
                                       int py) {
-    const int X = WOFF + ox + x + (px >> 2), Y = WOFF + oy + y0 + (py >> 2), ph = ((py & 3) << 2) | (px & 3);
-    const int c0 = X - 2, sh = c0 & 3;
-    int V[9][6];
-#pragma unroll
-    for (int i = 0; i < 9; ++i) {
-        const uint32_t *row = (const uint32_t *)(w + (Y - 2 + i) * WIN + (c0 & ~3));
-        const uint32_t w0 = row[0], w1 = row[1], w2 = row[2];
-        const uint32_t a = __builtin_amdgcn_alignbyte(w1, w0, sh), b = __builtin_amdgcn_alignbyte(w2, w1, sh);
-#pragma unroll
-        for (int j = 0; j < 4; ++j) V[i][j] = (int)((a >> (8 * j)) & 255u);
-        V[i][4] = (int)(b & 255u);
-        V[i][5] = (int)((b >> 8) & 255u);
-    }
-    int Hr[9];
-#pragma unroll
-    for (int i = 0; i < 9; ++i) Hr[i] = tap6(V[i][0], V[i][1], V[i][2], V[i][3], V[i][4], V[i][5]);
-    // per phase: the two values averaged (indices into G, b, hh, j, Gr, Gd, bd, hhr)
-    const uint64_t A = 0x7322332211101100ull, B = 0x6665733273224110ull;
-    // A, B: 4 bits per phase, phase 0 in the low nibble
-    const int sa = (int)((A >> (4 * ph)) & 15u), sb = (int)((B >> (4 * ph)) & 15u);
-    int acc = 0;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const int G = V[k + 2][2], Gr = V[k + 2][3], Gd = V[k + 3][2];
-        const int b = clip8((Hr[k + 2] + 16) >> 5), bd = clip8((Hr[k + 3] + 16) >> 5);
-        const int hh = clip8((tap6(V[k][2], V[k + 1][2], V[k + 2][2], V[k + 3][2], V[k + 4][2], V[k + 5][2]) + 16) >> 5);
-        const int hhr = clip8((tap6(V[k][3], V[k + 1][3], V[k + 2][3], V[k + 3][3], V[k + 4][3], V[k + 5][3]) + 16) >> 5);
-        const int jj = clip8((tap6(Hr[k], Hr[k + 1], Hr[k + 2], Hr[k + 3], Hr[k + 4], Hr[k + 5]) + 512) >> 10);
-        const int d = cur[(oy + y0 + k) * 16 + ox + x] -
-                      av2(pick8(sa, G, b, hh, jj, Gr, Gd, bd, hhr), pick8(sb, G, b, hh, jj, Gr, Gd, bd, hhr));
-        acc += d < 0 ? -d : d;
-    }
-    return acc;
+    return qerr4<WIN, false>(w, WOFF + ox + x + (px >> 2), WOFF + oy + y0 + (py >> 2), ((py & 3) << 2) | (px & 3),
+                             cur + (oy + y0) * 16 + ox + x, 16);
 }
 
 // The reference's sub-pel step (half- then quarter-pel four-point cross,
