@@ -941,9 +941,18 @@ k_pcost(const fdvg_seq *__restrict__ seqs, fdvg_pcost *__restrict__ pcs, uint8_t
             edge[(size_t)mb * 32 + tid] = skip ? s.win[0][(WOFF + y) * WIN + WOFF + x] : sedge[tid];
         }
     }
-    if (T->flags & 1) {                      // the proposer expects SKIP: no INTER8 (and no INTRA)
-        if (tid == 0) P->d8 = P->b8 = 0;
-        return;
+    // INTER8 is evaluated only when the 16x16 SAD of the chosen reference (either
+    // table SAD) exceeds the quadrants' sum by more than 10% (fdv.h:6014); where
+    // neither does, its cost is never read.  Also not where the proposer expects
+    // SKIP.
+    {
+        const int ssad = T->sad8[0] + T->sad8[1] + T->sad8[2] + T->sad8[3];
+        const bool g0 = T->sad16[0] > 0 && ssad * 10 >= T->sad16[0] * 9;
+        const bool g1 = q.navail < 2 || (T->sad16[1] > 0 && ssad * 10 >= T->sad16[1] * 9);
+        if ((T->flags & 1) || (g0 && g1)) {
+            if (tid == 0) P->d8 = P->b8 = 0;
+            return;
+        }
     }
     // INTER8 from the quadrant vectors
     {

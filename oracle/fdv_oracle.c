@@ -1507,6 +1507,11 @@ void orc_pframe_costs_par_nb(const uint8_t *cy, const uint8_t *nb, int w, int h,
                 P->dn = (int32_t)dn; P->bn = bn; P->nsub = (uint8_t)sel;
             }
             {
+                /* never read where the INTER8 gate closes for either table SAD */
+                const int ssad = T->sad8[0] + T->sad8[1] + T->sad8[2] + T->sad8[3];
+                const int g0 = T->sad16[0] > 0 && ssad * 10 >= T->sad16[0] * 9;
+                const int g1 = navail < 2 || (T->sad16[1] > 0 && ssad * 10 >= T->sad16[1] * 9);
+                if (g0 && g1) continue;
                 double d8 = 0.0; int b8 = 3;
                 for (int q = 0; q < 4; ++q) {
                     const int qx = (q & 1) * 8, qy = (q >> 1) * 8;
