@@ -331,6 +331,37 @@ def main():
                "payload_bytes_per_step": int(nbytes / args.steps),
                "includes": "source upload, device analysis, stream fetch, host entropy coding (one thread per "
                            "sequence, up to 16)"}
+        # the same steps pipelined (fixed QP): frame k + 1 is analysed while the
+        # host packs frame k (fdvamd_pframe_submit / fdvamd_pframe_collect)
+        enc = G.Encoder(sizes, me_mode=mode)
+        if mode == G.ME_LEARNED:
+            enc.set_proposer()
+        for b in range(nb):
+            enc.set_ref(b, 0, per_seq[b][0])
+        tcs = [G.TabCache() for _ in sizes]
+
+        def submit(i):
+            src = 1 + (i % (nframes - 1))
+            enc.pframe_submit([per_seq[b][src] for b in range(nb)], [1 if i == 0 else 2] * nb, qps)
+
+        def collect():
+            return sum(len(p) for p in enc.pframe_collect(tcs))
+
+        submit(0)
+        for i in range(1, args.warmup):
+            submit(i)
+            collect()
+        te = time.perf_counter()
+        pbytes = 0
+        for i in range(args.warmup, nsteps):
+            submit(i)
+            pbytes += collect()
+        pbytes += collect()
+        dp = time.perf_counter() - te
+        enc.close()
+        e2e["pipelined"] = {"value": round(args.steps * px_frame / dp / 1e6, 3), "unit": "Mpix/s",
+                            "ms_per_step": round(dp / args.steps * 1e3, 4),
+                            "payload_bytes_per_step": int(pbytes / args.steps)}
     # BD-rate of this search mode against the exhaustive one, from the committed
     # rate-distortion study over the reference's scene library (bdrate.py --library)
     bd = None

@@ -23,6 +23,8 @@ extern "C" int fdvg_launch_deblock(const void *, int, int, int, hipStream_t);
 extern "C" int fdvg_launch_border(const void *, int, int, hipStream_t);
 extern "C" int fdvg_launch_pad(const void *, const uint8_t *const *, int, int, hipStream_t);
 extern "C" int fdvg_launch_zero(uint32_t *, int, hipStream_t);
+extern "C" int fdvg_launch_stage(const uint32_t *, const unsigned *, int, const uint8_t *, size_t, uint32_t *, uint8_t *,
+                                 hipStream_t);
 extern "C" int fdvg_launch_symbols(const void *, int, int, const fdvg_mbrec *, uint32_t *, uint32_t *,
                                    uint32_t *, uint8_t *, size_t, int, hipStream_t);
 
@@ -119,6 +121,15 @@ struct fdvgpu {
     size_t compact_cap;
     uint32_t *flags_h;
     hipEvent_t ev[5];
+    // pipelined frame steps (fdvgpu_submit / fdvgpu_collect), two slots: the
+    // pinned upload copies of the sources, the host buffers the staging kernel
+    // writes (mapped for the device), and the event after that kernel
+    uint8_t *pipe_up[2];
+    uint8_t *pipe_sc[2], *pipe_sc_d[2];
+    uint32_t *pipe_sb[2], *pipe_sb_d[2];
+    hipEvent_t pipe_ev[2];
+    int pipe_nev;                   // events created
+    int pipe_ok;                    // all of the above allocated
     int have_frame, fetched;
     int can_rewind;                 // the last call was fdvgpu_pframe: its rotation can be undone
 };
@@ -165,6 +176,12 @@ extern "C" void fdvgpu_close(fdvgpu *g) {
     if (g->ev_rep) (void)hipEventDestroy(g->ev_rep);
     (void)hipFree(g->pflag_d);
     (void)hipFree(g->edge_d);
+    for (int k = 0; k < 2; ++k) {
+        if (g->pipe_up[k]) (void)hipHostFree(g->pipe_up[k]);
+        if (g->pipe_sc[k]) (void)hipHostFree(g->pipe_sc[k]);
+        if (g->pipe_sb[k]) (void)hipHostFree(g->pipe_sb[k]);
+        if (k < g->pipe_nev) (void)hipEventDestroy(g->pipe_ev[k]);
+    }
     free(g->S);
     free(g);
 }
@@ -508,6 +525,75 @@ extern "C" int fdvgpu_pframe(fdvgpu *g, const int *src_slot, const int *qp, cons
     g->have_frame = 1;
     g->fetched = 0;
     g->can_rewind = 1;
+    return 0;
+}
+
+// ---- pipelined frame steps
+
+static int pipe_alloc(fdvgpu *g) {
+    if (g->pipe_ok) return 0;
+    if (g->pipe_nev || g->pipe_up[0]) { seterr("pipelined buffers: an earlier allocation failed"); return -1; }
+    size_t up = 0;
+    for (int s = 0; s < g->nseq; ++s) up += (size_t)g->S[s].w * g->S[s].h + 2 * (size_t)g->S[s].cw * g->S[s].ch;
+    const size_t nb = ((size_t)g->nseq * FDVG_NS + 3) * sizeof(uint32_t);
+    for (int k = 0; k < 2; ++k) {
+        CK(hipHostMalloc(&g->pipe_up[k], up, hipHostMallocDefault));
+        CK(hipHostMalloc(&g->pipe_sc[k], g->compact_cap, hipHostMallocCoherent | hipHostMallocMapped));
+        CK(hipHostMalloc(&g->pipe_sb[k], nb, hipHostMallocCoherent | hipHostMallocMapped));
+        CK(hipHostGetDevicePointer((void **)&g->pipe_sc_d[k], g->pipe_sc[k], 0));
+        CK(hipHostGetDevicePointer((void **)&g->pipe_sb_d[k], g->pipe_sb[k], 0));
+        CK(hipEventCreateWithFlags(&g->pipe_ev[k], hipEventDisableTiming));
+        g->pipe_nev = k + 1;
+    }
+    g->pipe_ok = 1;
+    return 0;
+}
+
+extern "C" int fdvgpu_submit(fdvgpu *g, int slot, const uint8_t *const *frames, const int *qp, const double *lscale,
+                             const int *navail, const int *force_skip, int me_mode) {
+    if (!g || slot < 0 || slot > 1 || !frames) { seterr("fdvgpu_submit: bad arguments"); return -1; }
+    for (int s = 0; s < g->nseq; ++s) if (!frames[s]) { seterr("fdvgpu_submit: no frame for sequence %d", s); return -1; }
+    if (pipe_alloc(g) != 0) return -1;
+    // the slot's previous use (its upload and its staging) is complete
+    CK(hipEventSynchronize(g->pipe_ev[slot]));
+    size_t off = 0;
+    for (int s = 0; s < g->nseq; ++s) {
+        SeqMem &m = g->S[s];
+        const size_t fsz = (size_t)m.w * m.h + 2 * (size_t)m.cw * m.ch;
+        memcpy(g->pipe_up[slot] + off, frames[s], fsz);
+        CK(hipMemcpyAsync(m.src, g->pipe_up[slot] + off, fsz, hipMemcpyHostToDevice, g->st));
+        off += fsz;
+    }
+    int *slots = (int *)calloc((size_t)g->nseq, sizeof(int));
+    if (!slots) { seterr("fdvgpu_submit: out of memory"); return -1; }
+    const int rc = fdvgpu_pframe(g, slots, qp, lscale, navail, force_skip, me_mode);
+    free(slots);
+    if (rc != 0) return -1;
+    g->can_rewind = 0;
+    if (fdvg_launch_stage(g->bases_d, g->ticket_d, g->nseq * FDVG_NS, g->compact_d, g->compact_cap,
+                          g->pipe_sb_d[slot], g->pipe_sc_d[slot], g->st) != 0) {
+        seterr("fdvgpu_submit: staging launch failed: %s", hipGetErrorString(hipGetLastError())); return -1;
+    }
+    CK(hipEventRecord(g->pipe_ev[slot], g->st));
+    return 0;
+}
+
+extern "C" int fdvgpu_collect(fdvgpu *g, int slot) {
+    if (!g || slot < 0 || slot > 1 || !g->pipe_ok) { seterr("fdvgpu_collect: nothing submitted"); return -1; }
+    CK(hipEventSynchronize(g->pipe_ev[slot]));
+    const int n = g->nseq * FDVG_NS;
+    if (g->pipe_sb[slot][n + 1]) { seterr("MB wavefront: a row waited too long for the row above"); return -1; }
+    if (g->pipe_sb[slot][n + 2]) { seterr("symbol streams overflowed (%u bytes)", g->pipe_sb[slot][n]); return -1; }
+    return 0;
+}
+
+extern "C" int fdvgpu_collected_streams(fdvgpu *g, int slot, int seq, const uint8_t **sv, size_t *sn) {
+    if (!seq_ok(g, seq) || slot < 0 || slot > 1 || !g->pipe_ok || !sv || !sn) return -1;
+    for (int k = 0; k < FDVG_NS; ++k) {
+        const uint32_t a = g->pipe_sb[slot][seq * FDVG_NS + k], b = g->pipe_sb[slot][seq * FDVG_NS + k + 1];
+        sv[k] = g->pipe_sc[slot] + a;
+        sn[k] = b - a;
+    }
     return 0;
 }
 

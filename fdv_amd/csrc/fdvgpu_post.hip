@@ -380,3 +380,32 @@ extern "C" int fdvg_launch_symbols(const void *symseqs, int nseq, int maxmb, con
     hipLaunchKernelGGL(k_sym, dim3(gx, nseq), dim3(64 * waves), 0, st, S, recs, cnt, 1, compact, (const uint32_t *)bases, cap32);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
+
+// Staging of a frame step's symbol streams for the pipelined interface: the
+// stream bases (n + 1 words), the MB wavefront's timeout flag and an overflow
+// flag into dbases, then exactly the bases[n] bytes of the compacted streams
+// into dst (both in host memory mapped for the device), so that the host can
+// read them once this kernel has completed while later frame steps already run
+// on the stream.  Bytes beyond cap are not copied (the overflow flag says so).
+extern "C" __global__ void __launch_bounds__(256)
+k_stage_streams(const uint32_t *__restrict__ bases, const unsigned *__restrict__ ctl, int n,
+                const uint8_t *__restrict__ src, size_t cap, uint32_t *__restrict__ dbases,
+                uint8_t *__restrict__ dst) {
+    const size_t total = bases[n];
+    const size_t m = total < cap ? total : cap;
+    const size_t t0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x, nt = (size_t)gridDim.x * blockDim.x;
+    if (blockIdx.x == 0) {
+        for (int k = threadIdx.x; k <= n; k += blockDim.x) dbases[k] = bases[k];
+        if (threadIdx.x == 0) { dbases[n + 1] = ctl[1]; dbases[n + 2] = total > cap ? 1u : 0u; }
+    }
+    // 16-byte chunks (both buffers are 16-byte aligned), then the tail
+    const size_t nv = m / 16;
+    for (size_t k = t0; k < nv; k += nt) ((uint4 *)dst)[k] = ((const uint4 *)src)[k];
+    for (size_t k = nv * 16 + t0; k < m; k += nt) dst[k] = src[k];
+}
+
+extern "C" int fdvg_launch_stage(const uint32_t *bases, const unsigned *ctl, int n, const uint8_t *src, size_t cap,
+                                 uint32_t *dbases, uint8_t *dst, hipStream_t st) {
+    hipLaunchKernelGGL(k_stage_streams, dim3(64), dim3(256), 0, st, bases, ctl, n, src, cap, dbases, dst);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}

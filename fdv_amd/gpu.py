@@ -255,6 +255,8 @@ class Encoder:
         L.fdvamd_pframe_encode.argtypes = [vp, vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int,
                                            vp, C.c_size_t, vp, vp, vp, C.c_int, vp, C.c_double]
         L.fdvamd_pframe_batch.argtypes = [vp] * 11
+        L.fdvamd_pframe_submit.argtypes = [vp] * 6
+        L.fdvamd_pframe_collect.argtypes = [vp] * 5
         self.sizes = [(int(w), int(h)) for w, h in sizes]
         n = len(self.sizes)
         self.h = L.fdvamd_encoder_open(n, _ints([w for w, _ in self.sizes]),
@@ -349,6 +351,39 @@ class Encoder:
         if rc != 0:
             raise GpuError("fdvamd_pframe_batch: " + self.L.fdvamd_last_error().decode())
         return [outs[s][:lens[s]].tobytes() for s in range(n)], recs
+
+    def pframe_submit(self, frames, navail, qp, lscale=None, force_skip=None):
+        """Queue one P-frame of every sequence (fixed-QP pipelining, at most two in
+        flight); returns without waiting for the device."""
+        n = len(self.sizes)
+        fr = [np.ascontiguousarray(f, dtype=np.uint8) for f in frames]
+        P = C.c_void_p * n
+        fp = P(*[f.ctypes.data for f in fr])
+        nav = _ints(navail)
+        qps = _ints(qp)
+        ls = (C.c_double * n)(*([1.0] * n if lscale is None else lscale))
+        fsk = _ints(force_skip) if force_skip is not None else None
+        rc = self.L.fdvamd_pframe_submit(self.h, C.addressof(fp), C.addressof(nav), C.addressof(qps),
+                                         C.addressof(ls), C.addressof(fsk) if fsk is not None else None)
+        if rc != 0:
+            raise GpuError("fdvamd_pframe_submit: " + self.L.fdvamd_last_error().decode())
+
+    def pframe_collect(self, tcs=None):
+        """Payloads of the oldest submitted frame, packed while the next one is
+        analysed."""
+        n = len(self.sizes)
+        caps = [w * h * 3 + 65536 for w, h in self.sizes]
+        outs = [np.empty(c, np.uint8) for c in caps]
+        P = C.c_void_p * n
+        op = P(*[o.ctypes.data for o in outs])
+        cp = (C.c_size_t * n)(*caps)
+        lens = (C.c_size_t * n)()
+        tcp = P(*[t.buf.ctypes.data for t in tcs]) if tcs is not None else None
+        rc = self.L.fdvamd_pframe_collect(self.h, C.addressof(tcp) if tcp is not None else None, C.addressof(op),
+                                          C.addressof(cp), C.addressof(lens))
+        if rc != 0:
+            raise GpuError("fdvamd_pframe_collect: " + self.L.fdvamd_last_error().decode())
+        return [outs[s][:lens[s]].tobytes() for s in range(n)]
 
 
 def nn_nparam() -> int:

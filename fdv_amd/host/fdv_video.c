@@ -28,6 +28,10 @@ struct fdvamd_encoder {
     int nseq, me_mode;
     int *w, *h;
     uint8_t *tmp;            /* contiguous I420 staging for fdvamd_pframe_encode */
+    /* pipelined frames: up to two submitted and not yet collected, the oldest
+     * in slot pipe_head; each slot's QPs, for its packing */
+    int pipe_n, pipe_head;
+    int *pipe_qp[2];
 };
 
 fdvamd_encoder *fdvamd_encoder_open(int nseq, const int *w, const int *h, int me_mode) {
@@ -38,6 +42,8 @@ fdvamd_encoder *fdvamd_encoder_open(int nseq, const int *w, const int *h, int me
     e->w = malloc(nseq * sizeof(int)); e->h = malloc(nseq * sizeof(int));
     if (!e->w || !e->h) { fdvamd_encoder_close(e); return NULL; }
     memcpy(e->w, w, nseq * sizeof(int)); memcpy(e->h, h, nseq * sizeof(int));
+    e->pipe_qp[0] = malloc(nseq * sizeof(int)); e->pipe_qp[1] = malloc(nseq * sizeof(int));
+    if (!e->pipe_qp[0] || !e->pipe_qp[1]) { fdvamd_encoder_close(e); return NULL; }
     e->g = fdvgpu_open(nseq, w, h, 1);
     if (!e->g) { amd_seterr("%s", fdvgpu_last_error()); fdvamd_encoder_close(e); return NULL; }
     return e;
@@ -46,7 +52,7 @@ fdvamd_encoder *fdvamd_encoder_open(int nseq, const int *w, const int *h, int me
 void fdvamd_encoder_close(fdvamd_encoder *e) {
     if (!e) return;
     fdvgpu_close(e->g);
-    free(e->w); free(e->h); free(e->tmp);
+    free(e->w); free(e->h); free(e->tmp); free(e->pipe_qp[0]); free(e->pipe_qp[1]);
     free(e);
 }
 
@@ -76,6 +82,7 @@ typedef struct {
     uint8_t *const *out;
     const size_t *cap;
     size_t *len;
+    int slot;                /* -1: the last fdvgpu_pframe; 0/1: a collected slot */
     int first, step, rc;
 } packjob;
 
@@ -84,11 +91,66 @@ static void *pack_worker(void *arg) {
     for (int s = j->first; s < j->e->nseq; s += j->step) {
         const uint8_t *sv[16];
         size_t sn[16];
-        if (fdvgpu_streams(j->e->g, s, sv, sn) != 0) { j->len[s] = 0; j->rc = -1; continue; }
+        const int got = j->slot < 0 ? fdvgpu_streams(j->e->g, s, sv, sn)
+                                    : fdvgpu_collected_streams(j->e->g, j->slot, s, sv, sn);
+        if (got != 0) { j->len[s] = 0; j->rc = -1; continue; }
         j->len[s] = fdvamd_pack_pframe(j->qp[s], sv, sn, j->tc ? j->tc[s] : NULL, j->out[s], j->cap[s]);
         if (!j->len[s]) j->rc = -1;
     }
     return NULL;
+}
+
+/* Host entropy of every sequence: the sequences are independent, so they pack
+ * in parallel, one thread per sequence up to 16. */
+static int pack_all(fdvamd_encoder *e, int slot, const int *qp, fdvamd_tabcache *const *tc, uint8_t *const *out,
+                    const size_t *cap, size_t *len) {
+    int nt = e->nseq < 16 ? e->nseq : 16, rc = 0;
+    pthread_t tid[16];
+    packjob *jobs = calloc((size_t)nt, sizeof *jobs);
+    if (!jobs) { amd_seterr("out of memory"); return -1; }
+    int spawned[16] = {0};
+    for (int t = 0; t < nt; ++t) {
+        jobs[t] = (packjob){.e = e, .qp = qp, .tc = tc, .out = out, .cap = cap, .len = len,
+                            .slot = slot, .first = t, .step = nt, .rc = 0};
+        if (t > 0) spawned[t] = pthread_create(&tid[t], NULL, pack_worker, &jobs[t]) == 0;
+    }
+    pack_worker(&jobs[0]);
+    for (int t = 1; t < nt; ++t) {
+        if (spawned[t]) pthread_join(tid[t], NULL);
+        else pack_worker(&jobs[t]);
+    }
+    for (int t = 0; t < nt; ++t) if (jobs[t].rc) rc = -1;
+    free(jobs);
+    return rc;
+}
+
+int fdvamd_pframe_submit(fdvamd_encoder *e, const uint8_t *const *frames, const int *navail, const int *qp,
+                         const double *lscale, const int *force_skip) {
+    if (!e || !frames || !navail || !qp) { amd_seterr("bad arguments"); return -1; }
+    if (e->pipe_n >= 2) { amd_seterr("two frames in flight: collect one first"); return -1; }
+    double *ls = malloc((size_t)e->nseq * sizeof(double));
+    if (!ls) { amd_seterr("out of memory"); return -1; }
+    for (int s = 0; s < e->nseq; ++s) ls[s] = lscale ? lscale[s] : 1.0;
+    const int slot = (e->pipe_head + e->pipe_n) % 2;
+    const int rc = fdvgpu_submit(e->g, slot, frames, qp, ls, navail, force_skip, e->me_mode);
+    free(ls);
+    if (rc != 0) { amd_seterr("%s", fdvgpu_last_error()); return -1; }
+    memcpy(e->pipe_qp[slot], qp, (size_t)e->nseq * sizeof(int));
+    e->pipe_n++;
+    return 0;
+}
+
+int fdvamd_pframe_collect(fdvamd_encoder *e, fdvamd_tabcache *const *tc, uint8_t *const *out, const size_t *cap,
+                          size_t *len) {
+    if (!e || !out || !cap || !len) { amd_seterr("bad arguments"); return -1; }
+    if (!e->pipe_n) { amd_seterr("no frame submitted"); return -1; }
+    const int slot = e->pipe_head;
+    e->pipe_head ^= 1;
+    e->pipe_n--;
+    if (fdvgpu_collect(e->g, slot) != 0) { amd_seterr("%s", fdvgpu_last_error()); return -1; }
+    const int rc = pack_all(e, slot, e->pipe_qp[slot], tc, out, cap, len);
+    if (rc) amd_seterr("a sequence failed to pack");
+    return rc;
 }
 
 int fdvamd_pframe_batch(fdvamd_encoder *e, const uint8_t *const *frames, const int *navail,
@@ -96,6 +158,7 @@ int fdvamd_pframe_batch(fdvamd_encoder *e, const uint8_t *const *frames, const i
                         fdvamd_tabcache *const *tc, uint8_t *const *out, const size_t *cap,
                         size_t *len, uint8_t *const *recon) {
     if (!e || !frames || !navail || !qp || !out || !cap || !len) { amd_seterr("bad arguments"); return -1; }
+    if (e->pipe_n) { amd_seterr("submitted frames not collected"); return -1; }
     int *slots = calloc((size_t)e->nseq, sizeof(int));
     double *ls = malloc((size_t)e->nseq * sizeof(double));
     if (!slots || !ls) { free(slots); free(ls); amd_seterr("out of memory"); return -1; }
@@ -108,24 +171,7 @@ int fdvamd_pframe_batch(fdvamd_encoder *e, const uint8_t *const *frames, const i
     free(slots); free(ls);
     if (rc != 0) { amd_seterr("%s", fdvgpu_last_error()); return -1; }
 
-    /* host entropy: the sequences are independent, so they pack in parallel */
-    int nt = e->nseq < 16 ? e->nseq : 16;
-    pthread_t tid[16];
-    packjob *jobs = calloc((size_t)nt, sizeof *jobs);
-    if (!jobs) { amd_seterr("out of memory"); return -1; }
-    int spawned[16] = {0};
-    for (int t = 0; t < nt; ++t) {
-        jobs[t] = (packjob){.e = e, .qp = qp, .tc = tc, .out = out, .cap = cap, .len = len,
-                            .first = t, .step = nt, .rc = 0};
-        if (t > 0) spawned[t] = pthread_create(&tid[t], NULL, pack_worker, &jobs[t]) == 0;
-    }
-    pack_worker(&jobs[0]);
-    for (int t = 1; t < nt; ++t) {
-        if (spawned[t]) pthread_join(tid[t], NULL);
-        else pack_worker(&jobs[t]);
-    }
-    for (int t = 0; t < nt; ++t) if (jobs[t].rc) rc = -1;
-    free(jobs);
+    rc = pack_all(e, -1, qp, tc, out, cap, len);
     if (recon)
         for (int s = 0; s < e->nseq; ++s)
             if (recon[s] && fdvgpu_recon(e->g, s, recon[s]) != 0) { amd_seterr("%s", fdvgpu_last_error()); rc = -1; }
@@ -165,6 +211,7 @@ size_t fdvamd_pframe_encode(fdvamd_encoder *e, const uint8_t *cy, const uint8_t 
 }
 
 int fdvamd_rewind(fdvamd_encoder *e) {
+    if (e && e->pipe_n) { amd_seterr("submitted frames not collected"); return -1; }
     if (!e || fdvgpu_rewind(e->g) != 0) { amd_seterr("%s", e ? fdvgpu_last_error() : "bad arguments"); return -1; }
     return 0;
 }

@@ -83,6 +83,21 @@ int fdvamd_pframe_batch(fdvamd_encoder *e, const uint8_t *const *frames, const i
                         fdvamd_tabcache *const *tc, uint8_t *const *out, const size_t *cap,
                         size_t *len, uint8_t *const *recon);
 
+/* Pipelined P-frames for fixed-QP sequences (fdv_video_encode, fdv.h:6741-6854,
+ * whose next frame does not wait on the previous frame's size).  No counterpart
+ * in the reference.  fdvamd_pframe_submit queues one frame of every sequence
+ * (frames copied before it returns) and returns without waiting for the device;
+ * at most two frames are in flight.  fdvamd_pframe_collect returns the oldest
+ * submitted frame's payloads (as fdvamd_pframe_batch, without the
+ * reconstruction, which stays on the device as the next reference) while the
+ * next frame is analysed.  Order: submit 1, submit 2, collect 1, submit 3,
+ * collect 2, ...  fdvamd_pframe_batch, fdvamd_pframe_encode and fdvamd_rewind
+ * fail while frames are in flight.  Both return 0, or -1. */
+int fdvamd_pframe_submit(fdvamd_encoder *e, const uint8_t *const *frames, const int *navail, const int *qp,
+                         const double *lscale, const int *force_skip);
+int fdvamd_pframe_collect(fdvamd_encoder *e, fdvamd_tabcache *const *tc, uint8_t *const *out, const size_t *cap,
+                          size_t *len);
+
 /* Install the learned proposer's weights (fdvgpu_nn_nparam() floats) for
  * me_mode 2.  No counterpart in the reference. */
 int fdvamd_set_proposer(fdvamd_encoder *e, const float *weights, int n);

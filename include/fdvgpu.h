@@ -88,6 +88,23 @@ int fdvgpu_pframe(fdvgpu *g, const int *src_slot, const int *qp, const double *l
 int fdvgpu_fetch(fdvgpu *g);
 int fdvgpu_streams(fdvgpu *g, int seq, const uint8_t **sv, size_t *sn);
 
+/* Pipelined frame steps, for fixed-QP sequences whose next frame does not wait
+ * on the previous frame's size (fdv_video_encode, fdv.h:6741-6854).  No
+ * counterpart in the reference.  fdvgpu_submit uploads one frame of every
+ * sequence (frames[s], tight I420; copied before it returns), queues its
+ * analysis behind any earlier step, and stages its symbol streams in host
+ * buffer `slot` (0 or 1); it returns without waiting for the device.
+ * fdvgpu_collect waits for that slot's step only (later steps may still run)
+ * and checks its flags; fdvgpu_collected_streams then returns sequence seq's
+ * 16 streams, valid until the slot is submitted again.  A slot must be
+ * collected before it is submitted again; with two slots, frame k + 1 is
+ * analysed while the host packs frame k.  The reconstruction stays on the
+ * device as the next reference; rewinding is not available after a submit. */
+int fdvgpu_submit(fdvgpu *g, int slot, const uint8_t *const *frames, const int *qp, const double *lscale,
+                  const int *navail, const int *force_skip, int me_mode);
+int fdvgpu_collect(fdvgpu *g, int slot);
+int fdvgpu_collected_streams(fdvgpu *g, int slot, int seq, const uint8_t **sv, size_t *sn);
+
 /* Undo the reference rotation of the last fdvgpu_pframe, so that the same
  * frame can be coded again from the same references -- the requantising retry
  * of the streaming encoder (fdv.h:8303-8334), which calls pframe_encode twice

@@ -125,3 +125,57 @@ def test_requantising_retry_after_rewind():
         enc.rewind()
         enc.rewind()
     enc.close()
+
+
+@pytest.mark.parametrize("mode", [G.ME_DIAMOND, G.ME_PARALLEL, G.ME_LEARNED])
+def test_pipelined_submit_collect_matches_sequential(mode):
+    """Frames submitted two at a time and collected while the next one is
+    analysed give the payloads of the synchronous batch call, frame by frame;
+    misuse (a third frame in flight, a collect with none, a synchronous call
+    while frames are in flight) fails without disturbing the pipeline."""
+    scenes = [content.scene_objects(64, 48, 31, nobj=2, smin=8, smax=24),
+              content.scene_pan(128, 64, 32, nobj=3, smin=8, smax=40)]
+    qps = [24, 30]
+    nfr = 6
+    frames = [[s.frame(i) for i in range(nfr + 3)] for s in scenes]
+    sizes = [(s.w, s.h) for s in scenes]
+
+    def fresh():
+        e = G.Encoder(sizes, me_mode=mode)
+        if mode == G.ME_LEARNED:
+            e.set_proposer()
+        for k in range(2):
+            e.set_ref(k, 0, frames[k][0])
+        return e, [G.TabCache() for _ in scenes]
+
+    def fr(f):
+        return [frames[k][f] for k in range(2)]
+
+    def nav(f):
+        return [1 if f == 1 else 2] * 2
+
+    say(f"  pipelined encoder, mode {mode}: synchronous chain")
+    a, ta = fresh()
+    want = [a.pframe_batch(fr(f), nav(f), qps, ta, recon=False)[0] for f in range(1, nfr)]
+    a.close()
+    say(f"  pipelined encoder, mode {mode}: submit / collect chain")
+    b, tb = fresh()
+    got = []
+    b.pframe_submit(fr(1), nav(1), qps)
+    for f in range(2, nfr):
+        b.pframe_submit(fr(f), nav(f), qps)
+        got.append(b.pframe_collect(tb))
+    got.append(b.pframe_collect(tb))
+    for f in range(len(want)):
+        assert got[f] == want[f], f"frame {f + 1}: pipelined payloads differ"
+    with pytest.raises(G.GpuError):
+        b.pframe_collect(tb)
+    b.pframe_submit(fr(nfr), nav(nfr), qps)
+    b.pframe_submit(fr(nfr + 1), nav(nfr + 1), qps)
+    with pytest.raises(G.GpuError):
+        b.pframe_submit(fr(nfr + 2), nav(nfr + 2), qps)
+    with pytest.raises(G.GpuError):
+        b.pframe_batch(fr(nfr + 2), nav(nfr + 2), qps, tb, recon=False)
+    assert all(len(p) > 0 for p in b.pframe_collect(tb))
+    assert all(len(p) > 0 for p in b.pframe_collect(tb))
+    b.close()
