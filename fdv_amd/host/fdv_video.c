@@ -23,6 +23,29 @@ static void amd_seterr(const char *fmt, ...) {
 }
 const char *fdvamd_last_error(void) { return amd_err; }
 
+typedef struct {
+    fdvamd_encoder *e;
+    const int *qp;
+    fdvamd_tabcache *const *tc;
+    uint8_t *const *out;
+    const size_t *cap;
+    size_t *len;
+    int slot;                /* -1: the last fdvgpu_pframe; 0/1: a collected slot */
+    int first, step, rc;
+} packjob;
+
+/* Packing threads, created with the encoder: each frame step hands them one
+ * job each (generation counter) and waits until all are done. */
+typedef struct {
+    pthread_mutex_t mu;
+    pthread_cond_t go, done;
+    pthread_t tid[15];
+    int nw;                  /* worker threads running */
+    unsigned gen;
+    int pending, quit;
+    packjob jobs[16];
+} packpool;
+
 struct fdvamd_encoder {
     fdvgpu *g;
     int nseq, me_mode;
@@ -32,7 +55,60 @@ struct fdvamd_encoder {
      * in slot pipe_head; each slot's QPs, for its packing */
     int pipe_n, pipe_head;
     int *pipe_qp[2];
+    packpool *pool;
 };
+
+static void *pack_worker(void *arg);
+
+typedef struct { packpool *p; int k; } poolarg;
+
+static void *pool_main(void *arg) {
+    poolarg a = *(poolarg *)arg;
+    free(arg);
+    packpool *p = a.p;
+    unsigned seen = 0;
+    for (;;) {
+        pthread_mutex_lock(&p->mu);
+        while (!p->quit && p->gen == seen) pthread_cond_wait(&p->go, &p->mu);
+        if (p->quit) { pthread_mutex_unlock(&p->mu); return NULL; }
+        seen = p->gen;
+        pthread_mutex_unlock(&p->mu);
+        pack_worker(&p->jobs[a.k]);
+        pthread_mutex_lock(&p->mu);
+        if (--p->pending == 0) pthread_cond_signal(&p->done);
+        pthread_mutex_unlock(&p->mu);
+    }
+}
+
+static packpool *pool_open(int nseq) {
+    packpool *p = calloc(1, sizeof *p);
+    if (!p) return NULL;
+    pthread_mutex_init(&p->mu, NULL);
+    pthread_cond_init(&p->go, NULL);
+    pthread_cond_init(&p->done, NULL);
+    const int want = (nseq < 16 ? nseq : 16) - 1;
+    for (int k = 1; k <= want; ++k) {
+        poolarg *a = malloc(sizeof *a);
+        if (!a) break;
+        *a = (poolarg){p, k};
+        if (pthread_create(&p->tid[k - 1], NULL, pool_main, a) != 0) { free(a); break; }
+        p->nw = k;
+    }
+    return p;
+}
+
+static void pool_close(packpool *p) {
+    if (!p) return;
+    pthread_mutex_lock(&p->mu);
+    p->quit = 1;
+    pthread_cond_broadcast(&p->go);
+    pthread_mutex_unlock(&p->mu);
+    for (int k = 0; k < p->nw; ++k) pthread_join(p->tid[k], NULL);
+    pthread_mutex_destroy(&p->mu);
+    pthread_cond_destroy(&p->go);
+    pthread_cond_destroy(&p->done);
+    free(p);
+}
 
 fdvamd_encoder *fdvamd_encoder_open(int nseq, const int *w, const int *h, int me_mode) {
     if (nseq <= 0 || !w || !h || me_mode < 0 || me_mode > 3) { amd_seterr("bad arguments"); return NULL; }
@@ -43,7 +119,8 @@ fdvamd_encoder *fdvamd_encoder_open(int nseq, const int *w, const int *h, int me
     if (!e->w || !e->h) { fdvamd_encoder_close(e); return NULL; }
     memcpy(e->w, w, nseq * sizeof(int)); memcpy(e->h, h, nseq * sizeof(int));
     e->pipe_qp[0] = malloc(nseq * sizeof(int)); e->pipe_qp[1] = malloc(nseq * sizeof(int));
-    if (!e->pipe_qp[0] || !e->pipe_qp[1]) { fdvamd_encoder_close(e); return NULL; }
+    e->pool = pool_open(nseq);
+    if (!e->pipe_qp[0] || !e->pipe_qp[1] || !e->pool) { fdvamd_encoder_close(e); return NULL; }
     e->g = fdvgpu_open(nseq, w, h, 1);
     if (!e->g) { amd_seterr("%s", fdvgpu_last_error()); fdvamd_encoder_close(e); return NULL; }
     return e;
@@ -51,6 +128,7 @@ fdvamd_encoder *fdvamd_encoder_open(int nseq, const int *w, const int *h, int me
 
 void fdvamd_encoder_close(fdvamd_encoder *e) {
     if (!e) return;
+    pool_close(e->pool);
     fdvgpu_close(e->g);
     free(e->w); free(e->h); free(e->tmp); free(e->pipe_qp[0]); free(e->pipe_qp[1]);
     free(e);
@@ -75,17 +153,6 @@ int fdvamd_set_ref_planes(fdvamd_encoder *e, int seq, int which,
     return rc;
 }
 
-typedef struct {
-    fdvamd_encoder *e;
-    const int *qp;
-    fdvamd_tabcache *const *tc;
-    uint8_t *const *out;
-    const size_t *cap;
-    size_t *len;
-    int slot;                /* -1: the last fdvgpu_pframe; 0/1: a collected slot */
-    int first, step, rc;
-} packjob;
-
 static void *pack_worker(void *arg) {
     packjob *j = arg;
     for (int s = j->first; s < j->e->nseq; s += j->step) {
@@ -104,23 +171,22 @@ static void *pack_worker(void *arg) {
  * in parallel, one thread per sequence up to 16. */
 static int pack_all(fdvamd_encoder *e, int slot, const int *qp, fdvamd_tabcache *const *tc, uint8_t *const *out,
                     const size_t *cap, size_t *len) {
-    int nt = e->nseq < 16 ? e->nseq : 16, rc = 0;
-    pthread_t tid[16];
-    packjob *jobs = calloc((size_t)nt, sizeof *jobs);
-    if (!jobs) { amd_seterr("out of memory"); return -1; }
-    int spawned[16] = {0};
-    for (int t = 0; t < nt; ++t) {
-        jobs[t] = (packjob){.e = e, .qp = qp, .tc = tc, .out = out, .cap = cap, .len = len,
-                            .slot = slot, .first = t, .step = nt, .rc = 0};
-        if (t > 0) spawned[t] = pthread_create(&tid[t], NULL, pack_worker, &jobs[t]) == 0;
-    }
-    pack_worker(&jobs[0]);
-    for (int t = 1; t < nt; ++t) {
-        if (spawned[t]) pthread_join(tid[t], NULL);
-        else pack_worker(&jobs[t]);
-    }
-    for (int t = 0; t < nt; ++t) if (jobs[t].rc) rc = -1;
-    free(jobs);
+    packpool *p = e->pool;
+    const int nt = p->nw + 1;
+    int rc = 0;
+    for (int t = 0; t < nt; ++t)
+        p->jobs[t] = (packjob){.e = e, .qp = qp, .tc = tc, .out = out, .cap = cap, .len = len,
+                               .slot = slot, .first = t, .step = nt, .rc = 0};
+    pthread_mutex_lock(&p->mu);
+    p->pending = p->nw;
+    p->gen++;
+    pthread_cond_broadcast(&p->go);
+    pthread_mutex_unlock(&p->mu);
+    pack_worker(&p->jobs[0]);
+    pthread_mutex_lock(&p->mu);
+    while (p->pending) pthread_cond_wait(&p->done, &p->mu);
+    pthread_mutex_unlock(&p->mu);
+    for (int t = 0; t < nt; ++t) if (p->jobs[t].rc) rc = -1;
     return rc;
 }
 
