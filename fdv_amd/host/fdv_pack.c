@@ -52,17 +52,28 @@ static void freqs_from_counts(const uint32_t cnt[256], uint16_t f[256]) {
 }
 
 /* Encode back to front into the tail of `buf`, then move the result to the
- * front; the final state leads, little-endian (fdv.h:1936-1948). */
+ * front; the final state leads, little-endian (fdv.h:1936-1948).  The state's
+ * division by the frequency is a multiplication by M = ceil(2^64 / f) (the
+ * high 64 bits of the 128-bit product): with x < 2^31 and f <= 2^12 the error
+ * of x M / 2^64 against x / f is below 2^-33, less than the 1/f that separates
+ * x / f from the next integer, so the quotient is exact. */
 static size_t rans_pack(const uint8_t *sym, size_t n, const uint16_t f[256], uint8_t *buf, size_t cap) {
     uint32_t start[256], acc = 0;
-    for (int s = 0; s < 256; ++s) { start[s] = acc; acc += f[s]; }
+    uint64_t rcp[256];
+    for (int s = 0; s < 256; ++s) {
+        start[s] = acc;
+        acc += f[s];
+        rcp[s] = f[s] > 1 ? ~0ull / f[s] + 1 : 0;
+    }
     uint8_t *p = buf + cap;
     uint32_t x = STATE_LO;
     for (size_t i = n; i > 0; --i) {
-        const uint32_t fs = f[sym[i - 1]];
+        const unsigned sm = sym[i - 1];
+        const uint32_t fs = f[sm];
         const uint64_t bound = ((uint64_t)(STATE_LO >> SCALE_BITS) << 8) * fs;
         while ((uint64_t)x >= bound) { *--p = (uint8_t)(x & 0xff); x >>= 8; }
-        x = ((x / fs) << SCALE_BITS) + (x % fs) + start[sym[i - 1]];
+        const uint32_t q = fs == 1 ? x : (uint32_t)(((unsigned __int128)x * rcp[sm]) >> 64);
+        x = (q << SCALE_BITS) + (x - q * fs) + start[sm];
     }
     p -= 4;
     for (int k = 0; k < 4; ++k) p[k] = (uint8_t)(x >> (8 * k));
