@@ -41,10 +41,12 @@ typedef struct {
 // The learned proposer's output for one MB: 8 whole-pel vectors on the nearest
 // reference (pels, best first), the 4 lowest-cost positions of its half-res
 // correlation map (pels; ties to the smaller |dx| + |dy|, then the lower
-// position), the 8 vectors' total probability, the SKIP probability.
+// position), the 2 lowest of each 8x8 quadrant's own map (same ties), the 8
+// vectors' total probability, the SKIP probability.
 typedef struct {
     int8_t cx[8], cy[8];
     int8_t kx[4], ky[4];
+    int8_t qx[4][2], qy[4][2];   // per 8x8 quadrant: the 2 lowest-cost positions of its map
     float conf, pskip;
 } fdvg_nnout;
 

@@ -89,10 +89,11 @@ __device__ __forceinline__ void refine_all(MSmem &s, int nr, int *bmx, int *bmy,
 }
 
 // With a proposer output `nn` whose confidence is at least `thr`, the integer
-// search covers only the 3x3 whole-pel neighbourhoods of its 12 vectors (the
-// network's 8, then the correlation map's 4; scaled by 2 on the far reference)
-// and the zero vector, same tie order; below `thr` it is the full search.  A
-// SKIP probability of at least `sthr` sets flags bit 0.
+// search covers only the 3x3 whole-pel neighbourhoods of its vectors -- on the
+// near reference the network's 8, the correlation map's 4 and the quadrant
+// maps' 8, on the far one the first 12 doubled -- and the zero vector, same tie
+// order; below `thr` it is the full search.  A SKIP probability of at least
+// `sthr` sets flags bit 0.
 extern "C" __global__ void __launch_bounds__(256)
 k_me_exhaustive(const fdvg_seq *__restrict__ seqs, fdvg_metab *__restrict__ out,
                 const fdvg_nnout *__restrict__ nn, float thr, float sthr) {
@@ -121,21 +122,25 @@ k_me_exhaustive(const fdvg_seq *__restrict__ seqs, fdvg_metab *__restrict__ out,
     for (int b = 0; b < 6; ++b) best[b] = ~0ull;
     const fdvg_nnout *pr = nn ? &nn[q.mb0 + i] : nullptr;
     const bool cand = pr && pr->conf >= thr;
-    // candidate mode: thread t < 108 takes point t%9 around vector t/9 on the near
-    // reference (and the quadrants), t = 108 the zero vector; 128 <= t <= 236 the
-    // same on the far one
-    const int npts = cand ? 1 : (33 * 33 + 255) / 256;
+    // candidate mode, 290 points in two passes: point t < 108 is point t % 9
+    // around vector t / 9 (the network's 8, the map's 4) on the near reference
+    // (and the quadrants), 108 <= t < 180 the same around the quadrants' 8, t =
+    // 180 the zero vector; 181 <= t < 289 the first 12 again, doubled, on the far
+    // reference, t = 289 its zero vector
+    const int npts = cand ? 2 : (33 * 33 + 255) / 256;
     for (int it = 0; it < npts; ++it) {
         int dx, dy, rlo = 0, rhi = nr;
         if (cand) {
-            const int tt = tid & 127, c = tt / 9, o = tt % 9, far = tid >> 7;
-            if (tt > 108 || (far && nr < 2)) continue;
-            const int sc = far ? 2 : 1;
-            const int vx = c < 8 ? pr->cx[c < 8 ? c : 0] : pr->kx[c < 12 ? c - 8 : 0];
-            const int vy = c < 8 ? pr->cy[c < 8 ? c : 0] : pr->ky[c < 12 ? c - 8 : 0];
-            // point 108: the zero vector, so that every search has a point in range
-            dx = tt == 108 ? 0 : sc * vx + o % 3 - 1;
-            dy = tt == 108 ? 0 : sc * vy + o / 3 - 1;
+            const int t = tid + 256 * it, far = t >= 181, u = far ? t - 181 : t;
+            if (t >= 290 || (far && nr < 2)) continue;
+            const int c = u / 9, o = u % 9, sc = far ? 2 : 1;
+            const int cq = c >= 12 && c < 20 ? c - 12 : 0;
+            const int vx = c < 8 ? pr->cx[c < 8 ? c : 0] : c < 12 ? pr->kx[c < 12 ? c - 8 : 0] : pr->qx[cq >> 1][cq & 1];
+            const int vy = c < 8 ? pr->cy[c < 8 ? c : 0] : c < 12 ? pr->ky[c < 12 ? c - 8 : 0] : pr->qy[cq >> 1][cq & 1];
+            // the zero vector, so that every search has a point in range
+            const bool zero = far ? u == 108 : u == 180;
+            dx = zero ? 0 : sc * vx + o % 3 - 1;
+            dy = zero ? 0 : sc * vy + o / 3 - 1;
             if (dx < -16 || dx > 16 || dy < -16 || dy > 16) continue;
             rlo = far; rhi = far + 1;
         } else {

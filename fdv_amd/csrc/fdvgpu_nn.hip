@@ -482,7 +482,8 @@ __device__ __forceinline__ u4v ni_pack8(const float *f) {
 // Integer SADs and the fp32 feature formulas are those of nn_features.  Also
 // the map's 4 lowest-cost positions into o->kx, o->ky (whole pels; ties to the
 // smaller |dx| + |dy|, then the lower position): classical coarse search
-// candidates beside the network's.
+// candidates beside the network's; and each 8x8 quadrant's 2 lowest-cost
+// positions of its own map into o->qx, o->qy (same ties).
 __device__ __forceinline__ void ni_features(NiWave &w, const uint8_t *cur, int cw, const uint8_t *ref, int rs,
                                             int bx, int by, int qp, fdvg_nnout *o) {
     const int l = threadIdx.x & 63;
@@ -524,6 +525,7 @@ __device__ __forceinline__ void ni_features(NiWave &w, const uint8_t *cur, int c
     for (int k = 0; k < 16; ++k) cr[k] = *(const uint32_t *)(c8 + 4 * k);
     int mn = 1 << 30;
     int cv[5];
+    uint32_t qkey[4][5];                 // per quadrant (raster) and position: (cost, |dx| + |dy|, position)
 #pragma unroll
     for (int m = 0; m < 5; ++m) {
         const int p = l + 64 * m;
@@ -531,18 +533,40 @@ __device__ __forceinline__ void ni_features(NiWave &w, const uint8_t *cur, int c
         if (p < NN_P) {
             const int dy = p / NN_G - 8, dx = p % NN_G - 8, c0 = 12 + dx, sh = c0 & 3;
             const uint8_t *row = r32 + (12 + dy) * 32 + (c0 & ~3);
-            uint32_t acc = 0;
+            uint32_t aq[4] = {0u, 0u, 0u, 0u};
 #pragma unroll
             for (int y = 0; y < 8; ++y) {
                 const uint32_t d0 = *(const uint32_t *)(row + 32 * y), d1 = *(const uint32_t *)(row + 32 * y + 4),
                                d2 = *(const uint32_t *)(row + 32 * y + 8);
-                acc = __builtin_amdgcn_sad_u8(__builtin_amdgcn_alignbyte(d1, d0, sh), cr[2 * y], acc);
-                acc = __builtin_amdgcn_sad_u8(__builtin_amdgcn_alignbyte(d2, d1, sh), cr[2 * y + 1], acc);
+                const int qr = (y >> 2) * 2;
+                aq[qr] = __builtin_amdgcn_sad_u8(__builtin_amdgcn_alignbyte(d1, d0, sh), cr[2 * y], aq[qr]);
+                aq[qr + 1] = __builtin_amdgcn_sad_u8(__builtin_amdgcn_alignbyte(d2, d1, sh), cr[2 * y + 1], aq[qr + 1]);
             }
-            sad = (int)acc;
+            sad = (int)(aq[0] + aq[1] + aq[2] + aq[3]);
             mn = min(mn, sad);
+            const uint32_t tie = ((uint32_t)(abs(dx) + abs(dy)) << 9) | (uint32_t)p;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) qkey[q][m] = (aq[q] << 14) | tie;
+        } else {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) qkey[q][m] = 0xFFFFFFFFu;
         }
         cv[m] = sad;
+    }
+    // each quadrant's 2 lowest-cost positions (a quadrant's cost < 2^12)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            uint32_t b = qkey[q][0];
+#pragma unroll
+            for (int m = 1; m < 5; ++m) b = min(b, qkey[q][m]);
+            for (int s2 = 32; s2 > 0; s2 >>= 1) b = min(b, (uint32_t)__shfl_xor((int)b, s2));
+#pragma unroll
+            for (int m = 0; m < 5; ++m) if (qkey[q][m] == b) qkey[q][m] = 0xFFFFFFFFu;
+            const int p = (int)(b & 511u);
+            if (l == 0) { o->qx[q][k] = (int8_t)(2 * (p % NN_G - 8)); o->qy[q][k] = (int8_t)(2 * (p / NN_G - 8)); }
+        }
     }
     {
         // keys (cost, |dx| + |dy|, position): cost < 2^14, distance < 2^5, position < 2^9

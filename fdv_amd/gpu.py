@@ -17,7 +17,7 @@ NS = 16
 MBINFO_DTYPE = np.dtype([("mode", "u1"), ("ref", "u1"), ("merge_idx", "u1"), ("sub", "u1"),
                          ("mv", "<i2", (4, 2))])
 PROPOSAL_DTYPE = np.dtype([("cx", "i1", 8), ("cy", "i1", 8), ("kx", "i1", 4), ("ky", "i1", 4),
-                           ("conf", "<f4"), ("pskip", "<f4")])
+                           ("qx", "i1", (4, 2)), ("qy", "i1", (4, 2)), ("conf", "<f4"), ("pskip", "<f4")])
 MOTION_DTYPE = np.dtype([("mv16", "<i2", (2, 2)), ("sad16", "<i4", 2), ("mv8", "<i2", (4, 2)),
                          ("sad8", "<i4", 4), ("flags", "<i4")])
 PROPOSER_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "learned", "proposer.f32")

@@ -144,11 +144,13 @@ int fdvgpu_autotune(fdvgpu *g, const int *src_slot, const int *qp, const double 
 
 /* One MB's proposal: 8 whole-pel vectors on the nearest reference (best first),
  * the 4 lowest-cost positions of the proposer's half-res correlation map
- * (whole pels; ties to the smaller |dx| + |dy|, then the lower position), the 8
- * vectors' total probability, and the SKIP probability. */
+ * (whole pels; ties to the smaller |dx| + |dy|, then the lower position), the
+ * 2 lowest of each 8x8 quadrant's own map (same ties), the 8 vectors' total
+ * probability, and the SKIP probability. */
 typedef struct {
     int8_t cx[8], cy[8];
     int8_t kx[4], ky[4];
+    int8_t qx[4][2], qy[4][2];
     float conf, pskip;
 } fdvgpu_proposal;
 

@@ -1388,10 +1388,11 @@ void orc_metab_candidates(const uint8_t *cy, int w, int h, const uint8_t *const 
         if (!(P->conf >= thr)) continue;
         const int bx = (i % mw) * 16, by = (i / mw) * 16;
         for (int r = 0; r < navail; ++r) {
-            int pts[109][2], n = 0;
-            for (int c = 0; c < 12; ++c)
+            int pts[181][2], n = 0;
+            for (int c = 0; c < (r ? 12 : 20); ++c)     /* the quadrant maps' 8 on the near reference only */
                 for (int o = 0; o < 9; ++o) {
-                    const int vx = c < 8 ? P->cx[c] : P->kx[c - 8], vy = c < 8 ? P->cy[c] : P->ky[c - 8];
+                    const int vx = c < 8 ? P->cx[c] : c < 12 ? P->kx[c - 8] : P->qx[(c - 12) >> 1][(c - 12) & 1];
+                    const int vy = c < 8 ? P->cy[c] : c < 12 ? P->ky[c - 8] : P->qy[(c - 12) >> 1][(c - 12) & 1];
                     const int dx = (r + 1) * vx + o % 3 - 1, dy = (r + 1) * vy + o / 3 - 1;
                     if (dx < -16 || dx > 16 || dy < -16 || dy > 16) continue;
                     pts[n][0] = dx; pts[n][1] = dy; ++n;

@@ -140,6 +140,7 @@ typedef struct {
 typedef struct {
     int8_t cx[8], cy[8];     /* whole-pel vectors on the nearest reference */
     int8_t kx[4], ky[4];     /* the correlation map's 4 lowest-cost positions */
+    int8_t qx[4][2], qy[4][2];   /* per quadrant: its map's 2 lowest-cost positions */
     float conf, pskip;
 } orc_nnout;
 
@@ -147,9 +148,9 @@ typedef struct {
 void orc_metab_exhaustive(const uint8_t *cy, int w, int h, const uint8_t *const RY[2], int ys,
                           int navail, orc_metab *mt);
 /* The table of the learned mode: where the proposer's confidence is at least
- * thr, the whole-pel search covers the 3x3 neighbourhoods of its 12 vectors
- * (the network's 8, then the correlation map's 4; scaled by 2 on the far
- * reference) and the zero vector, with the exhaustive
+ * thr, the whole-pel search covers the 3x3 neighbourhoods of its vectors (near
+ * reference: the network's 8, the correlation map's 4, the quadrant maps' 8;
+ * far reference: the first 12 scaled by 2) and the zero vector, with the exhaustive
  * search's tie order, then the sub-pel cross; elsewhere the exhaustive search.
  * flags bit 0 = SKIP probability >= sthr: the decision then weighs SKIP against
  * INTER16 only. */

@@ -335,7 +335,7 @@ def oracle_metab_candidates(cur, r0, r1, w, h, navail, nnout, thr=0.7, sthr=2.0)
     mt = (Metab * nmb)()
     cy = split_i420(cur, w, h)[0]
     nn = np.ascontiguousarray(nnout)
-    assert nn.dtype.itemsize == 32 and nn.size == nmb
+    assert nn.dtype.itemsize == 48 and nn.size == nmb
     lib.orc_metab_candidates.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_void_p,
                                          C.c_float, C.c_float, C.c_void_p]
     lib.orc_metab_candidates(C.c_void_p(cy.ctypes.data), w, h, C.addressof(RY), ys, navail,

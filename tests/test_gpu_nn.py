@@ -99,6 +99,31 @@ def _gpu_features(an, seq, w, h, qp):
     return feats.cpu().numpy().reshape(nmb, N.P, 2)
 
 
+def _quadrant_lowest_restated(cur, ref, w, h):
+    """Per MB and 8x8 quadrant, the 2 lowest-cost positions of the quadrant's own
+    half-res correlation map (ties: |dx| + |dy|, then position), in pels."""
+    Y = cur[:w * h].reshape(h, w).astype(np.int64)
+    R = np.pad(ref[:w * h].reshape(h, w), 32, mode="edge").astype(np.int64)
+    out = []
+    for i in range((w // 16) * (h // 16)):
+        bx, by = (i % (w // 16)) * 16, (i // (w // 16)) * 16
+        b = Y[by:by + 16, bx:bx + 16]
+        c8 = (b[0::2, 0::2] + b[0::2, 1::2] + b[1::2, 0::2] + b[1::2, 1::2] + 2) >> 2
+        win = R[32 + by - 24:32 + by + 40, 32 + bx - 24:32 + bx + 40]
+        r32 = (win[0::2, 0::2] + win[0::2, 1::2] + win[1::2, 0::2] + win[1::2, 1::2] + 2) >> 2
+        mb = []
+        for q in range(4):
+            oy, ox = (q >> 1) * 4, (q & 1) * 4
+            keys = []
+            for p in range(289):
+                dy, dx = p // 17 - 8, p % 17 - 8
+                c = np.abs(c8[oy:oy + 4, ox:ox + 4] - r32[12 + dy + oy:16 + dy + oy, 12 + dx + ox:16 + dx + ox]).sum()
+                keys.append((int(c), abs(dx) + abs(dy), p))
+            mb.append([(2 * (k[2] % 17 - 8), 2 * (k[2] // 17 - 8)) for k in sorted(keys)[:2]])
+        out.append(mb)
+    return out
+
+
 def _features_restated(cur, ref, w, h, qp):
     """The correlation map of every MB in numpy (the definition in fdvgpu_nn.hip)."""
     Y = cur[:w * h].reshape(h, w).astype(np.int64)
@@ -128,6 +153,7 @@ def test_features_and_inference_match_oracle():
     an.put_source(0, 0, f1)
     feats = _gpu_features(an, 0, w, h, 26)
     want = _features_restated(f1, f0, w, h, 26)
+    quads = _quadrant_lowest_restated(f1, f0, w, h)
     assert np.allclose(feats, want, rtol=1e-6, atol=1e-6)
     say("  nn inference: learned frame 96x64")
     an.pframe([0], [26], navail=[1], me_mode=G.ME_LEARNED)
@@ -151,6 +177,8 @@ def test_features_and_inference_match_oracle():
         low = [k[2] for k in sorted(key)[:4]]
         assert list(zip(props[i]["kx"].tolist(), props[i]["ky"].tolist())) == \
             [(2 * (p % 17 - 8), 2 * (p // 17 - 8)) for p in low]
+        for q in range(4):
+            assert list(zip(props[i]["qx"][q].tolist(), props[i]["qy"][q].tolist())) == quads[i][q], (i, q)
     an.close()
 
 
