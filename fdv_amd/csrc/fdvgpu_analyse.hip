@@ -1023,8 +1023,10 @@ __device__ __forceinline__ uint64_t mrec_pack(uint64_t tag, int inter, int ref, 
            ((uint64_t)(uint16_t)x << 16) | (uint64_t)(uint16_t)y;
 }
 
+#ifdef FDVG_PHASE_PROF
 // Quarter-pel luma sample from a small tile of stride S whose integer sample
-// for the output pixel is at (X, Y): the H.264 positions of fdv.h:2931-2990.
+// for the output pixel is at (X, Y): the H.264 positions of fdv.h:2931-2990,
+// one sample at a time (the profiling build checks qerr4 against it).
 template <int S> __device__ __forceinline__ int t_h(const uint8_t *w, int x, int y) {
     const uint8_t *p = w + y * S + x;
     return tap6(p[-2], p[-1], p[0], p[1], p[2], p[3]);
@@ -1061,6 +1063,7 @@ template <int S> __device__ __forceinline__ int lumaq_t(const uint8_t *w, int X,
     default: return av2(t_hh<S>(w, X + 1, Y), t_b<S>(w, X, Y + 1));
     }
 }
+#endif
 
 // SSD of the 16x16 block `curb` against its prediction from a tile (row stride
 // S, the block's (0, 0) at tile (2, 2)) at quarter-pel phase ph, by one
