@@ -93,6 +93,24 @@ def test_720p_eight_bands_full_size():
     assert run_parity(bands, 3, [24] * 8, label="720p-bands") == 0
 
 
+def test_720p_eight_bands_full_size_parallel_mode():
+    """The block-parallel decisions at their real shape: every band's table,
+    stand-in costs, replay, coding, streams and recon equal to the oracle's
+    restatement over a chain of P-frames."""
+    sc = content.scene_pan(1280, 720, 4)
+    bands = [_Band(sc, y0, min(96, 720 - y0)) for y0 in range(0, 720, 96)]
+    assert run_parity(bands, 4, [24] * 8, me_mode=G.ME_PARALLEL, label="720p-parallel") == 0
+
+
+def test_stress_blobs_parallel_mode():
+    """Fast small blobs with a scene cut and heavy grain (the stress content at
+    640x368): many intra MBs, so long chains through the intra pass's handoff
+    words, and every decision path of the replay; equal to the oracle."""
+    sc = content.scene_blobs(640, 368, 7, nblob=120, cut=3)
+    bands = [_Band(sc, y0, min(96, 368 - y0)) for y0 in range(0, 368, 96)]
+    assert run_parity(bands, 5, [20, 28, 34, 40], me_mode=G.ME_PARALLEL, label="stress-parallel") == 0
+
+
 def test_720p_eight_bands_given_reference_decisions():
     sc = content.scene_pan(1280, 720, 3)
     bands = [_Band(sc, y0, min(96, 720 - y0)) for y0 in range(0, 720, 96)]
