@@ -1120,8 +1120,9 @@ __device__ __forceinline__ void pr_load(PrPre &R, const fdvg_seq &q, const fdvg_
         }
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
-    const uint32_t *gp = (const uint32_t *)(pcs + q.mb0 + i0);
-    const uint32_t *gm = (const uint32_t *)(q.metab + i0);
+    const gmem_t<uint32_t> *gp = gmem((const uint32_t *)(pcs + q.mb0 + i0));
+    const gmem_t<uint32_t> *gm = gmem((const uint32_t *)(q.metab + i0));
+    const gmem_t<uint8_t> *gc = gmem(q.cy);
     const int np = n * (int)(sizeof(fdvg_pcost) / 4), nm = n * (int)(sizeof(fdvg_metab) / 4);
 #pragma unroll
     for (int j = 0; j < PR_PW; ++j) { const int k = L + 64 * j; R.p[j] = k < np ? gp[k] : 0u; }
@@ -1130,7 +1131,7 @@ __device__ __forceinline__ void pr_load(PrPre &R, const fdvg_seq &q, const fdvg_
 #pragma unroll
     for (int j = 0; j < PR_CW; ++j) {
         const int k = L + 64 * j, m = k >> 6, rr = (k >> 2) & 15, cc = (k & 3) * 4;
-        R.c[j] = m < n ? *(const uint32_t *)(q.cy + (size_t)(my * 16 + rr) * q.w + (mx0 + m) * 16 + cc) : 0u;
+        R.c[j] = m < n ? *(const gmem_t<uint32_t> *)(gc + (size_t)(my * 16 + rr) * q.w + (mx0 + m) * 16 + cc) : 0u;
     }
     R.n = n;
 }
@@ -1149,11 +1150,13 @@ extern "C" __global__ void __launch_bounds__(512)
 k_prepair(const fdvg_seq *__restrict__ seqs, int mwmax, unsigned *ctl,
           const fdvg_pcost *__restrict__ pcs, const unsigned *__restrict__ pflag, fdvg_mbinfo *__restrict__ given,
           fdvg_pctx *__restrict__ pctx, unsigned *__restrict__ ilist) {
-    extern __shared__ uint64_t dyn[];
+    extern __shared__ __align__(16) uint64_t dyn[];
     const int NW = blockDim.x >> 6, wv = threadIdx.x >> 6, L = threadIdx.x & 63;
     uint64_t *ring = dyn;                                    // NW slots x mwmax motion records
     int *prog = (int *)(ring + (size_t)NW * mwmax);          // NW progress words
-    PrWave *pw = (PrWave *)(((uintptr_t)(prog + NW) + 15) & ~(uintptr_t)15);
+    // offset from the LDS base, not an integer round trip, so that the
+    // compiler keeps the LDS address space (ds_* rather than flat accesses)
+    PrWave *pw = (PrWave *)((char *)dyn + ((((size_t)NW * mwmax) * 8 + (size_t)NW * 4 + 15) & ~(size_t)15));
     PrWave &W = pw[wv];
     const fdvg_seq q = seqs[blockIdx.x];
     const int mw = q.w >> 4, mh = q.h >> 4;
@@ -1201,7 +1204,7 @@ k_prepair(const fdvg_seq *__restrict__ seqs, int mwmax, unsigned *ctl,
             const bool hit0 = __ballot(L < nsc && sx_l == pmx && sy_l == pmy && sr_l == 0) != 0;
             uint8_t tv0[7];
             if (!hit0) {
-                const uint8_t *rp = q.ry[0] + (ptrdiff_t)(by + (pmy >> 2) - 2) * q.ys + bx + (pmx >> 2) - 2;
+                const gmem_t<uint8_t> *rp = gmem(q.ry[0]) + (ptrdiff_t)(by + (pmy >> 2) - 2) * q.ys + bx + (pmx >> 2) - 2;
 #pragma unroll
                 for (int j = 0; j < 7; ++j) {
                     const int k = L + 64 * j;
@@ -1253,7 +1256,7 @@ k_prepair(const fdvg_seq *__restrict__ seqs, int mwmax, unsigned *ctl,
 #pragma unroll
                         for (int j = 0; j < 7; ++j) tv[j] = tv0[j];
                     } else {
-                        const uint8_t *rp = (r ? q.ry[1] : q.ry[0]) +
+                        const gmem_t<uint8_t> *rp = gmem(r ? q.ry[1] : q.ry[0]) +
                                             (ptrdiff_t)(by + (y >> 2) - 2) * q.ys + bx + (x >> 2) - 2;
 #pragma unroll
                         for (int j = 0; j < 7; ++j) {
