@@ -349,10 +349,10 @@ __device__ __forceinline__ int hbyte(const Smem &s, int j, int b) {
 // chroma windows of each reference around it.  The caller synchronises.
 __device__ __forceinline__ void mb_stage(Smem &s, const fdvg_seq &q, int mb, int bx, int by) {
     const int tid = threadIdx.x, cw = q.w >> 1, cbx = bx >> 1, cby = by >> 1;
-    s.cur[tid] = q.cy[(size_t)(by + (tid >> 4)) * q.w + bx + (tid & 15)];
+    s.cur[tid] = gmem(q.cy)[(size_t)(by + (tid >> 4)) * q.w + bx + (tid & 15)];
     if (tid < 128) {
         const int pl = tid >> 6, p = tid & 63;
-        const uint8_t *src = pl ? q.cv : q.cu;
+        const gmem_t<uint8_t> *src = gmem(pl ? q.cv : q.cu);
         s.ccur[tid] = src[(size_t)(cby + (p >> 3)) * cw + cbx + (p & 7)];
     }
     const bool need_ref = !q.force_skip && !(q.given && q.given[mb - q.mb0].mode == 2);
@@ -361,15 +361,15 @@ __device__ __forceinline__ void mb_stage(Smem &s, const fdvg_seq &q, int mb, int
         for (int r = 0; r < nwin; ++r) {
             // 64 rows x 64 bytes: 16 bytes a thread, two 8-byte loads (8-aligned)
             const int wy = tid >> 2, wx = (tid & 3) * 16;
-            const uint8_t *src = (r ? q.ry[1] : q.ry[0]) + (ptrdiff_t)(by - WOFF + wy) * q.ys + bx - WOFF + wx;
-            uint64_t a = *(const uint64_t *)src, b = *(const uint64_t *)(src + 8);
+            const gmem_t<uint8_t> *src = gmem(r ? q.ry[1] : q.ry[0]) + (ptrdiff_t)(by - WOFF + wy) * q.ys + bx - WOFF + wx;
+            uint64_t a = *(const gmem_t<uint64_t> *)src, b = *(const gmem_t<uint64_t> *)(src + 8);
             *(uint64_t *)&s.win[r][wy * WIN + wx] = a;
             *(uint64_t *)&s.win[r][wy * WIN + wx + 8] = b;
             // 2 planes x 32 rows x 32 bytes: 8 bytes a thread (4-aligned)
             const int pl = tid >> 7, t2 = tid & 127, cyy = t2 >> 2, cxx = (t2 & 3) * 8;
-            const uint8_t *cp = (pl ? (r ? q.rv[1] : q.rv[0]) : (r ? q.ru[1] : q.ru[0])) +
-                                (ptrdiff_t)(cby - CWOFF + cyy) * q.cs + cbx - CWOFF + cxx;
-            const uint32_t *c32 = (const uint32_t *)cp;
+            const gmem_t<uint8_t> *cp = gmem(pl ? (r ? q.rv[1] : q.rv[0]) : (r ? q.ru[1] : q.ru[0])) +
+                                        (ptrdiff_t)(cby - CWOFF + cyy) * q.cs + cbx - CWOFF + cxx;
+            const gmem_t<uint32_t> *c32 = (const gmem_t<uint32_t> *)cp;
             uint32_t u0 = c32[0], u1 = c32[1];
             *(uint32_t *)&s.cwin[r][pl][cyy * CWIN + cxx] = u0;
             *(uint32_t *)&s.cwin[r][pl][cyy * CWIN + cxx + 4] = u1;
@@ -614,10 +614,10 @@ __device__ __forceinline__ const uint8_t *mb_finish(Smem &s, const fdvg_seq &q, 
     }
 
     // reconstruction out (the next frame's reference, deblocked later)
-    q.oy[(ptrdiff_t)(by + (tid >> 4)) * q.ys + bx + (tid & 15)] = yrec[tid];
+    gmemw(q.oy)[(ptrdiff_t)(by + (tid >> 4)) * q.ys + bx + (tid & 15)] = yrec[tid];
     if (tid < 128) {
         const int pl = tid >> 6, p = tid & 63;
-        (pl ? q.ov : q.ou)[(ptrdiff_t)(cby + (p >> 3)) * q.cs + cbx + (p & 7)] = s.crec[tid];
+        gmemw(pl ? q.ov : q.ou)[(ptrdiff_t)(cby + (p >> 3)) * q.cs + cbx + (p & 7)] = s.crec[tid];
     }
     // decision record for the serialiser
     const int16_t *lsrc = mode == 1 ? s.levI : mode == 2 ? s.levN : s.lev8;
@@ -647,7 +647,7 @@ __device__ __forceinline__ const uint8_t *mb_finish(Smem &s, const fdvg_seq &q, 
             }
         }
         if (q.info) {
-            fdvg_mbinfo *I = &q.info[mb - q.mb0];
+            gmem_t<fdvg_mbinfo> *I = gmemw(&q.info[mb - q.mb0]);
             I->mode = (uint8_t)mode; I->ref = R->ref; I->merge_idx = (uint8_t)D.bmi; I->sub = (uint8_t)D.nsub;
             for (int k = 0; k < 4; ++k) { I->mv[k][0] = R->mv[k][0]; I->mv[k][1] = R->mv[k][1]; }
         }
@@ -809,30 +809,6 @@ k_analyse(const fdvg_seq *__restrict__ seqs, const int2 *__restrict__ rowtab, in
 // every decision is made; the intra sub-mode is screened again at coding time on
 // the true reconstruction.
 
-// Neighbours of an MB from a plane (gather_nb, fdv.h:5201-5212): luma corner +
-// top row into s.hbytes, left column into s.lcol[0..15]; with `chroma`, also
-// the chroma corners + top rows (s.ctop) and left columns (s.lcol[16..31]).
-__device__ __forceinline__ void nb_load(Smem &s, const uint8_t *py, int ys, const uint8_t *pu,
-                                        const uint8_t *pv, int cs, int bx, int by, int ht, int hl,
-                                        bool chroma) {
-    const int tid = threadIdx.x, cbx = bx >> 1, cby = by >> 1;
-    if (tid < 16) s.hbytes[1 + tid] = ht ? py[(ptrdiff_t)(by - 1) * ys + bx + tid] : 0;
-    else if (tid == 16) s.hbytes[0] = (ht && hl) ? py[(ptrdiff_t)(by - 1) * ys + bx - 1] : 128;
-    else if (tid >= 32 && tid < 48) s.lcol[tid - 32] = hl ? py[(ptrdiff_t)(by + tid - 32) * ys + bx - 1] : 0;
-    if (chroma) {
-        if (tid >= 64 && tid < 82) {
-            const int pl = (tid - 64) / 9, k = (tid - 64) % 9;
-            const uint8_t *p = pl ? pv : pu;
-            s.ctop[pl][k] = k ? (ht ? p[(ptrdiff_t)(cby - 1) * cs + cbx + k - 1] : 0)
-                              : ((ht && hl) ? p[(ptrdiff_t)(cby - 1) * cs + cbx - 1] : 128);
-        } else if (tid >= 96 && tid < 112) {
-            const int pl = (tid - 96) >> 3, k = (tid - 96) & 7;
-            const uint8_t *p = pl ? pv : pu;
-            s.lcol[16 + 8 * pl + k] = hl ? p[(ptrdiff_t)(cby + k) * cs + cbx - 1] : 0;
-        }
-    }
-}
-
 __device__ __forceinline__ int mb_count(const fdvg_seq &q) { return (q.w >> 4) * (q.h >> 4); }
 
 // An MB's costs are complete: every wave's stores are released at agent scope,
@@ -868,7 +844,7 @@ k_pcost(const fdvg_seq *__restrict__ seqs, fdvg_pcost *__restrict__ pcs, uint8_t
     const int tid = threadIdx.x, wave = tid >> 6;
     const int mw = q.w >> 4, mx = i % mw, my = i / mw, bx = mx * 16, by = my * 16;
     const int mb = q.mb0 + i;
-    const fdvg_metab *T = &q.metab[i];
+    const gmem_t<fdvg_metab> *T = gmem(&q.metab[i]);
     fdvg_pcost *P = &pcs[mb];
     mb_stage(s, q, mb, bx, by);
     __syncthreads();
@@ -889,7 +865,7 @@ k_pcost(const fdvg_seq *__restrict__ seqs, fdvg_pcost *__restrict__ pcs, uint8_t
     const int nbx[5] = {mx - 1, mx, mx + 1, mx - 1, mx - 2}, nby[5] = {my, my - 1, my - 1, my - 1, my};
     for (int k = 0; k < 5; ++k) {
         if (nbx[k] < 0 || nbx[k] >= mw || nby[k] < 0) continue;
-        const fdvg_metab *A = &q.metab[nby[k] * mw + nbx[k]];
+        const gmem_t<fdvg_metab> *A = gmem(&q.metab[nby[k] * mw + nbx[k]]);
         add(A->mv16[0][0], A->mv16[0][1], 0);
         if (k < 3) { add(A->mv16[1][0], A->mv16[1][1], 1); add(A->mv8[3][0], A->mv8[3][1], 0); }
     }
@@ -981,12 +957,12 @@ k_picost(const fdvg_seq *__restrict__ seqs, fdvg_pcost *__restrict__ pcs, const 
     const int mw = q.w >> 4, mx = i % mw, my = i / mw, bx = mx * 16, by = my * 16;
     const int mb = q.mb0 + i;
     fdvg_pcost *P = &pcs[mb];
-    if (q.metab[i].flags & 1) {              // the proposer expects SKIP: no INTRA
+    if (gmem(q.metab)[i].flags & 1) {        // the proposer expects SKIP: no INTRA
         if (tid == 0) { P->dn = P->bn = 0; P->nsub = 0; }
         pcost_publish(pflag ? pflag + mb : nullptr, q.step);
         return;
     }
-    s.cur[tid] = q.cy[(size_t)(by + (tid >> 4)) * q.w + bx + (tid & 15)];
+    s.cur[tid] = gmem(q.cy)[(size_t)(by + (tid >> 4)) * q.w + bx + (tid & 15)];
     const uint8_t *E = edge + (size_t)q.mb0 * 32;
     if (tid < 16) s.hbytes[1 + tid] = my > 0 ? E[(size_t)(i - mw) * 32 + tid] : 0;
     else if (tid == 16) s.hbytes[0] = (my > 0 && mx > 0) ? E[(size_t)(i - mw - 1) * 32 + 15] : 128;
@@ -1514,9 +1490,11 @@ __device__ __forceinline__ int ih_byte(const uint64_t *ih, int w, int byte, unsi
     return (int)((v >> (8 * byte)) & 0xff);
 }
 
-// The neighbours of an intra MB in the block-parallel pass (the layout of
-// nb_load): intra neighbours from their handoff words, the others (coded by
-// an earlier kernel) from the reconstruction planes.
+// The neighbours of an intra MB in the block-parallel pass (gather_nb,
+// fdv.h:5201-5212): luma corner + top row into s.hbytes, left column into
+// s.lcol[0..15], chroma corners + top rows into s.ctop and left columns into
+// s.lcol[16..31].  Intra neighbours come from their handoff words, the others
+// (coded by an earlier kernel) from the reconstruction planes.
 __device__ __forceinline__ void nb_load_ih(Smem &s, const fdvg_seq &q, const fdvg_mbinfo *gq, const uint64_t *ih,
                                            unsigned *ctl, int mx, int my, int mw, int bx, int by) {
     const int tid = threadIdx.x, cbx = bx >> 1, cby = by >> 1, ht = my > 0, hl = mx > 0;
@@ -1525,7 +1503,7 @@ __device__ __forceinline__ void nb_load_ih(Smem &s, const fdvg_seq &q, const fdv
     const bool tl_i = ht && hl && gq[i_tl].mode == 2;
     const uint64_t *ih0 = ih + (size_t)q.mb0 * IH_WORDS;
     const unsigned step = q.step;
-    const uint8_t *py = q.oy, *pu = q.ou, *pv = q.ov;
+    const gmem_t<uint8_t> *py = gmem(q.oy), *pu = gmem(q.ou), *pv = gmem(q.ov);
     const int ys = q.ys, cs = q.cs;
     if (tid < 16) {
         s.hbytes[1 + tid] = !ht ? 0 : top_i ? ih_byte(ih0 + (size_t)i_top * IH_WORDS, tid >> 2, tid & 3, step, ctl)
@@ -1539,7 +1517,7 @@ __device__ __forceinline__ void nb_load_ih(Smem &s, const fdvg_seq &q, const fdv
                                      : py[(ptrdiff_t)(by + k) * ys + bx - 1];
     } else if (tid >= 64 && tid < 82) {
         const int pl = (tid - 64) / 9, k = (tid - 64) % 9;
-        const uint8_t *p = pl ? pv : pu;
+        const gmem_t<uint8_t> *p = pl ? pv : pu;
         if (k == 0)
             s.ctop[pl][0] = !(ht && hl) ? 128 : tl_i ? ih_byte(ih0 + (size_t)i_tl * IH_WORDS, pl ? 7 : 5, 3, step, ctl)
                                                      : p[(ptrdiff_t)(cby - 1) * cs + cbx - 1];
@@ -1549,7 +1527,7 @@ __device__ __forceinline__ void nb_load_ih(Smem &s, const fdvg_seq &q, const fdv
                                             : p[(ptrdiff_t)(cby - 1) * cs + cbx + k - 1];
     } else if (tid >= 96 && tid < 112) {
         const int pl = (tid - 96) >> 3, k = (tid - 96) & 7;
-        const uint8_t *p = pl ? pv : pu;
+        const gmem_t<uint8_t> *p = pl ? pv : pu;
         s.lcol[16 + 8 * pl + k] = !hl ? 0 : left_i ? ih_byte(ih0 + (size_t)i_left * IH_WORDS, (pl ? 14 : 12) + (k >> 2),
                                                              k & 3, step, ctl)
                                                    : p[(ptrdiff_t)(cby + k) * cs + cbx - 1];

@@ -55,6 +55,7 @@ template <class T> using gmem_t = __attribute__((address_space(1))) T;
 template <class T> __device__ __forceinline__ const gmem_t<T> *gmem(const T *p) {
     return (const gmem_t<T> *)p;
 }
+template <class T> __device__ __forceinline__ gmem_t<T> *gmemw(T *p) { return (gmem_t<T> *)p; }
 __device__ __forceinline__ int clip8(int v) { return v < 0 ? 0 : (v > 255 ? 255 : v); }
 __device__ __forceinline__ uint32_t zmap(int v) { return ((uint32_t)v << 1) ^ (uint32_t)(v >> 31); }
 __device__ __forceinline__ int egbits(uint32_t v) { return 2 * (31 - __clz((int)(v + 1u))) + 1; }

@@ -492,18 +492,19 @@ __device__ __forceinline__ void ni_features(NiWave &w, const uint8_t *cur, int c
     int *cost = (int *)(c8 + 1088);
     {
         const int y = l >> 3, x = l & 7;
-        const uint8_t *p = cur + (size_t)(by + 2 * y) * cw + bx + 2 * x;
+        const gmem_t<uint8_t> *p = gmem(cur) + (size_t)(by + 2 * y) * cw + bx + 2 * x;
         c8[l] = (uint8_t)((p[0] + p[1] + p[cw] + p[cw + 1] + 2) >> 2);
     }
     {
         // lane: r32 row l/2, columns 16 (l&1) .. +15, from two reference rows of 32 bytes (8-byte aligned)
         const int y = l >> 1, h = l & 1;
-        const uint8_t *p0 = ref + (ptrdiff_t)(by - 24 + 2 * y) * rs + bx - 24 + 32 * h;
-        const uint8_t *p1 = p0 + rs;
+        const gmem_t<uint8_t> *p0 = gmem(ref) + (ptrdiff_t)(by - 24 + 2 * y) * rs + bx - 24 + 32 * h;
+        const gmem_t<uint8_t> *p1 = p0 + rs;
         uint32_t o[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            const uint2 a = *(const uint2 *)(p0 + 8 * k), b = *(const uint2 *)(p1 + 8 * k);
+            const uint64_t a64 = *(const gmem_t<uint64_t> *)(p0 + 8 * k), b64 = *(const gmem_t<uint64_t> *)(p1 + 8 * k);
+            const uint2 a = make_uint2((uint32_t)a64, (uint32_t)(a64 >> 32)), b = make_uint2((uint32_t)b64, (uint32_t)(b64 >> 32));
             uint32_t v = 0;
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
