@@ -851,7 +851,9 @@ k_pcost(const fdvg_seq *__restrict__ seqs, fdvg_pcost *__restrict__ pcs, uint8_t
 
     // SKIP SSDs of likely merge candidates: (0,0), and the table vectors of the
     // left, above, above-right (16x16 on both references, last quadrant), above-
-    // left and left-left MBs; deduplicated, references that exist only
+    // left and left-left MBs; deduplicated, references that exist only.  The left
+    // MB's vector reaches this MB only as the predictor, which is always tried on
+    // reference 0, so its reference-1 vector is taken on reference 0.
     typedef int i16v __attribute__((ext_vector_type(16)));
     static_assert(FDVG_NSC <= 16, "candidate vectors");
     int ncs = 0;
@@ -867,7 +869,10 @@ k_pcost(const fdvg_seq *__restrict__ seqs, fdvg_pcost *__restrict__ pcs, uint8_t
         if (nbx[k] < 0 || nbx[k] >= mw || nby[k] < 0) continue;
         const gmem_t<fdvg_metab> *A = gmem(&q.metab[nby[k] * mw + nbx[k]]);
         add(A->mv16[0][0], A->mv16[0][1], 0);
-        if (k < 3) { add(A->mv16[1][0], A->mv16[1][1], 1); add(A->mv8[3][0], A->mv8[3][1], 0); }
+        if (k < 3) {
+            if (q.navail > 1) add(A->mv16[1][0], A->mv16[1][1], k == 0 ? 0 : 1);
+            add(A->mv8[3][0], A->mv8[3][1], 0);
+        }
     }
     {
         // wave w takes candidates w, w + 4, w + 8; lane L column L & 15, rows 4 (L >> 4) .. + 3
