@@ -1052,7 +1052,7 @@ template <int S> __device__ __forceinline__ int lumaq_t(const uint8_t *w, int X,
 template <int S>
 __device__ __forceinline__ int ssd_tile_q(const uint8_t *tile, const uint8_t *curb, int ph, int L) {
     const int x = L & 15, y0 = (L >> 4) * 4;
-    return qerr4<S, true>(tile, x + 2, y0 + 2, ph, curb + y0 * 16 + x, 16);
+    return qerr4<S, true, true>(tile, x + 2, y0 + 2, ph, curb + y0 * 16 + x, 16);
 }
 
 // The decision wavefront of one sequence inside one workgroup: wave w replays

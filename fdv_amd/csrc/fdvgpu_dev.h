@@ -120,15 +120,23 @@ __device__ __forceinline__ int pick8(int i, int v0, int v1, int v2, int v3, int 
 // every quarter-pel sample of lumaq's definition is formed in registers; the
 // phase may differ between lanes, so the sample is picked without branches as
 // av2 of two of the eight candidate values.  Reads dwords up to 9 bytes past
-// column X + 3 of each row.
-template <int S, bool SQ>
+// column X + 3 of each row.  BURST issues all 27 reads before any is used (for
+// a caller on a serial path with registers to spare; otherwise the scheduler
+// may interleave them with their uses, one read in flight at a time).
+template <int S, bool SQ, bool BURST = false>
 __device__ __forceinline__ int qerr4(const uint8_t *w, int X, int Y, int ph, const uint8_t *cur, int cs) {
     const int c0 = X - 2, sh = c0 & 3;
-    int V[9][6];
+    uint32_t R[9][3];
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
         const uint32_t *row = (const uint32_t *)(w + (Y - 2 + i) * S + (c0 & ~3));
-        const uint32_t w0 = row[0], w1 = row[1], w2 = row[2];
+        R[i][0] = row[0]; R[i][1] = row[1]; R[i][2] = row[2];
+    }
+    if (BURST) __builtin_amdgcn_sched_barrier(0);
+    int V[9][6];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        const uint32_t w0 = R[i][0], w1 = R[i][1], w2 = R[i][2];
         const uint32_t a = __builtin_amdgcn_alignbyte(w1, w0, sh), b = __builtin_amdgcn_alignbyte(w2, w1, sh);
 #pragma unroll
         for (int j = 0; j < 4; ++j) V[i][j] = (int)((a >> (8 * j)) & 255u);
