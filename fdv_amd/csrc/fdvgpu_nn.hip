@@ -605,29 +605,31 @@ __device__ __forceinline__ void ni_features(NiWave &w, const uint8_t *cur, int c
     }
 }
 
-// The wavefront's best (value, position): larger value, then lower position.
-// Inside each 16-lane row by DPP (quad swaps, then rotations by 4 and 8), then
-// the four row results by readlane; the result is uniform.
-__device__ __forceinline__ void dpp_best(float &v, int &p, float ov, int op) {
-    if (ov > v || (ov == v && op < p)) { v = ov; p = op; }
-}
-#define NN_DPP_STEP(ctrl)                                                                                 \
-    dpp_best(v, p, __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), ctrl, 0xf, 0xf, false)), \
-             __builtin_amdgcn_update_dpp(0, p, ctrl, 0xf, 0xf, false))
-__device__ __forceinline__ void wave_argmax(float &v, int &p) {
-    NN_DPP_STEP(0xB1);             // quad_perm [1,0,3,2]
-    NN_DPP_STEP(0x4E);             // quad_perm [2,3,0,1]
-    NN_DPP_STEP(0x124);            // row_ror:4
-    NN_DPP_STEP(0x128);            // row_ror:8
-    float bv = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0));
-    int bp = __builtin_amdgcn_readlane(p, 0);
+// The wavefront's largest 64-bit key: inside each 16-lane row by DPP (quad
+// swaps, then rotations by 4 and 8), then the four row results by readlane;
+// the result is uniform.
+#define NN_DPP_MAX(ctrl)                                                                                \
+    do {                                                                                                \
+        const uint32_t lo_ = __builtin_amdgcn_update_dpp(0u, (uint32_t)k, ctrl, 0xf, 0xf, false);        \
+        const uint32_t hi_ = __builtin_amdgcn_update_dpp(0u, (uint32_t)(k >> 32), ctrl, 0xf, 0xf, false); \
+        const uint64_t o_ = ((uint64_t)hi_ << 32) | lo_;                                               \
+        k = o_ > k ? o_ : k;                                                                            \
+    } while (0)
+__device__ __forceinline__ uint64_t wave_max_u64(uint64_t k) {
+    NN_DPP_MAX(0xB1);              // quad_perm [1,0,3,2]
+    NN_DPP_MAX(0x4E);              // quad_perm [2,3,0,1]
+    NN_DPP_MAX(0x124);             // row_ror:4
+    NN_DPP_MAX(0x128);             // row_ror:8
+    uint64_t b = 0;
 #pragma unroll
-    for (int r = 16; r < 64; r += 16)
-        dpp_best(bv, bp, __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), r)), __builtin_amdgcn_readlane(p, r));
-    v = bv;
-    p = bp;
+    for (int r = 0; r < 64; r += 16) {
+        const uint64_t o = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(k >> 32), r) << 32) |
+                           (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)k, r);
+        b = o > b ? o : b;
+    }
+    return b;
 }
-#undef NN_DPP_STEP
+#undef NN_DPP_MAX
 
 // Per MB of every sequence: the 8 most likely whole-pel vectors (nearest
 // reference), their total probability and the SKIP probability.
@@ -800,25 +802,33 @@ k_nn_infer(const fdvg_seq *__restrict__ seqs, int nseq, int total, const float *
         }
         for (int o = 32; o > 0; o >>= 1) se += __shfl_xor(se, o);
         NPH(3);
-        // top 8, ties to the lower position
+        // top 8, ties to the lower position.  Each logit becomes a key: its bits
+        // mapped to an unsigned order (-0 taken as +0, which compares equal to
+        // it) above the position's complement, so the largest key is the largest
+        // logit, the lowest position among equal ones; a taken or absent
+        // position's key is 0, below every logit's.
         fdvg_nnout &o = out[gi];
         float conf = 0.f;
-        int taken = 0;
+        uint64_t key[10];
+#pragma unroll
+        for (int tl = 0; tl < 10; ++tl) {
+            const int pp = tl * 32 + (rr >> 2) * 8 + g * 4 + (rr & 3);
+            uint32_t u = __float_as_uint(lg[tl] + 0.0f);
+            u = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+            key[tl] = pp < NN_P ? ((uint64_t)u << 32) | (uint32_t)(0xffff - pp) : 0ull;
+        }
+#pragma unroll
         for (int kk = 0; kk < 8; ++kk) {
-            float bv = -3.0e38f;
-            int bp = NN_PT;
+            uint64_t b = key[0];
 #pragma unroll
-            for (int tl = 0; tl < 10; ++tl) {
-                const int pp = tl * 32 + (rr >> 2) * 8 + g * 4 + (rr & 3);
-                if (!((taken >> tl) & 1) && pp < NN_P && (lg[tl] > bv || (lg[tl] == bv && pp < bp))) { bv = lg[tl]; bp = pp; }
-            }
-            wave_argmax(bv, bp);
+            for (int tl = 1; tl < 10; ++tl) b = key[tl] > b ? key[tl] : b;
+            b = wave_max_u64(b);
+#pragma unroll
+            for (int tl = 0; tl < 10; ++tl) key[tl] = key[tl] == b ? 0ull : key[tl];
+            const uint32_t u = (uint32_t)(b >> 32);
+            const float bv = __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u);
+            const int bp = 0xffff - (int)(b & 0xffffu);
             conf += expf(bv - m) / se;
-#pragma unroll
-            for (int tl = 0; tl < 10; ++tl) {
-                const int pp = tl * 32 + (rr >> 2) * 8 + g * 4 + (rr & 3);
-                if (pp == bp) taken |= 1 << tl;
-            }
             if (l == 0) { o.cx[kk] = (int8_t)(2 * (bp % NN_G - 8)); o.cy[kk] = (int8_t)(2 * (bp / NN_G - 8)); }
         }
         if (l == 0) { o.conf = conf; o.pskip = 1.f / (1.f + expf(-skl)); }
