@@ -484,6 +484,18 @@ __device__ __forceinline__ u4v ni_pack8(const float *f) {
 // smaller |dx| + |dy|, then the lower position): classical coarse search
 // candidates beside the network's; and each 8x8 quadrant's 2 lowest-cost
 // positions of its own map into o->qx, o->qy (same ties).
+// The wavefront's smallest unsigned value: inside each 16-lane row by DPP (quad
+// swaps, then rotations by 4 and 8), then the four row results by readlane; the
+// result is uniform.
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t k) {
+    k = min(k, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)k, 0xB1, 0xf, 0xf, false));   // quad_perm [1,0,3,2]
+    k = min(k, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)k, 0x4E, 0xf, 0xf, false));   // quad_perm [2,3,0,1]
+    k = min(k, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)k, 0x124, 0xf, 0xf, false));  // row_ror:4
+    k = min(k, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)k, 0x128, 0xf, 0xf, false));  // row_ror:8
+    const uint32_t a = (uint32_t)__builtin_amdgcn_readlane((int)k, 0), b = (uint32_t)__builtin_amdgcn_readlane((int)k, 16);
+    const uint32_t c = (uint32_t)__builtin_amdgcn_readlane((int)k, 32), d = (uint32_t)__builtin_amdgcn_readlane((int)k, 48);
+    return min(min(a, b), min(c, d));
+}
 __device__ __forceinline__ void ni_features(NiWave &w, const uint8_t *cur, int cw, const uint8_t *ref, int rs,
                                             int bx, int by, int qp, fdvg_nnout *o) {
     const int l = threadIdx.x & 63;
@@ -562,7 +574,7 @@ __device__ __forceinline__ void ni_features(NiWave &w, const uint8_t *cur, int c
             uint32_t b = qkey[q][0];
 #pragma unroll
             for (int m = 1; m < 5; ++m) b = min(b, qkey[q][m]);
-            for (int s2 = 32; s2 > 0; s2 >>= 1) b = min(b, (uint32_t)__shfl_xor((int)b, s2));
+            b = wave_min_u32(b);
 #pragma unroll
             for (int m = 0; m < 5; ++m) if (qkey[q][m] == b) qkey[q][m] = 0xFFFFFFFFu;
             const int p = (int)(b & 511u);
@@ -583,14 +595,14 @@ __device__ __forceinline__ void ni_features(NiWave &w, const uint8_t *cur, int c
             uint32_t b = key[0];
 #pragma unroll
             for (int m = 1; m < 5; ++m) b = min(b, key[m]);
-            for (int s = 32; s > 0; s >>= 1) b = min(b, (uint32_t)__shfl_xor((int)b, s));
+            b = wave_min_u32(b);
 #pragma unroll
             for (int m = 0; m < 5; ++m) if (key[m] == b) key[m] = 0xFFFFFFFFu;
             const int p = (int)(b & 511u);
             if (l == 0) { o->kx[k] = (int8_t)(2 * (p % NN_G - 8)); o->ky[k] = (int8_t)(2 * (p / NN_G - 8)); }
         }
     }
-    for (int o = 32; o > 0; o >>= 1) mn = min(mn, __shfl_xor(mn, o));
+    mn = (int)wave_min_u32((uint32_t)mn);
     (void)cost;
     const float s0 = 1.0f / (16.0f * exp2f((float)(qp - 4) * (1.0f / 6.0f)));
 #pragma unroll
