@@ -460,10 +460,22 @@ typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef unsigned int u4v __attribute__((ext_vector_type(4)));
 #define NI_RS 40
 #define NI_WAVES 4
+// The map's positions sit in a zero-bordered grid of rows NI_PW wide (one zero
+// column, shared by the right edge of a row and the left edge of the next), so
+// that every 3x3 tap of a position is a constant offset from it and lands on a
+// zero outside the map: position p at ni_pad(p); entries below NI_P0, at or
+// above NI_P0 + 17 NI_PW, and in column 17 are zero.  NI_PR covers the last
+// position's taps.
+#define NI_PW 18
+#define NI_P0 19
+#define NI_PR (NI_P0 + 18 * NI_PW)        // 343: past the bottom zero row
+__host__ __device__ constexpr int ni_pad(int p) { return (p / NN_G) * NI_PW + p % NN_G + NI_P0; }
+__device__ __forceinline__ bool ni_zero(int e) { return e < NI_P0 || e >= NI_P0 + NN_G * NI_PW || (e - NI_P0) % NI_PW == NN_G; }
+static_assert(ni_pad(NN_P - 1) + NI_PW + 1 < NI_PR && ni_pad(0) - NI_PW - 1 >= 0, "every tap inside the grid");
 struct __align__(16) NiWave {
-    uint16_t act[(NN_PT + 1) * NI_RS];     // [pos][channel], row NN_PT zero; also the features' scratch
-    uint32_t fb[NN_PT + 1];                // the two bf16 features of each position; entry NN_PT zero
-    int pad_[3];
+    uint16_t act[NI_PR * NI_RS];           // [padded pos][channel]; rows 0 .. NI_P0 - 1 also the features' scratch
+    uint32_t fb[NI_PR];                    // the two bf16 features of each padded position
+    int pad_[1];
 };
 struct __align__(16) NiShared {
     u4v wl[2][18][64];                     // conv2 / conv3 B operands, bf16 x 8 per (step, lane)
@@ -478,12 +490,6 @@ __device__ __forceinline__ u4v ni_pack8(const float *f) {
     return r;
 }
 
-// The correlation map of one MB into w.fb (bf16 pairs), by one wavefront.
-// Integer SADs and the fp32 feature formulas are those of nn_features.  Also
-// the map's 4 lowest-cost positions into o->kx, o->ky (whole pels; ties to the
-// smaller |dx| + |dy|, then the lower position): classical coarse search
-// candidates beside the network's; and each 8x8 quadrant's 2 lowest-cost
-// positions of its own map into o->qx, o->qy (same ties).
 // The wavefront's smallest unsigned value: inside each 16-lane row by DPP (quad
 // swaps, then rotations by 4 and 8), then the four row results by readlane; the
 // result is uniform.
@@ -496,6 +502,13 @@ __device__ __forceinline__ uint32_t wave_min_u32(uint32_t k) {
     const uint32_t c = (uint32_t)__builtin_amdgcn_readlane((int)k, 32), d = (uint32_t)__builtin_amdgcn_readlane((int)k, 48);
     return min(min(a, b), min(c, d));
 }
+
+// The correlation map of one MB into w.fb (bf16 pairs), by one wavefront.
+// Integer SADs and the fp32 feature formulas are those of nn_features.  Also
+// the map's 4 lowest-cost positions into o->kx, o->ky (whole pels; ties to the
+// smaller |dx| + |dy|, then the lower position): classical coarse search
+// candidates beside the network's; and each 8x8 quadrant's 2 lowest-cost
+// positions of its own map into o->qx, o->qy (same ties).
 __device__ __forceinline__ void ni_features(NiWave &w, const uint8_t *cur, int cw, const uint8_t *ref, int rs,
                                             int bx, int by, int qp, fdvg_nnout *o) {
     const int l = threadIdx.x & 63;
@@ -608,12 +621,8 @@ __device__ __forceinline__ void ni_features(NiWave &w, const uint8_t *cur, int c
 #pragma unroll
     for (int m = 0; m < 5; ++m) {
         const int p = l + 64 * m;
-        if (p < NN_PT) {
-            uint32_t v = 0;
-            if (p < NN_P)
-                v = (uint32_t)f2bf((float)cv[m] * s0) | ((uint32_t)f2bf((float)(cv[m] - mn) * (1.0f / 256.0f)) << 16);
-            w.fb[p] = v;
-        }
+        if (p < NN_P)
+            w.fb[ni_pad(p)] = (uint32_t)f2bf((float)cv[m] * s0) | ((uint32_t)f2bf((float)(cv[m] - mn) * (1.0f / 256.0f)) << 16);
     }
 }
 
@@ -668,8 +677,12 @@ k_nn_infer(const fdvg_seq *__restrict__ seqs, int nseq, int total, const float *
     if (t < 64) S.wf[t] = rbf(W[OFF_WF + t]);
     if (t < 2) S.bf[t] = W[OFF_BF + t];
     // this wave's zero rows
-    if (l < NI_RS / 2) ((uint32_t *)&Wv.act[NN_PT * NI_RS])[l] = 0;
-    if (l == 0) Wv.fb[NN_PT] = 0;
+    for (int e = l; e < NI_PR; e += 64)
+        if (ni_zero(e)) {
+            Wv.fb[e] = 0;
+#pragma unroll
+            for (int c = 0; c < NI_RS / 2; ++c) ((uint32_t *)&Wv.act[e * NI_RS])[c] = 0;
+        }
     // conv1's B operands in registers: k = 16 s + 8 g + j (k = tap*2 + ci, zero beyond 18)
     u4v w1r[2];
 #pragma unroll
@@ -682,20 +695,23 @@ k_nn_infer(const fdvg_seq *__restrict__ seqs, int nseq, int total, const float *
         }
         w1r[s2] = ni_pack8(f);
     }
-    // per tile: this lane's position and which of the 9 taps stay inside the map
-    int tp[10], tmask[10];
+    // per tile: this lane's padded position (the last tile's lanes past the map
+    // take position 0: their outputs are discarded), and the lane half's four
+    // conv1 taps as padded offsets
+    int pb[10];
 #pragma unroll
     for (int tl = 0; tl < 10; ++tl) {
         const int p = tl * 32 + i32;
-        int m = 0;
+        pb[tl] = p < NN_P ? ni_pad(p) : ni_pad(0);
+    }
+    int sh1[4];
 #pragma unroll
-        for (int tap = 0; tap < 9; ++tap) m |= (nn_shift(p, tap) >= 0 ? 1 : 0) << tap;
-        tp[tl] = p;
-        tmask[tl] = m;
+    for (int j = 0; j < 4; ++j) {
+        const int tap = 4 * g + j;
+        sh1[j] = (tap / 3 - 1) * NI_PW + tap % 3 - 1;
     }
     __syncthreads();
     NPH(0);
-    const int zrow = NN_PT * NI_RS * 2;    // byte offset of the zero row
     const uint8_t *actb = (const uint8_t *)Wv.act;
     for (int gi = blockIdx.x * NI_WAVES + wv; gi < total; gi += gridDim.x * NI_WAVES) {
         int k = 0;
@@ -703,6 +719,11 @@ k_nn_infer(const fdvg_seq *__restrict__ seqs, int nseq, int total, const float *
         const fdvg_seq &q = seqs[k];
         const int i = gi - q.mb0, mw = q.w >> 4;
         ni_features(Wv, q.cy, q.w, q.ry[0], q.ys, (i % mw) * 16, (i / mw) * 16, q.qp, &out[gi]);
+        // the features used the first rows as scratch: the zero rows below NI_P0
+        // are zeroed again; the rest it reaches are positions 0 .. 16 of the
+        // first map row, which conv1 rewrites (the next zero entry is NI_P0 + 17)
+        static_assert(64 + 1024 + 4 * NN_P <= (NI_P0 + NN_G) * NI_RS * 2, "features scratch short of the next zero entry");
+        for (int e = l; e < NI_P0 * NI_RS / 2; e += 64) ((uint32_t *)Wv.act)[e] = 0;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -713,19 +734,15 @@ k_nn_infer(const fdvg_seq *__restrict__ seqs, int nseq, int total, const float *
             f16v c = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
             u4v a0, a1 = {0u, 0u, 0u, 0u};
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int tap = 4 * g + j;
-                const int qq = (tmask[tl] >> tap) & 1 ? tp[tl] + (tap / 3 - 1) * NN_G + tap % 3 - 1 : NN_PT;
-                a0[j] = Wv.fb[qq];
-            }
-            if (g == 0) a1[0] = (tmask[tl] >> 8) & 1 ? Wv.fb[tp[tl] + NN_G + 1] : 0u;
+            for (int j = 0; j < 4; ++j) a0[j] = Wv.fb[pb[tl] + sh1[j]];
+            if (g == 0) a1[0] = Wv.fb[pb[tl] + NI_PW + 1];
             c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf8, a0), __builtin_bit_cast(bf8, w1r[0]), c, 0, 0, 0);
             c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf8, a1), __builtin_bit_cast(bf8, w1r[1]), c, 0, 0, 0);
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
                 const int pp = tl * 32 + (r >> 2) * 8 + g * 4 + (r & 3);
-                const float v = pp < NN_P ? c[r] + S.b1[i32] : 0.f;
-                Wv.act[pp * NI_RS + i32] = f2bf(v > 0.f ? v : 0.f);
+                const float v = c[r] + S.b1[i32];
+                if (pp < NN_P) Wv.act[ni_pad(pp) * NI_RS + i32] = f2bf(v > 0.f ? v : 0.f);
             }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -741,15 +758,23 @@ k_nn_infer(const fdvg_seq *__restrict__ seqs, int nseq, int total, const float *
                 for (int r = 0; r < 16; ++r) acc[tl][r] = 0.f;
 #pragma unroll
             for (int st = 0; st < 18; ++st) {
-                const int tap = st >> 1, coff = (16 * (st & 1) + 8 * g) * 2;
-                const int sh = (tap / 3 - 1) * NN_G + tap % 3 - 1;
+                const int tap = st >> 1;
+                const int sh = (tap / 3 - 1) * NI_PW + tap % 3 - 1;
                 const bf8 b = __builtin_bit_cast(bf8, S.wl[layer][st][l]);
+                // the step's ten A operands are read before its MFMAs, so that
+                // the reads are in flight together (a per-tile base one row and
+                // one position back keeps every tap's offset a non-negative
+                // immediate)
+                u4v av[10];
 #pragma unroll
                 for (int tl = 0; tl < 10; ++tl) {
-                    const int off = (tmask[tl] >> tap) & 1 ? (tp[tl] + sh) * (NI_RS * 2) : zrow;
-                    const bf8 a = __builtin_bit_cast(bf8, *(const u4v *)(actb + off + coff));
-                    acc[tl] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[tl], 0, 0, 0);
+                    const int off = (pb[tl] - NI_PW - 1) * (NI_RS * 2) + 16 * g + (sh + NI_PW + 1) * (NI_RS * 2) + 32 * (st & 1);
+                    av[tl] = *(const u4v *)(actb + off);
                 }
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int tl = 0; tl < 10; ++tl)
+                    acc[tl] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf8, av[tl]), b, acc[tl], 0, 0, 0);
             }
             if (layer == 0) {
                 // every read of the buffer has completed (its values fed the MFMAs): rewrite it with conv2
@@ -758,8 +783,8 @@ k_nn_infer(const fdvg_seq *__restrict__ seqs, int nseq, int total, const float *
 #pragma unroll
                     for (int r = 0; r < 16; ++r) {
                         const int pp = tl * 32 + (r >> 2) * 8 + g * 4 + (r & 3);
-                        const float v = pp < NN_P ? acc[tl][r] + S.b2[i32] : 0.f;
-                        Wv.act[pp * NI_RS + i32] = f2bf(v > 0.f ? v : 0.f);
+                        const float v = acc[tl][r] + S.b2[i32];
+                        if (pp < NN_P) Wv.act[ni_pad(pp) * NI_RS + i32] = f2bf(v > 0.f ? v : 0.f);
                     }
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
