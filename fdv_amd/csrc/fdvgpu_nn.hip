@@ -652,6 +652,13 @@ __device__ __forceinline__ uint64_t wave_max_u64(uint64_t k) {
 }
 #undef NN_DPP_MAX
 
+// The value of the lane whose index differs in the bits a DPP row permutation
+// swaps: 0xB1 (quad [1,0,3,2]) is xor 1, 0x4E (quad [2,3,0,1]) xor 2, 0x128
+// (row rotation by 8) xor 8.
+template <int CTRL> __device__ __forceinline__ float dpp_xor(float x) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, 0xf, 0xf, false));
+}
+
 // Per MB of every sequence: the 8 most likely whole-pel vectors (nearest
 // reference), their total probability and the SKIP probability.
 extern "C" __global__ void __launch_bounds__(256, 1)
@@ -817,10 +824,14 @@ k_nn_infer(const fdvg_seq *__restrict__ seqs, int nseq, int total, const float *
 #pragma unroll
                 for (int j = 0; j < half; ++j) {
                     const float keep = bit ? v[j + half] : v[j], send = bit ? v[j] : v[j + half];
-                    v[j] = keep + __shfl_xor(send, 16 >> lev, 32);
+                    // xor 8 and xor 2 stay inside a 16-lane row: exact DPP moves
+                    // (a rotation by 8 and a quad swap), the others by permute
+                    const float got = lev == 1 ? dpp_xor<0x128>(send) : lev == 3 ? dpp_xor<0x4E>(send)
+                                                                      : __shfl_xor(send, 16 >> lev, 32);
+                    v[j] = keep + got;
                 }
             }
-            const float sum = v[0] + __shfl_xor(v[0], 1, 32);
+            const float sum = v[0] + dpp_xor<0xB1>(v[0]);
             const int pp = tl * 32 + (rr >> 2) * 8 + g * 4 + (rr & 3);
             lg[tl] = pp < NN_P ? sum + S.bf[0] : -3.0e38f;
         }
