@@ -500,7 +500,11 @@ extern "C" int fdvgpu_pframe(fdvgpu *g, const int *src_slot, const int *qp, cons
         q.me_mode = me_mode == FDVGPU_ME_DIAMOND ? FDVG_ME_DIAMOND : FDVG_ME_TABLE;
         q.force_skip = force_skip ? force_skip[s] : 0;
         q.mb0 = m.mb0;
-        m.tag = m.tag % 255 + 1;
+        // The wavefront's handoff words carry the tag of the frame that wrote
+        // them, and only that kernel writes them: advancing the tag on its frames
+        // alone keeps consecutive writers' tags distinct however many
+        // block-parallel frames come between them.
+        if (!par) m.tag = m.tag % 255 + 1;
         q.tag = m.tag;
         q.step = step;
         PlaneRef *pl = &planes_h[3 * s];

@@ -25,39 +25,68 @@ PROPOSER_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lea
 _lib = None
 
 
+def _signatures():
+    """Every entry point this module (and the training script) calls, with its C
+    signature: a handle passed without argtypes would be narrowed to a C int."""
+    vp, i, f, d, sz = C.c_void_p, C.c_int, C.c_float, C.c_double, C.c_size_t
+    ip, dp = C.POINTER(C.c_int), C.POINTER(C.c_double)
+    u8pp = C.POINTER(C.POINTER(C.c_uint8))
+    return {
+        # include/fdvgpu.h
+        "fdvgpu_open": (vp, [i, ip, ip, i]),
+        "fdvgpu_close": (None, [vp]),
+        "fdvgpu_last_error": (C.c_char_p, []),
+        "fdvgpu_put_source": (i, [vp, i, i, vp]),
+        "fdvgpu_put_ref": (i, [vp, i, i, vp]),
+        "fdvgpu_set_given": (i, [vp, i, vp]),
+        "fdvgpu_pframe": (i, [vp, ip, ip, dp, ip, ip, i]),
+        "fdvgpu_fetch": (i, [vp]),
+        "fdvgpu_streams": (i, [vp, i, u8pp, C.POINTER(sz)]),
+        "fdvgpu_rewind": (i, [vp]),
+        "fdvgpu_recon": (i, [vp, i, vp]),
+        "fdvgpu_decisions": (i, [vp, i, vp]),
+        "fdvgpu_set_graphs": (i, [vp, i]),
+        "fdvgpu_set_overlap": (i, [vp, i]),
+        "fdvgpu_get_tune": (i, [vp, vp]),
+        "fdvgpu_set_tune": (i, [vp, vp]),
+        "fdvgpu_autotune": (i, [vp, ip, ip, dp, ip, i, i]),
+        "fdvgpu_nn_nparam": (i, []),
+        "fdvgpu_set_proposer": (i, [vp, vp, i]),
+        "fdvgpu_set_proposer_thresholds": (i, [vp, f, f]),
+        "fdvgpu_proposals": (i, [vp, i, vp]),
+        "fdvgpu_motion": (i, [vp, i, vp]),
+        "fdvgpu_planes": (i, [vp, i, i, C.POINTER(vp), ip, C.POINTER(vp), ip]),
+        "fdvgpu_nn_features": (i, [vp, vp, vp, vp, vp, vp, i, vp, vp]),
+        "fdvgpu_nn_train_step": (i, [vp, vp, vp, vp, i, f, f, vp, vp, i, vp]),
+        "fdvgpu_nn_adam": (i, [vp, vp, vp, vp, i, f, i, i, vp]),
+        "fdvgpu_timings": (i, [vp, C.POINTER(C.c_float)]),
+        # include/fdv_amd.h
+        "fdvamd_pack_pframe": (sz, [i, vp, vp, vp, vp, sz]),
+        "fdvamd_encoder_open": (vp, [i, vp, vp, i]),
+        "fdvamd_encoder_close": (None, [vp]),
+        "fdvamd_last_error": (C.c_char_p, []),
+        "fdvamd_set_ref": (i, [vp, i, i, vp]),
+        "fdvamd_set_ref_planes": (i, [vp, i, i, vp, vp, vp]),
+        "fdvamd_pframe_encode": (sz, [vp, vp, vp, vp, i, i, i, i, vp, sz, vp, vp, vp, i, vp, d]),
+        "fdvamd_pframe_batch": (i, [vp] * 11),
+        "fdvamd_pframe_submit": (i, [vp] * 6),
+        "fdvamd_pframe_collect": (i, [vp] * 5),
+        "fdvamd_set_proposer": (i, [vp, vp, i]),
+        "fdvamd_set_proposer_thresholds": (i, [vp, f, f]),
+        "fdvamd_rewind": (i, [vp]),
+    }
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError("fdv_amd/libfdvgpu.so is not built: run __graft_entry__.build()")
         L = C.CDLL(LIB_PATH)
-        vp, ip, dp = C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double)
-        L.fdvgpu_open.restype = vp
-        L.fdvgpu_open.argtypes = [C.c_int, ip, ip, C.c_int]
-        L.fdvgpu_close.argtypes = [vp]
-        L.fdvgpu_last_error.restype = C.c_char_p
-        L.fdvgpu_put_source.argtypes = [vp, C.c_int, C.c_int, vp]
-        L.fdvgpu_put_ref.argtypes = [vp, C.c_int, C.c_int, vp]
-        L.fdvgpu_set_given.argtypes = [vp, C.c_int, vp]
-        L.fdvgpu_pframe.argtypes = [vp, ip, ip, dp, ip, ip, C.c_int]
-        L.fdvgpu_fetch.argtypes = [vp]
-        L.fdvgpu_streams.argtypes = [vp, C.c_int, C.POINTER(C.POINTER(C.c_uint8)), C.POINTER(C.c_size_t)]
-        L.fdvgpu_recon.argtypes = [vp, C.c_int, vp]
-        L.fdvgpu_decisions.argtypes = [vp, C.c_int, vp]
-        L.fdvgpu_timings.argtypes = [vp, C.POINTER(C.c_float)]
-        L.fdvgpu_set_proposer.argtypes = [vp, vp, C.c_int]
-        L.fdvgpu_set_graphs.argtypes = [vp, C.c_int]
-        L.fdvgpu_get_tune.argtypes = [vp, vp]
-        L.fdvgpu_set_tune.argtypes = [vp, vp]
-        L.fdvgpu_autotune.argtypes = [vp, ip, ip, dp, ip, C.c_int, C.c_int]
-        L.fdvgpu_proposals.argtypes = [vp, C.c_int, vp]
-        L.fdvgpu_motion.argtypes = [vp, C.c_int, vp]
-        L.fdvgpu_rewind.argtypes = [vp]
-        L.fdvgpu_planes.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_int),
-                                    C.POINTER(C.c_void_p), C.POINTER(C.c_int)]
-        L.fdvgpu_nn_features.argtypes = [vp, vp, vp, vp, vp, vp, C.c_int, vp, vp]
-        L.fdvgpu_nn_train_step.argtypes = [vp, vp, vp, vp, C.c_int, C.c_float, C.c_float, vp, vp, C.c_int, vp]
-        L.fdvgpu_nn_adam.argtypes = [vp, vp, vp, vp, C.c_int, C.c_float, C.c_int, C.c_int, vp]
+        for name, (res, args) in _signatures().items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
         _lib = L
     return _lib
 
@@ -157,7 +186,6 @@ class Analyser:
     def set_proposer_thresholds(self, conf=0.7, skip=2.0):
         """Confidence threshold of the candidate search and SKIP threshold of the
         pruning (above 1: never; the default)."""
-        self.L.fdvgpu_set_proposer_thresholds.argtypes = [C.c_void_p, C.c_float, C.c_float]
         self._ck(self.L.fdvgpu_set_proposer_thresholds(self.h, float(conf), float(skip)), "set_proposer_thresholds")
 
     def set_overlap(self, on: bool):
@@ -223,8 +251,6 @@ class TabCache:
 def pack_pframe(qp: int, streams, tc: TabCache | None = None) -> bytes:
     """Host entropy packing of one P-frame payload (fdvamd_pack_pframe)."""
     L = lib()
-    L.fdvamd_pack_pframe.restype = C.c_size_t
-    L.fdvamd_pack_pframe.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]
     arrs = [np.frombuffer(s, dtype=np.uint8) if len(s) else np.zeros(1, dtype=np.uint8) for s in streams]
     sv = (C.c_void_p * NS)(*[a.ctypes.data for a in arrs])
     sn = (C.c_size_t * NS)(*[len(s) for s in streams])
@@ -245,18 +271,6 @@ class Encoder:
     def __init__(self, sizes, me_mode: int = ME_DIAMOND):
         self.L = lib()
         L = self.L
-        vp = C.c_void_p
-        L.fdvamd_encoder_open.restype = vp
-        L.fdvamd_encoder_open.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_int]
-        L.fdvamd_encoder_close.argtypes = [vp]
-        L.fdvamd_last_error.restype = C.c_char_p
-        L.fdvamd_set_ref.argtypes = [vp, C.c_int, C.c_int, vp]
-        L.fdvamd_pframe_encode.restype = C.c_size_t
-        L.fdvamd_pframe_encode.argtypes = [vp, vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int,
-                                           vp, C.c_size_t, vp, vp, vp, C.c_int, vp, C.c_double]
-        L.fdvamd_pframe_batch.argtypes = [vp] * 11
-        L.fdvamd_pframe_submit.argtypes = [vp] * 6
-        L.fdvamd_pframe_collect.argtypes = [vp] * 5
         self.sizes = [(int(w), int(h)) for w, h in sizes]
         n = len(self.sizes)
         self.h = L.fdvamd_encoder_open(n, _ints([w for w, _ in self.sizes]),
@@ -282,12 +296,10 @@ class Encoder:
 
     def set_proposer(self, weights=None):
         w = load_proposer() if weights is None else np.ascontiguousarray(weights, dtype=np.float32)
-        self.L.fdvamd_set_proposer.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
         if self.L.fdvamd_set_proposer(self.h, w.ctypes.data, int(w.size)) != 0:
             raise GpuError("fdvamd_set_proposer: " + self.L.fdvamd_last_error().decode())
 
     def set_proposer_thresholds(self, conf=0.7, skip=2.0):
-        self.L.fdvamd_set_proposer_thresholds.argtypes = [C.c_void_p, C.c_float, C.c_float]
         if self.L.fdvamd_set_proposer_thresholds(self.h, float(conf), float(skip)) != 0:
             raise GpuError("fdvamd_set_proposer_thresholds: " + self.L.fdvamd_last_error().decode())
 
@@ -298,12 +310,10 @@ class Encoder:
         ys, cs = w * h, (w // 2) * (h // 2)
         f = np.ascontiguousarray(frame, dtype=np.uint8)
         y, u, v = f[:ys].copy(), f[ys:ys + cs].copy(), f[ys + cs:].copy()
-        self.L.fdvamd_set_ref_planes.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
         if self.L.fdvamd_set_ref_planes(self.h, seq, which, y.ctypes.data, u.ctypes.data, v.ctypes.data) != 0:
             raise GpuError("fdvamd_set_ref_planes: " + self.L.fdvamd_last_error().decode())
 
     def rewind(self):
-        self.L.fdvamd_rewind.argtypes = [C.c_void_p]
         if self.L.fdvamd_rewind(self.h) != 0:
             raise GpuError("fdvamd_rewind: " + self.L.fdvamd_last_error().decode())
 

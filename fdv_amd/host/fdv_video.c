@@ -249,7 +249,9 @@ size_t fdvamd_pframe_encode(fdvamd_encoder *e, const uint8_t *cy, const uint8_t 
                             const uint8_t *cv, int w, int h, int navail, int qp,
                             uint8_t *out, size_t cap, uint8_t *ry, uint8_t *ru, uint8_t *rv,
                             int force_skip, fdvamd_tabcache *tc, double lscale) {
-    if (!e || w != e->w[0] || h != e->h[0]) { amd_seterr("frame size does not match the encoder"); return 0; }
+    if (!e) { amd_seterr("bad arguments"); return 0; }
+    if (e->nseq != 1) { amd_seterr("fdvamd_pframe_encode needs a one-sequence encoder (use fdvamd_pframe_batch)"); return 0; }
+    if (w != e->w[0] || h != e->h[0]) { amd_seterr("frame size does not match the encoder"); return 0; }
     const size_t ys = (size_t)w * h, cs = (size_t)(w / 2) * (h / 2);
     if (!e->tmp && !(e->tmp = malloc(ys + 2 * cs))) { amd_seterr("out of memory"); return 0; }
     memcpy(e->tmp, cy, ys); memcpy(e->tmp + ys, cu, cs); memcpy(e->tmp + ys + cs, cv, cs);

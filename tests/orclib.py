@@ -348,3 +348,20 @@ def metab_to_np(mt):
     dt = _np.dtype([("mv16", "<i2", (2, 2)), ("sad16", "<i4", 2), ("mv8", "<i2", (4, 2)),
                     ("sad8", "<i4", 4), ("flags", "<i4")])
     return _np.frombuffer(bytes(mt), dtype=dt).copy()
+
+
+def ref_pframe_decode(pay, r0, r1, w, h, navail, tc=None):
+    """The reference decoder (pframe_decode, fdv.h:6492) on one P-frame payload:
+    the decoded I420 frame, or None if it rejects the payload."""
+    lib = ref()
+    pay = np.ascontiguousarray(np.frombuffer(bytes(pay), dtype=np.uint8))
+    a = split_i420(r0, w, h)
+    b = split_i420(r1 if r1 is not None else r0, w, h)
+    out = np.zeros(w * h + 2 * (w // 2) * (h // 2), dtype=np.uint8)
+    oy, ou, ov = split_i420(out, w, h)
+    rc = lib.fdvref_pframe_decode(pay.ctypes.data, len(pay), w, h,
+                                  a[0].ctypes.data, a[1].ctypes.data, a[2].ctypes.data,
+                                  b[0].ctypes.data, b[1].ctypes.data, b[2].ctypes.data,
+                                  navail, oy.ctypes.data, ou.ctypes.data, ov.ctypes.data,
+                                  tc.buf.ctypes.data if tc is not None else None)
+    return out if rc == 0 else None

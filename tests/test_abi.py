@@ -63,3 +63,20 @@ def test_pack_of_empty_frame_has_no_device_dependency():
     assert len(out) > 0
     assert G.pack_pframe(20, st) == out
     assert isinstance(np.frombuffer(out, np.uint8), np.ndarray)
+
+
+def _called_entry_points(path):
+    src = open(path).read()
+    return set(re.findall(r"\b(?:L|lib\(\))\.(fdv(?:gpu|amd)_[a-z0-9_]+)\(", src))
+
+
+@pytest.mark.parametrize("path", ["fdv_amd/gpu.py", "fdv_amd/learned/train.py", "bench.py"])
+def test_every_called_entry_point_has_a_signature(path):
+    """A function called through ctypes without argtypes gets its handle
+    narrowed to a C int: every entry point the Python side calls is declared."""
+    lib = G.lib()
+    called = _called_entry_points(os.path.join(ROOT, path))
+    undeclared = [n for n in sorted(called) if getattr(lib, n).argtypes is None]
+    assert not undeclared, f"{path} calls entry points without argtypes: {undeclared}"
+    for n in G._signatures():
+        assert hasattr(lib, n), f"signature table names {n}, which the library does not export"

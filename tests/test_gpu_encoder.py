@@ -9,7 +9,7 @@ import pytest
 import orclib as L
 from fdv_amd import content
 from fdv_amd import gpu as G
-from gpucases import run_parity, say
+from gpucases import mbinfo_to_np, run_parity, say
 
 pytestmark = pytest.mark.gpu
 
@@ -197,3 +197,49 @@ def test_pipelined_submit_collect_matches_sequential(mode):
     assert all(len(p) > 0 for p in b.pframe_collect(tb))
     assert all(len(p) > 0 for p in b.pframe_collect(tb))
     b.close()
+
+
+def test_pframe_encode_rejects_a_batch_encoder():
+    """fdvamd_pframe_encode codes sequence 0 of a one-sequence encoder; with
+    more sequences it fails with an error instead of reading past its
+    one-element argument arrays."""
+    enc = G.Encoder([(64, 48), (64, 48)])
+    enc.set_ref(0, 0, np.zeros(64 * 48 * 3 // 2, np.uint8))
+    with pytest.raises(G.GpuError, match="one-sequence"):
+        enc.pframe_encode(np.zeros(64 * 48 * 3 // 2, np.uint8), 1, 24)
+    enc.close()
+
+
+def test_wavefront_tags_survive_many_block_parallel_frames():
+    """The exact wavefront's handoff words are tagged per frame of that kernel
+    only: a given-decisions frame 255 block-parallel frames after the previous
+    one still waits for this frame's neighbours (a stale tag would let it read
+    the old frame's merge candidates and reconstruction)."""
+    sc = content.scene_objects(64, 64, 77, nobj=4, smin=8, smax=24, vmax=6, grain=1.0)
+    w, h = sc.w, sc.h
+    nfr = 258
+    an = G.Analyser([(w, h)], nsrc=1)
+    an.set_proposer()
+    an.put_ref(0, 0, sc.frame(0))
+    recs = [sc.frame(0)]
+    say(f"  tag test: {nfr - 1} frames, given decisions at 1 and {nfr - 1}")
+    for f in range(1, nfr):
+        cur = sc.frame(f % 8 + 1)
+        navail = 1 if f == 1 else 2
+        given = f in (1, nfr - 1)
+        if given:
+            _, _, info = L.oracle_analyse(cur, recs[-1], recs[-2] if navail == 2 else None, w, h, navail, 24)
+            gv = mbinfo_to_np(info)
+            assert (gv["mode"] == 0).any() or (gv["mode"] == 2).any()
+            an.set_given(0, gv)
+        an.put_source(0, 0, cur)
+        an.pframe([0], [24], navail=[navail], me_mode=G.ME_LEARNED)
+        an.fetch()
+        rec = an.recon(0)
+        if given:
+            st, orec, _ = L.oracle_analyse(cur, recs[-1], recs[-2] if navail == 2 else None, w, h, navail, 24,
+                                           given=info)
+            assert an.streams(0) == st, f"frame {f}: streams"
+            assert np.array_equal(rec, orec), f"frame {f}: reconstruction"
+        recs = recs[-1:] + [rec]
+    an.close()

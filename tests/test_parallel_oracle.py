@@ -221,3 +221,56 @@ def test_standin_blocks_are_the_reference_or_the_inter16_reconstruction():
         _, pc_plane = L.oracle_parallel_decisions(f2, f1, f0, w, h, 2, qp, metab=mt, nb=sb.ravel())
         assert bytes(pc_default) == bytes(pc_plane)
     assert kinds == {"ref", "inter"}
+
+
+need_ref = pytest.mark.skipif(L.ref() is None, reason="reference build not present (oracle/_ref)")
+
+
+def _proposals_from_table(mt, seed):
+    """Confident proposals around the table's vectors with random decoys: the
+    candidate search then mixes hits and misses, as the learned mode does."""
+    rng = np.random.default_rng(seed)
+    m = L.metab_to_np(mt)
+    p = np.zeros(len(m), PROPOSAL)
+    p["conf"] = np.where(rng.random(len(m)) < 0.8, 0.9, 0.3)
+    p["pskip"] = rng.random(len(m))
+    for i in range(len(m)):
+        dx, dy = int(m["mv16"][i][0][0]) >> 2, int(m["mv16"][i][0][1]) >> 2
+        p["cx"][i] = rng.integers(-8, 9, 8) * 2
+        p["cy"][i] = rng.integers(-8, 9, 8) * 2
+        if rng.random() < 0.7:
+            p["cx"][i][0], p["cy"][i][0] = dx - (dx & 1), dy - (dy & 1)
+        p["kx"][i], p["ky"][i] = rng.integers(-8, 9, 4) * 2, rng.integers(-8, 9, 4) * 2
+        p["qx"][i], p["qy"][i] = rng.integers(-8, 9, (4, 2)) * 2, rng.integers(-8, 9, (4, 2)) * 2
+    return p
+
+
+@need_ref
+@pytest.mark.parametrize("table", ["exhaustive", "candidates", "candidates+skip"])
+def test_block_parallel_payloads_decode_with_the_reference_decoder(table):
+    """The parallel and learned modes only change the encoder's choices: their
+    payloads, packed with the carried entropy state, decode with the reference
+    decoder (fdv.h:6492) to exactly the encoder's reconstruction, over a chain
+    of P-frames whose later frames reference both earlier ones."""
+    scenes = [content.scene_objects(96, 64, 91, nobj=4, smin=8, smax=32, vmax=10, grain=2.0),
+              content.scene_blobs(128, 64, 92, nblob=40, vmax=14, grain=3.0, cut=3)]
+    sthr = 0.6 if table == "candidates+skip" else 2.0
+    for sc in scenes:
+        w, h = sc.w, sc.h
+        frames = [sc.frame(i) for i in range(5)]
+        tce, tcd = L.TabCache(), L.TabCache()
+        near, far = frames[0], None
+        for f in range(1, 5):
+            navail = 1 if f == 1 else 2
+            qp = (22, 30, 38, 26)[f - 1]
+            mt = L.oracle_metab_exhaustive(frames[f], near, far, w, h, navail)
+            if table != "exhaustive":
+                mt = L.oracle_metab_candidates(frames[f], near, far, w, h, navail,
+                                               _proposals_from_table(mt, f), 0.7, sthr)
+            st, rec, info, _, _ = L.oracle_parallel(frames[f], near, far, w, h, navail, qp, metab=mt)
+            pay = L.oracle_pack(qp, st, tce)
+            dec = L.ref_pframe_decode(pay, near, far, w, h, navail, tcd)
+            assert dec is not None, f"{w}x{h} frame {f}: the reference decoder rejected the payload"
+            assert np.array_equal(dec, rec), f"{w}x{h} frame {f}: decoded frame differs from the encoder's recon"
+            assert np.array_equal(tce.buf, tcd.buf), f"{w}x{h} frame {f}: entropy state"
+            far, near = near, rec
