@@ -58,6 +58,13 @@ static void seterr(const char *fmt, ...) {
     va_end(ap);
 }
 extern "C" const char *fdvgpu_last_error(void) { return g_err; }
+// The same message slot, for the library's other source files.
+extern "C" void fdvg_seterr(const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+}
 
 #define CK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { \
     seterr("%s: %s (%s:%d)", #x, hipGetErrorString(e_), __FILE__, __LINE__); return -1; } } while (0)

@@ -60,6 +60,7 @@ def _signatures():
         "fdvgpu_nn_train_step": (i, [vp, vp, vp, vp, i, f, f, vp, vp, i, vp]),
         "fdvgpu_nn_adam": (i, [vp, vp, vp, vp, i, f, i, i, vp]),
         "fdvgpu_timings": (i, [vp, C.POINTER(C.c_float)]),
+        "fdvgpu_motion_search": (i, [vp, vp, vp, i, i, i, i, i, vp, f, vp, vp]),
         # include/fdv_amd.h
         "fdvamd_pack_pframe": (sz, [i, vp, vp, vp, vp, sz]),
         "fdvamd_encoder_open": (vp, [i, vp, vp, i]),
@@ -394,6 +395,16 @@ class Encoder:
         if rc != 0:
             raise GpuError("fdvamd_pframe_collect: " + self.L.fdvamd_last_error().decode())
         return [outs[s][:lens[s]].tobytes() for s in range(n)]
+
+
+def motion_search(cur_y: int, ref0_y: int, ref1_y: int, ref_stride: int, w: int, h: int, navail: int, qp: int,
+                  table: int, weights: int = 0, conf: float = 0.7, stream: int = 0):
+    """fdvgpu_motion_search on device pointers (ints): the motion table of a
+    whole frame into `table` (MOTION_DTYPE entries, raster order)."""
+    rc = lib().fdvgpu_motion_search(cur_y, ref0_y, ref1_y or None, ref_stride, w, h, navail, qp, weights or None,
+                                    float(conf), table, stream or None)
+    if rc != 0:
+        raise GpuError(lib().fdvgpu_last_error().decode())
 
 
 def nn_nparam() -> int:

@@ -179,6 +179,28 @@ int fdvgpu_set_proposer_thresholds(fdvgpu *g, float conf, float skip);
 int fdvgpu_proposals(fdvgpu *g, int seq, fdvgpu_proposal *out);
 int fdvgpu_motion(fdvgpu *g, int seq, fdvgpu_motion_entry *out);
 
+/* Motion search of one frame on caller-owned device buffers, outside any
+ * context: the motion table of every MB in one pass.  It replaces a loop of
+ * fdv_me_search calls (fdv.h:795-797, body 3375-3466) over a frame's blocks
+ * with the block-parallel search of this library's table modes:
+ *   - weights NULL: the exhaustive +-16 pel search of every 16x16 block on each
+ *     reference and of every 8x8 quadrant on the first, then the reference's
+ *     sub-pel cross (FDVGPU_ME_EXHAUSTIVE's table);
+ *   - weights (fdvgpu_nn_nparam() floats, device memory): the learned
+ *     proposer's candidates where its confidence is at least `conf`, the
+ *     exhaustive search elsewhere (FDVGPU_ME_LEARNED's table; flags stay 0).
+ * cur_y: tight luma, stride w.  ref0_y / ref1_y: luma planes in the reference's
+ * padded layout (fdv.h:1548: pointer at sample (0,0), 32 replicated samples on
+ * every side, stride ref_stride >= w + 64, a multiple of 4); ref1_y is read
+ * only when navail is 2.  qp: the frame's quantiser (the proposer's input
+ * scale).  table: nmb entries of device memory, raster order.  Launched on
+ * `stream` (a hipStream_t, or NULL); returns once the search is done.  Its
+ * scratch (the frame descriptor and, with weights, the proposals) is
+ * allocated for the call. */
+int fdvgpu_motion_search(const uint8_t *cur_y, const uint8_t *ref0_y, const uint8_t *ref1_y, int ref_stride,
+                         int w, int h, int navail, int qp, const float *weights, float conf,
+                         fdvgpu_motion_entry *table, void *stream);
+
 /* Device pointers of a sequence's source slot luma (tight) and of the nearest
  * reference's padded luma (at its sample (0,0)), for feature extraction. */
 int fdvgpu_planes(fdvgpu *g, int seq, int src_slot, const uint8_t **cur_y, int *cur_stride,
