@@ -23,7 +23,7 @@ HOST = os.path.join(PKG, "host")
 OBJ = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libfdvgpu.so")
 
-HIP_SOURCES = ["fdvgpu_analyse.hip", "fdvgpu_post.hip", "fdvgpu_metab.hip", "fdvgpu_nn.hip", "fdvgpu_motion.hip", "fdvgpu_api.hip"]
+HIP_SOURCES = ["fdvgpu_analyse.hip", "fdvgpu_post.hip", "fdvgpu_metab.hip", "fdvgpu_nn.hip", "fdvgpu_motion.hip", "fdvgpu_noalloc.hip", "fdvgpu_api.hip"]
 C_SOURCES = ["fdv_pack.c", "fdv_video.c"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

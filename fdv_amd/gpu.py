@@ -61,6 +61,7 @@ def _signatures():
         "fdvgpu_nn_adam": (i, [vp, vp, vp, vp, i, f, i, i, vp]),
         "fdvgpu_timings": (i, [vp, C.POINTER(C.c_float)]),
         "fdvgpu_motion_search": (i, [vp, vp, vp, i, i, i, i, i, vp, f, vp, vp]),
+        "fdvgpu_motion_search_noalloc": (i, [vp, vp, vp, i, i, i, i, i, vp, f, vp, vp]),
         # include/fdv_amd.h
         "fdvamd_pack_pframe": (sz, [i, vp, vp, vp, vp, sz]),
         "fdvamd_encoder_open": (vp, [i, vp, vp, i]),
@@ -398,10 +399,12 @@ class Encoder:
 
 
 def motion_search(cur_y: int, ref0_y: int, ref1_y: int, ref_stride: int, w: int, h: int, navail: int, qp: int,
-                  table: int, weights: int = 0, conf: float = 0.7, stream: int = 0):
-    """fdvgpu_motion_search on device pointers (ints): the motion table of a
-    whole frame into `table` (MOTION_DTYPE entries, raster order)."""
-    rc = lib().fdvgpu_motion_search(cur_y, ref0_y, ref1_y or None, ref_stride, w, h, navail, qp, weights or None,
+                  table: int, weights: int = 0, conf: float = 0.7, stream: int = 0, noalloc: bool = False):
+    """fdvgpu_motion_search (or, with `noalloc`, fdvgpu_motion_search_noalloc)
+    on device pointers (ints): the motion table of a whole frame into `table`
+    (MOTION_DTYPE entries, raster order)."""
+    fn = lib().fdvgpu_motion_search_noalloc if noalloc else lib().fdvgpu_motion_search
+    rc = fn(cur_y, ref0_y, ref1_y or None, ref_stride, w, h, navail, qp, weights or None,
                                     float(conf), table, stream or None)
     if rc != 0:
         raise GpuError(lib().fdvgpu_last_error().decode())

@@ -201,6 +201,19 @@ int fdvgpu_motion_search(const uint8_t *cur_y, const uint8_t *ref0_y, const uint
                          int w, int h, int navail, int qp, const float *weights, float conf,
                          fdvgpu_motion_entry *table, void *stream);
 
+/* fdvgpu_motion_search without any allocation.  It has no counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The same
+ * arguments and result; frames of at most FDVGPU_NOALLOC_MAX_MB macroblocks
+ * (larger ones fail with an error).  Its scratch is a fixed device array that
+ * each device holds for the life of the process and that every call on that
+ * device shares: calls that overlap on one device race.  Asynchronous on
+ * `stream`: the table is ready once the stream reaches this point. */
+#define FDVGPU_NOALLOC_MAX_MB 34816      /* 4096 x 2176 */
+int fdvgpu_motion_search_noalloc(const uint8_t *cur_y, const uint8_t *ref0_y, const uint8_t *ref1_y, int ref_stride,
+                                 int w, int h, int navail, int qp, const float *weights, float conf,
+                                 fdvgpu_motion_entry *table, void *stream);
+
 /* Device pointers of a sequence's source slot luma (tight) and of the nearest
  * reference's padded luma (at its sample (0,0)), for feature extraction. */
 int fdvgpu_planes(fdvgpu *g, int seq, int src_slot, const uint8_t **cur_y, int *cur_stride,
