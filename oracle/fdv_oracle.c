@@ -744,7 +744,7 @@ static int merge_list(const mvrec_t *map, int mw, int mx, int my, int pmx, int p
 
 /* Test hook: with a nonzero value orc_pframe_analyse returns the reconstruction
  * before the deblocking pass (the planes the MB loop's intra neighbours read). */
-static int orc_no_deblock = 0;
+static _Thread_local int orc_no_deblock = 0;
 void orc_set_no_deblock(int on) { orc_no_deblock = on; }
 
 int orc_pframe_analyse(const uint8_t *cy, const uint8_t *cu, const uint8_t *cv,
@@ -1453,6 +1453,32 @@ void orc_standin_plane(const uint8_t *cy, int w, int h, const uint8_t *const RY[
         }
 }
 
+/* INTRA 16x16 cost of the MB at (bx, by) with its neighbours read from the
+ * tight plane `nb`: the SATD screen over the four modes (strict <, first
+ * wins), then one residual coding of the best (fdv.h:5950-5988). */
+static void intra_cost(const uint8_t *cy, const uint8_t *nb, int w, int bx, int by, int qp, double lambda,
+                       orc_pcost *P) {
+    uint8_t ntop[16], nleft[16], ntl, pI[256], rn[256]; int nht, nhl;
+    neighbours(nb, w, bx, by, 16, 0, ntop, nleft, &ntl, &nht, &nhl);
+    int sel = -1, sbest = 0;
+    for (int m = 0; m < 4; ++m) {
+        if (!nn_ok(m, 16, nht, nhl)) continue;
+        orc_intra_nxn(m, ntop, nleft, ntl, 16, nht, nhl, pI);
+        int sa = 0;
+        for (int i = 0; i < 16; i += 4)
+            for (int j = 0; j < 16; j += 4) {
+                uint8_t pk[16];
+                for (int k = 0; k < 4; ++k) memcpy(pk + 4 * k, pI + (i + k) * 16 + j, 4);
+                sa += orc_satd4(cy + (by + i) * w + bx + j, w, pk);
+            }
+        if (sel < 0 || sa < sbest) { sbest = sa; sel = m; }
+    }
+    orc_intra_nxn(sel, ntop, nleft, ntl, 16, nht, nhl, pI);
+    coded_t cn; double dn = 0.0; int bn = 6;
+    residual(cy, w, bx, by, pI, 16, qp, lambda, rn, &cn, &dn, &bn);
+    P->dn = (int32_t)dn; P->bn = bn; P->nsub = (uint8_t)sel;
+}
+
 /* The block-parallel costs with the intra neighbours read from plane `nb` (tight,
  * stride w), or, with nb == NULL, from the stand-in reconstruction above (what
  * the product does); the parity tests also pass the exact path's pre-deblock
@@ -1486,27 +1512,7 @@ void orc_pframe_costs_par_nb(const uint8_t *cy, const uint8_t *nb, int w, int h,
                 P->d16[r] = (int32_t)dd; P->b16[r] = bb;
             }
             if (T->flags & 1) continue;                 /* SKIP expected: no INTRA, no INTER8 costs */
-            {
-                uint8_t ntop[16], nleft[16], ntl, pI[256], rn[256]; int nht, nhl;
-                neighbours(nb, w, bx, by, 16, 0, ntop, nleft, &ntl, &nht, &nhl);
-                int sel = -1, sbest = 0;
-                for (int m = 0; m < 4; ++m) {
-                    if (!nn_ok(m, 16, nht, nhl)) continue;
-                    orc_intra_nxn(m, ntop, nleft, ntl, 16, nht, nhl, pI);
-                    int sa = 0;
-                    for (int i = 0; i < 16; i += 4)
-                        for (int j = 0; j < 16; j += 4) {
-                            uint8_t pk[16];
-                            for (int k = 0; k < 4; ++k) memcpy(pk + 4 * k, pI + (i + k) * 16 + j, 4);
-                            sa += orc_satd4(cy + (by + i) * w + bx + j, w, pk);
-                        }
-                    if (sel < 0 || sa < sbest) { sbest = sa; sel = m; }
-                }
-                orc_intra_nxn(sel, ntop, nleft, ntl, 16, nht, nhl, pI);
-                coded_t cn; double dn = 0.0; int bn = 6;
-                residual(cy, w, bx, by, pI, 16, qp, lambda, rn, &cn, &dn, &bn);
-                P->dn = (int32_t)dn; P->bn = bn; P->nsub = (uint8_t)sel;
-            }
+            intra_cost(cy, nb, w, bx, by, qp, lambda, P);
             {
                 /* never read where the INTER8 gate closes for either table SAD */
                 const int ssad = T->sad8[0] + T->sad8[1] + T->sad8[2] + T->sad8[3];
@@ -1526,9 +1532,30 @@ void orc_pframe_costs_par_nb(const uint8_t *cy, const uint8_t *nb, int w, int h,
     free(own);
 }
 
-void orc_pframe_repair_par(const uint8_t *cy, int w, int h, const uint8_t *const RY[2], int ys,
-                           int navail, double lambda, const orc_metab *mt, const orc_pcost *pc,
-                           orc_mbinfo *given) {
+/* The replay; consulted[i] (may be NULL) = 1 where MB i's decision read its
+ * INTRA cost (fdv.h:5950: SKIP not winning outright, not pruned, the best cost
+ * so far above the INTRA floor). */
+/* What the replay's decision of one MB weighed, so that a new INTRA cost can be
+ * judged without replaying: the SKIP cost, the best INTER16 cost, the best cost
+ * before INTRA, the INTER8 cost where its SAD gate opens (else 1e30), and the
+ * mode chosen. */
+typedef struct { double js, ji, jpre, j8; int mode; } dstate_t;
+
+static int decide4(const dstate_t *d, double jn, double lambda) {
+    const int FL_INTER8 = 3 + 4 * 2 + 4 * 2;
+    double jb = d->jpre;
+    if (jn < jb) jb = jn;
+    const double j8 = jb > lambda * FL_INTER8 ? d->j8 : 1e30;
+    int mode = 0; double jm = d->js;
+    if (d->ji < jm) { jm = d->ji; mode = 1; }
+    if (jn < jm) { jm = jn; mode = 2; }
+    if (j8 < jm) { jm = j8; mode = 3; }
+    return mode;
+}
+
+static void repair_core(const uint8_t *cy, int w, int h, const uint8_t *const RY[2], int ys,
+                        int navail, double lambda, const orc_metab *mt, const orc_pcost *pc,
+                        orc_mbinfo *given, uint8_t *consulted, dstate_t *dst) {
     const int mw = w / 16, mh = h / 16;
     const int FL_INTRA = 2 * 3 + 4 * 2, FL_INTER16 = 2 * 3 + 2 + 4 * 2, FL_INTER8 = 3 + 4 * 2 + 4 * 2;
     mvrec_t *mvm = calloc((size_t)mw * mh, sizeof *mvm);
@@ -1562,12 +1589,15 @@ void orc_pframe_repair_par(const uint8_t *cy, int w, int h, const uint8_t *const
                 if (J < Ji) { Ji = J; iref = r; sad16 = T->sad16[r]; }
             }
             double Jn = 1e30;
-            if (!skip_wins && !pruned && Jbest > lambda * FL_INTRA) {
+            const double Jpre = Jbest;
+            const int want_n = !skip_wins && !pruned && Jbest > lambda * FL_INTRA;
+            if (consulted) consulted[my * mw + mx] = (uint8_t)want_n;
+            if (want_n) {
                 Jn = (double)P->dn + lambda * P->bn;
                 if (Jn < Jbest) Jbest = Jn;
             }
-            double J8 = 1e30;
-            if (!skip_wins && !pruned && Jbest > lambda * FL_INTER8) {
+            double J8 = 1e30, J8v = 1e30;
+            if (!skip_wins && !pruned) {
                 const int ssad = T->sad8[0] + T->sad8[1] + T->sad8[2] + T->sad8[3];
                 if (!(sad16 > 0 && ssad * 10 >= sad16 * 9)) {
                     int b8 = P->b8, qpx = pmx, qpy = pmy;
@@ -1575,13 +1605,15 @@ void orc_pframe_repair_par(const uint8_t *cy, int w, int h, const uint8_t *const
                         b8 += orc_bits_val(zmap(T->mv8[q][0] - qpx)) + orc_bits_val(zmap(T->mv8[q][1] - qpy));
                         qpx = T->mv8[q][0]; qpy = T->mv8[q][1];
                     }
-                    J8 = (double)P->d8 + lambda * b8;
+                    J8v = (double)P->d8 + lambda * b8;
                 }
+                if (Jbest > lambda * FL_INTER8) J8 = J8v;
             }
             int mode = 0; double Jm = Js;
             if (Ji < Jm) { Jm = Ji; mode = 1; }
             if (Jn < Jm) { Jm = Jn; mode = 2; }
             if (J8 < Jm) { Jm = J8; mode = 3; }
+            if (dst) dst[my * mw + mx] = (dstate_t){Js, Ji, Jpre, J8v, mode};
             orc_mbinfo *G = &given[my * mw + mx];
             memset(G, 0, sizeof *G);
             G->mode = (uint8_t)mode;
@@ -1604,4 +1636,72 @@ void orc_pframe_repair_par(const uint8_t *cy, int w, int h, const uint8_t *const
         }
     }
     free(mvm);
+}
+
+void orc_pframe_repair_par(const uint8_t *cy, int w, int h, const uint8_t *const RY[2], int ys,
+                           int navail, double lambda, const orc_metab *mt, const orc_pcost *pc,
+                           orc_mbinfo *given) {
+    repair_core(cy, w, h, RY, ys, navail, lambda, mt, pc, given, NULL, NULL);
+}
+
+/* The 33 luma samples an MB's INTRA cost reads (neighbours(), n = 16): the top
+ * row, the left column, the corner. */
+static void nb_edges(const uint8_t *rec, int w, int bx, int by, uint8_t e[33]) {
+    int ht, hl;
+    neighbours(rec, w, bx, by, 16, 0, e, e + 16, e + 32, &ht, &hl);
+}
+
+int orc_pframe_par_exact(const uint8_t *cy, const uint8_t *cu, const uint8_t *cv, int w, int h,
+                         const uint8_t *const RY[2], const uint8_t *const RU[2], const uint8_t *const RV[2],
+                         int ys, int cs, int navail, int qp, double lambda, const orc_metab *mt, int npass,
+                         orc_mbinfo *given, orc_pcost *pc, int *converged) {
+    const int mw = w / 16, mh = h / 16, nmb = mw * mh, cw = w / 2, ch = h / 2;
+    const size_t scap = (size_t)nmb * 1600 + 1024;
+    uint8_t *sb = malloc((size_t)w * h), *used = malloc((size_t)nmb * 33), *cons = calloc((size_t)nmb, 1);
+    uint8_t *rec = malloc((size_t)w * h + 2 * (size_t)cw * ch), *sbuf = malloc(ORC_NS * scap);
+    dstate_t *dstate = malloc((size_t)nmb * sizeof *dstate);
+    int rc = -1, changed = 0;
+    *converged = 0;
+    if (!sb || !used || !cons || !rec || !sbuf || !dstate) goto done;
+    {
+        uint8_t *bufs[ORC_NS];
+        size_t sn[ORC_NS];
+        for (int k = 0; k < ORC_NS; ++k) bufs[k] = sbuf + (size_t)k * scap;
+        /* pass 0: intra costs from the stand-in, the replay */
+        orc_standin_plane(cy, w, h, RY, ys, navail, qp, lambda, mt, sb);
+        orc_pframe_costs_par_nb(cy, sb, w, h, RY, ys, navail, qp, lambda, mt, pc);
+        for (int i = 0; i < nmb; ++i) nb_edges(sb, w, (i % mw) * 16, (i / mw) * 16, used + (size_t)i * 33);
+        repair_core(cy, w, h, RY, ys, navail, lambda, mt, pc, given, cons, dstate);
+        /* repair passes: code the decisions (no deblock), re-cost every consulted
+         * INTRA whose neighbours' reconstruction is not what its cost read, replay */
+        for (int pass = 1; pass <= npass; ++pass) {
+            const int saved = orc_no_deblock;
+            orc_no_deblock = 1;
+            const int arc = orc_pframe_analyse(cy, cu, cv, w, h, RY, RU, RV, ys, cs, navail, qp, lambda, 0, 0, given,
+                                               rec, rec + (size_t)w * h, rec + (size_t)w * h + (size_t)cw * ch,
+                                               NULL, bufs, scap, sn);
+            orc_no_deblock = saved;
+            if (arc != 0) goto done;
+            int dirty = 0;
+            for (int i = 0; i < nmb; ++i) {
+                if (!cons[i]) continue;
+                const int bx = (i % mw) * 16, by = (i / mw) * 16;
+                uint8_t e[33];
+                nb_edges(rec, w, bx, by, e);
+                if (!memcmp(e, used + (size_t)i * 33, 33)) continue;
+                intra_cost(cy, rec, w, bx, by, qp, lambda, &pc[i]);
+                memcpy(used + (size_t)i * 33, e, 33);
+                /* with every earlier decision unchanged, this MB's inputs are the
+                 * ones the replay weighed: only a new mode moves anything */
+                if (decide4(&dstate[i], (double)pc[i].dn + lambda * pc[i].bn, lambda) != dstate[i].mode) dirty = 1;
+            }
+            if (!dirty) { *converged = 1; break; }
+            ++changed;
+            repair_core(cy, w, h, RY, ys, navail, lambda, mt, pc, given, cons, dstate);
+        }
+        rc = changed;
+    }
+done:
+    free(sb); free(used); free(cons); free(rec); free(sbuf); free(dstate);
+    return rc;
 }

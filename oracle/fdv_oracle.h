@@ -171,4 +171,20 @@ void orc_pframe_repair_par(const uint8_t *cy, int w, int h, const uint8_t *const
                            int navail, double lambda, const orc_metab *mt, const orc_pcost *pc,
                            orc_mbinfo *given);
 
+
+/* The block-parallel decisions made exact by repair passes: the decisions of
+ * orc_pframe_repair_par on costs whose INTRA entries read the stand-in plane;
+ * then, up to npass times: code them (orc_pframe_analyse, before deblocking),
+ * re-cost every INTRA the replay consulted whose 33 neighbour samples in that
+ * reconstruction differ from the ones its cost read, and replay again.  When a
+ * pass finds nothing to re-cost the decisions are a fixed point, and the only
+ * fixed point is the MB wavefront's own result (each MB's inputs come from MBs
+ * before it in raster order): *converged = 1.  Outputs the decisions to code
+ * (`given`) and the costs; returns the number of passes that re-costed
+ * something, or -1 on a stream overflow. */
+int orc_pframe_par_exact(const uint8_t *cy, const uint8_t *cu, const uint8_t *cv, int w, int h,
+                         const uint8_t *const RY[2], const uint8_t *const RU[2], const uint8_t *const RV[2],
+                         int ys, int cs, int navail, int qp, double lambda, const orc_metab *mt, int npass,
+                         orc_mbinfo *given, orc_pcost *pc, int *converged);
+
 #endif

@@ -365,3 +365,31 @@ def ref_pframe_decode(pay, r0, r1, w, h, navail, tc=None):
                                   navail, oy.ctypes.data, ou.ctypes.data, ov.ctypes.data,
                                   tc.buf.ctypes.data if tc is not None else None)
     return out if rc == 0 else None
+
+
+def oracle_parallel_exact(cur, r0, r1, w, h, navail, qp, lscale=1.0, metab=None, npass=3):
+    """The block-parallel analysis with repair passes (orc_pframe_par_exact):
+    (streams, recon, info, decisions given to the coder, costs, passes that
+    re-costed something, converged)."""
+    lib = oracle()
+    keep, ptrs, ys, cs = _padded_refs(r0, r1, w, h)
+    RY = (C.c_void_p * 2)(*ptrs[0])
+    RU = (C.c_void_p * 2)(*ptrs[1])
+    RV = (C.c_void_p * 2)(*ptrs[2])
+    nmb = (w // 16) * (h // 16)
+    mt = metab if metab is not None else oracle_metab_exhaustive(cur, r0, r1, w, h, navail)
+    pc = (PCost * nmb)()
+    given = (MbInfo * nmb)()
+    cy, cu, cv = split_i420(cur, w, h)
+    conv = C.c_int()
+    lam = lib.orc_lambda(qp, lscale)
+    lib.orc_pframe_par_exact.restype = C.c_int
+    lib.orc_pframe_par_exact.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p,
+                                         C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
+                                         C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+    passes = lib.orc_pframe_par_exact(cy.ctypes.data, cu.ctypes.data, cv.ctypes.data, w, h, C.addressof(RY),
+                                      C.addressof(RU), C.addressof(RV), ys, cs, navail, qp, lam, C.addressof(mt),
+                                      int(npass), C.addressof(given), C.addressof(pc), C.byref(conv))
+    assert passes >= 0, "oracle stream overflow"
+    st, rec, info = oracle_analyse(cur, r0, r1, w, h, navail, qp, lscale, 0, 0, given=given)
+    return st, rec, info, given, pc, passes, bool(conv.value)

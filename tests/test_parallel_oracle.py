@@ -274,3 +274,26 @@ def test_block_parallel_payloads_decode_with_the_reference_decoder(table):
             assert np.array_equal(dec, rec), f"{w}x{h} frame {f}: decoded frame differs from the encoder's recon"
             assert np.array_equal(tce.buf, tcd.buf), f"{w}x{h} frame {f}: entropy state"
             far, near = near, rec
+
+
+@pytest.mark.parametrize("qp", [20, 30])
+def test_repair_passes_reach_the_wavefront_decisions(qp):
+    """Re-costing every consulted INTRA from the reconstruction of the current
+    decisions and replaying (orc_pframe_par_exact) stops at a fixed point, and
+    that fixed point is the exact wavefront's result: the same streams and
+    reconstruction as the MB wavefront with the exhaustive search."""
+    sc = content.scene_blobs(128, 64, 93, nblob=30, vmax=12, grain=3.0, cut=9)
+    w, h = sc.w, sc.h
+    frames = [sc.frame(i) for i in range(3)]
+    near, far = frames[0], None
+    repaired = 0
+    for f in (1, 2):
+        navail = 1 if f == 1 else 2
+        st, rec, info, given, pc, passes, conv = L.oracle_parallel_exact(frames[f], near, far, w, h, navail, qp,
+                                                                         npass=32)
+        assert conv, f"frame {f}: no fixed point within 32 passes"
+        ost, orec, _ = L.oracle_analyse(frames[f], near, far, w, h, navail, qp, me_mode=1)
+        assert st == ost and np.array_equal(rec, orec), f"frame {f}: fixed point differs from the wavefront"
+        repaired += passes
+        far, near = near, orec
+    assert repaired > 0, "the stand-in costs were already exact: nothing was repaired"
