@@ -76,6 +76,11 @@ def _signatures():
         "fdvamd_set_proposer": (i, [vp, vp, i]),
         "fdvamd_set_proposer_thresholds": (i, [vp, f, f]),
         "fdvamd_rewind": (i, [vp]),
+        "fdvamd_set_iframe_coder": (i, [vp, vp]),
+        "fdvamd_set_sequence_mode": (i, [i]),
+        "fdvamd_set_sequence_proposer": (i, [vp, i]),
+        "fdvamd_video_encode": (sz, [vp, i, i, i, i, i, vp, sz]),
+        "fdvamd_vtile_encode": (sz, [vp, i, i, i, i, i, i, i, vp, sz]),
     }
 
 
@@ -408,6 +413,52 @@ def motion_search(cur_y: int, ref0_y: int, ref1_y: int, ref_stride: int, w: int,
                                     float(conf), table, stream or None)
     if rc != 0:
         raise GpuError(lib().fdvgpu_last_error().decode())
+
+
+def set_iframe_coder(encode_fn: int, decode_fn: int = 0):
+    """Register the key-frame coder of the sequence entry points: C function
+    pointers (ints) with the fdvamd_iframe_encode_fn / _decode_fn contracts."""
+    lib().fdvamd_set_iframe_coder(encode_fn or None, decode_fn or None)
+
+
+def set_sequence_mode(me_mode: int, weights=None):
+    """Search mode of video_encode / vtile_encode (and, for the learned mode,
+    the proposer's weights: default the shipped ones)."""
+    L = lib()
+    if me_mode == ME_LEARNED:
+        w = load_proposer() if weights is None else np.ascontiguousarray(weights, dtype=np.float32)
+        if L.fdvamd_set_sequence_proposer(w.ctypes.data, int(w.size)) != 0:
+            raise GpuError(L.fdvamd_last_error().decode())
+    if L.fdvamd_set_sequence_mode(int(me_mode)) != 0:
+        raise GpuError(L.fdvamd_last_error().decode())
+
+
+def _frame_ptrs(frames):
+    fr = [np.ascontiguousarray(f, dtype=np.uint8) for f in frames]
+    return fr, (C.c_void_p * len(fr))(*[f.ctypes.data for f in fr])
+
+
+def video_encode(frames, w: int, h: int, qp: int, keyint: int) -> bytes:
+    """fdvamd_video_encode: the reference's fdv_video_encode container."""
+    fr, fp = _frame_ptrs(frames)
+    cap = len(fr) * (w * h * 3 + 8192) + 64
+    out = np.empty(cap, np.uint8)
+    n = lib().fdvamd_video_encode(C.addressof(fp), len(fr), w, h, qp, keyint, out.ctypes.data, cap)
+    if n == 0:
+        raise GpuError("fdvamd_video_encode: " + lib().fdvamd_last_error().decode())
+    return out[:n].tobytes()
+
+
+def vtile_encode(frames, w: int, h: int, qp: int, keyint: int, band_mbrows: int, nthreads: int = 8) -> bytes:
+    """fdvamd_vtile_encode: the reference's fdv_vtile_encode container."""
+    fr, fp = _frame_ptrs(frames)
+    cap = len(fr) * (w * h * 3 + 8192) + 65536
+    out = np.empty(cap, np.uint8)
+    n = lib().fdvamd_vtile_encode(C.addressof(fp), len(fr), w, h, qp, keyint, band_mbrows, nthreads,
+                                  out.ctypes.data, cap)
+    if n == 0:
+        raise GpuError("fdvamd_vtile_encode: " + lib().fdvamd_last_error().decode())
+    return out[:n].tobytes()
 
 
 def nn_nparam() -> int:

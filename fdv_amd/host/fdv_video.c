@@ -22,6 +22,13 @@ static void amd_seterr(const char *fmt, ...) {
     va_end(ap);
 }
 const char *fdvamd_last_error(void) { return amd_err; }
+/* The same message slot, for the library's other host files. */
+void fdvamd_seterr_(const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(amd_err, sizeof amd_err, fmt, ap);
+    va_end(ap);
+}
 
 typedef struct {
     fdvamd_encoder *e;

@@ -113,6 +113,50 @@ int fdvamd_set_proposer_thresholds(fdvamd_encoder *e, float conf, float skip);
  * frame to undo. */
 int fdvamd_rewind(fdvamd_encoder *e);
 
+/* ---- sequences, with the reference's signatures and containers ---------- */
+
+/* A key-frame coder: codes the tight planes y (w x h), u, v (w/2 x h/2) at qp
+ * into out (at most cap bytes) and leaves its reconstruction in ry, ru, rv;
+ * returns the blob length, or 0 on failure -- the contract of the reference's
+ * iframe_encode (fdv.h:5487) over a whole frame.  The decoder's counterpart
+ * (iframe_decode, fdv.h:5541) returns 0, or -1 on a malformed blob.  The intra
+ * quadtree (fdv.h:3637-5154) has no device implementation in this library, so
+ * the sequence entry points take key frames from coders registered here
+ * (process-wide; both may be NULL to unregister).  No counterpart in the
+ * reference, whose sequence encoders call their own. */
+typedef size_t (*fdvamd_iframe_encode_fn)(const uint8_t *y, const uint8_t *u, const uint8_t *v, int w, int h, int qp,
+                                          uint8_t *out, size_t cap, uint8_t *ry, uint8_t *ru, uint8_t *rv);
+typedef int (*fdvamd_iframe_decode_fn)(const uint8_t *in, size_t len, uint8_t *y, uint8_t *u, uint8_t *v, int w,
+                                       int h);
+int fdvamd_set_iframe_coder(fdvamd_iframe_encode_fn enc, fdvamd_iframe_decode_fn dec);
+
+/* The search / decision mode of the sequence entry points below (process-wide,
+ * as fdvamd_encoder_open's me_mode; default 0, the reference's diamond search:
+ * their output is then byte-identical to the reference's).  No counterpart in
+ * the reference. */
+int fdvamd_set_sequence_mode(int me_mode);
+/* The learned proposer's weights for mode 2 of the sequence entry points
+ * (copied; NULL clears them).  No counterpart in the reference. */
+int fdvamd_set_sequence_proposer(const float *weights, int n);
+
+/* fdv_video_encode (fdv.h:903-904, body 6741-6854), same signature and output:
+ * the 8-byte sequence header (frame count, width, height, qp, keyint clamped
+ * to 0..255), then per frame a u32 length and the frame's blob; frame 0 and
+ * every keyint-th frame are key frames (the registered coder), the others
+ * P-frames coded on the device against a two-frame reference pool held there.
+ * Returns the stream length, or 0 on error (fdvamd_last_error() says why). */
+size_t fdvamd_video_encode(const uint8_t *const *frames, int nframes, int w, int h, int qp, int keyint,
+                           uint8_t *out, size_t cap);
+
+/* fdv_vtile_encode (fdv.h:973-975, body 7161-7243), same signature and output:
+ * horizontal bands of band_mbrows MB rows, each an independent
+ * fdvamd_video_encode stream (with the reference's per-band capacity of two
+ * band frames + 64 KiB), behind the 16-byte tiled header and per-band u32
+ * lengths in band order.  Every band's P-frame is coded in one device pass;
+ * nthreads bounds the host threads of the key frames. */
+size_t fdvamd_vtile_encode(const uint8_t *const *frames, int nframes, int w, int h, int qp, int keyint,
+                           int band_mbrows, int nthreads, uint8_t *out, size_t cap);
+
 #ifdef __cplusplus
 }
 #endif

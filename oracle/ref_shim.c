@@ -12,6 +12,7 @@
  *   pframe_encode      5764    pframe_decode     6492
  *   code_residual      5417    chroma_block      5631
  *   load_ref           5183    c8_init / C8 / QSTEP8 2585-2612
+ *   iframe_encode      5487    iframe_decode     5541
  */
 #define FDV_IMPLEMENTATION
 #include "fdv.h"
@@ -105,4 +106,15 @@ void fdvref_mc(const uint8_t *refp, int w, int h, int chroma, int bx, int by, in
     if (chroma) fdv_mc_chroma(&f.planes[0], bx, by, bw, bh, mvx, mvy, dst, bw);
     else        fdv_mc_luma(&f.planes[0], bx, by, bw, bh, mvx, mvy, dst, bw);
     fdv_frame_free(&f);
+}
+
+/* The key-frame coder over a whole frame (iframe_encode, fdv.h:5487), with the
+ * contract of fdvamd_iframe_encode_fn, and its decoder (iframe_decode, 5541). */
+size_t fdvref_iframe_encode(const uint8_t *y, const uint8_t *u, const uint8_t *v, int w, int h, int qp,
+                            uint8_t *out, size_t cap, uint8_t *ry, uint8_t *ru, uint8_t *rv) {
+    return iframe_encode(y, u, v, w, h, 0, h, qp, out, cap, ry, ru, rv);
+}
+
+int fdvref_iframe_decode(const uint8_t *in, size_t len, uint8_t *y, uint8_t *u, uint8_t *v, int w, int h) {
+    return iframe_decode(in, len, y, u, v, w, h, 0, h);
 }

@@ -97,6 +97,12 @@ def ref():
         lib.fdv_video_encode.restype = C.c_size_t
         lib.fdv_video_encode.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int,
                                          C.c_int, C.c_void_p, C.c_size_t]
+        lib.fdv_vtile_encode.restype = C.c_size_t
+        lib.fdv_vtile_encode.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                         C.c_void_p, C.c_size_t]
+        lib.fdv_vtile_decode.restype = C.c_int
+        lib.fdv_vtile_decode.argtypes = [C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                         C.c_int]
         lib.fdv_video_decode.restype = C.c_int
         lib.fdv_video_decode.argtypes = [C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p,
                                          C.c_void_p, C.c_void_p]
@@ -393,3 +399,48 @@ def oracle_parallel_exact(cur, r0, r1, w, h, navail, qp, lscale=1.0, metab=None,
     assert passes >= 0, "oracle stream overflow"
     st, rec, info = oracle_analyse(cur, r0, r1, w, h, navail, qp, lscale, 0, 0, given=given)
     return st, rec, info, given, pc, passes, bool(conv.value)
+
+
+def ref_video_encode(frames, w, h, qp, keyint, band_mbrows=0, nthreads=8):
+    """The reference's fdv_video_encode (or, with band_mbrows, fdv_vtile_encode)."""
+    lib = ref()
+    fr = [np.ascontiguousarray(f, dtype=np.uint8) for f in frames]
+    fp = (C.c_void_p * len(fr))(*[f.ctypes.data for f in fr])
+    cap = len(fr) * (w * h * 3 + 8192) + 65536
+    out = np.zeros(cap, np.uint8)
+    if band_mbrows:
+        n = lib.fdv_vtile_encode(C.addressof(fp), len(fr), w, h, qp, keyint, band_mbrows, nthreads,
+                                 out.ctypes.data, cap)
+    else:
+        n = lib.fdv_video_encode(C.addressof(fp), len(fr), w, h, qp, keyint, out.ctypes.data, cap)
+    return out[:n].tobytes()
+
+
+def ref_video_decode(stream, tiled=False, nthreads=8):
+    """The reference's fdv_video_decode / fdv_vtile_decode: (frames array
+    [n, w*h*3/2], w, h), or None if it rejects the stream."""
+    lib = ref()
+    data = np.frombuffer(stream, dtype=np.uint8).copy()
+    b = bytes(stream[:8])
+    if tiled:
+        w, h, n = b[0] | b[1] << 8, b[2] | b[3] << 8, b[4] | b[5] << 8
+    else:
+        n, w, h = b[0] | b[1] << 8, b[2] | b[3] << 8, b[4] | b[5] << 8
+    out = np.zeros(n * (w * h * 3 // 2), np.uint8)
+    nf, ww, hh = C.c_int(), C.c_int(), C.c_int()
+    if tiled:
+        rc = lib.fdv_vtile_decode(data.ctypes.data, len(data), out.ctypes.data, C.byref(nf), C.byref(ww),
+                                  C.byref(hh), nthreads)
+    else:
+        rc = lib.fdv_video_decode(data.ctypes.data, len(data), out.ctypes.data, C.byref(nf), C.byref(ww),
+                                  C.byref(hh))
+    if rc != 0:
+        return None
+    return out.reshape(n, -1), w, h
+
+
+def ref_iframe_coder():
+    """Function pointers (ints) of the reference's key-frame coder and decoder
+    (iframe_encode / iframe_decode through oracle/ref_shim.c)."""
+    lib = ref()
+    return (C.cast(lib.fdvref_iframe_encode, C.c_void_p).value, C.cast(lib.fdvref_iframe_decode, C.c_void_p).value)
