@@ -17,6 +17,9 @@
 #include "../../include/fdvgpu.h"
 
 extern "C" int fdvg_set_tables(const int32_t *c8, const int32_t *qs8);
+extern "C" int fdvg_set_tables_decode(const int32_t *c8, const int32_t *qs8);
+extern "C" int fdvg_launch_decode(const fdvg_seq *, int, int, const fdvg_mbinfo *, int, const fdvg_mbrec *, unsigned *,
+                                  unsigned *, uint64_t *, hipStream_t);
 extern "C" int fdvg_launch_analyse(const fdvg_seq *, const int2 *, int, unsigned *, fdvg_mbrec *,
                                    uint64_t *, hipStream_t);
 extern "C" int fdvg_launch_deblock(const void *, int, int, int, hipStream_t);
@@ -302,7 +305,9 @@ extern "C" fdvgpu *fdvgpu_open(int nseq, const int *w, const int *h, int nsrc) {
     free(rowtab_h); rowtab_h = NULL;
     free(sym_h); sym_h = NULL;
     tables8(c8, qs8);
-    if (fdvg_set_tables(c8, qs8) != 0) { seterr("could not load the 8x8 tables"); goto fail; }
+    if (fdvg_set_tables(c8, qs8) != 0 || fdvg_set_tables_decode(c8, qs8) != 0) {
+        seterr("could not load the 8x8 tables"); goto fail;
+    }
     return g;
 fail:
     free(rowtab_h); free(sym_h);
@@ -537,6 +542,79 @@ extern "C" int fdvgpu_pframe(fdvgpu *g, const int *src_slot, const int *qp, cons
     g->fetched = 0;
     g->can_rewind = 1;
     return 0;
+}
+
+// ---- decoding
+
+extern "C" int fdvgpu_decode(fdvgpu *g, const fdvgpu_mbcode *const *codes, const int *qp, const int *navail) {
+    static_assert(sizeof(fdvgpu_mbcode) == sizeof(fdvg_mbrec), "MB code layout");
+    if (!g || !codes || !qp || !navail) { seterr("fdvgpu_decode: bad arguments"); return -1; }
+    for (int s = 0; s < g->nseq; ++s)
+        if (!codes[s] || qp[s] < 0 || qp[s] > 51 || navail[s] < 1 || navail[s] > 2) {
+            seterr("fdvgpu_decode: bad parameters for sequence %d", s); return -1;
+        }
+    fdvgpu_mbinfo *modes = (fdvgpu_mbinfo *)calloc((size_t)g->total_mb, sizeof(fdvgpu_mbinfo));
+    if (!modes) { seterr("out of host memory"); return -1; }
+    g->step_ctr = g->step_ctr + 1 ? g->step_ctr + 1 : 1;
+    const int slot = g->ring_pos;
+    g->ring_pos = (g->ring_pos + 1) % 8;
+    int rc = -1;
+    if (hipEventSynchronize(g->ring_ev[slot]) != hipSuccess) { seterr("fdvgpu_decode: staging wait failed"); goto out; }
+    {
+        fdvg_seq *seq_h = g->seq_h + (size_t)slot * g->nseq;
+        PlaneRef *planes_h = g->planes_h + (size_t)slot * 3 * g->nseq;
+        for (int s = 0; s < g->nseq; ++s) {
+            SeqMem &m = g->S[s];
+            fdvg_seq &q = seq_h[s];
+            memset(&q, 0, sizeof q);
+            const int r1 = navail[s] >= 2 ? m.far_ : m.near_;
+            q.ry[0] = fy(m, m.near_); q.ru[0] = fu(m, m.near_); q.rv[0] = fv(m, m.near_);
+            q.ry[1] = fy(m, r1);      q.ru[1] = fu(m, r1);      q.rv[1] = fv(m, r1);
+            q.oy = fy(m, m.out_); q.ou = fu(m, m.out_); q.ov = fv(m, m.out_);
+            q.w = m.w; q.h = m.h; q.ys = m.ys; q.cs = m.cs;
+            q.navail = navail[s]; q.qp = qp[s];
+            q.mb0 = m.mb0;
+            q.step = g->step_ctr;
+            planes_h[3 * s + 0] = PlaneRef{q.oy, m.w, m.h, m.ys, qp[s]};
+            planes_h[3 * s + 1] = PlaneRef{q.ou, m.cw, m.ch, m.cs, qp[s]};
+            planes_h[3 * s + 2] = PlaneRef{q.ov, m.cw, m.ch, m.cs, qp[s]};
+            for (int i = 0; i < m.nmb; ++i) modes[m.mb0 + i].mode = codes[s][i].mode;
+            if (hipMemcpyAsync(g->recs_d + m.mb0, codes[s], (size_t)m.nmb * sizeof(fdvg_mbrec), hipMemcpyHostToDevice,
+                               g->st) != hipSuccess) { seterr("fdvgpu_decode: upload failed"); goto out; }
+        }
+        if (hipMemcpyAsync(g->seq_d, seq_h, g->nseq * sizeof(fdvg_seq), hipMemcpyHostToDevice, g->st) != hipSuccess ||
+            hipMemcpyAsync(g->planes_d, planes_h, 3 * g->nseq * sizeof(PlaneRef), hipMemcpyHostToDevice, g->st) !=
+                hipSuccess ||
+            hipMemcpyAsync(g->pgiven_d, modes, (size_t)g->total_mb * sizeof(fdvgpu_mbinfo), hipMemcpyHostToDevice,
+                           g->st) != hipSuccess ||
+            hipEventRecord(g->ring_ev[slot], g->st) != hipSuccess) {
+            seterr("fdvgpu_decode: upload failed"); goto out;
+        }
+    }
+    if (fdvg_launch_zero(g->ticket_d, 8, g->st) != 0 ||
+        fdvg_launch_decode(g->seq_d, g->nseq, g->xgrid, (const fdvg_mbinfo *)g->pgiven_d, g->total_mb, g->recs_d,
+                           g->ticket_d, g->ilist_d, g->ih_d, g->st) != 0) {
+        seterr("fdvgpu_decode: launch failed: %s", hipGetErrorString(hipGetLastError())); goto out;
+    }
+    if (launch_stage(g, 2, FDVGPU_ME_DIAMOND, false) != 0) goto out;
+    // the decoded frames are read right after: wait here, and check the
+    // intra pass's bounded waits
+    if (hipMemcpyAsync(g->flags_h, g->ticket_d, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, g->st) != hipSuccess ||
+        hipStreamSynchronize(g->st) != hipSuccess) {
+        seterr("fdvgpu_decode: %s", hipGetErrorString(hipGetLastError())); goto out;
+    }
+    if (g->flags_h[1]) { seterr("fdvgpu_decode: an intra MB waited too long for its neighbours"); goto out; }
+    for (int s = 0; s < g->nseq; ++s) {
+        SeqMem &m = g->S[s];
+        const int old_far = m.far_;
+        m.far_ = m.near_; m.near_ = m.out_; m.out_ = old_far;
+    }
+    g->have_frame = 0;
+    g->can_rewind = 0;
+    rc = 0;
+out:
+    free(modes);
+    return rc;
 }
 
 // ---- pipelined frame steps

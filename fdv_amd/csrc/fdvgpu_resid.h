@@ -16,8 +16,10 @@
 #pragma once
 #include "fdvgpu_dev.h"
 
-__constant__ int32_t c_T8[64];      // 8x8 DCT basis (built on the host from libm)
-__constant__ int32_t c_QS8[52];     // 8x8 quantiser steps
+// Each source file that includes this has its own copy of the tables (constant
+// memory belongs to a code object); each fills its copy at context creation.
+static __constant__ int32_t c_T8[64];      // 8x8 DCT basis (built on the host from libm)
+static __constant__ int32_t c_QS8[52];     // 8x8 quantiser steps
 
 #ifdef FDVG_PHASE_PROF
 __device__ unsigned long long g_rphase[8];

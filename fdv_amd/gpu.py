@@ -20,6 +20,10 @@ PROPOSAL_DTYPE = np.dtype([("cx", "i1", 8), ("cy", "i1", 8), ("kx", "i1", 4), ("
                            ("qx", "i1", (4, 2)), ("qy", "i1", (4, 2)), ("conf", "<f4"), ("pskip", "<f4")])
 MOTION_DTYPE = np.dtype([("mv16", "<i2", (2, 2)), ("sad16", "<i4", 2), ("mv8", "<i2", (4, 2)),
                          ("sad8", "<i4", 4), ("flags", "<i4")])
+MBCODE_DTYPE = np.dtype([("mode", "u1"), ("ref", "u1"), ("merge_idx", "u1"), ("ncand", "u1"), ("sub", "u1"),
+                         ("ckind", "u1"), ("pad0", "u1"), ("pad1", "u1"), ("lflag", "u1", 4), ("cflag", "u1", 2),
+                         ("pad2", "u1", 2), ("mvd", "<i2", (4, 2)), ("mv", "<i2", (4, 2)), ("lev", "<i2", 256),
+                         ("clev", "<i2", 128)])
 PROPOSER_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "learned", "proposer.f32")
 
 _lib = None
@@ -60,6 +64,7 @@ def _signatures():
         "fdvgpu_nn_train_step": (i, [vp, vp, vp, vp, i, f, f, vp, vp, i, vp]),
         "fdvgpu_nn_adam": (i, [vp, vp, vp, vp, i, f, i, i, vp]),
         "fdvgpu_timings": (i, [vp, C.POINTER(C.c_float)]),
+        "fdvgpu_decode": (i, [vp, vp, ip, ip]),
         "fdvgpu_motion_search": (i, [vp, vp, vp, i, i, i, i, i, vp, f, vp, vp]),
         "fdvgpu_motion_search_noalloc": (i, [vp, vp, vp, i, i, i, i, i, vp, f, vp, vp]),
         # include/fdv_amd.h
@@ -77,6 +82,14 @@ def _signatures():
         "fdvamd_set_proposer_thresholds": (i, [vp, f, f]),
         "fdvamd_rewind": (i, [vp]),
         "fdvamd_set_iframe_coder": (i, [vp, vp]),
+        "fdvamd_unpack_pframe": (i, [vp, sz, sz, vp, ip, C.POINTER(vp), C.POINTER(sz)]),
+        "fdvamd_parse_pframe": (i, [vp, C.POINTER(sz), i, i, vp]),
+        "fdvamd_decoder_open": (vp, [i, ip, ip]),
+        "fdvamd_decoder_close": (None, [vp]),
+        "fdvamd_decoder_set_ref": (i, [vp, i, vp]),
+        "fdvamd_pframe_decode_batch": (i, [vp, vp, vp, vp, vp, vp]),
+        "fdvamd_video_decode": (i, [vp, sz, vp, ip, ip, ip]),
+        "fdvamd_vtile_decode": (i, [vp, sz, vp, ip, ip, ip, i]),
         "fdvamd_set_sequence_mode": (i, [i]),
         "fdvamd_set_sequence_proposer": (i, [vp, i]),
         "fdvamd_video_encode": (sz, [vp, i, i, i, i, i, vp, sz]),
@@ -459,6 +472,107 @@ def vtile_encode(frames, w: int, h: int, qp: int, keyint: int, band_mbrows: int,
     if n == 0:
         raise GpuError("fdvamd_vtile_encode: " + lib().fdvamd_last_error().decode())
     return out[:n].tobytes()
+
+
+def unpack_pframe(payload: bytes, nmb: int, tc: TabCache | None = None):
+    """fdvamd_unpack_pframe: (qp, the 16 symbol streams) of a P-frame payload;
+    raises GpuError for a payload the reference decoder rejects."""
+    L = lib()
+    buf = np.frombuffer(bytes(payload), dtype=np.uint8).copy() if payload else np.zeros(1, np.uint8)
+    qp = C.c_int()
+    syms = C.c_void_p()
+    sn = (C.c_size_t * NS)()
+    rc = L.fdvamd_unpack_pframe(buf.ctypes.data, len(payload), nmb, tc.buf.ctypes.data if tc is not None else None,
+                                C.byref(qp), C.byref(syms), sn)
+    if rc != 0:
+        raise GpuError("fdvamd_unpack_pframe: malformed payload")
+    out, off = [], 0
+    for k in range(NS):
+        out.append(C.string_at(syms.value + off, sn[k]) if sn[k] else b"")
+        off += sn[k]
+    C.CDLL(None).free(C.c_void_p(syms.value))
+    return qp.value, out
+
+
+def parse_pframe(streams, w: int, h: int) -> np.ndarray:
+    """fdvamd_parse_pframe: the MB codes (MBCODE_DTYPE, raster order) of a
+    P-frame's 16 symbol streams; raises GpuError where the reference decoder
+    would reject them."""
+    allb = b"".join(streams)
+    buf = np.frombuffer(allb, dtype=np.uint8).copy() if allb else np.zeros(1, np.uint8)
+    sn = (C.c_size_t * NS)(*[len(s) for s in streams])
+    codes = np.zeros((w // 16) * (h // 16), dtype=MBCODE_DTYPE)
+    if lib().fdvamd_parse_pframe(buf.ctypes.data, sn, w, h, codes.ctypes.data) != 0:
+        raise GpuError("fdvamd_parse_pframe: malformed streams")
+    return codes
+
+
+def video_decode(stream: bytes, tiled: bool = False, nthreads: int = 8):
+    """fdvamd_video_decode / fdvamd_vtile_decode: (frames [n, w*h*3/2], w, h)."""
+    b = bytes(stream[:8])
+    if tiled:
+        w, h, n = b[0] | b[1] << 8, b[2] | b[3] << 8, b[4] | b[5] << 8
+    else:
+        n, w, h = b[0] | b[1] << 8, b[2] | b[3] << 8, b[4] | b[5] << 8
+    data = np.frombuffer(bytes(stream), dtype=np.uint8).copy()
+    out = np.zeros(max(n * (w * h * 3 // 2), 1), np.uint8)
+    nf, ww, hh = C.c_int(), C.c_int(), C.c_int()
+    L = lib()
+    if tiled:
+        rc = L.fdvamd_vtile_decode(data.ctypes.data, len(data), out.ctypes.data, C.byref(nf), C.byref(ww),
+                                   C.byref(hh), nthreads)
+    else:
+        rc = L.fdvamd_video_decode(data.ctypes.data, len(data), out.ctypes.data, C.byref(nf), C.byref(ww),
+                                   C.byref(hh))
+    if rc != 0:
+        raise GpuError("fdvamd_video_decode: " + L.fdvamd_last_error().decode())
+    return out[:n * (w * h * 3 // 2)].reshape(n, -1), w, h
+
+
+class Decoder:
+    """The P-frame decoder (fdvamd_decoder): payloads unpacked and parsed on the
+    host, frames reconstructed on the device against its reference pool."""
+
+    def __init__(self, sizes):
+        self.L = lib()
+        self.sizes = [(int(w), int(h)) for w, h in sizes]
+        self.h = self.L.fdvamd_decoder_open(len(self.sizes), _ints([w for w, _ in self.sizes]),
+                                            _ints([h for _, h in self.sizes]))
+        if not self.h:
+            raise GpuError("fdvamd_decoder_open: " + self.L.fdvamd_last_error().decode())
+
+    def close(self):
+        if self.h:
+            self.L.fdvamd_decoder_close(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_ref(self, seq, frame):
+        f = np.ascontiguousarray(frame, dtype=np.uint8)
+        if self.L.fdvamd_decoder_set_ref(self.h, seq, f.ctypes.data) != 0:
+            raise GpuError("fdvamd_decoder_set_ref: " + self.L.fdvamd_last_error().decode())
+
+    def pframe(self, payloads, navail, tcs=None):
+        """One P-frame of every sequence: the decoded frames (tight I420)."""
+        n = len(self.sizes)
+        bufs = [np.frombuffer(bytes(p), dtype=np.uint8).copy() if len(p) else np.zeros(1, np.uint8) for p in payloads]
+        outs = [np.zeros(w * h * 3 // 2, np.uint8) for w, h in self.sizes]
+        P = C.c_void_p * n
+        ip_ = P(*[b.ctypes.data for b in bufs])
+        ln = (C.c_size_t * n)(*[len(p) for p in payloads])
+        nav = _ints(navail)
+        tcp = P(*[t.buf.ctypes.data for t in tcs]) if tcs is not None else None
+        op = P(*[o.ctypes.data for o in outs])
+        rc = self.L.fdvamd_pframe_decode_batch(self.h, C.addressof(ip_), C.addressof(ln), C.addressof(nav),
+                                               C.addressof(tcp) if tcp is not None else None, C.addressof(op))
+        if rc != 0:
+            raise GpuError("fdvamd_pframe_decode_batch: " + self.L.fdvamd_last_error().decode())
+        return outs
 
 
 def nn_nparam() -> int:

@@ -36,6 +36,14 @@ int fdvamd_set_iframe_coder(fdvamd_iframe_encode_fn enc, fdvamd_iframe_decode_fn
     return 0;
 }
 
+/* The registered key-frame decoder, for the sequence decoders. */
+fdvamd_iframe_decode_fn fdvamd_iframe_decoder_(void) {
+    pthread_mutex_lock(&g_mu);
+    const fdvamd_iframe_decode_fn f = g_idec;
+    pthread_mutex_unlock(&g_mu);
+    return f;
+}
+
 int fdvamd_set_sequence_mode(int me_mode) {
     if (me_mode < 0 || me_mode > 3) { fdvamd_seterr_("fdvamd_set_sequence_mode: bad mode %d", me_mode); return -1; }
     pthread_mutex_lock(&g_mu);

@@ -157,6 +157,55 @@ size_t fdvamd_video_encode(const uint8_t *const *frames, int nframes, int w, int
 size_t fdvamd_vtile_encode(const uint8_t *const *frames, int nframes, int w, int h, int qp, int keyint,
                            int band_mbrows, int nthreads, uint8_t *out, size_t cap);
 
+/* ---- decoding ----------------------------------------------------------- */
+
+/* The inverse of fdvamd_pack_pframe: a P-frame payload's entropy coding undone
+ * (pframe_decode's first half, fdv.h:6492-6625) into its 16 symbol streams,
+ * stored one after another in *syms (malloc'd, the caller frees it) with their
+ * lengths in sn[]; *qp is the frame's quantiser.  nmb (the frame's MB count)
+ * bounds the symbol count as the reference does.  tc (may be NULL) is the
+ * entropy state carried between P-frames, updated as the reference decoder
+ * updates it.  Returns 0, or -1 for a payload the reference decoder rejects. */
+int fdvamd_unpack_pframe(const uint8_t *in, size_t len, size_t nmb, fdvamd_tabcache *tc, int *qp, uint8_t **syms,
+                         size_t sn[16]);
+
+/* The MB syntax of a P-frame's streams (pframe_decode, fdv.h:6626-6729; the
+ * residual syntax of 5280, 5426, 6429) parsed into one fdvgpu_mbcode per MB
+ * (include/fdvgpu.h), raster order, for a w x h frame.  Returns 0, or -1 for
+ * streams the reference decoder rejects (including vectors outside its
+ * bounds check, fdv.h:6480). */
+int fdvamd_parse_pframe(const uint8_t *syms, const size_t sn[16], int w, int h, void *codes);
+
+typedef struct fdvamd_decoder fdvamd_decoder;
+
+/* A P-frame decoder for nseq sequences of the given sizes: a two-frame
+ * reference pool per sequence in device memory (fdv_video_decode's pool,
+ * fdv.h:6870-6874).  fdvamd_decoder_set_ref installs a decoded key frame
+ * (tight I420) as the near reference (load_ref + pool swap, 6921-6923). */
+fdvamd_decoder *fdvamd_decoder_open(int nseq, const int *w, const int *h);
+void fdvamd_decoder_close(fdvamd_decoder *d);
+int fdvamd_decoder_set_ref(fdvamd_decoder *d, int seq, const uint8_t *i420);
+
+/* One P-frame of every sequence: payload in[s] (len[s] bytes) against navail[s]
+ * references, entropy state tc[s] (may be NULL); the decoded frame (tight
+ * I420) into out[s] (may be NULL: it stays on the device as the next
+ * reference).  pframe_decode (fdv.h:6492-6737) for all sequences: the payloads
+ * are unpacked and parsed on host threads, the frames reconstructed and
+ * deblocked on the device in one pass.  Returns 0, or -1 if any payload is
+ * malformed. */
+int fdvamd_pframe_decode_batch(fdvamd_decoder *d, const uint8_t *const *in, const size_t *len, const int *navail,
+                               fdvamd_tabcache *const *tc, uint8_t *const *out);
+
+/* fdv_video_decode (fdv.h:907-908, body 6857-6929) and fdv_vtile_decode
+ * (980-981, body 7280-7378), same signatures and results: all nframes decoded
+ * frames, consecutive tight I420, into out; key frames from the registered
+ * decoder (fdvamd_set_iframe_coder), P-frames on the device, every band of a
+ * tiled frame in one pass (the bands of a tiled stream must share their key
+ * interval, as the reference's tiled encoder writes them).  Returns 0, or -1
+ * for a stream the reference decoder rejects. */
+int fdvamd_video_decode(const uint8_t *in, size_t len, uint8_t *out, int *nframes, int *w, int *h);
+int fdvamd_vtile_decode(const uint8_t *in, size_t len, uint8_t *out, int *nframes, int *w, int *h, int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

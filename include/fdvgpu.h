@@ -118,6 +118,43 @@ int fdvgpu_recon(fdvgpu *g, int seq, uint8_t *i420);
 /* The per-MB decisions of the last frame (nmb records, raster order). */
 int fdvgpu_decisions(fdvgpu *g, int seq, fdvgpu_mbinfo *out);
 
+/* ---- decoding.  One MB of a parsed P-frame: its decision and its quantised
+ * levels, the form in which the symbol streams carry them (pframe_encode's
+ * emission, fdv.h:6036-6160) and the reference's decoder reads them back
+ * (pframe_decode, fdv.h:6492-6737).
+ *   mode 0 SKIP, 1 INTER16, 2 INTRA, 3 INTER8; ref: the reference a SKIP or
+ *   INTER16 MB predicts from; sub: the INTRA sub-mode (0..3);
+ *   mv[k]: quarter-pel vectors (SKIP / INTER16 [0], INTER8 all four quadrants,
+ *   raster); merge_idx, ncand, mvd: as coded (not read by the decoder);
+ *   lflag[r]: luma 8x8 region r (raster): 0 nothing coded, 1 four 4x4
+ *   blocks, 2 one 8x8 block; levels at lev[64 r ..]: four raster 4x4 blocks
+ *   of 16 levels in raster order, or 64 levels of the 8x8 in raster order;
+ *   ckind: the chroma coding of the mode: 0 none (SKIP), 1 an any-flag per
+ *   plane then four 4x4 blocks (INTER16), 2 a region flag per plane as lflag
+ *   (INTRA), 3 four 4x4 blocks per plane without a flag (INTER8); cflag[p]
+ *   the flag of plane p (U, V); levels at clev[64 p ..] in lev's layout. */
+typedef struct {
+    uint8_t mode, ref, merge_idx, ncand;
+    uint8_t sub, ckind, pad0, pad1;
+    uint8_t lflag[4];
+    uint8_t cflag[2];
+    uint8_t pad2[2];
+    int16_t mvd[4][2];
+    int16_t mv[4][2];
+    int16_t lev[256];
+    int16_t clev[128];
+} fdvgpu_mbcode;
+
+/* Reconstruct one P-frame of every sequence on the device from its MB codes
+ * (codes[s]: nmb records in raster order, host memory): prediction from the
+ * near / far references (navail[s] of them) or intra neighbours, dequantised
+ * inverse transforms, reconstruction and the band deblock at qp[s] -- the
+ * device half of pframe_decode (fdv.h:6492-6737).  Vectors must pass the
+ * reference decoder's bounds check (mv_in_bounds, fdv.h:6480).  The decoded
+ * frame becomes the near reference, as after fdvgpu_pframe; read it with
+ * fdvgpu_recon.  Synchronous. */
+int fdvgpu_decode(fdvgpu *g, const fdvgpu_mbcode *const *codes, const int *qp, const int *navail);
+
 /* ---- launch machinery.  Each frame step's device work after its descriptor
  * copies is a fixed launch sequence per (search mode, decision structure); from
  * its second use it is captured into a hipGraph and replayed (on by default).
