@@ -142,6 +142,8 @@ struct fdvgpu {
     int pipe_ok;                    // all of the above allocated
     int have_frame, fetched;
     int can_rewind;                 // the last call was fdvgpu_pframe: its rotation can be undone
+    int last_mode;                  // the search mode of the last fdvgpu_pframe
+    int keep_tables;                // the next fdvgpu_pframe reuses the motion tables of the undone one
 };
 
 static uint8_t *fy(const SeqMem &m, int f) { return m.frames + (size_t)f * m.fsz + (size_t)FDVG_BORDER * m.ys + FDVG_BORDER; }
@@ -333,6 +335,7 @@ extern "C" int fdvgpu_put_ref(fdvgpu *g, int seq, int which, const uint8_t *i420
     if (!seq_ok(g, seq) || (which != 0 && which != 1) || !i420) { seterr("fdvgpu_put_ref: bad arguments"); return -1; }
     SeqMem &m = g->S[seq];
     g->can_rewind = 0;
+    g->keep_tables = 0;
     int f = which == 0 ? m.near_ : m.far_;
     size_t tsz = (size_t)m.w * m.h + 2 * (size_t)m.cw * m.ch;
     uint8_t *tmp = NULL;
@@ -527,11 +530,18 @@ extern "C" int fdvgpu_pframe(fdvgpu *g, const int *src_slot, const int *qp, cons
     CK(hipMemcpyAsync(g->seq_d, seq_h, g->nseq * sizeof(fdvg_seq), hipMemcpyHostToDevice, g->st));
     CK(hipMemcpyAsync(g->planes_d, planes_h, 3 * g->nseq * sizeof(PlaneRef), hipMemcpyHostToDevice, g->st));
     CK(hipEventRecord(g->ring_ev[slot], g->st));
-    // the stages, with the timing events between them (outside any graph)
+    // the stages, with the timing events between them (outside any graph); a
+    // retry after fdvgpu_rewind_keep_motion keeps the undone frame's motion
+    // tables and only clears the counters of stage 0
+    const bool reuse = g->keep_tables && me_mode == g->last_mode && me_mode != FDVGPU_ME_DIAMOND;
+    g->keep_tables = 0;
     for (int stage = 0; stage < 4; ++stage) {
-        if (run_stage(g, stage, me_mode, par) != 0) return -1;
+        if (stage == 0 && reuse) {
+            if (fdvg_launch_zero(g->ticket_d, 8, g->st) != 0) { seterr("ticket reset launch failed"); return -1; }
+        } else if (run_stage(g, stage, me_mode, par) != 0) return -1;
         CK(hipEventRecord(g->ev[stage + 1], g->st));
     }
+    g->last_mode = me_mode;
     for (int s = 0; s < g->nseq; ++s) {
         SeqMem &m = g->S[s];
         int old_far = m.far_;
@@ -683,6 +693,12 @@ extern "C" int fdvgpu_collected_streams(fdvgpu *g, int slot, int seq, const uint
         sv[k] = g->pipe_sc[slot] + a;
         sn[k] = b - a;
     }
+    return 0;
+}
+
+extern "C" int fdvgpu_rewind_keep_motion(fdvgpu *g) {
+    if (fdvgpu_rewind(g) != 0) return -1;
+    g->keep_tables = 1;
     return 0;
 }
 

@@ -47,6 +47,7 @@ def _signatures():
         "fdvgpu_fetch": (i, [vp]),
         "fdvgpu_streams": (i, [vp, i, u8pp, C.POINTER(sz)]),
         "fdvgpu_rewind": (i, [vp]),
+        "fdvgpu_rewind_keep_motion": (i, [vp]),
         "fdvgpu_recon": (i, [vp, i, vp]),
         "fdvgpu_decisions": (i, [vp, i, vp]),
         "fdvgpu_set_graphs": (i, [vp, i]),
@@ -81,6 +82,7 @@ def _signatures():
         "fdvamd_set_proposer": (i, [vp, vp, i]),
         "fdvamd_set_proposer_thresholds": (i, [vp, f, f]),
         "fdvamd_rewind": (i, [vp]),
+        "fdvamd_rewind_keep_motion": (i, [vp]),
         "fdvamd_set_iframe_coder": (i, [vp, vp]),
         "fdvamd_unpack_pframe": (i, [vp, sz, sz, vp, ip, C.POINTER(vp), C.POINTER(sz)]),
         "fdvamd_parse_pframe": (i, [vp, C.POINTER(sz), i, i, vp]),
@@ -179,6 +181,12 @@ class Analyser:
 
     def fetch(self):
         self._ck(self.L.fdvgpu_fetch(self.h), "fetch")
+
+    def rewind(self, keep_motion=False):
+        """Undo the last frame step's reference rotation (fdvgpu_rewind); with
+        keep_motion the next step reuses its motion tables."""
+        fn = self.L.fdvgpu_rewind_keep_motion if keep_motion else self.L.fdvgpu_rewind
+        self._ck(fn(self.h), "rewind")
 
     def streams(self, seq):
         sv = (C.POINTER(C.c_uint8) * NS)()
@@ -333,8 +341,9 @@ class Encoder:
         if self.L.fdvamd_set_ref_planes(self.h, seq, which, y.ctypes.data, u.ctypes.data, v.ctypes.data) != 0:
             raise GpuError("fdvamd_set_ref_planes: " + self.L.fdvamd_last_error().decode())
 
-    def rewind(self):
-        if self.L.fdvamd_rewind(self.h) != 0:
+    def rewind(self, keep_motion=False):
+        fn = self.L.fdvamd_rewind_keep_motion if keep_motion else self.L.fdvamd_rewind
+        if fn(self.h) != 0:
             raise GpuError("fdvamd_rewind: " + self.L.fdvamd_last_error().decode())
 
     def pframe_encode(self, frame, navail, qp, tc: TabCache | None = None, lscale=1.0, force_skip=0):

@@ -291,6 +291,14 @@ int fdvamd_rewind(fdvamd_encoder *e) {
     return 0;
 }
 
+int fdvamd_rewind_keep_motion(fdvamd_encoder *e) {
+    if (e && e->pipe_n) { amd_seterr("submitted frames not collected"); return -1; }
+    if (!e || fdvgpu_rewind_keep_motion(e->g) != 0) {
+        amd_seterr("%s", e ? fdvgpu_last_error() : "bad arguments"); return -1;
+    }
+    return 0;
+}
+
 int fdvamd_set_proposer(fdvamd_encoder *e, const float *weights, int n) {
     if (!e || fdvgpu_set_proposer(e->g, weights, n) != 0) {
         amd_seterr("%s", e ? fdvgpu_last_error() : "bad arguments"); return -1;

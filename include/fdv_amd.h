@@ -113,6 +113,13 @@ int fdvamd_set_proposer_thresholds(fdvamd_encoder *e, float conf, float skip);
  * frame to undo. */
 int fdvamd_rewind(fdvamd_encoder *e);
 
+/* fdvamd_rewind for a retry of the same frames: the retry (fdvamd_pframe_batch
+ * or fdvamd_pframe_encode with the same frames, any QP) reuses the undone
+ * step's motion tables in the table modes (fdvgpu_rewind_keep_motion): the
+ * "a requantising retry should reuse ME" of fdv.h:8303-8334.  Returns 0, or -1
+ * if there is no frame to undo. */
+int fdvamd_rewind_keep_motion(fdvamd_encoder *e);
+
 /* ---- sequences, with the reference's signatures and containers ---------- */
 
 /* A key-frame coder: codes the tight planes y (w x h), u, v (w/2 x h/2) at qp

@@ -112,6 +112,17 @@ int fdvgpu_collected_streams(fdvgpu *g, int slot, int seq, const uint8_t **sv, s
  * first is allowed); -1 otherwise. */
 int fdvgpu_rewind(fdvgpu *g);
 
+/* fdvgpu_rewind, and the next fdvgpu_pframe -- which must code the same source
+ * frames in the same table mode (exhaustive, parallel or learned) -- reuses the
+ * motion tables of the frame step it undoes instead of searching again: the
+ * requantising retry at another QP then redoes only the decisions, coding,
+ * deblock and serialisation.  The exhaustive tables do not depend on the QP,
+ * so the retry's result equals a fresh step's; the learned mode keeps the
+ * first attempt's proposals.  In the diamond mode the search depends on the
+ * decisions and runs again.  The reference's retry (fdv.h:8303-8334) calls
+ * pframe_encode again from the start. */
+int fdvgpu_rewind_keep_motion(fdvgpu *g);
+
 /* The deblocked reconstruction of the last frame (tight I420, host memory). */
 int fdvgpu_recon(fdvgpu *g, int seq, uint8_t *i420);
 

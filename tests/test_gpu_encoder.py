@@ -243,3 +243,75 @@ def test_wavefront_tags_survive_many_block_parallel_frames():
             assert np.array_equal(rec, orec), f"frame {f}: reconstruction"
         recs = recs[-1:] + [rec]
     an.close()
+
+
+@pytest.mark.parametrize("mode", [G.ME_EXHAUSTIVE, G.ME_PARALLEL, G.ME_LEARNED])
+def test_retry_reuses_the_motion_tables(mode):
+    """A requantising retry after fdvgpu_rewind_keep_motion keeps the first
+    attempt's motion tables, and its result is the oracle's at the retry QP
+    from those tables (the exhaustive tables do not depend on the QP, so for
+    them this is a fresh step's result)."""
+    scenes = [content.scene_pan(320, 192, 35, nobj=4, smin=16, smax=48),
+              content.scene_objects(128, 48, 36, nobj=3, smin=8, smax=32, vmax=8)]
+    sizes = [(s.w, s.h) for s in scenes]
+    frames = [[s.frame(i) for i in range(4)] for s in scenes]
+    an = G.Analyser(sizes, nsrc=1)
+    if mode == G.ME_LEARNED:
+        an.set_proposer()
+    near = [fr[0] for fr in frames]
+    far = [None, None]
+    for k in range(2):
+        an.put_ref(k, 0, near[k])
+    for f in range(1, 4):
+        navail = [1 if f == 1 else 2] * 2
+        for k in range(2):
+            an.put_source(k, 0, frames[k][f])
+        say(f"  retry mode {mode} frame {f}: qp 20, rewind keeping motion, qp 34")
+        an.pframe([0, 0], [20, 20], navail=navail, me_mode=mode)
+        an.fetch()
+        tables = [an.motion(k) for k in range(2)]
+        props = [an.proposals(k) for k in range(2)] if mode == G.ME_LEARNED else None
+        an.rewind(keep_motion=True)
+        an.pframe([0, 0], [34, 34], navail=navail, me_mode=mode)
+        an.fetch()
+        for k in range(2):
+            w, h = sizes[k]
+            assert an.motion(k).tobytes() == tables[k].tobytes()
+            if mode == G.ME_LEARNED:
+                mt = L.oracle_metab_candidates(frames[k][f], near[k], far[k], w, h, navail[k], props[k])
+            else:
+                mt = L.oracle_metab_exhaustive(frames[k][f], near[k], far[k], w, h, navail[k])
+            if mode == G.ME_EXHAUSTIVE:
+                st, rec, _ = L.oracle_analyse(frames[k][f], near[k], far[k], w, h, navail[k], 34, me_mode=1)
+            else:
+                st, rec, _, _, _ = L.oracle_parallel(frames[k][f], near[k], far[k], w, h, navail[k], 34, metab=mt)
+            assert an.streams(k) == st, f"seq {k} frame {f}: streams"
+            grec = an.recon(k)
+            assert np.array_equal(grec, rec), f"seq {k} frame {f}: recon"
+            far[k], near[k] = near[k], grec
+    an.close()
+
+
+def test_retry_skips_the_search_time():
+    """At 1280x720 the exhaustive search is most of the tables stage: a retry
+    that keeps the motion tables spends a fraction of a first attempt's time
+    there (the median of several retries against several first attempts)."""
+    sc = content.scene_pan(1280, 720, 37)
+    frames = [sc.frame(i) for i in range(3)]
+    an = G.Analyser([(1280, 720)], nsrc=2)
+    an.put_ref(0, 0, frames[0])
+    an.put_source(0, 0, frames[1])
+    first, retry = [], []
+    for rep in range(6):
+        an.pframe([0], [20], navail=[1], me_mode=G.ME_EXHAUSTIVE)
+        an.fetch()
+        first.append(an.timings()[0])
+        an.rewind(keep_motion=True)
+        an.pframe([0], [34], navail=[1], me_mode=G.ME_EXHAUSTIVE)
+        an.fetch()
+        retry.append(an.timings()[0])
+        an.rewind()
+    f, r = sorted(first)[3], sorted(retry)[3]
+    say(f"  tables stage: first attempt {f:.4f} ms, retry {r:.4f} ms")
+    assert r < 0.5 * f
+    an.close()
