@@ -102,8 +102,21 @@ def cpu_baseline(frames, w, layout, qp, seconds=12.0):
     dt = time.perf_counter() - t0
     px = nfr * w * h
     return {"value": round(px / dt / 1e6, 3), "unit": "Mpix/s", "cores": nb, "kind": kind,
+            "nproc": os.cpu_count(), "cpu_model": _cpu_model(),
             "sample": "%d P-frames of the same 1280x720 8-band workload (frames cycled), "
-                      "%d threads, one per band (%.1f s)" % (nfr, nb, dt)}
+                      "%d threads, one per band (%.1f s)" % (nfr, nb, dt),
+            "includes": "the reference's whole P-frame encode (pframe_encode), entropy coding of the payload "
+                        "included; the GPU value excludes entropy coding (end_to_end includes it)"}
+
+
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def profiled_traffic(mode: str):
@@ -177,6 +190,38 @@ def _job_stress(w, h):
 
 WORKLOADS = {"720p": _job_720p, "batched": _job_batched, "stress1080": _job_stress(1920, 1088),
              "stress4k": _job_stress(3840, 2160)}
+
+
+def bench_1080p(content, G, steps, warmup):
+    """Learned and exhaustive Mpix/s of 1920x1088 P-frames (the headline's pan
+    content at 1080p), QP 24, as 12 bands of 6 MB rows in one batch."""
+    w, h = 1920, 1088
+    layout = bands(h, 6)
+    sc = content.scene_pan(w, h, seed=2)
+    nfr = steps + warmup + 1
+    per = [band_frames(sc.frame(i), w, h, layout) for i in range(nfr)]
+    sizes = [(w, bh) for _, bh in layout]
+    nb = len(sizes)
+    res = {}
+    for name, m in (("learned", G.ME_LEARNED), ("exhaustive", G.ME_EXHAUSTIVE)):
+        a = G.Analyser(sizes, nsrc=nfr)
+        if m == G.ME_LEARNED:
+            a.set_proposer()
+        for b in range(nb):
+            a.put_ref(b, 0, per[0][b])
+            for i in range(1, nfr):
+                a.put_source(b, i, per[i][b])
+        a.autotune([1] * nb, [24] * nb, navail=[1] * nb, me_mode=m, reps=2)
+        for i in range(warmup + steps):
+            if i == warmup:
+                t0 = time.perf_counter()
+            a.pframe([1 + i] * nb, [24] * nb, [1.0] * nb, [1 if i == 0 else 2] * nb, [0] * nb, m)
+            a.fetch()
+        res[name] = steps * w * h / (time.perf_counter() - t0) / 1e6
+        a.close()
+    return {"frame": [w, h], "bands": nb, "qp": 24, "learned_mpix_s": round(res["learned"], 3),
+            "exhaustive_mpix_s": round(res["exhaustive"], 3),
+            "speedup_vs_exhaustive": round(res["learned"] / res["exhaustive"], 3)}
 
 
 def main():
@@ -301,6 +346,23 @@ def main():
             step(i + 2, ab, G.ME_EXHAUSTIVE)
         base = args.baseline_steps * px_frame / (time.perf_counter() - tb) / 1e6
         ab.close()
+    # the parallel mode (exhaustive tables, the same block-parallel decisions,
+    # no network) on the same frames: what the proposer adds or costs
+    par_mpix = None
+    if args.baseline_steps > 0 and mode == G.ME_LEARNED:
+        ap_ = open_analyser(G.ME_PARALLEL)
+        for i in range(min(args.warmup, 2)):
+            step(i, ap_, G.ME_PARALLEL)
+        tb = time.perf_counter()
+        for i in range(args.baseline_steps):
+            step(i + 2, ap_, G.ME_PARALLEL)
+        par_mpix = args.baseline_steps * px_frame / (time.perf_counter() - tb) / 1e6
+        ap_.close()
+    # the 1080p half of the metric: 1920x1088 (the same pan content) as
+    # 6-MB-row bands, learned and exhaustive on the same frames
+    line1080 = None
+    if args.baseline_steps > 0 and args.workload == "720p" and mode == G.ME_LEARNED:
+        line1080 = bench_1080p(content, G, args.baseline_steps, min(args.warmup, 2))
     e2e = None
     if args.end_to_end:
         # the drop-in encoder (include/fdv_amd.h): every band's source uploaded,
@@ -386,6 +448,9 @@ def main():
         "launch_shapes": {"serialiser_waves": shapes[0], "deblock_threads": shapes[1], "proposer_wg_per_cu": shapes[2]},
         "exhaustive_mpix_s": None if base is None else round(base, 3),
         "speedup_vs_exhaustive": None if base is None else round(value / base, 3),
+        "parallel_mpix_s": None if par_mpix is None else round(par_mpix, 3),
+        "speedup_vs_parallel": None if par_mpix is None else round(value / par_mpix, 3),
+        "line_1080p": line1080,
         "bd_rate_vs_exhaustive": bd,
         "roofline": roof,
         "cpu_baseline": cpu,
