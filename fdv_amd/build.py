@@ -80,9 +80,8 @@ def build_scenes() -> None:
 
 
 def build_all(verbose: bool = False) -> None:
-    build_oracle()
-    build_scenes()
     build_gpu(verbose)
+    build_oracle()             # after the library: the drop-in demonstration links it
 
 
 if __name__ == "__main__":
