@@ -595,37 +595,18 @@ __device__ __forceinline__ const uint8_t *mb_finish(Smem &s, const fdvg_seq &q, 
 
 // ------------------------------------------------------------------ kernel
 
-// Optional per-phase cycle accounting (thread 0 of every workgroup), compiled in
-// only with -DFDVG_PHASE_PROF for performance investigation.
-#ifdef FDVG_PHASE_PROF
-__device__ unsigned long long g_phase[16];
-#define PH(k) do { if (tid == 0) { long long t_ = clock64(); ph[k] += t_ - ph_last; ph_last = t_; } } while (0)
-extern "C" int fdvg_phase_read(unsigned long long *out) {
-    return hipMemcpyFromSymbol(out, HIP_SYMBOL(g_phase), sizeof g_phase) == hipSuccess ? 0 : -1;
-}
-#else
-#define PH(k) do { } while (0)
-#endif
-
 extern "C" __global__ void __launch_bounds__(256)
 k_analyse(const fdvg_seq *__restrict__ seqs, const int2 *__restrict__ rowtab, int nrows,
           unsigned *ticket, fdvg_mbrec *__restrict__ recs, uint64_t *__restrict__ hand) {
     __shared__ Smem s;
     const int tid = threadIdx.x;
     int par = 0;
-#ifdef FDVG_PHASE_PROF
-    long long ph[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    long long ph_last = clock64();
-#endif
     for (;;) {
         if (tid == 0) s.row = (int)atomicAdd(ticket, 1u);
         __syncthreads();
         const int row = s.row;
         __syncthreads();
         if (row >= nrows) {
-#ifdef FDVG_PHASE_PROF
-            if (tid == 0) for (int k = 0; k < 8; ++k) atomicAdd(&g_phase[k], (unsigned long long)ph[k]);
-#endif
             return;
         }
         const int2 rt = rowtab[row];
@@ -656,7 +637,6 @@ k_analyse(const fdvg_seq *__restrict__ seqs, const int2 *__restrict__ rowtab, in
             }
             mb_stage(s, q, mb, bx, by);
             __syncthreads();
-            PH(0);
 
             // ---- neighbours from the row above / the previous MB
             MbCtx c;
@@ -697,7 +677,6 @@ k_analyse(const fdvg_seq *__restrict__ seqs, const int2 *__restrict__ rowtab, in
             const fdvg_metab *mt = (q.me_mode == FDVG_ME_TABLE && q.metab) ? &q.metab[mb - q.mb0] : nullptr;
             MbDec D;
             mb_decide(s, q, c, gv, mt, par, D);
-            PH(1);
             const uint8_t *yrec = mb_finish(s, q, mb, bx, by, c, D, &recs[mb]);
             int nmx, nmy, nref, ninter;
             mb_motion(c, D, &nmx, &nmy, &nref, &ninter);
@@ -725,7 +704,6 @@ k_analyse(const fdvg_seq *__restrict__ seqs, const int2 *__restrict__ rowtab, in
             }
             if (mode != 2) { pmx = nmx; pmy = nmy; }
             __syncthreads();
-            PH(7);
         }
     }
 }
@@ -942,47 +920,6 @@ __device__ __forceinline__ uint64_t mrec_pack(uint64_t tag, int inter, int ref, 
            ((uint64_t)(uint16_t)x << 16) | (uint64_t)(uint16_t)y;
 }
 
-#ifdef FDVG_PHASE_PROF
-// Quarter-pel luma sample from a small tile of stride S whose integer sample
-// for the output pixel is at (X, Y): the H.264 positions of fdv.h:2931-2990,
-// one sample at a time (the profiling build checks qerr4 against it).
-template <int S> __device__ __forceinline__ int t_h(const uint8_t *w, int x, int y) {
-    const uint8_t *p = w + y * S + x;
-    return tap6(p[-2], p[-1], p[0], p[1], p[2], p[3]);
-}
-template <int S> __device__ __forceinline__ int t_v(const uint8_t *w, int x, int y) {
-    const uint8_t *p = w + y * S + x;
-    return tap6(p[-2 * S], p[-S], p[0], p[S], p[2 * S], p[3 * S]);
-}
-template <int S> __device__ __forceinline__ int t_b(const uint8_t *w, int x, int y) { return clip8((t_h<S>(w, x, y) + 16) >> 5); }
-template <int S> __device__ __forceinline__ int t_hh(const uint8_t *w, int x, int y) { return clip8((t_v<S>(w, x, y) + 16) >> 5); }
-template <int S> __device__ __forceinline__ int t_j(const uint8_t *w, int x, int y) {
-    const int v = tap6(t_h<S>(w, x, y - 2), t_h<S>(w, x, y - 1), t_h<S>(w, x, y), t_h<S>(w, x, y + 1),
-                       t_h<S>(w, x, y + 2), t_h<S>(w, x, y + 3));
-    return clip8((v + 512) >> 10);
-}
-template <int S> __device__ __forceinline__ int lumaq_t(const uint8_t *w, int X, int Y, int ph) {
-    const int G = w[Y * S + X];
-    switch (ph) {
-    case 0:  return G;
-    case 1:  return av2(G, t_b<S>(w, X, Y));
-    case 2:  return t_b<S>(w, X, Y);
-    case 3:  return av2(t_b<S>(w, X, Y), w[Y * S + X + 1]);
-    case 4:  return av2(G, t_hh<S>(w, X, Y));
-    case 5:  return av2(t_b<S>(w, X, Y), t_hh<S>(w, X, Y));
-    case 6:  return av2(t_b<S>(w, X, Y), t_j<S>(w, X, Y));
-    case 7:  return av2(t_b<S>(w, X, Y), t_hh<S>(w, X + 1, Y));
-    case 8:  return t_hh<S>(w, X, Y);
-    case 9:  return av2(t_hh<S>(w, X, Y), t_j<S>(w, X, Y));
-    case 10: return t_j<S>(w, X, Y);
-    case 11: return av2(t_j<S>(w, X, Y), t_hh<S>(w, X + 1, Y));
-    case 12: return av2(t_hh<S>(w, X, Y), w[(Y + 1) * S + X]);
-    case 13: return av2(t_hh<S>(w, X, Y), t_b<S>(w, X, Y + 1));
-    case 14: return av2(t_j<S>(w, X, Y), t_b<S>(w, X, Y + 1));
-    default: return av2(t_hh<S>(w, X + 1, Y), t_b<S>(w, X, Y + 1));
-    }
-}
-#endif
 
 // SSD of the 16x16 block `curb` against its prediction from a tile (row stride
 // S, the block's (0, 0) at tile (2, 2)) at quarter-pel phase ph, by one
@@ -997,13 +934,6 @@ __device__ __forceinline__ int ssd_tile_q(const uint8_t *tile, const uint8_t *cu
 // rows w, w + NW, ...; a row's motion records go to an LDS ring slot per row
 // and its progress to prog[w], which the wave of the row below polls.  Each
 // wave stages its row's costs and tables PR_CHUNK MBs at a time.
-#ifdef FDVG_PHASE_PROF
-#define RPP(k) do { if (L == 0) { long long t_ = clock64(); atomicAdd(&g_phase[8 + (k)], (unsigned long long)(t_ - rp_last)); rp_last = t_; } } while (0)
-#define RPC(k) do { if (L == 0) atomicAdd(&g_phase[(k)], 1ull); } while (0)
-#else
-#define RPP(k) do { } while (0)
-#define RPC(k) do { } while (0)
-#endif
 #define PR_CHUNK 8
 #define PR_TS 24                         // tile stride: 21 samples a row
 struct __align__(16) PrWave {
@@ -1082,9 +1012,6 @@ k_prepair(const fdvg_seq *__restrict__ seqs, int mwmax, unsigned *ctl,
     const double lambda = q.lambda;
     if (L == 0) __hip_atomic_store(&prog[wv], -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     __syncthreads();
-#ifdef FDVG_PHASE_PROF
-    long long rp_last = clock64();
-#endif
     for (int my = wv; my < mh; my += NW) {
         uint64_t *cur_slot = ring + (size_t)(my % NW) * mwmax;
         const uint64_t *above = ring + (size_t)((my + NW - 1) % NW) * mwmax;
@@ -1102,7 +1029,6 @@ k_prepair(const fdvg_seq *__restrict__ seqs, int mwmax, unsigned *ctl,
                 wave_sync();
                 if (mx + PR_CHUNK < mw) pr_load(pre, q, pcs, pflag, ctl, my, mx + PR_CHUNK, L);
             }
-            RPP(0);
             const fdvg_pcost &P = W.pc[mx % PR_CHUNK];
             const fdvg_metab &T = W.mt[mx % PR_CHUNK];
             // everything this MB reads from LDS, before the wait
@@ -1140,7 +1066,6 @@ k_prepair(const fdvg_seq *__restrict__ seqs, int mwmax, unsigned *ctl,
                     v = __hip_atomic_load(up, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
                 }
             }
-            RPP(1);
             const uint64_t va = my > 0 ? above[mx] : 0;
             const uint64_t vr = (my > 0 && mx + 1 < mw) ? above[mx + 1] : 0;
             // merge list: predictor, then the inter MBs above and above-right (fdv.h:5711-5732)
@@ -1167,8 +1092,6 @@ k_prepair(const fdvg_seq *__restrict__ seqs, int mwmax, unsigned *ctl,
                 const uint64_t hm = __ballot(hit);
                 if (hm) {
                     ssd[c] = __builtin_amdgcn_readlane(se_l, __builtin_ctzll(hm));
-                    RPP(2);
-                    RPC(0);
                 } else {
                     uint8_t tv[7];
                     if (c == 0) {
@@ -1190,25 +1113,12 @@ k_prepair(const fdvg_seq *__restrict__ seqs, int mwmax, unsigned *ctl,
                     }
                     wave_sync();
                     ssd[c] = wave_sum(ssd_tile_q<PR_TS>(W.tile, curb, ((y & 3) << 2) | (x & 3), L));
-#ifdef FDVG_PHASE_PROF
-                    {   // the per-sample definition, for the profiling build's self-check
-                        const int ph = ((y & 3) << 2) | (x & 3);
-                        int e = 0;
-                        for (int k = L; k < 256; k += 64) {
-                            const int d = curb[k] - lumaq_t<PR_TS>(W.tile, (k & 15) + 2, (k >> 4) + 2, ph);
-                            e += d * d;
-                        }
-                        if (wave_sum(e) != ssd[c]) RPC(3);
-                    }
-#endif
                     wave_sync();
-                    if (c == 0) { RPP(3); RPC(1); } else { RPP(7); RPC(2); }
                 }
             }
             int Ds = ssd[0], bmi = 0;
             if (ncand > 1 && ssd[1] < Ds) { Ds = ssd[1]; bmi = 1; }
             if (ncand > 2 && ssd[2] < Ds) { Ds = ssd[2]; bmi = 2; }
-            RPP(4);
             // the reference's gates and commit on the stored costs
             const double Js = (double)Ds + lambda * (double)(3 + (ncand > 1 ? 3 : 0));
             double Jbest = Js;
@@ -1257,7 +1167,6 @@ k_prepair(const fdvg_seq *__restrict__ seqs, int mwmax, unsigned *ctl,
             } else if (mode == 1) { nx = iref ? m1x : m0x; ny = iref ? m1y : m0y; nr = iref; }
             else if (mode == 2) ni = 0;
             else { nx = q8x[3]; ny = q8y[3]; }
-            RPP(5);
             if (L == 0) {
                 // the row below needs only the motion record: publish it first
                 cur_slot[mx] = mrec_pack(0, ni, nr, nx, ny);
@@ -1276,7 +1185,6 @@ k_prepair(const fdvg_seq *__restrict__ seqs, int mwmax, unsigned *ctl,
                 X.cx[2] = (int16_t)cx2; X.cy[2] = (int16_t)cy2; X.cr[2] = (uint8_t)cr2;
                 pctx[mb] = X;
             }
-            RPP(6);
             if (mode != 2) { pmx = nx; pmy = ny; }
         }
     }

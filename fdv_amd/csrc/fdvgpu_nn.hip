@@ -16,7 +16,7 @@
 //
 // Numerics: weights are rounded to bf16 for the forward pass; the inputs of the
 // three convolutions (the features, then the two 32-channel activations) are
-// bf16 -- the convolutions run on v_mfma_f32_32x32x8bf16_1k with fp32
+// bf16 -- the convolutions run on v_mfma_f32_32x32x16_bf16 with fp32
 // accumulation -- and the linear layer runs on fp32 VALU with the bf16-rounded
 // weights; biases are fp32.  The backward pass feeds MFMA with bf16 gradients
 // and accumulates fp32.  Master weights and Adam state are fp32.
@@ -44,7 +44,17 @@
 #define NN_NPARAM (OFF_BF + 2)
 
 typedef short bf4 __attribute__((ext_vector_type(4)));
+typedef short s8v __attribute__((ext_vector_type(8)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef float f16v __attribute__((ext_vector_type(16)));
+
+// One gfx950 32x32x16 bf16 MFMA.  Lane l holds row (A) / column (B) l % 32 and
+// the eight k slots 8 * (l / 32) .. +7; the caller fills A and B with the same
+// k in the same slot, so which slot holds which k is free.
+__device__ __forceinline__ f16v mfma16(s8v a, s8v b, f16v c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf8, a), __builtin_bit_cast(bf8, b), c, 0, 0, 0);
+}
+__device__ __forceinline__ s8v cat8(bf4 lo, bf4 hi) { return __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7); }
 
 __device__ __forceinline__ uint16_t f2bf(float f) {          // round to nearest even
     const uint32_t u = __float_as_uint(f);
@@ -140,32 +150,34 @@ __device__ __forceinline__ f16v conv_tile(const uint16_t *in, const uint16_t *w,
     for (int tap = 0; tap < 9; ++tap) {
         const int q = nn_shift(p, tap);
 #pragma unroll
-        for (int c0 = 0; c0 < 32; c0 += 8) {
-            const int ci = c0 + 4 * g;
-            bf4 a = {0, 0, 0, 0}, b;
-            if (q >= 0) a = *(const bf4 *)&in[q * NN_C + ci];
-            b = *(const bf4 *)&w[i * NN_K + tap * 32 + ci];          // B[k][co = i] = w[co][k]
-            c = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, c, 0, 0, 0);
+        for (int c0 = 0; c0 < 32; c0 += 16) {
+            const int ci = c0 + 8 * g;
+            bf4 a0 = {0, 0, 0, 0}, a1 = {0, 0, 0, 0};
+            if (q >= 0) { a0 = *(const bf4 *)&in[q * NN_C + ci]; a1 = *(const bf4 *)&in[q * NN_C + ci + 4]; }
+            const uint16_t *wb = &w[i * NN_K + tap * 32 + ci];           // B[k][co = i] = w[co][k]
+            c = mfma16(cat8(a0, a1), cat8(*(const bf4 *)wb, *(const bf4 *)(wb + 4)), c);
         }
     }
     return c;
 }
 
-// conv1 tile (2 -> 32 channels): K = 18 padded to 24, k = tap*2 + ci; the
-// four k of a lane are two taps x two channels.
+// conv1 tile (2 -> 32 channels): K = 18 (k = tap*2 + ci) padded to 32; the
+// eight k of a lane are four taps x two channels.
 __device__ __forceinline__ f16v conv1_tile(const uint16_t *fb, const uint16_t *w1, int tile) {
     const int l = threadIdx.x & 63, i = l & 31, g = l >> 5;
     const int p = tile * 32 + i;
     f16v c = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int k0 = 0; k0 < 24; k0 += 8) {
-        const int ta = (k0 + 4 * g) >> 1;
-        const int qa = ta < 9 ? nn_shift(p, ta) : -1, qb = ta + 1 < 9 ? nn_shift(p, ta + 1) : -1;
-        bf4 a = {0, 0, 0, 0};
-        if (qa >= 0) { a[0] = (short)fb[2 * qa]; a[1] = (short)fb[2 * qa + 1]; }
-        if (qb >= 0) { a[2] = (short)fb[2 * qb]; a[3] = (short)fb[2 * qb + 1]; }
-        const bf4 b = *(const bf4 *)&w1[i * 24 + k0 + 4 * g];
-        c = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, c, 0, 0, 0);
+    for (int k0 = 0; k0 < 32; k0 += 16) {
+        const int kb = k0 + 8 * g;
+        s8v a = {0, 0, 0, 0, 0, 0, 0, 0}, b = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int ta = (kb >> 1) + u, q = ta < 9 ? nn_shift(p, ta) : -1;
+            if (q >= 0) { a[2 * u] = (short)fb[2 * q]; a[2 * u + 1] = (short)fb[2 * q + 1]; }
+        }
+        if (kb < 24) b = cat8(*(const bf4 *)&w1[i * 24 + kb], *(const bf4 *)&w1[i * 24 + kb + 4]);
+        c = mfma16(a, b, c);
     }
     return c;
 }
@@ -346,16 +358,16 @@ k_nn_train(const float *__restrict__ W, const float *__restrict__ feats, const i
             for (int tap = wv; tap < 9; tap += nw) {
                 f16v c = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
                 const int i = l & 31, g = l >> 5;
-                for (int p0 = 0; p0 < NN_PT; p0 += 8) {
-                    bf4 a, b;
+                for (int p0 = 0; p0 < NN_PT; p0 += 16) {
+                    s8v a, b;
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        const int p = p0 + 4 * g + u;
+                    for (int u = 0; u < 8; ++u) {
+                        const int p = p0 + 8 * g + u;
                         a[u] = (short)S.dz[p * NN_C + i];                       // A[co = i][p]
                         const int q = nn_shift(p, tap);
                         b[u] = q >= 0 ? (short)in[q * NN_C + i] : (short)0;      // B[p][ci = i]
                     }
-                    c = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, c, 0, 0, 0);
+                    c = mfma16(a, b, c);
                 }
 #pragma unroll
                 for (int r = 0; r < 16; ++r) {
@@ -375,13 +387,14 @@ k_nn_train(const float *__restrict__ W, const float *__restrict__ feats, const i
                 for (int tap = 0; tap < 9; ++tap) {
                     const int q = nn_unshift(p, tap);
 #pragma unroll
-                    for (int c0 = 0; c0 < 32; c0 += 8) {
-                        const int co = c0 + 4 * g;
-                        bf4 a = {0, 0, 0, 0}, b;
-                        if (q >= 0) a = *(const bf4 *)&S.dz[q * NN_C + co];
+                    for (int c0 = 0; c0 < 32; c0 += 16) {
+                        const int co = c0 + 8 * g;
+                        bf4 a0 = {0, 0, 0, 0}, a1 = {0, 0, 0, 0};
+                        if (q >= 0) { a0 = *(const bf4 *)&S.dz[q * NN_C + co]; a1 = *(const bf4 *)&S.dz[q * NN_C + co + 4]; }
+                        s8v b;
 #pragma unroll
-                        for (int u = 0; u < 4; ++u) b[u] = (short)w[(co + u) * NN_K + tap * 32 + i];
-                        c = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, c, 0, 0, 0);
+                        for (int u = 0; u < 8; ++u) b[u] = (short)w[(co + u) * NN_K + tap * 32 + i];
+                        c = mfma16(cat8(a0, a1), b, c);
                     }
                 }
                 // masked by this layer's input activation; stage in a3 (fp32) until dz is free
@@ -431,15 +444,6 @@ extern "C" __global__ void k_nn_adam(float *__restrict__ W, float *__restrict__ 
 
 // --------------------------------------------------------------- inference
 
-#ifdef FDVG_PHASE_PROF
-__device__ unsigned long long g_nnph[8];
-#define NPH(k) do { if (t == 0) { long long t_ = clock64(); atomicAdd(&g_nnph[k], (unsigned long long)(t_ - nph_last)); nph_last = t_; } } while (0)
-extern "C" int fdvg_nnphase_read(unsigned long long *out) {
-    return hipMemcpyFromSymbol(out, HIP_SYMBOL(g_nnph), sizeof g_nnph) == hipSuccess ? 0 : -1;
-}
-#else
-#define NPH(k) do { } while (0)
-#endif
 
 // The inference path: one wavefront per MB, four per workgroup, no
 // workgroup barriers after the weights are staged.
@@ -456,7 +460,6 @@ extern "C" int fdvg_nnphase_read(unsigned long long *out) {
 //  - The linear layer is fused into the conv3 epilogue: per tile a
 //    reduce-scatter over the 32 lanes of each half-wave leaves every position's
 //    logit on two lanes (fp32 throughout).
-typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef unsigned int u4v __attribute__((ext_vector_type(4)));
 #define NI_RS 40
 #define NI_WAVES 4
@@ -668,9 +671,6 @@ k_nn_infer(const fdvg_seq *__restrict__ seqs, int nseq, int total, const float *
     NiShared &S = *(NiShared *)dyn_nn;
     const int t = threadIdx.x, wv = t >> 6, l = t & 63, i32 = l & 31, g = l >> 5;
     NiWave &Wv = S.wave[wv];
-#ifdef FDVG_PHASE_PROF
-    long long nph_last = clock64();
-#endif
     // stage the weights: B[k][co] for k = 16 s + 8 (lane / 32) + j, co = lane % 32
     for (int e = t; e < 2 * 18 * 64; e += blockDim.x) {
         const int layer = e / (18 * 64), st = (e / 64) % 18, ln = e % 64;
@@ -718,7 +718,6 @@ k_nn_infer(const fdvg_seq *__restrict__ seqs, int nseq, int total, const float *
         sh1[j] = (tap / 3 - 1) * NI_PW + tap % 3 - 1;
     }
     __syncthreads();
-    NPH(0);
     const uint8_t *actb = (const uint8_t *)Wv.act;
     for (int gi = blockIdx.x * NI_WAVES + wv; gi < total; gi += gridDim.x * NI_WAVES) {
         int k = 0;
@@ -734,7 +733,6 @@ k_nn_infer(const fdvg_seq *__restrict__ seqs, int nseq, int total, const float *
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        NPH(1);
         // conv1: two K steps (taps 0-3 | 4-7 by lane half, then tap 8)
 #pragma unroll
         for (int tl = 0; tl < 10; ++tl) {
@@ -755,7 +753,6 @@ k_nn_infer(const fdvg_seq *__restrict__ seqs, int nseq, int total, const float *
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        NPH(6);
         f16v acc[10];
 #pragma unroll
         for (int layer = 0; layer < 2; ++layer) {
@@ -796,7 +793,6 @@ k_nn_infer(const fdvg_seq *__restrict__ seqs, int nseq, int total, const float *
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                NPH(7);
             }
         }
         // conv3 epilogue + linear layer.  Reduce-scatter of the 16 per-lane values
@@ -837,7 +833,6 @@ k_nn_infer(const fdvg_seq *__restrict__ seqs, int nseq, int total, const float *
         }
         for (int o = 32; o > 0; o >>= 1) skacc += __shfl_xor(skacc, o);
         const float skl = skacc * (1.0f / NN_P) + S.bf[1];
-        NPH(2);
         // softmax statistics (each position counted once: even lanes)
         float m = -3.0e38f;
 #pragma unroll
@@ -849,7 +844,6 @@ k_nn_infer(const fdvg_seq *__restrict__ seqs, int nseq, int total, const float *
             for (int tl = 0; tl < 10; ++tl) se += lg[tl] > -1.0e38f ? expf(lg[tl] - m) : 0.f;
         }
         for (int o = 32; o > 0; o >>= 1) se += __shfl_xor(se, o);
-        NPH(3);
         // top 8, ties to the lower position.  Each logit becomes a key: its bits
         // mapped to an unsigned order (-0 taken as +0, which compares equal to
         // it) above the position's complement, so the largest key is the largest
@@ -880,7 +874,6 @@ k_nn_infer(const fdvg_seq *__restrict__ seqs, int nseq, int total, const float *
             if (l == 0) { o.cx[kk] = (int8_t)(2 * (bp % NN_G - 8)); o.cy[kk] = (int8_t)(2 * (bp / NN_G - 8)); }
         }
         if (l == 0) { o.conf = conf; o.pskip = 1.f / (1.f + expf(-skl)); }
-        NPH(4);
     }
 }
 

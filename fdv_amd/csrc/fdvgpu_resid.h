@@ -21,15 +21,6 @@
 static __constant__ int32_t c_T8[64];      // 8x8 DCT basis (built on the host from libm)
 static __constant__ int32_t c_QS8[52];     // 8x8 quantiser steps
 
-#ifdef FDVG_PHASE_PROF
-__device__ unsigned long long g_rphase[8];
-#define RPH(k) do { if (threadIdx.x == 0 && KIND == 0) { long long t_ = clock64(); atomicAdd(&g_rphase[k], (unsigned long long)(t_ - rp_last)); rp_last = t_; } } while (0)
-extern "C" int fdvg_rphase_read(unsigned long long *out) {
-    return hipMemcpyFromSymbol(out, HIP_SYMBOL(g_rphase), 8 * sizeof(unsigned long long)) == hipSuccess ? 0 : -1;
-}
-#else
-#define RPH(k) do { } while (0)
-#endif
 struct RegionWs {
     double rdJ[4][4][17];        // [region][block][L+1] RDOQ scores
     int16_t levA[4][64];         // option A levels, four raster 4x4 blocks
@@ -86,9 +77,6 @@ __device__ __forceinline__ void region_code(RegionWs &ws, int r, const uint8_t *
                                             int16_t *lev, uint8_t *flag, int qp, double lambda) {
     const int L = threadIdx.x & 63, blk = L >> 4, c = L & 15;
     const int oy4 = (blk >> 1) * 4, ox4 = (blk & 1) * 4;
-#ifdef FDVG_PHASE_PROF
-    long long rp_last = clock64();
-#endif
     const int m6 = qp % 6, sh = 1 << (qp / 6), qb = 15 + qp / 6, off = (1 << qb) / 3;
     const int mf0 = k_MF[m6][0], mf1 = k_MF[m6][1], mf2 = k_MF[m6][2];
     const int dv0 = k_DV[m6][0], dv1 = k_DV[m6][1], dv2 = k_DV[m6][2];
@@ -116,7 +104,6 @@ __device__ __forceinline__ void region_code(RegionWs &ws, int r, const uint8_t *
         }
     }
     int bestL = 15;
-    RPH(0);
     if (KIND != 1) {
         int last = -1, nzm = 0, drop[16];
 #pragma unroll
@@ -146,7 +133,6 @@ __device__ __forceinline__ void region_code(RegionWs &ws, int r, const uint8_t *
             for (int i = 0; i < 16; ++i) { const int d = res[i] - rr[i]; D += d * d; }
             ws.rdJ[r][blk][Lc + 1] = (double)D + lambda * (double)bits;
         }
-        RPH(1);
         wave_sync();
         if (c == 0) {
             // the reference's scan, L = last down to -1, over scores held in registers
@@ -163,7 +149,6 @@ __device__ __forceinline__ void region_code(RegionWs &ws, int r, const uint8_t *
             }
         }
     }
-    RPH(2);
     int e4 = 0, b4 = 0, any4 = 0;
     if (c == 0) {
         int lf[16], dq[16], rr[16];
@@ -202,7 +187,6 @@ __device__ __forceinline__ void region_code(RegionWs &ws, int r, const uint8_t *
     const int anyA = __builtin_amdgcn_readlane(any4, 0) | __builtin_amdgcn_readlane(any4, 16) |
                      __builtin_amdgcn_readlane(any4, 32) | __builtin_amdgcn_readlane(any4, 48);
     wave_sync();
-    RPH(3);
     const int y8 = L >> 3, x8 = L & 7;
     if (KIND == 0) {
         const int c8 = cb[y8 * cst + x8], p8 = pb[y8 * pst + x8];
@@ -237,7 +221,6 @@ __device__ __forceinline__ void region_code(RegionWs &ws, int r, const uint8_t *
         const int rr8 = (int16_t)((X + (1LL << 35)) >> 36);
         const int rec8 = clip8(p8 + rr8);
         const int d8 = c8 - rec8;
-        RPH(4);
         const int eB = wave_sum(d8 * d8);
         const int z8 = k_INV8[L];
         const int last8 = wave_max(lv8 ? z8 : -1);
@@ -247,7 +230,6 @@ __device__ __forceinline__ void region_code(RegionWs &ws, int r, const uint8_t *
         const bool takeA = (double)eA + lambda * (double)bA <= (double)eB + lambda * (double)bB;
         rb[y8 * rst + x8] = takeA ? ws.recA[r][L] : (uint8_t)rec8;
         lev[r * 64 + L] = takeA ? ws.levA[r][L] : (int16_t)lv8;
-        RPH(5);
         if (L == 0) {
             flag[r] = takeA ? (anyA ? 1 : 0) : (last8 >= 0 ? 2 : 0);
             ws.e[r] = takeA ? eA : eB;
