@@ -238,7 +238,8 @@ def main():
                     help="also time the drop-in encoder end to end (source upload, device analysis, "
                          "stream fetch, host entropy coding) on the same frames")
     ap.add_argument("--no-overlap", action="store_true",
-                    help="run the block-parallel decision stage's costs and replay one after the other")
+                    help="run the block-parallel decision stage's replay and coding as separate launches "
+                         "instead of the fused stage")
     ap.add_argument("--conf", type=float, default=None, help="proposer confidence threshold (default 0.7)")
     ap.add_argument("--skip", type=float, default=None, help="proposer SKIP threshold (default: off; 0.7 prunes)")
     ap.add_argument("--baseline-steps", type=int, default=10,
@@ -445,7 +446,8 @@ def main():
                        args.mode, ", block-parallel decisions" if mode in (G.ME_LEARNED, G.ME_PARALLEL) else ""),
                    "sequences": nb, "coded_px_per_step": px_frame, "search": args.mode, **job["extra"]},
         "stage_ms": stage_ms,
-        "launch_shapes": {"serialiser_waves": shapes[0], "deblock_threads": shapes[1], "proposer_wg_per_cu": shapes[2]},
+        "launch_shapes": {"serialiser_waves": shapes[0], "deblock_threads": shapes[1], "proposer_wg_per_cu": shapes[2],
+                          "decision_workers_per_cu": shapes[3]},
         "exhaustive_mpix_s": None if base is None else round(base, 3),
         "speedup_vs_exhaustive": None if base is None else round(value / base, 3),
         "parallel_mpix_s": None if par_mpix is None else round(par_mpix, 3),

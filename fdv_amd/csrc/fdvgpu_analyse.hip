@@ -285,7 +285,7 @@ __device__ __forceinline__ int hbyte(const Smem &s, int j, int b) {
 
 // Stage the source MB and, when the MB may use them, the 64x64 luma and 32x32
 // chroma windows of each reference around it.  The caller synchronises.
-__device__ __forceinline__ void mb_stage(Smem &s, const fdvg_seq &q, int mb, int bx, int by) {
+__device__ __forceinline__ void mb_stage(Smem &s, const fdvg_seq &q, int mb, int bx, int by, int gmode = -1) {
     const int tid = threadIdx.x, cw = q.w >> 1, cbx = bx >> 1, cby = by >> 1;
     s.cur[tid] = gmem(q.cy)[(size_t)(by + (tid >> 4)) * q.w + bx + (tid & 15)];
     if (tid < 128) {
@@ -293,7 +293,9 @@ __device__ __forceinline__ void mb_stage(Smem &s, const fdvg_seq &q, int mb, int
         const gmem_t<uint8_t> *src = gmem(pl ? q.cv : q.cu);
         s.ccur[tid] = src[(size_t)(cby + (p >> 3)) * cw + cbx + (p & 7)];
     }
-    const bool need_ref = !q.force_skip && !(q.given && q.given[mb - q.mb0].mode == 2);
+    // gmode >= 0: the MB's given mode, passed by a caller that holds it
+    const bool intra = gmode >= 0 ? gmode == 2 : (q.given && q.given[mb - q.mb0].mode == 2);
+    const bool need_ref = !q.force_skip && !intra;
     const int nwin = q.force_skip ? 1 : q.navail;
     if (need_ref || q.force_skip) {
         for (int r = 0; r < nwin; ++r) {
@@ -995,6 +997,244 @@ __device__ __forceinline__ void pr_store(PrWave &W, const PrPre &R, int L) {
     for (int j = 0; j < PR_CW; ++j) lc[L + 64 * j] = R.c[j];
 }
 
+// One row of the decision replay (the reference's decision logic on the stored
+// costs, fdv.h:5847-6040), one wave: MB mx waits until the row above has
+// decided MB mx + 1 (or its last MB), whose motion records it reads from the
+// LDS slot `above`: written by a wave of this workgroup (progress word *up) or,
+// with above_g, copied there a chunk at a time from another workgroup's tagged
+// records (the row then runs at least a chunk behind the row above).  FUSED also publishes each
+// decision for the coding workers of k_pdecide: the motion record in mrec_g and
+// the decision with its context in decw, both tagged with the frame step.
+#define PD_WORDS 10
+__device__ __forceinline__ uint32_t pk16(int lo, int hi) { return (uint32_t)(uint16_t)lo | (uint32_t)(uint16_t)hi << 16; }
+__device__ __forceinline__ uint64_t ld_rec(const uint64_t *p, int k) {
+    return __hip_atomic_load(p + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <bool FUSED>
+__device__ __forceinline__ void replay_row(PrWave &W, const fdvg_seq &q, const fdvg_pcost *__restrict__ pcs,
+                                           const unsigned *__restrict__ pflag, unsigned *ctl, int my,
+                                           uint64_t *cur_slot, int *myprog, uint64_t *above, const int *up,
+                                           const uint64_t *above_g, uint64_t *mrec_g, uint64_t *decw,
+                                           fdvg_mbinfo *__restrict__ given, fdvg_pctx *__restrict__ pctx, int L) {
+    const int mw = q.w >> 4;
+    const double lambda = q.lambda;
+    const uint64_t tagv = (uint64_t)(q.step & 0x3fffffffu) << 34;
+    int pmx = 0, pmy = 0;
+    PrPre pre;
+    uint64_t abv = tagv;
+    for (int mx = 0; mx < mw; ++mx) {
+        const int i = my * mw + mx, mb = q.mb0 + i, bx = mx * 16, by = my * 16;
+        if (mx % PR_CHUNK == 0) {
+            // this chunk's costs, tables and source MBs into LDS (from the
+            // registers loaded during the previous chunk, or now at a row
+            // start), then the next chunk's loads issued into registers
+            if (mx == 0) pr_load(pre, q, pcs, pflag, ctl, my, 0, L);
+            pr_store(W, pre, L);
+            if (my > 0 && above_g) {
+                // the row above, in another workgroup: its records of MBs
+                // mx .. mx + PR_CHUNK (tagged), waited for and copied into
+                // `above` once a chunk; the next chunk's are loaded early
+                if (mx == 0) abv = L <= PR_CHUNK ? ld_rec(above_g, mx + L < mw ? mx + L : mw - 1) : tagv;
+                for (unsigned spins = 0; !__all((abv >> 34) == (tagv >> 34)); ++spins) {
+                    if (spins > (1u << 21)) { if (L == 0) atomicOr(ctl + 1, 1u); break; }
+                    __builtin_amdgcn_s_sleep(1);
+                    if (L <= PR_CHUNK && (abv >> 34) != (tagv >> 34)) abv = ld_rec(above_g, mx + L < mw ? mx + L : mw - 1);
+                }
+                if (L <= PR_CHUNK && mx + L < mw) above[mx + L] = abv;
+                if (mx + PR_CHUNK < mw)
+                    abv = L <= PR_CHUNK ? ld_rec(above_g, mx + PR_CHUNK + L < mw ? mx + PR_CHUNK + L : mw - 1) : tagv;
+            }
+            wave_sync();
+            if (mx + PR_CHUNK < mw) pr_load(pre, q, pcs, pflag, ctl, my, mx + PR_CHUNK, L);
+        }
+        const fdvg_pcost &P = W.pc[mx % PR_CHUNK];
+        const fdvg_metab &T = W.mt[mx % PR_CHUNK];
+        // everything this MB reads from LDS, before the wait
+        const int nsc = P.nsc;
+        const int sx_l = L < FDVG_NSC ? P.scx[L] : 0, sy_l = L < FDVG_NSC ? P.scy[L] : 0;
+        const int sr_l = L < FDVG_NSC ? P.scr[L] : 0, se_l = L < FDVG_NSC ? P.sce[L] : 0;
+        const int d16_0 = P.d16[0], d16_1 = P.d16[1], b16_0 = P.b16[0], b16_1 = P.b16[1];
+        const int pdn = P.dn, pbn = P.bn, pd8 = P.d8, pb8 = P.b8;
+        const int m0x = T.mv16[0][0], m0y = T.mv16[0][1], m1x = T.mv16[1][0], m1y = T.mv16[1][1];
+        const int s16_0 = T.sad16[0], s16_1 = T.sad16[1], tfl = T.flags;
+        int q8x[4], q8y[4];
+#pragma unroll
+        for (int qd = 0; qd < 4; ++qd) { q8x[qd] = T.mv8[qd][0]; q8y[qd] = T.mv8[qd][1]; }
+        const int ssad = T.sad8[0] + T.sad8[1] + T.sad8[2] + T.sad8[3];
+        // merge candidate 0 is the predictor on reference 0, known already:
+        // outside the precomputed set, its 21x21 tile loads are issued now so
+        // that they complete during the wait below
+        const bool hit0 = __ballot(L < nsc && sx_l == pmx && sy_l == pmy && sr_l == 0) != 0;
+        uint8_t tv0[7];
+        if (!hit0) {
+            const gmem_t<uint8_t> *rp = gmem(q.ry[0]) + (ptrdiff_t)(by + (pmy >> 2) - 2) * q.ys + bx + (pmx >> 2) - 2;
+#pragma unroll
+            for (int j = 0; j < 7; ++j) {
+                const int k = L + 64 * j;
+                tv0[j] = k < 21 * 21 ? rp[(ptrdiff_t)(k / 21) * q.ys + k % 21] : 0;
+            }
+        }
+        // the row above must have decided the above-right MB (or the last one);
+        // from another workgroup, the chunk fetch above has waited already
+        uint64_t va = 0, vr = 0;
+        if (my > 0) {
+            if (!above_g) {
+                const int need = (my - 1) * 65536 + (mx + 2 < mw ? mx + 2 : mw);
+                int v = __hip_atomic_load(up, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+                for (unsigned spins = 0; v < need; ++spins) {
+                    if (spins > (1u << 24)) { if (L == 0) atomicOr(ctl + 1, 1u); break; }
+                    __builtin_amdgcn_s_sleep(1);
+                    v = __hip_atomic_load(up, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+                }
+            }
+            va = above[mx];
+            vr = mx + 1 < mw ? above[mx + 1] : 0;
+        }
+        // merge list: predictor, then the inter MBs above and above-right (fdv.h:5711-5732)
+        int cx0 = pmx, cy0 = pmy, cx1 = 0, cy1 = 0, cr1 = 0, cx2 = 0, cy2 = 0, cr2 = 0, ncand = 1;
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const uint64_t w = k ? vr : va;
+            if (!((w >> 33) & 1) || ncand >= 3) continue;
+            const int x = (int16_t)((w >> 16) & 0xffff), y = (int16_t)(w & 0xffff), r = (int)((w >> 32) & 1);
+            const bool dup = (cx0 == x && cy0 == y && r == 0) || (ncand > 1 && cx1 == x && cy1 == y && cr1 == r);
+            if (dup) continue;
+            if (ncand == 1) { cx1 = x; cy1 = y; cr1 = r; } else { cx2 = x; cy2 = y; cr2 = r; }
+            ++ncand;
+        }
+        // SKIP SSDs: from the precomputed set, else from a 21x21 tile of the reference
+        int ssd[3] = {0, 0, 0};
+        const uint8_t *curb = W.cur[mx % PR_CHUNK];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            if (c >= ncand) continue;
+            const int x = c == 0 ? cx0 : c == 1 ? cx1 : cx2, y = c == 0 ? cy0 : c == 1 ? cy1 : cy2;
+            const int r = c == 0 ? 0 : c == 1 ? cr1 : cr2;
+            const bool hit = L < nsc && sx_l == x && sy_l == y && sr_l == r;
+            const uint64_t hm = __ballot(hit);
+            if (hm) {
+                ssd[c] = __builtin_amdgcn_readlane(se_l, __builtin_ctzll(hm));
+            } else {
+                uint8_t tv[7];
+                if (c == 0) {
+#pragma unroll
+                    for (int j = 0; j < 7; ++j) tv[j] = tv0[j];
+                } else {
+                    const gmem_t<uint8_t> *rp = gmem(r ? q.ry[1] : q.ry[0]) +
+                                        (ptrdiff_t)(by + (y >> 2) - 2) * q.ys + bx + (x >> 2) - 2;
+#pragma unroll
+                    for (int j = 0; j < 7; ++j) {
+                        const int k = L + 64 * j;
+                        tv[j] = k < 21 * 21 ? rp[(ptrdiff_t)(k / 21) * q.ys + k % 21] : 0;
+                    }
+                }
+#pragma unroll
+                for (int j = 0; j < 7; ++j) {
+                    const int k = L + 64 * j;
+                    if (k < 21 * 21) W.tile[(k / 21) * PR_TS + k % 21] = tv[j];
+                }
+                wave_sync();
+                ssd[c] = wave_sum(ssd_tile_q<PR_TS>(W.tile, curb, ((y & 3) << 2) | (x & 3), L));
+                wave_sync();
+            }
+        }
+        int Ds = ssd[0], bmi = 0;
+        if (ncand > 1 && ssd[1] < Ds) { Ds = ssd[1]; bmi = 1; }
+        if (ncand > 2 && ssd[2] < Ds) { Ds = ssd[2]; bmi = 2; }
+        // the reference's gates and commit on the stored costs
+        const double Js = (double)Ds + lambda * (double)(3 + (ncand > 1 ? 3 : 0));
+        double Jbest = Js;
+        const bool skip_wins = Js <= lambda * (double)FL_INTRA;
+        const bool pruned = tfl & 1;          // the proposer expects SKIP: no INTRA, no INTER8
+        double Ji = 1e30;
+        int iref = 0, sad16 = 0;
+        if (!skip_wins) {
+            const int bb = b16_0 + egbits(zmap(m0x - pmx)) + egbits(zmap(m0y - pmy));
+            const double J = (double)d16_0 + lambda * (double)bb;
+            if (J < Jbest) Jbest = J;
+            Ji = J; sad16 = s16_0;
+            if (q.navail > 1 && !(Jbest <= lambda * (double)FL_INTER16)) {
+                const int bb1 = b16_1 + egbits(zmap(m1x - pmx)) + egbits(zmap(m1y - pmy));
+                const double J1 = (double)d16_1 + lambda * (double)bb1;
+                if (J1 < Jbest) Jbest = J1;
+                if (J1 < Ji) { Ji = J1; iref = 1; sad16 = s16_1; }
+            }
+        }
+        double Jn = 1e30;
+        if (!skip_wins && !pruned && Jbest > lambda * (double)FL_INTRA) {
+            Jn = (double)pdn + lambda * (double)pbn;
+            if (Jn < Jbest) Jbest = Jn;
+        }
+        double J8 = 1e30;
+        if (!skip_wins && !pruned && Jbest > lambda * (double)FL_INTER8) {
+            if (!(sad16 > 0 && ssad * 10 >= sad16 * 9)) {
+                int b8 = pb8, qpx = pmx, qpy = pmy;
+#pragma unroll
+                for (int qd = 0; qd < 4; ++qd) {
+                    b8 += egbits(zmap(q8x[qd] - qpx)) + egbits(zmap(q8y[qd] - qpy));
+                    qpx = q8x[qd]; qpy = q8y[qd];
+                }
+                J8 = (double)pd8 + lambda * (double)b8;
+            }
+        }
+        int mode = 0;
+        double Jm = Js;
+        if (Ji < Jm) { Jm = Ji; mode = 1; }
+        if (Jn < Jm) { Jm = Jn; mode = 2; }
+        if (J8 < Jm) { Jm = J8; mode = 3; }
+        int nx = 0, ny = 0, nr = 0, ni = 1;
+        if (mode == 0) {
+            nx = bmi == 0 ? cx0 : bmi == 1 ? cx1 : cx2; ny = bmi == 0 ? cy0 : bmi == 1 ? cy1 : cy2;
+            nr = bmi == 0 ? 0 : bmi == 1 ? cr1 : cr2;
+        } else if (mode == 1) { nx = iref ? m1x : m0x; ny = iref ? m1y : m0y; nr = iref; }
+        else if (mode == 2) ni = 0;
+        else { nx = q8x[3]; ny = q8y[3]; }
+        if constexpr (FUSED) {
+            // the coding workers' copy of the decision and its context:
+            // ten tagged words (lane k: 32-bit payload word k), no fence
+            const uint32_t mvw0 = mode == 3 ? pk16(q8x[0], q8y[0]) : mode <= 1 ? pk16(nx, ny) : 0u;
+            uint32_t d = 0;
+            switch (L) {
+            case 0: d = (uint32_t)mode | (uint32_t)(mode <= 1 ? nr : 0) << 8 | (uint32_t)(mode == 0 ? bmi : 0) << 16 |
+                        (uint32_t)(mode == 2 ? FDVG_SUB_SCREEN : 0) << 24; break;
+            case 1: d = mvw0; break;
+            case 2: d = mode == 3 ? pk16(q8x[1], q8y[1]) : 0u; break;
+            case 3: d = mode == 3 ? pk16(q8x[2], q8y[2]) : 0u; break;
+            case 4: d = mode == 3 ? pk16(q8x[3], q8y[3]) : 0u; break;
+            case 5: d = pk16(pmx, pmy); break;
+            case 6: d = pk16(cx0, cx1); break;
+            case 7: d = pk16(cx2, cy0); break;
+            case 8: d = pk16(cy1, cy2); break;
+            case 9: d = (uint32_t)cr1 << 8 | (uint32_t)cr2 << 16 | (uint32_t)ncand << 24; break;
+            default: break;
+            }
+            if (L == 0) __hip_atomic_store(mrec_g + mb, mrec_pack(tagv, ni, nr, nx, ny), __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+            if (L < PD_WORDS) __hip_atomic_store(decw + (size_t)mb * PD_WORDS + L, (uint64_t)q.step << 32 | d,
+                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (L == 0) {
+            // the row below needs only the motion record: publish it first
+            cur_slot[mx] = mrec_pack(0, ni, nr, nx, ny);
+            __hip_atomic_store(myprog, my * 65536 + mx + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+            fdvg_mbinfo G;
+            G.mode = (uint8_t)mode; G.ref = (uint8_t)(mode == 0 || mode == 1 ? nr : 0);
+            G.merge_idx = (uint8_t)(mode == 0 ? bmi : 0); G.sub = (uint8_t)(mode == 2 ? FDVG_SUB_SCREEN : 0);
+            for (int k = 0; k < 4; ++k) { G.mv[k][0] = 0; G.mv[k][1] = 0; }
+            if (mode == 0 || mode == 1) { G.mv[0][0] = (int16_t)nx; G.mv[0][1] = (int16_t)ny; }
+            if (mode == 3) for (int k = 0; k < 4; ++k) { G.mv[k][0] = (int16_t)q8x[k]; G.mv[k][1] = (int16_t)q8y[k]; }
+            given[mb] = G;
+            fdvg_pctx X;
+            X.pmx = (int16_t)pmx; X.pmy = (int16_t)pmy; X.ncand = (uint8_t)ncand;
+            X.cx[0] = (int16_t)cx0; X.cy[0] = (int16_t)cy0; X.cr[0] = 0;
+            X.cx[1] = (int16_t)cx1; X.cy[1] = (int16_t)cy1; X.cr[1] = (uint8_t)cr1;
+            X.cx[2] = (int16_t)cx2; X.cy[2] = (int16_t)cy2; X.cr[2] = (uint8_t)cr2;
+            pctx[mb] = X;
+        }
+        if (mode != 2) { pmx = nx; pmy = ny; }
+    }
+}
+
 extern "C" __global__ void __launch_bounds__(512)
 k_prepair(const fdvg_seq *__restrict__ seqs, int mwmax, unsigned *ctl,
           const fdvg_pcost *__restrict__ pcs, const unsigned *__restrict__ pflag, fdvg_mbinfo *__restrict__ given,
@@ -1006,188 +1246,14 @@ k_prepair(const fdvg_seq *__restrict__ seqs, int mwmax, unsigned *ctl,
     // offset from the LDS base, not an integer round trip, so that the
     // compiler keeps the LDS address space (ds_* rather than flat accesses)
     PrWave *pw = (PrWave *)((char *)dyn + ((((size_t)NW * mwmax) * 8 + (size_t)NW * 4 + 15) & ~(size_t)15));
-    PrWave &W = pw[wv];
     const fdvg_seq q = seqs[blockIdx.x];
-    const int mw = q.w >> 4, mh = q.h >> 4;
-    const double lambda = q.lambda;
+    const int mh = q.h >> 4;
     if (L == 0) __hip_atomic_store(&prog[wv], -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     __syncthreads();
-    for (int my = wv; my < mh; my += NW) {
-        uint64_t *cur_slot = ring + (size_t)(my % NW) * mwmax;
-        const uint64_t *above = ring + (size_t)((my + NW - 1) % NW) * mwmax;
-        int *up = &prog[(wv + NW - 1) % NW];
-        int pmx = 0, pmy = 0;
-        PrPre pre;
-        for (int mx = 0; mx < mw; ++mx) {
-            const int i = my * mw + mx, mb = q.mb0 + i, bx = mx * 16, by = my * 16;
-            if (mx % PR_CHUNK == 0) {
-                // this chunk's costs, tables and source MBs into LDS (from the
-                // registers loaded during the previous chunk, or now at a row
-                // start), then the next chunk's loads issued into registers
-                if (mx == 0) pr_load(pre, q, pcs, pflag, ctl, my, 0, L);
-                pr_store(W, pre, L);
-                wave_sync();
-                if (mx + PR_CHUNK < mw) pr_load(pre, q, pcs, pflag, ctl, my, mx + PR_CHUNK, L);
-            }
-            const fdvg_pcost &P = W.pc[mx % PR_CHUNK];
-            const fdvg_metab &T = W.mt[mx % PR_CHUNK];
-            // everything this MB reads from LDS, before the wait
-            const int nsc = P.nsc;
-            const int sx_l = L < FDVG_NSC ? P.scx[L] : 0, sy_l = L < FDVG_NSC ? P.scy[L] : 0;
-            const int sr_l = L < FDVG_NSC ? P.scr[L] : 0, se_l = L < FDVG_NSC ? P.sce[L] : 0;
-            const int d16_0 = P.d16[0], d16_1 = P.d16[1], b16_0 = P.b16[0], b16_1 = P.b16[1];
-            const int pdn = P.dn, pbn = P.bn, pd8 = P.d8, pb8 = P.b8;
-            const int m0x = T.mv16[0][0], m0y = T.mv16[0][1], m1x = T.mv16[1][0], m1y = T.mv16[1][1];
-            const int s16_0 = T.sad16[0], s16_1 = T.sad16[1], tfl = T.flags;
-            int q8x[4], q8y[4];
-#pragma unroll
-            for (int qd = 0; qd < 4; ++qd) { q8x[qd] = T.mv8[qd][0]; q8y[qd] = T.mv8[qd][1]; }
-            const int ssad = T.sad8[0] + T.sad8[1] + T.sad8[2] + T.sad8[3];
-            // merge candidate 0 is the predictor on reference 0, known already:
-            // outside the precomputed set, its 21x21 tile loads are issued now so
-            // that they complete during the wait below
-            const bool hit0 = __ballot(L < nsc && sx_l == pmx && sy_l == pmy && sr_l == 0) != 0;
-            uint8_t tv0[7];
-            if (!hit0) {
-                const gmem_t<uint8_t> *rp = gmem(q.ry[0]) + (ptrdiff_t)(by + (pmy >> 2) - 2) * q.ys + bx + (pmx >> 2) - 2;
-#pragma unroll
-                for (int j = 0; j < 7; ++j) {
-                    const int k = L + 64 * j;
-                    tv0[j] = k < 21 * 21 ? rp[(ptrdiff_t)(k / 21) * q.ys + k % 21] : 0;
-                }
-            }
-            // the row above must have finished the above-right MB (or the last one)
-            if (my > 0) {
-                const int need = (my - 1) * 65536 + (mx + 2 < mw ? mx + 2 : mw);
-                int v = __hip_atomic_load(up, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
-                for (unsigned spins = 0; v < need; ++spins) {
-                    if (spins > (1u << 24)) { if (L == 0) atomicOr(ctl + 1, 1u); break; }
-                    __builtin_amdgcn_s_sleep(1);
-                    v = __hip_atomic_load(up, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
-                }
-            }
-            const uint64_t va = my > 0 ? above[mx] : 0;
-            const uint64_t vr = (my > 0 && mx + 1 < mw) ? above[mx + 1] : 0;
-            // merge list: predictor, then the inter MBs above and above-right (fdv.h:5711-5732)
-            int cx0 = pmx, cy0 = pmy, cx1 = 0, cy1 = 0, cr1 = 0, cx2 = 0, cy2 = 0, cr2 = 0, ncand = 1;
-#pragma unroll
-            for (int k = 0; k < 2; ++k) {
-                const uint64_t w = k ? vr : va;
-                if (!((w >> 33) & 1) || ncand >= 3) continue;
-                const int x = (int16_t)((w >> 16) & 0xffff), y = (int16_t)(w & 0xffff), r = (int)((w >> 32) & 1);
-                const bool dup = (cx0 == x && cy0 == y && r == 0) || (ncand > 1 && cx1 == x && cy1 == y && cr1 == r);
-                if (dup) continue;
-                if (ncand == 1) { cx1 = x; cy1 = y; cr1 = r; } else { cx2 = x; cy2 = y; cr2 = r; }
-                ++ncand;
-            }
-            // SKIP SSDs: from the precomputed set, else from a 21x21 tile of the reference
-            int ssd[3] = {0, 0, 0};
-            const uint8_t *curb = W.cur[mx % PR_CHUNK];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                if (c >= ncand) continue;
-                const int x = c == 0 ? cx0 : c == 1 ? cx1 : cx2, y = c == 0 ? cy0 : c == 1 ? cy1 : cy2;
-                const int r = c == 0 ? 0 : c == 1 ? cr1 : cr2;
-                const bool hit = L < nsc && sx_l == x && sy_l == y && sr_l == r;
-                const uint64_t hm = __ballot(hit);
-                if (hm) {
-                    ssd[c] = __builtin_amdgcn_readlane(se_l, __builtin_ctzll(hm));
-                } else {
-                    uint8_t tv[7];
-                    if (c == 0) {
-#pragma unroll
-                        for (int j = 0; j < 7; ++j) tv[j] = tv0[j];
-                    } else {
-                        const gmem_t<uint8_t> *rp = gmem(r ? q.ry[1] : q.ry[0]) +
-                                            (ptrdiff_t)(by + (y >> 2) - 2) * q.ys + bx + (x >> 2) - 2;
-#pragma unroll
-                        for (int j = 0; j < 7; ++j) {
-                            const int k = L + 64 * j;
-                            tv[j] = k < 21 * 21 ? rp[(ptrdiff_t)(k / 21) * q.ys + k % 21] : 0;
-                        }
-                    }
-#pragma unroll
-                    for (int j = 0; j < 7; ++j) {
-                        const int k = L + 64 * j;
-                        if (k < 21 * 21) W.tile[(k / 21) * PR_TS + k % 21] = tv[j];
-                    }
-                    wave_sync();
-                    ssd[c] = wave_sum(ssd_tile_q<PR_TS>(W.tile, curb, ((y & 3) << 2) | (x & 3), L));
-                    wave_sync();
-                }
-            }
-            int Ds = ssd[0], bmi = 0;
-            if (ncand > 1 && ssd[1] < Ds) { Ds = ssd[1]; bmi = 1; }
-            if (ncand > 2 && ssd[2] < Ds) { Ds = ssd[2]; bmi = 2; }
-            // the reference's gates and commit on the stored costs
-            const double Js = (double)Ds + lambda * (double)(3 + (ncand > 1 ? 3 : 0));
-            double Jbest = Js;
-            const bool skip_wins = Js <= lambda * (double)FL_INTRA;
-            const bool pruned = tfl & 1;          // the proposer expects SKIP: no INTRA, no INTER8
-            double Ji = 1e30;
-            int iref = 0, sad16 = 0;
-            if (!skip_wins) {
-                const int bb = b16_0 + egbits(zmap(m0x - pmx)) + egbits(zmap(m0y - pmy));
-                const double J = (double)d16_0 + lambda * (double)bb;
-                if (J < Jbest) Jbest = J;
-                Ji = J; sad16 = s16_0;
-                if (q.navail > 1 && !(Jbest <= lambda * (double)FL_INTER16)) {
-                    const int bb1 = b16_1 + egbits(zmap(m1x - pmx)) + egbits(zmap(m1y - pmy));
-                    const double J1 = (double)d16_1 + lambda * (double)bb1;
-                    if (J1 < Jbest) Jbest = J1;
-                    if (J1 < Ji) { Ji = J1; iref = 1; sad16 = s16_1; }
-                }
-            }
-            double Jn = 1e30;
-            if (!skip_wins && !pruned && Jbest > lambda * (double)FL_INTRA) {
-                Jn = (double)pdn + lambda * (double)pbn;
-                if (Jn < Jbest) Jbest = Jn;
-            }
-            double J8 = 1e30;
-            if (!skip_wins && !pruned && Jbest > lambda * (double)FL_INTER8) {
-                if (!(sad16 > 0 && ssad * 10 >= sad16 * 9)) {
-                    int b8 = pb8, qpx = pmx, qpy = pmy;
-#pragma unroll
-                    for (int qd = 0; qd < 4; ++qd) {
-                        b8 += egbits(zmap(q8x[qd] - qpx)) + egbits(zmap(q8y[qd] - qpy));
-                        qpx = q8x[qd]; qpy = q8y[qd];
-                    }
-                    J8 = (double)pd8 + lambda * (double)b8;
-                }
-            }
-            int mode = 0;
-            double Jm = Js;
-            if (Ji < Jm) { Jm = Ji; mode = 1; }
-            if (Jn < Jm) { Jm = Jn; mode = 2; }
-            if (J8 < Jm) { Jm = J8; mode = 3; }
-            int nx = 0, ny = 0, nr = 0, ni = 1;
-            if (mode == 0) {
-                nx = bmi == 0 ? cx0 : bmi == 1 ? cx1 : cx2; ny = bmi == 0 ? cy0 : bmi == 1 ? cy1 : cy2;
-                nr = bmi == 0 ? 0 : bmi == 1 ? cr1 : cr2;
-            } else if (mode == 1) { nx = iref ? m1x : m0x; ny = iref ? m1y : m0y; nr = iref; }
-            else if (mode == 2) ni = 0;
-            else { nx = q8x[3]; ny = q8y[3]; }
-            if (L == 0) {
-                // the row below needs only the motion record: publish it first
-                cur_slot[mx] = mrec_pack(0, ni, nr, nx, ny);
-                __hip_atomic_store(&prog[wv], my * 65536 + mx + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
-                fdvg_mbinfo G;
-                G.mode = (uint8_t)mode; G.ref = (uint8_t)(mode == 0 || mode == 1 ? nr : 0);
-                G.merge_idx = (uint8_t)(mode == 0 ? bmi : 0); G.sub = (uint8_t)(mode == 2 ? FDVG_SUB_SCREEN : 0);
-                for (int k = 0; k < 4; ++k) { G.mv[k][0] = 0; G.mv[k][1] = 0; }
-                if (mode == 0 || mode == 1) { G.mv[0][0] = (int16_t)nx; G.mv[0][1] = (int16_t)ny; }
-                if (mode == 3) for (int k = 0; k < 4; ++k) { G.mv[k][0] = (int16_t)q8x[k]; G.mv[k][1] = (int16_t)q8y[k]; }
-                given[mb] = G;
-                fdvg_pctx X;
-                X.pmx = (int16_t)pmx; X.pmy = (int16_t)pmy; X.ncand = (uint8_t)ncand;
-                X.cx[0] = (int16_t)cx0; X.cy[0] = (int16_t)cy0; X.cr[0] = 0;
-                X.cx[1] = (int16_t)cx1; X.cy[1] = (int16_t)cy1; X.cr[1] = (uint8_t)cr1;
-                X.cx[2] = (int16_t)cx2; X.cy[2] = (int16_t)cy2; X.cr[2] = (uint8_t)cr2;
-                pctx[mb] = X;
-            }
-            if (mode != 2) { pmx = nx; pmy = ny; }
-        }
-    }
+    for (int my = wv; my < mh; my += NW)
+        replay_row<false>(pw[wv], q, pcs, pflag, ctl, my, ring + (size_t)(my % NW) * mwmax, &prog[wv],
+                          ring + (size_t)((my + NW - 1) % NW) * mwmax, &prog[(wv + NW - 1) % NW], nullptr, nullptr,
+                          nullptr, given, pctx, L);
 }
 
 // The intra MBs of the batch ordered by anti-diagonal (mx + my), all sequences
@@ -1309,13 +1375,14 @@ k_pencode(const fdvg_seq *__restrict__ seqs, const fdvg_mbinfo *__restrict__ giv
 // fdv.h:5201-5212): luma corner + top row into s.hbytes, left column into
 // s.lcol[0..15], chroma corners + top rows into s.ctop and left columns into
 // s.lcol[16..31].  Intra neighbours come from their handoff words, the others
-// (coded by an earlier kernel) from the reconstruction planes.
+// (coded by an earlier kernel) from the reconstruction planes; with gq NULL
+// every neighbour comes from its handoff words.
 __device__ __forceinline__ void nb_load_ih(Smem &s, const fdvg_seq &q, const fdvg_mbinfo *gq, const uint64_t *ih,
                                            unsigned *ctl, int mx, int my, int mw, int bx, int by) {
     const int tid = threadIdx.x, cbx = bx >> 1, cby = by >> 1, ht = my > 0, hl = mx > 0;
     const int i_top = (my - 1) * mw + mx, i_left = my * mw + mx - 1, i_tl = (my - 1) * mw + mx - 1;
-    const bool top_i = ht && gq[i_top].mode == 2, left_i = hl && gq[i_left].mode == 2;
-    const bool tl_i = ht && hl && gq[i_tl].mode == 2;
+    const bool top_i = ht && (!gq || gq[i_top].mode == 2), left_i = hl && (!gq || gq[i_left].mode == 2);
+    const bool tl_i = ht && hl && (!gq || gq[i_tl].mode == 2);
     const uint64_t *ih0 = ih + (size_t)q.mb0 * IH_WORDS;
     const unsigned step = q.step;
     const gmem_t<uint8_t> *py = gmem(q.oy), *pu = gmem(q.ou), *pv = gmem(q.ov);
@@ -1384,6 +1451,159 @@ k_pintra(const fdvg_seq *__restrict__ seqs, int nseq, const fdvg_mbinfo *__restr
         __syncthreads();                       // the chroma reconstruction is complete
         ih_publish(ih + (size_t)mb * IH_WORDS, yrec, s.crec, q.step);
     }
+}
+
+// The replay and the coding of its decisions in one launch (k_pdecide).  A
+// workgroup's role is its ticket, taken when it starts, so a lower ticket has
+// always started already:
+// - tickets 0 .. nunit - 1 replay: unit (seq, g) replays rows 4g .. 4g + 3 of its
+//   sequence, one wave a row; the first row reads the row above from the
+//   previous unit of its sequence (a lower ticket);
+// - the other workgroups are coding workers: they take MBs in the order of
+//   `corder` (by mx + 2 my, the order in which the replay decides them), wait for
+//   the MB's decision words, code it as k_pencode / k_pintra do, and publish its
+//   reconstruction's bottom row and right column as handoff words.  An intra MB
+//   waits for the handoff words of its left, top and top-left neighbours, all
+//   earlier in that order.
+// Every wait is on work of a lower ticket or an earlier job, so none can wait on
+// a workgroup that has not started; every wait is bounded and sets the error
+// flag on timeout.  Rows at most PD_MAXMW MBs wide.
+#define PD_MAXMW 256
+#define PD_ROWS 4
+struct __align__(16) PdRep {
+    uint64_t ring[PD_ROWS][PD_MAXMW];
+    uint64_t above0[PD_MAXMW];            // the first row's row above, from the previous unit
+    int prog[PD_ROWS];
+    PrWave pw[PD_ROWS];
+};
+union PdLds {
+    Smem s;
+    PdRep r;
+};
+
+extern "C" __global__ void __launch_bounds__(256)
+k_pdecide(const fdvg_seq *__restrict__ seqs, int nseq, const int2 *__restrict__ units, int nunit,
+          const unsigned *__restrict__ corder, int total_mb, unsigned *ctl, const fdvg_pcost *__restrict__ pcs,
+          fdvg_mbinfo *__restrict__ given, fdvg_pctx *__restrict__ pctx, uint64_t *__restrict__ mrec,
+          uint64_t *__restrict__ decw, uint64_t *__restrict__ ih, fdvg_mbrec *__restrict__ recs) {
+    __shared__ PdLds U;
+    __shared__ unsigned role_s;
+    __shared__ union {
+        uint32_t w[PD_WORDS];
+        struct { fdvg_mbinfo g; fdvg_pctx x; } d;
+    } decb;
+    const int tid = threadIdx.x, wv = tid >> 6, L = tid & 63;
+    if (tid == 0) role_s = atomicAdd(ctl + 6, 1u);
+    __syncthreads();
+    const unsigned role = role_s;
+    if (role < (unsigned)nunit) {
+        const int2 u = units[role];
+        const fdvg_seq q = seqs[u.x];
+        const int mw = q.w >> 4, mh = q.h >> 4, my = PD_ROWS * u.y + wv;
+        if (L == 0) __hip_atomic_store(&U.r.prog[wv], -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __syncthreads();
+        if (my >= mh) return;
+        const bool first = wv == 0;
+        replay_row<true>(U.r.pw[wv], q, pcs, nullptr, ctl, my, U.r.ring[wv], &U.r.prog[wv],
+                         first ? U.r.above0 : U.r.ring[wv - 1], first ? nullptr : &U.r.prog[wv - 1],
+                         first && my > 0 ? mrec + q.mb0 + (size_t)(my - 1) * mw : nullptr, mrec, decw, given, pctx, L);
+        return;
+    }
+    __shared__ int job_s;
+    Smem &s = U.s;
+    for (;;) {
+        if (tid == 0) job_s = (int)atomicAdd(ctl + 7, 1u);
+        __syncthreads();
+        const int job = job_s;
+        __syncthreads();
+        if (job >= total_mb) return;
+        const int mb = (int)corder[job];
+        int sq = 0;
+        for (int k = 1; k < nseq; ++k) if (mb >= seqs[k].mb0) sq = k;
+        fdvg_seq q = seqs[sq];
+        q.given = nullptr;
+        const int i = mb - q.mb0, mw = q.w >> 4, mx = i % mw, my = i / mw, bx = mx * 16, by = my * 16;
+        if (tid < PD_WORDS) {
+            const uint64_t *w = decw + (size_t)mb * PD_WORDS + tid;
+            uint64_t v = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            for (unsigned spins = 0; (unsigned)(v >> 32) != q.step; ++spins) {
+                if (spins > (1u << 21)) { atomicOr(ctl + 1, 1u); break; }
+                __builtin_amdgcn_s_sleep(2);
+                v = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            decb.w[tid] = (uint32_t)v;
+        }
+        __syncthreads();
+        const fdvg_mbinfo &G = decb.d.g;
+        const fdvg_pctx &X = decb.d.x;
+        const int gmode = G.mode;
+        mb_stage(s, q, mb, bx, by, gmode);
+        if (gmode == 2) nb_load_ih(s, q, nullptr, ih, ctl, mx, my, mw, bx, by);
+        MbCtx c;
+        ctx_from(c, X, my > 0, mx > 0);
+        __syncthreads();
+        int par = 0;
+        MbDec D;
+        mb_decide(s, q, c, &G, nullptr, par, D);
+        const uint8_t *yrec = mb_finish(s, q, mb, bx, by, c, D, &recs[mb]);
+        __syncthreads();                       // the chroma reconstruction is complete
+        ih_publish(ih + (size_t)mb * IH_WORDS, yrec, s.crec, q.step);
+    }
+}
+static_assert(sizeof(fdvg_mbinfo) == 20 && sizeof(fdvg_pctx) == 20 && alignof(fdvg_pctx) <= 4,
+              "decision words: 5 + 5 payload words");
+
+// Replay units and coding order of k_pdecide for a batch: units[] = (seq, g)
+// for every PD_ROWS-row group, ordered by g then seq (a unit's row above is in a
+// lower-numbered unit); corder[] = every MB, ordered by mx + 2 my (then
+// sequence, then row), the order in which the replay decides them.  Returns the
+// unit count, or -1 if a row is wider than PD_MAXMW MBs.
+extern "C" int fdvg_pdecide_plan(const int *w, const int *h, const int *mb0, int nseq, int2 *units,
+                                 unsigned *corder) {
+    int nunit = 0, maxg = 0, maxt = 0, total = 0;
+    for (int k = 0; k < nseq; ++k) {
+        const int mw = w[k] >> 4, mh = h[k] >> 4;
+        if (mw > PD_MAXMW) return -1;
+        const int ng = (mh + PD_ROWS - 1) / PD_ROWS;
+        if (ng > maxg) maxg = ng;
+        if (mw - 1 + 2 * (mh - 1) > maxt) maxt = mw - 1 + 2 * (mh - 1);
+        total += mw * mh;
+    }
+    for (int gi = 0; gi < maxg; ++gi)
+        for (int k = 0; k < nseq; ++k)
+            if (gi * PD_ROWS < (h[k] >> 4)) units[nunit++] = make_int2(k, gi);
+    int n = 0;
+    for (int t = 0; t <= maxt; ++t)
+        for (int k = 0; k < nseq; ++k) {
+            const int mw = w[k] >> 4, mh = h[k] >> 4;
+            for (int my = 0; my < mh; ++my) {
+                const int mx = t - 2 * my;
+                if (mx >= 0 && mx < mw) corder[n++] = (unsigned)(mb0[k] + my * mw + mx);
+            }
+        }
+    return n == total ? nunit : -1;
+}
+
+extern "C" int fdvg_pdecide_grid(int nunit, int total_mb, int workers_per_cu) {
+    static int ncu = 0, per = 0;
+    if (!ncu) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess) return -1;
+        if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return -1;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void *)k_pdecide, 256, 0) != hipSuccess) return -1;
+    }
+    int wk = ncu * (workers_per_cu < per ? workers_per_cu : per);
+    if (wk < 1) wk = 1;
+    return nunit + (wk < total_mb ? wk : total_mb);
+}
+
+extern "C" int fdvg_launch_pdecide(const fdvg_seq *seqs, int nseq, const int2 *units, int nunit, const unsigned *corder,
+                                   int total_mb, int grid, unsigned *ctl, const fdvg_pcost *pcs, fdvg_mbinfo *given,
+                                   fdvg_pctx *pctx, uint64_t *mrec, uint64_t *decw, uint64_t *ih, fdvg_mbrec *recs,
+                                   hipStream_t st) {
+    hipLaunchKernelGGL(k_pdecide, dim3(grid), dim3(256), 0, st, seqs, nseq, units, nunit, corder, total_mb, ctl, pcs,
+                       given, pctx, mrec, decw, ih, recs);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 // The intra MBs of a batch in dependency order (k_plist), for other passes

@@ -36,6 +36,7 @@ struct Tune {
     int sym_waves;      // serialiser MBs (wavefronts) per workgroup: 1, 2, 4, 8
     int deblock_bs;     // deblock threads per workgroup: 64, 128, 256
     int nn_wpc;         // proposer workgroups per CU: 1, 2
+    int pd_wpc;         // coding workers of the fused decision stage per CU: 1, 2
 };
 extern "C" int fdvg_launch_metab(const fdvg_seq *, int, int, void *, const void *, float, float, hipStream_t);
 extern "C" int fdvg_launch_nn_infer(const fdvg_seq *, int, int, const float *, fdvg_nnout *, int, hipStream_t);
@@ -52,6 +53,11 @@ extern "C" int fdvg_launch_prepair(const fdvg_seq *, int, int, int, unsigned *, 
                                    fdvg_mbinfo *, fdvg_pctx *, unsigned *, hipStream_t);
 extern "C" int fdvg_launch_pfinish(const fdvg_seq *, int, int, unsigned *, fdvg_mbinfo *, fdvg_pctx *, unsigned *,
                                    uint64_t *, int, fdvg_mbrec *, hipStream_t);
+extern "C" int fdvg_pdecide_plan(const int *, const int *, const int *, int, int2 *, unsigned *);
+extern "C" int fdvg_pdecide_grid(int, int, int);
+extern "C" int fdvg_launch_pdecide(const fdvg_seq *, int, const int2 *, int, const unsigned *, int, int, unsigned *,
+                                   const fdvg_pcost *, fdvg_mbinfo *, fdvg_pctx *, uint64_t *, uint64_t *, uint64_t *,
+                                   fdvg_mbrec *, hipStream_t);
 
 static thread_local char g_err[256];
 static void seterr(const char *fmt, ...) {
@@ -112,17 +118,18 @@ struct fdvgpu {
     uint64_t *ih_d;                 // intra handoff words, 16 per MB (bottom row, right column)
     unsigned *pflag_d;              // per MB: the id of the frame step whose costs are written
     unsigned step_ctr;              // frame-step ids (never rewound)
-    // The decision stage overlaps the costs and the replay: the replay's
-    // workgroups (one per sequence) get CUs of their own through CU-masked
-    // streams, so they run while the costs fill the rest of the chip.
-    hipStream_t st_cost, st_rep;
-    hipEvent_t ev_fork, ev_cost, ev_rep;
+    // The fused decision stage (k_pdecide): the replay and the coding of its
+    // decisions in one launch, so the coding follows the replay MB by MB.
+    int2 *units_d;                  // replay units (sequence, row group)
+    unsigned *corder_d;             // the MBs in the order the replay decides them
+    uint64_t *decw_d;               // per MB: the decision and its context as tagged words
+    int nunit;
     int overlap, overlap_avail;
     // launch shapes, and one instantiated graph per (search mode, decision structure)
     Tune tune;
     int graphs;
-    hipGraphExec_t gexec[64];       // per (stage, search mode, decision structure)
-    int warm[64];                   // the key has run once eagerly (launch caches filled)
+    hipGraphExec_t gexec[128];      // per (stage, search mode, decision structure, fused stage)
+    int warm[128];                  // the key has run once eagerly (launch caches filled)
     // learned proposer
     float *nnw_d;
     fdvg_nnout *nn_d;
@@ -177,15 +184,11 @@ extern "C" void fdvgpu_close(fdvgpu *g) {
     (void)hipFree(g->pcost_d); (void)hipFree(g->pctx_d); (void)hipFree(g->pgiven_d); (void)hipFree(g->mrec_d);
     (void)hipFree(g->ilist_d); (void)hipFree(g->ih_d);
     (void)hipFree(g->nnw_d); (void)hipFree(g->nn_d);
-    for (int k = 0; k < 64; ++k) if (g->gexec[k]) (void)hipGraphExecDestroy(g->gexec[k]);
+    for (int k = 0; k < 128; ++k) if (g->gexec[k]) (void)hipGraphExecDestroy(g->gexec[k]);
     for (int i = 0; i < 5; ++i) if (g->ev[i]) (void)hipEventDestroy(g->ev[i]);
     for (int i = 0; i < 8; ++i) if (g->ring_ev[i]) (void)hipEventDestroy(g->ring_ev[i]);
     if (g->st) (void)hipStreamDestroy(g->st);
-    if (g->st_cost) (void)hipStreamDestroy(g->st_cost);
-    if (g->st_rep) (void)hipStreamDestroy(g->st_rep);
-    if (g->ev_fork) (void)hipEventDestroy(g->ev_fork);
-    if (g->ev_cost) (void)hipEventDestroy(g->ev_cost);
-    if (g->ev_rep) (void)hipEventDestroy(g->ev_rep);
+    (void)hipFree(g->units_d); (void)hipFree(g->corder_d); (void)hipFree(g->decw_d);
     (void)hipFree(g->pflag_d);
     (void)hipFree(g->edge_d);
     for (int k = 0; k < 2; ++k) {
@@ -241,14 +244,6 @@ extern "C" fdvgpu *fdvgpu_open(int nseq, const int *w, const int *h, int nsrc) {
         g->compact_cap += (size_t)m.nmb * 2048 + 4096;
     }
     CKN(hipStreamCreate(&g->st));
-    // The overlapped decision stage (costs and replay on CU-masked streams) is
-    // switched off: its first device run ended in a host segmentation fault
-    // whose cause has not been found.  The streams are not created and the stage
-    // runs its kernels one after the other.
-    g->overlap = g->overlap_avail = 0;
-    CKN(hipEventCreateWithFlags(&g->ev_fork, hipEventDisableTiming));
-    CKN(hipEventCreateWithFlags(&g->ev_cost, hipEventDisableTiming));
-    CKN(hipEventCreateWithFlags(&g->ev_rep, hipEventDisableTiming));
     for (int i = 0; i < 5; ++i) CKN(hipEventCreate(&g->ev[i]));
     for (int i = 0; i < 8; ++i) CKN(hipEventCreateWithFlags(&g->ring_ev[i], hipEventDisableTiming));
     CKN(hipMalloc(&g->seq_d, nseq * sizeof(fdvg_seq)));
@@ -283,9 +278,32 @@ extern "C" fdvgpu *fdvgpu_open(int nseq, const int *w, const int *h, int nsrc) {
     CKN(hipMemset(g->ih_d, 0, (size_t)g->total_mb * 16 * sizeof(uint64_t)));   // id 0: never a step
     CKN(hipMalloc(&g->pflag_d, (size_t)g->total_mb * sizeof(unsigned)));
     CKN(hipMemset(g->pflag_d, 0, (size_t)g->total_mb * sizeof(unsigned)));
+    {
+        // the fused decision stage's plan (rows up to its width limit; wider
+        // batches run the stage as separate launches)
+        int *wv = (int *)malloc(3 * nseq * sizeof(int));
+        int2 *units_h = (int2 *)malloc(((size_t)g->nrows + nseq) * sizeof(int2));
+        unsigned *corder_h = (unsigned *)malloc((size_t)g->total_mb * sizeof(unsigned));
+        if (!wv || !units_h || !corder_h) { free(wv); free(units_h); free(corder_h); seterr("out of host memory"); goto fail; }
+        for (int k = 0; k < nseq; ++k) { wv[k] = g->S[k].w; wv[nseq + k] = g->S[k].h; wv[2 * nseq + k] = g->S[k].mb0; }
+        g->nunit = fdvg_pdecide_plan(wv, wv + nseq, wv + 2 * nseq, nseq, units_h, corder_h);
+        g->overlap = g->overlap_avail = g->nunit > 0;
+        hipError_t e = hipSuccess;
+        if (g->overlap_avail) {
+            e = hipMalloc(&g->units_d, (size_t)g->nunit * sizeof(int2));
+            if (e == hipSuccess) e = hipMalloc(&g->corder_d, (size_t)g->total_mb * sizeof(unsigned));
+            if (e == hipSuccess) e = hipMalloc(&g->decw_d, (size_t)g->total_mb * 10 * sizeof(uint64_t));
+            if (e == hipSuccess) e = hipMemset(g->decw_d, 0, (size_t)g->total_mb * 10 * sizeof(uint64_t));
+            if (e == hipSuccess) e = hipMemcpy(g->units_d, units_h, (size_t)g->nunit * sizeof(int2), hipMemcpyHostToDevice);
+            if (e == hipSuccess)
+                e = hipMemcpy(g->corder_d, corder_h, (size_t)g->total_mb * sizeof(unsigned), hipMemcpyHostToDevice);
+        }
+        free(wv); free(units_h); free(corder_h);
+        if (e != hipSuccess) { seterr("fused decision stage: %s", hipGetErrorString(e)); goto fail; }
+    }
     CKN(hipMalloc(&g->nnw_d, (size_t)fdvg_nn_nparam() * sizeof(float)));
     if (fdvg_nn_setup() != 0) { seterr("proposer kernel setup failed"); goto fail; }
-    g->tune = Tune{4, 256, 1};
+    g->tune = Tune{4, 256, 1, 2};
     g->graphs = 1;
     g->nn_thr = FDVG_NN_THRESHOLD;
     g->nn_sthr = FDVG_NN_SKIP_OFF;
@@ -395,25 +413,14 @@ static int launch_stage(fdvgpu *g, int stage, int me_mode, bool par) {
         if (fdvg_launch_zero(g->ticket_d, 8, g->st) != 0) { seterr("ticket reset launch failed"); return -1; }
     } else if (stage == 1) {
         if (par && g->overlap) {
-            // costs and replay side by side on their CU-masked streams, joined
-            // back before the coding of the decisions
-            if (fdvg_launch_zero(g->ilist_d, 1, g->st) != 0) {
-                seterr("parallel analysis launch failed"); return -1;
-            }
-            CK(hipEventRecord(g->ev_fork, g->st));
-            CK(hipStreamWaitEvent(g->st_cost, g->ev_fork, 0));
-            CK(hipStreamWaitEvent(g->st_rep, g->ev_fork, 0));
-            if (fdvg_launch_pcost(g->seq_d, g->nseq, g->xgrid, g->pcost_d, g->edge_d, g->pflag_d, g->st_cost) != 0 ||
-                fdvg_launch_prepair(g->seq_d, g->nseq, g->maxmh, g->maxmw, g->ticket_d, g->pcost_d, g->pflag_d,
-                                    g->pgiven_d, g->pctx_d, g->ilist_d, g->st_rep) != 0) {
-                seterr("parallel analysis launch failed: %s", hipGetErrorString(hipGetLastError())); return -1;
-            }
-            CK(hipEventRecord(g->ev_cost, g->st_cost));
-            CK(hipEventRecord(g->ev_rep, g->st_rep));
-            CK(hipStreamWaitEvent(g->st, g->ev_cost, 0));
-            CK(hipStreamWaitEvent(g->st, g->ev_rep, 0));
-            if (fdvg_launch_pfinish(g->seq_d, g->nseq, g->xgrid, g->ticket_d, g->pgiven_d, g->pctx_d, g->ilist_d,
-                                    g->ih_d, g->total_mb, g->recs_d, g->st) != 0) {
+            // the costs, then the replay and the coding of its decisions in
+            // one launch (the coding follows the replay MB by MB)
+            const int grid = fdvg_pdecide_grid(g->nunit, g->total_mb, g->tune.pd_wpc);
+            if (grid <= 0 ||
+                fdvg_launch_pcost(g->seq_d, g->nseq, g->xgrid, g->pcost_d, g->edge_d, nullptr, g->st) != 0 ||
+                fdvg_launch_pdecide(g->seq_d, g->nseq, g->units_d, g->nunit, g->corder_d, g->total_mb, grid,
+                                    g->ticket_d, g->pcost_d, g->pgiven_d, g->pctx_d, g->mrec_d, g->decw_d, g->ih_d,
+                                    g->recs_d, g->st) != 0) {
                 seterr("parallel analysis launch failed: %s", hipGetErrorString(hipGetLastError())); return -1;
             }
         } else if (par) {
@@ -441,10 +448,7 @@ static int launch_stage(fdvgpu *g, int stage, int me_mode, bool par) {
 // One stage: eagerly the first time its key is seen (which also fills the
 // launchers' one-time caches), from then on as a captured hipGraph.
 static int run_stage(fdvgpu *g, int stage, int me_mode, bool par) {
-    const int key = (stage * 4 + me_mode) * 2 + (par ? 1 : 0);
-    // the overlapped decision stage runs eagerly: its CU masks belong to its
-    // streams, which a captured graph would not keep
-    if (stage == 1 && par && g->overlap) return launch_stage(g, stage, me_mode, par);
+    const int key = ((stage * 4 + me_mode) * 2 + (par ? 1 : 0)) * 2 + (stage == 1 && par && g->overlap ? 1 : 0);
     if (!g->graphs || !g->warm[key]) {
         if (launch_stage(g, stage, me_mode, par) != 0) return -1;
         g->warm[key] = 1;
@@ -820,7 +824,9 @@ extern "C" int fdvgpu_planes(fdvgpu *g, int seq, int src_slot, const uint8_t **c
 
 extern "C" int fdvgpu_set_overlap(fdvgpu *g, int on) {
     if (!g) { seterr("fdvgpu_set_overlap: bad arguments"); return -1; }
-    if (on && !g->overlap_avail) { seterr("fdvgpu_set_overlap: the overlapped decision stage is switched off"); return -1; }
+    if (on && !g->overlap_avail) {
+        seterr("fdvgpu_set_overlap: the fused decision stage takes rows of at most 256 MBs"); return -1;
+    }
     g->overlap = on ? 1 : 0;
     return 0;
 }
@@ -832,19 +838,23 @@ extern "C" int fdvgpu_set_graphs(fdvgpu *g, int on) {
 }
 
 static void drop_graphs(fdvgpu *g) {
-    for (int k = 0; k < 64; ++k)
+    for (int k = 0; k < 128; ++k)
         if (g->gexec[k]) { (void)hipGraphExecDestroy(g->gexec[k]); g->gexec[k] = NULL; }
 }
 
-extern "C" int fdvgpu_get_tune(fdvgpu *g, int out[3]) {
+extern "C" int fdvgpu_get_tune(fdvgpu *g, int out[4]) {
     if (!g || !out) { seterr("fdvgpu_get_tune: bad arguments"); return -1; }
-    out[0] = g->tune.sym_waves; out[1] = g->tune.deblock_bs; out[2] = g->tune.nn_wpc;
+    out[0] = g->tune.sym_waves; out[1] = g->tune.deblock_bs; out[2] = g->tune.nn_wpc; out[3] = g->tune.pd_wpc;
     return 0;
 }
 
-extern "C" int fdvgpu_set_tune(fdvgpu *g, const int in[3]) {
+extern "C" int fdvgpu_set_tune(fdvgpu *g, const int in[4]) {
     if (!g || !in) { seterr("fdvgpu_set_tune: bad arguments"); return -1; }
-    g->tune = Tune{in[0], in[1], in[2]};
+    if (in[0] <= 0 || in[0] > 8 || (in[1] != 64 && in[1] != 128 && in[1] != 256) || in[2] < 1 || in[2] > 2 ||
+        in[3] < 1 || in[3] > 2) {
+        seterr("fdvgpu_set_tune: shapes out of range"); return -1;
+    }
+    g->tune = Tune{in[0], in[1], in[2], in[3]};
     drop_graphs(g);
     return 0;
 }
@@ -855,16 +865,19 @@ extern "C" int fdvgpu_set_tune(fdvgpu *g, const int in[3]) {
 extern "C" int fdvgpu_autotune(fdvgpu *g, const int *src_slot, const int *qp, const double *lscale,
                                const int *navail, int me_mode, int reps) {
     if (!g || reps <= 0) { seterr("fdvgpu_autotune: bad arguments"); return -1; }
-    static const int opts[3][3] = {{2, 4, 8}, {64, 128, 256}, {1, 2, 0}};
-    for (int prm = 0; prm < 3; ++prm) {
+    static const int opts[4][3] = {{2, 4, 8}, {64, 128, 256}, {1, 2, 0}, {1, 2, 0}};
+    const bool par = me_mode == FDVGPU_ME_LEARNED || me_mode == FDVGPU_ME_PARALLEL;
+    for (int prm = 0; prm < 4; ++prm) {
         if (prm == 2 && me_mode != FDVGPU_ME_LEARNED) continue;
+        if (prm == 3 && !(par && g->overlap)) continue;
         float best = 1e30f;
         int bestv = 0;
         for (int k = 0; k < 3 && opts[prm][k]; ++k) {
             Tune t = g->tune;
             if (prm == 0) t.sym_waves = opts[0][k];
             else if (prm == 1) t.deblock_bs = opts[1][k];
-            else t.nn_wpc = opts[2][k];
+            else if (prm == 2) t.nn_wpc = opts[2][k];
+            else t.pd_wpc = opts[3][k];
             g->tune = t;
             drop_graphs(g);
             float tot = 0.f;
@@ -880,7 +893,8 @@ extern "C" int fdvgpu_autotune(fdvgpu *g, const int *src_slot, const int *qp, co
         }
         if (prm == 0) g->tune.sym_waves = bestv;
         else if (prm == 1) g->tune.deblock_bs = bestv;
-        else g->tune.nn_wpc = bestv;
+        else if (prm == 2) g->tune.nn_wpc = bestv;
+        else g->tune.pd_wpc = bestv;
         drop_graphs(g);
     }
     g->have_frame = 0;

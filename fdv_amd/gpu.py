@@ -217,7 +217,8 @@ class Analyser:
         self._ck(self.L.fdvgpu_set_proposer_thresholds(self.h, float(conf), float(skip)), "set_proposer_thresholds")
 
     def set_overlap(self, on: bool):
-        """Overlap the block-parallel decision stage's costs and replay."""
+        """The fused block-parallel decision stage (replay and coding in one
+        launch; the default) or separate launches."""
         self._ck(self.L.fdvgpu_set_overlap(self.h, 1 if on else 0), "set_overlap")
 
     def set_graphs(self, on: bool):
@@ -225,10 +226,11 @@ class Analyser:
 
     def tune(self, shapes=None):
         """Get (or set) the launch shapes: [serialiser waves per workgroup,
-        deblock workgroup size, proposer workgroups per CU]."""
+        deblock workgroup size, proposer workgroups per CU, decision coding
+        workers per CU]."""
         if shapes is not None:
             self._ck(self.L.fdvgpu_set_tune(self.h, _ints(shapes)), "set_tune")
-        out = (C.c_int * 3)()
+        out = (C.c_int * 4)()
         self._ck(self.L.fdvgpu_get_tune(self.h, out), "get_tune")
         return list(out)
 

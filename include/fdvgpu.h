@@ -170,18 +170,23 @@ int fdvgpu_decode(fdvgpu *g, const fdvgpu_mbcode *const *codes, const int *qp, c
  * copies is a fixed launch sequence per (search mode, decision structure); from
  * its second use it is captured into a hipGraph and replayed (on by default).
  * Launch shapes (serialiser wavefronts per workgroup, deblock workgroup size,
- * proposer workgroups per CU) are chosen by fdvgpu_autotune, which times each
+ * proposer workgroups per CU, decision coding workers per CU) are chosen by fdvgpu_autotune, which times each
  * candidate on a real frame step (rewinding the reference pool after each) and
  * keeps the fastest.  None of this has a counterpart in the reference. */
 int fdvgpu_set_graphs(fdvgpu *g, int on);
 
-/* Overlap of the block-parallel decision stage: the candidate costs on most CUs
- * while the decision replay, waiting per MB for its costs, runs on CUs reserved
- * for it.  Currently switched off in this build: on = 1 returns -1 and the
- * stage runs its kernels one after the other.  No counterpart in the reference. */
+/* The fused block-parallel decision stage (on by default where every row is at
+ * most 256 MBs wide): the decision replay and the coding of its decisions in
+ * one launch, each MB coded as soon as its decision (and, for INTRA, its
+ * neighbours' reconstruction) exists.  Off: the replay, then the coding, as
+ * separate launches.  Both give the same results.  on = 1 returns -1 where the
+ * fused stage is unavailable.  No counterpart in the reference. */
 int fdvgpu_set_overlap(fdvgpu *g, int on);
-int fdvgpu_get_tune(fdvgpu *g, int shapes[3]);
-int fdvgpu_set_tune(fdvgpu *g, const int shapes[3]);
+/* shapes: serialiser wavefronts per workgroup {1..8}, deblock workgroup size
+ * {64, 128, 256}, proposer workgroups per CU {1, 2}, coding workers of the
+ * fused decision stage per CU {1, 2}. */
+int fdvgpu_get_tune(fdvgpu *g, int shapes[4]);
+int fdvgpu_set_tune(fdvgpu *g, const int shapes[4]);
 int fdvgpu_autotune(fdvgpu *g, const int *src_slot, const int *qp, const double *lscale,
                     const int *navail, int me_mode, int reps);
 
