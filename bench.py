@@ -237,9 +237,9 @@ def main():
     ap.add_argument("--end-to-end", action="store_true",
                     help="also time the drop-in encoder end to end (source upload, device analysis, "
                          "stream fetch, host entropy coding) on the same frames")
-    ap.add_argument("--no-overlap", action="store_true",
-                    help="run the block-parallel decision stage's replay and coding as separate launches "
-                         "instead of the fused stage")
+    ap.add_argument("--stage", type=int, default=None, choices=[0, 1, 2],
+                    help="structure of the block-parallel decision stage: 0 separate launches (costs, replay, "
+                         "coding), 1 costs then replay + coding fused, 2 all fused (the default)")
     ap.add_argument("--conf", type=float, default=None, help="proposer confidence threshold (default 0.7)")
     ap.add_argument("--skip", type=float, default=None, help="proposer SKIP threshold (default: off; 0.7 prunes)")
     ap.add_argument("--baseline-steps", type=int, default=10,
@@ -281,8 +281,8 @@ def main():
 
     def open_analyser(m):
         a = G.Analyser(sizes, nsrc=nframes)
-        if args.no_overlap:
-            a.set_overlap(False)
+        if args.stage is not None:
+            a.set_overlap(args.stage)
         if m == G.ME_LEARNED:
             a.set_proposer()
             if args.conf is not None or args.skip is not None:

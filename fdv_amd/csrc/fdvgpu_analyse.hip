@@ -713,31 +713,24 @@ k_analyse(const fdvg_seq *__restrict__ seqs, const int2 *__restrict__ rowtab, in
 // ---------------------------------------------------- block-parallel decisions
 //
 // The parallel and learned modes replace the decision wavefront by
-//   k_pcost    every MB's candidate costs except INTRA, and its stand-in
-//              reconstruction edges, all MBs at once;
-//   k_picost   every MB's INTRA cost from its neighbours' stand-in edges;
+//   k_pcosts   every MB's candidate costs, INTRA from its neighbours'
+//              stand-in reconstruction edges (persistent workgroups, MBs in
+//              the replay's order, costs published as tagged words);
 //   k_prepair  the reference's decision logic (fdv.h:5847-6040) replayed in a
 //              light wavefront (one wavefront per MB row, motion records only)
 //              on those costs, with the true vector predictor and merge list;
 //   k_pencode  the decided non-intra MBs coded exactly, all at once;
 //   k_pintra   the intra MBs coded in dependency order (an intra MB waits only
-//              for intra neighbours; inter ones are already reconstructed).
+//              for intra neighbours; inter ones are already reconstructed);
+//   k_pdecide  k_prepair, k_pencode and k_pintra in one launch, which may run
+//              at the same time as k_pcosts (its replay waits for the costs'
+//              tags).
 // The one approximation is that intra costs are computed from stand-in
-// neighbours (k_pcost) instead of the reconstruction, which exists only once
-// every decision is made; the intra sub-mode is screened again at coding time on
-// the true reconstruction.
+// neighbours instead of the reconstruction, which exists only once every
+// decision is made; the intra sub-mode is screened again at coding time on the
+// true reconstruction.
 
 __device__ __forceinline__ int mb_count(const fdvg_seq &q) { return (q.w >> 4) * (q.h >> 4); }
-
-// An MB's costs are complete: every wave's stores are released at agent scope,
-// then one store of the frame-step id (the replay may run concurrently).  No
-// flags (nullptr): the replay runs after this kernel, nothing to publish.
-__device__ __forceinline__ void pcost_publish(unsigned *flag, unsigned step) {
-    if (!flag) return;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    __syncthreads();
-    if (threadIdx.x == 0) __hip_atomic_store(flag, step, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-}
 
 // The costs of every MB's candidates except INTRA, and the MB's stand-in
 // reconstruction edges for its neighbours' intra costs (k_picost): the true
@@ -750,20 +743,18 @@ __device__ __forceinline__ void pcost_publish(unsigned *flag, unsigned step) {
 //   - the stand-in is the co-located reference-0 block if Js <= 14 lambda (the
 //     INTRA floor) or Js <= the best INTER16 J, else that INTER16 reconstruction.
 // edge[mb * 32 ..]: the stand-in's bottom row (16), then its right column (16).
-extern "C" __global__ void __launch_bounds__(256, 4)
-k_pcost(const fdvg_seq *__restrict__ seqs, fdvg_pcost *__restrict__ pcs, uint8_t *__restrict__ edge) {
-    __shared__ Smem s;
-    __shared__ int32_t sred[FDVG_NSC][4];
-    __shared__ uint8_t sedge[32];
-    fdvg_seq q = seqs[blockIdx.y];
-    int i;
-    if (!xcd_mb(blockIdx.x, q.w >> 4, q.h >> 4, i)) return;
-    q.given = nullptr;
+#define PC_WORDS 36                      // a MB's costs as tagged words (4 payload bytes each)
+#define EW_WORDS 8                       // a MB's stand-in edges as tagged words
+static_assert(sizeof(fdvg_pcost) == 4 * PC_WORDS, "cost words");
+
+// The costs are built in LDS (P) and published by k_pcosts as tagged words;
+// the stand-in edges go to sout (LDS).  q.given must be NULL.
+__device__ __forceinline__ void pcost_mb(Smem &s, int32_t (*sred)[4], uint8_t *sedge, uint8_t *sout,
+                                         const fdvg_seq &q, int i, fdvg_pcost *P) {
     const int tid = threadIdx.x, wave = tid >> 6;
     const int mw = q.w >> 4, mx = i % mw, my = i / mw, bx = mx * 16, by = my * 16;
     const int mb = q.mb0 + i;
     const gmem_t<fdvg_metab> *T = gmem(&q.metab[i]);
-    fdvg_pcost *P = &pcs[mb];
     mb_stage(s, q, mb, bx, by);
     __syncthreads();
 
@@ -837,7 +828,7 @@ k_pcost(const fdvg_seq *__restrict__ seqs, fdvg_pcost *__restrict__ pcs, uint8_t
         const bool skip = js <= lambda * 14.0 || js <= bj;
         if (tid < 32) {
             const int y = tid < 16 ? 15 : tid - 16, x = tid < 16 ? tid : 15;
-            edge[(size_t)mb * 32 + tid] = skip ? s.win[0][(WOFF + y) * WIN + WOFF + x] : sedge[tid];
+            sout[tid] = skip ? s.win[0][(WOFF + y) * WIN + WOFF + x] : sedge[tid];
         }
     }
     // INTER8 is evaluated only when the 16x16 SAD of the chosen reference (either
@@ -866,30 +857,25 @@ k_pcost(const fdvg_seq *__restrict__ seqs, fdvg_pcost *__restrict__ pcs, uint8_t
     }
 }
 
-// INTRA 16x16 costs of every MB from its neighbours' stand-in edges (k_pcost):
-// SATD screen of the four modes, then one coding of the best.  Publishes the
-// MB's costs as complete (the replay may run concurrently).
-extern "C" __global__ void __launch_bounds__(256, 4)
-k_picost(const fdvg_seq *__restrict__ seqs, fdvg_pcost *__restrict__ pcs, const uint8_t *__restrict__ edge,
-         unsigned *__restrict__ pflag) {
-    __shared__ Smem s;
-    const fdvg_seq q = seqs[blockIdx.y];
-    int i;
-    if (!xcd_mb(blockIdx.x, q.w >> 4, q.h >> 4, i)) return;
+// INTRA 16x16 cost of one MB from its neighbours' stand-in edges (edge words,
+// EW_WORDS tagged words a MB: bottom row, then right column, 4 bytes a word;
+// waited for): SATD screen of the four modes, then one coding of the best.
+__device__ __forceinline__ void picost_mb(Smem &s, const fdvg_seq &q, int i, fdvg_pcost *P, const uint64_t *edgew,
+                                          unsigned *ctl) {
     const int tid = threadIdx.x, wave = tid >> 6;
     const int mw = q.w >> 4, mx = i % mw, my = i / mw, bx = mx * 16, by = my * 16;
-    const int mb = q.mb0 + i;
-    fdvg_pcost *P = &pcs[mb];
     if (gmem(q.metab)[i].flags & 1) {        // the proposer expects SKIP: no INTRA
         if (tid == 0) { P->dn = P->bn = 0; P->nsub = 0; }
-        pcost_publish(pflag ? pflag + mb : nullptr, q.step);
         return;
     }
     s.cur[tid] = gmem(q.cy)[(size_t)(by + (tid >> 4)) * q.w + bx + (tid & 15)];
-    const uint8_t *E = edge + (size_t)q.mb0 * 32;
-    if (tid < 16) s.hbytes[1 + tid] = my > 0 ? E[(size_t)(i - mw) * 32 + tid] : 0;
-    else if (tid == 16) s.hbytes[0] = (my > 0 && mx > 0) ? E[(size_t)(i - mw - 1) * 32 + 15] : 128;
-    else if (tid >= 32 && tid < 48) s.lcol[tid - 32] = mx > 0 ? E[(size_t)(i - 1) * 32 + 16 + tid - 32] : 0;
+    const uint64_t *E = edgew + (size_t)q.mb0 * EW_WORDS;
+    const unsigned st = q.step;
+    if (tid < 16) s.hbytes[1 + tid] = my > 0 ? ih_byte(E + (size_t)(i - mw) * EW_WORDS, tid >> 2, tid & 3, st, ctl) : 0;
+    else if (tid == 16)
+        s.hbytes[0] = (my > 0 && mx > 0) ? ih_byte(E + (size_t)(i - mw - 1) * EW_WORDS, 3, 3, st, ctl) : 128;
+    else if (tid >= 32 && tid < 48)
+        s.lcol[tid - 32] = mx > 0 ? ih_byte(E + (size_t)(i - 1) * EW_WORDS, 4 + ((tid - 32) >> 2), tid & 3, st, ctl) : 0;
     __syncthreads();
     const int qp = q.qp;
     const double lambda = q.lambda;
@@ -913,7 +899,56 @@ k_picost(const fdvg_seq *__restrict__ seqs, fdvg_pcost *__restrict__ pcs, const 
         P->bn = 6 + s.ws.b[0] + s.ws.b[1] + s.ws.b[2] + s.ws.b[3];
         P->nsub = (uint8_t)sel;
     }
-    pcost_publish(pflag ? pflag + mb : nullptr, q.step);
+}
+
+// LDS of one MB's costs (k_pcosts).
+struct CostLds {
+    Smem s;
+    int32_t sred[FDVG_NSC][4];
+    uint8_t sedge[32], sout[32];
+    union { fdvg_pcost c; uint32_t w[PC_WORDS]; } sp;
+};
+
+// The candidate costs of MB mb (pcost_mb, then picost_mb, whose INTRA cost
+// waits for the left, top and top-left neighbours' stand-in edges), published
+// as tagged words: the costs (PC_WORDS a MB), read by the replay, which may run
+// at the same time, and the edges (EW_WORDS a MB).
+__device__ __forceinline__ void cost_job(CostLds &C, const fdvg_seq *__restrict__ seqs, int nseq, int mb, unsigned *ctl,
+                                         uint64_t *__restrict__ pcw, uint64_t *__restrict__ edgew) {
+    const int tid = threadIdx.x;
+    int sq = 0;
+    for (int k = 1; k < nseq; ++k) if (mb >= seqs[k].mb0) sq = k;
+    fdvg_seq q = seqs[sq];
+    q.given = nullptr;
+    const int i = mb - q.mb0;
+    pcost_mb(C.s, C.sred, C.sedge, C.sout, q, i, &C.sp.c);
+    __syncthreads();
+    if (tid < EW_WORDS) {
+        const uint32_t v = (uint32_t)C.sout[4 * tid] | (uint32_t)C.sout[4 * tid + 1] << 8 |
+                           (uint32_t)C.sout[4 * tid + 2] << 16 | (uint32_t)C.sout[4 * tid + 3] << 24;
+        __hip_atomic_store(edgew + (size_t)mb * EW_WORDS + tid, (uint64_t)q.step << 32 | v, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    }
+    picost_mb(C.s, q, i, &C.sp.c, edgew, ctl);
+    __syncthreads();
+    if (tid < PC_WORDS)
+        __hip_atomic_store(pcw + (size_t)mb * PC_WORDS + tid, (uint64_t)q.step << 32 | C.sp.w[tid], __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// The costs of every MB, one workgroup a MB taken by ticket (ctl[3]) in the
+// order of `corder` (the replay's order): every MB's left, top and top-left
+// neighbours have lower tickets, so their workgroups have started.
+extern "C" __global__ void __launch_bounds__(256, 4)
+k_pcosts(const fdvg_seq *__restrict__ seqs, int nseq, const unsigned *__restrict__ corder, int total_mb,
+         unsigned *ctl, uint64_t *__restrict__ pcw, uint64_t *__restrict__ edgew) {
+    __shared__ CostLds C;
+    __shared__ int job_s;
+    if (threadIdx.x == 0) job_s = (int)atomicAdd(ctl + 3, 1u);
+    __syncthreads();
+    const int job = __builtin_amdgcn_readfirstlane(job_s);
+    if (job >= total_mb) return;
+    cost_job(C, seqs, nseq, __builtin_amdgcn_readfirstlane((int)corder[job]), ctl, pcw, edgew);
 }
 
 // Motion record handoff of the decision wavefront: tag | inter | ref | x | y.
@@ -947,36 +982,28 @@ struct __align__(16) PrWave {
 
 // A chunk of a row in flight: every lane's share of the costs, tables and
 // source MBs, fixed trip counts so the loads issue back to back.
-#define PR_PW ((PR_CHUNK * (int)sizeof(fdvg_pcost) / 4 + 63) / 64)
+#define PR_PW ((PR_CHUNK * PC_WORDS + 63) / 64)
 #define PR_MW ((PR_CHUNK * (int)sizeof(fdvg_metab) / 4 + 63) / 64)
 #define PR_CW (PR_CHUNK * 64 / 64)
 struct PrPre {
-    uint32_t p[PR_PW], m[PR_MW], c[PR_CW];
+    uint64_t p[PR_PW];                   // tagged cost words
+    uint32_t m[PR_MW], c[PR_CW];
     int n;
 };
-__device__ __forceinline__ void pr_load(PrPre &R, const fdvg_seq &q, const fdvg_pcost *pcs, const unsigned *pflag,
-                                        unsigned *ctl, int my, int mx0, int L) {
+// Issues the chunk's loads; the cost words may still be in flight (k_pcosts
+// running concurrently): pr_store waits for their tags.
+__device__ __forceinline__ void pr_load(PrPre &R, const fdvg_seq &q, const uint64_t *pcw, int my, int mx0, int L) {
     const int mw = q.w >> 4, i0 = my * mw + mx0;
     const int n = mw - mx0 < PR_CHUNK ? mw - mx0 : PR_CHUNK;
-    // with pflag, the chunk's costs may still be in flight (k_picost running
-    // concurrently): wait for their flags to carry this step's id (bounded;
-    // error flag on timeout).  Without, the costs are complete before this launch.
-    if (pflag) {
-        const unsigned *f = pflag + q.mb0 + i0 + (L < n ? L : 0);
-        bool ok = L >= n || __hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == q.step;
-        for (unsigned spins = 0; !__all(ok); ++spins) {
-            if (spins > (1u << 24)) { if (L == 0) atomicOr(ctl + 1, 1u); break; }
-            __builtin_amdgcn_s_sleep(1);
-            ok = ok || __hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == q.step;
-        }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    }
-    const gmem_t<uint32_t> *gp = gmem((const uint32_t *)(pcs + q.mb0 + i0));
+    const uint64_t *gp = pcw + (size_t)(q.mb0 + i0) * PC_WORDS;
     const gmem_t<uint32_t> *gm = gmem((const uint32_t *)(q.metab + i0));
     const gmem_t<uint8_t> *gc = gmem(q.cy);
-    const int np = n * (int)(sizeof(fdvg_pcost) / 4), nm = n * (int)(sizeof(fdvg_metab) / 4);
+    const int np = n * PC_WORDS, nm = n * (int)(sizeof(fdvg_metab) / 4);
 #pragma unroll
-    for (int j = 0; j < PR_PW; ++j) { const int k = L + 64 * j; R.p[j] = k < np ? gp[k] : 0u; }
+    for (int j = 0; j < PR_PW; ++j) {
+        const int k = L + 64 * j;
+        R.p[j] = k < np ? __hip_atomic_load(gp + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : (uint64_t)q.step << 32;
+    }
 #pragma unroll
     for (int j = 0; j < PR_MW; ++j) { const int k = L + 64 * j; R.m[j] = k < nm ? gm[k] : 0u; }
 #pragma unroll
@@ -986,11 +1013,32 @@ __device__ __forceinline__ void pr_load(PrPre &R, const fdvg_seq &q, const fdvg_
     }
     R.n = n;
 }
-__device__ __forceinline__ void pr_store(PrWave &W, const PrPre &R, int L) {
-    uint32_t *lp = (uint32_t *)W.pc, *lm = (uint32_t *)W.mt, *lc = (uint32_t *)W.cur;
-    const int np = R.n * (int)(sizeof(fdvg_pcost) / 4), nm = R.n * (int)(sizeof(fdvg_metab) / 4);
+// Waits until every cost word of the chunk carries this frame step's id
+// (reloading the stale ones; bounded, error flag on timeout), then stores the
+// chunk into the wave's LDS block.
+__device__ __forceinline__ void pr_store(PrWave &W, PrPre &R, const fdvg_seq &q, const uint64_t *pcw, unsigned *ctl,
+                                         int my, int mx0, int L) {
+    const int mw = q.w >> 4, i0 = my * mw + mx0;
+    const uint64_t *gp = pcw + (size_t)(q.mb0 + i0) * PC_WORDS;
+    const int np = R.n * PC_WORDS, nm = R.n * (int)(sizeof(fdvg_metab) / 4);
+    bool ok = true;
 #pragma unroll
-    for (int j = 0; j < PR_PW; ++j) { const int k = L + 64 * j; if (k < np) lp[k] = R.p[j]; }
+    for (int j = 0; j < PR_PW; ++j) ok = ok && (uint32_t)(R.p[j] >> 32) == q.step;
+    for (unsigned spins = 0; !__all(ok); ++spins) {
+        if (spins > (1u << 22)) { if (L == 0) atomicOr(ctl + 1, 1u); break; }
+        __builtin_amdgcn_s_sleep(1);
+        ok = true;
+#pragma unroll
+        for (int j = 0; j < PR_PW; ++j) {
+            const int k = L + 64 * j;
+            if (k < np && (uint32_t)(R.p[j] >> 32) != q.step)
+                R.p[j] = __hip_atomic_load(gp + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            ok = ok && (uint32_t)(R.p[j] >> 32) == q.step;
+        }
+    }
+    uint32_t *lp = (uint32_t *)W.pc, *lm = (uint32_t *)W.mt, *lc = (uint32_t *)W.cur;
+#pragma unroll
+    for (int j = 0; j < PR_PW; ++j) { const int k = L + 64 * j; if (k < np) lp[k] = (uint32_t)R.p[j]; }
 #pragma unroll
     for (int j = 0; j < PR_MW; ++j) { const int k = L + 64 * j; if (k < nm) lm[k] = R.m[j]; }
 #pragma unroll
@@ -1011,8 +1059,8 @@ __device__ __forceinline__ uint64_t ld_rec(const uint64_t *p, int k) {
     return __hip_atomic_load(p + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 template <bool FUSED>
-__device__ __forceinline__ void replay_row(PrWave &W, const fdvg_seq &q, const fdvg_pcost *__restrict__ pcs,
-                                           const unsigned *__restrict__ pflag, unsigned *ctl, int my,
+__device__ __forceinline__ void replay_row(PrWave &W, const fdvg_seq &q, const uint64_t *__restrict__ pcw,
+                                           unsigned *ctl, int my,
                                            uint64_t *cur_slot, int *myprog, uint64_t *above, const int *up,
                                            const uint64_t *above_g, uint64_t *mrec_g, uint64_t *decw,
                                            fdvg_mbinfo *__restrict__ given, fdvg_pctx *__restrict__ pctx, int L) {
@@ -1028,8 +1076,8 @@ __device__ __forceinline__ void replay_row(PrWave &W, const fdvg_seq &q, const f
             // this chunk's costs, tables and source MBs into LDS (from the
             // registers loaded during the previous chunk, or now at a row
             // start), then the next chunk's loads issued into registers
-            if (mx == 0) pr_load(pre, q, pcs, pflag, ctl, my, 0, L);
-            pr_store(W, pre, L);
+            if (mx == 0) pr_load(pre, q, pcw, my, 0, L);
+            pr_store(W, pre, q, pcw, ctl, my, mx, L);
             if (my > 0 && above_g) {
                 // the row above, in another workgroup: its records of MBs
                 // mx .. mx + PR_CHUNK (tagged), waited for and copied into
@@ -1045,7 +1093,7 @@ __device__ __forceinline__ void replay_row(PrWave &W, const fdvg_seq &q, const f
                     abv = L <= PR_CHUNK ? ld_rec(above_g, mx + PR_CHUNK + L < mw ? mx + PR_CHUNK + L : mw - 1) : tagv;
             }
             wave_sync();
-            if (mx + PR_CHUNK < mw) pr_load(pre, q, pcs, pflag, ctl, my, mx + PR_CHUNK, L);
+            if (mx + PR_CHUNK < mw) pr_load(pre, q, pcw, my, mx + PR_CHUNK, L);
         }
         const fdvg_pcost &P = W.pc[mx % PR_CHUNK];
         const fdvg_metab &T = W.mt[mx % PR_CHUNK];
@@ -1236,9 +1284,8 @@ __device__ __forceinline__ void replay_row(PrWave &W, const fdvg_seq &q, const f
 }
 
 extern "C" __global__ void __launch_bounds__(512)
-k_prepair(const fdvg_seq *__restrict__ seqs, int mwmax, unsigned *ctl,
-          const fdvg_pcost *__restrict__ pcs, const unsigned *__restrict__ pflag, fdvg_mbinfo *__restrict__ given,
-          fdvg_pctx *__restrict__ pctx, unsigned *__restrict__ ilist) {
+k_prepair(const fdvg_seq *__restrict__ seqs, int mwmax, unsigned *ctl, const uint64_t *__restrict__ pcw,
+          fdvg_mbinfo *__restrict__ given, fdvg_pctx *__restrict__ pctx) {
     extern __shared__ __align__(16) uint64_t dyn[];
     const int NW = blockDim.x >> 6, wv = threadIdx.x >> 6, L = threadIdx.x & 63;
     uint64_t *ring = dyn;                                    // NW slots x mwmax motion records
@@ -1251,7 +1298,7 @@ k_prepair(const fdvg_seq *__restrict__ seqs, int mwmax, unsigned *ctl,
     if (L == 0) __hip_atomic_store(&prog[wv], -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     __syncthreads();
     for (int my = wv; my < mh; my += NW)
-        replay_row<false>(pw[wv], q, pcs, pflag, ctl, my, ring + (size_t)(my % NW) * mwmax, &prog[wv],
+        replay_row<false>(pw[wv], q, pcw, ctl, my, ring + (size_t)(my % NW) * mwmax, &prog[wv],
                           ring + (size_t)((my + NW - 1) % NW) * mwmax, &prog[(wv + NW - 1) % NW], nullptr, nullptr,
                           nullptr, given, pctx, L);
 }
@@ -1430,10 +1477,10 @@ k_pintra(const fdvg_seq *__restrict__ seqs, int nseq, const fdvg_mbinfo *__restr
     for (;;) {
         if (tid == 0) job_s = (int)atomicAdd(ctl + 4, 1u);
         __syncthreads();
-        const unsigned job = (unsigned)job_s;
+        const unsigned job = (unsigned)__builtin_amdgcn_readfirstlane(job_s);   // uniform: scalar registers
         __syncthreads();
         if (job >= n) return;
-        const int mb = (int)ilist[1 + job];
+        const int mb = __builtin_amdgcn_readfirstlane((int)ilist[1 + job]);
         int sq = 0;
         for (int k = 1; k < nseq; ++k) if (mb >= seqs[k].mb0) sq = k;
         fdvg_seq q = seqs[sq];
@@ -1467,7 +1514,9 @@ k_pintra(const fdvg_seq *__restrict__ seqs, int nseq, const fdvg_mbinfo *__restr
 //   earlier in that order.
 // Every wait is on work of a lower ticket or an earlier job, so none can wait on
 // a workgroup that has not started; every wait is bounded and sets the error
-// flag on timeout.  Rows at most PD_MAXMW MBs wide.
+// flag on timeout.  Rows at most PD_MAXMW MBs wide.  The replays and the
+// workers may also be two launches: one of the nunit replays only (grid nunit),
+// then one of workers only (nunit 0), which take no tickets.
 #define PD_MAXMW 256
 #define PD_ROWS 4
 struct __align__(16) PdRep {
@@ -1483,7 +1532,7 @@ union PdLds {
 
 extern "C" __global__ void __launch_bounds__(256)
 k_pdecide(const fdvg_seq *__restrict__ seqs, int nseq, const int2 *__restrict__ units, int nunit,
-          const unsigned *__restrict__ corder, int total_mb, unsigned *ctl, const fdvg_pcost *__restrict__ pcs,
+          const unsigned *__restrict__ corder, int total_mb, unsigned *ctl, const uint64_t *__restrict__ pcw,
           fdvg_mbinfo *__restrict__ given, fdvg_pctx *__restrict__ pctx, uint64_t *__restrict__ mrec,
           uint64_t *__restrict__ decw, uint64_t *__restrict__ ih, fdvg_mbrec *__restrict__ recs) {
     __shared__ PdLds U;
@@ -1493,9 +1542,10 @@ k_pdecide(const fdvg_seq *__restrict__ seqs, int nseq, const int2 *__restrict__ 
         struct { fdvg_mbinfo g; fdvg_pctx x; } d;
     } decb;
     const int tid = threadIdx.x, wv = tid >> 6, L = tid & 63;
-    if (tid == 0) role_s = atomicAdd(ctl + 6, 1u);
+    // nunit 0: a launch of coding workers only, which takes no ticket
+    if (tid == 0) role_s = nunit > 0 ? atomicAdd(ctl + 6, 1u) : 0u;
     __syncthreads();
-    const unsigned role = role_s;
+    const unsigned role = (unsigned)__builtin_amdgcn_readfirstlane((int)role_s);
     if (role < (unsigned)nunit) {
         const int2 u = units[role];
         const fdvg_seq q = seqs[u.x];
@@ -1504,7 +1554,7 @@ k_pdecide(const fdvg_seq *__restrict__ seqs, int nseq, const int2 *__restrict__ 
         __syncthreads();
         if (my >= mh) return;
         const bool first = wv == 0;
-        replay_row<true>(U.r.pw[wv], q, pcs, nullptr, ctl, my, U.r.ring[wv], &U.r.prog[wv],
+        replay_row<true>(U.r.pw[wv], q, pcw, ctl, my, U.r.ring[wv], &U.r.prog[wv],
                          first ? U.r.above0 : U.r.ring[wv - 1], first ? nullptr : &U.r.prog[wv - 1],
                          first && my > 0 ? mrec + q.mb0 + (size_t)(my - 1) * mw : nullptr, mrec, decw, given, pctx, L);
         return;
@@ -1514,10 +1564,10 @@ k_pdecide(const fdvg_seq *__restrict__ seqs, int nseq, const int2 *__restrict__ 
     for (;;) {
         if (tid == 0) job_s = (int)atomicAdd(ctl + 7, 1u);
         __syncthreads();
-        const int job = job_s;
+        const int job = __builtin_amdgcn_readfirstlane(job_s);        // uniform: scalar registers
         __syncthreads();
         if (job >= total_mb) return;
-        const int mb = (int)corder[job];
+        const int mb = __builtin_amdgcn_readfirstlane((int)corder[job]);
         int sq = 0;
         for (int k = 1; k < nseq; ++k) if (mb >= seqs[k].mb0) sq = k;
         fdvg_seq q = seqs[sq];
@@ -1553,25 +1603,31 @@ k_pdecide(const fdvg_seq *__restrict__ seqs, int nseq, const int2 *__restrict__ 
 static_assert(sizeof(fdvg_mbinfo) == 20 && sizeof(fdvg_pctx) == 20 && alignof(fdvg_pctx) <= 4,
               "decision words: 5 + 5 payload words");
 
-// Replay units and coding order of k_pdecide for a batch: units[] = (seq, g)
-// for every PD_ROWS-row group, ordered by g then seq (a unit's row above is in a
-// lower-numbered unit); corder[] = every MB, ordered by mx + 2 my (then
-// sequence, then row), the order in which the replay decides them.  Returns the
-// unit count, or -1 if a row is wider than PD_MAXMW MBs.
+// The block-parallel order of a batch: corder[] = every MB, ordered by
+// mx + 2 my (then sequence, then row), the order in which the replay decides
+// them (k_pcosts costs them in this order, k_pdecide codes them in it); and
+// k_pdecide's replay units: units[] = (seq, g) for every PD_ROWS-row group,
+// ordered by g then seq (a unit's row above is in a lower-numbered unit).
+// Returns the unit count; 0 if a row is wider than PD_MAXMW MBs (no units:
+// k_pdecide does not take the batch; corder is complete); -1 on inconsistent
+// sizes.
 extern "C" int fdvg_pdecide_plan(const int *w, const int *h, const int *mb0, int nseq, int2 *units,
                                  unsigned *corder) {
     int nunit = 0, maxg = 0, maxt = 0, total = 0;
+    bool wide = false;
     for (int k = 0; k < nseq; ++k) {
         const int mw = w[k] >> 4, mh = h[k] >> 4;
-        if (mw > PD_MAXMW) return -1;
+        if (mw <= 0 || mh <= 0 || mb0[k] != total) return -1;
+        if (mw > PD_MAXMW) wide = true;
         const int ng = (mh + PD_ROWS - 1) / PD_ROWS;
         if (ng > maxg) maxg = ng;
         if (mw - 1 + 2 * (mh - 1) > maxt) maxt = mw - 1 + 2 * (mh - 1);
         total += mw * mh;
     }
-    for (int gi = 0; gi < maxg; ++gi)
-        for (int k = 0; k < nseq; ++k)
-            if (gi * PD_ROWS < (h[k] >> 4)) units[nunit++] = make_int2(k, gi);
+    if (!wide)
+        for (int gi = 0; gi < maxg; ++gi)
+            for (int k = 0; k < nseq; ++k)
+                if (gi * PD_ROWS < (h[k] >> 4)) units[nunit++] = make_int2(k, gi);
     int n = 0;
     for (int t = 0; t <= maxt; ++t)
         for (int k = 0; k < nseq; ++k) {
@@ -1584,6 +1640,9 @@ extern "C" int fdvg_pdecide_plan(const int *w, const int *h, const int *mb0, int
     return n == total ? nunit : -1;
 }
 
+// Workgroups of k_pdecide: the replays and workers_per_cu coding workers a CU
+// (at most what a CU holds); with workers_per_cu < 0, workers only, as many as
+// leave room on the device for the nunit replays of another launch.
 extern "C" int fdvg_pdecide_grid(int nunit, int total_mb, int workers_per_cu) {
     static int ncu = 0, per = 0;
     if (!ncu) {
@@ -1592,16 +1651,20 @@ extern "C" int fdvg_pdecide_grid(int nunit, int total_mb, int workers_per_cu) {
         if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return -1;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void *)k_pdecide, 256, 0) != hipSuccess) return -1;
     }
+    if (workers_per_cu < 0) {
+        const int wk = ncu * per - nunit;
+        return wk < 1 ? 1 : wk < total_mb ? wk : total_mb;
+    }
     int wk = ncu * (workers_per_cu < per ? workers_per_cu : per);
     if (wk < 1) wk = 1;
     return nunit + (wk < total_mb ? wk : total_mb);
 }
 
 extern "C" int fdvg_launch_pdecide(const fdvg_seq *seqs, int nseq, const int2 *units, int nunit, const unsigned *corder,
-                                   int total_mb, int grid, unsigned *ctl, const fdvg_pcost *pcs, fdvg_mbinfo *given,
+                                   int total_mb, int grid, unsigned *ctl, const uint64_t *pcw, fdvg_mbinfo *given,
                                    fdvg_pctx *pctx, uint64_t *mrec, uint64_t *decw, uint64_t *ih, fdvg_mbrec *recs,
                                    hipStream_t st) {
-    hipLaunchKernelGGL(k_pdecide, dim3(grid), dim3(256), 0, st, seqs, nseq, units, nunit, corder, total_mb, ctl, pcs,
+    hipLaunchKernelGGL(k_pdecide, dim3(grid), dim3(256), 0, st, seqs, nseq, units, nunit, corder, total_mb, ctl, pcw,
                        given, pctx, mrec, decw, ih, recs);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
@@ -1614,26 +1677,19 @@ extern "C" int fdvg_launch_plist(const fdvg_seq *seqs, int nseq, const fdvg_mbin
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-// The decision stage in three launches: the costs, the replay (which waits on
-// the costs' per-MB flags, so it may run on another stream at the same time),
-// and the coding of the decisions.  fdvg_launch_parallel issues all three on one
-// stream.
-extern "C" int fdvg_launch_pcost(const fdvg_seq *seqs, int nseq, int xgrid, fdvg_pcost *pcs, uint8_t *edge,
-                                 unsigned *pflag, hipStream_t st) {
-    hipLaunchKernelGGL(k_pcost, dim3(xgrid, nseq), dim3(256), 0, st, seqs, pcs, edge);
-    hipLaunchKernelGGL(k_picost, dim3(xgrid, nseq), dim3(256), 0, st, seqs, pcs, (const uint8_t *)edge, pflag);
+extern "C" int fdvg_launch_pcosts(const fdvg_seq *seqs, int nseq, const unsigned *corder, int total_mb, unsigned *ctl,
+                                  uint64_t *pcw, uint64_t *edgew, hipStream_t st) {
+    hipLaunchKernelGGL(k_pcosts, dim3(total_mb), dim3(256), 0, st, seqs, nseq, corder, total_mb, ctl, pcw, edgew);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 extern "C" int fdvg_launch_prepair(const fdvg_seq *seqs, int nseq, int maxmh, int mwmax, unsigned *ctl,
-                                   const fdvg_pcost *pcs, const unsigned *pflag, fdvg_mbinfo *given,
-                                   fdvg_pctx *pctx, unsigned *ilist, hipStream_t st) {
+                                   const uint64_t *pcw, fdvg_mbinfo *given, fdvg_pctx *pctx, hipStream_t st) {
     int nw = maxmh < 8 ? maxmh : 8;
     size_t lds = prepair_lds(nw, mwmax);
     while (nw > 1 && lds > 64 * 1024) lds = prepair_lds(--nw, mwmax);
     if (lds > 64 * 1024) return -1;
-    hipLaunchKernelGGL(k_prepair, dim3(nseq), dim3(64 * nw), lds, st, seqs, mwmax, ctl, pcs, pflag, given, pctx,
-                       ilist);
+    hipLaunchKernelGGL(k_prepair, dim3(nseq), dim3(64 * nw), lds, st, seqs, mwmax, ctl, pcw, given, pctx);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -1656,14 +1712,14 @@ extern "C" int fdvg_launch_pfinish(const fdvg_seq *seqs, int nseq, int xgrid, un
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+// The decision stage as separate launches on one stream: the costs, the
+// replay, the coding of the decisions.
 extern "C" int fdvg_launch_parallel(const fdvg_seq *seqs, int nseq, int xgrid, int maxmh, int mwmax,
-                                    unsigned *ctl, fdvg_pcost *pcs, uint8_t *edge, unsigned *pflag,
+                                    const unsigned *corder, unsigned *ctl, uint64_t *pcw, uint64_t *edgew,
                                     fdvg_mbinfo *given, fdvg_pctx *pctx, unsigned *ilist, uint64_t *ih, int total_mb,
                                     fdvg_mbrec *recs, hipStream_t st) {
-    (void)pflag;
-    if (fdvg_launch_pcost(seqs, nseq, xgrid, pcs, edge, nullptr, st) != 0) return -1;
-    // same stream: the costs are complete when the replay starts, no flag waits
-    if (fdvg_launch_prepair(seqs, nseq, maxmh, mwmax, ctl, pcs, nullptr, given, pctx, ilist, st) != 0) return -1;
+    if (fdvg_launch_pcosts(seqs, nseq, corder, total_mb, ctl, pcw, edgew, st) != 0) return -1;
+    if (fdvg_launch_prepair(seqs, nseq, maxmh, mwmax, ctl, pcw, given, pctx, st) != 0) return -1;
     return fdvg_launch_pfinish(seqs, nseq, xgrid, ctl, given, pctx, ilist, ih, total_mb, recs, st);
 }
 

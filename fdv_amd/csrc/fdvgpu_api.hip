@@ -36,7 +36,7 @@ struct Tune {
     int sym_waves;      // serialiser MBs (wavefronts) per workgroup: 1, 2, 4, 8
     int deblock_bs;     // deblock threads per workgroup: 64, 128, 256
     int nn_wpc;         // proposer workgroups per CU: 1, 2
-    int pd_wpc;         // coding workers of the fused decision stage per CU: 1, 2
+    int pd_wpc;         // workers of the fused decision stage per CU: 1, 2
 };
 extern "C" int fdvg_launch_metab(const fdvg_seq *, int, int, void *, const void *, float, float, hipStream_t);
 extern "C" int fdvg_launch_nn_infer(const fdvg_seq *, int, int, const float *, fdvg_nnout *, int, hipStream_t);
@@ -46,18 +46,19 @@ extern "C" int fdvg_nn_setup(void);
 // The proposer's confidence threshold: below it an MB gets the exhaustive search.
 #define FDVG_NN_THRESHOLD 0.7f     // default confidence threshold of the proposer
 #define FDVG_NN_SKIP_OFF 2.0f      // default SKIP threshold: no pruning
-extern "C" int fdvg_launch_parallel(const fdvg_seq *, int, int, int, int, unsigned *, fdvg_pcost *, uint8_t *, unsigned *,
-                                    fdvg_mbinfo *, fdvg_pctx *, unsigned *, uint64_t *, int, fdvg_mbrec *, hipStream_t);
-extern "C" int fdvg_launch_pcost(const fdvg_seq *, int, int, fdvg_pcost *, uint8_t *, unsigned *, hipStream_t);
-extern "C" int fdvg_launch_prepair(const fdvg_seq *, int, int, int, unsigned *, const fdvg_pcost *, const unsigned *,
-                                   fdvg_mbinfo *, fdvg_pctx *, unsigned *, hipStream_t);
-extern "C" int fdvg_launch_pfinish(const fdvg_seq *, int, int, unsigned *, fdvg_mbinfo *, fdvg_pctx *, unsigned *,
-                                   uint64_t *, int, fdvg_mbrec *, hipStream_t);
+extern "C" int fdvg_launch_parallel(const fdvg_seq *, int, int, int, int, const unsigned *, unsigned *, uint64_t *,
+                                    uint64_t *, fdvg_mbinfo *, fdvg_pctx *, unsigned *, uint64_t *, int, fdvg_mbrec *,
+                                    hipStream_t);
+extern "C" int fdvg_launch_pcosts(const fdvg_seq *, int, const unsigned *, int, unsigned *, uint64_t *, uint64_t *,
+                                  hipStream_t);
 extern "C" int fdvg_pdecide_plan(const int *, const int *, const int *, int, int2 *, unsigned *);
 extern "C" int fdvg_pdecide_grid(int, int, int);
 extern "C" int fdvg_launch_pdecide(const fdvg_seq *, int, const int2 *, int, const unsigned *, int, int, unsigned *,
-                                   const fdvg_pcost *, fdvg_mbinfo *, fdvg_pctx *, uint64_t *, uint64_t *, uint64_t *,
+                                   const uint64_t *, fdvg_mbinfo *, fdvg_pctx *, uint64_t *, uint64_t *, uint64_t *,
                                    fdvg_mbrec *, hipStream_t);
+#define PC_WORDS 36                     // a MB's candidate costs as tagged words
+#define EW_WORDS 8                      // a MB's stand-in edges as tagged words
+#define PD_WORDS 10                     // a MB's decision and context as tagged words
 
 static thread_local char g_err[256];
 static void seterr(const char *fmt, ...) {
@@ -109,27 +110,30 @@ struct fdvgpu {
     uint32_t *cnt_d, *totals_d, *bases_d, *bases_h;
     uint8_t *compact_d, *compact_h;
     // block-parallel decisions
-    fdvg_pcost *pcost_d;
-    uint8_t *edge_d;                // per MB: the stand-in reconstruction's bottom row and right column
+    uint64_t *pcw_d;                // per MB: the candidate costs as tagged words
+    uint64_t *edgew_d;              // per MB: the stand-in reconstruction's bottom row and right column, tagged
+    unsigned *corder_d;             // the MBs in the order the replay decides them
     fdvg_pctx *pctx_d;
     fdvg_mbinfo *pgiven_d;
     uint64_t *mrec_d;               // motion-record handoff of the decision wavefront
     unsigned *ilist_d;              // intra MBs in dependency order (count first)
     uint64_t *ih_d;                 // intra handoff words, 16 per MB (bottom row, right column)
-    unsigned *pflag_d;              // per MB: the id of the frame step whose costs are written
     unsigned step_ctr;              // frame-step ids (never rewound)
     // The fused decision stage (k_pdecide): the replay and the coding of its
-    // decisions in one launch, so the coding follows the replay MB by MB.
+    // decisions in one launch, so the coding follows the replay MB by MB; with
+    // overlap 2 it runs on a second stream beside the costs (k_pcosts), whose
+    // tagged words the replay waits for.
     int2 *units_d;                  // replay units (sequence, row group)
-    unsigned *corder_d;             // the MBs in the order the replay decides them
     uint64_t *decw_d;               // per MB: the decision and its context as tagged words
     int nunit;
-    int overlap, overlap_avail;
+    int overlap, overlap_avail;     // 0: separate launches; 1: costs, then the fused launch; 2: side by side
+    hipStream_t st2;
+    hipEvent_t ev_fork, ev_join;
     // launch shapes, and one instantiated graph per (search mode, decision structure)
     Tune tune;
     int graphs;
-    hipGraphExec_t gexec[128];      // per (stage, search mode, decision structure, fused stage)
-    int warm[128];                  // the key has run once eagerly (launch caches filled)
+    hipGraphExec_t gexec[96];       // per (stage, search mode, decision structure, fused stage)
+    int warm[96];                   // the key has run once eagerly (launch caches filled)
     // learned proposer
     float *nnw_d;
     fdvg_nnout *nn_d;
@@ -181,16 +185,18 @@ extern "C" void fdvgpu_close(fdvgpu *g) {
     (void)hipFree(g->metab_d); (void)hipFree(g->planes_d); (void)hipHostFree(g->planes_h); (void)hipFree(g->sym_d);
     (void)hipFree(g->cnt_d); (void)hipFree(g->totals_d); (void)hipFree(g->bases_d); (void)hipHostFree(g->bases_h);
     (void)hipFree(g->compact_d); (void)hipHostFree(g->compact_h); (void)hipHostFree(g->flags_h);
-    (void)hipFree(g->pcost_d); (void)hipFree(g->pctx_d); (void)hipFree(g->pgiven_d); (void)hipFree(g->mrec_d);
+    (void)hipFree(g->pcw_d); (void)hipFree(g->pctx_d); (void)hipFree(g->pgiven_d); (void)hipFree(g->mrec_d);
     (void)hipFree(g->ilist_d); (void)hipFree(g->ih_d);
     (void)hipFree(g->nnw_d); (void)hipFree(g->nn_d);
-    for (int k = 0; k < 128; ++k) if (g->gexec[k]) (void)hipGraphExecDestroy(g->gexec[k]);
+    for (int k = 0; k < 96; ++k) if (g->gexec[k]) (void)hipGraphExecDestroy(g->gexec[k]);
     for (int i = 0; i < 5; ++i) if (g->ev[i]) (void)hipEventDestroy(g->ev[i]);
     for (int i = 0; i < 8; ++i) if (g->ring_ev[i]) (void)hipEventDestroy(g->ring_ev[i]);
     if (g->st) (void)hipStreamDestroy(g->st);
+    if (g->st2) (void)hipStreamDestroy(g->st2);
+    if (g->ev_fork) (void)hipEventDestroy(g->ev_fork);
+    if (g->ev_join) (void)hipEventDestroy(g->ev_join);
     (void)hipFree(g->units_d); (void)hipFree(g->corder_d); (void)hipFree(g->decw_d);
-    (void)hipFree(g->pflag_d);
-    (void)hipFree(g->edge_d);
+    (void)hipFree(g->edgew_d);
     for (int k = 0; k < 2; ++k) {
         if (g->pipe_up[k]) (void)hipHostFree(g->pipe_up[k]);
         if (g->pipe_sc[k]) (void)hipHostFree(g->pipe_sc[k]);
@@ -244,6 +250,9 @@ extern "C" fdvgpu *fdvgpu_open(int nseq, const int *w, const int *h, int nsrc) {
         g->compact_cap += (size_t)m.nmb * 2048 + 4096;
     }
     CKN(hipStreamCreate(&g->st));
+    CKN(hipStreamCreate(&g->st2));
+    CKN(hipEventCreateWithFlags(&g->ev_fork, hipEventDisableTiming));
+    CKN(hipEventCreateWithFlags(&g->ev_join, hipEventDisableTiming));
     for (int i = 0; i < 5; ++i) CKN(hipEventCreate(&g->ev[i]));
     for (int i = 0; i < 8; ++i) CKN(hipEventCreateWithFlags(&g->ring_ev[i], hipEventDisableTiming));
     CKN(hipMalloc(&g->seq_d, nseq * sizeof(fdvg_seq)));
@@ -267,8 +276,11 @@ extern "C" fdvgpu *fdvgpu_open(int nseq, const int *w, const int *h, int nsrc) {
     CKN(hipMalloc(&g->compact_d, g->compact_cap));
     CKN(hipHostMalloc(&g->compact_h, g->compact_cap, hipHostMallocDefault));
     CKN(hipHostMalloc(&g->flags_h, 64, hipHostMallocDefault));
-    CKN(hipMalloc(&g->pcost_d, (size_t)g->total_mb * sizeof(fdvg_pcost)));
-    CKN(hipMalloc(&g->edge_d, (size_t)g->total_mb * 32));
+    // tagged words: id 0 is never a frame step
+    CKN(hipMalloc(&g->pcw_d, (size_t)g->total_mb * PC_WORDS * sizeof(uint64_t)));
+    CKN(hipMemset(g->pcw_d, 0, (size_t)g->total_mb * PC_WORDS * sizeof(uint64_t)));
+    CKN(hipMalloc(&g->edgew_d, (size_t)g->total_mb * EW_WORDS * sizeof(uint64_t)));
+    CKN(hipMemset(g->edgew_d, 0, (size_t)g->total_mb * EW_WORDS * sizeof(uint64_t)));
     CKN(hipMalloc(&g->pctx_d, (size_t)g->total_mb * sizeof(fdvg_pctx)));
     CKN(hipMalloc(&g->pgiven_d, (size_t)g->total_mb * sizeof(fdvg_mbinfo)));
     CKN(hipMalloc(&g->mrec_d, (size_t)g->total_mb * sizeof(uint64_t)));
@@ -276,30 +288,29 @@ extern "C" fdvgpu *fdvgpu_open(int nseq, const int *w, const int *h, int nsrc) {
     CKN(hipMalloc(&g->ilist_d, ((size_t)g->total_mb + 1) * sizeof(unsigned)));
     CKN(hipMalloc(&g->ih_d, (size_t)g->total_mb * 16 * sizeof(uint64_t)));
     CKN(hipMemset(g->ih_d, 0, (size_t)g->total_mb * 16 * sizeof(uint64_t)));   // id 0: never a step
-    CKN(hipMalloc(&g->pflag_d, (size_t)g->total_mb * sizeof(unsigned)));
-    CKN(hipMemset(g->pflag_d, 0, (size_t)g->total_mb * sizeof(unsigned)));
     {
-        // the fused decision stage's plan (rows up to its width limit; wider
-        // batches run the stage as separate launches)
+        // the block-parallel order, and the fused decision stage's units (rows
+        // up to its width limit; wider batches run the stage as separate launches)
         int *wv = (int *)malloc(3 * nseq * sizeof(int));
         int2 *units_h = (int2 *)malloc(((size_t)g->nrows + nseq) * sizeof(int2));
         unsigned *corder_h = (unsigned *)malloc((size_t)g->total_mb * sizeof(unsigned));
         if (!wv || !units_h || !corder_h) { free(wv); free(units_h); free(corder_h); seterr("out of host memory"); goto fail; }
         for (int k = 0; k < nseq; ++k) { wv[k] = g->S[k].w; wv[nseq + k] = g->S[k].h; wv[2 * nseq + k] = g->S[k].mb0; }
         g->nunit = fdvg_pdecide_plan(wv, wv + nseq, wv + 2 * nseq, nseq, units_h, corder_h);
-        g->overlap = g->overlap_avail = g->nunit > 0;
-        hipError_t e = hipSuccess;
-        if (g->overlap_avail) {
+        g->overlap_avail = g->nunit > 0;
+        g->overlap = g->overlap_avail ? 2 : 0;
+        hipError_t e = g->nunit < 0 ? hipErrorInvalidValue : hipSuccess;
+        if (e == hipSuccess) e = hipMalloc(&g->corder_d, (size_t)g->total_mb * sizeof(unsigned));
+        if (e == hipSuccess)
+            e = hipMemcpy(g->corder_d, corder_h, (size_t)g->total_mb * sizeof(unsigned), hipMemcpyHostToDevice);
+        if (e == hipSuccess && g->overlap_avail) {
             e = hipMalloc(&g->units_d, (size_t)g->nunit * sizeof(int2));
-            if (e == hipSuccess) e = hipMalloc(&g->corder_d, (size_t)g->total_mb * sizeof(unsigned));
-            if (e == hipSuccess) e = hipMalloc(&g->decw_d, (size_t)g->total_mb * 10 * sizeof(uint64_t));
-            if (e == hipSuccess) e = hipMemset(g->decw_d, 0, (size_t)g->total_mb * 10 * sizeof(uint64_t));
+            if (e == hipSuccess) e = hipMalloc(&g->decw_d, (size_t)g->total_mb * PD_WORDS * sizeof(uint64_t));
+            if (e == hipSuccess) e = hipMemset(g->decw_d, 0, (size_t)g->total_mb * PD_WORDS * sizeof(uint64_t));
             if (e == hipSuccess) e = hipMemcpy(g->units_d, units_h, (size_t)g->nunit * sizeof(int2), hipMemcpyHostToDevice);
-            if (e == hipSuccess)
-                e = hipMemcpy(g->corder_d, corder_h, (size_t)g->total_mb * sizeof(unsigned), hipMemcpyHostToDevice);
         }
         free(wv); free(units_h); free(corder_h);
-        if (e != hipSuccess) { seterr("fused decision stage: %s", hipGetErrorString(e)); goto fail; }
+        if (e != hipSuccess) { seterr("block-parallel plan: %s", hipGetErrorString(e)); goto fail; }
     }
     CKN(hipMalloc(&g->nnw_d, (size_t)fdvg_nn_nparam() * sizeof(float)));
     if (fdvg_nn_setup() != 0) { seterr("proposer kernel setup failed"); goto fail; }
@@ -412,21 +423,45 @@ static int launch_stage(fdvgpu *g, int stage, int me_mode, bool par) {
         }
         if (fdvg_launch_zero(g->ticket_d, 8, g->st) != 0) { seterr("ticket reset launch failed"); return -1; }
     } else if (stage == 1) {
-        if (par && g->overlap) {
+        if (par && g->overlap == 2) {
+            // the costs on this stream, the replays beside them on the second
+            // one (they wait for each chunk's cost words), then, after the
+            // costs, the coding workers on this stream.  The costs go first:
+            // where the two streams share a hardware queue all three run one
+            // after the other, correctly.  Side by side, the costs never wait
+            // for the replays, and the workers leave the device room for them.
+            const int wgrid = fdvg_pdecide_grid(g->nunit, g->total_mb, -1);
+            if (wgrid <= 0) { seterr("parallel analysis: no launch shape"); return -1; }
+            CK(hipEventRecord(g->ev_fork, g->st));
+            CK(hipStreamWaitEvent(g->st2, g->ev_fork, 0));
+            if (fdvg_launch_pcosts(g->seq_d, g->nseq, g->corder_d, g->total_mb, g->ticket_d, g->pcw_d, g->edgew_d,
+                                   g->st) != 0 ||
+                fdvg_launch_pdecide(g->seq_d, g->nseq, g->units_d, g->nunit, g->corder_d, g->total_mb, g->nunit,
+                                    g->ticket_d, g->pcw_d, g->pgiven_d, g->pctx_d, g->mrec_d, g->decw_d, g->ih_d,
+                                    g->recs_d, g->st2) != 0 ||
+                fdvg_launch_pdecide(g->seq_d, g->nseq, g->units_d, 0, g->corder_d, g->total_mb, wgrid, g->ticket_d,
+                                    g->pcw_d, g->pgiven_d, g->pctx_d, g->mrec_d, g->decw_d, g->ih_d, g->recs_d,
+                                    g->st) != 0) {
+                seterr("parallel analysis launch failed: %s", hipGetErrorString(hipGetLastError())); return -1;
+            }
+            CK(hipEventRecord(g->ev_join, g->st2));
+            CK(hipStreamWaitEvent(g->st, g->ev_join, 0));
+        } else if (par && g->overlap == 1) {
             // the costs, then the replay and the coding of its decisions in
             // one launch (the coding follows the replay MB by MB)
-            const int grid = fdvg_pdecide_grid(g->nunit, g->total_mb, g->tune.pd_wpc);
-            if (grid <= 0 ||
-                fdvg_launch_pcost(g->seq_d, g->nseq, g->xgrid, g->pcost_d, g->edge_d, nullptr, g->st) != 0 ||
-                fdvg_launch_pdecide(g->seq_d, g->nseq, g->units_d, g->nunit, g->corder_d, g->total_mb, grid,
-                                    g->ticket_d, g->pcost_d, g->pgiven_d, g->pctx_d, g->mrec_d, g->decw_d, g->ih_d,
+            const int dgrid = fdvg_pdecide_grid(g->nunit, g->total_mb, g->tune.pd_wpc);
+            if (dgrid <= 0 ||
+                fdvg_launch_pcosts(g->seq_d, g->nseq, g->corder_d, g->total_mb, g->ticket_d, g->pcw_d, g->edgew_d,
+                                   g->st) != 0 ||
+                fdvg_launch_pdecide(g->seq_d, g->nseq, g->units_d, g->nunit, g->corder_d, g->total_mb, dgrid,
+                                    g->ticket_d, g->pcw_d, g->pgiven_d, g->pctx_d, g->mrec_d, g->decw_d, g->ih_d,
                                     g->recs_d, g->st) != 0) {
                 seterr("parallel analysis launch failed: %s", hipGetErrorString(hipGetLastError())); return -1;
             }
         } else if (par) {
             if (fdvg_launch_zero(g->ilist_d, 1, g->st) != 0 ||
-                fdvg_launch_parallel(g->seq_d, g->nseq, g->xgrid, g->maxmh, g->maxmw, g->ticket_d, g->pcost_d,
-                                     g->edge_d, g->pflag_d, g->pgiven_d, g->pctx_d, g->ilist_d, g->ih_d, g->total_mb,
+                fdvg_launch_parallel(g->seq_d, g->nseq, g->xgrid, g->maxmh, g->maxmw, g->corder_d, g->ticket_d,
+                                     g->pcw_d, g->edgew_d, g->pgiven_d, g->pctx_d, g->ilist_d, g->ih_d, g->total_mb,
                                      g->recs_d, g->st) != 0) {
                 seterr("parallel analysis launch failed: %s", hipGetErrorString(hipGetLastError())); return -1;
             }
@@ -448,7 +483,11 @@ static int launch_stage(fdvgpu *g, int stage, int me_mode, bool par) {
 // One stage: eagerly the first time its key is seen (which also fills the
 // launchers' one-time caches), from then on as a captured hipGraph.
 static int run_stage(fdvgpu *g, int stage, int me_mode, bool par) {
-    const int key = ((stage * 4 + me_mode) * 2 + (par ? 1 : 0)) * 2 + (stage == 1 && par && g->overlap ? 1 : 0);
+    const int key = ((stage * 4 + me_mode) * 2 + (par ? 1 : 0)) * 3 + (stage == 1 && par ? g->overlap : 0);
+    // the side-by-side stage runs eagerly: a captured graph need not run its
+    // two branches at the same time, and run first alone the fused launch
+    // would wait for costs that do not come
+    if (stage == 1 && par && g->overlap == 2) return launch_stage(g, stage, me_mode, par);
     if (!g->graphs || !g->warm[key]) {
         if (launch_stage(g, stage, me_mode, par) != 0) return -1;
         g->warm[key] = 1;
@@ -823,11 +862,11 @@ extern "C" int fdvgpu_planes(fdvgpu *g, int seq, int src_slot, const uint8_t **c
 }
 
 extern "C" int fdvgpu_set_overlap(fdvgpu *g, int on) {
-    if (!g) { seterr("fdvgpu_set_overlap: bad arguments"); return -1; }
+    if (!g || on < 0 || on > 2) { seterr("fdvgpu_set_overlap: bad arguments"); return -1; }
     if (on && !g->overlap_avail) {
         seterr("fdvgpu_set_overlap: the fused decision stage takes rows of at most 256 MBs"); return -1;
     }
-    g->overlap = on ? 1 : 0;
+    g->overlap = on;
     return 0;
 }
 
@@ -838,7 +877,7 @@ extern "C" int fdvgpu_set_graphs(fdvgpu *g, int on) {
 }
 
 static void drop_graphs(fdvgpu *g) {
-    for (int k = 0; k < 128; ++k)
+    for (int k = 0; k < 96; ++k)
         if (g->gexec[k]) { (void)hipGraphExecDestroy(g->gexec[k]); g->gexec[k] = NULL; }
 }
 
@@ -869,7 +908,7 @@ extern "C" int fdvgpu_autotune(fdvgpu *g, const int *src_slot, const int *qp, co
     const bool par = me_mode == FDVGPU_ME_LEARNED || me_mode == FDVGPU_ME_PARALLEL;
     for (int prm = 0; prm < 4; ++prm) {
         if (prm == 2 && me_mode != FDVGPU_ME_LEARNED) continue;
-        if (prm == 3 && !(par && g->overlap)) continue;
+        if (prm == 3 && !(par && g->overlap == 1)) continue;
         float best = 1e30f;
         int bestv = 0;
         for (int k = 0; k < 3 && opts[prm][k]; ++k) {

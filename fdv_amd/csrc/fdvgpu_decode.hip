@@ -293,10 +293,10 @@ k_dintra(const fdvg_seq *__restrict__ seqs, int nseq, const fdvg_mbrec *__restri
     for (;;) {
         if (tid == 0) job_s = (int)atomicAdd(ctl + 5, 1u);
         __syncthreads();
-        const unsigned job = (unsigned)job_s;
+        const unsigned job = (unsigned)__builtin_amdgcn_readfirstlane(job_s);   // uniform: scalar registers
         __syncthreads();
         if (job >= n) return;
-        const int mb = (int)ilist[1 + job];
+        const int mb = __builtin_amdgcn_readfirstlane((int)ilist[1 + job]);
         int sq = 0;
         for (int k = 1; k < nseq; ++k) if (mb >= seqs[k].mb0) sq = k;
         const fdvg_seq q = seqs[sq];

@@ -216,10 +216,12 @@ class Analyser:
         pruning (above 1: never; the default)."""
         self._ck(self.L.fdvgpu_set_proposer_thresholds(self.h, float(conf), float(skip)), "set_proposer_thresholds")
 
-    def set_overlap(self, on: bool):
-        """The fused block-parallel decision stage (replay and coding in one
-        launch; the default) or separate launches."""
-        self._ck(self.L.fdvgpu_set_overlap(self.h, 1 if on else 0), "set_overlap")
+    def set_overlap(self, mode):
+        """The block-parallel decision stage's structure: 0 separate launches
+        (costs, replay, coding), 1 the costs then the replay and coding fused,
+        2 all three in one launch (the default where available; True means 2)."""
+        mode = 2 if mode is True else int(mode)
+        self._ck(self.L.fdvgpu_set_overlap(self.h, mode), "set_overlap")
 
     def set_graphs(self, on: bool):
         self._ck(self.L.fdvgpu_set_graphs(self.h, 1 if on else 0), "set_graphs")

@@ -175,13 +175,15 @@ int fdvgpu_decode(fdvgpu *g, const fdvgpu_mbcode *const *codes, const int *qp, c
  * keeps the fastest.  None of this has a counterpart in the reference. */
 int fdvgpu_set_graphs(fdvgpu *g, int on);
 
-/* The fused block-parallel decision stage (on by default where every row is at
- * most 256 MBs wide): the decision replay and the coding of its decisions in
- * one launch, each MB coded as soon as its decision (and, for INTRA, its
- * neighbours' reconstruction) exists.  Off: the replay, then the coding, as
- * separate launches.  Both give the same results.  on = 1 returns -1 where the
- * fused stage is unavailable.  No counterpart in the reference. */
-int fdvgpu_set_overlap(fdvgpu *g, int on);
+/* The structure of the block-parallel decision stage, all with the same
+ * results: 0, the candidate costs, the decision replay and the coding of its
+ * decisions as separate launches; 1, the costs, then the replay and the coding
+ * in one launch, each MB coded as soon as its decision (and, for INTRA, its
+ * neighbours' reconstruction) exists; 2 (the default where every row is at
+ * most 256 MBs wide), all three in one launch, the replay consuming the costs
+ * as they are computed.  1 and 2 return -1 where unavailable.  No counterpart
+ * in the reference. */
+int fdvgpu_set_overlap(fdvgpu *g, int mode);
 /* shapes: serialiser wavefronts per workgroup {1..8}, deblock workgroup size
  * {64, 128, 256}, proposer workgroups per CU {1, 2}, coding workers of the
  * fused decision stage per CU {1, 2}. */

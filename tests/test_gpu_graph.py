@@ -74,17 +74,18 @@ def test_autotune_changes_no_result():
 
 @pytest.mark.parametrize("mode", [G.ME_PARALLEL, G.ME_LEARNED])
 def test_fused_decision_stage_matches_separate_launches(mode):
-    """The fused decision stage (the replay and the coding of its decisions in
-    one launch, synchronised by tagged words) gives the results of the replay
-    and the coding as separate launches, eagerly and from captured graphs.  The
-    third sequence has 11 MB rows: three replay units, whose first rows read
-    the row above from another workgroup."""
+    """The decision stage's three structures give the same results, eagerly and
+    from captured graphs: separate launches (costs, replay, coding); the costs
+    then the replay and coding fused; all three fused, the replay consuming the
+    costs as they are computed (tagged words).  The third sequence has 11 MB
+    rows: three replay units, whose first rows read the row above from another
+    workgroup."""
     scenes = _scenes() + [content.scene_pan(80, 176, 63, nobj=2, smin=8, smax=24, pan=(2, 1), grain=2.0)]
     sizes = [(s.w, s.h) for s in scenes]
     nfr = 5
     frames = [[s.frame(i) for i in range(nfr)] for s in scenes]
     outs = []
-    for overlap in (False, True):
+    for overlap in (0, 1, 2):
         an = G.Analyser(sizes, nsrc=nfr)
         an.set_overlap(overlap)
         if mode == G.ME_LEARNED:
@@ -102,7 +103,8 @@ def test_fused_decision_stage_matches_separate_launches(mode):
             res.append([(an.streams(k), an.recon(k).copy()) for k in range(ns)])
         an.close()
         outs.append(res)
-    for f in range(nfr - 1):
-        for k in range(len(scenes)):
-            assert outs[0][f][k][0] == outs[1][f][k][0], f"frame {f + 1} seq {k}: streams"
-            assert np.array_equal(outs[0][f][k][1], outs[1][f][k][1]), f"frame {f + 1} seq {k}: recon"
+    for v in (1, 2):
+        for f in range(nfr - 1):
+            for k in range(len(scenes)):
+                assert outs[0][f][k][0] == outs[v][f][k][0], f"structure {v} frame {f + 1} seq {k}: streams"
+                assert np.array_equal(outs[0][f][k][1], outs[v][f][k][1]), f"structure {v} frame {f + 1} seq {k}: recon"

@@ -3,8 +3,9 @@ the library; no device needed).  The stage cannot wait on a workgroup that has
 not started only if:
 - every replay unit's row above (the last row of the unit before it in the same
   sequence) belongs to an earlier unit;
-- every MB's left, top and top-left neighbours come earlier in the coding order
-  (an intra MB waits for their reconstruction);
+- every MB's left, top and top-left neighbours come earlier in the cost and
+  coding order (an INTRA cost waits for their stand-in edges, an intra MB for
+  their reconstruction);
 and it codes every MB exactly once.  Checked here on the measured workloads'
 shapes and on ragged ones, with a planted mistake that the check must catch."""
 import ctypes as C
@@ -97,6 +98,11 @@ def test_plan_check_catches_a_swapped_pair():
         check(sizes, nunit, units, bad, mb0)
 
 
-def test_plan_rejects_rows_wider_than_the_stage_takes():
-    nunit, _, _, _ = plan([(4112, 32)])
-    assert nunit == -1
+def test_plan_of_rows_wider_than_the_fused_stage_takes():
+    """No replay units (the separate launches take the batch), but the cost
+    and coding order is complete."""
+    sizes = [(4112, 32), (64, 48)]
+    nunit, units, corder, mb0 = plan(sizes)
+    assert nunit == 0
+    tot = sum((w // 16) * (h // 16) for w, h in sizes)
+    assert sorted(corder.tolist()) == list(range(tot))
