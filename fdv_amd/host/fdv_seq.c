@@ -98,6 +98,35 @@ static void ss_frame(sstream *s, const uint8_t *blob, size_t len) {
     s->n += len;
 }
 
+/* The registered key-frame coder, for the streaming encoder. */
+fdvamd_iframe_encode_fn fdvamd_iframe_encoder_(void) {
+    pthread_mutex_lock(&g_mu);
+    const fdvamd_iframe_encode_fn f = g_ienc;
+    pthread_mutex_unlock(&g_mu);
+    return f;
+}
+
+/* A device encoder for nb sequences in the given search mode (-1: the
+ * process-wide sequence mode), with the registered proposer weights in the
+ * learned mode. */
+fdvamd_encoder *fdvamd_sequence_encoder_(int nb, const int *w, const int *h, int me_mode) {
+    pthread_mutex_lock(&g_mu);
+    if (me_mode < 0) me_mode = g_mode;
+    pthread_mutex_unlock(&g_mu);
+    fdvamd_encoder *e = fdvamd_encoder_open(nb, w, h, me_mode);
+    if (!e || me_mode != 2) return e;
+    pthread_mutex_lock(&g_mu);
+    const int have = g_nnw != NULL;
+    const int prc = have ? fdvamd_set_proposer(e, g_nnw, g_nnn) : -1;
+    pthread_mutex_unlock(&g_mu);
+    if (prc != 0) {
+        if (!have) fdvamd_seterr_("learned mode without proposer weights");
+        fdvamd_encoder_close(e);
+        return NULL;
+    }
+    return e;
+}
+
 /* Key frames of every band in parallel, as the reference's band threads code
  * them (fdv.h:7211-7222): each band's I-frame blob and reconstruction. */
 typedef struct {
@@ -120,8 +149,8 @@ static void *ijob_run(void *arg) {
     return NULL;
 }
 
-static int intra_all(fdvamd_iframe_encode_fn fn, const uint8_t *const *frames, int w, const int *bh, int nb, int qp,
-                     int nthreads, uint8_t **blob, size_t bcap, size_t *blen, uint8_t **rec) {
+int fdvamd_intra_bands_(fdvamd_iframe_encode_fn fn, const uint8_t *const *frames, int w, const int *bh, int nb, int qp,
+                        int nthreads, uint8_t **blob, size_t bcap, size_t *blen, uint8_t **rec) {
     int nt = nthreads < 1 ? 1 : nthreads > nb ? nb : nthreads;
     pthread_t tid[64];
     ijob jobs[64];
@@ -176,21 +205,14 @@ static int code_bands(const uint8_t *const *const *src, int nframes, int w, cons
         rec[b] = malloc((size_t)w * bh[b] + 2 * (size_t)(w / 2) * (bh[b] / 2));
         if (!blob[b] || !rec[b]) { fdvamd_seterr_("out of memory"); goto out; }
     }
-    e = fdvamd_encoder_open(nb, ws, bh, me_mode);
+    e = fdvamd_sequence_encoder_(nb, ws, bh, me_mode);
     if (!e) goto out;
-    if (me_mode == 2) {
-        pthread_mutex_lock(&g_mu);
-        const int have = g_nnw != NULL;
-        const int prc = have ? fdvamd_set_proposer(e, g_nnw, g_nnn) : -1;
-        pthread_mutex_unlock(&g_mu);
-        if (prc != 0) { if (!have) fdvamd_seterr_("learned mode without proposer weights"); goto out; }
-    }
     for (int f = 0; f < nframes; ++f) {
         const int is_intra = f == 0 || (keyint > 0 && f % keyint == 0);
         if (is_intra) {
             /* tables and history restart at a key frame (fdv.h:6784) */
             memset(tc, 0, (size_t)nb * sizeof *tc);
-            if (intra_all(ienc, src[f], w, bh, nb, qp, nthreads, blob, bcap, blen, rec) != 0) {
+            if (fdvamd_intra_bands_(ienc, src[f], w, bh, nb, qp, nthreads, blob, bcap, blen, rec) != 0) {
                 fdvamd_seterr_("the I-frame coder failed on frame %d", f);
                 goto out;
             }

@@ -237,11 +237,15 @@ int fdvamd_pframe_batch(fdvamd_encoder *e, const uint8_t *const *frames, const i
     if (!slots || !ls) { free(slots); free(ls); amd_seterr("out of memory"); return -1; }
     for (int s = 0; s < e->nseq; ++s) ls[s] = lscale ? lscale[s] : 1.0;
     int rc = 0;
-    for (int s = 0; s < e->nseq && rc == 0; ++s)
-        if (fdvgpu_put_source(e->g, s, 0, frames[s]) != 0) rc = -1;
+    for (int s = 0; s < e->nseq && rc == 0; ++s) {
+        /* a held frame (force_skip) reads no source: it may have none */
+        if (!frames[s] && !(force_skip && force_skip[s])) { amd_seterr("sequence %d has no frame", s); rc = -2; }
+        else if (frames[s] && fdvgpu_put_source(e->g, s, 0, frames[s]) != 0) rc = -1;
+    }
     if (rc == 0 && fdvgpu_pframe(e->g, slots, qp, ls, navail, force_skip, e->me_mode) != 0) rc = -1;
     if (rc == 0 && fdvgpu_fetch(e->g) != 0) rc = -1;
     free(slots); free(ls);
+    if (rc == -2) return -1;
     if (rc != 0) { amd_seterr("%s", fdvgpu_last_error()); return -1; }
 
     rc = pack_all(e, -1, qp, tc, out, cap, len);

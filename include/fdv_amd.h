@@ -213,6 +213,77 @@ int fdvamd_pframe_decode_batch(fdvamd_decoder *d, const uint8_t *const *in, cons
 int fdvamd_video_decode(const uint8_t *in, size_t len, uint8_t *out, int *nframes, int *w, int *h);
 int fdvamd_vtile_decode(const uint8_t *in, size_t len, uint8_t *out, int *nframes, int *w, int *h, int nthreads);
 
+/* ---- file container, streaming decode and encode, bitrate control ---------
+ * The reference's public streaming surface with its signatures and semantics;
+ * the P-frames of every band of a frame go through the device in one pass, key
+ * frames through the registered I-frame coder. */
+
+/* fdv_info (fdv.h:1076-1085), same layout. */
+typedef struct {
+    int kind;
+    int version;
+    int fps;
+    int w, h, nframes;
+    int qp, keyint;
+    int nbands;
+    const uint8_t *payload;
+    size_t payload_len;
+} fdvamd_info;
+#define FDVAMD_HEAD 12
+enum { FDVAMD_VIDEO = 0, FDVAMD_TILED = 1 };
+
+/* fdv_wrap / fdv_read / fdv_decode (fdv.h:1090-1105, bodies 7448-7520): the
+ * 12-byte "FDV1" head around a video or tiled stream. */
+size_t fdvamd_wrap(uint8_t *out, size_t cap, int kind, int fps, const uint8_t *payload, size_t payload_len);
+int fdvamd_read(const uint8_t *in, size_t len, fdvamd_info *info);
+int fdvamd_decode(const fdvamd_info *info, uint8_t *out, int nthreads);
+
+/* fdv_dec_* (fdv.h:1126-1142, bodies 7526-7795): a frame at a time, seekable;
+ * the bands of a tiled file step together (they must share their key
+ * interval, as the reference's tiled encoder writes them). */
+typedef struct fdvamd_dec fdvamd_dec;
+fdvamd_dec *fdvamd_dec_open(const fdvamd_info *info);
+int fdvamd_dec_next(fdvamd_dec *d, uint8_t *out);
+int fdvamd_dec_seek(fdvamd_dec *d, int frame);
+int fdvamd_dec_pos(const fdvamd_dec *d);
+size_t fdvamd_dec_frame_size(const fdvamd_dec *d);
+void fdvamd_dec_close(fdvamd_dec *d);
+
+/* fdv_enc_* (fdv.h:1146-1198, bodies 7798-8459): a frame at a time, with held
+ * (all-SKIP) frames, average-bitrate control and its requantising retry (which
+ * keeps the device's motion tables); the output equals fdv_enc_finish's. */
+typedef struct fdvamd_enc fdvamd_enc;
+fdvamd_enc *fdvamd_enc_open(int w, int h, int qp, int keyint);
+fdvamd_enc *fdvamd_enc_open_tiled(int w, int h, int qp, int keyint, int band_mbrows, int nthreads);
+int fdvamd_enc_frame(fdvamd_enc *e, const uint8_t *i420);
+int fdvamd_enc_repeat(fdvamd_enc *e);
+int fdvamd_enc_set_bitrate(fdvamd_enc *e, int bits_per_second, int fps, double bufsecs);
+int fdvamd_enc_count(const fdvamd_enc *e);
+size_t fdvamd_enc_bytes(const fdvamd_enc *e);
+uint8_t *fdvamd_enc_finish(fdvamd_enc *e, int fps, size_t *len);
+void fdvamd_enc_close(fdvamd_enc *e);
+
+/* fdv_rc (fdv.h:1023-1054, bodies 7880-8040), same layout and arithmetic. */
+typedef struct {
+    int bitrate;
+    int fps;
+    double bufbits;
+    int qpmin, qpmax, keyint;
+    double qp_p;
+    double ip_off;
+    double last_target;
+    double fill;
+    int used_qp;
+    double lscale;
+} fdvamd_rc;
+void fdvamd_rc_init(fdvamd_rc *rc, int bitrate, int fps, int w, int h, int keyint, double bufsecs, int qpmin,
+                    int qpmax);
+int fdvamd_rc_pick(fdvamd_rc *rc, int is_intra);
+void fdvamd_rc_update(fdvamd_rc *rc, int is_intra, size_t bytes);
+void fdvamd_rc_skip(fdvamd_rc *rc, size_t bytes);
+void fdvamd_rc_adopt(fdvamd_rc *rc, int is_intra, double q);
+void fdvamd_rc_split(const fdvamd_rc *rc, double q, int *qp, double *lscale);
+
 #ifdef __cplusplus
 }
 #endif

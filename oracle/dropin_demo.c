@@ -1,12 +1,16 @@
-/* dropin_demo.c -- a caller of the reference's sequence API switched to the
- * MI355X path by one include (include/fdv_amd_dropin.h), checked against the
- * reference's own coders in the same program.
+/* dropin_demo.c -- a caller of the reference's sequence, container and
+ * streaming API switched to the MI355X path by one include
+ * (include/fdv_amd_dropin.h), checked against the reference's own functions in
+ * the same program.
  *
  * TEST INFRASTRUCTURE ONLY.  Built by oracle/Makefile from the reference's
  * header where it lies (REF_DIR) into oracle/_ref/; run by
  * tests/test_gpu_dropin.py.  Usage: dropin_demo W H NFRAMES QP KEYINT BAND_MBROWS
  * Prints "dropin ok ..." and exits 0 when the drop-in's streams are byte-
- * identical to the reference's and both decoders return the same frames.
+ * identical to the reference's -- whole-sequence coders, and the streaming
+ * encoder at a fixed QP and under bitrate control, single-stream and tiled,
+ * with held frames -- and both decoders return the same frames, the streaming
+ * ones also after seeks.
  */
 #define FDV_IMPLEMENTATION
 #include "fdv.h"
@@ -23,6 +27,17 @@ static size_t (*const ref_vtile_encode)(const uint8_t *const *, int, int, int, i
                                         size_t) = fdv_vtile_encode;
 static int (*const ref_vtile_decode)(const uint8_t *, size_t, uint8_t *, int *, int *, int *, int) =
     fdv_vtile_decode;
+static fdv_encoder *(*const ref_enc_open)(int, int, int, int) = fdv_enc_open;
+static fdv_encoder *(*const ref_enc_open_tiled)(int, int, int, int, int, int) = fdv_enc_open_tiled;
+static int (*const ref_enc_frame)(fdv_encoder *, const uint8_t *) = fdv_enc_frame;
+static int (*const ref_enc_repeat)(fdv_encoder *) = fdv_enc_repeat;
+static int (*const ref_enc_set_bitrate)(fdv_encoder *, int, int, double) = fdv_enc_set_bitrate;
+static uint8_t *(*const ref_enc_finish)(fdv_encoder *, int, size_t *) = fdv_enc_finish;
+static int (*const ref_read)(const uint8_t *, size_t, fdv_info *) = fdv_read;
+static fdv_decoder *(*const ref_dec_open)(const fdv_info *) = fdv_dec_open;
+static int (*const ref_dec_next)(fdv_decoder *, uint8_t *) = fdv_dec_next;
+static int (*const ref_dec_seek)(fdv_decoder *, int) = fdv_dec_seek;
+static void (*const ref_dec_close)(fdv_decoder *) = fdv_dec_close;
 
 #include "fdv_amd_dropin.h"
 
@@ -50,6 +65,53 @@ static void make_frames(uint8_t *buf, int w, int h, int n) {
                 v[(size_t)r * (w / 2) + c] = (uint8_t)(160 - (r + f) % 48);
             }
     }
+}
+
+/* The streaming encoder through unchanged call sites against the reference's,
+ * frames 3 and 5 preceded by a held frame (5 is a key-frame slot when keyint
+ * divides it); then both files through both streaming decoders, frame by
+ * frame and after seeks.  Returns the number of mismatches. */
+static int stream_case(const uint8_t *const *fp, int n, int w, int h, int qp, int keyint, int tiled, int bmr, int bps) {
+    fdv_encoder *e = tiled ? fdv_enc_open_tiled(w, h, qp, keyint, bmr, 4) : fdv_enc_open(w, h, qp, keyint);
+    fdv_encoder *r = tiled ? ref_enc_open_tiled(w, h, qp, keyint, bmr, 4) : ref_enc_open(w, h, qp, keyint);
+    if (!e || !r) { printf("stream open failed: %s\n", fdvamd_last_error()); return 1; }
+    if (bps && (fdv_enc_set_bitrate(e, bps, 30, 1.0) != 0 || ref_enc_set_bitrate(r, bps, 30, 1.0) != 0)) return 1;
+    int total = 0;
+    for (int f = 0; f < n; ++f) {
+        if (f == 3 || f == 5) {
+            if (fdv_enc_repeat(e) != 0 || ref_enc_repeat(r) != 0) { printf("repeat failed\n"); return 1; }
+            ++total;
+        }
+        if (fdv_enc_frame(e, fp[f]) != 0 || ref_enc_frame(r, fp[f]) != 0) {
+            printf("frame %d failed: %s\n", f, fdvamd_last_error());
+            return 1;
+        }
+        ++total;
+    }
+    size_t la = 0, lb = 0;
+    uint8_t *a = fdv_enc_finish(e, 30, &la), *b = ref_enc_finish(r, 30, &lb);
+    if (!a || !b) { printf("finish failed\n"); return 1; }
+    const int same = la == lb && !memcmp(a, b, la);
+    fdv_info ia, ib;
+    int dsame = fdv_read(a, la, &ia) == 0 && ref_read(b, lb, &ib) == 0 && ia.nframes == total;
+    fdv_decoder *da = dsame ? fdv_dec_open(&ia) : NULL, *db = dsame ? ref_dec_open(&ib) : NULL;
+    const size_t fs = (size_t)w * h * 3 / 2;
+    uint8_t *x = malloc(fs), *y = malloc(fs);
+    if (!da || !db || !x || !y) dsame = 0;
+    for (int k = 0; dsame && k < total; ++k)
+        dsame = fdv_dec_next(da, x) == 1 && ref_dec_next(db, y) == 1 && !memcmp(x, y, fs);
+    dsame = dsame && fdv_dec_next(da, x) == 0 && ref_dec_next(db, y) == 0;
+    const int seeks[4] = {total - 2, 1, total / 2, total / 2 + 1};
+    for (int k = 0; dsame && k < 4; ++k)
+        dsame = fdv_dec_seek(da, seeks[k]) == 0 && ref_dec_seek(db, seeks[k]) == 0 && fdv_dec_next(da, x) == 1 &&
+                ref_dec_next(db, y) == 1 && !memcmp(x, y, fs);
+    printf("fdv_enc_%s%s: %zu B vs the reference's %zu B, streams %s, streaming decode and seeks %s\n",
+           tiled ? "open_tiled" : "open", bps ? " + fdv_enc_set_bitrate" : "", la, lb, same ? "identical" : "DIFFER",
+           dsame ? "identical" : "DIFFER");
+    if (da) fdv_dec_close(da);
+    if (db) ref_dec_close(db);
+    free(a); free(b); free(x); free(y);
+    return !same + !dsame;
 }
 
 int main(int argc, char **argv) {
@@ -81,6 +143,9 @@ int main(int argc, char **argv) {
                dsame ? "identical" : "DIFFER", na ? "" : fdvamd_last_error());
         fails += !same + !dsame;
     }
+    for (int tiled = 0; tiled < 2; ++tiled)
+        for (int rc = 0; rc < 2; ++rc)
+            fails += stream_case(fp, n, w, h, qp, keyint, tiled, bmr, rc ? w * h * 30 / 200 : 0);
     if (!fails) printf("dropin ok %dx%d %d frames qp %d keyint %d\n", w, h, n, qp, keyint);
     free(frames); free(a); free(b); free(da); free(db); free(fp);
     return fails ? 1 : 0;
