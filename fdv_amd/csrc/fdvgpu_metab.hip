@@ -12,7 +12,9 @@
 // One 256-thread workgroup per MB; the source MB and a 64x64 window of each
 // reference are staged in LDS.  Per reference-0 point the four 8x8 quadrant
 // SADs are accumulated separately (the INTER8 searches) and summed for 16x16.
-// SADs use v_sad_u8 on packed bytes, with v_alignbyte for unaligned windows.
+// The full search scores four horizontal offsets at once with
+// v_qsad_pk_u16_u8 on aligned window words; the candidate search uses v_sad_u8
+// on packed bytes, with v_alignbyte for unaligned windows.
 #include <hip/hip_runtime.h>
 #include "fdvgpu_common.h"
 #include "fdvgpu_dev.h"
@@ -127,10 +129,9 @@ k_me_exhaustive(const fdvg_seq *__restrict__ seqs, fdvg_metab *__restrict__ out,
     // (and the quadrants), 108 <= t < 180 the same around the quadrants' 8, t =
     // 180 the zero vector; 181 <= t < 289 the first 12 again, doubled, on the far
     // reference, t = 289 its zero vector
-    const int npts = cand ? 2 : (33 * 33 + 255) / 256;
-    for (int it = 0; it < npts; ++it) {
+    for (int it = 0; cand && it < 2; ++it) {
         int dx, dy, rlo = 0, rhi = nr;
-        if (cand) {
+        {
             const int t = tid + 256 * it, far = t >= 181, u = far ? t - 181 : t;
             if (t >= 290 || (far && nr < 2)) continue;
             const int c = u / 9, o = u % 9, sc = far ? 2 : 1;
@@ -143,10 +144,6 @@ k_me_exhaustive(const fdvg_seq *__restrict__ seqs, fdvg_metab *__restrict__ out,
             dy = zero ? 0 : sc * vy + o / 3 - 1;
             if (dx < -16 || dx > 16 || dy < -16 || dy > 16) continue;
             rlo = far; rhi = far + 1;
-        } else {
-            const int pidx = tid + 256 * it;
-            if (pidx >= 33 * 33) continue;
-            dy = pidx / 33 - 16; dx = pidx % 33 - 16;
         }
         const int pidx = (dy + 16) * 33 + dx + 16;
         const unsigned long long tie = ((unsigned long long)(abs(dx) + abs(dy)) << 16) | (unsigned)pidx;
@@ -173,6 +170,43 @@ k_me_exhaustive(const fdvg_seq *__restrict__ seqs, fdvg_metab *__restrict__ out,
 #pragma unroll
                 for (int qd = 0; qd < 4; ++qd)
                     best[2 + qd] = umin64(best[2 + qd], ((unsigned long long)qs[qd] << 32) | tie);
+        }
+    }
+    // full search: an item is (reference, dy, 4 consecutive dx from an
+    // aligned window word); v_qsad_pk_u16_u8 scores the 4 offsets of 4 source
+    // bytes against 8 window bytes at once, 16-bit sums (a 16x16 SAD fits),
+    // one accumulator per 8x8 quadrant.  Same keys, so the same minima.
+    for (int it = tid; !cand && it < nr * 33 * 9; it += 256) {
+        const int r = it / (33 * 9), rem = it % (33 * 9), dy = rem / 9 - 16, dx0 = 4 * (rem % 9) - 16;
+        const uint32_t *wrow = (const uint32_t *)(s.win[r] + (WOFF + dy) * WIN) + ((WOFF + dx0) >> 2);
+        uint64_t acc[4] = {0, 0, 0, 0};
+#pragma unroll
+        for (int y = 0; y < 16; ++y) {
+            const uint32_t *w = wrow + y * (WIN / 4);
+            const uint32_t w0 = w[0], w1 = w[1], w2 = w[2], w3 = w[3], w4 = w[4];
+            const int q = (y >> 3) * 2;
+            acc[q] = __builtin_amdgcn_qsad_pk_u16_u8((uint64_t)w1 << 32 | w0, s.cur32[y * 4 + 0], acc[q]);
+            acc[q] = __builtin_amdgcn_qsad_pk_u16_u8((uint64_t)w2 << 32 | w1, s.cur32[y * 4 + 1], acc[q]);
+            acc[q + 1] = __builtin_amdgcn_qsad_pk_u16_u8((uint64_t)w3 << 32 | w2, s.cur32[y * 4 + 2], acc[q + 1]);
+            acc[q + 1] = __builtin_amdgcn_qsad_pk_u16_u8((uint64_t)w4 << 32 | w3, s.cur32[y * 4 + 3], acc[q + 1]);
+        }
+#pragma unroll
+        for (int o = 0; o < 4; ++o) {
+            const int dx = dx0 + o;
+            if (dx > 16) continue;
+            const int pidx = (dy + 16) * 33 + dx + 16;
+            const unsigned long long tie = ((unsigned long long)(abs(dx) + abs(dy)) << 16) | (unsigned)pidx;
+            unsigned qs[4];
+#pragma unroll
+            for (int qd = 0; qd < 4; ++qd) qs[qd] = (unsigned)(acc[qd] >> (16 * o)) & 0xffffu;
+            const unsigned long long k16 = ((unsigned long long)(qs[0] + qs[1] + qs[2] + qs[3]) << 32) | tie;
+            if (r == 0) {
+                best[0] = umin64(best[0], k16);
+#pragma unroll
+                for (int qd = 0; qd < 4; ++qd) best[2 + qd] = umin64(best[2 + qd], ((unsigned long long)qs[qd] << 32) | tie);
+            } else {
+                best[1] = umin64(best[1], k16);
+            }
         }
     }
     const int wv = tid >> 6, L = tid & 63;
