@@ -580,9 +580,58 @@ static void block4(const uint8_t *cur, int cs, const uint8_t *pred, int ps, uint
     *bits = b;
 }
 
-/* An n x n residual (n = 16, 8 or 4): for each aligned 8x8 region the cheaper
- * in D + lambda*R of four RDOQ'd 4x4 transforms and one plain 8x8 transform,
- * ties to the 4x4 option; an option that codes no level costs only the flag
+/* One aligned 8x8 region of a residual: the cheaper in D + lambda*R of four
+ * RDOQ'd 4x4 transforms and one plain 8x8 transform, ties to the 4x4 option;
+ * an option that codes no level costs only the flag (the loop body of
+ * fdv_code_residual_inner, fdv.h:5322-5410).  cb / pb are the region's source
+ * and prediction, rb gets its reconstruction; flag 0 empty, 1 four 4x4 (lev
+ * holds them as four raster blocks of 16), 2 one 8x8 (lev raster). */
+static void region8(const uint8_t *cb, int cs, const uint8_t *pb, int ps, int qp, double lambda,
+                    uint8_t *rb, int rs, uint8_t *flag, int16_t lev[64], double *e, int *b) {
+    /* option A: four 4x4 */
+    uint8_t recA[64];
+    int16_t levA[64];
+    double eA = 0.0; int bA = 2, anyA = 0;    /* FDV_BITS_FLAG (fdv.h:3354) */
+    for (int s = 0; s < 4; ++s) {
+        int sx = (s & 1) * 4, sy = (s >> 1) * 4;
+        int32_t e4; int b4;
+        block4(cb + sy * cs + sx, cs, pb + sy * ps + sx, ps, recA + sy * 8 + sx, 8,
+               qp, lambda, levA + 16 * s, &e4, &b4);
+        eA += (double)e4; bA += b4;
+        for (int i = 0; i < 16; ++i) anyA |= levA[16 * s + i] != 0;
+    }
+    /* option B: one 8x8, no RDOQ */
+    int16_t res8[64], lev8[64], rr8[64];
+    int32_t co8[64], dq8[64];
+    for (int i = 0; i < 64; ++i) res8[i] = (int16_t)(cb[(i >> 3) * cs + (i & 7)] - pb[(i >> 3) * ps + (i & 7)]);
+    orc_fdct8(res8, co8);
+    orc_quant8(co8, lev8, qp);
+    orc_dequant8(lev8, dq8, qp);
+    orc_idct8(dq8, rr8);
+    uint8_t recB[64];
+    int32_t eBi = 0;
+    for (int i = 0; i < 64; ++i) {
+        int v = clip8(pb[(i >> 3) * ps + (i & 7)] + rr8[i]);
+        recB[i] = (uint8_t)v;
+        int d = cb[(i >> 3) * cs + (i & 7)] - v;
+        eBi += d * d;
+    }
+    double eB = (double)eBi;
+    int last8 = -1;
+    for (int k = 0; k < 64; ++k) if (lev8[SCAN8[k]]) last8 = k;
+    int bB = 2 + orc_bits_val((uint32_t)(last8 + 1));
+    for (int k = 0; k <= last8; ++k) bB += orc_bits_val(zmap(lev8[SCAN8[k]]));
+    if (!anyA) bA = 2;
+    if (last8 < 0) bB = 2;
+    const int takeA = eA + lambda * bA <= eB + lambda * bB;
+    *flag = takeA ? (anyA ? 1 : 0) : (last8 >= 0 ? 2 : 0);
+    memcpy(lev, takeA ? levA : lev8, 64 * sizeof *lev);
+    for (int i = 0; i < 64; ++i) rb[(i >> 3) * rs + (i & 7)] = takeA ? recA[i] : recB[i];
+    *e = takeA ? eA : eB;
+    *b = takeA ? bA : bB;
+}
+
+/* An n x n residual (n = 16, 8 or 4), region by region in raster order
  * (fdv_code_residual_inner, fdv.h:5300-5414).  pred is n x n packed; rec gets
  * the n x n reconstruction.  Adds to *ssd and *bits like the reference. */
 static void residual(const uint8_t *cur, int cs, int ox, int oy, const uint8_t *pred, int n,
@@ -600,54 +649,10 @@ static void residual(const uint8_t *cur, int cs, int ox, int oy, const uint8_t *
     int r = 0;
     for (int ry = 0; ry < n; ry += 8)
         for (int rx = 0; rx < n; rx += 8, ++r) {
-            const uint8_t *cb = cur + (ptrdiff_t)(oy + ry) * cs + ox + rx;
-            const uint8_t *pb = pred + ry * n + rx;
-            /* option A: four 4x4 */
-            uint8_t recA[64];
-            int16_t levA[64];
-            double eA = 0.0; int bA = 2, anyA = 0;    /* FDV_BITS_FLAG (fdv.h:3354) */
-            for (int s = 0; s < 4; ++s) {
-                int sx = (s & 1) * 4, sy = (s >> 1) * 4;
-                int32_t e; int b;
-                block4(cb + sy * cs + sx, cs, pb + sy * n + sx, n, recA + sy * 8 + sx, 8,
-                       qp, lambda, levA + 16 * s, &e, &b);
-                eA += (double)e; bA += b;
-                for (int i = 0; i < 16; ++i) anyA |= levA[16 * s + i] != 0;
-            }
-            /* option B: one 8x8, no RDOQ */
-            int16_t res8[64], lev8[64], rr8[64];
-            int32_t co8[64], dq8[64];
-            for (int i = 0; i < 64; ++i) res8[i] = (int16_t)(cb[(i >> 3) * cs + (i & 7)] - pb[(i >> 3) * n + (i & 7)]);
-            orc_fdct8(res8, co8);
-            orc_quant8(co8, lev8, qp);
-            orc_dequant8(lev8, dq8, qp);
-            orc_idct8(dq8, rr8);
-            uint8_t recB[64];
-            int32_t eBi = 0;
-            for (int i = 0; i < 64; ++i) {
-                int v = clip8(pb[(i >> 3) * n + (i & 7)] + rr8[i]);
-                recB[i] = (uint8_t)v;
-                int d = cb[(i >> 3) * cs + (i & 7)] - v;
-                eBi += d * d;
-            }
-            double eB = (double)eBi;
-            int last8 = -1;
-            for (int k = 0; k < 64; ++k) if (lev8[SCAN8[k]]) last8 = k;
-            int bB = 2 + orc_bits_val((uint32_t)(last8 + 1));
-            for (int k = 0; k <= last8; ++k) bB += orc_bits_val(zmap(lev8[SCAN8[k]]));
-            if (!anyA) bA = 2;
-            if (last8 < 0) bB = 2;
-            if (eA + lambda * bA <= eB + lambda * bB) {
-                out->flag[r] = anyA ? 1 : 0;
-                memcpy(out->lev[r], levA, sizeof levA);
-                for (int i = 0; i < 64; ++i) rec[(ry + (i >> 3)) * n + rx + (i & 7)] = recA[i];
-                et += eA; bt += bA;
-            } else {
-                out->flag[r] = last8 >= 0 ? 2 : 0;
-                memcpy(out->lev[r], lev8, sizeof lev8);
-                for (int i = 0; i < 64; ++i) rec[(ry + (i >> 3)) * n + rx + (i & 7)] = recB[i];
-                et += eB; bt += bB;
-            }
+            double e; int b;
+            region8(cur + (ptrdiff_t)(oy + ry) * cs + ox + rx, cs, pred + ry * n + rx, n, qp, lambda,
+                    rec + ry * n + rx, n, &out->flag[r], out->lev[r], &e, &b);
+            et += e; bt += b;
         }
     *ssd += et; *bits += bt;
 }
@@ -1704,4 +1709,492 @@ int orc_pframe_par_exact(const uint8_t *cy, const uint8_t *cu, const uint8_t *cv
 done:
     free(sb); free(used); free(cons); free(rec); free(sbuf); free(dstate);
     return rc;
+}
+
+/* ------------------------------------------------------- the key frame -- */
+
+/* The intra-only plane codec (fdv_image_encode, fdv.h:4845-5038), restated as
+ * two separate passes: a decision pass that walks the rate-distortion quadtree
+ * and records the winner of every node in position-indexed planes (split
+ * flags per depth, leaf modes, region flags, levels, reconstruction), and a
+ * serialising pass that walks the decided tree in coding order and emits the
+ * twelve streams.  The reference emits while it decides, keeping the
+ * symbols of both trials of a node until one wins; the stream contents are
+ * the same because a stream holds exactly the winners' symbols in pre-order. */
+
+#define ICTU 64                                   /* FDV_CTU (fdv.h:3651) */
+enum { IS_SPLIT = 0, IS_MODEB = 4, IS_MODE0, IS_MODE1, IS_CNT, IS_LVL, IS_CNT8, IS_LVL8, IS_FLAG, IS_NS };
+
+/* Z-order rank of 4x4 cell (lx, ly) inside a tree unit: vertical bit above the
+ * horizontal one at each level (zidx4, fdv.h:3659). */
+static int izrank(int lx, int ly) {
+    int z = 0;
+    for (int b = ICTU / 8; b; b >>= 1) z = (z << 2) | ((ly & b) ? 2 : 0) | ((lx & b) ? 1 : 0);
+    return z;
+}
+/* Sample (x, y) is reconstructed while the 4x4 block at (bx, by) is coded:
+ * an earlier unit row, an earlier unit of this row, or an earlier cell of
+ * this unit in z-order (cell_done, fdv.h:3677). */
+static int idone(int x, int y, int bx, int by) {
+    int ux = bx & ~(ICTU - 1), uy = by & ~(ICTU - 1);
+    if (y < uy) return 1;
+    if (x < ux) return 1;
+    if (x >= ux + ICTU) return 0;
+    return izrank((x - ux) >> 2, (y - uy) >> 2) < izrank((bx - ux) >> 2, (by - uy) >> 2);
+}
+
+/* The nine 4x4 predictors of H.264 (8.3.1.2), with the edge laid out as one
+ * array: E[3 - i] = left sample i, E[4] = the corner, E[5 + i] = top sample i
+ * (i = 0..7, top-right included).  DC follows the availability flags
+ * (fdv_intra_predict_4x4, fdv.h:2698). */
+static int f3(int a, int b, int c) { return (a + 2 * b + c + 2) >> 2; }
+static int f2(int a, int b) { return (a + b + 1) >> 1; }
+void orc_intra4(int mode, const uint8_t top[8], const uint8_t left[4], uint8_t tl, int ht, int hl,
+                uint8_t pred[16]) {
+    int E[13];
+    for (int i = 0; i < 4; ++i) E[3 - i] = left[i];
+    E[4] = tl;
+    for (int i = 0; i < 8; ++i) E[5 + i] = top[i];
+#define TP(i) E[5 + (i)]
+#define LF(i) E[3 - (i)]
+    for (int y = 0; y < 4; ++y)
+        for (int x = 0; x < 4; ++x) {
+            int v;
+            switch (mode) {
+            case 1: v = TP(x); break;                                    /* vertical */
+            case 2: v = LF(y); break;                                    /* horizontal */
+            case 3: v = (x == 3 && y == 3) ? (TP(6) + 3 * TP(7) + 2) >> 2   /* diagonal down-left */
+                                           : f3(TP(x + y), TP(x + y + 1), TP(x + y + 2)); break;
+            case 4: v = f3(E[3 + x - y], E[4 + x - y], E[5 + x - y]); break;   /* diagonal down-right */
+            case 5: {                                                    /* vertical-right */
+                int z = 2 * x - y, k = x - (y >> 1);
+                if (z >= 0) v = (z & 1) ? f3(E[3 + k], E[4 + k], E[5 + k]) : f2(E[4 + k], E[5 + k]);
+                else if (z == -1) v = f3(E[3], E[4], E[5]);
+                else v = f3(E[4 - y], E[5 - y], E[6 - y]);
+                break;
+            }
+            case 6: {                                                    /* horizontal-down */
+                int z = 2 * y - x, k = y - (x >> 1);
+                if (z >= 0) v = (z & 1) ? f3(E[5 - k], E[4 - k], E[3 - k]) : f2(E[4 - k], E[3 - k]);
+                else if (z == -1) v = f3(E[5], E[4], E[3]);
+                else v = f3(E[4 + x], E[3 + x], E[2 + x]);
+                break;
+            }
+            case 7: {                                                    /* vertical-left */
+                int k = x + (y >> 1);
+                v = (y & 1) ? f3(TP(k), TP(k + 1), TP(k + 2)) : f2(TP(k), TP(k + 1));
+                break;
+            }
+            case 8: {                                                    /* horizontal-up */
+                int z = x + 2 * y, k = y + (x >> 1);
+                if (z < 5) v = (z & 1) ? f3(LF(k), LF(k + 1), LF(k + 2)) : f2(LF(k), LF(k + 1));
+                else if (z == 5) v = (LF(2) + 3 * LF(3) + 2) >> 2;
+                else v = LF(3);
+                break;
+            }
+            default: {                                                   /* DC */
+                int s = 0;
+                if (ht && hl) { for (int i = 0; i < 4; ++i) s += top[i] + left[i]; v = (s + 4) >> 3; }
+                else if (ht) { for (int i = 0; i < 4; ++i) s += top[i]; v = (s + 2) >> 2; }
+                else if (hl) { for (int i = 0; i < 4; ++i) s += left[i]; v = (s + 2) >> 2; }
+                else v = 128;
+            }
+            }
+            pred[y * 4 + x] = (uint8_t)v;
+        }
+#undef TP
+#undef LF
+}
+
+/* Which 4x4 modes their neighbours allow (mode_ok, fdv.h:3709). */
+static int m4_ok(int m, int ht, int htr, int hl, int htl) {
+    switch (m) {
+    case 0: return 1;
+    case 1: return ht;
+    case 2: case 8: return hl;
+    case 3: case 7: return ht && htr;
+    default: return ht && hl && htl;
+    }
+}
+
+typedef struct {
+    const uint8_t *src; int ss, w, h, qp; double lambda;
+    orc_itree *t;
+} iplane_t;
+
+static int inodes_w(int w, int d) { int n = ICTU >> d; return (w + n - 1) / n; }
+
+/* A 4x4 leaf: the SATD screen over the allowed modes, the two best (all of
+ * them below QP 12) coded without RDOQ and compared in D + lambda*R, pruned
+ * once a candidate's SATD passes 5/4 of the best; the winner recoded with
+ * RDOQ (intra_leaf4, fdv.h:3907-4108).  Returns the winner's cost. */
+static double ileaf4(iplane_t *P, int bx, int by) {
+    orc_itree *t = P->t;
+    const int w = P->w, qp = P->qp;
+    const double lambda = P->lambda;
+    uint8_t *rec = t->rec;
+    const int ht = by > 0, hl = bx > 0, htl = ht && hl;
+    const int htr = ht && bx + 4 < w && idone(bx + 4, by - 1, bx, by);
+    uint8_t top[8] = {0}, left[4] = {0}, tl = 128;
+    if (ht) for (int j = 0; j < 8; ++j) top[j] = rec[(size_t)(by - 1) * w + bx + (j < 4 || htr ? j : 3)];
+    if (hl) for (int i = 0; i < 4; ++i) left[i] = rec[(size_t)(by + i) * w + bx - 1];
+    if (htl) tl = rec[(size_t)(by - 1) * w + bx - 1];
+    const uint8_t *sb = P->src + (size_t)by * P->ss + bx;
+
+    uint8_t pred[9][16];
+    int sad[9], md[9], nc = 0;
+    for (int m = 0; m < 9; ++m) {
+        if (!m4_ok(m, ht, htr, hl, htl)) continue;
+        orc_intra4(m, top, left, tl, ht, hl, pred[nc]);
+        sad[nc] = orc_satd4(sb, P->ss, pred[nc]);
+        md[nc] = m;
+        ++nc;
+    }
+    /* order the first ntry by SATD: selection with exchanges, as the reference
+     * does it (the exchange moves the displaced entry, which matters for ties) */
+    const int ntry = qp < 12 ? nc : (nc < 2 ? nc : 2);
+    int ord[9];
+    for (int i = 0; i < nc; ++i) ord[i] = i;
+    for (int a = 0; a < ntry; ++a) {
+        int bst = a;
+        for (int b = a + 1; b < nc; ++b) if (sad[ord[b]] < sad[ord[bst]]) bst = b;
+        int tmp = ord[a]; ord[a] = ord[bst]; ord[bst] = tmp;
+    }
+    int16_t res[16];
+    double best = -1.0;
+    int bc = 0;
+    for (int c = 0; c < ntry; ++c) {
+        if (c > 0 && qp >= 12 && (int64_t)sad[ord[c]] * 4 > (int64_t)sad[ord[0]] * 5) break;
+        const uint8_t *pp = pred[ord[c]];
+        int16_t lv[16], rr[16];
+        int32_t co[16], dq[16];
+        for (int i = 0; i < 16; ++i) res[i] = (int16_t)(sb[(i >> 2) * P->ss + (i & 3)] - pp[i]);
+        orc_fdct4(res, co);
+        orc_quant4(co, lv, qp);
+        orc_dequant4(lv, dq, qp);
+        orc_idct4(dq, rr);
+        int32_t e = 0;
+        for (int i = 0; i < 16; ++i) { int d = sb[(i >> 2) * P->ss + (i & 3)] - clip8(pp[i] + rr[i]); e += d * d; }
+        int last = -1;
+        for (int k = 0; k < 16; ++k) if (lv[SCAN4[k]]) last = k;
+        int bits = 3 + orc_bits_val((uint32_t)(last + 1));          /* FDV_BITS_MODE (fdv.h:3351) */
+        for (int k = 0; k <= last; ++k) bits += orc_bits_val(zmap(lv[SCAN4[k]]));
+        double J = (double)e + lambda * bits;
+        if (best < 0.0 || J < best) { best = J; bc = c; }
+    }
+    const uint8_t *pp = pred[ord[bc]];
+    int16_t lv[16], rr[16];
+    int32_t co[16], dq[16];
+    for (int i = 0; i < 16; ++i) res[i] = (int16_t)(sb[(i >> 2) * P->ss + (i & 3)] - pp[i]);
+    orc_fdct4(res, co);
+    orc_quant4(co, lv, qp);
+    orc_rdoq4(lv, res, qp, lambda);
+    orc_dequant4(lv, dq, qp);
+    orc_idct4(dq, rr);
+    int32_t e = 0;
+    for (int i = 0; i < 16; ++i) {
+        int v = clip8(pp[i] + rr[i]);
+        rec[(size_t)(by + (i >> 2)) * w + bx + (i & 3)] = (uint8_t)v;
+        t->lev[(size_t)(by + (i >> 2)) * w + bx + (i & 3)] = lv[i];
+        int d = sb[(i >> 2) * P->ss + (i & 3)] - v;
+        e += d * d;
+    }
+    int last = -1;
+    for (int k = 0; k < 16; ++k) if (lv[SCAN4[k]]) last = k;
+    int bits = 3 + orc_bits_val((uint32_t)(last + 1));
+    for (int k = 0; k <= last; ++k) bits += orc_bits_val(zmap(lv[SCAN4[k]]));
+    t->cmode[(size_t)(by / 4) * (w / 4) + bx / 4] = (uint8_t)md[ord[bc]];
+    return (double)e + lambda * bits;
+}
+
+/* An n x n leaf (n = 8 .. 64): each allowed DC / V / H / PLANE prediction from
+ * the unfiltered neighbours, its residual coded region by region; the first
+ * lowest D + lambda*R wins (intra_leafn, fdv.h:4442-4484).  The winner goes to
+ * the planes only if the leaf beats the split, so it is returned in buffers. */
+static double ileafn(iplane_t *P, int bx, int by, int n, int *mode, uint8_t *brec, int16_t *blev,
+                     uint8_t *bflag) {
+    const orc_itree *t = P->t;
+    const int w = P->w;
+    const int ht = by > 0, hl = bx > 0;
+    uint8_t top[ICTU], left[ICTU], tl = 128, pred[ICTU * ICTU], trec[ICTU * ICTU], tflag[64];
+    static __thread int16_t tlev[ICTU * ICTU];
+    for (int k = 0; k < n; ++k) {
+        top[k] = ht ? t->rec[(size_t)(by - 1) * w + bx + k] : 0;
+        left[k] = hl ? t->rec[(size_t)(by + k) * w + bx - 1] : 0;
+    }
+    if (ht && hl) tl = t->rec[(size_t)(by - 1) * w + bx - 1];
+    double best = -1.0;
+    const int nr = n / 8;
+    for (int m = 0; m < 4; ++m) {
+        if (!nn_ok(m, n, ht, hl)) continue;
+        orc_intra_nxn(m, top, left, tl, n, ht, hl, pred);
+        double D = 0.0;
+        int bits = 3;                                                  /* the mode symbol */
+        for (int r = 0; r < nr * nr; ++r) {
+            const int rx = (r % nr) * 8, ry = (r / nr) * 8;
+            double e; int b;
+            region8(P->src + (size_t)(by + ry) * P->ss + bx + rx, P->ss, pred + ry * n + rx, n, P->qp,
+                    P->lambda, trec + ry * n + rx, n, &tflag[r], tlev + 64 * r, &e, &b);
+            D += e; bits += b;
+        }
+        double J = D + P->lambda * bits;
+        if (best < 0.0 || J < best) {
+            best = J; *mode = m;
+            memcpy(brec, trec, (size_t)n * n);
+            memcpy(blev, tlev, (size_t)n * n * sizeof *blev);
+            memcpy(bflag, tflag, (size_t)nr * nr);
+        }
+    }
+    return best;
+}
+
+/* A 4x4-mode-context value for a large leaf's cells (nn_to_4x4_mode, fdv.h:4400). */
+static int nn_ctx(int m) { return m == 1 ? 1 : m == 2 ? 2 : 0; }
+
+/* One node: implied split off the plane's edge; else the split (children in
+ * z-order) against the node as one leaf, the leaf on ties (intra_node,
+ * fdv.h:4494-4584).  Returns the winner's cost; the planes hold the winner. */
+static double inode(iplane_t *P, int bx, int by, int d) {
+    orc_itree *t = P->t;
+    const int n = ICTU >> d, hn = n / 2, w = P->w, h = P->h;
+    if (n == 4) return ileaf4(P, bx, by);
+    const size_t ni = (size_t)(by / n) * inodes_w(w, d) + bx / n;
+    if (bx + n > w || by + n > h) {
+        t->split[d][ni] = 2;
+        double J = 0.0;
+        for (int k = 0; k < 4; ++k) {
+            int cx = bx + (k & 1) * hn, cy = by + (k >> 1) * hn;
+            if (cx < w && cy < h) J += inode(P, cx, cy, d + 1);
+        }
+        return J;
+    }
+    double Js = 0.0;
+    for (int k = 0; k < 4; ++k) Js += inode(P, bx + (k & 1) * hn, by + (k >> 1) * hn, d + 1);
+    uint8_t brec[ICTU * ICTU], bflag[64];
+    static __thread int16_t blev[ICTU * ICTU];
+    int mode = 0;
+    const double Jl = ileafn(P, bx, by, n, &mode, brec, blev, bflag);
+    const int split = Js < Jl;
+    t->split[d][ni] = (uint8_t)split;
+    if (split) return Js;
+    t->nmode[d][ni] = (uint8_t)mode;
+    const int nr = n / 8, rw = (w + 7) / 8;
+    for (int y = 0; y < n; ++y)
+        memcpy(t->rec + (size_t)(by + y) * w + bx, brec + y * n, (size_t)n);
+    for (int r = 0; r < nr * nr; ++r) {
+        const int rx = bx + (r % nr) * 8, ry = by + (r / nr) * 8;
+        t->rflag[(size_t)(ry / 8) * rw + rx / 8] = bflag[r];
+        for (int i = 0; i < 64; ++i) {
+            /* flag 1: four 4x4 blocks of 16 raster levels; else one 8x8 raster */
+            const int y = bflag[r] == 1 ? ((i >> 5) << 2) + ((i >> 2) & 3) : i >> 3;
+            const int x = bflag[r] == 1 ? (((i >> 4) & 1) << 2) + (i & 3) : i & 7;
+            t->lev[(size_t)(ry + y) * w + rx + x] = blev[64 * r + i];
+        }
+    }
+    for (int y = 0; y < n / 4; ++y)
+        memset(t->cmode + (size_t)(by / 4 + y) * (w / 4) + bx / 4, nn_ctx(mode), (size_t)n / 4);
+    return Jl;
+}
+
+void orc_image_analyse(const uint8_t *src, int w, int h, int stride, int qp, orc_itree *t) {
+    iplane_t P = {src, stride, w, h, qp, orc_lambda(qp, 1.0), t};
+    memset(t->cmode, 0, (size_t)(w / 4) * (h / 4));
+    for (int y = 0; y < h; y += ICTU)
+        for (int x = 0; x < w; x += ICTU) inode(&P, x, y, 0);
+}
+
+/* ---- serialising the decided tree */
+
+typedef struct { uint8_t *b[IS_NS]; size_t n[IS_NS], cap; int over; } isink_t;
+static void iput(isink_t *s, int k, unsigned v) {
+    if (s->n[k] >= s->cap) { s->over = 1; return; }
+    s->b[k][s->n[k]++] = (uint8_t)v;
+}
+static void iput_leb(isink_t *s, int k, uint32_t u) {
+    if (s->n[k] + 5 > s->cap) { s->over = 1; return; }
+    s->n[k] = leb_write(s->b[k], s->n[k], u);
+}
+/* count + levels of the 4x4 block at (x, y) of the level plane */
+static void iemit4(isink_t *s, const int16_t *lev, int w, int x, int y) {
+    int16_t l[16];
+    for (int i = 0; i < 16; ++i) l[i] = lev[(size_t)(y + (i >> 2)) * w + x + (i & 3)];
+    int last = -1;
+    for (int k = 0; k < 16; ++k) if (l[SCAN4[k]]) last = k;
+    iput(s, IS_CNT, (unsigned)(last + 1));
+    for (int k = 0; k <= last; ++k) iput_leb(s, IS_LVL, zmap(l[SCAN4[k]]));
+}
+static void iser(const orc_itree *t, int w, int h, int bx, int by, int d, isink_t *s) {
+    const int n = ICTU >> d, hn = n / 2;
+    if (n == 4) {
+        const int cx = bx / 4, cy = by / 4, mw = w / 4;
+        const int L = cx > 0 ? t->cmode[(size_t)cy * mw + cx - 1] : 0;
+        const int A = cy > 0 ? t->cmode[(size_t)(cy - 1) * mw + cx] : 0;
+        iput(s, L == A ? IS_MODE0 : IS_MODE1, t->cmode[(size_t)cy * mw + cx]);   /* intra_mode_ctx, fdv.h:4194 */
+        iemit4(s, t->lev, w, bx, by);
+        return;
+    }
+    const size_t ni = (size_t)(by / n) * inodes_w(w, d) + bx / n;
+    const int sp = t->split[d][ni];
+    if (sp != 2) iput(s, IS_SPLIT + d, (unsigned)sp);
+    if (sp) {
+        for (int k = 0; k < 4; ++k) {
+            int cx = bx + (k & 1) * hn, cy = by + (k >> 1) * hn;
+            if (cx < w && cy < h) iser(t, w, h, cx, cy, d + 1, s);
+        }
+        return;
+    }
+    iput(s, IS_MODEB, t->nmode[d][ni]);
+    const int rw = (w + 7) / 8;
+    for (int ry = by; ry < by + n; ry += 8)
+        for (int rx = bx; rx < bx + n; rx += 8) {
+            const int f = t->rflag[(size_t)(ry / 8) * rw + rx / 8];
+            iput(s, IS_FLAG, (unsigned)f);
+            if (f == 1) {
+                for (int q = 0; q < 4; ++q) iemit4(s, t->lev, w, rx + (q & 1) * 4, ry + (q >> 1) * 4);
+            } else if (f == 2) {
+                int16_t l[64];
+                for (int i = 0; i < 64; ++i) l[i] = t->lev[(size_t)(ry + (i >> 3)) * w + rx + (i & 7)];
+                int last = -1;
+                for (int k = 0; k < 64; ++k) if (l[SCAN8[k]]) last = k;
+                iput(s, IS_CNT8, (unsigned)(last + 1));
+                for (int k = 0; k <= last; ++k) iput_leb(s, IS_LVL8, zmap(l[SCAN8[k]]));
+            }
+        }
+}
+
+/* The container of one plane (fdv.h:4926-5030): the twelve streams, then the
+ * smallest of one shared static table, a static table per non-empty stream,
+ * and the adaptive coder from flat models (ties to the lower mode). */
+size_t orc_image_pack(const orc_itree *t, int w, int h, int qp, uint8_t *out, size_t cap) {
+    isink_t s;
+    memset(&s, 0, sizeof s);
+    s.cap = (size_t)w * h * 5 + 1024;
+    for (int k = 0; k < IS_NS; ++k) s.b[k] = malloc(s.cap);
+    size_t o = 0;
+    uint8_t *all = NULL, *p1 = NULL, *pay[IS_NS] = {0}, *pada = NULL;
+    for (int k = 0; k < IS_NS; ++k) if (!s.b[k]) goto done;
+    for (int y = 0; y < h; y += ICTU)
+        for (int x = 0; x < w; x += ICTU) iser(t, w, h, x, y, 0, &s);
+    if (s.over) goto done;
+    size_t total = 0;
+    for (int k = 0; k < IS_NS; ++k) total += s.n[k];
+    all = malloc(total ? total : 1);
+    p1 = malloc(total * 2 + 1024);
+    if (!all || !p1) goto done;
+    uint32_t c1[256] = {0};
+    {
+        size_t q = 0;
+        for (int k = 0; k < IS_NS; ++k) { memcpy(all + q, s.b[k], s.n[k]); q += s.n[k]; }
+    }
+    for (size_t i = 0; i < total; ++i) c1[all[i]]++;
+    uint16_t f1[256], fk[IS_NS][256];
+    freq_norm(c1, f1);
+    const size_t l1 = rans_enc(all, total, f1, p1, total * 2 + 1024);
+    size_t len[IS_NS], hdr = 0;
+    for (int k = 0; k < IS_NS; ++k) {
+        uint32_t ck[256] = {0};
+        for (size_t i = 0; i < s.n[k]; ++i) ck[s.b[k][i]]++;
+        freq_norm(ck, fk[k]);
+        pay[k] = malloc(s.n[k] * 2 + 1024);
+        if (!pay[k]) goto done;
+        len[k] = rans_enc(s.b[k], s.n[k], fk[k], pay[k], s.n[k] * 2 + 1024);
+        hdr += (size_t)leb_len((uint32_t)s.n[k]);
+    }
+    const size_t size_one = hdr + freqs_size(f1) + (size_t)leb_len((uint32_t)l1) + l1;
+    size_t size_two = hdr;
+    for (int k = 0; k < IS_NS; ++k)
+        if (s.n[k]) size_two += freqs_size(fk[k]) + (size_t)leb_len((uint32_t)len[k]) + len[k];
+    size_t size_ada = (size_t)-1, lada = 0;
+    pada = malloc(total * 3 + 4096);
+    if (pada) {
+        rc_t e = {pada, total * 3 + 4096, 0, 0, 0xFFFFFFFFu, 0, 1, 0};
+        const uint32_t flat[256] = {0};
+        for (int k = 0; k < IS_NS; ++k) {
+            amod_t m; amod_init(&m, flat);
+            for (size_t i = 0; i < s.n[k]; ++i) rc_put(&e, &m, s.b[k][i]);
+        }
+        lada = rc_done(&e);
+        if (lada) size_ada = hdr + (size_t)leb_len((uint32_t)lada) + lada;
+    }
+    int mode = 0; size_t best = size_one;
+    if (size_two < best) { best = size_two; mode = 1; }
+    if (size_ada < best) { best = size_ada; mode = 2; }
+    if (5 + best > cap) goto done;
+    out[o++] = (uint8_t)w; out[o++] = (uint8_t)(w >> 8);
+    out[o++] = (uint8_t)h; out[o++] = (uint8_t)(h >> 8);
+    out[o++] = (uint8_t)qp;
+    out[o++] = (uint8_t)mode;
+    for (int k = 0; k < IS_NS; ++k) o = leb_write(out, o, (uint32_t)s.n[k]);
+    if (mode == 2) {
+        o = leb_write(out, o, (uint32_t)lada); memcpy(out + o, pada, lada); o += lada;
+    } else if (mode == 0) {
+        o = freqs_write(out, o, f1); o = leb_write(out, o, (uint32_t)l1); memcpy(out + o, p1, l1); o += l1;
+    } else {
+        for (int k = 0; k < IS_NS; ++k) {
+            if (!s.n[k]) continue;
+            o = freqs_write(out, o, fk[k]); o = leb_write(out, o, (uint32_t)len[k]);
+            memcpy(out + o, pay[k], len[k]); o += len[k];
+        }
+    }
+done:
+    for (int k = 0; k < IS_NS; ++k) { free(s.b[k]); free(pay[k]); }
+    free(all); free(p1); free(pada);
+    return o;
+}
+
+int orc_itree_alloc(orc_itree *t, int w, int h) {
+    memset(t, 0, sizeof *t);
+    t->rec = malloc((size_t)w * h);
+    t->lev = calloc((size_t)w * h, sizeof *t->lev);
+    t->rflag = calloc((size_t)((w + 7) / 8) * ((h + 7) / 8), 1);
+    t->cmode = calloc((size_t)(w / 4) * (h / 4), 1);
+    int ok = t->rec && t->lev && t->rflag && t->cmode;
+    for (int d = 0; d < 4; ++d) {
+        size_t nn = (size_t)inodes_w(w, d) * inodes_w(h, d);
+        t->split[d] = calloc(nn, 1);
+        t->nmode[d] = calloc(nn, 1);
+        ok = ok && t->split[d] && t->nmode[d];
+    }
+    if (!ok) { orc_itree_free(t); return -1; }
+    return 0;
+}
+void orc_itree_free(orc_itree *t) {
+    free(t->rec); free(t->lev); free(t->rflag); free(t->cmode);
+    for (int d = 0; d < 4; ++d) { free(t->split[d]); free(t->nmode[d]); }
+    memset(t, 0, sizeof *t);
+}
+
+size_t orc_image_encode(const uint8_t *src, int w, int h, int stride, int qp, uint8_t *out, size_t cap,
+                        uint8_t *recon) {
+    if (w <= 0 || h <= 0 || (w & 3) || (h & 3)) return 0;
+    orc_itree t;
+    if (orc_itree_alloc(&t, w, h) != 0) return 0;
+    orc_image_analyse(src, w, h, stride, qp, &t);
+    orc_deblock_plane(t.rec, w, h, w, qp);
+    if (recon) memcpy(recon, t.rec, (size_t)w * h);
+    size_t n = orc_image_pack(&t, w, h, qp, out, cap);
+    orc_itree_free(&t);
+    return n;
+}
+
+/* A key frame: the three planes coded separately, each behind a 32-bit
+ * little-endian length (iframe_encode, fdv.h:5487-5539). */
+size_t orc_iframe_encode(const uint8_t *y, const uint8_t *u, const uint8_t *v, int w, int h, int qp,
+                         uint8_t *out, size_t cap, uint8_t *ry, uint8_t *ru, uint8_t *rv) {
+    const uint8_t *src[3] = {y, u, v};
+    uint8_t *rec[3] = {ry, ru, rv};
+    const int pw[3] = {w, w / 2, w / 2}, ph[3] = {h, h / 2, h / 2};
+    size_t o = 0;
+    for (int i = 0; i < 3; ++i) {
+        size_t bcap = (size_t)pw[i] * ph[i] * 2 + 4096;
+        uint8_t *blob = malloc(bcap);
+        if (!blob) return 0;
+        size_t n = orc_image_encode(src[i], pw[i], ph[i], pw[i], qp, blob, bcap, rec[i]);
+        if (!n || o + 4 + n > cap) { free(blob); return 0; }
+        out[o++] = (uint8_t)n; out[o++] = (uint8_t)(n >> 8); out[o++] = (uint8_t)(n >> 16); out[o++] = (uint8_t)(n >> 24);
+        memcpy(out + o, blob, n);
+        o += n;
+        free(blob);
+    }
+    return o;
 }

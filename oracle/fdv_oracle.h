@@ -187,4 +187,30 @@ int orc_pframe_par_exact(const uint8_t *cy, const uint8_t *cu, const uint8_t *cv
                          int ys, int cs, int navail, int qp, double lambda, const orc_metab *mt, int npass,
                          orc_mbinfo *given, orc_pcost *pc, int *converged);
 
+/* ---- the key frame (fdv_image_encode, fdv.h:4845; iframe_encode 5487) */
+/* The decided quadtree of a plane, in position-indexed planes.  Depth d has
+ * nodes of side 64 >> d on a ceil(w/n) x ceil(h/n) grid; split[d] is 0 for a
+ * leaf, 1 for a split, 2 for a split implied by the plane's edge; nmode[d] is
+ * a leaf's DC/V/H/PLANE mode.  cmode holds per 4x4 cell the mode context map:
+ * a 4x4 leaf's mode (0..8), or a large leaf's mode mapped to 4x4 (V 1, H 2,
+ * else DC 0).  rflag holds per 8x8 region of a large leaf 0 / 1 (four 4x4) / 2
+ * (one 8x8).  lev holds each transform block's levels in raster order at the
+ * block's own position; rec the reconstruction before deblocking. */
+typedef struct {
+    uint8_t *rec;
+    int16_t *lev;
+    uint8_t *rflag, *cmode;
+    uint8_t *split[4], *nmode[4];
+} orc_itree;
+int orc_itree_alloc(orc_itree *t, int w, int h);
+void orc_itree_free(orc_itree *t);
+void orc_intra4(int mode, const uint8_t top[8], const uint8_t left[4], uint8_t tl, int ht, int hl,
+                uint8_t pred[16]);
+void orc_image_analyse(const uint8_t *src, int w, int h, int stride, int qp, orc_itree *t);
+size_t orc_image_pack(const orc_itree *t, int w, int h, int qp, uint8_t *out, size_t cap);
+size_t orc_image_encode(const uint8_t *src, int w, int h, int stride, int qp, uint8_t *out, size_t cap,
+                        uint8_t *recon);
+size_t orc_iframe_encode(const uint8_t *y, const uint8_t *u, const uint8_t *v, int w, int h, int qp,
+                         uint8_t *out, size_t cap, uint8_t *ry, uint8_t *ru, uint8_t *rv);
+
 #endif

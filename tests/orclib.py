@@ -67,6 +67,18 @@ def oracle():
             C.c_int, C.c_void_p, C.c_double, C.c_int, C.c_void_p]
         lib.orc_pframe_pack.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
                                         C.c_void_p, C.c_size_t]
+        lib.orc_image_encode.restype = C.c_size_t
+        lib.orc_image_encode.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p,
+                                         C.c_size_t, C.c_void_p]
+        lib.orc_iframe_encode.restype = C.c_size_t
+        lib.orc_iframe_encode.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int,
+                                          C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p, C.c_void_p]
+        lib.orc_intra4.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_uint8, C.c_int, C.c_int, C.c_void_p]
+        lib.orc_itree_alloc.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        lib.orc_itree_free.argtypes = [C.c_void_p]
+        lib.orc_image_analyse.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p]
+        lib.orc_image_pack.restype = C.c_size_t
+        lib.orc_image_pack.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_size_t]
         _oracle = lib
     return _oracle
 
@@ -103,6 +115,12 @@ def ref():
         lib.fdv_vtile_decode.restype = C.c_int
         lib.fdv_vtile_decode.argtypes = [C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                          C.c_int]
+        lib.fdv_image_encode.restype = C.c_size_t
+        lib.fdv_image_encode.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p,
+                                         C.c_size_t, C.c_void_p]
+        lib.fdvref_iframe_encode.restype = C.c_size_t
+        lib.fdvref_iframe_encode.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int,
+                                             C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p, C.c_void_p]
         lib.fdv_video_decode.restype = C.c_int
         lib.fdv_video_decode.argtypes = [C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p,
                                          C.c_void_p, C.c_void_p]
@@ -444,3 +462,47 @@ def ref_iframe_coder():
     (iframe_encode / iframe_decode through oracle/ref_shim.c)."""
     lib = ref()
     return (C.cast(lib.fdvref_iframe_encode, C.c_void_p).value, C.cast(lib.fdvref_iframe_decode, C.c_void_p).value)
+
+
+class ITree(C.Structure):
+    """orc_itree: a plane's decided intra quadtree in position-indexed planes."""
+    _fields_ = [("rec", C.c_void_p), ("lev", C.c_void_p), ("rflag", C.c_void_p), ("cmode", C.c_void_p),
+                ("split", C.c_void_p * 4), ("nmode", C.c_void_p * 4)]
+
+
+def image_encode(lib, fn, src: np.ndarray, qp: int):
+    """fdv_image_encode (or the oracle's) on a tight plane: (bytes, recon)."""
+    h, w = src.shape
+    src = np.ascontiguousarray(src, dtype=np.uint8)
+    cap = w * h * 6 + 8192
+    out = np.zeros(cap, np.uint8)
+    rec = np.zeros((h, w), np.uint8)
+    n = getattr(lib, fn)(src.ctypes.data, w, h, w, qp, out.ctypes.data, cap, rec.ctypes.data)
+    return out[:n].copy(), rec
+
+
+def image_tree(src: np.ndarray, qp: int):
+    """The oracle's decided tree of a plane, as numpy arrays."""
+    lib = oracle()
+    h, w = src.shape
+    src = np.ascontiguousarray(src, dtype=np.uint8)
+    t = ITree()
+    if lib.orc_itree_alloc(C.byref(t), w, h) != 0:
+        raise MemoryError
+    try:
+        lib.orc_image_analyse(src.ctypes.data, w, h, w, qp, C.byref(t))
+
+        def arr(p, shape, dt):
+            n = int(np.prod(shape))
+            return np.ctypeslib.as_array(C.cast(p, C.POINTER(np.ctypeslib.as_ctypes_type(dt))), (n,)).reshape(shape).copy()
+        out = {"rec": arr(t.rec, (h, w), np.uint8), "lev": arr(t.lev, (h, w), np.int16),
+               "rflag": arr(t.rflag, ((h + 7) // 8, (w + 7) // 8), np.uint8),
+               "cmode": arr(t.cmode, (h // 4, w // 4), np.uint8)}
+        for d in range(4):
+            n = 64 >> d
+            shp = ((h + n - 1) // n, (w + n - 1) // n)
+            out["split%d" % d] = arr(t.split[d], shp, np.uint8)
+            out["nmode%d" % d] = arr(t.nmode[d], shp, np.uint8)
+        return out
+    finally:
+        lib.orc_itree_free(C.byref(t))
