@@ -23,8 +23,8 @@ HOST = os.path.join(PKG, "host")
 OBJ = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libfdvgpu.so")
 
-HIP_SOURCES = ["fdvgpu_analyse.hip", "fdvgpu_post.hip", "fdvgpu_metab.hip", "fdvgpu_nn.hip", "fdvgpu_motion.hip", "fdvgpu_noalloc.hip", "fdvgpu_decode.hip", "fdvgpu_api.hip"]
-C_SOURCES = ["fdv_pack.c", "fdv_video.c", "fdv_seq.c", "fdv_unpack.c", "fdv_decode.c", "fdv_stream.c"]
+HIP_SOURCES = ["fdvgpu_analyse.hip", "fdvgpu_post.hip", "fdvgpu_metab.hip", "fdvgpu_nn.hip", "fdvgpu_motion.hip", "fdvgpu_noalloc.hip", "fdvgpu_decode.hip", "fdvgpu_intra.hip", "fdvgpu_api.hip"]
+C_SOURCES = ["fdv_pack.c", "fdv_video.c", "fdv_seq.c", "fdv_unpack.c", "fdv_decode.c", "fdv_stream.c", "fdv_image.c"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",

@@ -163,7 +163,7 @@ static uint8_t *fv(const SeqMem &m, int f) { return m.frames + (size_t)f * m.fsz
 
 // 8x8 DCT basis and quantiser steps, from libm exactly as the reference builds
 // them (c8_init, fdv.h:2597-2612).
-static void tables8(int32_t c8[64], int32_t qs8[52]) {
+extern "C" void fdvg_tables8(int32_t c8[64], int32_t qs8[52]) {
     const double pi = 3.14159265358979323846;
     for (int u = 0; u < 8; ++u) {
         double a = u ? sqrt(2.0 / 8.0) : sqrt(1.0 / 8.0);
@@ -335,7 +335,7 @@ extern "C" fdvgpu *fdvgpu_open(int nseq, const int *w, const int *h, int nsrc) {
     CKN(hipMemcpy(g->sym_d, sym_h, nseq * sizeof(SymSeq), hipMemcpyHostToDevice));
     free(rowtab_h); rowtab_h = NULL;
     free(sym_h); sym_h = NULL;
-    tables8(c8, qs8);
+    fdvg_tables8(c8, qs8);
     if (fdvg_set_tables(c8, qs8) != 0 || fdvg_set_tables_decode(c8, qs8) != 0) {
         seterr("could not load the 8x8 tables"); goto fail;
     }

@@ -122,3 +122,24 @@ typedef struct { uint8_t *p; int w, h, stride, qp; } PlaneRef;
 
 // Where one sequence's MBs live, for the serialiser.
 typedef struct { int mb0, mw, mh, pad; } SymSeq;
+
+// One plane of a key frame for the intra kernels (fdv_image_encode, fdv.h:4845).
+// Coding tree units of 64x64, cols x rows of them; depth d of the quadtree has
+// nodes of side 64 >> d on a tw[d] x th[d] grid, whose split flags (0 leaf,
+// 1 split, 2 split implied by the plane's edge) start at tree + toff[d] and
+// leaf modes at tree + toff[d] + tw[d] * th[d].  cmode: per 4x4 cell the mode
+// context value (a 4x4 leaf's mode, or a large leaf's mode mapped to 4x4).
+// lev: the levels of each surviving transform block in raster order at the
+// block's own position; rflag: per 8x8 region of a large leaf 0 / 1 / 2.
+// edge: per unit 32 tagged words, its bottom row then its right column, four
+// samples a word.
+typedef struct {
+    const uint8_t *src;
+    uint8_t *rec;
+    int16_t *lev;
+    uint8_t *rflag, *cmode, *tree;
+    uint64_t *edge;
+    double lambda;
+    int w, h, qp, cols, rows, pad;
+    int toff[4], tw[4], th[4];
+} fdvg_iplane;

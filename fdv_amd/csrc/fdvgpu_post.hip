@@ -38,7 +38,7 @@ extern "C" __global__ void __launch_bounds__(256)
 k_deblock(const PlaneRef *__restrict__ planes, int pass) {
     const PlaneRef P = planes[blockIdx.y];
     const int al = 6 + P.qp, be = 2 + P.qp / 4, tc = 1 + P.qp / 10;
-    const int ne_v = P.w / 8 - 1;                   // vertical edges per row
+    const int ne_v = (P.w - 1) / 8;                 // vertical edges per row: x = 8, 16, .. < w
     const int ne_h = (P.h - 1) / 8;                 // horizontal edges
     const int n = pass == 0 ? P.h * (ne_v > 0 ? ne_v : 0) : ne_h * P.w;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {

@@ -68,6 +68,9 @@ def _signatures():
         "fdvgpu_decode": (i, [vp, vp, ip, ip]),
         "fdvgpu_motion_search": (i, [vp, vp, vp, i, i, i, i, i, vp, f, vp, vp]),
         "fdvgpu_motion_search_noalloc": (i, [vp, vp, vp, i, i, i, i, i, vp, f, vp, vp]),
+        "fdvgpu_itree_bytes": (sz, [i, i]),
+        "fdvgpu_image_batch": (i, [i, vp, vp]),
+        "fdvgpu_image_pack": (sz, [vp, i, i, i, vp, sz]),
         # include/fdv_amd.h
         "fdvamd_pack_pframe": (sz, [i, vp, vp, vp, vp, sz]),
         "fdvamd_encoder_open": (vp, [i, vp, vp, i]),
@@ -96,6 +99,9 @@ def _signatures():
         "fdvamd_set_sequence_proposer": (i, [vp, i]),
         "fdvamd_video_encode": (sz, [vp, i, i, i, i, i, vp, sz]),
         "fdvamd_vtile_encode": (sz, [vp, i, i, i, i, i, i, i, vp, sz]),
+        "fdvamd_image_encode": (sz, [vp, i, i, i, i, vp, sz, vp]),
+        "fdvamd_image_encode_batch": (i, [i, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
+        "fdvamd_iframe_encode": (sz, [vp, vp, vp, i, i, i, vp, sz, vp, vp, vp]),
     }
 
 
@@ -599,3 +605,87 @@ def load_proposer(path: str = PROPOSER_WEIGHTS) -> np.ndarray:
     if w.size != nn_nparam():
         raise GpuError("proposer weights: expected %d floats, found %d" % (nn_nparam(), w.size))
     return w
+
+
+# ---- key frames (fdvamd_image_encode_batch, fdvgpu_image_batch, fdvgpu_image_pack)
+
+class _ITreeOut(C.Structure):
+    _fields_ = [("rec", C.c_void_p), ("lev", C.c_void_p), ("rflag", C.c_void_p), ("cmode", C.c_void_p),
+                ("tree", C.c_void_p)]
+
+
+class _IPlaneIn(C.Structure):
+    _fields_ = [("src", C.c_void_p), ("w", C.c_int), ("h", C.c_int), ("stride", C.c_int), ("qp", C.c_int)]
+
+
+def _tree_arrays(w, h):
+    L = lib()
+    return {"rec": np.zeros((h, w), np.uint8), "lev": np.zeros((h, w), np.int16),
+            "rflag": np.zeros(((h + 7) // 8, (w + 7) // 8), np.uint8), "cmode": np.zeros((h // 4, w // 4), np.uint8),
+            "tree": np.zeros(int(L.fdvgpu_itree_bytes(w, h)), np.uint8)}
+
+
+def _tree_struct(t):
+    return _ITreeOut(t["rec"].ctypes.data, t["lev"].ctypes.data, t["rflag"].ctypes.data, t["cmode"].ctypes.data,
+                     t["tree"].ctypes.data)
+
+
+def image_trees(planes, qps):
+    """The device decisions of fdvgpu_image_batch for tight planes: per plane a
+    dict of rec (deblocked), lev, rflag, cmode and the tree planes split<d> /
+    nmode<d>."""
+    L = lib()
+    planes = [np.ascontiguousarray(p, dtype=np.uint8) for p in planes]
+    ins = (_IPlaneIn * len(planes))(*[_IPlaneIn(p.ctypes.data, p.shape[1], p.shape[0], p.shape[1], int(q))
+                                      for p, q in zip(planes, qps)])
+    outs = [_tree_arrays(p.shape[1], p.shape[0]) for p in planes]
+    st = (_ITreeOut * len(planes))(*[_tree_struct(t) for t in outs])
+    if L.fdvgpu_image_batch(len(planes), ins, st) != 0:
+        raise GpuError(L.fdvgpu_last_error().decode())
+    for p, t in zip(planes, outs):
+        h, w = p.shape
+        o = 0
+        for d in range(4):
+            n = 64 >> d
+            th, tw = -(-h // n), -(-w // n)
+            t["split%d" % d] = t["tree"][o:o + tw * th].reshape(th, tw)
+            t["nmode%d" % d] = t["tree"][o + tw * th:o + 2 * tw * th].reshape(th, tw)
+            o += 2 * tw * th
+    return outs
+
+
+def image_pack(t, w, h, qp):
+    """fdvgpu_image_pack: a decided plane (dict as image_trees returns, or the
+    same planes from elsewhere) serialised into the image container."""
+    L = lib()
+    tree = np.concatenate([np.concatenate([np.ascontiguousarray(t["split%d" % d]).ravel(),
+                                           np.ascontiguousarray(t["nmode%d" % d]).ravel()]) for d in range(4)])
+    a = {"rec": np.zeros((h, w), np.uint8), "lev": np.ascontiguousarray(t["lev"], np.int16),
+         "rflag": np.ascontiguousarray(t["rflag"], np.uint8), "cmode": np.ascontiguousarray(t["cmode"], np.uint8),
+         "tree": np.ascontiguousarray(tree, np.uint8)}
+    cap = w * h * 6 + 8192
+    out = np.zeros(cap, np.uint8)
+    st = _tree_struct(a)
+    n = L.fdvgpu_image_pack(C.byref(st), w, h, qp, out.ctypes.data, cap)
+    if not n:
+        raise GpuError(L.fdvamd_last_error().decode())
+    return out[:n].copy()
+
+
+def image_encode_batch(planes, qps):
+    """fdvamd_image_encode_batch on tight planes: [(container bytes, recon)]."""
+    L = lib()
+    planes = [np.ascontiguousarray(p, dtype=np.uint8) for p in planes]
+    n = len(planes)
+    caps = [p.size * 6 + 8192 for p in planes]
+    outs = [np.zeros(c, np.uint8) for c in caps]
+    recs = [np.zeros_like(p) for p in planes]
+    P = lambda arrs: (C.c_void_p * n)(*[a.ctypes.data for a in arrs])
+    ws = _ints([p.shape[1] for p in planes])
+    hs = _ints([p.shape[0] for p in planes])
+    lens = (C.c_size_t * n)()
+    rc = L.fdvamd_image_encode_batch(n, P(planes), ws, hs, ws, _ints(qps), P(outs), (C.c_size_t * n)(*caps),
+                                     P(recs), lens)
+    if rc != 0:
+        raise GpuError(L.fdvamd_last_error().decode())
+    return [(o[:lens[i]].copy(), r) for i, (o, r) in enumerate(zip(outs, recs))]

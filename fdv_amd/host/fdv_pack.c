@@ -354,3 +354,87 @@ size_t fdvamd_pack_pframe(int qp, const uint8_t *const sv[16], const size_t sn[1
     packwork_free(w);
     return p;
 }
+
+/* ---- the key frame's container (fdv_image_encode, fdv.h:4926-5030) ----------
+ *
+ * Twelve streams, sized three ways and the smallest kept (ties to the lower
+ * mode): 0 one static table over all streams, 1 a static table per non-empty
+ * stream, 2 one adaptive range-coded blob with a flat model per stream (no
+ * tables; a key frame has no history).  Header: width and height as u16, qp,
+ * the mode, a varint symbol count per stream; the table and blob lengths are
+ * varints. */
+#define NIS 12
+size_t fdvamd_pack_image_(int w, int h, int qp, const uint8_t *const sv[NIS], const size_t sn[NIS], uint8_t *out,
+                          size_t cap) {
+    size_t total = 0, hdr = 0, res = 0;
+    for (int k = 0; k < NIS; ++k) { total += sn[k]; hdr += (size_t)varint_len((uint32_t)sn[k]); }
+    uint32_t call[256] = {0}, (*cnt)[256] = calloc(NIS, sizeof *cnt);
+    uint16_t f1[256], (*fk)[256] = malloc(NIS * sizeof *fk);
+    uint8_t *all = malloc(total ? total : 1), *shared = malloc(total * 2 + 1024), *ada = malloc(total * 3 + 4096);
+    uint8_t *own[NIS] = {0};
+    size_t l_own[NIS] = {0};
+    if (!cnt || !fk || !all || !shared || !ada) goto done;
+    {
+        size_t o = 0;
+        for (int k = 0; k < NIS; ++k) {
+            for (size_t i = 0; i < sn[k]; ++i) cnt[k][sv[k][i]]++;
+            if (sn[k]) memcpy(all + o, sv[k], sn[k]);
+            o += sn[k];
+        }
+    }
+    for (int k = 0; k < NIS; ++k) for (int s = 0; s < 256; ++s) call[s] += cnt[k][s];
+    freqs_from_counts(call, f1);
+    const size_t l_shared = rans_pack(all, total, f1, shared, total * 2 + 1024);
+    const size_t size0 = hdr + table_bytes(f1) + (size_t)varint_len((uint32_t)l_shared) + l_shared;
+    size_t size1 = hdr;
+    for (int k = 0; k < NIS; ++k) {
+        freqs_from_counts(cnt[k], fk[k]);
+        const size_t c = sn[k] * 2 + 1024;
+        if (!(own[k] = malloc(c))) goto done;
+        l_own[k] = rans_pack(sv[k], sn[k], fk[k], own[k], c);
+        if (sn[k]) size1 += table_bytes(fk[k]) + (size_t)varint_len((uint32_t)l_own[k]) + l_own[k];
+    }
+    size_t size2 = (size_t)-1, l_ada;
+    {
+        rcoder rc = {ada, total * 3 + 4096, 0, 0, 0xFFFFFFFFu, 0, 1, 0};
+        static const uint32_t flat[256];
+        for (int k = 0; k < NIS; ++k) {
+            amodel m;
+            am_seed(&m, flat);
+            for (size_t i = 0; i < sn[k]; ++i) rc_code(&rc, &m, sv[k][i]);
+        }
+        l_ada = rc_finish(&rc);
+        if (l_ada) size2 = hdr + (size_t)varint_len((uint32_t)l_ada) + l_ada;
+    }
+    int mode = 0;
+    size_t best = size0;
+    if (size1 < best) { best = size1; mode = 1; }
+    if (size2 < best) { best = size2; mode = 2; }
+    if (5 + best > cap) goto done;
+    size_t o = 0;
+    out[o++] = (uint8_t)w; out[o++] = (uint8_t)(w >> 8);
+    out[o++] = (uint8_t)h; out[o++] = (uint8_t)(h >> 8);
+    out[o++] = (uint8_t)qp;
+    out[o++] = (uint8_t)mode;
+    for (int k = 0; k < NIS; ++k) o = varint_put(out, o, (uint32_t)sn[k]);
+    if (mode == 2) {
+        o = varint_put(out, o, (uint32_t)l_ada);
+        memcpy(out + o, ada, l_ada); o += l_ada;
+    } else if (mode == 0) {
+        o = table_put(out, o, f1);
+        o = varint_put(out, o, (uint32_t)l_shared);
+        memcpy(out + o, shared, l_shared); o += l_shared;
+    } else {
+        for (int k = 0; k < NIS; ++k) {
+            if (!sn[k]) continue;
+            o = table_put(out, o, fk[k]);
+            o = varint_put(out, o, (uint32_t)l_own[k]);
+            memcpy(out + o, own[k], l_own[k]); o += l_own[k];
+        }
+    }
+    res = o;
+done:
+    for (int k = 0; k < NIS; ++k) free(own[k]);
+    free(cnt); free(fk); free(all); free(shared); free(ada);
+    return res;
+}

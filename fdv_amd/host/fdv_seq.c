@@ -5,10 +5,10 @@
  *   fdvamd_vtile_encode  = fdv_vtile_encode  (fdv.h:973-975, body 7161-7243)
  *
  * P-frames go through the device encoder (fdvamd_pframe_batch: every band of a
- * frame in one device pass, entropy coded on host threads).  Key frames come
- * from an I-frame coder the caller registers with fdvamd_set_iframe_coder: the
- * intra quadtree (fdv.h:3637-5154) has no device implementation here, and this
- * library links no reference code.  The container bytes -- the sequence
+ * frame in one device pass, entropy coded on host threads).  Key frames go
+ * through the device key-frame coder (fdvamd_iframe_bands_: every plane of
+ * every band in one device pass), or through an I-frame coder the caller
+ * registers with fdvamd_set_iframe_coder.  The container bytes -- the sequence
  * header, the per-frame lengths, the tiled header and the per-band lengths in
  * band order -- follow the reference exactly, so a reference decoder reads the
  * output unchanged.
@@ -149,8 +149,13 @@ static void *ijob_run(void *arg) {
     return NULL;
 }
 
+extern int fdvamd_iframe_bands_(const uint8_t *const *frames, int w, const int *bh, int nb, int qp, uint8_t **blob,
+                                size_t bcap, size_t *blen, uint8_t **rec);
+
+/* fn NULL: the device key-frame coder, all bands in one pass. */
 int fdvamd_intra_bands_(fdvamd_iframe_encode_fn fn, const uint8_t *const *frames, int w, const int *bh, int nb, int qp,
                         int nthreads, uint8_t **blob, size_t bcap, size_t *blen, uint8_t **rec) {
+    if (!fn) return fdvamd_iframe_bands_(frames, w, bh, nb, qp, blob, bcap, blen, rec);
     int nt = nthreads < 1 ? 1 : nthreads > nb ? nb : nthreads;
     pthread_t tid[64];
     ijob jobs[64];
@@ -178,7 +183,6 @@ static int code_bands(const uint8_t *const *const *src, int nframes, int w, cons
     pthread_mutex_lock(&g_mu);
     ienc = g_ienc;
     pthread_mutex_unlock(&g_mu);
-    if (!ienc) { fdvamd_seterr_("no I-frame coder registered (fdvamd_set_iframe_coder)"); return -1; }
     int *ws = malloc((size_t)nb * sizeof(int)), *nav = malloc((size_t)nb * sizeof(int));
     int *qps = malloc((size_t)nb * sizeof(int)), *nref = calloc((size_t)nb, sizeof(int));
     uint8_t **blob = calloc((size_t)nb, sizeof(uint8_t *)), **rec = calloc((size_t)nb, sizeof(uint8_t *));

@@ -563,7 +563,6 @@ static int code_frame(fdvamd_enc *e, int is_intra, int repeat, int qp, double ls
     }
     if (is_intra) {
         const fdvamd_iframe_encode_fn ienc = fdvamd_iframe_encoder_();
-        if (!ienc) { fdvamd_seterr_("no I-frame coder registered (fdvamd_set_iframe_coder)"); return -1; }
         if (fdvamd_intra_bands_(ienc, src, e->w, bh, nb, qp, e->nthreads, blob, e->bmax, blen, rec) != 0) {
             fdvamd_seterr_("the I-frame coder failed on frame %d", e->n);
             return -1;

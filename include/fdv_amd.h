@@ -122,15 +122,39 @@ int fdvamd_rewind_keep_motion(fdvamd_encoder *e);
 
 /* ---- sequences, with the reference's signatures and containers ---------- */
 
+/* ---- key frames on the device
+ *
+ * fdvamd_image_encode replaces fdv_image_encode (fdv.h:871, body 4845-5038),
+ * same arguments and result: the intra-only plane codec of w x h samples
+ * (multiples of 4) with row stride `stride` at qp; writes the container to
+ * out (cap bytes) and the deblocked reconstruction (w x h, tight) to recon
+ * when not NULL; returns the container's length, or 0.  The rate-distortion
+ * quadtree runs on the device, the serialisation and entropy coding on the
+ * host.  fdvamd_image_encode_batch codes n planes in one device pass (len[i]
+ * the length of plane i; returns 0, or -1 with fdvamd_last_error set).
+ * fdvamd_iframe_encode codes a whole key frame -- the three planes of a tight
+ * I420 frame, each behind its u32 little-endian length, as iframe_encode
+ * (fdv.h:5487-5539) -- with the contract of fdvamd_iframe_encode_fn; it is the
+ * key-frame coder of the sequence and streaming encoders unless another is
+ * registered with fdvamd_set_iframe_coder. */
+size_t fdvamd_image_encode(const uint8_t *src, int w, int h, int stride, int qp, uint8_t *out, size_t cap,
+                           uint8_t *recon);
+int fdvamd_image_encode_batch(int n, const uint8_t *const *src, const int *w, const int *h, const int *stride,
+                              const int *qp, uint8_t *const *out, const size_t *cap, uint8_t *const *recon,
+                              size_t *len);
+size_t fdvamd_iframe_encode(const uint8_t *y, const uint8_t *u, const uint8_t *v, int w, int h, int qp, uint8_t *out,
+                            size_t cap, uint8_t *ry, uint8_t *ru, uint8_t *rv);
+
 /* A key-frame coder: codes the tight planes y (w x h), u, v (w/2 x h/2) at qp
  * into out (at most cap bytes) and leaves its reconstruction in ry, ru, rv;
  * returns the blob length, or 0 on failure -- the contract of the reference's
  * iframe_encode (fdv.h:5487) over a whole frame.  The decoder's counterpart
- * (iframe_decode, fdv.h:5541) returns 0, or -1 on a malformed blob.  The intra
- * quadtree (fdv.h:3637-5154) has no device implementation in this library, so
- * the sequence entry points take key frames from coders registered here
- * (process-wide; both may be NULL to unregister).  No counterpart in the
- * reference, whose sequence encoders call their own. */
+ * (iframe_decode, fdv.h:5541) returns 0, or -1 on a malformed blob.  Key
+ * frames are coded on the device (fdvamd_iframe_encode) unless an encoder is
+ * registered here; key frames are decoded by the registered decoder, which the
+ * sequence and streaming decoders need (process-wide; both may be NULL to
+ * unregister).  No counterpart in the reference, whose sequence encoders call
+ * their own. */
 typedef size_t (*fdvamd_iframe_encode_fn)(const uint8_t *y, const uint8_t *u, const uint8_t *v, int w, int h, int qp,
                                           uint8_t *out, size_t cap, uint8_t *ry, uint8_t *ru, uint8_t *rv);
 typedef int (*fdvamd_iframe_decode_fn)(const uint8_t *in, size_t len, uint8_t *y, uint8_t *u, uint8_t *v, int w,
@@ -149,7 +173,8 @@ int fdvamd_set_sequence_proposer(const float *weights, int n);
 /* fdv_video_encode (fdv.h:903-904, body 6741-6854), same signature and output:
  * the 8-byte sequence header (frame count, width, height, qp, keyint clamped
  * to 0..255), then per frame a u32 length and the frame's blob; frame 0 and
- * every keyint-th frame are key frames (the registered coder), the others
+ * every keyint-th frame are key frames (coded on the device, or by the
+ * registered coder), the others
  * P-frames coded on the device against a two-frame reference pool held there.
  * Returns the stream length, or 0 on error (fdvamd_last_error() says why). */
 size_t fdvamd_video_encode(const uint8_t *const *frames, int nframes, int w, int h, int qp, int keyint,
@@ -216,7 +241,8 @@ int fdvamd_vtile_decode(const uint8_t *in, size_t len, uint8_t *out, int *nframe
 /* ---- file container, streaming decode and encode, bitrate control ---------
  * The reference's public streaming surface with its signatures and semantics;
  * the P-frames of every band of a frame go through the device in one pass, key
- * frames through the registered I-frame coder. */
+ * frames through the device key-frame coder (or the registered one) and the
+ * registered I-frame decoder. */
 
 /* fdv_info (fdv.h:1076-1085), same layout. */
 typedef struct {

@@ -299,6 +299,36 @@ int fdvgpu_nn_adam(float *w, float *grad, float *m1, float *m2, int n, float lr,
  * modes), [1] the MB wavefront, [2] deblock + borders, [3] serialisation. */
 int fdvgpu_timings(fdvgpu *g, float ms[4]);
 
+/* ---- key frames on the device (fdv_image_encode, fdv.h:4845-4890)
+ *
+ * The rate-distortion decisions of the intra plane codec for n planes in one
+ * device pass: the quadtree of 64x64 units, the leaves' modes and levels, the
+ * reconstruction, deblocked.  in[i]: a plane of w x h samples (multiples of 4)
+ * with row stride `stride`, coded at qp.  out[i]: host buffers the call fills:
+ *   rec   w*h           the deblocked reconstruction;
+ *   lev   w*h int16     every surviving transform block's levels in raster
+ *                       order at the block's own position;
+ *   rflag ceil(w/8)*ceil(h/8)   per 8x8 region of a leaf above 4x4: 0 empty,
+ *                       1 four 4x4 transforms, 2 one 8x8;
+ *   cmode (w/4)*(h/4)   per 4x4 cell the mode context value: a 4x4 leaf's
+ *                       mode (0..8), or a larger leaf's mode mapped to 4x4
+ *                       (vertical 1, horizontal 2, else 0);
+ *   tree  fdvgpu_itree_bytes(w, h): for depth d = 0..3 (nodes of side 64 >> d
+ *                       on a ceil(w/n) x ceil(h/n) grid) the split flags (0
+ *                       leaf, 1 split, 2 split implied by the plane's edge),
+ *                       then the leaf modes (DC, V, H, plane).
+ * Returns 0, or -1 with fdvgpu_last_error set.  The process's device context
+ * for key frames is created on the first call; calls are serialised. */
+typedef struct { const uint8_t *src; int w, h, stride, qp; } fdvgpu_iplane_in;
+typedef struct { uint8_t *rec; int16_t *lev; uint8_t *rflag, *cmode, *tree; } fdvgpu_itree_out;
+size_t fdvgpu_itree_bytes(int w, int h);
+int fdvgpu_image_batch(int n, const fdvgpu_iplane_in *in, fdvgpu_itree_out *out);
+/* The host half: a decided plane (the layout above; rec is not read)
+ * serialised into the codec's twelve streams and written as the image
+ * container (fdv.h:4926-5030) into out (cap bytes).  Returns its length, or 0
+ * with fdvamd_last_error set.  No device work. */
+size_t fdvgpu_image_pack(const fdvgpu_itree_out *t, int w, int h, int qp, uint8_t *out, size_t cap);
+
 #ifdef __cplusplus
 }
 #endif
