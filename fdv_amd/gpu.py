@@ -102,6 +102,8 @@ def _signatures():
         "fdvamd_image_encode": (sz, [vp, i, i, i, i, vp, sz, vp]),
         "fdvamd_image_encode_batch": (i, [i, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
         "fdvamd_iframe_encode": (sz, [vp, vp, vp, i, i, i, vp, sz, vp, vp, vp]),
+        "fdvamd_image_decode": (i, [vp, sz, vp, i, ip, ip]),
+        "fdvamd_iframe_decode": (i, [vp, sz, vp, vp, vp, i, i]),
     }
 
 

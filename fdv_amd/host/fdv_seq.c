@@ -36,12 +36,12 @@ int fdvamd_set_iframe_coder(fdvamd_iframe_encode_fn enc, fdvamd_iframe_decode_fn
     return 0;
 }
 
-/* The registered key-frame decoder, for the sequence decoders. */
+/* The registered key-frame decoder, or the built-in one, for the sequence decoders. */
 fdvamd_iframe_decode_fn fdvamd_iframe_decoder_(void) {
     pthread_mutex_lock(&g_mu);
     const fdvamd_iframe_decode_fn f = g_idec;
     pthread_mutex_unlock(&g_mu);
-    return f;
+    return f ? f : fdvamd_iframe_decode;
 }
 
 int fdvamd_set_sequence_mode(int me_mode) {

@@ -447,3 +447,331 @@ int fdvamd_parse_pframe(const uint8_t *syms, const size_t sn[16], int w, int h, 
     free(mvm);
     return c.ok ? 0 : -1;
 }
+
+/* ---- key frames (fdv_image_decode, fdv.h:5040-5152; iframe_decode 5541) ----
+ *
+ * The image container undone into its twelve streams, then each 64x64 unit's
+ * quadtree walked in coding order: a split flag per node (none where the
+ * plane's edge implies the split), a mode per leaf (the 4x4 leaves' under the
+ * context of their left and above cells), the leaf predicted from the
+ * reconstruction so far and its residual added (decode_residual, 5426).  The
+ * plane is deblocked at the end.  Malformed input the reference rejects is
+ * rejected (-1). */
+
+#include <pthread.h>
+
+void fdvg_tables8(int32_t c8[64], int32_t qs8[52]);
+static int32_t g_c8[64], g_qs8[52];
+static pthread_once_t g_c8_once = PTHREAD_ONCE_INIT;
+static void c8_load(void) { fdvg_tables8(g_c8, g_qs8); }
+
+enum { I_SPLIT = 0, I_MODEB = 4, I_MODE0, I_MODE1, I_CNT, I_LVL, I_CNT8, I_LVL8, I_FLAG, I_N };
+
+static const int DQ4[6][3] = {{10, 16, 13}, {11, 18, 14}, {13, 20, 16}, {14, 23, 18}, {16, 25, 20}, {18, 29, 23}};
+static int cls4(int i) { const int r = (i >> 2) & 1, c = i & 1; return r == c ? r : 2; }
+static int clip255(int v) { return v < 0 ? 0 : v > 255 ? 255 : v; }
+
+/* dequantisation and the inverse 4x4 transform (fdv.h:2246-2292, 2546) */
+static void inv4(const int16_t lev[16], int qp, int16_t out[16]) {
+    int c[16], t[16];
+    for (int i = 0; i < 16; ++i) c[i] = (int32_t)(lev[i] * DQ4[qp % 6][cls4(i)]) * (int32_t)(1 << (qp / 6));
+    for (int i = 0; i < 4; ++i) {
+        const int a = c[4 * i], b = c[4 * i + 1], d2 = c[4 * i + 2], d = c[4 * i + 3];
+        const int e0 = a + d2, e1 = a - d2, e2 = (b >> 1) - d, e3 = b + (d >> 1);
+        t[4 * i] = e0 + e3; t[4 * i + 1] = e1 + e2; t[4 * i + 2] = e1 - e2; t[4 * i + 3] = e0 - e3;
+    }
+    for (int j = 0; j < 4; ++j) {
+        const int a = t[j], b = t[4 + j], d2 = t[8 + j], d = t[12 + j];
+        const int e0 = a + d2, e1 = a - d2, e2 = (b >> 1) - d, e3 = b + (d >> 1);
+        out[j] = (int16_t)((e0 + e3 + 32) >> 6); out[4 + j] = (int16_t)((e1 + e2 + 32) >> 6);
+        out[8 + j] = (int16_t)((e1 - e2 + 32) >> 6); out[12 + j] = (int16_t)((e0 - e3 + 32) >> 6);
+    }
+}
+/* dequantisation (saturated) and the inverse 8x8 transform (fdv.h:2639-2685) */
+static void inv8(const int16_t lev[64], int qp, int16_t out[64]) {
+    pthread_once(&g_c8_once, c8_load);
+    int32_t y[64];
+    int64_t b[64];
+    for (int i = 0; i < 64; ++i) {
+        const int64_t v = (int64_t)lev[i] * g_qs8[qp];
+        y[i] = v < INT32_MIN ? INT32_MIN : v > INT32_MAX ? INT32_MAX : (int32_t)v;
+    }
+    for (int r = 0; r < 8; ++r)
+        for (int v = 0; v < 8; ++v) {
+            int64_t s = 0;
+            for (int u = 0; u < 8; ++u) s += (int64_t)g_c8[u * 8 + r] * y[u * 8 + v];
+            b[r * 8 + v] = s;
+        }
+    for (int r = 0; r < 8; ++r)
+        for (int x = 0; x < 8; ++x) {
+            int64_t s = 0;
+            for (int v = 0; v < 8; ++v) s += b[r * 8 + v] * g_c8[v * 8 + x];
+            out[r * 8 + x] = (int16_t)((s + ((int64_t)1 << 35)) >> 36);
+        }
+}
+
+/* Z-order rank of a unit's 4x4 cell and the availability of a sample while
+ * the 4x4 block at (bx, by) is reconstructed (fdv.h:3659, 3677). */
+static int zrank(int lx, int ly) {
+    int z = 0;
+    for (int b = 8; b; b >>= 1) z = (z << 2) | ((ly & b) ? 2 : 0) | ((lx & b) ? 1 : 0);
+    return z;
+}
+static int sample_done(int x, int y, int bx, int by) {
+    const int ux = bx & ~63, uy = by & ~63;
+    if (y < uy || x < ux) return 1;
+    if (x >= ux + 64) return 0;
+    return zrank((x - ux) >> 2, (y - uy) >> 2) < zrank((bx - ux) >> 2, (by - uy) >> 2);
+}
+/* The nine 4x4 predictors of H.264 (8.3.1.2) over the edge E (E[3 - i] left
+ * sample i, E[4] the corner, E[5 + i] top sample i, top-right included). */
+static int pred4_at(const int *E, int m, int x, int y, int dc) {
+    int kind = 2, i = 0;
+    switch (m) {
+    case 1: kind = 0; i = 5 + x; break;
+    case 2: kind = 0; i = 3 - y; break;
+    case 3: if (x == 3 && y == 3) { kind = 3; i = 11; } else i = 5 + x + y; break;
+    case 4: i = 3 + x - y; break;
+    case 5: {
+        const int z = 2 * x - y, k = x - (y >> 1);
+        if (z >= 0) { kind = (z & 1) ? 2 : 1; i = (z & 1) ? 3 + k : 4 + k; }
+        else i = z == -1 ? 3 : 4 - y;
+        break;
+    }
+    case 6: {
+        const int z = 2 * y - x, k = y - (x >> 1);
+        if (z >= 0) { kind = (z & 1) ? 2 : 1; i = 3 - k; }
+        else i = z == -1 ? 3 : 2 + x;
+        break;
+    }
+    case 7: kind = (y & 1) ? 2 : 1; i = 5 + x + (y >> 1); break;
+    case 8: {
+        const int z = x + 2 * y, k = y + (x >> 1);
+        if (z < 5) { kind = (z & 1) ? 2 : 1; i = (z & 1) ? 1 - k : 2 - k; }
+        else if (z == 5) kind = 4;
+        else kind = 0;
+        break;
+    }
+    default: return dc;
+    }
+    const int a = E[i], b = E[i + 1], c = E[i + 2];
+    return kind == 0 ? a : kind == 1 ? (a + b + 1) >> 1 : kind == 2 ? (a + 2 * b + c + 2) >> 2 :
+           kind == 3 ? (a + 3 * b + 2) >> 2 : (3 * a + b + 2) >> 2;
+}
+
+typedef struct {
+    const uint8_t *b;
+    size_t cur[I_N], end[I_N];
+    int ok, w, h, qp, stride;
+    uint8_t *dst, *cmap;
+} idec;
+
+static int i_count(idec *d, int k, int max) { return rd_count(d->b, &d->cur[k], d->end[k], max, &d->ok); }
+static void i_block4(idec *d, const uint8_t *pred, int ps, uint8_t *o, int os) {
+    int16_t lev[16] = {0}, rr[16];
+    const int n = i_count(d, I_CNT, 16);
+    for (int k = 0; k < n; ++k) lev[SCAN4[k]] = (int16_t)rd_level(d->b, &d->cur[I_LVL], d->end[I_LVL], &d->ok);
+    inv4(lev, d->qp, rr);
+    for (int i = 0; i < 16; ++i) o[(i >> 2) * os + (i & 3)] = (uint8_t)clip255(pred[(i >> 2) * ps + (i & 3)] + rr[i]);
+}
+
+static void i_node(idec *d, int bx, int by, int n, int depth) {
+    if (!d->ok) return;
+    const int w = d->w, h = d->h, st = d->stride, hn = n / 2;
+    uint8_t *D = d->dst;
+    if (bx + n > w || by + n > h) {
+        for (int k = 0; k < 4; ++k) {
+            const int cx = bx + (k & 1) * hn, cy = by + (k >> 1) * hn;
+            if (cx < w && cy < h) i_node(d, cx, cy, hn, depth + 1);
+        }
+        return;
+    }
+    if (n == 4) {
+        const int cx = bx / 4, cy = by / 4, cw = w / 4;
+        const int lm = cx ? d->cmap[(size_t)cy * cw + cx - 1] : 0, am = cy ? d->cmap[(size_t)(cy - 1) * cw + cx] : 0;
+        const int mode = i_count(d, lm == am ? I_MODE0 : I_MODE1, 8);
+        if (!d->ok) return;
+        d->cmap[(size_t)cy * cw + cx] = (uint8_t)mode;
+        const int ht = by > 0, hl = bx > 0, htl = ht && hl;
+        const int htr = ht && bx + 4 < w && sample_done(bx + 4, by - 1, bx, by);
+        int E[16] = {0};
+        for (int i = 0; i < 4; ++i) E[3 - i] = hl ? D[(size_t)(by + i) * st + bx - 1] : 0;
+        E[4] = htl ? D[(size_t)(by - 1) * st + bx - 1] : 128;
+        for (int j = 0; j < 8; ++j) E[5 + j] = ht ? D[(size_t)(by - 1) * st + bx + (j < 4 || htr ? j : 3)] : 0;
+        int s = 0;
+        if (ht) for (int j = 0; j < 4; ++j) s += E[5 + j];
+        if (hl) for (int i = 0; i < 4; ++i) s += E[3 - i];
+        const int dc = ht && hl ? (s + 4) >> 3 : ht || hl ? (s + 2) >> 2 : 128;
+        uint8_t pred[16];
+        for (int y = 0; y < 4; ++y)
+            for (int x = 0; x < 4; ++x) pred[y * 4 + x] = (uint8_t)pred4_at(E, mode, x, y, dc);
+        i_block4(d, pred, 4, D + (size_t)by * st + bx, st);
+        return;
+    }
+    const int split = i_count(d, I_SPLIT + depth, 1);
+    if (!d->ok) return;
+    if (split) {
+        for (int k = 0; k < 4; ++k) i_node(d, bx + (k & 1) * hn, by + (k >> 1) * hn, hn, depth + 1);
+        return;
+    }
+    const int mode = i_count(d, I_MODEB, 3);
+    if (!d->ok) return;
+    /* the n x n prediction (gather_nb 5201, fdv_intra_nxn_inner 2879): an
+     * unavailable mode falls back to DC */
+    const int ht = by > 0, hl = bx > 0;
+    uint8_t top[64], left[64], pred[64 * 64];
+    for (int k = 0; k < n; ++k) {
+        top[k] = ht ? D[(size_t)(by - 1) * st + bx + k] : 0;
+        left[k] = hl ? D[(size_t)(by + k) * st + bx - 1] : 0;
+    }
+    const int tl = ht && hl ? D[(size_t)(by - 1) * st + bx - 1] : 128;
+    if (mode == 3 && ht && hl && n <= 32) {
+        int H = 0, V = 0;
+        for (int k = 1; k <= hn; ++k) {
+            const int j = hn - 1 - k;
+            H += k * (top[hn - 1 + k] - (j < 0 ? tl : top[j]));
+            V += k * (left[hn - 1 + k] - (j < 0 ? tl : left[j]));
+        }
+        const int a = 16 * (left[n - 1] + top[n - 1]);
+        const int b = n == 32 ? (11 * H + 512) >> 10 : n == 16 ? (5 * H + 32) >> 6 : (17 * H + 16) >> 5;
+        const int c = n == 32 ? (11 * V + 512) >> 10 : n == 16 ? (5 * V + 32) >> 6 : (17 * V + 16) >> 5;
+        for (int y = 0; y < n; ++y)
+            for (int x = 0; x < n; ++x) pred[y * n + x] = (uint8_t)clip255((a + c * (y - hn + 1) + 16 + b * (x - hn + 1)) >> 5);
+    } else if (mode == 1 && ht) {
+        for (int y = 0; y < n; ++y) memcpy(pred + y * n, top, (size_t)n);
+    } else if (mode == 2 && hl) {
+        for (int y = 0; y < n; ++y) memset(pred + y * n, left[y], (size_t)n);
+    } else {
+        int s = 0, dc = 128;
+        if (ht) for (int k = 0; k < n; ++k) s += top[k];
+        if (hl) for (int k = 0; k < n; ++k) s += left[k];
+        if (ht && hl) dc = (s + n) / (2 * n);
+        else if (ht || hl) dc = (s + n / 2) / n;
+        memset(pred, dc, (size_t)n * n);
+    }
+    for (int ry = 0; ry < n && d->ok; ry += 8)
+        for (int rx = 0; rx < n && d->ok; rx += 8) {
+            const int fl = rd_byte(d->b, &d->cur[I_FLAG], d->end[I_FLAG], &d->ok);
+            const uint8_t *pb = pred + ry * n + rx;
+            uint8_t *o = D + (size_t)(by + ry) * st + bx + rx;
+            if (fl == 0) {
+                for (int y = 0; y < 8; ++y) memcpy(o + (size_t)y * st, pb + y * n, 8);
+            } else if (fl == 1) {
+                for (int q = 0; q < 4; ++q)
+                    i_block4(d, pb + (q >> 1) * 4 * n + (q & 1) * 4, n, o + (size_t)((q >> 1) * 4) * st + (q & 1) * 4, st);
+            } else if (fl == 2) {
+                int16_t lev[64] = {0}, rr[64];
+                const int cnt = i_count(d, I_CNT8, 64);
+                for (int k = 0; k < cnt; ++k)
+                    lev[SCAN8[k]] = (int16_t)rd_level(d->b, &d->cur[I_LVL8], d->end[I_LVL8], &d->ok);
+                inv8(lev, d->qp, rr);
+                for (int i = 0; i < 64; ++i)
+                    o[(size_t)(i >> 3) * st + (i & 7)] = (uint8_t)clip255(pb[(i >> 3) * n + (i & 7)] + rr[i]);
+            } else {
+                d->ok = 0;
+            }
+        }
+    for (int i = 0; i < n / 4; ++i)
+        memset(d->cmap + (size_t)(by / 4 + i) * (w / 4) + bx / 4, mode == 1 ? 1 : mode == 2 ? 2 : 0, (size_t)n / 4);
+}
+
+/* the in-loop filter of a plane, vertical edges then horizontal (fdv.h:3479-3627) */
+static void i_edge(uint8_t *q, ptrdiff_t s, int al, int be, int tc) {
+    const int p1 = q[-2 * s], p0 = q[-s], q0 = q[0], q1 = q[s];
+    if (abs(p0 - q0) < al && abs(p1 - p0) < be && abs(q1 - q0) < be) {
+        int dd = ((q0 - p0) * 4 + (p1 - q1) + 4) >> 3;
+        dd = dd < -tc ? -tc : dd > tc ? tc : dd;
+        q[-s] = (uint8_t)clip255(p0 + dd);
+        q[0] = (uint8_t)clip255(q0 - dd);
+    }
+}
+static void i_deblock(uint8_t *p, int w, int h, int st, int qp) {
+    const int al = 6 + qp, be = 2 + qp / 4, tc = 1 + qp / 10;
+    for (int y = 0; y < h; ++y)
+        for (int x = 8; x < w; x += 8) i_edge(p + (size_t)y * st + x, 1, al, be, tc);
+    for (int y = 8; y < h; y += 8)
+        for (int x = 0; x < w; ++x) i_edge(p + (size_t)y * st + x, st, al, be, tc);
+}
+
+int fdvamd_image_decode(const uint8_t *in, size_t len, uint8_t *dst, int stride, int *w_out, int *h_out) {
+    if (!in || !dst || len < 6) return -1;
+    size_t p = 0;
+    const int w = in[0] | in[1] << 8, h = in[2] | in[3] << 8, qp = in[4], mode = in[5];
+    p = 6;
+    if (w <= 0 || h <= 0 || (w & 3) || (h & 3) || w > 8192 || h > 8192 || qp > 51 || mode > 2) return -1;
+    if (stride < w) return -1;
+    /* what the geometry lets each stream hold (fdv.h:5058-5074) */
+    const size_t units = (size_t)((w + 63) / 64) * ((h + 63) / 64), b4 = (size_t)(w / 4) * (h / 4);
+    const size_t cnt_cap = units * (256 + 64 + 8), lvl_cap = units * (64 * 64 * 5 + 64);
+    size_t cap[I_N] = {units, 4 * units, 16 * units, 64 * units, b4, b4, b4, cnt_cap, lvl_cap, cnt_cap, lvl_cap, cnt_cap};
+    size_t sn[I_N], off[I_N], total = 0;
+    int ok = 1;
+    for (int k = 0; k < I_N; ++k) {
+        const uint32_t v = rd_count32(in, &p, len, &ok);
+        if (!ok || v > cap[k]) return -1;
+        off[k] = total; sn[k] = v; total += v;
+    }
+    uint8_t *syms = malloc(total ? total : 1);
+    rtable *t = malloc(sizeof *t);
+    int rc = -1;
+    idec d;
+    memset(&d, 0, sizeof d);
+    if (!syms || !t) goto out;
+    if (mode == 2) {
+        const uint32_t la = rd_count32(in, &p, len, &ok);
+        if (!ok || p + la > len) goto out;
+        static const uint32_t flat[256];
+        rdecoder rd;
+        rd_init(&rd, in + p, la);
+        for (int k = 0; k < I_N; ++k) {
+            amodel m;
+            am_seed(&m, flat);
+            for (size_t i = 0; i < sn[k]; ++i) syms[off[k] + i] = (uint8_t)rd_sym(&rd, &m);
+        }
+    } else {
+        for (int k = 0; k < (mode == 0 ? 1 : I_N); ++k) {
+            const size_t n = mode == 0 ? total : sn[k];
+            if (mode == 1 && n == 0) continue;
+            uint16_t f[256];
+            p = read_table(in, p, len, f, &ok);
+            if (!ok) goto out;
+            const uint32_t l = rd_count32(in, &p, len, &ok);
+            if (!ok || p + l > len) goto out;
+            rtable_build(f, t);
+            rans_unpack(in + p, l, n, t, syms + (mode == 0 ? 0 : off[k]));
+            p += l;
+        }
+    }
+    d.b = syms; d.ok = 1; d.w = w; d.h = h; d.qp = qp; d.stride = stride; d.dst = dst;
+    for (int k = 0; k < I_N; ++k) { d.cur[k] = off[k]; d.end[k] = off[k] + sn[k]; }
+    if (!(d.cmap = calloc(b4 ? b4 : 1, 1))) goto out;
+    for (int by = 0; by < h && d.ok; by += 64)
+        for (int bx = 0; bx < w && d.ok; bx += 64) i_node(&d, bx, by, 64, 0);
+    if (!d.ok) goto out;
+    i_deblock(dst, w, h, stride, qp);
+    if (w_out) *w_out = w;
+    if (h_out) *h_out = h;
+    rc = 0;
+out:
+    free(syms); free(t); free(d.cmap);
+    return rc;
+}
+
+int fdvamd_iframe_decode(const uint8_t *in, size_t len, uint8_t *y, uint8_t *u, uint8_t *v, int w, int h) {
+    if (!in || !y || !u || !v) return -1;
+    uint8_t *dst[3] = {y, u, v};
+    const int pw[3] = {w, w / 2, w / 2}, ph[3] = {h, h / 2, h / 2};
+    size_t p = 0;
+    for (int i = 0; i < 3; ++i) {
+        if (p + 4 > len) return -1;
+        const uint32_t blob = (uint32_t)in[p] | (uint32_t)in[p + 1] << 8 | (uint32_t)in[p + 2] << 16 |
+                              (uint32_t)in[p + 3] << 24;
+        p += 4;
+        if (p + blob > len || blob < 4) return -1;
+        /* the blob's own dimensions must be the plane's (fdv.h:5556-5561) */
+        if ((in[p] | in[p + 1] << 8) != pw[i] || (in[p + 2] | in[p + 3] << 8) != ph[i]) return -1;
+        if (fdvamd_image_decode(in + p, blob, dst[i], pw[i], NULL, NULL) != 0) return -1;
+        p += blob;
+    }
+    return 0;
+}

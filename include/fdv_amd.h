@@ -144,16 +144,24 @@ int fdvamd_image_encode_batch(int n, const uint8_t *const *src, const int *w, co
                               size_t *len);
 size_t fdvamd_iframe_encode(const uint8_t *y, const uint8_t *u, const uint8_t *v, int w, int h, int qp, uint8_t *out,
                             size_t cap, uint8_t *ry, uint8_t *ru, uint8_t *rv);
+/* fdvamd_image_decode replaces fdv_image_decode (fdv.h:5040-5152), same
+ * arguments and result: one plane's container decoded into dst (row stride
+ * `stride`, at least the plane's width), its size in *w, *h when not NULL;
+ * returns 0, or -1 for input the reference decoder rejects.  On the host.
+ * fdvamd_iframe_decode decodes a whole key frame (iframe_decode, fdv.h:5541)
+ * with the contract of fdvamd_iframe_decode_fn; it is the key-frame decoder of
+ * the sequence and streaming decoders unless another is registered. */
+int fdvamd_image_decode(const uint8_t *in, size_t len, uint8_t *dst, int stride, int *w, int *h);
+int fdvamd_iframe_decode(const uint8_t *in, size_t len, uint8_t *y, uint8_t *u, uint8_t *v, int w, int h);
 
 /* A key-frame coder: codes the tight planes y (w x h), u, v (w/2 x h/2) at qp
  * into out (at most cap bytes) and leaves its reconstruction in ry, ru, rv;
  * returns the blob length, or 0 on failure -- the contract of the reference's
  * iframe_encode (fdv.h:5487) over a whole frame.  The decoder's counterpart
  * (iframe_decode, fdv.h:5541) returns 0, or -1 on a malformed blob.  Key
- * frames are coded on the device (fdvamd_iframe_encode) unless an encoder is
- * registered here; key frames are decoded by the registered decoder, which the
- * sequence and streaming decoders need (process-wide; both may be NULL to
- * unregister).  No counterpart in the reference, whose sequence encoders call
+ * frames are coded on the device (fdvamd_iframe_encode) and decoded on the host
+ * (fdvamd_iframe_decode) unless coders are registered here (process-wide;
+ * either may be NULL to fall back to the built-in one).  No counterpart in the reference, whose sequence encoders call
  * their own. */
 typedef size_t (*fdvamd_iframe_encode_fn)(const uint8_t *y, const uint8_t *u, const uint8_t *v, int w, int h, int qp,
                                           uint8_t *out, size_t cap, uint8_t *ry, uint8_t *ru, uint8_t *rv);

@@ -173,3 +173,26 @@ def test_consecutive_batches_of_different_layouts():
         for (a, ra), (b, rb) in zip(got, want):
             assert a.tobytes() == b.tobytes(), rep
             assert (ra == rb).all(), rep
+
+
+def test_streams_round_trip_with_no_coder_registered():
+    """Nothing of the reference in the loop: key frames coded on the device and
+    decoded on the host by the built-in coders, P-frames both ways on the
+    device; the streams decode to what the reference decoder makes of them."""
+    G.set_iframe_coder(0, 0)
+    G.set_sequence_mode(G.ME_DIAMOND)
+    sc = content.scene_pan(640, 368, 5)
+    frames = [sc.frame(i) for i in range(6)]
+    say("  vtile 640x368, 6 frames, keyint 3, QP 26, 4 bands, built-in key-frame coders")
+    tiled = G.vtile_encode(frames, 640, 368, 26, 3, 6, 4)
+    ours = G.video_decode(tiled, tiled=True)
+    theirs = L.ref_video_decode(tiled, tiled=True)
+    assert theirs is not None
+    assert (ours[0] == theirs[0]).all()
+    say("  video 320x192, 5 frames, keyint 2, QP 33")
+    sc2 = content.scene_objects(320, 192, 6)
+    single = G.video_encode([sc2.frame(i) for i in range(5)], 320, 192, 33, 2)
+    ours = G.video_decode(single)
+    theirs = L.ref_video_decode(single)
+    assert theirs is not None
+    assert (ours[0] == theirs[0]).all()
