@@ -3,6 +3,7 @@
  * nothing but one include.
  *
  * Include it after fdv.h in a source file whose calls to
+ *   fdv_image_encode / fdv_image_decode  (fdv.h:871-876)
  *   fdv_video_encode / fdv_video_decode  (fdv.h:903-908)
  *   fdv_vtile_encode / fdv_vtile_decode  (fdv.h:973-981)
  *   fdv_wrap / fdv_read / fdv_decode     (fdv.h:1090-1105)
@@ -12,11 +13,12 @@
  * should go to this library: the calls after the include are renamed to the
  * fdvamd_ functions of the same signatures (include/fdv_amd.h), which write and
  * read the same containers; fdvamd_info, fdvamd_rc and the encoder / decoder
- * handles stand for fdv_info, fdv_rc, fdv_encoder and fdv_decoder.  Key frames stay with the reference's intra coder:
- * in the one file that defines FDV_IMPLEMENTATION (where fdv.h's file-static
- * iframe_encode / iframe_decode, fdv.h:5487 / 5541, are visible), call
- * fdvamd_dropin_register() once before the first encode or decode.  Link with
- * fdv_amd/libfdvgpu.so.
+ * handles stand for fdv_info, fdv_rc, fdv_encoder and fdv_decoder.  Key frames
+ * are coded by this library too (on the device) and decoded by it (on the
+ * host).  A file that defines FDV_IMPLEMENTATION (where fdv.h's file-static
+ * iframe_encode / iframe_decode, fdv.h:5487 / 5541, are visible) may call
+ * fdvamd_dropin_register() to route key frames to the reference's own intra
+ * coder instead.  Link with fdv_amd/libfdvgpu.so.
  */
 #ifndef FDV_AMD_DROPIN_H
 #define FDV_AMD_DROPIN_H
@@ -35,6 +37,8 @@ static inline int fdvamd_dropin_register(void) {
 }
 #endif
 
+#define fdv_image_encode fdvamd_image_encode
+#define fdv_image_decode fdvamd_image_decode
 #define fdv_video_encode fdvamd_video_encode
 #define fdv_video_decode fdvamd_video_decode
 #define fdv_vtile_encode fdvamd_vtile_encode

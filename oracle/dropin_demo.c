@@ -6,11 +6,12 @@
  * TEST INFRASTRUCTURE ONLY.  Built by oracle/Makefile from the reference's
  * header where it lies (REF_DIR) into oracle/_ref/; run by
  * tests/test_gpu_dropin.py.  Usage: dropin_demo W H NFRAMES QP KEYINT BAND_MBROWS
- * Prints "dropin ok ..." and exits 0 when the drop-in's streams are byte-
- * identical to the reference's -- whole-sequence coders, and the streaming
- * encoder at a fixed QP and under bitrate control, single-stream and tiled,
- * with held frames -- and both decoders return the same frames, the streaming
- * ones also after seeks.
+ * Prints "dropin ok ..." and exits 0 when the drop-in's output is byte-
+ * identical to the reference's -- the plane codec, the whole-sequence coders,
+ * and the streaming encoder at a fixed QP and under bitrate control,
+ * single-stream and tiled, with held frames -- and both decoders return the
+ * same frames, the streaming ones also after seeks.  Key frames go through the
+ * drop-in's own coders (nothing is registered).
  */
 #define FDV_IMPLEMENTATION
 #include "fdv.h"
@@ -20,6 +21,9 @@
 #include <string.h>
 
 /* the reference's coders, taken before the include renames the calls below */
+static size_t (*const ref_image_encode)(const uint8_t *, int, int, int, int, uint8_t *, size_t, uint8_t *) =
+    fdv_image_encode;
+static int (*const ref_image_decode)(const uint8_t *, size_t, uint8_t *, int, int *, int *) = fdv_image_decode;
 static size_t (*const ref_video_encode)(const uint8_t *const *, int, int, int, int, int, uint8_t *, size_t) =
     fdv_video_encode;
 static int (*const ref_video_decode)(const uint8_t *, size_t, uint8_t *, int *, int *, int *) = fdv_video_decode;
@@ -125,8 +129,23 @@ int main(int argc, char **argv) {
     if (!frames || !a || !b || !da || !db || !fp) return 3;
     make_frames(frames, w, h, n);
     for (int f = 0; f < n; ++f) fp[f] = frames + (size_t)f * fs;
-    if (fdvamd_dropin_register() != 0) return 4;
     int fails = 0;
+    {
+        /* an unchanged call site of the plane codec: frame 0's luma */
+        uint8_t *ra = malloc((size_t)w * h), *rb = malloc((size_t)w * h), *xa = malloc((size_t)w * h);
+        uint8_t *xb = malloc((size_t)w * h);
+        if (!ra || !rb || !xa || !xb) return 3;
+        const size_t na = fdv_image_encode(frames, w, h, w, qp, a, cap, ra);
+        const size_t nb = ref_image_encode(frames, w, h, w, qp, b, cap, rb);
+        int w1, h1, w2, h2;
+        const int da_ = fdv_image_decode(a, na, xa, w, &w1, &h1), db_ = ref_image_decode(b, nb, xb, w, &w2, &h2);
+        const int same = na && na == nb && !memcmp(a, b, na) && !memcmp(ra, rb, (size_t)w * h);
+        const int dsame = da_ == 0 && db_ == 0 && !memcmp(xa, xb, (size_t)w * h) && !memcmp(xa, ra, (size_t)w * h);
+        printf("fdv_image_encode: %zu B vs the reference's %zu B, planes %s, decoded %s\n", na, nb,
+               same ? "identical" : "DIFFER", dsame ? "identical" : "DIFFER");
+        fails += !same + !dsame;
+        free(ra); free(rb); free(xa); free(xb);
+    }
     for (int tiled = 0; tiled < 2; ++tiled) {
         /* an unchanged call site: fdv_video_encode / fdv_vtile_encode */
         const size_t na = tiled ? fdv_vtile_encode(fp, n, w, h, qp, keyint, bmr, 4, a, cap)

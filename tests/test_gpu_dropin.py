@@ -1,9 +1,11 @@
-"""A caller of the reference's sequence API moved to the device path by one
-include: oracle/dropin_demo.c includes the reference's fdv.h, then
-include/fdv_amd_dropin.h, and calls fdv_video_encode / fdv_video_decode /
-fdv_vtile_encode / fdv_vtile_decode unchanged (fdv.h:903-908, 973-981).  The
-demonstration checks, in the same process, that the streams are byte-identical
-to the reference's own coders' and that both decoders return the same frames.
+"""A caller of the reference's API moved to the device path by one include:
+oracle/dropin_demo.c includes the reference's fdv.h, then
+include/fdv_amd_dropin.h, and calls fdv_image_encode / fdv_image_decode,
+fdv_video_encode / fdv_video_decode, fdv_vtile_encode / fdv_vtile_decode and
+the streaming fdv_enc_* / fdv_dec_* unchanged (fdv.h:871-876, 903-908,
+973-981, 1126-1198), key frames through the library's own coders.  The
+demonstration checks, in the same process, that the output is byte-identical to
+the reference's own functions' and that both decoders return the same frames.
 It is built by oracle/Makefile into oracle/_ref/ where the reference tree is
 present.
 """
