@@ -109,6 +109,74 @@ def cpu_baseline(frames, w, layout, qp, seconds=12.0):
                         "included; the GPU value excludes entropy coding (end_to_end includes it)"}
 
 
+def bench_keyframe(args, content, G):
+    """Key frames (the device intra quadtree, fdvamd_image_encode_batch): frame
+    0 of the headline 1280x720 scene as 8 bands of 6 MB rows (all 24 planes in
+    one device pass) and as one whole frame, QP 24, timed from host buffers to
+    containers (uploads, device decisions, host serialisation and entropy
+    coding); beside it the reference's key-frame coder (iframe_encode) on the
+    same planes, one thread per band."""
+    import threading
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    W, H, QP = 1280, 720, 24
+    f = content.scene_pan(W, H, seed=2).frame(0)
+    Y = f[:W * H].reshape(H, W)
+    U = f[W * H:W * H * 5 // 4].reshape(H // 2, W // 2)
+    V = f[W * H * 5 // 4:].reshape(H // 2, W // 2)
+    bands_ = [(b * 96, 96) for b in range(8)]
+    tiled = []
+    for y0, bh in bands_:
+        tiled += [np.ascontiguousarray(Y[y0:y0 + bh]), np.ascontiguousarray(U[y0 // 2:(y0 + bh) // 2]),
+                  np.ascontiguousarray(V[y0 // 2:(y0 + bh) // 2])]
+    whole = [np.ascontiguousarray(Y), np.ascontiguousarray(U), np.ascontiguousarray(V)]
+    res = {}
+    for name, planes in (("tiled", tiled), ("whole", whole)):
+        for _ in range(max(1, args.warmup)):
+            G.image_encode_batch(planes, [QP] * len(planes))
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            out = G.image_encode_batch(planes, [QP] * len(planes))
+        res[name] = (time.perf_counter() - t0) / args.steps
+        res[name + "_bytes"] = sum(len(o) for o, _ in out)
+    cpu = None
+    if not args.no_cpu:
+        try:
+            import orclib as L
+            lib, fn, kind = (L.ref(), "fdvref_iframe_encode", "reference") if L.ref() is not None else \
+                (L.oracle(), "orc_iframe_encode", "port")
+
+            def band(b):
+                y, u, v = tiled[3 * b], tiled[3 * b + 1], tiled[3 * b + 2]
+                cap = y.size * 4 + 65536
+                o = np.zeros(cap, np.uint8)
+                r = [np.zeros_like(y), np.zeros_like(u), np.zeros_like(v)]
+                getattr(lib, fn)(y.ctypes.data, u.ctypes.data, v.ctypes.data, W, y.shape[0], QP, o.ctypes.data, cap,
+                                 r[0].ctypes.data, r[1].ctypes.data, r[2].ctypes.data)
+            reps = 3
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                th = [threading.Thread(target=band, args=(b,)) for b in range(8)]
+                [t.start() for t in th]
+                [t.join() for t in th]
+            dt = (time.perf_counter() - t0) / reps
+            cpu = {"value": round(W * H / dt / 1e6, 3), "unit": "Mpix/s", "ms_per_key_frame": round(dt * 1e3, 2),
+                   "cores": 8, "kind": kind, "nproc": os.cpu_count(), "cpu_model": _cpu_model(),
+                   "sample": "%d key frames of the same 8 bands, one thread per band" % reps}
+        except Exception as e:  # pragma: no cover
+            cpu = {"value": None, "sample": "unavailable: %s" % e}
+    print(json.dumps({
+        "metric": "key-frame encode Mpix/s at 720p (8 bands)", "value": round(W * H / res["tiled"] / 1e6, 3),
+        "unit": "Mpix/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(res["tiled"] * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u8", "data": "synthetic (seeded pan + objects + grain), frame 0",
+        "config": {"workload": "1280x720 key frame, QP 24, 8 bands of 6 MB rows: 24 planes in one device pass",
+                   "container_bytes": res["tiled_bytes"]},
+        "whole_frame": {"ms_per_key_frame": round(res["whole"] * 1e3, 3),
+                        "value": round(W * H / res["whole"] / 1e6, 3), "unit": "Mpix/s",
+                        "container_bytes": res["whole_bytes"]},
+        "cpu_baseline": cpu}))
+
+
 def _cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -246,6 +314,9 @@ def main():
                     help="steps of the exhaustive search timed for comparison (0: skip)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--keyframe", action="store_true",
+                    help="only time key frames (the device intra quadtree) at 720p, tiled and whole, beside the "
+                         "reference's key-frame coder on the host")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (reads MIB, writes MIB): "
                          "the calibration for HBM byte counters")
@@ -267,6 +338,10 @@ def main():
 
     from fdv_amd import content
     from fdv_amd import gpu as G
+
+    if args.keyframe:
+        bench_keyframe(args, content, G)
+        return
 
     nsteps = args.warmup + args.steps
     nframes = min(nsteps + 1, 60)
