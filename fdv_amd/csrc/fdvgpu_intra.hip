@@ -39,8 +39,29 @@
 __host__ __device__ constexpr int trec_off(int d, int m) {
     return d == 0 ? m * 4096 : d == 1 ? 12288 + m * 1024 : d == 2 ? 16384 + m * 256 : 17408 + m * 64;
 }
+// Every sample of a 4x4 prediction is an entry of one array built from the
+// edge E: A[i] = E[i] (copies), A[16 + i] = (E[i] + E[i+1] + 1) >> 1,
+// A[32 + i] = (E[i] + 2 E[i+1] + E[i+2] + 2) >> 2, A[48] and A[49] the two
+// corner taps of diagonal-down-left and horizontal-up, A[50] DC.  k_P4IDX
+// holds, per (mode, row), the four entries' indices (the mapping p4 states).
+static __device__ __constant__ uint32_t k_P4IDX[36] = {
+    0x32323232, 0x32323232, 0x32323232, 0x32323232, 0x08070605, 0x08070605, 0x08070605, 0x08070605, 0x03030303,
+    0x02020202, 0x01010101, 0x00000000, 0x28272625, 0x29282726, 0x2a292827, 0x302a2928, 0x26252423, 0x25242322,
+    0x24232221, 0x23222120, 0x17161514, 0x26252423, 0x16151422, 0x25242321, 0x25242313, 0x23132212, 0x22122111,
+    0x21112010, 0x18171615, 0x28272625, 0x19181716, 0x29282726, 0x20112112, 0x31102011, 0x00003110, 0x00000000};
+
 // unit-local node index: depth-major, raster within a depth
 __device__ __forceinline__ constexpr int nbase(int d) { return d == 0 ? 0 : d == 1 ? 1 : d == 2 ? 5 : 21; }
+
+// One 4x4 candidate at each RDOQ truncation point t - 1 (t = 0 .. 16): the
+// RDOQ's score, the cost of the point's own coding (clipped reconstruction
+// SSD + lambda bits), its reconstruction; whether the RDOQ may stop there.
+struct L4Res {
+    double rdj[17], jf[17];
+    uint32_t rw[17][4];
+    uint8_t valid[17];
+    int last;
+};
 
 struct __align__(16) IntraLds {
     uint8_t src[IU * IU];
@@ -56,8 +77,9 @@ struct __align__(16) IntraLds {
     int16_t hlev[NWI][64];
     uint8_t hflag[NWI];
     uint8_t pred4[9][16];             // the walker's nine 4x4 predictions
-    int E[16];                        // the walker's 4x4 edge
-    double rdj[17];                   // RDOQ scores of the walker's final recode
+    int A[64];                        // the walker's 4x4 edge and its filtered taps
+    uint32_t p4idx[36];               // k_P4IDX
+    L4Res l4[9];                      // the walker's candidates at every truncation point
     uint8_t split[85], nmode[85];
     uint8_t cm[256];                  // the unit's 4x4 mode-context map
     int quit, plane, unit;
@@ -295,6 +317,13 @@ __device__ __forceinline__ int pn(const int prm[8], const uint8_t *top, const ui
     if (m == 3) return clip8((prm[5] + prm[7] * (py - c0) + 16 + prm[6] * (px - c0)) >> 5);
     return prm[4];
 }
+// wave-uniform minimum of unsigned values
+__device__ __forceinline__ unsigned wave_umin(unsigned v) {
+    v = min(v, (unsigned)dpp_mov<DPP_XOR1>((int)v)); v = min(v, (unsigned)dpp_mov<DPP_XOR2>((int)v));
+    v = min(v, (unsigned)dpp_mov<DPP_ROR4>((int)v)); v = min(v, (unsigned)dpp_mov<DPP_ROR8>((int)v));
+    return min(min((unsigned)__builtin_amdgcn_readlane((int)v, 0), (unsigned)__builtin_amdgcn_readlane((int)v, 16)),
+               min((unsigned)__builtin_amdgcn_readlane((int)v, 32), (unsigned)__builtin_amdgcn_readlane((int)v, 48)));
+}
 // a large leaf's mode in the 4x4 mode context (nn_to_4x4_mode, fdv.h:4400)
 __device__ __forceinline__ int nn_ctx(int m) { return m == 1 ? 1 : m == 2 ? 2 : 0; }
 
@@ -314,7 +343,17 @@ __device__ __forceinline__ double leaf4(IntraLds &S, const UnitCtx &U, int ux, i
     const int L = threadIdx.x & 63;
     const int bx = U.x0 + ux, by = U.y0 + uy;
     int ht, hl, htr, htl;
-    const int dcv = edge4(S.cv, S.E, ux, uy, bx, by, U.w, ht, hl, htr, htl, L);
+    const int dcv = edge4(S.cv, S.A, ux, uy, bx, by, U.w, ht, hl, htr, htl, L);
+    wave_sync();
+    {
+        int a = 0;
+        if (L >= 16 && L < 28) a = (S.A[L - 16] + S.A[L - 15] + 1) >> 1;
+        else if (L >= 32 && L < 43) a = (S.A[L - 32] + 2 * S.A[L - 31] + S.A[L - 30] + 2) >> 2;
+        else if (L == 48) a = (S.A[11] + 3 * S.A[12] + 2) >> 2;
+        else if (L == 49) a = (3 * S.A[0] + S.A[1] + 2) >> 2;
+        else if (L == 50) a = dcv;
+        if (L >= 16) S.A[L] = a;
+    }
     wave_sync();
     // nine predictions and their SATD: lane 4m + y takes row y of mode m
     const int m = L >> 2, y = L & 3;
@@ -322,11 +361,12 @@ __device__ __forceinline__ double leaf4(IntraLds &S, const UnitCtx &U, int ux, i
     int h[4];
     {
         const uint32_t sw = *(const uint32_t *)&S.src[(uy + y) * IU + ux];
+        const uint32_t iw = L < 36 ? S.p4idx[L] : 0u;
         int d[4];
         uint32_t pw = 0;
 #pragma unroll
         for (int x = 0; x < 4; ++x) {
-            const int p = act ? p4(S.E, m, x, y, dcv) : 0;
+            const int p = act ? S.A[(iw >> (8 * x)) & 0xff] : 0;
             pw |= (uint32_t)p << (8 * x);
             d[x] = (int)((sw >> (8 * x)) & 0xff) - p;
         }
@@ -346,22 +386,49 @@ __device__ __forceinline__ double leaf4(IntraLds &S, const UnitCtx &U, int ux, i
     sacc += dpp_mov<DPP_XOR1>(sacc);
     sacc += dpp_mov<DPP_XOR2>(sacc);
     wave_sync();                                      // the predictions, for the candidate lanes
-    // candidate list in mode order, then the reference's ordering of the first ntry
-    int ks[9], km[9], nc = 0;
+    // candidate list in mode order (position = rank among the allowed modes),
+    // then the reference's ordering of the first ntry: selection with
+    // exchanges (fdv.h:3956-3969)
+    int ks[9], km[9];
 #pragma unroll
     for (int mm = 0; mm < 9; ++mm) { ks[mm] = 0x7fffffff; km[mm] = 0; }
+    const bool rep_l = (L & 3) == 0 && act;
+    const uint64_t okb = __ballot(rep_l);
+    int okmask = 0;
 #pragma unroll
-    for (int mm = 0; mm < 9; ++mm) {
-        const int sv = __builtin_amdgcn_readlane(sacc, 4 * mm);
-        if (m4ok(mm, ht, htr, hl, htl)) {
-#pragma unroll
-            for (int s = 0; s < 9; ++s) if (s == nc) { ks[s] = sv; km[s] = mm; }
-            ++nc;
-        }
-    }
+    for (int mm = 0; mm < 9; ++mm) okmask |= (int)((okb >> (4 * mm)) & 1) << mm;
+    const int nc = __builtin_popcount(okmask);
     const int ntry = U.qp < 12 ? nc : (nc < 2 ? nc : 2);
+    if (ntry <= 2) {
+        // two rounds as minima of (SATD, position) keys: the first pick is the
+        // lowest; the exchange puts the first position's entry where the first
+        // pick was, and the second pick is the lowest of the rest by the new
+        // positions
+        const int slot = __builtin_popcount(okmask & ((1 << m) - 1));
+        const unsigned k0 = wave_umin(rep_l ? (unsigned)sacc << 4 | (unsigned)slot : 0xffffffffu);
+        const int i0 = (int)(k0 & 15);
+        const uint64_t b0 = __ballot(rep_l && slot == i0), bz = __ballot(rep_l && slot == 0);
+        ks[0] = (int)(k0 >> 4); km[0] = (int)(__builtin_ctzll(b0) >> 2);
+        if (ntry == 2) {
+            const unsigned k1 = wave_umin(rep_l && slot != i0 ? (unsigned)sacc << 4 | (unsigned)(slot == 0 ? i0 : slot)
+                                                              : 0xffffffffu);
+            const int p1 = (int)(k1 & 15);
+            const uint64_t b1 = p1 == i0 ? bz : __ballot(rep_l && slot == p1);
+            ks[1] = (int)(k1 >> 4); km[1] = (int)(__builtin_ctzll(b1) >> 2);
+        }
+    } else {
+        int n = 0;
 #pragma unroll
-    for (int a = 0; a < 8; ++a) {
+        for (int mm = 0; mm < 9; ++mm) {
+            const int sv = __builtin_amdgcn_readlane(sacc, 4 * mm);
+            if ((okmask >> mm) & 1) {
+#pragma unroll
+                for (int s2 = 0; s2 < 9; ++s2) if (s2 == n) { ks[s2] = sv; km[s2] = mm; }
+                ++n;
+            }
+        }
+#pragma unroll
+        for (int a = 0; a < 8; ++a) {
         if (a < ntry) {
             int bst = a, bv = ks[a];
 #pragma unroll
@@ -370,6 +437,7 @@ __device__ __forceinline__ double leaf4(IntraLds &S, const UnitCtx &U, int ux, i
             for (int b = a + 1; b < 9; ++b)
                 if (b == bst) { const int t = ks[a]; ks[a] = ks[b]; ks[b] = t; const int u = km[a]; km[a] = km[b]; km[b] = u; }
         }
+        }
     }
     int ne = ntry;
     if (U.qp >= 12) {
@@ -377,54 +445,107 @@ __device__ __forceinline__ double leaf4(IntraLds &S, const UnitCtx &U, int ux, i
         for (int c = 8; c >= 1; --c)
             if (c < ntry && (long long)ks[c] * 4 > (long long)ks[0] * 5) ne = c;
     }
-    // the ne candidates without RDOQ, lane c taking candidate c
+    // One pass per three candidates: lane 21 g + t takes candidate 3 pass + g
+    // at RDOQ truncation point t - 1 (t = 0 .. 16), every lane holding the
+    // candidate's block.  Each lane scores its truncation point as the RDOQ
+    // does (unclipped residual SSD, the RDOQ's rate) and also codes it: the
+    // clipped reconstruction, its SSD and its bits.  The point with no
+    // truncation is the candidate's own coding without RDOQ, so the same pass
+    // gives the candidates' comparison (the first pass of intra_leaf4) and,
+    // for the winner, the RDOQ's choice and its result.
     const uint32_t *srow = (const uint32_t *)&S.src[uy * IU + ux];
     const uint32_t sw[4] = {srow[0], srow[IU / 4], srow[2 * IU / 4], srow[3 * IU / 4]};
-    double Jc = 0.0;
     {
-        int mc = km[0];
+        const int qp = U.qp;
+        const int m6 = qp % 6, sh = 1 << (qp / 6), qb = 15 + qp / 6, off = (1 << qb) / 3;
+        const int mf0 = k_MF[m6][0], mf1 = k_MF[m6][1], mf2 = k_MF[m6][2];
+        const int dv0 = k_DV[m6][0], dv1 = k_DV[m6][1], dv2 = k_DV[m6][2];
+        const int grp = L / 21, t = L - 21 * grp, Lc = t - 1;
+        for (int pass = 0; pass * 3 < ne; ++pass) {
+            const int c = pass * 3 + grp;
+            int mc = km[0];
 #pragma unroll
-        for (int c = 1; c < 9; ++c) if (L == c) mc = km[c];
-        if (L < ne) {
-            const uint32_t *pp = (const uint32_t *)S.pred4[mc];
-            const int qp = U.qp;
-            const int m6 = qp % 6, sh = 1 << (qp / 6), qb = 15 + qp / 6, off = (1 << qb) / 3;
-            int res[16], lv[16], pr[16];
+            for (int cc = 1; cc < 9; ++cc) if (c == cc) mc = km[cc];
+            if (grp < 3 && t <= 16 && c < ne) {
+                const uint32_t *pp = (const uint32_t *)S.pred4[mc];
+                int cu[16], pr[16], res[16], lv[16];
 #pragma unroll
-            for (int i = 0; i < 16; ++i) {
-                pr[i] = (pp[i >> 2] >> (8 * (i & 3))) & 0xff;
-                res[i] = (int)((sw[i >> 2] >> (8 * (i & 3))) & 0xff) - pr[i];
+                for (int i = 0; i < 16; ++i) {
+                    cu[i] = (sw[i >> 2] >> (8 * (i & 3))) & 0xff;
+                    pr[i] = (pp[i >> 2] >> (8 * (i & 3))) & 0xff;
+                    res[i] = cu[i] - pr[i];
+                }
+                {
+                    int co[16];
+                    fdct4_reg(res, co);
+#pragma unroll
+                    for (int i = 0; i < 16; ++i) {
+                        const int mf = cls4(i) == 0 ? mf0 : cls4(i) == 1 ? mf1 : mf2;
+                        const int a = co[i] < 0 ? -co[i] : co[i];
+                        const int v = (a * mf + off) >> qb;
+                        lv[i] = co[i] < 0 ? -v : v;
+                    }
+                }
+                int last = -1, nzm = 0, drop[16];
+#pragma unroll
+                for (int k = 0; k < 16; ++k) {
+                    const uint32_t z = zmap(lv[k_SCAN4c(k)]);
+                    drop[k] = 8 * leblen(z);
+                    if (z) { last = k; nzm |= 1 << k; }
+                }
+                L4Res &R = S.l4[c];
+                if (t == 0) R.last = last;
+                const bool valid = Lc <= last && (Lc < 0 || ((nzm >> Lc) & 1));
+                R.valid[t] = (uint8_t)valid;
+                if (Lc <= last) {
+                    int bits0 = egbits((uint32_t)(last + 1));
+#pragma unroll
+                    for (int k = 0; k < 16; ++k) if (k <= last) bits0 += egbits(zmap(lv[k_SCAN4c(k)]));
+                    int rb = bits0, lt[16], dq[16], rr[16];
+#pragma unroll
+                    for (int k = 0; k < 16; ++k) if (k > Lc && k <= last) rb -= drop[k];
+#pragma unroll
+                    for (int i = 0; i < 16; ++i) {
+                        lt[i] = k_INV4c(i) <= Lc ? lv[i] : 0;
+                        const int dv = cls4(i) == 0 ? dv0 : cls4(i) == 1 ? dv1 : dv2;
+                        dq[i] = (lt[i] * dv) * sh;
+                    }
+                    idct4_reg(dq, rr);
+                    int D = 0, e = 0;
+                    uint32_t rw[4] = {0, 0, 0, 0};
+#pragma unroll
+                    for (int i = 0; i < 16; ++i) {
+                        const int d = res[i] - rr[i];
+                        D += d * d;
+                        const int v = clip8(pr[i] + rr[i]);
+                        rw[i >> 2] |= (uint32_t)v << (8 * (i & 3));
+                        const int d2 = cu[i] - v;
+                        e += d2 * d2;
+                    }
+                    int lastF = -1;
+#pragma unroll
+                    for (int k = 0; k < 16; ++k) if (lt[k_SCAN4c(k)]) lastF = k;
+                    int fb = 3 + egbits((uint32_t)(lastF + 1));               // FDV_BITS_MODE (fdv.h:3351)
+#pragma unroll
+                    for (int k = 0; k < 16; ++k) if (k <= lastF) fb += egbits(zmap(lt[k_SCAN4c(k)]));
+                    R.rdj[t] = (double)D + U.lambda * (double)rb;
+                    R.jf[t] = (double)e + U.lambda * (double)fb;
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) R.rw[t][r] = rw[r];
+                }
             }
-            int co[16];
-            fdct4_reg(res, co);
-#pragma unroll
-            for (int i = 0; i < 16; ++i) {
-                const int mf = k_MF[m6][cls4(i)];
-                const int aa = co[i] < 0 ? -co[i] : co[i];
-                const int v = (aa * mf + off) >> qb;
-                lv[i] = co[i] < 0 ? -v : v;
-            }
-            int dq[16], rr[16];
-#pragma unroll
-            for (int i = 0; i < 16; ++i) dq[i] = (lv[i] * k_DV[m6][cls4(i)]) * sh;
-            idct4_reg(dq, rr);
-            int e = 0;
-#pragma unroll
-            for (int i = 0; i < 16; ++i) { const int d = res[i] + pr[i] - clip8(pr[i] + rr[i]); e += d * d; }
-            int last = -1;
-#pragma unroll
-            for (int k = 0; k < 16; ++k) if (lv[k_SCAN4c(k)]) last = k;
-            int bits = 3 + egbits((uint32_t)(last + 1));
-#pragma unroll
-            for (int k = 0; k < 16; ++k) if (k <= last) bits += egbits(zmap(lv[k_SCAN4c(k)]));
-            Jc = (double)e + U.lambda * (double)bits;
         }
     }
+    wave_sync();
+    // the candidates' costs without RDOQ, in the reference's order of
+    // comparison: lane c reads candidate c's untruncated point
     int bc = 0;
     {
-        double best = -1.0;
+        double Jc = 0.0;
+        if (L < ne) Jc = S.l4[L].jf[S.l4[L].last + 1];
         const uint64_t jb = __double_as_longlong(Jc);
         const uint32_t jlo = (uint32_t)jb, jhi = (uint32_t)(jb >> 32);
+        double best = -1.0;
 #pragma unroll
         for (int c = 0; c < 9; ++c) {
             if (c < ne) {
@@ -438,19 +559,35 @@ __device__ __forceinline__ double leaf4(IntraLds &S, const UnitCtx &U, int ux, i
     int mw = km[0];
 #pragma unroll
     for (int c = 1; c < 9; ++c) if (bc == c) mw = km[c];
-    // the winner with RDOQ
-    const uint32_t *pp = (const uint32_t *)S.pred4[mw];
-    const uint32_t pw[4] = {pp[0], pp[1], pp[2], pp[3]};
-    uint32_t rw[4];
-    int lf[16], e, bits;
-    final4(pw, sw, U.qp, U.lambda, S.rdj, L, rw, lf, e, bits);
-    if (L == 0) {
+    // the RDOQ's choice for the winner: its scan from the last level down,
+    // the first lowest score among the points it may stop at
+    int bt = 0;
+    {
+        const L4Res &R = S.l4[bc];
+        const int last = R.last;
+        double rj = 0.0;
+        int vl = 0;
+        if (L <= last + 1) { rj = R.rdj[L]; vl = R.valid[L]; }
+        const uint64_t vmask = __ballot(vl != 0);
+        const uint64_t jb = __double_as_longlong(rj);
+        const uint32_t jlo = (uint32_t)jb, jhi = (uint32_t)(jb >> 32);
+        double bestJ = -1.0;
+        bt = last + 1;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) *(uint32_t *)&S.cv[CV(uy + r, ux)] = rw[r];
-        S.cm[(uy >> 2) * 16 + (ux >> 2)] = (uint8_t)mw;
+        for (int k = 16; k >= 0; --k) {
+            if (k <= last + 1 && ((vmask >> k) & 1)) {
+                const uint64_t v = (uint64_t)(uint32_t)__builtin_amdgcn_readlane(jhi, k) << 32 |
+                                   (uint32_t)__builtin_amdgcn_readlane(jlo, k);
+                const double J = __longlong_as_double((long long)v);
+                if (bestJ < 0.0 || J < bestJ) { bestJ = J; bt = k; }
+            }
+        }
     }
+    const L4Res &W = S.l4[bc];
+    if (L < 4) *(uint32_t *)&S.cv[CV(uy + L, ux)] = W.rw[bt][L];
+    if (L == 0) S.cm[(uy >> 2) * 16 + (ux >> 2)] = (uint8_t)mw;
+    const double J = W.jf[bt];
     wave_sync();
-    const double J = (double)e + U.lambda * (double)bits;
     const uint64_t jb = __double_as_longlong(J);
     const uint64_t v = (uint64_t)(uint32_t)__builtin_amdgcn_readlane((uint32_t)(jb >> 32), 0) << 32 |
                        (uint32_t)__builtin_amdgcn_readlane((uint32_t)jb, 0);
@@ -594,6 +731,7 @@ k_intra_unit(const fdvg_iplane *__restrict__ planes, const uint32_t *__restrict_
     }
     for (int i = tid; i < 4; i += blockDim.x) { S.claim[i] = 0; S.gen[i] = 0; S.njobs[i] = 0; S.done[i] = 0; }
     for (int i = tid; i < 85; i += blockDim.x) { S.split[i] = 0; S.nmode[i] = 0; }
+    for (int i = tid; i < 36; i += blockDim.x) S.p4idx[i] = k_P4IDX[i];
     if (tid == 0) S.quit = 0;
     __syncthreads();
     if (S.plane >= 4095) return;
