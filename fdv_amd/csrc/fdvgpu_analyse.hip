@@ -712,45 +712,41 @@ k_analyse(const fdvg_seq *__restrict__ seqs, const int2 *__restrict__ rowtab, in
 
 // ---------------------------------------------------- block-parallel decisions
 //
-// The parallel and learned modes replace the decision wavefront by
-//   k_pcosts   every MB's candidate costs, INTRA from its neighbours'
-//              stand-in reconstruction edges (persistent workgroups, MBs in
-//              the replay's order, costs published as tagged words);
-//   k_prepair  the reference's decision logic (fdv.h:5847-6040) replayed in a
-//              light wavefront (one wavefront per MB row, motion records only)
-//              on those costs, with the true vector predictor and merge list;
-//   k_pencode  the decided non-intra MBs coded exactly, all at once;
-//   k_pintra   the intra MBs coded in dependency order (an intra MB waits only
-//              for intra neighbours; inter ones are already reconstructed);
-//   k_pdecide  k_prepair, k_pencode and k_pintra in one launch, which may run
-//              at the same time as k_pcosts (its replay waits for the costs'
-//              tags).
-// The one approximation is that intra costs are computed from stand-in
-// neighbours instead of the reconstruction, which exists only once every
-// decision is made; the intra sub-mode is screened again at coding time on the
-// true reconstruction.
+// The parallel and learned modes make the MB loop's own decisions
+// (fdv.h:5808-6162) on a motion table, as the exhaustive mode does, but take
+// the per-MB coding off the MB chain:
+//   k_pcosts   every MB's candidate costs except INTRA, all MBs at once (one
+//              workgroup a MB, taken in the replay's order): the SKIP SSDs of
+//              likely merge candidates, and INTER16 on each reference and
+//              INTER8 coded, with their reconstructions' bottom rows and right
+//              columns; published as tagged words;
+//   k_pxrep    the reference's decision logic replayed on those costs, one
+//              workgroup a MB row: the true vector predictor and merge list,
+//              the gates and the strict-< commit, and INTRA costed in the row
+//              from the true reconstruction of the MB's left, top and top-left
+//              neighbours (the chosen candidates' edges), so that every
+//              decision is the MB loop's;
+//   k_pencode / k_pintra, or the coding workers of k_pdecide: the decided MBs
+//              coded exactly.
+// The costs never wait; a replay row waits for its MBs' cost words, for the
+// row above (a lower ticket) and nothing else; the coding workers wait only
+// for decisions.
 
 __device__ __forceinline__ int mb_count(const fdvg_seq &q) { return (q.w >> 4) * (q.h >> 4); }
 
-// The costs of every MB's candidates except INTRA, and the MB's stand-in
-// reconstruction edges for its neighbours' intra costs (k_picost): the true
-// reconstruction exists only once every decision is made, so each MB stands in
-// with what its own candidates say it will most likely be:
-//   - INTER16 on each reference with the table vector, coded, costs
-//     J = SSD + lambda (bits + vector bits against (0, 0)); the best reference is
-//     the first with the smallest J;
-//   - SKIP at (0, 0) on reference 0 costs Js = SSD + 3 lambda;
-//   - the stand-in is the co-located reference-0 block if Js <= 14 lambda (the
-//     INTRA floor) or Js <= the best INTER16 J, else that INTER16 reconstruction.
-// edge[mb * 32 ..]: the stand-in's bottom row (16), then its right column (16).
-#define PC_WORDS 36                      // a MB's costs as tagged words (4 payload bytes each)
-#define EW_WORDS 8                       // a MB's stand-in edges as tagged words
+#define PC_WORDS 34                      // a MB's costs as tagged words (4 payload bytes each)
+#define EW_WORDS 24                      // a MB's candidate edges as tagged words
+#define BW_WORDS 4                       // a decided MB's bottom reconstruction row, tagged
 static_assert(sizeof(fdvg_pcost) == 4 * PC_WORDS, "cost words");
 
-// The costs are built in LDS (P) and published by k_pcosts as tagged words;
-// the stand-in edges go to sout (LDS).  q.given must be NULL.
-__device__ __forceinline__ void pcost_mb(Smem &s, int32_t (*sred)[4], uint8_t *sedge, uint8_t *sout,
-                                         const fdvg_seq &q, int i, fdvg_pcost *P) {
+// The costs of one MB's candidates except INTRA, built in LDS (P), and the
+// bottom rows and right columns of its coded candidates' reconstructions in
+// sedge: [0..31] INTER16 on reference 0 (bottom row 16, then right column 16),
+// [32..63] on reference 1, [64..95] INTER8.  Entries of candidates that are not
+// coded (a reference that does not exist, INTER8 below its gate) are never
+// read.  q.given must be NULL.
+__device__ __forceinline__ void pcost_mb(Smem &s, int32_t (*sred)[4], uint8_t *sedge, const fdvg_seq &q, int i,
+                                         fdvg_pcost *P) {
     const int tid = threadIdx.x, wave = tid >> 6;
     const int mw = q.w >> 4, mx = i % mw, my = i / mw, bx = mx * 16, by = my * 16;
     const int mb = q.mb0 + i;
@@ -803,11 +799,9 @@ __device__ __forceinline__ void pcost_mb(Smem &s, int32_t (*sred)[4], uint8_t *s
         P->sce[c] = c < ncs ? sred[c][0] : 0;
     }
     if (tid == 0) P->nsc = (uint8_t)ncs;
-    const int e00 = sred[0][0];                      // candidate 0 is (0, 0) on ref 0
     const int qp = q.qp;
     const double lambda = q.lambda;
     // INTER16 on each reference
-    double bj = 0.0;
     for (int r = 0; r < q.navail; ++r) {
         s.pred[tid] = (uint8_t)lumaq(s.win[r], tid & 15, tid >> 4, T->mv16[r][0], T->mv16[r][1]);
         __syncthreads();
@@ -815,21 +809,8 @@ __device__ __forceinline__ void pcost_mb(Smem &s, int32_t (*sred)[4], uint8_t *s
         const int d = s.ws.e[0] + s.ws.e[1] + s.ws.e[2] + s.ws.e[3];
         const int b = 6 + s.ws.b[0] + s.ws.b[1] + s.ws.b[2] + s.ws.b[3];
         if (tid == 0) { P->d16[r] = d; P->b16[r] = b; }
-        const double j = (double)d + lambda * (double)(b + egbits(zmap(T->mv16[r][0])) + egbits(zmap(T->mv16[r][1])));
-        if (r == 0 || j < bj) {                  // uniform
-            bj = j;
-            if (tid < 16) sedge[tid] = s.recT[15 * 16 + tid];
-            else if (tid < 32) sedge[tid] = s.recT[(tid - 16) * 16 + 15];
-        }
+        if (tid < 32) sedge[32 * r + tid] = tid < 16 ? s.recT[15 * 16 + tid] : s.recT[(tid - 16) * 16 + 15];
         __syncthreads();
-    }
-    {
-        const double js = (double)e00 + lambda * 3.0;
-        const bool skip = js <= lambda * 14.0 || js <= bj;
-        if (tid < 32) {
-            const int y = tid < 16 ? 15 : tid - 16, x = tid < 16 ? tid : 15;
-            sout[tid] = skip ? s.win[0][(WOFF + y) * WIN + WOFF + x] : sedge[tid];
-        }
     }
     // INTER8 is evaluated only when the 16x16 SAD of the chosen reference (either
     // table SAD) exceeds the quadrants' sum by more than 10% (fdv.h:6014); where
@@ -855,100 +836,47 @@ __device__ __forceinline__ void pcost_mb(Smem &s, int32_t (*sred)[4], uint8_t *s
         P->d8 = s.ws.e[0] + s.ws.e[1] + s.ws.e[2] + s.ws.e[3];
         P->b8 = 3 + s.ws.b[0] + s.ws.b[1] + s.ws.b[2] + s.ws.b[3];
     }
-}
-
-// INTRA 16x16 cost of one MB from its neighbours' stand-in edges (edge words,
-// EW_WORDS tagged words a MB: bottom row, then right column, 4 bytes a word;
-// waited for): SATD screen of the four modes, then one coding of the best.
-__device__ __forceinline__ void picost_mb(Smem &s, const fdvg_seq &q, int i, fdvg_pcost *P, const uint64_t *edgew,
-                                          unsigned *ctl) {
-    const int tid = threadIdx.x, wave = tid >> 6;
-    const int mw = q.w >> 4, mx = i % mw, my = i / mw, bx = mx * 16, by = my * 16;
-    if (gmem(q.metab)[i].flags & 1) {        // the proposer expects SKIP: no INTRA
-        if (tid == 0) { P->dn = P->bn = 0; P->nsub = 0; }
-        return;
-    }
-    s.cur[tid] = gmem(q.cy)[(size_t)(by + (tid >> 4)) * q.w + bx + (tid & 15)];
-    const uint64_t *E = edgew + (size_t)q.mb0 * EW_WORDS;
-    const unsigned st = q.step;
-    if (tid < 16) s.hbytes[1 + tid] = my > 0 ? ih_byte(E + (size_t)(i - mw) * EW_WORDS, tid >> 2, tid & 3, st, ctl) : 0;
-    else if (tid == 16)
-        s.hbytes[0] = (my > 0 && mx > 0) ? ih_byte(E + (size_t)(i - mw - 1) * EW_WORDS, 3, 3, st, ctl) : 128;
-    else if (tid >= 32 && tid < 48)
-        s.lcol[tid - 32] = mx > 0 ? ih_byte(E + (size_t)(i - 1) * EW_WORDS, 4 + ((tid - 32) >> 2), tid & 3, st, ctl) : 0;
-    __syncthreads();
-    const int qp = q.qp;
-    const double lambda = q.lambda;
-    const Nb nbL = nb_setup(&s.hbytes[1], s.lcol, s.hbytes[0], 16, my > 0, mx > 0);
-    if (wave == 0) {
-        int sat[4];
-        satd_modes(s.cur, nbL, &s.hbytes[1], s.lcol, sat);
-        if (tid == 0) for (int m = 0; m < 4; ++m) s.me[6][m] = sat[m];
-    }
-    __syncthreads();
-    int sel = -1, sbest = 0;
-    for (int m = 0; m < 4; ++m) {
-        if (!nn_ok(m, nbL)) continue;
-        if (sel < 0 || s.me[6][m] < sbest) { sbest = s.me[6][m]; sel = m; }
-    }
-    s.pred[tid] = (uint8_t)ipred(sel, nbL, &s.hbytes[1], s.lcol, 16, tid & 15, tid >> 4);
-    __syncthreads();
-    code_luma(s, s.pred, s.recN, s.levN, s.flagN, qp, lambda);
-    if (tid == 0) {
-        P->dn = s.ws.e[0] + s.ws.e[1] + s.ws.e[2] + s.ws.e[3];
-        P->bn = 6 + s.ws.b[0] + s.ws.b[1] + s.ws.b[2] + s.ws.b[3];
-        P->nsub = (uint8_t)sel;
-    }
+    if (tid < 32) sedge[64 + tid] = tid < 16 ? s.rec8[15 * 16 + tid] : s.rec8[(tid - 16) * 16 + 15];
 }
 
 // LDS of one MB's costs (k_pcosts).
 struct CostLds {
     Smem s;
     int32_t sred[FDVG_NSC][4];
-    uint8_t sedge[32], sout[32];
+    uint8_t sedge[96];
     union { fdvg_pcost c; uint32_t w[PC_WORDS]; } sp;
 };
 
-// The candidate costs of MB mb (pcost_mb, then picost_mb, whose INTRA cost
-// waits for the left, top and top-left neighbours' stand-in edges), published
-// as tagged words: the costs (PC_WORDS a MB), read by the replay, which may run
-// at the same time, and the edges (EW_WORDS a MB).
-__device__ __forceinline__ void cost_job(CostLds &C, const fdvg_seq *__restrict__ seqs, int nseq, int mb, unsigned *ctl,
-                                         uint64_t *__restrict__ pcw, uint64_t *__restrict__ edgew) {
-    const int tid = threadIdx.x;
-    int sq = 0;
-    for (int k = 1; k < nseq; ++k) if (mb >= seqs[k].mb0) sq = k;
-    fdvg_seq q = seqs[sq];
-    q.given = nullptr;
-    const int i = mb - q.mb0;
-    pcost_mb(C.s, C.sred, C.sedge, C.sout, q, i, &C.sp.c);
-    __syncthreads();
-    if (tid < EW_WORDS) {
-        const uint32_t v = (uint32_t)C.sout[4 * tid] | (uint32_t)C.sout[4 * tid + 1] << 8 |
-                           (uint32_t)C.sout[4 * tid + 2] << 16 | (uint32_t)C.sout[4 * tid + 3] << 24;
-        __hip_atomic_store(edgew + (size_t)mb * EW_WORDS + tid, (uint64_t)q.step << 32 | v, __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-    }
-    picost_mb(C.s, q, i, &C.sp.c, edgew, ctl);
-    __syncthreads();
-    if (tid < PC_WORDS)
-        __hip_atomic_store(pcw + (size_t)mb * PC_WORDS + tid, (uint64_t)q.step << 32 | C.sp.w[tid], __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-}
-
 // The costs of every MB, one workgroup a MB taken by ticket (ctl[3]) in the
-// order of `corder` (the replay's order): every MB's left, top and top-left
-// neighbours have lower tickets, so their workgroups have started.
+// order of `corder` (the replay's order), published as tagged words: the costs
+// (PC_WORDS a MB) and the candidate edges (EW_WORDS a MB, 4 samples a word),
+// read by the replay, which may run at the same time.
 extern "C" __global__ void __launch_bounds__(256, 4)
 k_pcosts(const fdvg_seq *__restrict__ seqs, int nseq, const unsigned *__restrict__ corder, int total_mb,
          unsigned *ctl, uint64_t *__restrict__ pcw, uint64_t *__restrict__ edgew) {
     __shared__ CostLds C;
     __shared__ int job_s;
-    if (threadIdx.x == 0) job_s = (int)atomicAdd(ctl + 3, 1u);
+    const int tid = threadIdx.x;
+    if (tid == 0) job_s = (int)atomicAdd(ctl + 3, 1u);
     __syncthreads();
     const int job = __builtin_amdgcn_readfirstlane(job_s);
     if (job >= total_mb) return;
-    cost_job(C, seqs, nseq, __builtin_amdgcn_readfirstlane((int)corder[job]), ctl, pcw, edgew);
+    const int mb = __builtin_amdgcn_readfirstlane((int)corder[job]);
+    int sq = 0;
+    for (int k = 1; k < nseq; ++k) if (mb >= seqs[k].mb0) sq = k;
+    fdvg_seq q = seqs[sq];
+    q.given = nullptr;
+    pcost_mb(C.s, C.sred, C.sedge, q, mb - q.mb0, &C.sp.c);
+    __syncthreads();
+    const uint64_t tag = (uint64_t)q.step << 32;
+    if (tid < EW_WORDS) {
+        const uint32_t v = (uint32_t)C.sedge[4 * tid] | (uint32_t)C.sedge[4 * tid + 1] << 8 |
+                           (uint32_t)C.sedge[4 * tid + 2] << 16 | (uint32_t)C.sedge[4 * tid + 3] << 24;
+        __hip_atomic_store(edgew + (size_t)mb * EW_WORDS + tid, tag | v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else if (tid >= 64 && tid < 64 + PC_WORDS) {
+        __hip_atomic_store(pcw + (size_t)mb * PC_WORDS + tid - 64, tag | C.sp.w[tid - 64], __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    }
 }
 
 // Motion record handoff of the decision wavefront: tag | inter | ref | x | y.
@@ -956,7 +884,6 @@ __device__ __forceinline__ uint64_t mrec_pack(uint64_t tag, int inter, int ref, 
     return tag | ((uint64_t)(inter & 1) << 33) | ((uint64_t)(ref & 1) << 32) |
            ((uint64_t)(uint16_t)x << 16) | (uint64_t)(uint16_t)y;
 }
-
 
 // SSD of the 16x16 block `curb` against its prediction from a tile (row stride
 // S, the block's (0, 0) at tile (2, 2)) at quarter-pel phase ph, by one
@@ -967,42 +894,49 @@ __device__ __forceinline__ int ssd_tile_q(const uint8_t *tile, const uint8_t *cu
     return qerr4<S, true, true>(tile, x + 2, y0 + 2, ph, curb + y0 * 16 + x, 16);
 }
 
-// The decision wavefront of one sequence inside one workgroup: wave w replays
-// rows w, w + NW, ...; a row's motion records go to an LDS ring slot per row
-// and its progress to prog[w], which the wave of the row below polls.  Each
-// wave stages its row's costs and tables PR_CHUNK MBs at a time.
+// The replay wave's staging of a row, PR_CHUNK MBs at a time: costs, candidate
+// edges, table entries and source MBs.
 #define PR_CHUNK 8
 #define PR_TS 24                         // tile stride: 21 samples a row
 struct __align__(16) PrWave {
     fdvg_pcost pc[PR_CHUNK];
     fdvg_metab mt[PR_CHUNK];
+    uint32_t ew[PR_CHUNK][EW_WORDS];     // candidate edges, 4 samples a word
     uint8_t cur[PR_CHUNK][256];          // the source MBs, for SSDs outside the precomputed set
     uint8_t tile[21 * PR_TS];
 };
 
-// A chunk of a row in flight: every lane's share of the costs, tables and
-// source MBs, fixed trip counts so the loads issue back to back.
+// A chunk of a row in flight: every lane's share of the costs, edges, tables
+// and source MBs, fixed trip counts so the loads issue back to back.
 #define PR_PW ((PR_CHUNK * PC_WORDS + 63) / 64)
+#define PR_EW ((PR_CHUNK * EW_WORDS + 63) / 64)
 #define PR_MW ((PR_CHUNK * (int)sizeof(fdvg_metab) / 4 + 63) / 64)
 #define PR_CW (PR_CHUNK * 64 / 64)
 struct PrPre {
-    uint64_t p[PR_PW];                   // tagged cost words
+    uint64_t p[PR_PW], e[PR_EW];         // tagged cost and edge words
     uint32_t m[PR_MW], c[PR_CW];
     int n;
 };
-// Issues the chunk's loads; the cost words may still be in flight (k_pcosts
-// running concurrently): pr_store waits for their tags.
-__device__ __forceinline__ void pr_load(PrPre &R, const fdvg_seq &q, const uint64_t *pcw, int my, int mx0, int L) {
+// Issues the chunk's loads; the cost and edge words may still be in flight
+// (k_pcosts running concurrently): pr_store waits for their tags.
+__device__ __forceinline__ void pr_load(PrPre &R, const fdvg_seq &q, const uint64_t *pcw, const uint64_t *edgew,
+                                        int my, int mx0, int L) {
     const int mw = q.w >> 4, i0 = my * mw + mx0;
     const int n = mw - mx0 < PR_CHUNK ? mw - mx0 : PR_CHUNK;
-    const uint64_t *gp = pcw + (size_t)(q.mb0 + i0) * PC_WORDS;
+    const uint64_t *gp = pcw + (size_t)(q.mb0 + i0) * PC_WORDS, *ge = edgew + (size_t)(q.mb0 + i0) * EW_WORDS;
     const gmem_t<uint32_t> *gm = gmem((const uint32_t *)(q.metab + i0));
     const gmem_t<uint8_t> *gc = gmem(q.cy);
-    const int np = n * PC_WORDS, nm = n * (int)(sizeof(fdvg_metab) / 4);
+    const int np = n * PC_WORDS, ne = n * EW_WORDS, nm = n * (int)(sizeof(fdvg_metab) / 4);
+    const uint64_t none = (uint64_t)q.step << 32;
 #pragma unroll
     for (int j = 0; j < PR_PW; ++j) {
         const int k = L + 64 * j;
-        R.p[j] = k < np ? __hip_atomic_load(gp + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : (uint64_t)q.step << 32;
+        R.p[j] = k < np ? __hip_atomic_load(gp + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : none;
+    }
+#pragma unroll
+    for (int j = 0; j < PR_EW; ++j) {
+        const int k = L + 64 * j;
+        R.e[j] = k < ne ? __hip_atomic_load(ge + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : none;
     }
 #pragma unroll
     for (int j = 0; j < PR_MW; ++j) { const int k = L + 64 * j; R.m[j] = k < nm ? gm[k] : 0u; }
@@ -1013,17 +947,19 @@ __device__ __forceinline__ void pr_load(PrPre &R, const fdvg_seq &q, const uint6
     }
     R.n = n;
 }
-// Waits until every cost word of the chunk carries this frame step's id
-// (reloading the stale ones; bounded, error flag on timeout), then stores the
-// chunk into the wave's LDS block.
-__device__ __forceinline__ void pr_store(PrWave &W, PrPre &R, const fdvg_seq &q, const uint64_t *pcw, unsigned *ctl,
-                                         int my, int mx0, int L) {
+// Waits until every cost and edge word of the chunk carries this frame step's
+// id (reloading the stale ones; bounded, error flag on timeout), then stores
+// the chunk into the wave's LDS block.
+__device__ __forceinline__ void pr_store(PrWave &W, PrPre &R, const fdvg_seq &q, const uint64_t *pcw,
+                                         const uint64_t *edgew, unsigned *ctl, int my, int mx0, int L) {
     const int mw = q.w >> 4, i0 = my * mw + mx0;
-    const uint64_t *gp = pcw + (size_t)(q.mb0 + i0) * PC_WORDS;
-    const int np = R.n * PC_WORDS, nm = R.n * (int)(sizeof(fdvg_metab) / 4);
+    const uint64_t *gp = pcw + (size_t)(q.mb0 + i0) * PC_WORDS, *ge = edgew + (size_t)(q.mb0 + i0) * EW_WORDS;
+    const int np = R.n * PC_WORDS, ne = R.n * EW_WORDS, nm = R.n * (int)(sizeof(fdvg_metab) / 4);
     bool ok = true;
 #pragma unroll
     for (int j = 0; j < PR_PW; ++j) ok = ok && (uint32_t)(R.p[j] >> 32) == q.step;
+#pragma unroll
+    for (int j = 0; j < PR_EW; ++j) ok = ok && (uint32_t)(R.e[j] >> 32) == q.step;
     for (unsigned spins = 0; !__all(ok); ++spins) {
         if (spins > (1u << 22)) { if (L == 0) atomicOr(ctl + 1, 1u); break; }
         __builtin_amdgcn_s_sleep(1);
@@ -1035,76 +971,141 @@ __device__ __forceinline__ void pr_store(PrWave &W, PrPre &R, const fdvg_seq &q,
                 R.p[j] = __hip_atomic_load(gp + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             ok = ok && (uint32_t)(R.p[j] >> 32) == q.step;
         }
+#pragma unroll
+        for (int j = 0; j < PR_EW; ++j) {
+            const int k = L + 64 * j;
+            if (k < ne && (uint32_t)(R.e[j] >> 32) != q.step)
+                R.e[j] = __hip_atomic_load(ge + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            ok = ok && (uint32_t)(R.e[j] >> 32) == q.step;
+        }
     }
-    uint32_t *lp = (uint32_t *)W.pc, *lm = (uint32_t *)W.mt, *lc = (uint32_t *)W.cur;
+    uint32_t *lp = (uint32_t *)W.pc, *le = &W.ew[0][0], *lm = (uint32_t *)W.mt, *lc = (uint32_t *)W.cur;
 #pragma unroll
     for (int j = 0; j < PR_PW; ++j) { const int k = L + 64 * j; if (k < np) lp[k] = (uint32_t)R.p[j]; }
+#pragma unroll
+    for (int j = 0; j < PR_EW; ++j) { const int k = L + 64 * j; if (k < ne) le[k] = (uint32_t)R.e[j]; }
 #pragma unroll
     for (int j = 0; j < PR_MW; ++j) { const int k = L + 64 * j; if (k < nm) lm[k] = R.m[j]; }
 #pragma unroll
     for (int j = 0; j < PR_CW; ++j) lc[L + 64 * j] = R.c[j];
 }
 
-// One row of the decision replay (the reference's decision logic on the stored
-// costs, fdv.h:5847-6040), one wave: MB mx waits until the row above has
-// decided MB mx + 1 (or its last MB), whose motion records it reads from the
-// LDS slot `above`: written by a wave of this workgroup (progress word *up) or,
-// with above_g, copied there a chunk at a time from another workgroup's tagged
-// records (the row then runs at least a chunk behind the row above).  FUSED also publishes each
-// decision for the coding workers of k_pdecide: the motion record in mrec_g and
-// the decision with its context in decw, both tagged with the frame step.
+// ---- the exact replay (k_pxrep)
+//
+// One workgroup of XR_WAVES waves replays one MB row.  Wave 0 runs the
+// reference's decision logic (fdv.h:5847-6040) on the stored costs, MB by MB;
+// waves 1-4 are the INTRA trial: for MB x they screen the four sub-modes by
+// SATD from the true reconstruction around it (the row above's bottom rows,
+// published by the row above's workgroup; the left MB's right column, just
+// decided by wave 0) and code the chosen prediction, one 8x8 region a wave,
+// while wave 0 weighs the other candidates.  When wave 0 does not consult
+// INTRA for an MB it tells the trial to stop (`cancel`).  After each decision
+// wave 0 forms the chosen candidate's bottom row and right column (INTER16 and
+// INTER8: from k_pcosts; SKIP: the prediction itself, from a tile of the
+// reference; INTRA: the trial's reconstruction): the right column is the next
+// MB's left neighbour, the bottom row goes to the row below as BW_WORDS tagged
+// words.  The trial's inputs are double-buffered by MB parity; wave 0 refills
+// a buffer only once the trial has finished the MB that used it.
+#define XR_WAVES 5
+struct __align__(16) XrLds {
+    PrWave pw;
+    uint8_t itop[2][20];                  // per MB parity: [0] the corner, [1..16] the row above
+    uint8_t ileft[2][16];                 // per MB parity: the left MB's right column
+    uint8_t icur[4][256];                 // each trial wave's copy of the source MB
+    uint8_t ipred[256], irec[256];
+    int16_t ilev[256];
+    uint8_t iflag[4];
+    RegionWs ws;
+    int go;                               // MB x's trial inputs are ready once go > x
+    int cancel[2];                        // per MB parity: x + 1 when MB x's INTRA cost is not wanted
+    int done[4];                          // per trial wave: MBs finished
+};
+
+__device__ __forceinline__ int lds_acq(int *p) { return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP); }
+// A bounded wait gives up after `limit` spins, or sooner once another wait
+// has given up (the error flag ctl[1], read every 256 spins): a failed step
+// then ends quickly instead of every later wait running to its own limit.
+__device__ __forceinline__ bool xr_give_up(unsigned spins, unsigned limit, unsigned *ctl) {
+    if (spins > limit) { atomicOr(ctl + 1, 1u); return true; }
+    return (spins & 255u) == 255u && __hip_atomic_load(ctl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u;
+}
+__device__ __forceinline__ void lds_rel(int *p, int v) { __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP); }
+
+// Wave 0 waits until the trial waves have finished MB x - 1 (done >= x).
+__device__ __forceinline__ void xr_wait_done(XrLds &X, int x, unsigned *ctl, int L) {
+    int v = L < 4 ? lds_acq(&X.done[L]) : x;
+    for (unsigned spins = 0; !__all(v >= x); ++spins) {
+        if (xr_give_up(spins, 1u << 24, ctl)) break;
+        __builtin_amdgcn_s_sleep(1);
+        if (L < 4) v = lds_acq(&X.done[L]);
+    }
+}
+
 #define PD_WORDS 10
 __device__ __forceinline__ uint32_t pk16(int lo, int hi) { return (uint32_t)(uint16_t)lo | (uint32_t)(uint16_t)hi << 16; }
-__device__ __forceinline__ uint64_t ld_rec(const uint64_t *p, int k) {
-    return __hip_atomic_load(p + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-template <bool FUSED>
-__device__ __forceinline__ void replay_row(PrWave &W, const fdvg_seq &q, const uint64_t *__restrict__ pcw,
-                                           unsigned *ctl, int my,
-                                           uint64_t *cur_slot, int *myprog, uint64_t *above, const int *up,
-                                           const uint64_t *above_g, uint64_t *mrec_g, uint64_t *decw,
-                                           fdvg_mbinfo *__restrict__ given, fdvg_pctx *__restrict__ pctx, int L) {
+
+// Wave 0 of k_pxrep: the decisions of row my.  Publishes each decision as
+// given[] / pctx[] (for the separate coding launches), as ten tagged words in
+// decw (for the coding workers of k_pdecide), and as a tagged motion record in
+// mrec (for the row below), followed by the bottom row in botw.
+__device__ __forceinline__ void xr_decide(XrLds &X, const fdvg_seq &q, const uint64_t *__restrict__ pcw,
+                                          const uint64_t *__restrict__ edgew, unsigned *ctl, int my,
+                                          uint64_t *__restrict__ mrec, uint64_t *__restrict__ botw,
+                                          uint64_t *__restrict__ decw, fdvg_mbinfo *__restrict__ given,
+                                          fdvg_pctx *__restrict__ pctx, int L) {
+    PrWave &W = X.pw;
     const int mw = q.w >> 4;
     const double lambda = q.lambda;
     const uint64_t tagv = (uint64_t)(q.step & 0x3fffffffu) << 34;
+    const unsigned step = q.step;
+    const uint64_t *arec = my > 0 ? mrec + q.mb0 + (size_t)(my - 1) * mw : nullptr;
+    const uint64_t *abot = my > 0 ? botw + (size_t)(q.mb0 + (my - 1) * mw) * BW_WORDS : nullptr;
+    // The row above, lane k < 4: bottom word k of an above MB; lanes 4, 5:
+    // motion records.  Waits (bounded) for the words of this step.
+    auto above = [&](int ib, int ir0, int ir1) -> uint64_t {
+        const uint64_t *p = L < 4 ? abot + (size_t)ib * BW_WORDS + L : L == 4 ? arec + ir0 : arec + ir1;
+        const bool act = L < 6 && (L < 4 ? ib >= 0 : true);
+        uint64_t v = act ? __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
+        auto fresh = [&](uint64_t w) { return !act || (L < 4 ? (uint32_t)(w >> 32) == step : (w >> 34) == (tagv >> 34)); };
+        for (unsigned spins = 0; !__all(fresh(v)); ++spins) {
+            if (xr_give_up(spins, 1u << 22, ctl)) break;
+            __builtin_amdgcn_s_sleep(1);
+            if (!fresh(v)) v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        return v;
+    };
+    // MB 0's trial inputs
+    uint64_t ab = 0;
+    if (my > 0) ab = above(0, 0, mw > 1 ? 1 : 0);
+    if (L < 4) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) X.itop[0][1 + 4 * L + j] = (uint8_t)(ab >> (8 * j));
+    }
+    if (L == 0) X.itop[0][0] = 128;
+    wave_sync();
+    if (L == 0) lds_rel(&X.go, 1);
+
     int pmx = 0, pmy = 0;
     PrPre pre;
-    uint64_t abv = tagv;
     for (int mx = 0; mx < mw; ++mx) {
         const int i = my * mw + mx, mb = q.mb0 + i, bx = mx * 16, by = my * 16;
         if (mx % PR_CHUNK == 0) {
-            // this chunk's costs, tables and source MBs into LDS (from the
-            // registers loaded during the previous chunk, or now at a row
-            // start), then the next chunk's loads issued into registers
-            if (mx == 0) pr_load(pre, q, pcw, my, 0, L);
-            pr_store(W, pre, q, pcw, ctl, my, mx, L);
-            if (my > 0 && above_g) {
-                // the row above, in another workgroup: its records of MBs
-                // mx .. mx + PR_CHUNK (tagged), waited for and copied into
-                // `above` once a chunk; the next chunk's are loaded early
-                if (mx == 0) abv = L <= PR_CHUNK ? ld_rec(above_g, mx + L < mw ? mx + L : mw - 1) : tagv;
-                for (unsigned spins = 0; !__all((abv >> 34) == (tagv >> 34)); ++spins) {
-                    if (spins > (1u << 21)) { if (L == 0) atomicOr(ctl + 1, 1u); break; }
-                    __builtin_amdgcn_s_sleep(1);
-                    if (L <= PR_CHUNK && (abv >> 34) != (tagv >> 34)) abv = ld_rec(above_g, mx + L < mw ? mx + L : mw - 1);
-                }
-                if (L <= PR_CHUNK && mx + L < mw) above[mx + L] = abv;
-                if (mx + PR_CHUNK < mw)
-                    abv = L <= PR_CHUNK ? ld_rec(above_g, mx + PR_CHUNK + L < mw ? mx + PR_CHUNK + L : mw - 1) : tagv;
-            }
+            if (mx == 0) pr_load(pre, q, pcw, edgew, my, 0, L);
+            pr_store(W, pre, q, pcw, edgew, ctl, my, mx, L);
             wave_sync();
-            if (mx + PR_CHUNK < mw) pr_load(pre, q, pcw, my, mx + PR_CHUNK, L);
+            if (mx + PR_CHUNK < mw) pr_load(pre, q, pcw, edgew, my, mx + PR_CHUNK, L);
         }
         const fdvg_pcost &P = W.pc[mx % PR_CHUNK];
         const fdvg_metab &T = W.mt[mx % PR_CHUNK];
-        // everything this MB reads from LDS, before the wait
         const int nsc = P.nsc;
         const int sx_l = L < FDVG_NSC ? P.scx[L] : 0, sy_l = L < FDVG_NSC ? P.scy[L] : 0;
         const int sr_l = L < FDVG_NSC ? P.scr[L] : 0, se_l = L < FDVG_NSC ? P.sce[L] : 0;
         const int d16_0 = P.d16[0], d16_1 = P.d16[1], b16_0 = P.b16[0], b16_1 = P.b16[1];
-        const int pdn = P.dn, pbn = P.bn, pd8 = P.d8, pb8 = P.b8;
+        const int pd8 = P.d8, pb8 = P.b8;
         const int m0x = T.mv16[0][0], m0y = T.mv16[0][1], m1x = T.mv16[1][0], m1y = T.mv16[1][1];
         const int s16_0 = T.sad16[0], s16_1 = T.sad16[1], tfl = T.flags;
+        const bool pruned = tfl & 1;          // the proposer expects SKIP: no INTRA, no INTER8
+        if (pruned && L == 0) lds_rel(&X.cancel[mx & 1], mx + 1);
         int q8x[4], q8y[4];
 #pragma unroll
         for (int qd = 0; qd < 4; ++qd) { q8x[qd] = T.mv8[qd][0]; q8y[qd] = T.mv8[qd][1]; }
@@ -1122,21 +1123,14 @@ __device__ __forceinline__ void replay_row(PrWave &W, const fdvg_seq &q, const u
                 tv0[j] = k < 21 * 21 ? rp[(ptrdiff_t)(k / 21) * q.ys + k % 21] : 0;
             }
         }
-        // the row above must have decided the above-right MB (or the last one);
-        // from another workgroup, the chunk fetch above has waited already
-        uint64_t va = 0, vr = 0;
+        // the row above: the motion records of the above and above-right MBs
+        // (or the last), and the bottom row of the above-right MB (the next
+        // MB's trial)
+        uint64_t va = 0, vr = 0, nb = 0;
         if (my > 0) {
-            if (!above_g) {
-                const int need = (my - 1) * 65536 + (mx + 2 < mw ? mx + 2 : mw);
-                int v = __hip_atomic_load(up, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
-                for (unsigned spins = 0; v < need; ++spins) {
-                    if (spins > (1u << 24)) { if (L == 0) atomicOr(ctl + 1, 1u); break; }
-                    __builtin_amdgcn_s_sleep(1);
-                    v = __hip_atomic_load(up, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
-                }
-            }
-            va = above[mx];
-            vr = mx + 1 < mw ? above[mx + 1] : 0;
+            nb = above(mx + 1 < mw ? mx + 1 : -1, mx, mx + 1 < mw ? mx + 1 : mw - 1);
+            va = __shfl(nb, 4);
+            vr = mx + 1 < mw ? __shfl(nb, 5) : 0;
         }
         // merge list: predictor, then the inter MBs above and above-right (fdv.h:5711-5732)
         int cx0 = pmx, cy0 = pmy, cx1 = 0, cy1 = 0, cr1 = 0, cx2 = 0, cy2 = 0, cr2 = 0, ncand = 1;
@@ -1152,6 +1146,7 @@ __device__ __forceinline__ void replay_row(PrWave &W, const fdvg_seq &q, const u
         }
         // SKIP SSDs: from the precomputed set, else from a 21x21 tile of the reference
         int ssd[3] = {0, 0, 0};
+        int tile_c = -1;                      // the candidate whose tile W.tile holds
         const uint8_t *curb = W.cur[mx % PR_CHUNK];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
@@ -1183,17 +1178,32 @@ __device__ __forceinline__ void replay_row(PrWave &W, const fdvg_seq &q, const u
                 }
                 wave_sync();
                 ssd[c] = wave_sum(ssd_tile_q<PR_TS>(W.tile, curb, ((y & 3) << 2) | (x & 3), L));
+                tile_c = c;
                 wave_sync();
             }
         }
         int Ds = ssd[0], bmi = 0;
         if (ncand > 1 && ssd[1] < Ds) { Ds = ssd[1]; bmi = 1; }
         if (ncand > 2 && ssd[2] < Ds) { Ds = ssd[2]; bmi = 2; }
+        const int sx = bmi == 0 ? cx0 : bmi == 1 ? cx1 : cx2, sy = bmi == 0 ? cy0 : bmi == 1 ? cy1 : cy2;
+        const int sref = bmi == 0 ? 0 : bmi == 1 ? cr1 : cr2;
+        // the SKIP winner's tile, for its edges should SKIP be chosen: loads
+        // issued now, they complete during the INTRA wait
+        const bool need_tile = tile_c != bmi;
+        uint8_t tvs[7];
+        if (need_tile) {
+            const gmem_t<uint8_t> *rp = gmem(sref ? q.ry[1] : q.ry[0]) +
+                                        (ptrdiff_t)(by + (sy >> 2) - 2) * q.ys + bx + (sx >> 2) - 2;
+#pragma unroll
+            for (int j = 0; j < 7; ++j) {
+                const int k = L + 64 * j;
+                tvs[j] = k < 21 * 21 ? rp[(ptrdiff_t)(k / 21) * q.ys + k % 21] : 0;
+            }
+        }
         // the reference's gates and commit on the stored costs
         const double Js = (double)Ds + lambda * (double)(3 + (ncand > 1 ? 3 : 0));
         double Jbest = Js;
         const bool skip_wins = Js <= lambda * (double)FL_INTRA;
-        const bool pruned = tfl & 1;          // the proposer expects SKIP: no INTRA, no INTER8
         double Ji = 1e30;
         int iref = 0, sad16 = 0;
         if (!skip_wins) {
@@ -1210,8 +1220,14 @@ __device__ __forceinline__ void replay_row(PrWave &W, const fdvg_seq &q, const u
         }
         double Jn = 1e30;
         if (!skip_wins && !pruned && Jbest > lambda * (double)FL_INTRA) {
-            Jn = (double)pdn + lambda * (double)pbn;
+            // the trial of this MB: its four region costs
+            xr_wait_done(X, mx + 1, ctl, L);
+            const int dn = X.ws.e[0] + X.ws.e[1] + X.ws.e[2] + X.ws.e[3];
+            const int bn = 6 + X.ws.b[0] + X.ws.b[1] + X.ws.b[2] + X.ws.b[3];
+            Jn = (double)dn + lambda * (double)bn;
             if (Jn < Jbest) Jbest = Jn;
+        } else if (L == 0) {
+            lds_rel(&X.cancel[mx & 1], mx + 1);
         }
         double J8 = 1e30;
         if (!skip_wins && !pruned && Jbest > lambda * (double)FL_INTER8) {
@@ -1231,15 +1247,13 @@ __device__ __forceinline__ void replay_row(PrWave &W, const fdvg_seq &q, const u
         if (Jn < Jm) { Jm = Jn; mode = 2; }
         if (J8 < Jm) { Jm = J8; mode = 3; }
         int nx = 0, ny = 0, nr = 0, ni = 1;
-        if (mode == 0) {
-            nx = bmi == 0 ? cx0 : bmi == 1 ? cx1 : cx2; ny = bmi == 0 ? cy0 : bmi == 1 ? cy1 : cy2;
-            nr = bmi == 0 ? 0 : bmi == 1 ? cr1 : cr2;
-        } else if (mode == 1) { nx = iref ? m1x : m0x; ny = iref ? m1y : m0y; nr = iref; }
+        if (mode == 0) { nx = sx; ny = sy; nr = sref; }
+        else if (mode == 1) { nx = iref ? m1x : m0x; ny = iref ? m1y : m0y; nr = iref; }
         else if (mode == 2) ni = 0;
         else { nx = q8x[3]; ny = q8y[3]; }
-        if constexpr (FUSED) {
-            // the coding workers' copy of the decision and its context:
-            // ten tagged words (lane k: 32-bit payload word k), no fence
+        // the decision: the coding workers' copy (ten tagged words, lane k:
+        // payload word k), the motion record for the row below, given / pctx
+        {
             const uint32_t mvw0 = mode == 3 ? pk16(q8x[0], q8y[0]) : mode <= 1 ? pk16(nx, ny) : 0u;
             uint32_t d = 0;
             switch (L) {
@@ -1256,15 +1270,43 @@ __device__ __forceinline__ void replay_row(PrWave &W, const fdvg_seq &q, const u
             case 9: d = (uint32_t)cr1 << 8 | (uint32_t)cr2 << 16 | (uint32_t)ncand << 24; break;
             default: break;
             }
-            if (L == 0) __hip_atomic_store(mrec_g + mb, mrec_pack(tagv, ni, nr, nx, ny), __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_AGENT);
-            if (L < PD_WORDS) __hip_atomic_store(decw + (size_t)mb * PD_WORDS + L, (uint64_t)q.step << 32 | d,
+            if (L < PD_WORDS) __hip_atomic_store(decw + (size_t)mb * PD_WORDS + L, (uint64_t)step << 32 | d,
                                                  __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
+        // the chosen reconstruction's bottom row (lanes 0-15) and right column
+        // (lanes 16-31)
+        if (mode == 0) {
+            if (need_tile) {
+#pragma unroll
+                for (int j = 0; j < 7; ++j) {
+                    const int k = L + 64 * j;
+                    if (k < 21 * 21) W.tile[(k / 21) * PR_TS + k % 21] = tvs[j];
+                }
+                wave_sync();
+            }
+        }
+        int e = 0;
+        if (L < 32) {
+            const int ex = L < 16 ? L : 15, ey = L < 16 ? 15 : L - 16;
+            if (mode == 0) e = lumaq_at<PR_TS>(W.tile, 2 + ex, 2 + ey, ((sy & 3) << 2) | (sx & 3));
+            else if (mode == 2) e = X.irec[ey * 16 + ex];
+            else {
+                const int base = mode == 1 ? 32 * iref : 64;
+                e = (int)((W.ew[mx % PR_CHUNK][(base + L) >> 2] >> (8 * ((base + L) & 3))) & 0xffu);
+            }
+        }
+        // bottom row to the row below, then the motion record
+        {
+            const int b0 = __shfl(e, 4 * (L & 3)), b1 = __shfl(e, 4 * (L & 3) + 1);
+            const int b2 = __shfl(e, 4 * (L & 3) + 2), b3 = __shfl(e, 4 * (L & 3) + 3);
+            if (L < BW_WORDS)
+                __hip_atomic_store(botw + (size_t)mb * BW_WORDS + L,
+                                   (uint64_t)step << 32 | (uint32_t)(b0 | b1 << 8 | b2 << 16 | b3 << 24),
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (L == 0) __hip_atomic_store(mrec + mb, mrec_pack(tagv, ni, nr, nx, ny), __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+        }
         if (L == 0) {
-            // the row below needs only the motion record: publish it first
-            cur_slot[mx] = mrec_pack(0, ni, nr, nx, ny);
-            __hip_atomic_store(myprog, my * 65536 + mx + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
             fdvg_mbinfo G;
             G.mode = (uint8_t)mode; G.ref = (uint8_t)(mode == 0 || mode == 1 ? nr : 0);
             G.merge_idx = (uint8_t)(mode == 0 ? bmi : 0); G.sub = (uint8_t)(mode == 2 ? FDVG_SUB_SCREEN : 0);
@@ -1272,35 +1314,100 @@ __device__ __forceinline__ void replay_row(PrWave &W, const fdvg_seq &q, const u
             if (mode == 0 || mode == 1) { G.mv[0][0] = (int16_t)nx; G.mv[0][1] = (int16_t)ny; }
             if (mode == 3) for (int k = 0; k < 4; ++k) { G.mv[k][0] = (int16_t)q8x[k]; G.mv[k][1] = (int16_t)q8y[k]; }
             given[mb] = G;
-            fdvg_pctx X;
-            X.pmx = (int16_t)pmx; X.pmy = (int16_t)pmy; X.ncand = (uint8_t)ncand;
-            X.cx[0] = (int16_t)cx0; X.cy[0] = (int16_t)cy0; X.cr[0] = 0;
-            X.cx[1] = (int16_t)cx1; X.cy[1] = (int16_t)cy1; X.cr[1] = (uint8_t)cr1;
-            X.cx[2] = (int16_t)cx2; X.cy[2] = (int16_t)cy2; X.cr[2] = (uint8_t)cr2;
-            pctx[mb] = X;
+            fdvg_pctx Xc;
+            Xc.pmx = (int16_t)pmx; Xc.pmy = (int16_t)pmy; Xc.ncand = (uint8_t)ncand;
+            Xc.cx[0] = (int16_t)cx0; Xc.cy[0] = (int16_t)cy0; Xc.cr[0] = 0;
+            Xc.cx[1] = (int16_t)cx1; Xc.cy[1] = (int16_t)cy1; Xc.cr[1] = (uint8_t)cr1;
+            Xc.cx[2] = (int16_t)cx2; Xc.cy[2] = (int16_t)cy2; Xc.cr[2] = (uint8_t)cr2;
+            pctx[mb] = Xc;
+        }
+        // the next MB's trial inputs, into the buffer of MB mx - 1's parity once
+        // the trial has finished that MB
+        if (mx + 1 < mw) {
+            const int p = (mx + 1) & 1;
+            xr_wait_done(X, mx, ctl, L);
+            if (L >= 16 && L < 32) X.ileft[p][L - 16] = (uint8_t)e;
+            if (L < 4) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) X.itop[p][1 + 4 * L + j] = (uint8_t)(nb >> (8 * j));
+            }
+            if (L == 0) X.itop[p][0] = X.itop[mx & 1][16];
+            wave_sync();
+            if (L == 0) lds_rel(&X.go, mx + 2);
         }
         if (mode != 2) { pmx = nx; pmy = ny; }
     }
 }
 
-extern "C" __global__ void __launch_bounds__(512)
-k_prepair(const fdvg_seq *__restrict__ seqs, int mwmax, unsigned *ctl, const uint64_t *__restrict__ pcw,
-          fdvg_mbinfo *__restrict__ given, fdvg_pctx *__restrict__ pctx) {
-    extern __shared__ __align__(16) uint64_t dyn[];
-    const int NW = blockDim.x >> 6, wv = threadIdx.x >> 6, L = threadIdx.x & 63;
-    uint64_t *ring = dyn;                                    // NW slots x mwmax motion records
-    int *prog = (int *)(ring + (size_t)NW * mwmax);          // NW progress words
-    // offset from the LDS base, not an integer round trip, so that the
-    // compiler keeps the LDS address space (ds_* rather than flat accesses)
-    PrWave *pw = (PrWave *)((char *)dyn + ((((size_t)NW * mwmax) * 8 + (size_t)NW * 4 + 15) & ~(size_t)15));
-    const fdvg_seq q = seqs[blockIdx.x];
-    const int mh = q.h >> 4;
-    if (L == 0) __hip_atomic_store(&prog[wv], -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+// Waves 1-4 of k_pxrep: the INTRA trial of every MB of row my, region r of
+// each (fdv.h:5950-5988: the SATD screen over DC / V / H / PLANE, then one
+// coding of the best), unless wave 0 cancels it.
+__device__ __forceinline__ void xr_trial(XrLds &X, const fdvg_seq &q, unsigned *ctl, int my, int r, int L) {
+    const int mw = q.w >> 4;
+    uint8_t *cur = X.icur[r];
+    const gmem_t<uint32_t> *src = (const gmem_t<uint32_t> *)gmem(q.cy);
+    const size_t srow = (size_t)(my * 16 + (L >> 2)) * (q.w >> 2) + (L & 3);
+    uint32_t cnext = src[srow];
+    const int ox = (r & 1) * 8, oy = (r >> 1) * 8, o = oy * 16 + ox;
+    for (int mx = 0; mx < mw; ++mx) {
+        const uint32_t cv = cnext;
+        if (mx + 1 < mw) cnext = src[srow + 4 * (mx + 1)];
+        *(uint32_t *)&cur[(L >> 2) * 16 + (L & 3) * 4] = cv;
+        int g = lds_acq(&X.go);
+        for (unsigned spins = 0; g <= mx; ++spins) {
+            if (xr_give_up(spins, 1u << 24, ctl)) break;
+            __builtin_amdgcn_s_sleep(1);
+            g = lds_acq(&X.go);
+        }
+        const int p = mx & 1;
+        if (lds_acq(&X.cancel[p]) != mx + 1) {
+            wave_sync();
+            const uint8_t *top = &X.itop[p][1], *left = X.ileft[p];
+            const Nb nbq = nb_setup(top, left, X.itop[p][0], 16, my > 0, mx > 0);
+            int sat[4];
+            satd_modes(cur, nbq, top, left, sat);
+            int sel = -1, sbest = 0;
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                if (!nn_ok(m, nbq)) continue;
+                if (sel < 0 || sat[m] < sbest) { sbest = sat[m]; sel = m; }
+            }
+            if (lds_acq(&X.cancel[p]) != mx + 1) {
+                const int y = oy + (L >> 3), x = ox + (L & 7);
+                X.ipred[y * 16 + x] = (uint8_t)ipred(sel, nbq, top, left, 16, x, y);
+                wave_sync();
+                region_code<0>(X.ws, r, cur + o, 16, X.ipred + o, 16, X.irec + o, 16, X.ilev, X.iflag, q.qp,
+                               q.lambda);
+            }
+        }
+        wave_sync();
+        if (L == 0) lds_rel(&X.done[r], mx + 1);
+    }
+}
+
+// The exact replay: one workgroup a MB row, taken by ticket (ctl[6]) in the
+// order of `units` (row by row, every sequence's row my before any row my + 1),
+// so the row above has always started.  Every wait is bounded (error flag).
+extern "C" __global__ void __launch_bounds__(64 * XR_WAVES)
+k_pxrep(const fdvg_seq *__restrict__ seqs, const int2 *__restrict__ units, int nunit, unsigned *ctl,
+        const uint64_t *__restrict__ pcw, const uint64_t *__restrict__ edgew, fdvg_mbinfo *__restrict__ given,
+        fdvg_pctx *__restrict__ pctx, uint64_t *__restrict__ mrec, uint64_t *__restrict__ botw,
+        uint64_t *__restrict__ decw) {
+    __shared__ XrLds X;
+    __shared__ unsigned role_s;
+    const int tid = threadIdx.x, wv = tid >> 6, L = tid & 63;
+    if (tid == 0) {
+        role_s = atomicAdd(ctl + 6, 1u);
+        X.go = 0; X.cancel[0] = X.cancel[1] = 0;
+    }
+    if (tid < 4) X.done[tid] = 0;
     __syncthreads();
-    for (int my = wv; my < mh; my += NW)
-        replay_row<false>(pw[wv], q, pcw, ctl, my, ring + (size_t)(my % NW) * mwmax, &prog[wv],
-                          ring + (size_t)((my + NW - 1) % NW) * mwmax, &prog[(wv + NW - 1) % NW], nullptr, nullptr,
-                          nullptr, given, pctx, L);
+    const unsigned role = (unsigned)__builtin_amdgcn_readfirstlane((int)role_s);
+    if (role >= (unsigned)nunit) return;
+    const int2 u = units[role];
+    const fdvg_seq q = seqs[u.x];
+    if (wv == 0) xr_decide(X, q, pcw, edgew, ctl, u.y, mrec, botw, decw, given, pctx, L);
+    else xr_trial(X, q, ctl, u.y, wv - 1, L);
 }
 
 // The intra MBs of the batch ordered by anti-diagonal (mx + my), all sequences
@@ -1382,11 +1489,6 @@ k_plist(const fdvg_seq *__restrict__ seqs, int nseq, const fdvg_mbinfo *__restri
         __syncthreads();
     }
     if (t == 0) ilist[0] = base;
-}
-
-// LDS bytes of k_prepair for NW waves and rows up to mwmax MBs wide.
-static size_t prepair_lds(int nw, int mwmax) {
-    return ((size_t)nw * mwmax * 8 + (size_t)nw * 4 + 15) / 16 * 16 + (size_t)nw * sizeof(PrWave);
 }
 
 __device__ __forceinline__ void ctx_from(MbCtx &c, const fdvg_pctx &X, int ht, int hl) {
@@ -1500,67 +1602,24 @@ k_pintra(const fdvg_seq *__restrict__ seqs, int nseq, const fdvg_mbinfo *__restr
     }
 }
 
-// The replay and the coding of its decisions in one launch (k_pdecide).  A
-// workgroup's role is its ticket, taken when it starts, so a lower ticket has
-// always started already:
-// - tickets 0 .. nunit - 1 replay: unit (seq, g) replays rows 4g .. 4g + 3 of its
-//   sequence, one wave a row; the first row reads the row above from the
-//   previous unit of its sequence (a lower ticket);
-// - the other workgroups are coding workers: they take MBs in the order of
-//   `corder` (by mx + 2 my, the order in which the replay decides them), wait for
-//   the MB's decision words, code it as k_pencode / k_pintra do, and publish its
-//   reconstruction's bottom row and right column as handoff words.  An intra MB
-//   waits for the handoff words of its left, top and top-left neighbours, all
-//   earlier in that order.
-// Every wait is on work of a lower ticket or an earlier job, so none can wait on
-// a workgroup that has not started; every wait is bounded and sets the error
-// flag on timeout.  Rows at most PD_MAXMW MBs wide.  The replays and the
-// workers may also be two launches: one of the nunit replays only (grid nunit),
-// then one of workers only (nunit 0), which take no tickets.
-#define PD_MAXMW 256
-#define PD_ROWS 4
-struct __align__(16) PdRep {
-    uint64_t ring[PD_ROWS][PD_MAXMW];
-    uint64_t above0[PD_MAXMW];            // the first row's row above, from the previous unit
-    int prog[PD_ROWS];
-    PrWave pw[PD_ROWS];
-};
-union PdLds {
-    Smem s;
-    PdRep r;
-};
-
+// The coding workers of the decisions (k_pdecide): they take MBs in the order
+// of `corder` (by mx + 2 my, the order in which the replays decide them), wait
+// for the MB's decision words, code it as k_pencode / k_pintra do, and publish
+// its reconstruction's bottom row and right column as handoff words.  An intra
+// MB waits for the handoff words of its left, top and top-left neighbours, all
+// earlier in that order.  Every wait is on a decision (made by a replay, which
+// never waits for the workers) or on an earlier job, and is bounded (error flag
+// on timeout).
 extern "C" __global__ void __launch_bounds__(256)
-k_pdecide(const fdvg_seq *__restrict__ seqs, int nseq, const int2 *__restrict__ units, int nunit,
-          const unsigned *__restrict__ corder, int total_mb, unsigned *ctl, const uint64_t *__restrict__ pcw,
-          fdvg_mbinfo *__restrict__ given, fdvg_pctx *__restrict__ pctx, uint64_t *__restrict__ mrec,
-          uint64_t *__restrict__ decw, uint64_t *__restrict__ ih, fdvg_mbrec *__restrict__ recs) {
-    __shared__ PdLds U;
-    __shared__ unsigned role_s;
+k_pdecide(const fdvg_seq *__restrict__ seqs, int nseq, const unsigned *__restrict__ corder, int total_mb,
+          unsigned *ctl, const uint64_t *__restrict__ decw, uint64_t *__restrict__ ih, fdvg_mbrec *__restrict__ recs) {
+    __shared__ Smem s;
     __shared__ union {
         uint32_t w[PD_WORDS];
         struct { fdvg_mbinfo g; fdvg_pctx x; } d;
     } decb;
-    const int tid = threadIdx.x, wv = tid >> 6, L = tid & 63;
-    // nunit 0: a launch of coding workers only, which takes no ticket
-    if (tid == 0) role_s = nunit > 0 ? atomicAdd(ctl + 6, 1u) : 0u;
-    __syncthreads();
-    const unsigned role = (unsigned)__builtin_amdgcn_readfirstlane((int)role_s);
-    if (role < (unsigned)nunit) {
-        const int2 u = units[role];
-        const fdvg_seq q = seqs[u.x];
-        const int mw = q.w >> 4, mh = q.h >> 4, my = PD_ROWS * u.y + wv;
-        if (L == 0) __hip_atomic_store(&U.r.prog[wv], -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        __syncthreads();
-        if (my >= mh) return;
-        const bool first = wv == 0;
-        replay_row<true>(U.r.pw[wv], q, pcw, ctl, my, U.r.ring[wv], &U.r.prog[wv],
-                         first ? U.r.above0 : U.r.ring[wv - 1], first ? nullptr : &U.r.prog[wv - 1],
-                         first && my > 0 ? mrec + q.mb0 + (size_t)(my - 1) * mw : nullptr, mrec, decw, given, pctx, L);
-        return;
-    }
+    const int tid = threadIdx.x;
     __shared__ int job_s;
-    Smem &s = U.s;
     for (;;) {
         if (tid == 0) job_s = (int)atomicAdd(ctl + 7, 1u);
         __syncthreads();
@@ -1604,30 +1663,24 @@ static_assert(sizeof(fdvg_mbinfo) == 20 && sizeof(fdvg_pctx) == 20 && alignof(fd
               "decision words: 5 + 5 payload words");
 
 // The block-parallel order of a batch: corder[] = every MB, ordered by
-// mx + 2 my (then sequence, then row), the order in which the replay decides
-// them (k_pcosts costs them in this order, k_pdecide codes them in it); and
-// k_pdecide's replay units: units[] = (seq, g) for every PD_ROWS-row group,
-// ordered by g then seq (a unit's row above is in a lower-numbered unit).
-// Returns the unit count; 0 if a row is wider than PD_MAXMW MBs (no units:
-// k_pdecide does not take the batch; corder is complete); -1 on inconsistent
-// sizes.
+// mx + 2 my (then sequence, then row), the order in which the replays decide
+// them (k_pcosts costs them in this order, the coding workers code them in it);
+// and the replay units: units[] = (seq, row) for every MB row, ordered by row
+// then sequence (a row's row above is in a lower-numbered unit).  Returns the
+// unit count, or -1 on inconsistent sizes.
 extern "C" int fdvg_pdecide_plan(const int *w, const int *h, const int *mb0, int nseq, int2 *units,
                                  unsigned *corder) {
-    int nunit = 0, maxg = 0, maxt = 0, total = 0;
-    bool wide = false;
+    int nunit = 0, maxh = 0, maxt = 0, total = 0;
     for (int k = 0; k < nseq; ++k) {
         const int mw = w[k] >> 4, mh = h[k] >> 4;
         if (mw <= 0 || mh <= 0 || mb0[k] != total) return -1;
-        if (mw > PD_MAXMW) wide = true;
-        const int ng = (mh + PD_ROWS - 1) / PD_ROWS;
-        if (ng > maxg) maxg = ng;
+        if (mh > maxh) maxh = mh;
         if (mw - 1 + 2 * (mh - 1) > maxt) maxt = mw - 1 + 2 * (mh - 1);
         total += mw * mh;
     }
-    if (!wide)
-        for (int gi = 0; gi < maxg; ++gi)
-            for (int k = 0; k < nseq; ++k)
-                if (gi * PD_ROWS < (h[k] >> 4)) units[nunit++] = make_int2(k, gi);
+    for (int my = 0; my < maxh; ++my)
+        for (int k = 0; k < nseq; ++k)
+            if (my < (h[k] >> 4)) units[nunit++] = make_int2(k, my);
     int n = 0;
     for (int t = 0; t <= maxt; ++t)
         for (int k = 0; k < nseq; ++k) {
@@ -1640,9 +1693,11 @@ extern "C" int fdvg_pdecide_plan(const int *w, const int *h, const int *mb0, int
     return n == total ? nunit : -1;
 }
 
-// Workgroups of k_pdecide: the replays and workers_per_cu coding workers a CU
-// (at most what a CU holds); with workers_per_cu < 0, workers only, as many as
-// leave room on the device for the nunit replays of another launch.
+// Coding workers of k_pdecide: workers_per_cu a CU (at most what a CU holds);
+// with workers_per_cu < 0, for a launch beside the nunit replays of k_pxrep:
+// ncu - nunit workers, so that however the workers are placed (one a CU at
+// worst) every replay not yet resident finds a CU with no worker and no
+// other replay on it; 0 when that leaves fewer than a quarter of the CUs.
 extern "C" int fdvg_pdecide_grid(int nunit, int total_mb, int workers_per_cu) {
     static int ncu = 0, per = 0;
     if (!ncu) {
@@ -1652,20 +1707,26 @@ extern "C" int fdvg_pdecide_grid(int nunit, int total_mb, int workers_per_cu) {
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void *)k_pdecide, 256, 0) != hipSuccess) return -1;
     }
     if (workers_per_cu < 0) {
-        const int wk = ncu * per - nunit;
-        return wk < 1 ? 1 : wk < total_mb ? wk : total_mb;
+        const int wk = ncu - nunit;
+        return wk < ncu / 4 ? 0 : wk < total_mb ? wk : total_mb;
     }
     int wk = ncu * (workers_per_cu < per ? workers_per_cu : per);
     if (wk < 1) wk = 1;
-    return nunit + (wk < total_mb ? wk : total_mb);
+    return wk < total_mb ? wk : total_mb;
 }
 
-extern "C" int fdvg_launch_pdecide(const fdvg_seq *seqs, int nseq, const int2 *units, int nunit, const unsigned *corder,
-                                   int total_mb, int grid, unsigned *ctl, const uint64_t *pcw, fdvg_mbinfo *given,
-                                   fdvg_pctx *pctx, uint64_t *mrec, uint64_t *decw, uint64_t *ih, fdvg_mbrec *recs,
+extern "C" int fdvg_launch_pdecide(const fdvg_seq *seqs, int nseq, const unsigned *corder, int total_mb, int grid,
+                                   unsigned *ctl, const uint64_t *decw, uint64_t *ih, fdvg_mbrec *recs,
                                    hipStream_t st) {
-    hipLaunchKernelGGL(k_pdecide, dim3(grid), dim3(256), 0, st, seqs, nseq, units, nunit, corder, total_mb, ctl, pcw,
-                       given, pctx, mrec, decw, ih, recs);
+    hipLaunchKernelGGL(k_pdecide, dim3(grid), dim3(256), 0, st, seqs, nseq, corder, total_mb, ctl, decw, ih, recs);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+extern "C" int fdvg_launch_pxrep(const fdvg_seq *seqs, const int2 *units, int nunit, unsigned *ctl,
+                                 const uint64_t *pcw, const uint64_t *edgew, fdvg_mbinfo *given, fdvg_pctx *pctx,
+                                 uint64_t *mrec, uint64_t *botw, uint64_t *decw, hipStream_t st) {
+    hipLaunchKernelGGL(k_pxrep, dim3(nunit), dim3(64 * XR_WAVES), 0, st, seqs, units, nunit, ctl, pcw, edgew, given,
+                       pctx, mrec, botw, decw);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -1680,16 +1741,6 @@ extern "C" int fdvg_launch_plist(const fdvg_seq *seqs, int nseq, const fdvg_mbin
 extern "C" int fdvg_launch_pcosts(const fdvg_seq *seqs, int nseq, const unsigned *corder, int total_mb, unsigned *ctl,
                                   uint64_t *pcw, uint64_t *edgew, hipStream_t st) {
     hipLaunchKernelGGL(k_pcosts, dim3(total_mb), dim3(256), 0, st, seqs, nseq, corder, total_mb, ctl, pcw, edgew);
-    return hipGetLastError() == hipSuccess ? 0 : -1;
-}
-
-extern "C" int fdvg_launch_prepair(const fdvg_seq *seqs, int nseq, int maxmh, int mwmax, unsigned *ctl,
-                                   const uint64_t *pcw, fdvg_mbinfo *given, fdvg_pctx *pctx, hipStream_t st) {
-    int nw = maxmh < 8 ? maxmh : 8;
-    size_t lds = prepair_lds(nw, mwmax);
-    while (nw > 1 && lds > 64 * 1024) lds = prepair_lds(--nw, mwmax);
-    if (lds > 64 * 1024) return -1;
-    hipLaunchKernelGGL(k_prepair, dim3(nseq), dim3(64 * nw), lds, st, seqs, mwmax, ctl, pcw, given, pctx);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -1713,13 +1764,14 @@ extern "C" int fdvg_launch_pfinish(const fdvg_seq *seqs, int nseq, int xgrid, un
 }
 
 // The decision stage as separate launches on one stream: the costs, the
-// replay, the coding of the decisions.
-extern "C" int fdvg_launch_parallel(const fdvg_seq *seqs, int nseq, int xgrid, int maxmh, int mwmax,
+// replays, the coding of the decisions.
+extern "C" int fdvg_launch_parallel(const fdvg_seq *seqs, int nseq, int xgrid, const int2 *units, int nunit,
                                     const unsigned *corder, unsigned *ctl, uint64_t *pcw, uint64_t *edgew,
-                                    fdvg_mbinfo *given, fdvg_pctx *pctx, unsigned *ilist, uint64_t *ih, int total_mb,
-                                    fdvg_mbrec *recs, hipStream_t st) {
+                                    fdvg_mbinfo *given, fdvg_pctx *pctx, uint64_t *mrec, uint64_t *botw,
+                                    uint64_t *decw, unsigned *ilist, uint64_t *ih, int total_mb, fdvg_mbrec *recs,
+                                    hipStream_t st) {
     if (fdvg_launch_pcosts(seqs, nseq, corder, total_mb, ctl, pcw, edgew, st) != 0) return -1;
-    if (fdvg_launch_prepair(seqs, nseq, maxmh, mwmax, ctl, pcw, given, pctx, st) != 0) return -1;
+    if (fdvg_launch_pxrep(seqs, units, nunit, ctl, pcw, edgew, given, pctx, mrec, botw, decw, st) != 0) return -1;
     return fdvg_launch_pfinish(seqs, nseq, xgrid, ctl, given, pctx, ilist, ih, total_mb, recs, st);
 }
 

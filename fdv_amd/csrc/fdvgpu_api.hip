@@ -46,18 +46,20 @@ extern "C" int fdvg_nn_setup(void);
 // The proposer's confidence threshold: below it an MB gets the exhaustive search.
 #define FDVG_NN_THRESHOLD 0.7f     // default confidence threshold of the proposer
 #define FDVG_NN_SKIP_OFF 2.0f      // default SKIP threshold: no pruning
-extern "C" int fdvg_launch_parallel(const fdvg_seq *, int, int, int, int, const unsigned *, unsigned *, uint64_t *,
-                                    uint64_t *, fdvg_mbinfo *, fdvg_pctx *, unsigned *, uint64_t *, int, fdvg_mbrec *,
-                                    hipStream_t);
+extern "C" int fdvg_launch_parallel(const fdvg_seq *, int, int, const int2 *, int, const unsigned *, unsigned *,
+                                    uint64_t *, uint64_t *, fdvg_mbinfo *, fdvg_pctx *, uint64_t *, uint64_t *,
+                                    uint64_t *, unsigned *, uint64_t *, int, fdvg_mbrec *, hipStream_t);
 extern "C" int fdvg_launch_pcosts(const fdvg_seq *, int, const unsigned *, int, unsigned *, uint64_t *, uint64_t *,
                                   hipStream_t);
+extern "C" int fdvg_launch_pxrep(const fdvg_seq *, const int2 *, int, unsigned *, const uint64_t *, const uint64_t *,
+                                 fdvg_mbinfo *, fdvg_pctx *, uint64_t *, uint64_t *, uint64_t *, hipStream_t);
 extern "C" int fdvg_pdecide_plan(const int *, const int *, const int *, int, int2 *, unsigned *);
 extern "C" int fdvg_pdecide_grid(int, int, int);
-extern "C" int fdvg_launch_pdecide(const fdvg_seq *, int, const int2 *, int, const unsigned *, int, int, unsigned *,
-                                   const uint64_t *, fdvg_mbinfo *, fdvg_pctx *, uint64_t *, uint64_t *, uint64_t *,
-                                   fdvg_mbrec *, hipStream_t);
-#define PC_WORDS 36                     // a MB's candidate costs as tagged words
-#define EW_WORDS 8                      // a MB's stand-in edges as tagged words
+extern "C" int fdvg_launch_pdecide(const fdvg_seq *, int, const unsigned *, int, int, unsigned *, const uint64_t *,
+                                   uint64_t *, fdvg_mbrec *, hipStream_t);
+#define PC_WORDS 34                     // a MB's candidate costs as tagged words
+#define EW_WORDS 24                     // a MB's candidates' reconstruction edges as tagged words
+#define BW_WORDS 4                      // a decided MB's bottom reconstruction row, tagged
 #define PD_WORDS 10                     // a MB's decision and context as tagged words
 
 static thread_local char g_err[256];
@@ -111,7 +113,8 @@ struct fdvgpu {
     uint8_t *compact_d, *compact_h;
     // block-parallel decisions
     uint64_t *pcw_d;                // per MB: the candidate costs as tagged words
-    uint64_t *edgew_d;              // per MB: the stand-in reconstruction's bottom row and right column, tagged
+    uint64_t *edgew_d;              // per MB: the coded candidates' bottom rows and right columns, tagged
+    uint64_t *botw_d;               // per MB: the decided reconstruction's bottom row, tagged (for the row below)
     unsigned *corder_d;             // the MBs in the order the replay decides them
     fdvg_pctx *pctx_d;
     fdvg_mbinfo *pgiven_d;
@@ -119,14 +122,15 @@ struct fdvgpu {
     unsigned *ilist_d;              // intra MBs in dependency order (count first)
     uint64_t *ih_d;                 // intra handoff words, 16 per MB (bottom row, right column)
     unsigned step_ctr;              // frame-step ids (never rewound)
-    // The fused decision stage (k_pdecide): the replay and the coding of its
-    // decisions in one launch, so the coding follows the replay MB by MB; with
-    // overlap 2 it runs on a second stream beside the costs (k_pcosts), whose
-    // tagged words the replay waits for.
-    int2 *units_d;                  // replay units (sequence, row group)
+    // The decision stage: the costs (k_pcosts), the replays (k_pxrep, one MB
+    // row a workgroup), the coding of the decisions.  Structure 0: separate
+    // launches, the coding by k_pencode / k_pintra; 1: the coding by the
+    // workers of k_pdecide after the replays; 2: the replays on a second stream
+    // beside the costs, the workers beside the replays.
+    int2 *units_d;                  // replay units (sequence, row)
     uint64_t *decw_d;               // per MB: the decision and its context as tagged words
     int nunit;
-    int overlap, overlap_avail;     // 0: separate launches; 1: costs, then the fused launch; 2: side by side
+    int overlap, overlap_avail;     // the decision structure; whether 2 fits beside the replays
     hipStream_t st2;
     hipEvent_t ev_fork, ev_join;
     // launch shapes, and one instantiated graph per (search mode, decision structure)
@@ -196,7 +200,7 @@ extern "C" void fdvgpu_close(fdvgpu *g) {
     if (g->ev_fork) (void)hipEventDestroy(g->ev_fork);
     if (g->ev_join) (void)hipEventDestroy(g->ev_join);
     (void)hipFree(g->units_d); (void)hipFree(g->corder_d); (void)hipFree(g->decw_d);
-    (void)hipFree(g->edgew_d);
+    (void)hipFree(g->edgew_d); (void)hipFree(g->botw_d);
     for (int k = 0; k < 2; ++k) {
         if (g->pipe_up[k]) (void)hipHostFree(g->pipe_up[k]);
         if (g->pipe_sc[k]) (void)hipHostFree(g->pipe_sc[k]);
@@ -281,6 +285,8 @@ extern "C" fdvgpu *fdvgpu_open(int nseq, const int *w, const int *h, int nsrc) {
     CKN(hipMemset(g->pcw_d, 0, (size_t)g->total_mb * PC_WORDS * sizeof(uint64_t)));
     CKN(hipMalloc(&g->edgew_d, (size_t)g->total_mb * EW_WORDS * sizeof(uint64_t)));
     CKN(hipMemset(g->edgew_d, 0, (size_t)g->total_mb * EW_WORDS * sizeof(uint64_t)));
+    CKN(hipMalloc(&g->botw_d, (size_t)g->total_mb * BW_WORDS * sizeof(uint64_t)));
+    CKN(hipMemset(g->botw_d, 0, (size_t)g->total_mb * BW_WORDS * sizeof(uint64_t)));
     CKN(hipMalloc(&g->pctx_d, (size_t)g->total_mb * sizeof(fdvg_pctx)));
     CKN(hipMalloc(&g->pgiven_d, (size_t)g->total_mb * sizeof(fdvg_mbinfo)));
     CKN(hipMalloc(&g->mrec_d, (size_t)g->total_mb * sizeof(uint64_t)));
@@ -289,26 +295,24 @@ extern "C" fdvgpu *fdvgpu_open(int nseq, const int *w, const int *h, int nsrc) {
     CKN(hipMalloc(&g->ih_d, (size_t)g->total_mb * 16 * sizeof(uint64_t)));
     CKN(hipMemset(g->ih_d, 0, (size_t)g->total_mb * 16 * sizeof(uint64_t)));   // id 0: never a step
     {
-        // the block-parallel order, and the fused decision stage's units (rows
-        // up to its width limit; wider batches run the stage as separate launches)
+        // the block-parallel order and the replay units (one a MB row)
         int *wv = (int *)malloc(3 * nseq * sizeof(int));
         int2 *units_h = (int2 *)malloc(((size_t)g->nrows + nseq) * sizeof(int2));
         unsigned *corder_h = (unsigned *)malloc((size_t)g->total_mb * sizeof(unsigned));
         if (!wv || !units_h || !corder_h) { free(wv); free(units_h); free(corder_h); seterr("out of host memory"); goto fail; }
         for (int k = 0; k < nseq; ++k) { wv[k] = g->S[k].w; wv[nseq + k] = g->S[k].h; wv[2 * nseq + k] = g->S[k].mb0; }
         g->nunit = fdvg_pdecide_plan(wv, wv + nseq, wv + 2 * nseq, nseq, units_h, corder_h);
-        g->overlap_avail = g->nunit > 0;
-        g->overlap = g->overlap_avail ? 2 : 0;
-        hipError_t e = g->nunit < 0 ? hipErrorInvalidValue : hipSuccess;
+        // side by side only where the workers leave room for every replay
+        g->overlap_avail = g->nunit > 0 && fdvg_pdecide_grid(g->nunit, g->total_mb, -1) > 0;
+        g->overlap = g->overlap_avail ? 2 : 1;
+        hipError_t e = g->nunit <= 0 ? hipErrorInvalidValue : hipSuccess;
         if (e == hipSuccess) e = hipMalloc(&g->corder_d, (size_t)g->total_mb * sizeof(unsigned));
         if (e == hipSuccess)
             e = hipMemcpy(g->corder_d, corder_h, (size_t)g->total_mb * sizeof(unsigned), hipMemcpyHostToDevice);
-        if (e == hipSuccess && g->overlap_avail) {
-            e = hipMalloc(&g->units_d, (size_t)g->nunit * sizeof(int2));
-            if (e == hipSuccess) e = hipMalloc(&g->decw_d, (size_t)g->total_mb * PD_WORDS * sizeof(uint64_t));
-            if (e == hipSuccess) e = hipMemset(g->decw_d, 0, (size_t)g->total_mb * PD_WORDS * sizeof(uint64_t));
-            if (e == hipSuccess) e = hipMemcpy(g->units_d, units_h, (size_t)g->nunit * sizeof(int2), hipMemcpyHostToDevice);
-        }
+        if (e == hipSuccess) e = hipMalloc(&g->units_d, (size_t)g->nunit * sizeof(int2));
+        if (e == hipSuccess) e = hipMalloc(&g->decw_d, (size_t)g->total_mb * PD_WORDS * sizeof(uint64_t));
+        if (e == hipSuccess) e = hipMemset(g->decw_d, 0, (size_t)g->total_mb * PD_WORDS * sizeof(uint64_t));
+        if (e == hipSuccess) e = hipMemcpy(g->units_d, units_h, (size_t)g->nunit * sizeof(int2), hipMemcpyHostToDevice);
         free(wv); free(units_h); free(corder_h);
         if (e != hipSuccess) { seterr("block-parallel plan: %s", hipGetErrorString(e)); goto fail; }
     }
@@ -426,43 +430,42 @@ static int launch_stage(fdvgpu *g, int stage, int me_mode, bool par) {
         if (par && g->overlap == 2) {
             // the costs on this stream, the replays beside them on the second
             // one (they wait for each chunk's cost words), then, after the
-            // costs, the coding workers on this stream.  The costs go first:
-            // where the two streams share a hardware queue all three run one
-            // after the other, correctly.  Side by side, the costs never wait
-            // for the replays, and the workers leave the device room for them.
+            // costs, the coding workers on this stream beside the replays.  The
+            // costs go first: where the two streams share a hardware queue the
+            // launches simply run one after the other.  The costs never wait;
+            // the workers run once the costs are done, and are few enough that
+            // every replay still finds a CU of its own.
             const int wgrid = fdvg_pdecide_grid(g->nunit, g->total_mb, -1);
             if (wgrid <= 0) { seterr("parallel analysis: no launch shape"); return -1; }
             CK(hipEventRecord(g->ev_fork, g->st));
             CK(hipStreamWaitEvent(g->st2, g->ev_fork, 0));
             if (fdvg_launch_pcosts(g->seq_d, g->nseq, g->corder_d, g->total_mb, g->ticket_d, g->pcw_d, g->edgew_d,
                                    g->st) != 0 ||
-                fdvg_launch_pdecide(g->seq_d, g->nseq, g->units_d, g->nunit, g->corder_d, g->total_mb, g->nunit,
-                                    g->ticket_d, g->pcw_d, g->pgiven_d, g->pctx_d, g->mrec_d, g->decw_d, g->ih_d,
-                                    g->recs_d, g->st2) != 0 ||
-                fdvg_launch_pdecide(g->seq_d, g->nseq, g->units_d, 0, g->corder_d, g->total_mb, wgrid, g->ticket_d,
-                                    g->pcw_d, g->pgiven_d, g->pctx_d, g->mrec_d, g->decw_d, g->ih_d, g->recs_d,
-                                    g->st) != 0) {
+                fdvg_launch_pxrep(g->seq_d, g->units_d, g->nunit, g->ticket_d, g->pcw_d, g->edgew_d, g->pgiven_d,
+                                  g->pctx_d, g->mrec_d, g->botw_d, g->decw_d, g->st2) != 0 ||
+                fdvg_launch_pdecide(g->seq_d, g->nseq, g->corder_d, g->total_mb, wgrid, g->ticket_d, g->decw_d,
+                                    g->ih_d, g->recs_d, g->st) != 0) {
                 seterr("parallel analysis launch failed: %s", hipGetErrorString(hipGetLastError())); return -1;
             }
             CK(hipEventRecord(g->ev_join, g->st2));
             CK(hipStreamWaitEvent(g->st, g->ev_join, 0));
         } else if (par && g->overlap == 1) {
-            // the costs, then the replay and the coding of its decisions in
-            // one launch (the coding follows the replay MB by MB)
+            // the costs, the replays, then the coding workers, one stream
             const int dgrid = fdvg_pdecide_grid(g->nunit, g->total_mb, g->tune.pd_wpc);
             if (dgrid <= 0 ||
                 fdvg_launch_pcosts(g->seq_d, g->nseq, g->corder_d, g->total_mb, g->ticket_d, g->pcw_d, g->edgew_d,
                                    g->st) != 0 ||
-                fdvg_launch_pdecide(g->seq_d, g->nseq, g->units_d, g->nunit, g->corder_d, g->total_mb, dgrid,
-                                    g->ticket_d, g->pcw_d, g->pgiven_d, g->pctx_d, g->mrec_d, g->decw_d, g->ih_d,
-                                    g->recs_d, g->st) != 0) {
+                fdvg_launch_pxrep(g->seq_d, g->units_d, g->nunit, g->ticket_d, g->pcw_d, g->edgew_d, g->pgiven_d,
+                                  g->pctx_d, g->mrec_d, g->botw_d, g->decw_d, g->st) != 0 ||
+                fdvg_launch_pdecide(g->seq_d, g->nseq, g->corder_d, g->total_mb, dgrid, g->ticket_d, g->decw_d,
+                                    g->ih_d, g->recs_d, g->st) != 0) {
                 seterr("parallel analysis launch failed: %s", hipGetErrorString(hipGetLastError())); return -1;
             }
         } else if (par) {
             if (fdvg_launch_zero(g->ilist_d, 1, g->st) != 0 ||
-                fdvg_launch_parallel(g->seq_d, g->nseq, g->xgrid, g->maxmh, g->maxmw, g->corder_d, g->ticket_d,
-                                     g->pcw_d, g->edgew_d, g->pgiven_d, g->pctx_d, g->ilist_d, g->ih_d, g->total_mb,
-                                     g->recs_d, g->st) != 0) {
+                fdvg_launch_parallel(g->seq_d, g->nseq, g->xgrid, g->units_d, g->nunit, g->corder_d, g->ticket_d,
+                                     g->pcw_d, g->edgew_d, g->pgiven_d, g->pctx_d, g->mrec_d, g->botw_d, g->decw_d,
+                                     g->ilist_d, g->ih_d, g->total_mb, g->recs_d, g->st) != 0) {
                 seterr("parallel analysis launch failed: %s", hipGetErrorString(hipGetLastError())); return -1;
             }
         } else if (fdvg_launch_analyse(g->seq_d, g->rowtab_d, g->nrows, g->ticket_d, g->recs_d, g->hand_d, g->st) != 0) {
@@ -863,8 +866,8 @@ extern "C" int fdvgpu_planes(fdvgpu *g, int seq, int src_slot, const uint8_t **c
 
 extern "C" int fdvgpu_set_overlap(fdvgpu *g, int on) {
     if (!g || on < 0 || on > 2) { seterr("fdvgpu_set_overlap: bad arguments"); return -1; }
-    if (on && !g->overlap_avail) {
-        seterr("fdvgpu_set_overlap: the fused decision stage takes rows of at most 256 MBs"); return -1;
+    if (on == 2 && !g->overlap_avail) {
+        seterr("fdvgpu_set_overlap: the replays and the coding workers do not fit side by side"); return -1;
     }
     g->overlap = on;
     return 0;

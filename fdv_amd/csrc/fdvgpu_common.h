@@ -51,21 +51,21 @@ typedef struct {
 } fdvg_nnout;
 
 // Block-parallel decisions (parallel and learned modes).  Per MB: the residual
-// SSD and bits of every candidate mode, without the vector-difference bits
-// (which depend on the predictor), and the SKIP SSDs of a set of likely merge
-// candidates (the MB's own and its neighbours' table vectors).
+// SSD and bits of the INTER16 and INTER8 candidates, without the
+// vector-difference bits (which depend on the predictor), and the SKIP SSDs of
+// a set of likely merge candidates (the MB's own and its neighbours' table
+// vectors).  INTRA is costed in the replay, from the true reconstruction.
 #define FDVG_NSC 12
 // A given INTRA decision whose sub-mode is chosen when the MB is coded, by the
 // SATD screen over its reconstructed neighbours (fdv.h:5950-5970).
 #define FDVG_SUB_SCREEN 0xFF
 typedef struct {
     int32_t d16[2], b16[2];           // INTER16 on each reference (bits incl. the 6 mode/ref bits)
-    int32_t dn, bn;                   // INTRA 16x16 from the source frame's neighbours
     int32_t d8, b8;                   // INTER8 (bits incl. the 3 mode bits)
     int16_t scx[FDVG_NSC], scy[FDVG_NSC];
     int32_t sce[FDVG_NSC];
     uint8_t scr[FDVG_NSC];
-    uint8_t nsub, nsc, pad[2];
+    uint8_t nsc, pad[3];
 } fdvg_pcost;
 
 // The true vector predictor and merge list of an MB once its neighbours are

@@ -85,20 +85,25 @@ template <int W> __device__ __forceinline__ int gor(int v) {
 __device__ __forceinline__ int tap6(int a, int b, int c, int d, int e, int f) {
     return a - 5 * b + 20 * c + 20 * d - 5 * e + f;
 }
-__device__ __forceinline__ int wpx(const uint8_t *w, int x, int y) { return w[y * WIN + x]; }
-__device__ __forceinline__ int whr(const uint8_t *w, int x, int y) {
-    const uint8_t *p = w + y * WIN + x;
+// Planes of row stride S (an LDS window of WIN, or a tile).
+template <int S = WIN> __device__ __forceinline__ int wpx(const uint8_t *w, int x, int y) { return w[y * S + x]; }
+template <int S = WIN> __device__ __forceinline__ int whr(const uint8_t *w, int x, int y) {
+    const uint8_t *p = w + y * S + x;
     return tap6(p[-2], p[-1], p[0], p[1], p[2], p[3]);
 }
-__device__ __forceinline__ int wvr(const uint8_t *w, int x, int y) {
-    const uint8_t *p = w + y * WIN + x;
-    return tap6(p[-2 * WIN], p[-WIN], p[0], p[WIN], p[2 * WIN], p[3 * WIN]);
+template <int S = WIN> __device__ __forceinline__ int wvr(const uint8_t *w, int x, int y) {
+    const uint8_t *p = w + y * S + x;
+    return tap6(p[-2 * S], p[-S], p[0], p[S], p[2 * S], p[3 * S]);
 }
-__device__ __forceinline__ int wb(const uint8_t *w, int x, int y) { return clip8((whr(w, x, y) + 16) >> 5); }
-__device__ __forceinline__ int wh(const uint8_t *w, int x, int y) { return clip8((wvr(w, x, y) + 16) >> 5); }
-__device__ __forceinline__ int wj(const uint8_t *w, int x, int y) {
-    int v = tap6(whr(w, x, y - 2), whr(w, x, y - 1), whr(w, x, y), whr(w, x, y + 1),
-                 whr(w, x, y + 2), whr(w, x, y + 3));
+template <int S = WIN> __device__ __forceinline__ int wb(const uint8_t *w, int x, int y) {
+    return clip8((whr<S>(w, x, y) + 16) >> 5);
+}
+template <int S = WIN> __device__ __forceinline__ int wh(const uint8_t *w, int x, int y) {
+    return clip8((wvr<S>(w, x, y) + 16) >> 5);
+}
+template <int S = WIN> __device__ __forceinline__ int wj(const uint8_t *w, int x, int y) {
+    int v = tap6(whr<S>(w, x, y - 2), whr<S>(w, x, y - 1), whr<S>(w, x, y), whr<S>(w, x, y + 1),
+                 whr<S>(w, x, y + 2), whr<S>(w, x, y + 3));
     return clip8((v + 512) >> 10);
 }
 __device__ __forceinline__ int av2(int a, int b) { return (a + b + 1) >> 1; }
@@ -163,27 +168,32 @@ __device__ __forceinline__ int qerr4(const uint8_t *w, int X, int Y, int ph, con
     }
     return acc;
 }
+// Predicted luma sample whose whole-pel position is (X, Y) of a plane of row
+// stride S, at quarter-pel phase ph = (mvy & 3) << 2 | (mvx & 3).
+template <int S>
+__device__ __forceinline__ int lumaq_at(const uint8_t *w, int X, int Y, int ph) {
+    switch (ph) {
+    case 0:  return wpx<S>(w, X, Y);
+    case 1:  return av2(wpx<S>(w, X, Y), wb<S>(w, X, Y));
+    case 2:  return wb<S>(w, X, Y);
+    case 3:  return av2(wb<S>(w, X, Y), wpx<S>(w, X + 1, Y));
+    case 4:  return av2(wpx<S>(w, X, Y), wh<S>(w, X, Y));
+    case 5:  return av2(wb<S>(w, X, Y), wh<S>(w, X, Y));
+    case 6:  return av2(wb<S>(w, X, Y), wj<S>(w, X, Y));
+    case 7:  return av2(wb<S>(w, X, Y), wh<S>(w, X + 1, Y));
+    case 8:  return wh<S>(w, X, Y);
+    case 9:  return av2(wh<S>(w, X, Y), wj<S>(w, X, Y));
+    case 10: return wj<S>(w, X, Y);
+    case 11: return av2(wj<S>(w, X, Y), wh<S>(w, X + 1, Y));
+    case 12: return av2(wh<S>(w, X, Y), wpx<S>(w, X, Y + 1));
+    case 13: return av2(wh<S>(w, X, Y), wb<S>(w, X, Y + 1));
+    case 14: return av2(wj<S>(w, X, Y), wb<S>(w, X, Y + 1));
+    default: return av2(wh<S>(w, X + 1, Y), wb<S>(w, X, Y + 1));
+    }
+}
 // Predicted luma sample for the MB-relative pixel (x, y) displaced by (mvx, mvy).
 __device__ __forceinline__ int lumaq(const uint8_t *w, int x, int y, int mvx, int mvy) {
-    int X = WOFF + x + (mvx >> 2), Y = WOFF + y + (mvy >> 2);
-    switch (((mvy & 3) << 2) | (mvx & 3)) {
-    case 0:  return wpx(w, X, Y);
-    case 1:  return av2(wpx(w, X, Y), wb(w, X, Y));
-    case 2:  return wb(w, X, Y);
-    case 3:  return av2(wb(w, X, Y), wpx(w, X + 1, Y));
-    case 4:  return av2(wpx(w, X, Y), wh(w, X, Y));
-    case 5:  return av2(wb(w, X, Y), wh(w, X, Y));
-    case 6:  return av2(wb(w, X, Y), wj(w, X, Y));
-    case 7:  return av2(wb(w, X, Y), wh(w, X + 1, Y));
-    case 8:  return wh(w, X, Y);
-    case 9:  return av2(wh(w, X, Y), wj(w, X, Y));
-    case 10: return wj(w, X, Y);
-    case 11: return av2(wj(w, X, Y), wh(w, X + 1, Y));
-    case 12: return av2(wh(w, X, Y), wpx(w, X, Y + 1));
-    case 13: return av2(wh(w, X, Y), wb(w, X, Y + 1));
-    case 14: return av2(wj(w, X, Y), wb(w, X, Y + 1));
-    default: return av2(wh(w, X + 1, Y), wb(w, X, Y + 1));
-    }
+    return lumaq_at<WIN>(w, WOFF + x + (mvx >> 2), WOFF + y + (mvy >> 2), ((mvy & 3) << 2) | (mvx & 3));
 }
 // Bilinear chroma sample for chroma-MB-relative (x, y) with a chroma quarter-pel
 // vector (fdv.h:3260-3283).

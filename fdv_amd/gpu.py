@@ -226,8 +226,10 @@ class Analyser:
 
     def set_overlap(self, mode):
         """The block-parallel decision stage's structure: 0 separate launches
-        (costs, replay, coding), 1 the costs then the replay and coding fused,
-        2 all three in one launch (the default where available; True means 2)."""
+        (costs, replays, coding kernels); 1 the costs, the replays, then the
+        coding workers on one stream; 2 the replays beside the costs and the
+        coding workers beside the replays (the default where the device holds
+        them side by side; True means 2)."""
         mode = 2 if mode is True else int(mode)
         self._ck(self.L.fdvgpu_set_overlap(self.h, mode), "set_overlap")
 

@@ -752,14 +752,18 @@ static int merge_list(const mvrec_t *map, int mw, int mx, int my, int pmx, int p
 static _Thread_local int orc_no_deblock = 0;
 void orc_set_no_deblock(int on) { orc_no_deblock = on; }
 
-int orc_pframe_analyse(const uint8_t *cy, const uint8_t *cu, const uint8_t *cv,
-                       int w, int h,
-                       const uint8_t *const RY[2], const uint8_t *const RU[2],
-                       const uint8_t *const RV[2], int ys, int cs,
-                       int navail, int qp, double lambda, int force_skip, int me_mode,
-                       const orc_mbinfo *given,
-                       uint8_t *ry, uint8_t *ru, uint8_t *rv, orc_mbinfo *info,
-                       uint8_t *const bufs[ORC_NS], size_t scap, size_t sn[ORC_NS]) {
+/* The MB loop of pframe_encode.  With `mt` the motion vectors and SADs come
+ * from the per-MB table instead of orc_me_search (16x16 per reference, the four
+ * quadrants on reference 0), and an MB whose flags bit 0 is set weighs SKIP
+ * against INTER16 only. */
+static int analyse_core(const uint8_t *cy, const uint8_t *cu, const uint8_t *cv,
+                        int w, int h,
+                        const uint8_t *const RY[2], const uint8_t *const RU[2],
+                        const uint8_t *const RV[2], int ys, int cs,
+                        int navail, int qp, double lambda, int force_skip, int me_mode,
+                        const orc_metab *mt, const orc_mbinfo *given,
+                        uint8_t *ry, uint8_t *ru, uint8_t *rv, orc_mbinfo *info,
+                        uint8_t *const bufs[ORC_NS], size_t scap, size_t sn[ORC_NS]) {
     const int cw = w / 2, mw = w / 16, mh = h / 16;
     /* Exact floors (fdv.h:5887-5896) with FDV_BITS_MODE 3, FDV_BITS_FLAG 2. */
     const int FL_INTRA = 2 * 3 + 4 * 2, FL_INTER16 = 2 * 3 + 2 + 4 * 2, FL_INTER8 = 3 + 4 * 2 + 4 * 2;
@@ -794,6 +798,8 @@ int orc_pframe_analyse(const uint8_t *cy, const uint8_t *cu, const uint8_t *cv,
                 continue;
             }
             const orc_mbinfo *gv = given ? &given[my * mw + mx] : NULL;
+            const orc_metab *tm = mt && !gv ? &mt[my * mw + mx] : NULL;
+            const int pruned = tm && (tm->flags & 1);      /* SKIP vs INTER16 only */
 
             /* SKIP: best merge candidate by SSD, first wins (fdv.h:5847-5866) */
             mvrec_t cand[3];
@@ -822,6 +828,7 @@ int orc_pframe_analyse(const uint8_t *cy, const uint8_t *cu, const uint8_t *cv,
                 if (!gv && r > 0 && Jbest <= lambda * FL_INTER16) break;
                 int rmx, rmy, rsad;
                 if (gv) { rmx = gv->mv[0][0]; rmy = gv->mv[0][1]; rsad = 0; }
+                else if (tm) { rmx = tm->mv16[r][0]; rmy = tm->mv16[r][1]; rsad = tm->sad16[r]; }
                 else rsad = orc_me_search(cy, w, RY[r], ys, bx, by, 16, 16, 16, pmx, pmy, me_mode, &rmx, &rmy);
                 uint8_t pr[256], rr[256];
                 orc_mc_luma(RY[r], ys, bx, by, 16, 16, rmx, rmy, pr, 16);
@@ -842,7 +849,7 @@ int orc_pframe_analyse(const uint8_t *cy, const uint8_t *cu, const uint8_t *cv,
             uint8_t recN[256]; coded_t codN;
             uint8_t ntop[16], nleft[16], ntl; int nht, nhl;
             neighbours(ry, w, bx, by, 16, 0, ntop, nleft, &ntl, &nht, &nhl);
-            want_n = gv ? gv->mode == 2 : (!skip_wins && Jbest > lambda * FL_INTRA);
+            want_n = gv ? gv->mode == 2 : (!skip_wins && !pruned && Jbest > lambda * FL_INTRA);
             if (want_n) {
                 uint8_t pI[256];
                 int sel = -1, sbest = 0;
@@ -877,12 +884,13 @@ int orc_pframe_analyse(const uint8_t *cy, const uint8_t *cu, const uint8_t *cv,
             uint8_t rec8[256]; coded_t cod8[4];
             uint8_t s8[64]; int s8n = 0;      /* the four (dx, dy) LEB pairs */
             uint8_t s8ctx[64];
-            want_8 = gv ? gv->mode == 3 : (!skip_wins && Jbest > lambda * FL_INTER8);
+            want_8 = gv ? gv->mode == 3 : (!skip_wins && !pruned && Jbest > lambda * FL_INTER8);
             if (want_8) {
                 int ssad = 0, sx = pmx, syv = pmy;
                 for (int q = 0; q < 4; ++q) {
                     int qx = (q & 1) * 8, qy = (q >> 1) * 8, a, b;
                     if (gv) { a = gv->mv[q][0]; b = gv->mv[q][1]; }
+                    else if (tm) { a = tm->mv8[q][0]; b = tm->mv8[q][1]; ssad += tm->sad8[q]; }
                     else ssad += orc_me_search(cy, w, RY[0], ys, bx + qx, by + qy, 8, 8, 16, sx, syv, me_mode, &a, &b);
                     qmv[q][0] = a; qmv[q][1] = b; sx = a; syv = b;
                 }
@@ -1000,6 +1008,29 @@ int orc_pframe_analyse(const uint8_t *cy, const uint8_t *cu, const uint8_t *cv,
     for (int k = 0; k < ORC_NS; ++k) sn[k] = S.n[k];
     free(mmap); free(mvm); free(st_sym); free(st_ctx);
     return S.over ? -1 : 0;
+}
+
+int orc_pframe_analyse(const uint8_t *cy, const uint8_t *cu, const uint8_t *cv,
+                       int w, int h,
+                       const uint8_t *const RY[2], const uint8_t *const RU[2],
+                       const uint8_t *const RV[2], int ys, int cs,
+                       int navail, int qp, double lambda, int force_skip, int me_mode,
+                       const orc_mbinfo *given,
+                       uint8_t *ry, uint8_t *ru, uint8_t *rv, orc_mbinfo *info,
+                       uint8_t *const bufs[ORC_NS], size_t scap, size_t sn[ORC_NS]) {
+    return analyse_core(cy, cu, cv, w, h, RY, RU, RV, ys, cs, navail, qp, lambda, force_skip, me_mode, NULL,
+                        given, ry, ru, rv, info, bufs, scap, sn);
+}
+
+int orc_pframe_analyse_table(const uint8_t *cy, const uint8_t *cu, const uint8_t *cv,
+                             int w, int h,
+                             const uint8_t *const RY[2], const uint8_t *const RU[2],
+                             const uint8_t *const RV[2], int ys, int cs,
+                             int navail, int qp, double lambda, const orc_metab *mt,
+                             uint8_t *ry, uint8_t *ru, uint8_t *rv, orc_mbinfo *info,
+                             uint8_t *const bufs[ORC_NS], size_t scap, size_t sn[ORC_NS]) {
+    return analyse_core(cy, cu, cv, w, h, RY, RU, RV, ys, cs, navail, qp, lambda, 0, ORC_ME_EXHAUSTIVE, mt,
+                        NULL, ry, ru, rv, info, bufs, scap, sn);
 }
 
 /* ------------------------------------------------------------- entropy -- */
@@ -1324,20 +1355,12 @@ size_t orc_pframe_encode(const uint8_t *cy, const uint8_t *cu, const uint8_t *cv
 }
 
 
-/* ====================================================== parallel decisions ==
- * The block-parallel analysis of this build's parallel and learned modes.  It
- * is not in the reference.  Every MB's candidate costs are computed
- * independently; a light sequential pass then replays the reference's decision
- * logic (fdv.h:5847-6040: SKIP over the true merge list, the gates, the strict-<
- * commit) with the true vector predictor and merge list; the decisions are then
- * coded exactly (orc_pframe_analyse with `given`).
- *   - costs: INTER16 on each reference and INTER8 from a motion table (the
- *     residual SSD and bits, without the vector-difference bits, which depend on
- *     the predictor), and INTRA 16x16 with its sub-mode screened and coded from
- *     the *source* frame's neighbours -- the one approximation;
- *   - repair: per MB in raster order, the true predictor and merge list, the
- *     SKIP SSDs of the true candidates, then the reference's logic on the
- *     stored costs plus the vector bits for the true predictor. */
+/* ========================================================= motion tables ==
+ * The per-MB motion tables of this build's exhaustive, parallel and learned
+ * modes (not in the reference): the vectors and SADs the MB loop would get
+ * from its searches, computed for every MB at once.  The exhaustive search
+ * ignores the predictor, so its table is a pure function of the frame and the
+ * references; orc_pframe_analyse_table runs the MB loop on a table. */
 
 void orc_metab_exhaustive(const uint8_t *cy, int w, int h, const uint8_t *const RY[2], int ys,
                           int navail, orc_metab *mt) {
@@ -1415,300 +1438,6 @@ void orc_metab_candidates(const uint8_t *cy, int w, int h, const uint8_t *const 
         }
         if (navail < 2) { m->mv16[1][0] = m->mv16[0][0]; m->mv16[1][1] = m->mv16[0][1]; m->sad16[1] = m->sad16[0]; }
     }
-}
-
-/* The stand-in reconstruction of every MB, from which the block-parallel intra
- * costs read their neighbours (the true reconstruction exists only once every
- * decision is made).  Per MB, from its own candidates:
- *   - INTER16 on each reference with the table vector, coded; its cost
- *     J = SSD + lambda * (bits + vector bits against (0, 0)); the best
- *     reference is the first with the smallest J;
- *   - SKIP at (0, 0) on reference 0: Js = SSD + 3 lambda;
- *   - the stand-in is the co-located reference-0 block if Js <= 14 lambda (the
- *     INTRA floor) or Js <= the best INTER16 J, else that INTER16 reconstruction.
- * `sb` is a tight w x h plane. */
-void orc_standin_plane(const uint8_t *cy, int w, int h, const uint8_t *const RY[2], int ys, int navail, int qp,
-                       double lambda, const orc_metab *mt, uint8_t *sb) {
-    const int mw = w / 16, mh = h / 16;
-    for (int my = 0; my < mh; ++my)
-        for (int mx = 0; mx < mw; ++mx) {
-            const int bx = mx * 16, by = my * 16;
-            const orc_metab *T = &mt[my * mw + mx];
-            uint8_t best[256];
-            double bj = 0.0;
-            for (int r = 0; r < navail; ++r) {
-                uint8_t pr[256], rr[256]; coded_t tc; double dd = 0.0; int bb = 6;
-                orc_mc_luma(RY[r], ys, bx, by, 16, 16, T->mv16[r][0], T->mv16[r][1], pr, 16);
-                residual(cy, w, bx, by, pr, 16, qp, lambda, rr, &tc, &dd, &bb);
-                bb += orc_bits_val(zmap(T->mv16[r][0])) + orc_bits_val(zmap(T->mv16[r][1]));
-                const double j = dd + lambda * (double)bb;
-                if (r == 0 || j < bj) { bj = j; memcpy(best, rr, 256); }
-            }
-            int e0 = 0;
-            for (int y = 0; y < 16; ++y)
-                for (int x = 0; x < 16; ++x) {
-                    const int d = cy[(by + y) * w + bx + x] - RY[0][(ptrdiff_t)(by + y) * ys + bx + x];
-                    e0 += d * d;
-                }
-            const double js = (double)e0 + lambda * 3.0;
-            const int skip = js <= lambda * 14.0 || js <= bj;
-            for (int y = 0; y < 16; ++y)
-                for (int x = 0; x < 16; ++x)
-                    sb[(by + y) * w + bx + x] = skip ? RY[0][(ptrdiff_t)(by + y) * ys + bx + x] : best[y * 16 + x];
-        }
-}
-
-/* INTRA 16x16 cost of the MB at (bx, by) with its neighbours read from the
- * tight plane `nb`: the SATD screen over the four modes (strict <, first
- * wins), then one residual coding of the best (fdv.h:5950-5988). */
-static void intra_cost(const uint8_t *cy, const uint8_t *nb, int w, int bx, int by, int qp, double lambda,
-                       orc_pcost *P) {
-    uint8_t ntop[16], nleft[16], ntl, pI[256], rn[256]; int nht, nhl;
-    neighbours(nb, w, bx, by, 16, 0, ntop, nleft, &ntl, &nht, &nhl);
-    int sel = -1, sbest = 0;
-    for (int m = 0; m < 4; ++m) {
-        if (!nn_ok(m, 16, nht, nhl)) continue;
-        orc_intra_nxn(m, ntop, nleft, ntl, 16, nht, nhl, pI);
-        int sa = 0;
-        for (int i = 0; i < 16; i += 4)
-            for (int j = 0; j < 16; j += 4) {
-                uint8_t pk[16];
-                for (int k = 0; k < 4; ++k) memcpy(pk + 4 * k, pI + (i + k) * 16 + j, 4);
-                sa += orc_satd4(cy + (by + i) * w + bx + j, w, pk);
-            }
-        if (sel < 0 || sa < sbest) { sbest = sa; sel = m; }
-    }
-    orc_intra_nxn(sel, ntop, nleft, ntl, 16, nht, nhl, pI);
-    coded_t cn; double dn = 0.0; int bn = 6;
-    residual(cy, w, bx, by, pI, 16, qp, lambda, rn, &cn, &dn, &bn);
-    P->dn = (int32_t)dn; P->bn = bn; P->nsub = (uint8_t)sel;
-}
-
-/* The block-parallel costs with the intra neighbours read from plane `nb` (tight,
- * stride w), or, with nb == NULL, from the stand-in reconstruction above (what
- * the product does); the parity tests also pass the exact path's pre-deblock
- * reconstruction. */
-void orc_pframe_costs_par_nb(const uint8_t *cy, const uint8_t *nb, int w, int h, const uint8_t *const RY[2],
-                             int ys, int navail, int qp, double lambda, const orc_metab *mt, orc_pcost *pc);
-void orc_pframe_costs_par(const uint8_t *cy, int w, int h, const uint8_t *const RY[2], int ys,
-                          int navail, int qp, double lambda, const orc_metab *mt, orc_pcost *pc) {
-    orc_pframe_costs_par_nb(cy, NULL, w, h, RY, ys, navail, qp, lambda, mt, pc);
-}
-void orc_pframe_costs_par_nb(const uint8_t *cy, const uint8_t *nb, int w, int h, const uint8_t *const RY[2],
-                             int ys, int navail, int qp, double lambda, const orc_metab *mt, orc_pcost *pc) {
-    const int mw = w / 16, mh = h / 16;
-    uint8_t *own = NULL;
-    if (!nb) {
-        own = malloc((size_t)w * h);
-        if (!own) return;
-        orc_standin_plane(cy, w, h, RY, ys, navail, qp, lambda, mt, own);
-        nb = own;
-    }
-    for (int my = 0; my < mh; ++my)
-        for (int mx = 0; mx < mw; ++mx) {
-            const int bx = mx * 16, by = my * 16;
-            const orc_metab *T = &mt[my * mw + mx];
-            orc_pcost *P = &pc[my * mw + mx];
-            memset(P, 0, sizeof *P);
-            for (int r = 0; r < navail; ++r) {
-                uint8_t pr[256], rr[256]; coded_t tc; double dd = 0.0; int bb = 6;
-                orc_mc_luma(RY[r], ys, bx, by, 16, 16, T->mv16[r][0], T->mv16[r][1], pr, 16);
-                residual(cy, w, bx, by, pr, 16, qp, lambda, rr, &tc, &dd, &bb);
-                P->d16[r] = (int32_t)dd; P->b16[r] = bb;
-            }
-            if (T->flags & 1) continue;                 /* SKIP expected: no INTRA, no INTER8 costs */
-            intra_cost(cy, nb, w, bx, by, qp, lambda, P);
-            {
-                /* never read where the INTER8 gate closes for either table SAD */
-                const int ssad = T->sad8[0] + T->sad8[1] + T->sad8[2] + T->sad8[3];
-                const int g0 = T->sad16[0] > 0 && ssad * 10 >= T->sad16[0] * 9;
-                const int g1 = navail < 2 || (T->sad16[1] > 0 && ssad * 10 >= T->sad16[1] * 9);
-                if (g0 && g1) continue;
-                double d8 = 0.0; int b8 = 3;
-                for (int q = 0; q < 4; ++q) {
-                    const int qx = (q & 1) * 8, qy = (q >> 1) * 8;
-                    uint8_t p8[64], r8[64]; coded_t c8;
-                    orc_mc_luma(RY[0], ys, bx + qx, by + qy, 8, 8, T->mv8[q][0], T->mv8[q][1], p8, 8);
-                    residual(cy, w, bx + qx, by + qy, p8, 8, qp, lambda, r8, &c8, &d8, &b8);
-                }
-                P->d8 = (int32_t)d8; P->b8 = b8;
-            }
-        }
-    free(own);
-}
-
-/* The replay; consulted[i] (may be NULL) = 1 where MB i's decision read its
- * INTRA cost (fdv.h:5950: SKIP not winning outright, not pruned, the best cost
- * so far above the INTRA floor). */
-/* What the replay's decision of one MB weighed, so that a new INTRA cost can be
- * judged without replaying: the SKIP cost, the best INTER16 cost, the best cost
- * before INTRA, the INTER8 cost where its SAD gate opens (else 1e30), and the
- * mode chosen. */
-typedef struct { double js, ji, jpre, j8; int mode; } dstate_t;
-
-static int decide4(const dstate_t *d, double jn, double lambda) {
-    const int FL_INTER8 = 3 + 4 * 2 + 4 * 2;
-    double jb = d->jpre;
-    if (jn < jb) jb = jn;
-    const double j8 = jb > lambda * FL_INTER8 ? d->j8 : 1e30;
-    int mode = 0; double jm = d->js;
-    if (d->ji < jm) { jm = d->ji; mode = 1; }
-    if (jn < jm) { jm = jn; mode = 2; }
-    if (j8 < jm) { jm = j8; mode = 3; }
-    return mode;
-}
-
-static void repair_core(const uint8_t *cy, int w, int h, const uint8_t *const RY[2], int ys,
-                        int navail, double lambda, const orc_metab *mt, const orc_pcost *pc,
-                        orc_mbinfo *given, uint8_t *consulted, dstate_t *dst) {
-    const int mw = w / 16, mh = h / 16;
-    const int FL_INTRA = 2 * 3 + 4 * 2, FL_INTER16 = 2 * 3 + 2 + 4 * 2, FL_INTER8 = 3 + 4 * 2 + 4 * 2;
-    mvrec_t *mvm = calloc((size_t)mw * mh, sizeof *mvm);
-    if (!mvm) return;
-    for (int my = 0; my < mh; ++my) {
-        int pmx = 0, pmy = 0;
-        for (int mx = 0; mx < mw; ++mx) {
-            const int bx = mx * 16, by = my * 16;
-            const orc_metab *T = &mt[my * mw + mx];
-            const orc_pcost *P = &pc[my * mw + mx];
-            mvrec_t cand[3];
-            const int ncand = merge_list(mvm, mw, mx, my, pmx, pmy, cand);
-            int bmi = 0; int32_t Ds = 0;
-            for (int c = 0; c < ncand; ++c) {
-                uint8_t tp[256];
-                orc_mc_luma(RY[cand[c].ref], ys, bx, by, 16, 16, cand[c].x, cand[c].y, tp, 16);
-                int32_t e = 0;
-                for (int i = 0; i < 256; ++i) { int d = cy[(by + (i >> 4)) * w + bx + (i & 15)] - tp[i]; e += d * d; }
-                if (c == 0 || e < Ds) { Ds = e; bmi = c; }
-            }
-            const double Js = (double)Ds + lambda * (double)(3 + (ncand > 1 ? 3 : 0));
-            double Jbest = Js;
-            const int skip_wins = Js <= lambda * FL_INTRA;
-            const int pruned = T->flags & 1;             /* SKIP expected: no INTRA, no INTER8 */
-            double Ji = 1e30; int iref = 0, sad16 = 0;
-            for (int r = 0; r < navail && !skip_wins; ++r) {
-                if (r > 0 && Jbest <= lambda * FL_INTER16) break;
-                const int bb = P->b16[r] + orc_bits_val(zmap(T->mv16[r][0] - pmx)) + orc_bits_val(zmap(T->mv16[r][1] - pmy));
-                const double J = (double)P->d16[r] + lambda * bb;
-                if (J < Jbest) Jbest = J;
-                if (J < Ji) { Ji = J; iref = r; sad16 = T->sad16[r]; }
-            }
-            double Jn = 1e30;
-            const double Jpre = Jbest;
-            const int want_n = !skip_wins && !pruned && Jbest > lambda * FL_INTRA;
-            if (consulted) consulted[my * mw + mx] = (uint8_t)want_n;
-            if (want_n) {
-                Jn = (double)P->dn + lambda * P->bn;
-                if (Jn < Jbest) Jbest = Jn;
-            }
-            double J8 = 1e30, J8v = 1e30;
-            if (!skip_wins && !pruned) {
-                const int ssad = T->sad8[0] + T->sad8[1] + T->sad8[2] + T->sad8[3];
-                if (!(sad16 > 0 && ssad * 10 >= sad16 * 9)) {
-                    int b8 = P->b8, qpx = pmx, qpy = pmy;
-                    for (int q = 0; q < 4; ++q) {
-                        b8 += orc_bits_val(zmap(T->mv8[q][0] - qpx)) + orc_bits_val(zmap(T->mv8[q][1] - qpy));
-                        qpx = T->mv8[q][0]; qpy = T->mv8[q][1];
-                    }
-                    J8v = (double)P->d8 + lambda * b8;
-                }
-                if (Jbest > lambda * FL_INTER8) J8 = J8v;
-            }
-            int mode = 0; double Jm = Js;
-            if (Ji < Jm) { Jm = Ji; mode = 1; }
-            if (Jn < Jm) { Jm = Jn; mode = 2; }
-            if (J8 < Jm) { Jm = J8; mode = 3; }
-            if (dst) dst[my * mw + mx] = (dstate_t){Js, Ji, Jpre, J8v, mode};
-            orc_mbinfo *G = &given[my * mw + mx];
-            memset(G, 0, sizeof *G);
-            G->mode = (uint8_t)mode;
-            mvrec_t *M = &mvm[my * mw + mx];
-            if (mode == 0) {
-                G->merge_idx = (uint8_t)bmi; G->ref = cand[bmi].ref;
-                G->mv[0][0] = cand[bmi].x; G->mv[0][1] = cand[bmi].y;
-                *M = (mvrec_t){cand[bmi].x, cand[bmi].y, cand[bmi].ref, 1};
-            } else if (mode == 1) {
-                G->ref = (uint8_t)iref; G->mv[0][0] = T->mv16[iref][0]; G->mv[0][1] = T->mv16[iref][1];
-                *M = (mvrec_t){T->mv16[iref][0], T->mv16[iref][1], (uint8_t)iref, 1};
-            } else if (mode == 2) {
-                G->sub = ORC_SUB_SCREEN;          /* chosen when coded, from the reconstruction */
-                *M = (mvrec_t){0, 0, 0, 0};
-            } else {
-                for (int q = 0; q < 4; ++q) { G->mv[q][0] = T->mv8[q][0]; G->mv[q][1] = T->mv8[q][1]; }
-                *M = (mvrec_t){T->mv8[3][0], T->mv8[3][1], 0, 1};
-            }
-            if (mode != 2) { pmx = M->x; pmy = M->y; }
-        }
-    }
-    free(mvm);
-}
-
-void orc_pframe_repair_par(const uint8_t *cy, int w, int h, const uint8_t *const RY[2], int ys,
-                           int navail, double lambda, const orc_metab *mt, const orc_pcost *pc,
-                           orc_mbinfo *given) {
-    repair_core(cy, w, h, RY, ys, navail, lambda, mt, pc, given, NULL, NULL);
-}
-
-/* The 33 luma samples an MB's INTRA cost reads (neighbours(), n = 16): the top
- * row, the left column, the corner. */
-static void nb_edges(const uint8_t *rec, int w, int bx, int by, uint8_t e[33]) {
-    int ht, hl;
-    neighbours(rec, w, bx, by, 16, 0, e, e + 16, e + 32, &ht, &hl);
-}
-
-int orc_pframe_par_exact(const uint8_t *cy, const uint8_t *cu, const uint8_t *cv, int w, int h,
-                         const uint8_t *const RY[2], const uint8_t *const RU[2], const uint8_t *const RV[2],
-                         int ys, int cs, int navail, int qp, double lambda, const orc_metab *mt, int npass,
-                         orc_mbinfo *given, orc_pcost *pc, int *converged) {
-    const int mw = w / 16, mh = h / 16, nmb = mw * mh, cw = w / 2, ch = h / 2;
-    const size_t scap = (size_t)nmb * 1600 + 1024;
-    uint8_t *sb = malloc((size_t)w * h), *used = malloc((size_t)nmb * 33), *cons = calloc((size_t)nmb, 1);
-    uint8_t *rec = malloc((size_t)w * h + 2 * (size_t)cw * ch), *sbuf = malloc(ORC_NS * scap);
-    dstate_t *dstate = malloc((size_t)nmb * sizeof *dstate);
-    int rc = -1, changed = 0;
-    *converged = 0;
-    if (!sb || !used || !cons || !rec || !sbuf || !dstate) goto done;
-    {
-        uint8_t *bufs[ORC_NS];
-        size_t sn[ORC_NS];
-        for (int k = 0; k < ORC_NS; ++k) bufs[k] = sbuf + (size_t)k * scap;
-        /* pass 0: intra costs from the stand-in, the replay */
-        orc_standin_plane(cy, w, h, RY, ys, navail, qp, lambda, mt, sb);
-        orc_pframe_costs_par_nb(cy, sb, w, h, RY, ys, navail, qp, lambda, mt, pc);
-        for (int i = 0; i < nmb; ++i) nb_edges(sb, w, (i % mw) * 16, (i / mw) * 16, used + (size_t)i * 33);
-        repair_core(cy, w, h, RY, ys, navail, lambda, mt, pc, given, cons, dstate);
-        /* repair passes: code the decisions (no deblock), re-cost every consulted
-         * INTRA whose neighbours' reconstruction is not what its cost read, replay */
-        for (int pass = 1; pass <= npass; ++pass) {
-            const int saved = orc_no_deblock;
-            orc_no_deblock = 1;
-            const int arc = orc_pframe_analyse(cy, cu, cv, w, h, RY, RU, RV, ys, cs, navail, qp, lambda, 0, 0, given,
-                                               rec, rec + (size_t)w * h, rec + (size_t)w * h + (size_t)cw * ch,
-                                               NULL, bufs, scap, sn);
-            orc_no_deblock = saved;
-            if (arc != 0) goto done;
-            int dirty = 0;
-            for (int i = 0; i < nmb; ++i) {
-                if (!cons[i]) continue;
-                const int bx = (i % mw) * 16, by = (i / mw) * 16;
-                uint8_t e[33];
-                nb_edges(rec, w, bx, by, e);
-                if (!memcmp(e, used + (size_t)i * 33, 33)) continue;
-                intra_cost(cy, rec, w, bx, by, qp, lambda, &pc[i]);
-                memcpy(used + (size_t)i * 33, e, 33);
-                /* with every earlier decision unchanged, this MB's inputs are the
-                 * ones the replay weighed: only a new mode moves anything */
-                if (decide4(&dstate[i], (double)pc[i].dn + lambda * pc[i].bn, lambda) != dstate[i].mode) dirty = 1;
-            }
-            if (!dirty) { *converged = 1; break; }
-            ++changed;
-            repair_core(cy, w, h, RY, ys, navail, lambda, mt, pc, given, cons, dstate);
-        }
-        rc = changed;
-    }
-done:
-    free(sb); free(used); free(cons); free(rec); free(sbuf); free(dstate);
-    return rc;
 }
 
 /* ------------------------------------------------------- the key frame -- */

@@ -117,8 +117,8 @@ size_t orc_pframe_encode(const uint8_t *cy, const uint8_t *cu, const uint8_t *cv
                          orc_tabcache *tc, double lscale, int me_mode,
                          orc_mbinfo *mbinfo);
 
-/* ---- block-parallel decisions (this build's parallel / learned modes; not in
- * the reference).  A per-MB motion table, as the GPU's table kernels write it. */
+/* ---- motion tables (this build's exhaustive, parallel and learned modes; not
+ * in the reference).  A per-MB motion table, as the GPU's table kernels write it. */
 typedef struct {
     int16_t mv16[2][2];
     int32_t sad16[2];
@@ -126,15 +126,6 @@ typedef struct {
     int32_t sad8[4];
     int32_t flags;           /* bit0: SKIP expected -- INTRA and INTER8 are not evaluated */
 } orc_metab;
-
-/* Per-MB candidate costs: SSD and bits of each mode's residual coding, without
- * the vector-difference bits (which depend on the predictor). */
-typedef struct {
-    int32_t d16[2], b16[2];  /* INTER16 on each reference (bits include the 6 mode/ref bits) */
-    int32_t dn, bn;          /* INTRA 16x16 from the stand-in neighbours (bits incl. 6) */
-    int32_t d8, b8;          /* INTER8 (bits include the 3 mode bits) */
-    uint8_t nsub, pad[3];
-} orc_pcost;
 
 /* The learned proposer's output for one MB (fdvg_nnout). */
 typedef struct {
@@ -156,36 +147,20 @@ void orc_metab_exhaustive(const uint8_t *cy, int w, int h, const uint8_t *const 
  * INTER16 only. */
 void orc_metab_candidates(const uint8_t *cy, int w, int h, const uint8_t *const RY[2], int ys, int navail,
                           const orc_nnout *nn, float thr, float sthr, orc_metab *mt);
-/* Every MB's candidate costs, independently of every other MB. */
-void orc_pframe_costs_par(const uint8_t *cy, int w, int h, const uint8_t *const RY[2], int ys,
-                          int navail, int qp, double lambda, const orc_metab *mt, orc_pcost *pc);
-void orc_standin_plane(const uint8_t *cy, int w, int h, const uint8_t *const RY[2], int ys, int navail, int qp,
-                       double lambda, const orc_metab *mt, uint8_t *sb);
-void orc_pframe_costs_par_nb(const uint8_t *cy, const uint8_t *nb, int w, int h, const uint8_t *const RY[2],
-                             int ys, int navail, int qp, double lambda, const orc_metab *mt, orc_pcost *pc);
 void orc_set_no_deblock(int on);
-/* The reference's decision logic in raster order on the stored costs, with the
- * true vector predictor and merge list.  The result is a `given` for
- * orc_pframe_analyse. */
-void orc_pframe_repair_par(const uint8_t *cy, int w, int h, const uint8_t *const RY[2], int ys,
-                           int navail, double lambda, const orc_metab *mt, const orc_pcost *pc,
-                           orc_mbinfo *given);
 
-
-/* The block-parallel decisions made exact by repair passes: the decisions of
- * orc_pframe_repair_par on costs whose INTRA entries read the stand-in plane;
- * then, up to npass times: code them (orc_pframe_analyse, before deblocking),
- * re-cost every INTRA the replay consulted whose 33 neighbour samples in that
- * reconstruction differ from the ones its cost read, and replay again.  When a
- * pass finds nothing to re-cost the decisions are a fixed point, and the only
- * fixed point is the MB wavefront's own result (each MB's inputs come from MBs
- * before it in raster order): *converged = 1.  Outputs the decisions to code
- * (`given`) and the costs; returns the number of passes that re-costed
- * something, or -1 on a stream overflow. */
-int orc_pframe_par_exact(const uint8_t *cy, const uint8_t *cu, const uint8_t *cv, int w, int h,
-                         const uint8_t *const RY[2], const uint8_t *const RU[2], const uint8_t *const RV[2],
-                         int ys, int cs, int navail, int qp, double lambda, const orc_metab *mt, int npass,
-                         orc_mbinfo *given, orc_pcost *pc, int *converged);
+/* The MB loop of pframe_encode (orc_pframe_analyse) with the motion vectors
+ * and SADs taken from a per-MB table instead of a search: 16x16 per reference,
+ * the four 8x8 quadrants on reference 0; an MB whose flags bit 0 is set weighs
+ * SKIP against INTER16 only.  With orc_metab_exhaustive's table this is the
+ * exhaustive mode's result; the block-parallel modes reproduce it. */
+int orc_pframe_analyse_table(const uint8_t *cy, const uint8_t *cu, const uint8_t *cv,
+                             int w, int h,
+                             const uint8_t *const RY[2], const uint8_t *const RU[2],
+                             const uint8_t *const RV[2], int ys, int cs,
+                             int navail, int qp, double lambda, const orc_metab *mt,
+                             uint8_t *ry, uint8_t *ru, uint8_t *rv, orc_mbinfo *info,
+                             uint8_t *const bufs[ORC_NS], size_t scap, size_t sn[ORC_NS]);
 
 /* ---- the key frame (fdv_image_encode, fdv.h:4845; iframe_encode 5487) */
 /* The decided quadtree of a plane, in position-indexed planes.  Depth d has

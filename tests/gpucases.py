@@ -79,7 +79,7 @@ def run_parity(scenes, nframes, qps, me_mode=G.ME_DIAMOND, lscales=None, force_s
         for k in range(n):
             w, h = sizes[k]
             if me_mode in (G.ME_PARALLEL, G.ME_LEARNED) and not fs[k] and not given:
-                st, rec, info, _, _ = L.oracle_parallel(frames[k][f], near[k], far[k], w, h, navail[k], qps[k],
+                st, rec, info = L.oracle_parallel(frames[k][f], near[k], far[k], w, h, navail[k], qps[k],
                                                         lscales[k])
             else:
                 st, rec, info = L.oracle_analyse(frames[k][f], near[k], far[k], w, h, navail[k], qps[k],

@@ -288,61 +288,35 @@ def oracle_metab_exhaustive(cur, r0, r1, w, h, navail):
     return mt
 
 
-class PCost(C.Structure):
-    _fields_ = [("d16", C.c_int32 * 2), ("b16", C.c_int32 * 2), ("dn", C.c_int32), ("bn", C.c_int32),
-                ("d8", C.c_int32), ("b8", C.c_int32), ("nsub", C.c_uint8), ("pad", C.c_uint8 * 3)]
-
-
 def oracle_parallel(cur, r0, r1, w, h, navail, qp, lscale=1.0, metab=None):
-    """The block-parallel analysis: candidate costs from the motion table
-    (exhaustive when none is given), the reference's decision logic replayed on
-    them in raster order, then the decisions coded exactly.
-    Returns (streams, recon, info, decisions given to the coder, costs)."""
+    """The block-parallel modes' result: pframe_encode's MB loop on a motion
+    table (orc_pframe_analyse_table), the exhaustive table when none is given.
+    Returns (16 streams as bytes, deblocked recon I420, per-MB MbInfo array)."""
     lib = oracle()
     keep, ptrs, ys, cs = _padded_refs(r0, r1, w, h)
     RY = (C.c_void_p * 2)(*ptrs[0])
+    RU = (C.c_void_p * 2)(*ptrs[1])
+    RV = (C.c_void_p * 2)(*ptrs[2])
     nmb = (w // 16) * (h // 16)
     mt = metab if metab is not None else oracle_metab_exhaustive(cur, r0, r1, w, h, navail)
-    pc = (PCost * nmb)()
-    given = (MbInfo * nmb)()
-    cy = split_i420(cur, w, h)[0]
+    cy, cu, cv = split_i420(cur, w, h)
+    scap = nmb * 1600 + 1024
+    sb = np.zeros(NS * scap, dtype=np.uint8)
+    bufs = (C.c_void_p * NS)(*[sb.ctypes.data + k * scap for k in range(NS)])
+    sn = (C.c_size_t * NS)()
+    rec = np.zeros_like(cur)
+    ry, ru, rv = split_i420(rec, w, h)
+    info = (MbInfo * nmb)()
     lam = lib.orc_lambda(qp, lscale)
-    lib.orc_pframe_costs_par.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int,
-                                         C.c_int, C.c_double, C.c_void_p, C.c_void_p]
-    lib.orc_pframe_repair_par.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int,
-                                          C.c_double, C.c_void_p, C.c_void_p, C.c_void_p]
-    lib.orc_pframe_costs_par(C.c_void_p(cy.ctypes.data), w, h, C.addressof(RY), ys, navail, qp, lam,
-                             C.addressof(mt), C.addressof(pc))
-    lib.orc_pframe_repair_par(C.c_void_p(cy.ctypes.data), w, h, C.addressof(RY), ys, navail, lam,
-                              C.addressof(mt), C.addressof(pc), C.addressof(given))
-    st, rec, info = oracle_analyse(cur, r0, r1, w, h, navail, qp, lscale, 0, 0, given=given)
-    return st, rec, info, given, pc
-
-
-def oracle_parallel_decisions(cur, r0, r1, w, h, navail, qp, lscale=1.0, metab=None, nb=None):
-    """The block-parallel decisions alone: (given decisions, costs).  `nb` is the
-    tight luma plane the intra costs read their neighbours from (default: the
-    stand-in reconstruction, as the product does)."""
-    lib = oracle()
-    keep, ptrs, ys, cs = _padded_refs(r0, r1, w, h)
-    RY = (C.c_void_p * 2)(*ptrs[0])
-    nmb = (w // 16) * (h // 16)
-    mt = metab if metab is not None else oracle_metab_exhaustive(cur, r0, r1, w, h, navail)
-    pc = (PCost * nmb)()
-    given = (MbInfo * nmb)()
-    cy = split_i420(cur, w, h)[0]
-    nbp = None if nb is None else np.ascontiguousarray(nb[:w * h], dtype=np.uint8)
-    lam = lib.orc_lambda(qp, lscale)
-    lib.orc_pframe_costs_par_nb.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int,
-                                            C.c_int, C.c_double, C.c_void_p, C.c_void_p]
-    lib.orc_pframe_repair_par.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int,
-                                          C.c_double, C.c_void_p, C.c_void_p, C.c_void_p]
-    lib.orc_pframe_costs_par_nb(C.c_void_p(cy.ctypes.data), C.c_void_p(None if nbp is None else nbp.ctypes.data),
-                                w, h, C.addressof(RY), ys,
-                                navail, qp, lam, C.addressof(mt), C.addressof(pc))
-    lib.orc_pframe_repair_par(C.c_void_p(cy.ctypes.data), w, h, C.addressof(RY), ys, navail, lam,
-                              C.addressof(mt), C.addressof(pc), C.addressof(given))
-    return given, pc
+    lib.orc_pframe_analyse_table.argtypes = [C.c_void_p] * 3 + [C.c_int] * 2 + [C.c_void_p] * 3 + \
+        [C.c_int] * 4 + [C.c_double, C.c_void_p] + [C.c_void_p] * 4 + [C.c_void_p, C.c_size_t, C.c_void_p]
+    rc = lib.orc_pframe_analyse_table(cy.ctypes.data, cu.ctypes.data, cv.ctypes.data, w, h,
+                                      C.addressof(RY), C.addressof(RU), C.addressof(RV), ys, cs, navail, qp, lam,
+                                      C.addressof(mt), ry.ctypes.data, ru.ctypes.data, rv.ctypes.data,
+                                      C.addressof(info), C.addressof(bufs), scap, C.addressof(sn))
+    assert rc == 0, "oracle stream overflow"
+    streams = [bytes(sb[k * scap:k * scap + sn[k]]) for k in range(NS)]
+    return streams, rec, info
 
 
 def set_no_deblock(on: bool):
@@ -389,34 +363,6 @@ def ref_pframe_decode(pay, r0, r1, w, h, navail, tc=None):
                                   navail, oy.ctypes.data, ou.ctypes.data, ov.ctypes.data,
                                   tc.buf.ctypes.data if tc is not None else None)
     return out if rc == 0 else None
-
-
-def oracle_parallel_exact(cur, r0, r1, w, h, navail, qp, lscale=1.0, metab=None, npass=3):
-    """The block-parallel analysis with repair passes (orc_pframe_par_exact):
-    (streams, recon, info, decisions given to the coder, costs, passes that
-    re-costed something, converged)."""
-    lib = oracle()
-    keep, ptrs, ys, cs = _padded_refs(r0, r1, w, h)
-    RY = (C.c_void_p * 2)(*ptrs[0])
-    RU = (C.c_void_p * 2)(*ptrs[1])
-    RV = (C.c_void_p * 2)(*ptrs[2])
-    nmb = (w // 16) * (h // 16)
-    mt = metab if metab is not None else oracle_metab_exhaustive(cur, r0, r1, w, h, navail)
-    pc = (PCost * nmb)()
-    given = (MbInfo * nmb)()
-    cy, cu, cv = split_i420(cur, w, h)
-    conv = C.c_int()
-    lam = lib.orc_lambda(qp, lscale)
-    lib.orc_pframe_par_exact.restype = C.c_int
-    lib.orc_pframe_par_exact.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p,
-                                         C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
-                                         C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
-    passes = lib.orc_pframe_par_exact(cy.ctypes.data, cu.ctypes.data, cv.ctypes.data, w, h, C.addressof(RY),
-                                      C.addressof(RU), C.addressof(RV), ys, cs, navail, qp, lam, C.addressof(mt),
-                                      int(npass), C.addressof(given), C.addressof(pc), C.byref(conv))
-    assert passes >= 0, "oracle stream overflow"
-    st, rec, info = oracle_analyse(cur, r0, r1, w, h, navail, qp, lscale, 0, 0, given=given)
-    return st, rec, info, given, pc, passes, bool(conv.value)
 
 
 def ref_video_encode(frames, w, h, qp, keyint, band_mbrows=0, nthreads=8):

@@ -79,7 +79,7 @@ def run_learned(sizes, seq_frames, qps, label, decode=True):
                 bad = [i for i in range(len(om)) if om[i].tobytes() != gm[i].tobytes()]
                 if bad:
                     return f"seq {k} {w}x{h}: motion table differs at {len(bad)} MBs, first {bad[0]}"
-                st, rec, info, _, _ = L.oracle_parallel(cur, near[k], far[k], w, h, navail[k], qps[k], metab=mt)
+                st, rec, info = L.oracle_parallel(cur, near[k], far[k], w, h, navail[k], qps[k], metab=mt)
                 if gst != st:
                     return f"seq {k} {w}x{h}: streams differ {[j for j in range(16) if gst[j] != st[j]]}"
                 if not np.array_equal(grec, rec):

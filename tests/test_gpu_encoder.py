@@ -284,7 +284,7 @@ def test_retry_reuses_the_motion_tables(mode):
             if mode == G.ME_EXHAUSTIVE:
                 st, rec, _ = L.oracle_analyse(frames[k][f], near[k], far[k], w, h, navail[k], 34, me_mode=1)
             else:
-                st, rec, _, _, _ = L.oracle_parallel(frames[k][f], near[k], far[k], w, h, navail[k], 34, metab=mt)
+                st, rec, _ = L.oracle_parallel(frames[k][f], near[k], far[k], w, h, navail[k], 34, metab=mt)
             assert an.streams(k) == st, f"seq {k} frame {f}: streams"
             grec = an.recon(k)
             assert np.array_equal(grec, rec), f"seq {k} frame {f}: recon"

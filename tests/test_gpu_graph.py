@@ -6,6 +6,7 @@ import pytest
 
 from fdv_amd import content
 from fdv_amd import gpu as G
+import orclib as L
 from gpucases import say
 
 pytestmark = pytest.mark.gpu
@@ -75,11 +76,12 @@ def test_autotune_changes_no_result():
 @pytest.mark.parametrize("mode", [G.ME_PARALLEL, G.ME_LEARNED])
 def test_fused_decision_stage_matches_separate_launches(mode):
     """The decision stage's three structures give the same results, eagerly and
-    from captured graphs: separate launches (costs, replay, coding); the costs
-    then the replay and coding fused; all three fused, the replay consuming the
-    costs as they are computed (tagged words).  The third sequence has 11 MB
-    rows: three replay units, whose first rows read the row above from another
-    workgroup."""
+    from captured graphs: separate launches (costs, replays, coding kernels);
+    the costs, the replays, then the coding workers on one stream; the replays
+    beside the costs on a second stream and the coding workers beside the
+    replays (tagged words throughout).  Each replay is one MB row; the third
+    sequence has 11 rows.  In the parallel mode the result is also the oracle's
+    MB loop on the exhaustive table, frame by frame."""
     scenes = _scenes() + [content.scene_pan(80, 176, 63, nobj=2, smin=8, smax=24, pan=(2, 1), grain=2.0)]
     sizes = [(s.w, s.h) for s in scenes]
     nfr = 5
@@ -103,6 +105,15 @@ def test_fused_decision_stage_matches_separate_launches(mode):
             res.append([(an.streams(k), an.recon(k).copy()) for k in range(ns)])
         an.close()
         outs.append(res)
+    if mode == G.ME_PARALLEL:
+        for k, sc in enumerate(scenes):
+            near, far = frames[k][0], None
+            for f in range(1, nfr):
+                st, rec, _ = L.oracle_parallel(frames[k][f], near, far, sc.w, sc.h, 1 if f == 1 else 2,
+                                               [24, 30, 27][k])
+                assert outs[0][f - 1][k][0] == st, f"frame {f} seq {k}: streams differ from the oracle"
+                assert np.array_equal(outs[0][f - 1][k][1], rec), f"frame {f} seq {k}: recon differs from the oracle"
+                far, near = near, rec
     for v in (1, 2):
         for f in range(nfr - 1):
             for k in range(len(scenes)):

@@ -218,7 +218,7 @@ def test_learned_mode_bit_exact_with_oracle(weights_seed):
             bad = [i for i in range(len(om)) if om[i].tobytes() != gm[i].tobytes()]
             assert not bad, (f"seq {k} frame {f}: motion table differs at {len(bad)} MBs, first {bad[0]}: "
                              f"oracle {om[bad[0]]} gpu {gm[bad[0]]} proposal {props[bad[0]]}")
-            st, rec, info, _, _ = L.oracle_parallel(frames[k][f], near[k], far[k], w, h, navail[k], [24, 30][k],
+            st, rec, info = L.oracle_parallel(frames[k][f], near[k], far[k], w, h, navail[k], [24, 30][k],
                                                     metab=mt)
             assert an.streams(k) == st, f"seq {k} frame {f}: streams"
             assert np.array_equal(an.recon(k), rec), f"seq {k} frame {f}: recon"

@@ -1,18 +1,19 @@
-"""Pinning of the oracle's block-parallel and learned-search functions
+"""Pinning of the oracle's motion-table and block-parallel functions
 (oracle/fdv_oracle.c: orc_metab_exhaustive, orc_metab_candidates,
-orc_standin_plane, orc_pframe_costs_par_nb, orc_pframe_repair_par) against
-functions the reference's vectors and brute force already pin:
+orc_pframe_analyse_table) against functions the reference's vectors and brute
+force already pin:
 
 - the motion table is orc_me_search's exhaustive search of every block;
 - the candidate search equals it wherever the proposals cover the whole-pel
   minimum, falls back to it below the confidence threshold, and flags SKIP by
   its own threshold;
-- the stand-in reconstruction is, block by block, the co-located reference or
-  the exact coder's INTER16 reconstruction (the reference itself on a static
-  frame), and the intra costs read their neighbours from it;
-- the decision replay, given intra costs read from the exact wavefront's own
-  pre-deblock reconstruction, reproduces the exact wavefront's decisions (modes,
-  references, merge indices and vectors) on every MB.
+- the MB loop run on the exhaustive table is the exhaustive mode's MB loop
+  (orc_pframe_analyse with the exhaustive search): the same decisions, streams
+  and reconstruction, so the block-parallel modes, which reproduce the table
+  loop, reproduce the exhaustive mode;
+- MBs flagged SKIP-likely are decided between SKIP and INTER16 only;
+- the payloads of every table decode with the reference decoder to the
+  encoder's reconstruction.
 """
 import ctypes as C
 
@@ -130,7 +131,11 @@ def test_candidate_search_covers_the_exhaustive_minimum_and_thresholds():
 
 
 @pytest.mark.parametrize("qp", [20, 28, 36])
-def test_replay_with_exact_intra_costs_reproduces_the_wavefront(qp):
+def test_table_loop_on_the_exhaustive_table_is_the_exhaustive_mode(qp):
+    """orc_pframe_analyse_table takes its vectors from the table; the exhaustive
+    mode runs orc_me_search inside the loop.  The exhaustive search ignores the
+    predictor, so the two must agree on every decision, stream byte and sample,
+    over a chain whose later frames use both references."""
     sc = content.scene_objects(128, 64, 73, nobj=4, smin=8, smax=40, vmax=8, grain=2.0)
     w, h = sc.w, sc.h
     frames = [sc.frame(i) for i in range(4)]
@@ -138,89 +143,37 @@ def test_replay_with_exact_intra_costs_reproduces_the_wavefront(qp):
     seen = set()
     for f in range(1, 4):
         navail = 1 if f == 1 else 2
-        L.set_no_deblock(True)
-        try:
-            _, rec_nd, info = L.oracle_analyse(frames[f], near, far, w, h, navail, qp, me_mode=1)
-        finally:
-            L.set_no_deblock(False)
-        _, rec, info_db = L.oracle_analyse(frames[f], near, far, w, h, navail, qp, me_mode=1)
-        assert [m.mode for m in info] == [m.mode for m in info_db]
-        given, _ = L.oracle_parallel_decisions(frames[f], near, far, w, h, navail, qp, nb=rec_nd)
+        st, rec, info = L.oracle_analyse(frames[f], near, far, w, h, navail, qp, me_mode=1)
+        pst, prec, pinfo = L.oracle_parallel(frames[f], near, far, w, h, navail, qp)
         for i in range(len(info)):
-            a, g = info[i], given[i]
-            assert a.mode == g.mode, f"frame {f} MB {i}: mode {a.mode} vs replay {g.mode}"
+            a, g = info[i], pinfo[i]
+            assert (a.mode, a.ref, a.merge_idx, a.sub) == (g.mode, g.ref, g.merge_idx, g.sub), f"frame {f} MB {i}"
+            assert [tuple(a.mv[k]) for k in range(4)] == [tuple(g.mv[k]) for k in range(4)], f"frame {f} MB {i}"
             seen.add(a.mode)
-            if a.mode in (0, 1):
-                assert (a.ref, a.mv[0][0], a.mv[0][1]) == (g.ref, g.mv[0][0], g.mv[0][1]), f"frame {f} MB {i}"
-            if a.mode == 0:
-                assert a.merge_idx == g.merge_idx, f"frame {f} MB {i}"
-            if a.mode == 3:
-                assert [tuple(a.mv[k]) for k in range(4)] == [tuple(g.mv[k]) for k in range(4)], f"frame {f} MB {i}"
-            if a.mode == 2:
-                assert g.sub == 0xFF                 # screened when coded
+        assert pst == st and np.array_equal(prec, rec), f"frame {f}"
         far, near = near, rec
     assert {0, 1, 2}.issubset(seen)
 
 
-def _standin(cur, r0, r1, w, h, navail, qp, mt):
-    lib = L.oracle()
-    keep, ptrs, ys, cs = L._padded_refs(r0, r1, w, h)
-    RY = (C.c_void_p * 2)(*ptrs[0])
-    sb = np.zeros(w * h, np.uint8)
-    cy = L.split_i420(cur, w, h)[0]
-    lib.orc_standin_plane.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_int,
-                                      C.c_double, C.c_void_p, C.c_void_p]
-    lib.orc_standin_plane(C.c_void_p(cy.ctypes.data), w, h, C.addressof(RY), ys, navail, qp,
-                          lib.orc_lambda(qp, C.c_double(1.0)), C.addressof(mt), C.c_void_p(sb.ctypes.data))
-    return sb.reshape(h, w)
-
-
-def test_standin_of_a_static_frame_is_the_reference():
-    w, h, (f0, f1, f2) = _frames()
-    for qp in (20, 36):
-        mt = L.oracle_metab_exhaustive(f1, f1, f0, w, h, 2)
-        sb = _standin(f1, f1, f0, w, h, 2, qp, mt)
-        assert np.array_equal(sb, _luma(f1, w, h))
-
-
-def test_standin_blocks_are_the_reference_or_the_inter16_reconstruction():
-    """Every stand-in block is the co-located reference-0 block or the exact
-    coder's INTER16 reconstruction of the MB with its table vector (both kinds
-    occur over the QPs), and the block-parallel intra costs are those read from
-    this plane."""
-    sc = content.scene_objects(128, 64, 79, nobj=4, smin=8, smax=40, vmax=8, grain=2.0)
+def test_skip_flagged_mbs_weigh_skip_against_inter16_only():
+    """A table MB with flags bit 0 set is decided between SKIP and INTER16: with
+    every MB flagged no MB is INTRA or INTER8, and the first MB, whose predictor
+    and merge list depend on no other decision, keeps an unflagged SKIP or
+    INTER16 decision (the INTRA and INTER8 costs it drops did not win)."""
+    sc = content.scene_blobs(128, 64, 94, nblob=40, vmax=14, grain=3.0, cut=9)
     w, h = sc.w, sc.h
-    f0, f1, f2 = (sc.frame(i) for i in range(3))
-    nmb = (w // 16) * (h // 16)
+    f0, f1, f2 = sc.frame(0), sc.frame(1), sc.frame(2)
+    qp = 30
     mt = L.oracle_metab_exhaustive(f2, f1, f0, w, h, 2)
-    ref0 = _luma(f1, w, h)
-    kinds = set()
-    for qp in (16, 28, 40):
-        sb = _standin(f2, f1, f0, w, h, 2, qp, mt)
-        recs = []
-        for r in range(2):
-            given = (L.MbInfo * nmb)()
-            for i in range(nmb):
-                given[i].mode, given[i].ref = 1, r
-                given[i].mv[0][0], given[i].mv[0][1] = mt[i].mv16[r][0], mt[i].mv16[r][1]
-            L.set_no_deblock(True)
-            try:
-                _, rec, info = L.oracle_analyse(f2, f1, f0, w, h, 2, qp, given=given)
-            finally:
-                L.set_no_deblock(False)
-            assert all(info[i].mode == 1 for i in range(nmb))
-            recs.append(_luma(rec, w, h))
-        for i in range(nmb):
-            bx, by = (i % (w // 16)) * 16, (i // (w // 16)) * 16
-            blk = sb[by:by + 16, bx:bx + 16]
-            k = [n for n, src in (("ref", ref0), ("inter", recs[0]), ("inter", recs[1]))
-                 if np.array_equal(blk, src[by:by + 16, bx:bx + 16])]
-            assert k, f"qp {qp} MB {i}: stand-in block is neither the reference nor an INTER16 reconstruction"
-            kinds.add(k[0])
-        _, pc_default = L.oracle_parallel_decisions(f2, f1, f0, w, h, 2, qp, metab=mt)
-        _, pc_plane = L.oracle_parallel_decisions(f2, f1, f0, w, h, 2, qp, metab=mt, nb=sb.ravel())
-        assert bytes(pc_default) == bytes(pc_plane)
-    assert kinds == {"ref", "inter"}
+    _, _, info = L.oracle_parallel(f2, f1, f0, w, h, 2, qp, metab=mt)
+    assert {m.mode for m in info} & {2, 3}, "content gives no INTRA or INTER8 decision to prune"
+    for i in range(len(info)):
+        mt[i].flags = 1
+    _, _, pinfo = L.oracle_parallel(f2, f1, f0, w, h, 2, qp, metab=mt)
+    assert all(m.mode in (0, 1) for m in pinfo)
+    # the first MB's predictor and merge list do not depend on other decisions
+    if info[0].mode in (0, 1):
+        assert (pinfo[0].mode, pinfo[0].ref) == (info[0].mode, info[0].ref)
 
 
 need_ref = pytest.mark.skipif(L.ref() is None, reason="reference build not present (oracle/_ref)")
@@ -267,33 +220,10 @@ def test_block_parallel_payloads_decode_with_the_reference_decoder(table):
             if table != "exhaustive":
                 mt = L.oracle_metab_candidates(frames[f], near, far, w, h, navail,
                                                _proposals_from_table(mt, f), 0.7, sthr)
-            st, rec, info, _, _ = L.oracle_parallel(frames[f], near, far, w, h, navail, qp, metab=mt)
+            st, rec, info = L.oracle_parallel(frames[f], near, far, w, h, navail, qp, metab=mt)
             pay = L.oracle_pack(qp, st, tce)
             dec = L.ref_pframe_decode(pay, near, far, w, h, navail, tcd)
             assert dec is not None, f"{w}x{h} frame {f}: the reference decoder rejected the payload"
             assert np.array_equal(dec, rec), f"{w}x{h} frame {f}: decoded frame differs from the encoder's recon"
             assert np.array_equal(tce.buf, tcd.buf), f"{w}x{h} frame {f}: entropy state"
             far, near = near, rec
-
-
-@pytest.mark.parametrize("qp", [20, 30])
-def test_repair_passes_reach_the_wavefront_decisions(qp):
-    """Re-costing every consulted INTRA from the reconstruction of the current
-    decisions and replaying (orc_pframe_par_exact) stops at a fixed point, and
-    that fixed point is the exact wavefront's result: the same streams and
-    reconstruction as the MB wavefront with the exhaustive search."""
-    sc = content.scene_blobs(128, 64, 93, nblob=30, vmax=12, grain=3.0, cut=9)
-    w, h = sc.w, sc.h
-    frames = [sc.frame(i) for i in range(3)]
-    near, far = frames[0], None
-    repaired = 0
-    for f in (1, 2):
-        navail = 1 if f == 1 else 2
-        st, rec, info, given, pc, passes, conv = L.oracle_parallel_exact(frames[f], near, far, w, h, navail, qp,
-                                                                         npass=32)
-        assert conv, f"frame {f}: no fixed point within 32 passes"
-        ost, orec, _ = L.oracle_analyse(frames[f], near, far, w, h, navail, qp, me_mode=1)
-        assert st == ost and np.array_equal(rec, orec), f"frame {f}: fixed point differs from the wavefront"
-        repaired += passes
-        far, near = near, orec
-    assert repaired > 0, "the stand-in costs were already exact: nothing was repaired"

@@ -1,11 +1,11 @@
-"""The plan of the fused decision stage (fdvg_pdecide_plan, a host function of
-the library; no device needed).  The stage cannot wait on a workgroup that has
-not started only if:
-- every replay unit's row above (the last row of the unit before it in the same
-  sequence) belongs to an earlier unit;
+"""The plan of the block-parallel decision stage (fdvg_pdecide_plan, a host
+function of the library; no device needed).  The stage cannot wait on a
+workgroup that has not started only if:
+- every replay unit (one MB row) comes after the unit of the row above it in
+  the same sequence (the replays take their units by ticket in this order);
 - every MB's left, top and top-left neighbours come earlier in the cost and
-  coding order (an INTRA cost waits for their stand-in edges, an intra MB for
-  their reconstruction);
+  coding order (the replay decides in that order, an intra MB waits for their
+  reconstruction);
 and it codes every MB exactly once.  Checked here on the measured workloads'
 shapes and on ragged ones, with a planted mistake that the check must catch."""
 import ctypes as C
@@ -15,7 +15,7 @@ import pytest
 
 from fdv_amd import gpu as G
 
-ROWS = 4          # rows per replay unit (PD_ROWS)
+ROWS = 1          # rows per replay unit
 
 
 class I2(C.Structure):
@@ -98,11 +98,16 @@ def test_plan_check_catches_a_swapped_pair():
         check(sizes, nunit, units, bad, mb0)
 
 
-def test_plan_of_rows_wider_than_the_fused_stage_takes():
-    """No replay units (the separate launches take the batch), but the cost
-    and coding order is complete."""
+def test_plan_of_wide_rows_and_inconsistent_sizes():
+    """Rows of any width get a replay unit each; a sequence whose first MB
+    index does not follow the one before is refused."""
     sizes = [(4112, 32), (64, 48)]
     nunit, units, corder, mb0 = plan(sizes)
-    assert nunit == 0
-    tot = sum((w // 16) * (h // 16) for w, h in sizes)
-    assert sorted(corder.tolist()) == list(range(tot))
+    assert nunit == 5
+    check(sizes, nunit, units, corder, mb0)
+    lib = G.lib()
+    w = (C.c_int * 2)(64, 64)
+    h = (C.c_int * 2)(32, 32)
+    units_b = (I2 * 8)()
+    corder_b = (C.c_uint * 16)()
+    assert lib.fdvg_pdecide_plan(w, h, (C.c_int * 2)(0, 7), 2, units_b, corder_b) == -1
