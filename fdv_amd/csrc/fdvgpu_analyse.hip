@@ -1009,13 +1009,17 @@ __device__ __forceinline__ void pr_store(PrWave &W, PrPre &R, const fdvg_seq &q,
 #define XR_WAVES 5
 struct __align__(16) XrLds {
     PrWave pw;
-    uint8_t itop[2][20];                  // per MB parity: [0] the corner, [1..16] the row above
-    uint8_t ileft[2][16];                 // per MB parity: the left MB's right column
+    uint32_t itop[2][4];                  // per MB parity: the row above, 4 samples a word
+    uint32_t ileft[2][4];                 // per MB parity: the left MB's right column
+    int icorner[2];                       // per MB parity: the above-left sample
     uint8_t icur[4][256];                 // each trial wave's copy of the source MB
     uint8_t ipred[256], irec[256];
     int16_t ilev[256];
     uint8_t iflag[4];
     RegionWs ws;
+    Tab8 tab;                             // the 8x8 basis for the trial's region coding
+    int sat[2][4];                        // per MB parity: the SATD of each sub-mode
+    int sdone[4];                         // per trial wave: MBs whose SATD it has published
     int go;                               // MB x's trial inputs are ready once go > x
     int cancel[2];                        // per MB parity: x + 1 when MB x's INTRA cost is not wanted
     int done[4];                          // per trial wave: MBs finished
@@ -1077,11 +1081,8 @@ __device__ __forceinline__ void xr_decide(XrLds &X, const fdvg_seq &q, const uin
     // MB 0's trial inputs
     uint64_t ab = 0;
     if (my > 0) ab = above(0, 0, mw > 1 ? 1 : 0);
-    if (L < 4) {
-#pragma unroll
-        for (int j = 0; j < 4; ++j) X.itop[0][1 + 4 * L + j] = (uint8_t)(ab >> (8 * j));
-    }
-    if (L == 0) X.itop[0][0] = 128;
+    if (L < 4) X.itop[0][L] = (uint32_t)ab;
+    if (L == 0) X.icorner[0] = 128;
     wave_sync();
     if (L == 0) lds_rel(&X.go, 1);
 
@@ -1187,8 +1188,7 @@ __device__ __forceinline__ void xr_decide(XrLds &X, const fdvg_seq &q, const uin
         if (ncand > 2 && ssd[2] < Ds) { Ds = ssd[2]; bmi = 2; }
         const int sx = bmi == 0 ? cx0 : bmi == 1 ? cx1 : cx2, sy = bmi == 0 ? cy0 : bmi == 1 ? cy1 : cy2;
         const int sref = bmi == 0 ? 0 : bmi == 1 ? cr1 : cr2;
-        // the SKIP winner's tile, for its edges should SKIP be chosen: loads
-        // issued now, they complete during the INTRA wait
+        // the SKIP winner's tile, for its edges should SKIP be chosen
         const bool need_tile = tile_c != bmi;
         uint8_t tvs[7];
         if (need_tile) {
@@ -1217,6 +1217,25 @@ __device__ __forceinline__ void xr_decide(XrLds &X, const fdvg_seq &q, const uin
                 if (J1 < Jbest) Jbest = J1;
                 if (J1 < Ji) { Ji = J1; iref = 1; sad16 = s16_1; }
             }
+        }
+        // the bottom row (lanes 0-15) and right column (lanes 16-31) of each
+        // candidate's reconstruction but INTRA's, formed while the trial runs:
+        // SKIP's is the prediction itself, from the winner's tile
+        if (need_tile) {
+#pragma unroll
+            for (int j = 0; j < 7; ++j) {
+                const int k = L + 64 * j;
+                if (k < 21 * 21) W.tile[(k / 21) * PR_TS + k % 21] = tvs[j];
+            }
+            wave_sync();
+        }
+        int e_s = 0, e_i = 0, e_8 = 0;
+        if (L < 32) {
+            const int ex = L < 16 ? L : 15, ey = L < 16 ? 15 : L - 16;
+            const uint32_t *ew = W.ew[mx % PR_CHUNK];
+            e_s = lumaq_at<PR_TS>(W.tile, 2 + ex, 2 + ey, ((sy & 3) << 2) | (sx & 3));
+            e_i = (int)((ew[(32 * iref + L) >> 2] >> (8 * (L & 3))) & 0xffu);
+            e_8 = (int)((ew[(64 + L) >> 2] >> (8 * (L & 3))) & 0xffu);
         }
         double Jn = 1e30;
         if (!skip_wins && !pruned && Jbest > lambda * (double)FL_INTRA) {
@@ -1251,8 +1270,24 @@ __device__ __forceinline__ void xr_decide(XrLds &X, const fdvg_seq &q, const uin
         else if (mode == 1) { nx = iref ? m1x : m0x; ny = iref ? m1y : m0y; nr = iref; }
         else if (mode == 2) ni = 0;
         else { nx = q8x[3]; ny = q8y[3]; }
+        // the chosen reconstruction's edges
+        int e = mode == 0 ? e_s : mode == 1 ? e_i : e_8;
+        if (mode == 2 && L < 32) e = X.irec[(L < 16 ? 15 : L - 16) * 16 + (L < 16 ? L : 15)];
+        // the next MB's trial inputs, into the buffer of MB mx - 1's parity once
+        // the trial has finished that MB: this is the chain, everything else
+        // is published after it
+        if (mx + 1 < mw) {
+            const int p = (mx + 1) & 1;
+            xr_wait_done(X, mx, ctl, L);
+            if (L >= 16 && L < 32) ((uint8_t *)X.ileft[p])[L - 16] = (uint8_t)e;
+            if (L < 4) X.itop[p][L] = (uint32_t)nb;
+            if (L == 0) X.icorner[p] = (int)(X.itop[mx & 1][3] >> 24);
+            wave_sync();
+            if (L == 0) lds_rel(&X.go, mx + 2);
+        }
         // the decision: the coding workers' copy (ten tagged words, lane k:
-        // payload word k), the motion record for the row below, given / pctx
+        // payload word k), the bottom row and motion record for the row below,
+        // given / pctx
         {
             const uint32_t mvw0 = mode == 3 ? pk16(q8x[0], q8y[0]) : mode <= 1 ? pk16(nx, ny) : 0u;
             uint32_t d = 0;
@@ -1272,31 +1307,6 @@ __device__ __forceinline__ void xr_decide(XrLds &X, const fdvg_seq &q, const uin
             }
             if (L < PD_WORDS) __hip_atomic_store(decw + (size_t)mb * PD_WORDS + L, (uint64_t)step << 32 | d,
                                                  __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        // the chosen reconstruction's bottom row (lanes 0-15) and right column
-        // (lanes 16-31)
-        if (mode == 0) {
-            if (need_tile) {
-#pragma unroll
-                for (int j = 0; j < 7; ++j) {
-                    const int k = L + 64 * j;
-                    if (k < 21 * 21) W.tile[(k / 21) * PR_TS + k % 21] = tvs[j];
-                }
-                wave_sync();
-            }
-        }
-        int e = 0;
-        if (L < 32) {
-            const int ex = L < 16 ? L : 15, ey = L < 16 ? 15 : L - 16;
-            if (mode == 0) e = lumaq_at<PR_TS>(W.tile, 2 + ex, 2 + ey, ((sy & 3) << 2) | (sx & 3));
-            else if (mode == 2) e = X.irec[ey * 16 + ex];
-            else {
-                const int base = mode == 1 ? 32 * iref : 64;
-                e = (int)((W.ew[mx % PR_CHUNK][(base + L) >> 2] >> (8 * ((base + L) & 3))) & 0xffu);
-            }
-        }
-        // bottom row to the row below, then the motion record
-        {
             const int b0 = __shfl(e, 4 * (L & 3)), b1 = __shfl(e, 4 * (L & 3) + 1);
             const int b2 = __shfl(e, 4 * (L & 3) + 2), b3 = __shfl(e, 4 * (L & 3) + 3);
             if (L < BW_WORDS)
@@ -1321,22 +1331,64 @@ __device__ __forceinline__ void xr_decide(XrLds &X, const fdvg_seq &q, const uin
             Xc.cx[2] = (int16_t)cx2; Xc.cy[2] = (int16_t)cy2; Xc.cr[2] = (uint8_t)cr2;
             pctx[mb] = Xc;
         }
-        // the next MB's trial inputs, into the buffer of MB mx - 1's parity once
-        // the trial has finished that MB
-        if (mx + 1 < mw) {
-            const int p = (mx + 1) & 1;
-            xr_wait_done(X, mx, ctl, L);
-            if (L >= 16 && L < 32) X.ileft[p][L - 16] = (uint8_t)e;
-            if (L < 4) {
-#pragma unroll
-                for (int j = 0; j < 4; ++j) X.itop[p][1 + 4 * L + j] = (uint8_t)(nb >> (8 * j));
-            }
-            if (L == 0) X.itop[p][0] = X.itop[mx & 1][16];
-            wave_sync();
-            if (L == 0) lds_rel(&X.go, mx + 2);
-        }
         if (mode != 2) { pmx = nx; pmy = ny; }
     }
+}
+
+// The neighbour parameters of the 16x16 intra predictions (nb_setup for n = 16)
+// with 16-lane DPP sums: lane i of each 16-lane row takes top[i] and left[i];
+// PLANE's H = sum (i - 7) top[i] - 8 corner, V likewise.
+__device__ __forceinline__ Nb xr_nb(const uint8_t *top, const uint8_t *left, int tl, int ht, int hl, int L) {
+    const int i = L & 15;
+    const int ti = ht ? top[i] : 0, li = hl ? left[i] : 0;
+    const int st = row_sum16(ti), sl = row_sum16(li);
+    Nb q;
+    q.ht = ht; q.hl = hl;
+    q.dc = ht && hl ? (st + sl + 16) / 32 : ht ? (st + 8) / 16 : hl ? (sl + 8) / 16 : 128;
+    q.a = q.b = q.c = 0;
+    if (ht && hl) {
+        const int H = row_sum16((i - 7) * ti) - 8 * tl, V = row_sum16((i - 7) * li) - 8 * tl;
+        q.a = 16 * (left[15] + top[15]);
+        q.b = (5 * H + 32) >> 6;
+        q.c = (5 * V + 32) >> 6;
+    }
+    return q;
+}
+
+// The SATD of sub-mode m over the MB (mb_satd, fdv.h:4411; the 4x4 Hadamard of
+// fdv.h:3803-3822), one wavefront: lane L takes row L & 3 of 4x4 block L >> 2,
+// transforms it in registers, and the columns across the quad's four lanes by
+// DPP (lane j: the sum with lane j ^ s, or the other minus its own value when
+// bit s of j is set), which leaves each quad holding the block's 16 Hadamard
+// coefficients in some order.  Returns the total (uniform).
+__device__ __forceinline__ int xr_satd(const uint8_t *cur, const uint8_t *top, const uint8_t *left, const Nb &q,
+                                       int m, int L) {
+    const int b = L >> 2, j = L & 3, ox = (b & 3) * 4, y = (b >> 2) * 4 + j;
+    const uint32_t cw = *(const uint32_t *)(cur + y * 16 + ox), tw = *(const uint32_t *)(top + ox);
+    const int lft = left[y];
+    int d[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int x = ox + k;
+        int p = q.dc;
+        if (m == 3 && q.ht && q.hl) p = clip8((q.a + q.c * (y - 7) + 16 + q.b * (x - 7)) >> 5);
+        else if (m == 1 && q.ht) p = (int)((tw >> (8 * k)) & 255u);
+        else if (m == 2 && q.hl) p = lft;
+        d[k] = (int)((cw >> (8 * k)) & 255u) - p;
+    }
+    const int p0 = d[0] + d[1], m0 = d[0] - d[1], p1 = d[2] + d[3], m1 = d[2] - d[3];
+    int t[4] = {p0 + p1, m0 + m1, p0 - p1, m0 - m1};
+    int sa = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        int v = t[k];
+        int o = dpp_mov<DPP_XOR1>(v);
+        v = (j & 1) ? o - v : v + o;
+        o = dpp_mov<DPP_XOR2>(v);
+        v = (j & 2) ? o - v : v + o;
+        sa += abs(v);
+    }
+    return wave_sum(sa);
 }
 
 // Waves 1-4 of k_pxrep: the INTRA trial of every MB of row my, region r of
@@ -1360,25 +1412,39 @@ __device__ __forceinline__ void xr_trial(XrLds &X, const fdvg_seq &q, unsigned *
             g = lds_acq(&X.go);
         }
         const int p = mx & 1;
-        if (lds_acq(&X.cancel[p]) != mx + 1) {
-            wave_sync();
-            const uint8_t *top = &X.itop[p][1], *left = X.ileft[p];
-            const Nb nbq = nb_setup(top, left, X.itop[p][0], 16, my > 0, mx > 0);
-            int sat[4];
-            satd_modes(cur, nbq, top, left, sat);
+        // the screen: this wave's sub-mode r, then the four totals; every wave
+        // publishes (a cancelled MB's totals are never used), so the exchange
+        // completes whichever waves saw the cancel
+        const uint8_t *top = (const uint8_t *)X.itop[p], *left = (const uint8_t *)X.ileft[p];
+        const bool live = lds_acq(&X.cancel[p]) != mx + 1;
+        Nb nbq = {};
+        wave_sync();
+        if (live) {
+            nbq = xr_nb(top, left, X.icorner[p], my > 0, mx > 0, L);
+            const int sr = xr_satd(cur, top, left, nbq, r, L);
+            if (L == 0) X.sat[p][r] = sr;
+        }
+        wave_sync();
+        if (L == 0) lds_rel(&X.sdone[r], mx + 1);
+        int sv = L < 4 ? lds_acq(&X.sdone[L]) : mx + 1;
+        for (unsigned spins = 0; !__all(sv > mx); ++spins) {
+            if (xr_give_up(spins, 1u << 24, ctl)) break;
+            __builtin_amdgcn_s_sleep(1);
+            if (L < 4) sv = lds_acq(&X.sdone[L]);
+        }
+        if (live && lds_acq(&X.cancel[p]) != mx + 1) {
             int sel = -1, sbest = 0;
 #pragma unroll
             for (int m = 0; m < 4; ++m) {
+                const int v = X.sat[p][m];
                 if (!nn_ok(m, nbq)) continue;
-                if (sel < 0 || sat[m] < sbest) { sbest = sat[m]; sel = m; }
+                if (sel < 0 || v < sbest) { sbest = v; sel = m; }
             }
-            if (lds_acq(&X.cancel[p]) != mx + 1) {
-                const int y = oy + (L >> 3), x = ox + (L & 7);
-                X.ipred[y * 16 + x] = (uint8_t)ipred(sel, nbq, top, left, 16, x, y);
-                wave_sync();
-                region_code<0>(X.ws, r, cur + o, 16, X.ipred + o, 16, X.irec + o, 16, X.ilev, X.iflag, q.qp,
-                               q.lambda);
-            }
+            const int y = oy + (L >> 3), x = ox + (L & 7);
+            X.ipred[y * 16 + x] = (uint8_t)ipred(sel, nbq, top, left, 16, x, y);
+            wave_sync();
+            region_code<0, true>(X.ws, r, cur + o, 16, X.ipred + o, 16, X.irec + o, 16, X.ilev, X.iflag, q.qp,
+                                 q.lambda, &X.tab);
         }
         wave_sync();
         if (L == 0) lds_rel(&X.done[r], mx + 1);
@@ -1400,7 +1466,8 @@ k_pxrep(const fdvg_seq *__restrict__ seqs, const int2 *__restrict__ units, int n
         role_s = atomicAdd(ctl + 6, 1u);
         X.go = 0; X.cancel[0] = X.cancel[1] = 0;
     }
-    if (tid < 4) X.done[tid] = 0;
+    if (tid < 4) { X.done[tid] = 0; X.sdone[tid] = 0; }
+    tab8_fill(X.tab, tid);
     __syncthreads();
     const unsigned role = (unsigned)__builtin_amdgcn_readfirstlane((int)role_s);
     if (role >= (unsigned)nunit) return;

@@ -226,6 +226,22 @@ __device__ __forceinline__ int row_max16(int v) {
     v = max(v, dpp_mov<DPP_ROR4>(v)); v = max(v, dpp_mov<DPP_ROR8>(v));
     return v;
 }
+__device__ __forceinline__ int row_min16(int v) {
+    v = min(v, dpp_mov<DPP_XOR1>(v)); v = min(v, dpp_mov<DPP_XOR2>(v));
+    v = min(v, dpp_mov<DPP_ROR4>(v)); v = min(v, dpp_mov<DPP_ROR8>(v));
+    return v;
+}
+// inclusive prefix sum over each 16-lane row (row_shr: lanes with no source
+// lane in the row read 0)
+#define DPP_SHR1 0x111
+#define DPP_SHR2 0x112
+#define DPP_SHR4 0x114
+#define DPP_SHR8 0x118
+__device__ __forceinline__ int row_prefix16(int v) {
+    v += dpp_mov<DPP_SHR1>(v); v += dpp_mov<DPP_SHR2>(v);
+    v += dpp_mov<DPP_SHR4>(v); v += dpp_mov<DPP_SHR8>(v);
+    return v;
+}
 // sum over each aligned group of 8 lanes; valid in lanes 8g+4 .. 8g+7
 __device__ __forceinline__ int oct_sum8(int v) {
     v += dpp_mov<DPP_XOR1>(v); v += dpp_mov<DPP_XOR2>(v);
