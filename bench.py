@@ -509,7 +509,19 @@ def main():
             rec = json.load(f)[args.mode]
         bd = {"median_percent": round(rec["summary"]["median"], 2), "mean_percent": round(rec["summary"]["mean"], 2),
               "scenes": rec["summary"]["scenes"], "worst": rec["summary"]["worst"],
-              "source": "fdv_amd/learned/bdrate_library_720p.json"}
+              "qps": rec["qps"], "source": "fdv_amd/learned/bdrate_library_720p.json"}
+        # the same study at the proposer configuration's QPs, and on seeded content
+        for key, name in (("library_qp20_32", "bdrate_library_720p_qp20-32.json"),
+                          ("seeded", "bdrate_seeded_720p.json")):
+            pth = os.path.join(ROOT, "fdv_amd", "learned", name)
+            if os.path.exists(pth):
+                with open(pth) as f:
+                    r2 = json.load(f).get(args.mode)
+                if r2 and r2.get("summary"):
+                    bd[key] = {"median_percent": round(r2["summary"]["median"], 2),
+                               "mean_percent": round(r2["summary"]["mean"], 2), "qps": r2["qps"],
+                               "scenes": r2["summary"]["scenes"], "worst": r2["summary"]["worst"],
+                               "source": "fdv_amd/learned/" + name}
     # the CPU leg runs on the headline workload only
     cpu = None if (args.no_cpu or args.workload != "720p") else cpu_baseline(frames, W, layout, QP, args.cpu_seconds)
     out = {
