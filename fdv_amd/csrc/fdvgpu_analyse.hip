@@ -284,8 +284,11 @@ __device__ __forceinline__ int hbyte(const Smem &s, int j, int b) {
 // ---- the MB body, shared by the wavefront kernel and the block-parallel ones
 
 // Stage the source MB and, when the MB may use them, the 64x64 luma and 32x32
-// chroma windows of each reference around it.  The caller synchronises.
-__device__ __forceinline__ void mb_stage(Smem &s, const fdvg_seq &q, int mb, int bx, int by, int gmode = -1) {
+// chroma windows of each reference around it (of the references in refmask,
+// for a caller that knows which the MB's decision reads).  The caller
+// synchronises.
+__device__ __forceinline__ void mb_stage(Smem &s, const fdvg_seq &q, int mb, int bx, int by, int gmode = -1,
+                                         int refmask = 3) {
     const int tid = threadIdx.x, cw = q.w >> 1, cbx = bx >> 1, cby = by >> 1;
     s.cur[tid] = gmem(q.cy)[(size_t)(by + (tid >> 4)) * q.w + bx + (tid & 15)];
     if (tid < 128) {
@@ -299,6 +302,7 @@ __device__ __forceinline__ void mb_stage(Smem &s, const fdvg_seq &q, int mb, int
     const int nwin = q.force_skip ? 1 : q.navail;
     if (need_ref || q.force_skip) {
         for (int r = 0; r < nwin; ++r) {
+            if (!((refmask >> r) & 1)) continue;     // uniform: a reference the caller does not use
             // 64 rows x 64 bytes: 16 bytes a thread, two 8-byte loads (8-aligned)
             const int wy = tid >> 2, wx = (tid & 3) * 16;
             const gmem_t<uint8_t> *src = gmem(r ? q.ry[1] : q.ry[0]) + (ptrdiff_t)(by - WOFF + wy) * q.ys + bx - WOFF + wx;
@@ -847,21 +851,25 @@ struct CostLds {
     union { fdvg_pcost c; uint32_t w[PC_WORDS]; } sp;
 };
 
-// The costs of every MB, one workgroup a MB taken by ticket (ctl[3]) in the
-// order of `corder` (the replay's order), published as tagged words: the costs
-// (PC_WORDS a MB) and the candidate edges (EW_WORDS a MB, 4 samples a word),
-// read by the replay, which may run at the same time.
+// The costs of every MB, one workgroup a MB, published as tagged words: the
+// costs (PC_WORDS a MB) and the candidate edges (EW_WORDS a MB, 4 samples a
+// word), read by the replays, which may run at the same time.  XCD k (the
+// workgroups with blockIdx.x % 8 == k) takes, by its own ticket (ctl[8 + k]),
+// the MBs of its rows in the replays' order (xorder[xoff[k] .. xoff[k + 1]]):
+// the windows that neighbouring MBs share are then fetched into one L2, and
+// the costs still arrive in about the order the replays use them.
 extern "C" __global__ void __launch_bounds__(256, 4)
-k_pcosts(const fdvg_seq *__restrict__ seqs, int nseq, const unsigned *__restrict__ corder, int total_mb,
-         unsigned *ctl, uint64_t *__restrict__ pcw, uint64_t *__restrict__ edgew) {
+k_pcosts(const fdvg_seq *__restrict__ seqs, int nseq, const unsigned *__restrict__ xorder,
+         const int *__restrict__ xoff, unsigned *ctl, uint64_t *__restrict__ pcw, uint64_t *__restrict__ edgew) {
     __shared__ CostLds C;
     __shared__ int job_s;
-    const int tid = threadIdx.x;
-    if (tid == 0) job_s = (int)atomicAdd(ctl + 3, 1u);
+    const int tid = threadIdx.x, xk = blockIdx.x & 7;
+    if (tid == 0) job_s = (int)atomicAdd(ctl + 8 + xk, 1u);
     __syncthreads();
     const int job = __builtin_amdgcn_readfirstlane(job_s);
-    if (job >= total_mb) return;
-    const int mb = __builtin_amdgcn_readfirstlane((int)corder[job]);
+    const int lo = xoff[xk], n = xoff[xk + 1] - lo;
+    if (job >= n) return;
+    const int mb = __builtin_amdgcn_readfirstlane((int)xorder[lo + job]);
     int sq = 0;
     for (int k = 1; k < nseq; ++k) if (mb >= seqs[k].mb0) sq = k;
     fdvg_seq q = seqs[sq];
@@ -1669,17 +1677,21 @@ k_pintra(const fdvg_seq *__restrict__ seqs, int nseq, const fdvg_mbinfo *__restr
     }
 }
 
-// The coding workers of the decisions (k_pdecide): they take MBs in the order
-// of `corder` (by mx + 2 my, the order in which the replays decide them), wait
-// for the MB's decision words, code it as k_pencode / k_pintra do, and publish
-// its reconstruction's bottom row and right column as handoff words.  An intra
-// MB waits for the handoff words of its left, top and top-left neighbours, all
-// earlier in that order.  Every wait is on a decision (made by a replay, which
-// never waits for the workers) or on an earlier job, and is bounded (error flag
-// on timeout).
+// The coding workers of the decisions (k_pdecide): the workers of XCD k take,
+// by its own ticket (ctl[16 + k]), the MBs of its rows in the order in
+// which the replays decide them (xorder, as k_pcosts), wait for the MB's
+// decision words, code it as k_pencode / k_pintra do (staging only the
+// reference window the decision reads), and publish its reconstruction's
+// bottom row and right column as handoff words.  An intra MB waits for the
+// handoff words of its left, top and top-left neighbours, which are earlier in
+// the replays' order (in this XCD's list or another's).  Every wait is on a
+// decision (made by a replay, which never waits for the workers) or on an
+// earlier MB in that order, and is bounded (error flag on timeout); with every
+// XCD holding workers, the earliest unfinished MB is always being coded.
 extern "C" __global__ void __launch_bounds__(256)
-k_pdecide(const fdvg_seq *__restrict__ seqs, int nseq, const unsigned *__restrict__ corder, int total_mb,
-          unsigned *ctl, const uint64_t *__restrict__ decw, uint64_t *__restrict__ ih, fdvg_mbrec *__restrict__ recs) {
+k_pdecide(const fdvg_seq *__restrict__ seqs, int nseq, const unsigned *__restrict__ xorder,
+          const int *__restrict__ xoff, unsigned *ctl, const uint64_t *__restrict__ decw, uint64_t *__restrict__ ih,
+          fdvg_mbrec *__restrict__ recs) {
     __shared__ Smem s;
     __shared__ union {
         uint32_t w[PD_WORDS];
@@ -1687,13 +1699,14 @@ k_pdecide(const fdvg_seq *__restrict__ seqs, int nseq, const unsigned *__restric
     } decb;
     const int tid = threadIdx.x;
     __shared__ int job_s;
+    const int xk = blockIdx.x & 7, lo = xoff[xk], n = xoff[xk + 1] - lo;
     for (;;) {
-        if (tid == 0) job_s = (int)atomicAdd(ctl + 7, 1u);
+        if (tid == 0) job_s = (int)atomicAdd(ctl + 16 + xk, 1u);
         __syncthreads();
         const int job = __builtin_amdgcn_readfirstlane(job_s);        // uniform: scalar registers
         __syncthreads();
-        if (job >= total_mb) return;
-        const int mb = __builtin_amdgcn_readfirstlane((int)corder[job]);
+        if (job >= n) return;
+        const int mb = __builtin_amdgcn_readfirstlane((int)xorder[lo + job]);
         int sq = 0;
         for (int k = 1; k < nseq; ++k) if (mb >= seqs[k].mb0) sq = k;
         fdvg_seq q = seqs[sq];
@@ -1713,7 +1726,7 @@ k_pdecide(const fdvg_seq *__restrict__ seqs, int nseq, const unsigned *__restric
         const fdvg_mbinfo &G = decb.d.g;
         const fdvg_pctx &X = decb.d.x;
         const int gmode = G.mode;
-        mb_stage(s, q, mb, bx, by, gmode);
+        mb_stage(s, q, mb, bx, by, gmode, 1 << G.ref);
         if (gmode == 2) nb_load_ih(s, q, nullptr, ih, ctl, mx, my, mw, bx, by);
         MbCtx c;
         ctx_from(c, X, my > 0, mx > 0);
@@ -1760,11 +1773,40 @@ extern "C" int fdvg_pdecide_plan(const int *w, const int *h, const int *mb0, int
     return n == total ? nunit : -1;
 }
 
+// The per-XCD job lists of k_pcosts and k_pdecide: every MB of corder (the
+// replays' order) goes, in that order, to the list of the XCD that owns its MB
+// row, the rows of the batch (sequence after sequence) dealt to the 8 XCDs in
+// turn.  An XCD then holds whole rows, whose neighbouring MBs share most of
+// their reference windows, and every XCD has rows on the replays' front at
+// any time (the front runs along the rows of every sequence at once).
+// xorder holds the eight lists one after the other, xoff[k] .. xoff[k + 1]
+// the k-th.  Returns the longest list's length.
+extern "C" int fdvg_xcd_order(const int *w, const int *h, const int *mb0, int nseq, const unsigned *corder,
+                              int total_mb, unsigned *xorder, int *xoff) {
+    int cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    auto xcd = [&](unsigned mb) {
+        int k = 0, rows = 0;
+        for (int j = 1; j < nseq; ++j) if ((int)mb >= mb0[j]) k = j;
+        for (int j = 0; j < k; ++j) rows += h[j] >> 4;
+        return (rows + ((int)mb - mb0[k]) / (w[k] >> 4)) & 7;
+    };
+    for (int t = 0; t < total_mb; ++t) ++cnt[xcd(corder[t])];
+    xoff[0] = 0;
+    int mx = 0;
+    for (int k = 0; k < 8; ++k) { xoff[k + 1] = xoff[k] + cnt[k]; mx = cnt[k] > mx ? cnt[k] : mx; cnt[k] = 0; }
+    for (int t = 0; t < total_mb; ++t) {
+        const int k = xcd(corder[t]);
+        xorder[xoff[k] + cnt[k]++] = corder[t];
+    }
+    return mx;
+}
+
 // Coding workers of k_pdecide: workers_per_cu a CU (at most what a CU holds);
 // with workers_per_cu < 0, for a launch beside the nunit replays of k_pxrep:
-// ncu - nunit workers, so that however the workers are placed (one a CU at
-// worst) every replay not yet resident finds a CU with no worker and no
-// other replay on it; 0 when that leaves fewer than a quarter of the CUs.
+// as many as leave, however they are placed (one a CU at worst), every
+// replay not yet resident a CU of its XCD with no worker and no other replay;
+// 0 when that leaves fewer than a quarter of the CUs.  Always a multiple of 8,
+// so that every XCD holds workers.
 extern "C" int fdvg_pdecide_grid(int nunit, int total_mb, int workers_per_cu) {
     static int ncu = 0, per = 0;
     if (!ncu) {
@@ -1774,18 +1816,19 @@ extern "C" int fdvg_pdecide_grid(int nunit, int total_mb, int workers_per_cu) {
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void *)k_pdecide, 256, 0) != hipSuccess) return -1;
     }
     if (workers_per_cu < 0) {
-        const int wk = ncu - nunit;
-        return wk < ncu / 4 ? 0 : wk < total_mb ? wk : total_mb;
+        // per XCD (workgroups go to the 8 XCDs round-robin): its CUs less the
+        // replays it may hold
+        const int wx = ncu / 8 - (nunit + 7) / 8;
+        return wx < ncu / 32 ? 0 : 8 * wx;
     }
-    int wk = ncu * (workers_per_cu < per ? workers_per_cu : per);
-    if (wk < 1) wk = 1;
-    return wk < total_mb ? wk : total_mb;
+    const int wk = ncu * (workers_per_cu < per ? workers_per_cu : per);
+    return wk < 8 ? 8 : wk / 8 * 8;
 }
 
-extern "C" int fdvg_launch_pdecide(const fdvg_seq *seqs, int nseq, const unsigned *corder, int total_mb, int grid,
+extern "C" int fdvg_launch_pdecide(const fdvg_seq *seqs, int nseq, const unsigned *xorder, const int *xoff, int grid,
                                    unsigned *ctl, const uint64_t *decw, uint64_t *ih, fdvg_mbrec *recs,
                                    hipStream_t st) {
-    hipLaunchKernelGGL(k_pdecide, dim3(grid), dim3(256), 0, st, seqs, nseq, corder, total_mb, ctl, decw, ih, recs);
+    hipLaunchKernelGGL(k_pdecide, dim3(grid), dim3(256), 0, st, seqs, nseq, xorder, xoff, ctl, decw, ih, recs);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -1805,9 +1848,10 @@ extern "C" int fdvg_launch_plist(const fdvg_seq *seqs, int nseq, const fdvg_mbin
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-extern "C" int fdvg_launch_pcosts(const fdvg_seq *seqs, int nseq, const unsigned *corder, int total_mb, unsigned *ctl,
-                                  uint64_t *pcw, uint64_t *edgew, hipStream_t st) {
-    hipLaunchKernelGGL(k_pcosts, dim3(total_mb), dim3(256), 0, st, seqs, nseq, corder, total_mb, ctl, pcw, edgew);
+// xmax: the longest per-XCD list (fdvg_xcd_order)
+extern "C" int fdvg_launch_pcosts(const fdvg_seq *seqs, int nseq, const unsigned *xorder, const int *xoff, int xmax,
+                                  unsigned *ctl, uint64_t *pcw, uint64_t *edgew, hipStream_t st) {
+    hipLaunchKernelGGL(k_pcosts, dim3(8 * xmax), dim3(256), 0, st, seqs, nseq, xorder, xoff, ctl, pcw, edgew);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -1833,11 +1877,11 @@ extern "C" int fdvg_launch_pfinish(const fdvg_seq *seqs, int nseq, int xgrid, un
 // The decision stage as separate launches on one stream: the costs, the
 // replays, the coding of the decisions.
 extern "C" int fdvg_launch_parallel(const fdvg_seq *seqs, int nseq, int xgrid, const int2 *units, int nunit,
-                                    const unsigned *corder, unsigned *ctl, uint64_t *pcw, uint64_t *edgew,
-                                    fdvg_mbinfo *given, fdvg_pctx *pctx, uint64_t *mrec, uint64_t *botw,
-                                    uint64_t *decw, unsigned *ilist, uint64_t *ih, int total_mb, fdvg_mbrec *recs,
-                                    hipStream_t st) {
-    if (fdvg_launch_pcosts(seqs, nseq, corder, total_mb, ctl, pcw, edgew, st) != 0) return -1;
+                                    const unsigned *xorder, const int *xoff, int xmax, unsigned *ctl, uint64_t *pcw,
+                                    uint64_t *edgew, fdvg_mbinfo *given, fdvg_pctx *pctx, uint64_t *mrec,
+                                    uint64_t *botw, uint64_t *decw, unsigned *ilist, uint64_t *ih, int total_mb,
+                                    fdvg_mbrec *recs, hipStream_t st) {
+    if (fdvg_launch_pcosts(seqs, nseq, xorder, xoff, xmax, ctl, pcw, edgew, st) != 0) return -1;
     if (fdvg_launch_pxrep(seqs, units, nunit, ctl, pcw, edgew, given, pctx, mrec, botw, decw, st) != 0) return -1;
     return fdvg_launch_pfinish(seqs, nseq, xgrid, ctl, given, pctx, ilist, ih, total_mb, recs, st);
 }

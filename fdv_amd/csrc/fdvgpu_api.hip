@@ -46,17 +46,22 @@ extern "C" int fdvg_nn_setup(void);
 // The proposer's confidence threshold: below it an MB gets the exhaustive search.
 #define FDVG_NN_THRESHOLD 0.7f     // default confidence threshold of the proposer
 #define FDVG_NN_SKIP_OFF 2.0f      // default SKIP threshold: no pruning
-extern "C" int fdvg_launch_parallel(const fdvg_seq *, int, int, const int2 *, int, const unsigned *, unsigned *,
-                                    uint64_t *, uint64_t *, fdvg_mbinfo *, fdvg_pctx *, uint64_t *, uint64_t *,
-                                    uint64_t *, unsigned *, uint64_t *, int, fdvg_mbrec *, hipStream_t);
-extern "C" int fdvg_launch_pcosts(const fdvg_seq *, int, const unsigned *, int, unsigned *, uint64_t *, uint64_t *,
-                                  hipStream_t);
+extern "C" int fdvg_launch_parallel(const fdvg_seq *, int, int, const int2 *, int, const unsigned *, const int *,
+                                    int, unsigned *, uint64_t *, uint64_t *, fdvg_mbinfo *, fdvg_pctx *, uint64_t *,
+                                    uint64_t *, uint64_t *, unsigned *, uint64_t *, int, fdvg_mbrec *, hipStream_t);
+extern "C" int fdvg_launch_pcosts(const fdvg_seq *, int, const unsigned *, const int *, int, unsigned *, uint64_t *,
+                                  uint64_t *, hipStream_t);
 extern "C" int fdvg_launch_pxrep(const fdvg_seq *, const int2 *, int, unsigned *, const uint64_t *, const uint64_t *,
                                  fdvg_mbinfo *, fdvg_pctx *, uint64_t *, uint64_t *, uint64_t *, hipStream_t);
 extern "C" int fdvg_pdecide_plan(const int *, const int *, const int *, int, int2 *, unsigned *);
+extern "C" int fdvg_xcd_order(const int *, const int *, const int *, int, const unsigned *, int, unsigned *, int *);
 extern "C" int fdvg_pdecide_grid(int, int, int);
-extern "C" int fdvg_launch_pdecide(const fdvg_seq *, int, const unsigned *, int, int, unsigned *, const uint64_t *,
-                                   uint64_t *, fdvg_mbrec *, hipStream_t);
+extern "C" int fdvg_launch_pdecide(const fdvg_seq *, int, const unsigned *, const int *, int, unsigned *,
+                                   const uint64_t *, uint64_t *, fdvg_mbrec *, hipStream_t);
+// the control words of a frame step: [0] row ticket, [1] error flag, [3..7]
+// kernel tickets, [8..15] and [16..23] the per-XCD tickets of the cost and
+// coding kernels
+#define CTL_WORDS 24
 #define PC_WORDS 34                     // a MB's candidate costs as tagged words
 #define EW_WORDS 24                     // a MB's candidates' reconstruction edges as tagged words
 #define BW_WORDS 4                      // a decided MB's bottom reconstruction row, tagged
@@ -100,7 +105,7 @@ struct fdvgpu {
     hipStream_t st;
     fdvg_seq *seq_d, *seq_h;        // seq_h: RING staging copies, one per frame in flight
     int2 *rowtab_d;
-    unsigned *ticket_d;
+    unsigned *ticket_d;             // CTL_WORDS control words (tickets, error flag)
     fdvg_mbrec *recs_d;
     uint64_t *hand_d;
     fdvgpu_mbinfo *info_d, *given_d;
@@ -115,7 +120,9 @@ struct fdvgpu {
     uint64_t *pcw_d;                // per MB: the candidate costs as tagged words
     uint64_t *edgew_d;              // per MB: the coded candidates' bottom rows and right columns, tagged
     uint64_t *botw_d;               // per MB: the decided reconstruction's bottom row, tagged (for the row below)
-    unsigned *corder_d;             // the MBs in the order the replay decides them
+    unsigned *xorder_d;             // per XCD: the MBs of its rows, in the order the replays decide them
+    int *xoff_d;                    // list k of xorder_d: xoff_d[k] .. xoff_d[k + 1]
+    int xmax;                       // the longest list
     fdvg_pctx *pctx_d;
     fdvg_mbinfo *pgiven_d;
     uint64_t *mrec_d;               // motion-record handoff of the decision wavefront
@@ -199,7 +206,7 @@ extern "C" void fdvgpu_close(fdvgpu *g) {
     if (g->st2) (void)hipStreamDestroy(g->st2);
     if (g->ev_fork) (void)hipEventDestroy(g->ev_fork);
     if (g->ev_join) (void)hipEventDestroy(g->ev_join);
-    (void)hipFree(g->units_d); (void)hipFree(g->corder_d); (void)hipFree(g->decw_d);
+    (void)hipFree(g->units_d); (void)hipFree(g->xorder_d); (void)hipFree(g->xoff_d); (void)hipFree(g->decw_d);
     (void)hipFree(g->edgew_d); (void)hipFree(g->botw_d);
     for (int k = 0; k < 2; ++k) {
         if (g->pipe_up[k]) (void)hipHostFree(g->pipe_up[k]);
@@ -262,7 +269,7 @@ extern "C" fdvgpu *fdvgpu_open(int nseq, const int *w, const int *h, int nsrc) {
     CKN(hipMalloc(&g->seq_d, nseq * sizeof(fdvg_seq)));
     CKN(hipHostMalloc(&g->seq_h, 8 * nseq * sizeof(fdvg_seq), hipHostMallocDefault));
     CKN(hipMalloc(&g->rowtab_d, g->nrows * sizeof(int2)));
-    CKN(hipMalloc(&g->ticket_d, 64));
+    CKN(hipMalloc(&g->ticket_d, CTL_WORDS * sizeof(unsigned)));
     CKN(hipMalloc(&g->recs_d, (size_t)g->total_mb * sizeof(fdvg_mbrec)));
     CKN(hipMalloc(&g->hand_d, (size_t)g->total_mb * FDVG_HWORDS * sizeof(uint64_t)));
     CKN(hipMemset(g->hand_d, 0, (size_t)g->total_mb * FDVG_HWORDS * sizeof(uint64_t)));
@@ -298,17 +305,22 @@ extern "C" fdvgpu *fdvgpu_open(int nseq, const int *w, const int *h, int nsrc) {
         // the block-parallel order and the replay units (one a MB row)
         int *wv = (int *)malloc(3 * nseq * sizeof(int));
         int2 *units_h = (int2 *)malloc(((size_t)g->nrows + nseq) * sizeof(int2));
-        unsigned *corder_h = (unsigned *)malloc((size_t)g->total_mb * sizeof(unsigned));
+        unsigned *corder_h = (unsigned *)malloc(2 * (size_t)g->total_mb * sizeof(unsigned));
+        int xoff_h[9];
         if (!wv || !units_h || !corder_h) { free(wv); free(units_h); free(corder_h); seterr("out of host memory"); goto fail; }
         for (int k = 0; k < nseq; ++k) { wv[k] = g->S[k].w; wv[nseq + k] = g->S[k].h; wv[2 * nseq + k] = g->S[k].mb0; }
         g->nunit = fdvg_pdecide_plan(wv, wv + nseq, wv + 2 * nseq, nseq, units_h, corder_h);
+        unsigned *xorder_h = corder_h + g->total_mb;
+        g->xmax = g->nunit > 0 ? fdvg_xcd_order(wv, wv + nseq, wv + 2 * nseq, nseq, corder_h, g->total_mb, xorder_h, xoff_h) : 0;
         // side by side only where the workers leave room for every replay
         g->overlap_avail = g->nunit > 0 && fdvg_pdecide_grid(g->nunit, g->total_mb, -1) > 0;
         g->overlap = g->overlap_avail ? 2 : 1;
         hipError_t e = g->nunit <= 0 ? hipErrorInvalidValue : hipSuccess;
-        if (e == hipSuccess) e = hipMalloc(&g->corder_d, (size_t)g->total_mb * sizeof(unsigned));
+        if (e == hipSuccess) e = hipMalloc(&g->xorder_d, (size_t)g->total_mb * sizeof(unsigned));
         if (e == hipSuccess)
-            e = hipMemcpy(g->corder_d, corder_h, (size_t)g->total_mb * sizeof(unsigned), hipMemcpyHostToDevice);
+            e = hipMemcpy(g->xorder_d, xorder_h, (size_t)g->total_mb * sizeof(unsigned), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMalloc(&g->xoff_d, 9 * sizeof(int));
+        if (e == hipSuccess) e = hipMemcpy(g->xoff_d, xoff_h, 9 * sizeof(int), hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipMalloc(&g->units_d, (size_t)g->nunit * sizeof(int2));
         if (e == hipSuccess) e = hipMalloc(&g->decw_d, (size_t)g->total_mb * PD_WORDS * sizeof(uint64_t));
         if (e == hipSuccess) e = hipMemset(g->decw_d, 0, (size_t)g->total_mb * PD_WORDS * sizeof(uint64_t));
@@ -425,7 +437,7 @@ static int launch_stage(fdvgpu *g, int stage, int me_mode, bool par) {
                               g->nn_thr, g->nn_sthr, g->st) != 0) {
             seterr("motion table launch failed"); return -1;
         }
-        if (fdvg_launch_zero(g->ticket_d, 8, g->st) != 0) { seterr("ticket reset launch failed"); return -1; }
+        if (fdvg_launch_zero(g->ticket_d, CTL_WORDS, g->st) != 0) { seterr("ticket reset launch failed"); return -1; }
     } else if (stage == 1) {
         if (par && g->overlap == 2) {
             // the costs on this stream, the replays beside them on the second
@@ -439,11 +451,11 @@ static int launch_stage(fdvgpu *g, int stage, int me_mode, bool par) {
             if (wgrid <= 0) { seterr("parallel analysis: no launch shape"); return -1; }
             CK(hipEventRecord(g->ev_fork, g->st));
             CK(hipStreamWaitEvent(g->st2, g->ev_fork, 0));
-            if (fdvg_launch_pcosts(g->seq_d, g->nseq, g->corder_d, g->total_mb, g->ticket_d, g->pcw_d, g->edgew_d,
-                                   g->st) != 0 ||
+            if (fdvg_launch_pcosts(g->seq_d, g->nseq, g->xorder_d, g->xoff_d, g->xmax, g->ticket_d, g->pcw_d,
+                                   g->edgew_d, g->st) != 0 ||
                 fdvg_launch_pxrep(g->seq_d, g->units_d, g->nunit, g->ticket_d, g->pcw_d, g->edgew_d, g->pgiven_d,
                                   g->pctx_d, g->mrec_d, g->botw_d, g->decw_d, g->st2) != 0 ||
-                fdvg_launch_pdecide(g->seq_d, g->nseq, g->corder_d, g->total_mb, wgrid, g->ticket_d, g->decw_d,
+                fdvg_launch_pdecide(g->seq_d, g->nseq, g->xorder_d, g->xoff_d, wgrid, g->ticket_d, g->decw_d,
                                     g->ih_d, g->recs_d, g->st) != 0) {
                 seterr("parallel analysis launch failed: %s", hipGetErrorString(hipGetLastError())); return -1;
             }
@@ -453,18 +465,18 @@ static int launch_stage(fdvgpu *g, int stage, int me_mode, bool par) {
             // the costs, the replays, then the coding workers, one stream
             const int dgrid = fdvg_pdecide_grid(g->nunit, g->total_mb, g->tune.pd_wpc);
             if (dgrid <= 0 ||
-                fdvg_launch_pcosts(g->seq_d, g->nseq, g->corder_d, g->total_mb, g->ticket_d, g->pcw_d, g->edgew_d,
-                                   g->st) != 0 ||
+                fdvg_launch_pcosts(g->seq_d, g->nseq, g->xorder_d, g->xoff_d, g->xmax, g->ticket_d, g->pcw_d,
+                                   g->edgew_d, g->st) != 0 ||
                 fdvg_launch_pxrep(g->seq_d, g->units_d, g->nunit, g->ticket_d, g->pcw_d, g->edgew_d, g->pgiven_d,
                                   g->pctx_d, g->mrec_d, g->botw_d, g->decw_d, g->st) != 0 ||
-                fdvg_launch_pdecide(g->seq_d, g->nseq, g->corder_d, g->total_mb, dgrid, g->ticket_d, g->decw_d,
+                fdvg_launch_pdecide(g->seq_d, g->nseq, g->xorder_d, g->xoff_d, dgrid, g->ticket_d, g->decw_d,
                                     g->ih_d, g->recs_d, g->st) != 0) {
                 seterr("parallel analysis launch failed: %s", hipGetErrorString(hipGetLastError())); return -1;
             }
         } else if (par) {
             if (fdvg_launch_zero(g->ilist_d, 1, g->st) != 0 ||
-                fdvg_launch_parallel(g->seq_d, g->nseq, g->xgrid, g->units_d, g->nunit, g->corder_d, g->ticket_d,
-                                     g->pcw_d, g->edgew_d, g->pgiven_d, g->pctx_d, g->mrec_d, g->botw_d, g->decw_d,
+                fdvg_launch_parallel(g->seq_d, g->nseq, g->xgrid, g->units_d, g->nunit, g->xorder_d, g->xoff_d,
+                                     g->xmax, g->ticket_d, g->pcw_d, g->edgew_d, g->pgiven_d, g->pctx_d, g->mrec_d, g->botw_d, g->decw_d,
                                      g->ilist_d, g->ih_d, g->total_mb, g->recs_d, g->st) != 0) {
                 seterr("parallel analysis launch failed: %s", hipGetErrorString(hipGetLastError())); return -1;
             }
@@ -583,7 +595,7 @@ extern "C" int fdvgpu_pframe(fdvgpu *g, const int *src_slot, const int *qp, cons
     g->keep_tables = 0;
     for (int stage = 0; stage < 4; ++stage) {
         if (stage == 0 && reuse) {
-            if (fdvg_launch_zero(g->ticket_d, 8, g->st) != 0) { seterr("ticket reset launch failed"); return -1; }
+            if (fdvg_launch_zero(g->ticket_d, CTL_WORDS, g->st) != 0) { seterr("ticket reset launch failed"); return -1; }
         } else if (run_stage(g, stage, me_mode, par) != 0) return -1;
         CK(hipEventRecord(g->ev[stage + 1], g->st));
     }
@@ -647,7 +659,7 @@ extern "C" int fdvgpu_decode(fdvgpu *g, const fdvgpu_mbcode *const *codes, const
             seterr("fdvgpu_decode: upload failed"); goto out;
         }
     }
-    if (fdvg_launch_zero(g->ticket_d, 8, g->st) != 0 ||
+    if (fdvg_launch_zero(g->ticket_d, CTL_WORDS, g->st) != 0 ||
         fdvg_launch_decode(g->seq_d, g->nseq, g->xgrid, (const fdvg_mbinfo *)g->pgiven_d, g->total_mb, g->recs_d,
                            g->ticket_d, g->ilist_d, g->ih_d, g->st) != 0) {
         seterr("fdvgpu_decode: launch failed: %s", hipGetErrorString(hipGetLastError())); goto out;

@@ -6,8 +6,11 @@ workgroup that has not started only if:
 - every MB's left, top and top-left neighbours come earlier in the cost and
   coding order (the replay decides in that order, an intra MB waits for their
   reconstruction);
-and it codes every MB exactly once.  Checked here on the measured workloads'
-shapes and on ragged ones, with a planted mistake that the check must catch."""
+and it codes every MB exactly once.  The cost and coding kernels take their
+MBs from eight per-XCD lists (fdvg_xcd_order): the batch's MB rows (sequence
+after sequence) are dealt to the lists in turn, each list keeps the order
+above, and together they hold every MB once.  Checked here on the measured workloads' shapes and on
+ragged ones, with planted mistakes that the checks must catch."""
 import ctypes as C
 
 import numpy as np
@@ -37,6 +40,34 @@ def plan(sizes):
     corder = (C.c_uint * tot)()
     nunit = f(w, h, (C.c_int * n)(*mb0), n, units, corder)
     return nunit, [(u.x, u.y) for u in units[:max(nunit, 0)]], np.array(corder[:], dtype=np.int64), mb0
+
+
+def xcd_lists(sizes, corder, mb0):
+    lib = G.lib()
+    f = lib.fdvg_xcd_order
+    f.restype = C.c_int
+    n, tot = len(sizes), len(corder)
+    xorder = (C.c_uint * tot)()
+    xoff = (C.c_int * 9)()
+    xmax = f((C.c_int * n)(*[s[0] for s in sizes]), (C.c_int * n)(*[s[1] for s in sizes]), (C.c_int * n)(*mb0), n,
+             (C.c_uint * tot)(*corder.tolist()), tot, xorder, xoff)
+    xo = np.array(xorder[:], dtype=np.int64)
+    return xmax, [xo[xoff[k]:xoff[k + 1]] for k in range(8)]
+
+
+def check_xcd(sizes, corder, mb0, xmax, lists):
+    tot = len(corder)
+    assert sorted(np.concatenate(lists).tolist()) == list(range(tot))
+    assert xmax == max(len(x) for x in lists)
+    at = np.empty(tot, dtype=np.int64)
+    at[corder] = np.arange(tot)
+    rowbase = np.cumsum([0] + [sh // 16 for _, sh in sizes])
+    for k, lst in enumerate(lists):
+        assert (np.diff(at[lst]) > 0).all(), "a list out of the replays' order"
+        for mb in lst.tolist():
+            j = max(j for j in range(len(sizes)) if mb >= mb0[j])
+            mw = sizes[j][0] // 16
+            assert (rowbase[j] + (mb - mb0[j]) // mw) % 8 == k, "an MB in another row's list"
 
 
 def check(sizes, nunit, units, corder, mb0):
@@ -85,6 +116,21 @@ def test_plan_orders_every_dependency_first(name):
     nunit, units, corder, mb0 = plan(sizes)
     assert nunit == len(units) > 0
     check(sizes, nunit, units, corder, mb0)
+    check_xcd(sizes, corder, mb0, *xcd_lists(sizes, corder, mb0))
+
+
+def test_xcd_check_catches_misplaced_and_reordered_mbs():
+    sizes = CASES["720p eight bands"]
+    _, _, corder, mb0 = plan(sizes)
+    xmax, lists = xcd_lists(sizes, corder, mb0)
+    swapped = [x.copy() for x in lists]
+    swapped[3][[0, 1]] = swapped[3][[1, 0]]
+    with pytest.raises(AssertionError):
+        check_xcd(sizes, corder, mb0, xmax, swapped)
+    moved = [x.copy() for x in lists]
+    moved[0], moved[1] = np.append(moved[0], moved[1][-1]), moved[1][:-1]
+    with pytest.raises(AssertionError):
+        check_xcd(sizes, corder, mb0, max(len(x) for x in moved), moved)
 
 
 def test_plan_check_catches_a_swapped_pair():
