@@ -124,52 +124,63 @@ k_me_exhaustive(const fdvg_seq *__restrict__ seqs, fdvg_metab *__restrict__ out,
     for (int b = 0; b < 6; ++b) best[b] = ~0ull;
     const fdvg_nnout *pr = nn ? &nn[q.mb0 + i] : nullptr;
     const bool cand = pr && pr->conf >= thr;
-    // candidate mode, 290 points in two passes: point t < 108 is point t % 9
-    // around vector t / 9 (the network's 8, the map's 4) on the near reference
-    // (and the quadrants), 108 <= t < 180 the same around the quadrants' 8, t =
-    // 180 the zero vector; 181 <= t < 289 the first 12 again, doubled, on the far
-    // reference, t = 289 its zero vector
-    for (int it = 0; cand && it < 2; ++it) {
-        int dx, dy, rlo = 0, rhi = nr;
-        {
-            const int t = tid + 256 * it, far = t >= 181, u = far ? t - 181 : t;
-            if (t >= 290 || (far && nr < 2)) continue;
-            const int c = u / 9, o = u % 9, sc = far ? 2 : 1;
-            const int cq = c >= 12 && c < 20 ? c - 12 : 0;
-            const int vx = c < 8 ? pr->cx[c < 8 ? c : 0] : c < 12 ? pr->kx[c < 12 ? c - 8 : 0] : pr->qx[cq >> 1][cq & 1];
-            const int vy = c < 8 ? pr->cy[c < 8 ? c : 0] : c < 12 ? pr->ky[c < 12 ? c - 8 : 0] : pr->qy[cq >> 1][cq & 1];
-            // the zero vector, so that every search has a point in range
-            const bool zero = far ? u == 108 : u == 180;
-            dx = zero ? 0 : sc * vx + o % 3 - 1;
-            dy = zero ? 0 : sc * vy + o / 3 - 1;
-            if (dx < -16 || dx > 16 || dy < -16 || dy > 16) continue;
-            rlo = far; rhi = far + 1;
-        }
-        const int pidx = (dy + 16) * 33 + dx + 16;
-        const unsigned long long tie = ((unsigned long long)(abs(dx) + abs(dy)) << 16) | (unsigned)pidx;
+    // candidate mode: 98 rows of 3x3 whole-pel neighbourhoods, two threads a
+    // row (source rows 0-7 | 8-15).  Row j < 60 is row j % 3 of the
+    // neighbourhood of near-reference vector j / 3 (the network's 8, the map's
+    // 4, the quadrant maps' 8), j = 60 the zero vector; 61 <= j < 97 the first
+    // 12 doubled on the far reference, j = 97 its zero vector.  A row scores the
+    // 4 offsets dx0 .. dx0 + 3 with v_qsad_pk_u16_u8 on byte-aligned window
+    // words and keeps those of the neighbourhood (the first 3; the zero vector:
+    // the first) inside +-16: the candidate search's 290 points, same keys.
+    if (cand) {
+        const int j = tid >> 1, hf = tid & 1, far = j >= 61, u = far ? j - 61 : j;
+        const bool zero = far ? u == 36 : u == 60;
+        const int c = zero ? 0 : u / 3, sc = far ? 2 : 1;
+        const int cq = c >= 12 && c < 20 ? c - 12 : 0;
+        const int vx = c < 8 ? pr->cx[c < 8 ? c : 0] : c < 12 ? pr->kx[c < 12 ? c - 8 : 0] : pr->qx[cq >> 1][cq & 1];
+        const int vy = c < 8 ? pr->cy[c < 8 ? c : 0] : c < 12 ? pr->ky[c < 12 ? c - 8 : 0] : pr->qy[cq >> 1][cq & 1];
+        const int dy = zero ? 0 : sc * vy + u % 3 - 1, dx0 = zero ? 0 : sc * vx - 1, nv = zero ? 1 : 3;
+        // in range: window columns WOFF + dx0 .. + 18 (>= 6, <= 58) and rows WOFF + dy .. + 15
+        const bool live = j < 98 && !(far && nr < 2) && dy >= -16 && dy <= 16 && dx0 + nv - 1 >= -16 && dx0 <= 16;
+        uint64_t acc[2] = {0, 0};
+        if (live) {
+            const int col = WOFF + dx0, sh = col & 3;
+            const uint32_t *w = (const uint32_t *)(s.win[far] + (WOFF + dy + 8 * hf) * WIN) + (col >> 2);
 #pragma unroll
-        for (int r = 0; r < 2; ++r) {
-            if (r >= rhi) break;
-            if (r < rlo) continue;
-            const uint8_t *w = s.win[r];
-            uint32_t qs[4] = {0, 0, 0, 0};
-#pragma unroll
-            for (int y = 0; y < 16; ++y) {
-                const int wy = WOFF + y + dy, wx = WOFF + dx;
-                const uint32_t r0 = win_word(w, wy, wx), r1 = win_word(w, wy, wx + 4);
-                const uint32_t r2 = win_word(w, wy, wx + 8), r3 = win_word(w, wy, wx + 12);
-                const int qr = (y >> 3) * 2;
-                qs[qr] = __builtin_amdgcn_sad_u8(s.cur32[y * 4 + 0], r0, qs[qr]);
-                qs[qr] = __builtin_amdgcn_sad_u8(s.cur32[y * 4 + 1], r1, qs[qr]);
-                qs[qr + 1] = __builtin_amdgcn_sad_u8(s.cur32[y * 4 + 2], r2, qs[qr + 1]);
-                qs[qr + 1] = __builtin_amdgcn_sad_u8(s.cur32[y * 4 + 3], r3, qs[qr + 1]);
+            for (int y = 0; y < 8; ++y) {
+                const uint32_t *r = w + y * (WIN / 4);
+                const uint32_t r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3], r4 = r[4], r5 = r[5];
+                const uint32_t s0 = __builtin_amdgcn_alignbyte(r1, r0, sh), s1 = __builtin_amdgcn_alignbyte(r2, r1, sh);
+                const uint32_t s2 = __builtin_amdgcn_alignbyte(r3, r2, sh), s3 = __builtin_amdgcn_alignbyte(r4, r3, sh);
+                const uint32_t s4 = __builtin_amdgcn_alignbyte(r5, r4, sh);
+                const uint32_t *cr = s.cur32 + (8 * hf + y) * 4;
+                acc[0] = __builtin_amdgcn_qsad_pk_u16_u8((uint64_t)s1 << 32 | s0, cr[0], acc[0]);
+                acc[0] = __builtin_amdgcn_qsad_pk_u16_u8((uint64_t)s2 << 32 | s1, cr[1], acc[0]);
+                acc[1] = __builtin_amdgcn_qsad_pk_u16_u8((uint64_t)s3 << 32 | s2, cr[2], acc[1]);
+                acc[1] = __builtin_amdgcn_qsad_pk_u16_u8((uint64_t)s4 << 32 | s3, cr[3], acc[1]);
             }
-            const unsigned s16 = qs[0] + qs[1] + qs[2] + qs[3];
-            best[r] = umin64(best[r], ((unsigned long long)s16 << 32) | tie);
-            if (r == 0)
+        }
+        // the partner's half: quadrants 0, 1 (top) | 2, 3 (bottom)
+        const uint64_t o0 = __shfl_xor(acc[0], 1), o1 = __shfl_xor(acc[1], 1);
+        const uint64_t qa[4] = {hf ? o0 : acc[0], hf ? o1 : acc[1], hf ? acc[0] : o0, hf ? acc[1] : o1};
+        // this lane keeps offsets hf and hf + 2
 #pragma unroll
-                for (int qd = 0; qd < 4; ++qd)
-                    best[2 + qd] = umin64(best[2 + qd], ((unsigned long long)qs[qd] << 32) | tie);
+        for (int k = 0; k < 2; ++k) {
+            const int o = hf + 2 * k, dx = dx0 + o;
+            if (!live || o >= nv || dx < -16 || dx > 16) continue;
+            const int pidx = (dy + 16) * 33 + dx + 16;
+            const unsigned long long tie = ((unsigned long long)(abs(dx) + abs(dy)) << 16) | (unsigned)pidx;
+            unsigned qs[4];
+#pragma unroll
+            for (int qd = 0; qd < 4; ++qd) qs[qd] = (unsigned)(qa[qd] >> (16 * o)) & 0xffffu;
+            const unsigned long long k16 = ((unsigned long long)(qs[0] + qs[1] + qs[2] + qs[3]) << 32) | tie;
+            if (!far) {
+                best[0] = umin64(best[0], k16);
+#pragma unroll
+                for (int qd = 0; qd < 4; ++qd) best[2 + qd] = umin64(best[2 + qd], ((unsigned long long)qs[qd] << 32) | tie);
+            } else {
+                best[1] = umin64(best[1], k16);
+            }
         }
     }
     // full search: an item is (reference, dy, 4 consecutive dx from an

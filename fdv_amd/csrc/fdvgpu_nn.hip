@@ -447,19 +447,22 @@ extern "C" __global__ void k_nn_adam(float *__restrict__ W, float *__restrict__ 
 
 // The inference path: one wavefront per MB, four per workgroup, no
 // workgroup barriers after the weights are staged.
-//  - conv2 and conv3 run on v_mfma_f32_32x32x16_bf16 (K = 16: half a tap's 32
-//    input channels per step, 18 steps).  All ten 32-position tiles accumulate
-//    at once (160 accumulator registers), so every B operand is read once per
-//    step and the wave's single activation buffer is rewritten in place after
-//    the last step.
-//  - B operands: both layers' weights in bf16, stored in LDS in lane order
-//    (step, lane) -> 16 bytes, shared by the workgroup's four waves.
-//  - A operands: rows of NI_RS bf16 (80 bytes: eight consecutive rows cover
-//    the 32 banks once for 16-byte reads); row NN_PT is zero (the padding of
-//    the 3x3 taps).
-//  - The linear layer is fused into the conv3 epilogue: per tile a
-//    reduce-scatter over the 32 lanes of each half-wave leaves every position's
-//    logit on two lanes (fp32 throughout).
+//  - The convolutions run on v_mfma_f32_32x32x16_bf16 in transposed form: a
+//    tile is (output channels) x (32 positions), weights as the A operand and
+//    activations as the B operand, so that a lane's accumulators hold 16
+//    channels of one position (lane l: position 32 t + l % 32, channels
+//    8 q + 4 (l / 32) + e for accumulator 4 q + e).  The epilogues then store
+//    four channels (8 bytes) at a time and the linear layer is an in-lane sum
+//    plus one exchange between the wave's halves.
+//  - conv2 and conv3: K = 16 (half a tap's 32 input channels) per step, 18
+//    steps.  All ten 32-position tiles accumulate at once (160 accumulator
+//    registers), so every weight operand is read once per step and the wave's
+//    single activation buffer is rewritten in place after the last step.
+//  - Weight operands: both layers' weights in bf16, stored in LDS in lane
+//    order (step, lane) -> 16 bytes, shared by the workgroup's four waves.
+//  - Activation operands: rows of NI_RS bf16 (80 bytes: eight consecutive rows
+//    cover the 32 banks once for 16-byte reads); the zero rows and column of
+//    the padded grid are the 3x3 taps' padding.
 typedef unsigned int u4v __attribute__((ext_vector_type(4)));
 #define NI_RS 40
 #define NI_WAVES 4
@@ -485,6 +488,25 @@ struct __align__(16) NiShared {
     float b1[32], b2[32], b3[32], wf[64], bf[2];
     NiWave wave[NI_WAVES];
 };
+
+// Four channels' ReLU(acc + bias) as bf16 (round to nearest even), 8 bytes.
+__device__ __forceinline__ uint2 ni_relu4(float a0, float a1, float a2, float a3, float4 b) {
+    typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
+    float v0 = a0 + b.x, v1 = a1 + b.y, v2 = a2 + b.z, v3 = a3 + b.w;
+    v0 = v0 > 0.f ? v0 : 0.f; v1 = v1 > 0.f ? v1 : 0.f; v2 = v2 > 0.f ? v2 : 0.f; v3 = v3 > 0.f ? v3 : 0.f;
+    const bf2 lo = {(__bf16)v0, (__bf16)v1}, hi = {(__bf16)v2, (__bf16)v3};
+    return make_uint2(__builtin_bit_cast(uint32_t, lo), __builtin_bit_cast(uint32_t, hi));
+}
+
+// A transposed tile's epilogue: this lane's position (padded index pad, inside
+// the map when live) gets its 16 channels' activations, four at a time.
+__device__ __forceinline__ void ni_put(uint16_t *act, int pad, bool live, const f16v &c, const float *bias, int g) {
+    if (!live) return;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        *(uint2 *)&act[pad * NI_RS + 8 * q + 4 * g] =
+            ni_relu4(c[4 * q], c[4 * q + 1], c[4 * q + 2], c[4 * q + 3], *(const float4 *)&bias[8 * q + 4 * g]);
+}
 
 __device__ __forceinline__ u4v ni_pack8(const float *f) {
     u4v r;
@@ -655,19 +677,12 @@ __device__ __forceinline__ uint64_t wave_max_u64(uint64_t k) {
 }
 #undef NN_DPP_MAX
 
-// The value of the lane whose index differs in the bits a DPP row permutation
-// swaps: 0xB1 (quad [1,0,3,2]) is xor 1, 0x4E (quad [2,3,0,1]) xor 2, 0x128
-// (row rotation by 8) xor 8.
-template <int CTRL> __device__ __forceinline__ float dpp_xor(float x) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, 0xf, 0xf, false));
-}
-
 // Per MB of every sequence: the 8 most likely whole-pel vectors (nearest
 // reference), their total probability and the SKIP probability.
 extern "C" __global__ void __launch_bounds__(256, 1)
 k_nn_infer(const fdvg_seq *__restrict__ seqs, int nseq, int total, const float *__restrict__ W,
            fdvg_nnout *__restrict__ out) {
-    extern __shared__ uint8_t dyn_nn[];
+    extern __shared__ __align__(16) uint8_t dyn_nn[];
     NiShared &S = *(NiShared *)dyn_nn;
     const int t = threadIdx.x, wv = t >> 6, l = t & 63, i32 = l & 31, g = l >> 5;
     NiWave &Wv = S.wave[wv];
@@ -734,20 +749,17 @@ k_nn_infer(const fdvg_seq *__restrict__ seqs, int nseq, int total, const float *
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         // conv1: two K steps (taps 0-3 | 4-7 by lane half, then tap 8)
+        {
 #pragma unroll
-        for (int tl = 0; tl < 10; ++tl) {
-            f16v c = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-            u4v a0, a1 = {0u, 0u, 0u, 0u};
+            for (int tl = 0; tl < 10; ++tl) {
+                f16v c = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+                u4v a0, a1 = {0u, 0u, 0u, 0u};
 #pragma unroll
-            for (int j = 0; j < 4; ++j) a0[j] = Wv.fb[pb[tl] + sh1[j]];
-            if (g == 0) a1[0] = Wv.fb[pb[tl] + NI_PW + 1];
-            c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf8, a0), __builtin_bit_cast(bf8, w1r[0]), c, 0, 0, 0);
-            c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf8, a1), __builtin_bit_cast(bf8, w1r[1]), c, 0, 0, 0);
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int pp = tl * 32 + (r >> 2) * 8 + g * 4 + (r & 3);
-                const float v = c[r] + S.b1[i32];
-                if (pp < NN_P) Wv.act[ni_pad(pp) * NI_RS + i32] = f2bf(v > 0.f ? v : 0.f);
+                for (int j = 0; j < 4; ++j) a0[j] = Wv.fb[pb[tl] + sh1[j]];
+                if (g == 0) a1[0] = Wv.fb[pb[tl] + NI_PW + 1];
+                c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf8, w1r[0]), __builtin_bit_cast(bf8, a0), c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf8, w1r[1]), __builtin_bit_cast(bf8, a1), c, 0, 0, 0);
+                ni_put(Wv.act, pb[tl], tl * 32 + i32 < NN_P, c, S.b1, g);
             }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -778,68 +790,54 @@ k_nn_infer(const fdvg_seq *__restrict__ seqs, int nseq, int total, const float *
                 __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                 for (int tl = 0; tl < 10; ++tl)
-                    acc[tl] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf8, av[tl]), b, acc[tl], 0, 0, 0);
+                    acc[tl] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b, __builtin_bit_cast(bf8, av[tl]), acc[tl], 0, 0, 0);
             }
             if (layer == 0) {
                 // every read of the buffer has completed (its values fed the MFMAs): rewrite it with conv2
 #pragma unroll
-                for (int tl = 0; tl < 10; ++tl)
-#pragma unroll
-                    for (int r = 0; r < 16; ++r) {
-                        const int pp = tl * 32 + (r >> 2) * 8 + g * 4 + (r & 3);
-                        const float v = acc[tl][r] + S.b2[i32];
-                        if (pp < NN_P) Wv.act[ni_pad(pp) * NI_RS + i32] = f2bf(v > 0.f ? v : 0.f);
-                    }
+                for (int tl = 0; tl < 10; ++tl) ni_put(Wv.act, pb[tl], tl * 32 + i32 < NN_P, acc[tl], S.b2, g);
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             }
         }
-        // conv3 epilogue + linear layer.  Reduce-scatter of the 16 per-lane values
-        // of a tile over the half-wave: after the steps xor 16, 8, 4, 2 (each
-        // halving the values a lane keeps) and xor 1, lanes i and i^1 hold the
-        // sum for r = (i >> 1) & 15.
-        const float wf0 = S.wf[i32], wf1 = S.wf[32 + i32], b3 = S.b3[i32];
+        // conv3 epilogue + linear layer: a lane sums its 16 channels of its
+        // position, the two halves of the wave (the other 16 channels) are added
+        // by one exchange; every position's logit is then on lanes i and i + 32.
         float skacc = 0.f;
         float lg[10];
-        const int rr = (i32 >> 1) & 15;
 #pragma unroll
         for (int tl = 0; tl < 10; ++tl) {
-            float v[16];
+            const bool live = tl * 32 + i32 < NN_P;
+            float lp = 0.f, sp = 0.f;
 #pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int pp = tl * 32 + (r >> 2) * 8 + g * 4 + (r & 3);
-                float a = acc[tl][r] + b3;
-                a = (pp < NN_P && a > 0.f) ? a : 0.f;
-                v[r] = a * wf0;
-                skacc += a * wf1;
-            }
+            for (int q = 0; q < 4; ++q) {
+                // this lane's channels 8 q + 4 g .. +3 (16-byte LDS reads, the same for a half-wave)
+                const float4 b = *(const float4 *)&S.b3[8 * q + 4 * g], w0 = *(const float4 *)&S.wf[8 * q + 4 * g],
+                             w1 = *(const float4 *)&S.wf[32 + 8 * q + 4 * g];
+                const float bb[4] = {b.x, b.y, b.z, b.w}, ww0[4] = {w0.x, w0.y, w0.z, w0.w},
+                            ww1[4] = {w1.x, w1.y, w1.z, w1.w};
 #pragma unroll
-            for (int lev = 0; lev < 4; ++lev) {
-                const int half = 8 >> lev, bit = (i32 >> (4 - lev)) & 1;
-#pragma unroll
-                for (int j = 0; j < half; ++j) {
-                    const float keep = bit ? v[j + half] : v[j], send = bit ? v[j] : v[j + half];
-                    // xor 8 and xor 2 stay inside a 16-lane row: exact DPP moves
-                    // (a rotation by 8 and a quad swap), the others by permute
-                    const float got = lev == 1 ? dpp_xor<0x128>(send) : lev == 3 ? dpp_xor<0x4E>(send)
-                                                                      : __shfl_xor(send, 16 >> lev, 32);
-                    v[j] = keep + got;
+                for (int e = 0; e < 4; ++e) {
+                    float a = acc[tl][4 * q + e] + bb[e];
+                    a = (live && a > 0.f) ? a : 0.f;
+                    lp += a * ww0[e];
+                    sp += a * ww1[e];
                 }
             }
-            const float sum = v[0] + dpp_xor<0xB1>(v[0]);
-            const int pp = tl * 32 + (rr >> 2) * 8 + g * 4 + (rr & 3);
-            lg[tl] = pp < NN_P ? sum + S.bf[0] : -3.0e38f;
+            skacc += sp;
+            lp += __shfl_xor(lp, 32);
+            lg[tl] = live ? lp + S.bf[0] : -3.0e38f;
         }
         for (int o = 32; o > 0; o >>= 1) skacc += __shfl_xor(skacc, o);
         const float skl = skacc * (1.0f / NN_P) + S.bf[1];
-        // softmax statistics (each position counted once: even lanes)
+        // softmax statistics (each position counted once: the first half-wave)
         float m = -3.0e38f;
 #pragma unroll
         for (int tl = 0; tl < 10; ++tl) m = fmaxf(m, lg[tl]);
         for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
         float se = 0.f;
-        if (!(i32 & 1)) {
+        if (g == 0) {
 #pragma unroll
             for (int tl = 0; tl < 10; ++tl) se += lg[tl] > -1.0e38f ? expf(lg[tl] - m) : 0.f;
         }
@@ -854,10 +852,10 @@ k_nn_infer(const fdvg_seq *__restrict__ seqs, int nseq, int total, const float *
         uint64_t key[10];
 #pragma unroll
         for (int tl = 0; tl < 10; ++tl) {
-            const int pp = tl * 32 + (rr >> 2) * 8 + g * 4 + (rr & 3);
+            const int pp = tl * 32 + i32;
             uint32_t u = __float_as_uint(lg[tl] + 0.0f);
             u = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
-            key[tl] = pp < NN_P ? ((uint64_t)u << 32) | (uint32_t)(0xffff - pp) : 0ull;
+            key[tl] = g == 0 && pp < NN_P ? ((uint64_t)u << 32) | (uint32_t)(0xffff - pp) : 0ull;
         }
 #pragma unroll
         for (int kk = 0; kk < 8; ++kk) {
