@@ -410,6 +410,7 @@ def main():
     stage_ms = {"motion_tables": round(float(kt[:, 0].mean()), 4), "decisions": round(k_ms, 4),
                 "deblock": round(float(kt[:, 2].mean()), 4), "serialise": round(float(kt[:, 3].mean()), 4)}
     shapes = an.tune()
+    structure = an.overlap() if par else None
     an.close()
     # the exhaustive search on the same frames, for the learned-vs-exhaustive comparison
     base = None
@@ -534,7 +535,7 @@ def main():
                    "sequences": nb, "coded_px_per_step": px_frame, "search": args.mode, **job["extra"]},
         "stage_ms": stage_ms,
         "launch_shapes": {"serialiser_waves": shapes[0], "deblock_threads": shapes[1], "proposer_wg_per_cu": shapes[2],
-                          "decision_workers_per_cu": shapes[3]},
+                          "decision_workers_per_cu": shapes[3], "decision_structure": structure},
         "exhaustive_mpix_s": None if base is None else round(base, 3),
         "speedup_vs_exhaustive": None if base is None else round(value / base, 3),
         "parallel_mpix_s": None if par_mpix is None else round(par_mpix, 3),

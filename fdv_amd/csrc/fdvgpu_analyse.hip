@@ -740,7 +740,7 @@ __device__ __forceinline__ int mb_count(const fdvg_seq &q) { return (q.w >> 4) *
 
 #define PC_WORDS 34                      // a MB's costs as tagged words (4 payload bytes each)
 #define EW_WORDS 24                      // a MB's candidate edges as tagged words
-#define BW_WORDS 4                       // a decided MB's bottom reconstruction row, tagged
+#define BW_WORDS 8                       // a decided MB's reconstruction edges, tagged: bottom row, right column
 static_assert(sizeof(fdvg_pcost) == 4 * PC_WORDS, "cost words");
 
 // The costs of one MB's candidates except INTRA, built in LDS (P), and the
@@ -1011,8 +1011,9 @@ __device__ __forceinline__ void pr_store(PrWave &W, PrPre &R, const fdvg_seq &q,
 // wave 0 forms the chosen candidate's bottom row and right column (INTER16 and
 // INTER8: from k_pcosts; SKIP: the prediction itself, from a tile of the
 // reference; INTRA: the trial's reconstruction): the right column is the next
-// MB's left neighbour, the bottom row goes to the row below as BW_WORDS tagged
-// words.  The trial's inputs are double-buffered by MB parity; wave 0 refills
+// MB's left neighbour, the bottom row goes to the row below as tagged words
+// (BW_WORDS a MB: 4 of the bottom row, then 4 of the right column, which the
+// coding workers read for an intra MB's luma prediction).  The trial's inputs are double-buffered by MB parity; wave 0 refills
 // a buffer only once the trial has finished the MB that used it.
 #define XR_WAVES 5
 struct __align__(16) XrLds {
@@ -1315,8 +1316,9 @@ __device__ __forceinline__ void xr_decide(XrLds &X, const fdvg_seq &q, const uin
             }
             if (L < PD_WORDS) __hip_atomic_store(decw + (size_t)mb * PD_WORDS + L, (uint64_t)step << 32 | d,
                                                  __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            const int b0 = __shfl(e, 4 * (L & 3)), b1 = __shfl(e, 4 * (L & 3) + 1);
-            const int b2 = __shfl(e, 4 * (L & 3) + 2), b3 = __shfl(e, 4 * (L & 3) + 3);
+            // lanes 0-3: the bottom row (e of lanes 0-15), 4-7: the right column (16-31)
+            const int eb = (L & 4) * 4 + 4 * (L & 3);
+            const int b0 = __shfl(e, eb), b1 = __shfl(e, eb + 1), b2 = __shfl(e, eb + 2), b3 = __shfl(e, eb + 3);
             if (L < BW_WORDS)
                 __hip_atomic_store(botw + (size_t)mb * BW_WORDS + L,
                                    (uint64_t)step << 32 | (uint32_t)(b0 | b1 << 8 | b2 << 16 | b3 << 24),
@@ -1640,6 +1642,43 @@ __device__ __forceinline__ void nb_load_ih(Smem &s, const fdvg_seq &q, const fdv
     }
 }
 
+// The neighbours of an intra MB for a coding worker of the decisions, in two
+// parts.  Luma (nb_load_rep): the top row, corner and left column of the
+// decided reconstruction from the replays' edge words (BW_WORDS a MB), which
+// never wait for a worker.  Chroma (nb_load_ihc): from the neighbours' handoff
+// words, published by the workers that coded them.  The luma can then be coded
+// as soon as the MB is decided; only the chroma waits for other workers.
+__device__ __forceinline__ void nb_load_rep(Smem &s, const fdvg_seq &q, const uint64_t *botw, unsigned *ctl, int mx,
+                                            int my, int mw) {
+    const int tid = threadIdx.x, ht = my > 0, hl = mx > 0;
+    const uint64_t *b0 = botw + (size_t)q.mb0 * BW_WORDS;
+    const size_t i_top = (size_t)(my - 1) * mw + mx, i_left = (size_t)my * mw + mx - 1, i_tl = i_top - 1;
+    if (tid < 16)
+        s.hbytes[1 + tid] = !ht ? 0 : ih_byte(b0 + i_top * BW_WORDS, tid >> 2, tid & 3, q.step, ctl);
+    else if (tid == 16)
+        s.hbytes[0] = !(ht && hl) ? 128 : ih_byte(b0 + i_tl * BW_WORDS, 3, 3, q.step, ctl);
+    else if (tid >= 32 && tid < 48)
+        s.lcol[tid - 32] = !hl ? 0 : ih_byte(b0 + i_left * BW_WORDS, 4 + ((tid - 32) >> 2), tid & 3, q.step, ctl);
+}
+__device__ __forceinline__ void nb_load_ihc(Smem &s, const fdvg_seq &q, const uint64_t *ih, unsigned *ctl, int mx,
+                                            int my, int mw) {
+    const int tid = threadIdx.x, ht = my > 0, hl = mx > 0;
+    const uint64_t *ih0 = ih + (size_t)q.mb0 * IH_WORDS;
+    const size_t i_top = (size_t)(my - 1) * mw + mx, i_left = (size_t)my * mw + mx - 1, i_tl = i_top - 1;
+    if (tid < 18) {
+        const int pl = tid / 9, k = tid % 9;
+        if (k == 0)
+            s.ctop[pl][0] = !(ht && hl) ? 128 : ih_byte(ih0 + i_tl * IH_WORDS, pl ? 7 : 5, 3, q.step, ctl);
+        else
+            s.ctop[pl][k] = !ht ? 0 : ih_byte(ih0 + i_top * IH_WORDS, (pl ? 6 : 4) + ((k - 1) >> 2), (k - 1) & 3,
+                                              q.step, ctl);
+    } else if (tid >= 32 && tid < 48) {
+        const int pl = (tid - 32) >> 3, k = (tid - 32) & 7;
+        s.lcol[16 + 8 * pl + k] = !hl ? 0 : ih_byte(ih0 + i_left * IH_WORDS, (pl ? 14 : 12) + (k >> 2), k & 3,
+                                                     q.step, ctl);
+    }
+}
+
 // Decided intra MBs, in the order of ilist (every intra neighbour first): each
 // waits for its left, top and top-left neighbours when those are intra, then
 // reads the unfiltered reconstruction around it.
@@ -1690,8 +1729,8 @@ k_pintra(const fdvg_seq *__restrict__ seqs, int nseq, const fdvg_mbinfo *__restr
 // XCD holding workers, the earliest unfinished MB is always being coded.
 extern "C" __global__ void __launch_bounds__(256)
 k_pdecide(const fdvg_seq *__restrict__ seqs, int nseq, const unsigned *__restrict__ xorder,
-          const int *__restrict__ xoff, unsigned *ctl, const uint64_t *__restrict__ decw, uint64_t *__restrict__ ih,
-          fdvg_mbrec *__restrict__ recs) {
+          const int *__restrict__ xoff, unsigned *ctl, const uint64_t *__restrict__ decw,
+          const uint64_t *__restrict__ botw, uint64_t *__restrict__ ih, fdvg_mbrec *__restrict__ recs) {
     __shared__ Smem s;
     __shared__ union {
         uint32_t w[PD_WORDS];
@@ -1727,13 +1766,18 @@ k_pdecide(const fdvg_seq *__restrict__ seqs, int nseq, const unsigned *__restric
         const fdvg_pctx &X = decb.d.x;
         const int gmode = G.mode;
         mb_stage(s, q, mb, bx, by, gmode, 1 << G.ref);
-        if (gmode == 2) nb_load_ih(s, q, nullptr, ih, ctl, mx, my, mw, bx, by);
+        if (gmode == 2) nb_load_rep(s, q, botw, ctl, mx, my, mw);
         MbCtx c;
         ctx_from(c, X, my > 0, mx > 0);
         __syncthreads();
         int par = 0;
         MbDec D;
         mb_decide(s, q, c, &G, nullptr, par, D);
+        if (gmode == 2) {
+            // the luma is coded; the chroma prediction waits for the neighbours' workers
+            nb_load_ihc(s, q, ih, ctl, mx, my, mw);
+            __syncthreads();
+        }
         const uint8_t *yrec = mb_finish(s, q, mb, bx, by, c, D, &recs[mb]);
         __syncthreads();                       // the chroma reconstruction is complete
         ih_publish(ih + (size_t)mb * IH_WORDS, yrec, s.crec, q.step);
@@ -1826,9 +1870,9 @@ extern "C" int fdvg_pdecide_grid(int nunit, int total_mb, int workers_per_cu) {
 }
 
 extern "C" int fdvg_launch_pdecide(const fdvg_seq *seqs, int nseq, const unsigned *xorder, const int *xoff, int grid,
-                                   unsigned *ctl, const uint64_t *decw, uint64_t *ih, fdvg_mbrec *recs,
-                                   hipStream_t st) {
-    hipLaunchKernelGGL(k_pdecide, dim3(grid), dim3(256), 0, st, seqs, nseq, xorder, xoff, ctl, decw, ih, recs);
+                                   unsigned *ctl, const uint64_t *decw, const uint64_t *botw, uint64_t *ih,
+                                   fdvg_mbrec *recs, hipStream_t st) {
+    hipLaunchKernelGGL(k_pdecide, dim3(grid), dim3(256), 0, st, seqs, nseq, xorder, xoff, ctl, decw, botw, ih, recs);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -57,14 +57,14 @@ extern "C" int fdvg_pdecide_plan(const int *, const int *, const int *, int, int
 extern "C" int fdvg_xcd_order(const int *, const int *, const int *, int, const unsigned *, int, unsigned *, int *);
 extern "C" int fdvg_pdecide_grid(int, int, int);
 extern "C" int fdvg_launch_pdecide(const fdvg_seq *, int, const unsigned *, const int *, int, unsigned *,
-                                   const uint64_t *, uint64_t *, fdvg_mbrec *, hipStream_t);
+                                   const uint64_t *, const uint64_t *, uint64_t *, fdvg_mbrec *, hipStream_t);
 // the control words of a frame step: [0] row ticket, [1] error flag, [3..7]
 // kernel tickets, [8..15] and [16..23] the per-XCD tickets of the cost and
 // coding kernels
 #define CTL_WORDS 24
 #define PC_WORDS 34                     // a MB's candidate costs as tagged words
 #define EW_WORDS 24                     // a MB's candidates' reconstruction edges as tagged words
-#define BW_WORDS 4                      // a decided MB's bottom reconstruction row, tagged
+#define BW_WORDS 8                      // a decided MB's reconstruction edges, tagged: bottom row, right column
 #define PD_WORDS 10                     // a MB's decision and context as tagged words
 
 static thread_local char g_err[256];
@@ -138,6 +138,7 @@ struct fdvgpu {
     uint64_t *decw_d;               // per MB: the decision and its context as tagged words
     int nunit;
     int overlap, overlap_avail;     // the decision structure; whether 2 fits beside the replays
+    int overlap_fixed;              // set by the caller: the autotuner keeps it
     hipStream_t st2;
     hipEvent_t ev_fork, ev_join;
     // launch shapes, and one instantiated graph per (search mode, decision structure)
@@ -456,7 +457,7 @@ static int launch_stage(fdvgpu *g, int stage, int me_mode, bool par) {
                 fdvg_launch_pxrep(g->seq_d, g->units_d, g->nunit, g->ticket_d, g->pcw_d, g->edgew_d, g->pgiven_d,
                                   g->pctx_d, g->mrec_d, g->botw_d, g->decw_d, g->st2) != 0 ||
                 fdvg_launch_pdecide(g->seq_d, g->nseq, g->xorder_d, g->xoff_d, wgrid, g->ticket_d, g->decw_d,
-                                    g->ih_d, g->recs_d, g->st) != 0) {
+                                    g->botw_d, g->ih_d, g->recs_d, g->st) != 0) {
                 seterr("parallel analysis launch failed: %s", hipGetErrorString(hipGetLastError())); return -1;
             }
             CK(hipEventRecord(g->ev_join, g->st2));
@@ -470,7 +471,7 @@ static int launch_stage(fdvgpu *g, int stage, int me_mode, bool par) {
                 fdvg_launch_pxrep(g->seq_d, g->units_d, g->nunit, g->ticket_d, g->pcw_d, g->edgew_d, g->pgiven_d,
                                   g->pctx_d, g->mrec_d, g->botw_d, g->decw_d, g->st) != 0 ||
                 fdvg_launch_pdecide(g->seq_d, g->nseq, g->xorder_d, g->xoff_d, dgrid, g->ticket_d, g->decw_d,
-                                    g->ih_d, g->recs_d, g->st) != 0) {
+                                    g->botw_d, g->ih_d, g->recs_d, g->st) != 0) {
                 seterr("parallel analysis launch failed: %s", hipGetErrorString(hipGetLastError())); return -1;
             }
         } else if (par) {
@@ -882,7 +883,13 @@ extern "C" int fdvgpu_set_overlap(fdvgpu *g, int on) {
         seterr("fdvgpu_set_overlap: the replays and the coding workers do not fit side by side"); return -1;
     }
     g->overlap = on;
+    g->overlap_fixed = 1;
     return 0;
+}
+
+extern "C" int fdvgpu_get_overlap(fdvgpu *g) {
+    if (!g) { seterr("fdvgpu_get_overlap: bad context"); return -1; }
+    return g->overlap;
 }
 
 extern "C" int fdvgpu_set_graphs(fdvgpu *g, int on) {
@@ -915,15 +922,23 @@ extern "C" int fdvgpu_set_tune(fdvgpu *g, const int in[4]) {
 
 // Each candidate launch shape times `reps` frame steps of the given frame (the
 // reference pool is rewound after each), one parameter at a time, keeping the
-// fastest; the chosen shapes stay in the context.
+// fastest; the chosen shapes, and the decision structure where two are
+// available and none was set explicitly (fdvgpu_set_overlap), stay in the
+// context.
 extern "C" int fdvgpu_autotune(fdvgpu *g, const int *src_slot, const int *qp, const double *lscale,
                                const int *navail, int me_mode, int reps) {
     if (!g || reps <= 0) { seterr("fdvgpu_autotune: bad arguments"); return -1; }
-    static const int opts[4][3] = {{2, 4, 8}, {64, 128, 256}, {1, 2, 0}, {1, 2, 0}};
+    // parameter 4, tried first: the decision structure (1: the coding workers
+    // after the replays, as many as the autotuned count; 2: beside them, as
+    // many as leave every replay a CU) where both are available
+    static const int opts[5][3] = {{2, 4, 8}, {64, 128, 256}, {1, 2, 0}, {1, 2, 0}, {2, 1, 0}};
+    static const int order[5] = {4, 0, 1, 2, 3};
     const bool par = me_mode == FDVGPU_ME_LEARNED || me_mode == FDVGPU_ME_PARALLEL;
-    for (int prm = 0; prm < 4; ++prm) {
+    for (int pi = 0; pi < 5; ++pi) {
+        const int prm = order[pi];
         if (prm == 2 && me_mode != FDVGPU_ME_LEARNED) continue;
         if (prm == 3 && !(par && g->overlap == 1)) continue;
+        if (prm == 4 && !(par && g->overlap_avail && !g->overlap_fixed)) continue;
         float best = 1e30f;
         int bestv = 0;
         for (int k = 0; k < 3 && opts[prm][k]; ++k) {
@@ -931,7 +946,8 @@ extern "C" int fdvgpu_autotune(fdvgpu *g, const int *src_slot, const int *qp, co
             if (prm == 0) t.sym_waves = opts[0][k];
             else if (prm == 1) t.deblock_bs = opts[1][k];
             else if (prm == 2) t.nn_wpc = opts[2][k];
-            else t.pd_wpc = opts[3][k];
+            else if (prm == 3) t.pd_wpc = opts[3][k];
+            else g->overlap = opts[4][k];
             g->tune = t;
             drop_graphs(g);
             float tot = 0.f;
@@ -948,7 +964,8 @@ extern "C" int fdvgpu_autotune(fdvgpu *g, const int *src_slot, const int *qp, co
         if (prm == 0) g->tune.sym_waves = bestv;
         else if (prm == 1) g->tune.deblock_bs = bestv;
         else if (prm == 2) g->tune.nn_wpc = bestv;
-        else g->tune.pd_wpc = bestv;
+        else if (prm == 3) g->tune.pd_wpc = bestv;
+        else g->overlap = bestv;
         drop_graphs(g);
     }
     g->have_frame = 0;

@@ -52,6 +52,7 @@ def _signatures():
         "fdvgpu_decisions": (i, [vp, i, vp]),
         "fdvgpu_set_graphs": (i, [vp, i]),
         "fdvgpu_set_overlap": (i, [vp, i]),
+        "fdvgpu_get_overlap": (i, [vp]),
         "fdvgpu_get_tune": (i, [vp, vp]),
         "fdvgpu_set_tune": (i, [vp, vp]),
         "fdvgpu_autotune": (i, [vp, ip, ip, dp, ip, i, i]),
@@ -232,6 +233,13 @@ class Analyser:
         them side by side; True means 2)."""
         mode = 2 if mode is True else int(mode)
         self._ck(self.L.fdvgpu_set_overlap(self.h, mode), "set_overlap")
+
+    def overlap(self):
+        """The decision structure in use (0, 1 or 2; see set_overlap)."""
+        v = self.L.fdvgpu_get_overlap(self.h)
+        if v < 0:
+            raise GpuError("get_overlap: " + self.L.fdvgpu_last_error().decode())
+        return v
 
     def set_graphs(self, on: bool):
         self._ck(self.L.fdvgpu_set_graphs(self.h, 1 if on else 0), "set_graphs")

@@ -176,14 +176,20 @@ int fdvgpu_decode(fdvgpu *g, const fdvgpu_mbcode *const *codes, const int *qp, c
 int fdvgpu_set_graphs(fdvgpu *g, int on);
 
 /* The structure of the block-parallel decision stage, all with the same
- * results: 0, the candidate costs, the decision replay and the coding of its
- * decisions as separate launches; 1, the costs, then the replay and the coding
- * in one launch, each MB coded as soon as its decision (and, for INTRA, its
- * neighbours' reconstruction) exists; 2 (the default where every row is at
- * most 256 MBs wide), all three in one launch, the replay consuming the costs
- * as they are computed.  1 and 2 return -1 where unavailable.  No counterpart
- * in the reference. */
+ * results: 0, the candidate costs, the decision replays and the coding of
+ * their decisions as separate launches; 1, the costs, the replays (one
+ * workgroup a MB row, consuming the cost words as they appear), then the
+ * coding workers, on one stream; 2, the replays on a second stream beside the
+ * costs and, after the costs, the coding workers beside the replays, each MB
+ * coded as soon as it is decided (an intra MB's chroma once its neighbours are
+ * coded).  2 is available where every replay still finds a CU beside the
+ * workers (at most 7/8 of the CUs' worth of MB rows) and is then the default;
+ * fdvgpu_autotune then keeps the faster of 1 and 2 unless the caller set the
+ * structure here.  1 and 2 return -1 where unavailable.  No counterpart in the
+ * reference. */
 int fdvgpu_set_overlap(fdvgpu *g, int mode);
+/* The decision structure in use (0, 1 or 2), or -1 for a bad context. */
+int fdvgpu_get_overlap(fdvgpu *g);
 /* shapes: serialiser wavefronts per workgroup {1..8}, deblock workgroup size
  * {64, 128, 256}, proposer workgroups per CU {1, 2}, coding workers of the
  * fused decision stage per CU {1, 2}. */
