@@ -305,9 +305,11 @@ def main():
     ap.add_argument("--end-to-end", action="store_true",
                     help="also time the drop-in encoder end to end (source upload, device analysis, "
                          "stream fetch, host entropy coding) on the same frames")
-    ap.add_argument("--stage", type=int, default=None, choices=[0, 1, 2],
-                    help="structure of the block-parallel decision stage: 0 separate launches (costs, replay, "
-                         "coding), 1 costs then replay + coding fused, 2 all fused (the default)")
+    ap.add_argument("--stage", type=int, default=None, choices=[0, 1, 2, 3],
+                    help="structure of the block-parallel decision stage: 0 separate launches (costs, replays, "
+                         "coding), 1 costs, replays, coding workers on one stream, 2 replays beside the costs and "
+                         "workers beside the replays, 3 replays beside the costs, then the workers (default: the "
+                         "autotuner's choice of 2 and 3)")
     ap.add_argument("--conf", type=float, default=None, help="proposer confidence threshold (default 0.7)")
     ap.add_argument("--skip", type=float, default=None, help="proposer SKIP threshold (default: off; 0.7 prunes)")
     ap.add_argument("--baseline-steps", type=int, default=10,

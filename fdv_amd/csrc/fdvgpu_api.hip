@@ -36,7 +36,7 @@ struct Tune {
     int sym_waves;      // serialiser MBs (wavefronts) per workgroup: 1, 2, 4, 8
     int deblock_bs;     // deblock threads per workgroup: 64, 128, 256
     int nn_wpc;         // proposer workgroups per CU: 1, 2
-    int pd_wpc;         // workers of the fused decision stage per CU: 1, 2
+    int pd_wpc;         // coding workers of the decision stage per CU: 1, 2
 };
 extern "C" int fdvg_launch_metab(const fdvg_seq *, int, int, void *, const void *, float, float, hipStream_t);
 extern "C" int fdvg_launch_nn_infer(const fdvg_seq *, int, int, const float *, fdvg_nnout *, int, hipStream_t);
@@ -315,7 +315,7 @@ extern "C" fdvgpu *fdvgpu_open(int nseq, const int *w, const int *h, int nsrc) {
         g->xmax = g->nunit > 0 ? fdvg_xcd_order(wv, wv + nseq, wv + 2 * nseq, nseq, corder_h, g->total_mb, xorder_h, xoff_h) : 0;
         // side by side only where the workers leave room for every replay
         g->overlap_avail = g->nunit > 0 && fdvg_pdecide_grid(g->nunit, g->total_mb, -1) > 0;
-        g->overlap = g->overlap_avail ? 2 : 1;
+        g->overlap = g->overlap_avail ? 2 : 3;
         hipError_t e = g->nunit <= 0 ? hipErrorInvalidValue : hipSuccess;
         if (e == hipSuccess) e = hipMalloc(&g->xorder_d, (size_t)g->total_mb * sizeof(unsigned));
         if (e == hipSuccess)
@@ -462,6 +462,27 @@ static int launch_stage(fdvgpu *g, int stage, int me_mode, bool par) {
             }
             CK(hipEventRecord(g->ev_join, g->st2));
             CK(hipStreamWaitEvent(g->st, g->ev_join, 0));
+        } else if (par && g->overlap == 3) {
+            // the replays beside the costs on the second stream (as in 2), then
+            // every coding worker after both (as many as in 1): for batches whose
+            // MB count makes the workers' throughput, not the replays' chain, the
+            // bound
+            const int dgrid = fdvg_pdecide_grid(g->nunit, g->total_mb, g->tune.pd_wpc);
+            if (dgrid <= 0) { seterr("parallel analysis: no launch shape"); return -1; }
+            CK(hipEventRecord(g->ev_fork, g->st));
+            CK(hipStreamWaitEvent(g->st2, g->ev_fork, 0));
+            if (fdvg_launch_pcosts(g->seq_d, g->nseq, g->xorder_d, g->xoff_d, g->xmax, g->ticket_d, g->pcw_d,
+                                   g->edgew_d, g->st) != 0 ||
+                fdvg_launch_pxrep(g->seq_d, g->units_d, g->nunit, g->ticket_d, g->pcw_d, g->edgew_d, g->pgiven_d,
+                                  g->pctx_d, g->mrec_d, g->botw_d, g->decw_d, g->st2) != 0) {
+                seterr("parallel analysis launch failed: %s", hipGetErrorString(hipGetLastError())); return -1;
+            }
+            CK(hipEventRecord(g->ev_join, g->st2));
+            CK(hipStreamWaitEvent(g->st, g->ev_join, 0));
+            if (fdvg_launch_pdecide(g->seq_d, g->nseq, g->xorder_d, g->xoff_d, dgrid, g->ticket_d, g->decw_d,
+                                    g->botw_d, g->ih_d, g->recs_d, g->st) != 0) {
+                seterr("parallel analysis launch failed: %s", hipGetErrorString(hipGetLastError())); return -1;
+            }
         } else if (par && g->overlap == 1) {
             // the costs, the replays, then the coding workers, one stream
             const int dgrid = fdvg_pdecide_grid(g->nunit, g->total_mb, g->tune.pd_wpc);
@@ -878,7 +899,7 @@ extern "C" int fdvgpu_planes(fdvgpu *g, int seq, int src_slot, const uint8_t **c
 }
 
 extern "C" int fdvgpu_set_overlap(fdvgpu *g, int on) {
-    if (!g || on < 0 || on > 2) { seterr("fdvgpu_set_overlap: bad arguments"); return -1; }
+    if (!g || on < 0 || on > 3) { seterr("fdvgpu_set_overlap: bad arguments"); return -1; }
     if (on == 2 && !g->overlap_avail) {
         seterr("fdvgpu_set_overlap: the replays and the coding workers do not fit side by side"); return -1;
     }
@@ -928,17 +949,17 @@ extern "C" int fdvgpu_set_tune(fdvgpu *g, const int in[4]) {
 extern "C" int fdvgpu_autotune(fdvgpu *g, const int *src_slot, const int *qp, const double *lscale,
                                const int *navail, int me_mode, int reps) {
     if (!g || reps <= 0) { seterr("fdvgpu_autotune: bad arguments"); return -1; }
-    // parameter 4, tried first: the decision structure (1: the coding workers
-    // after the replays, as many as the autotuned count; 2: beside them, as
-    // many as leave every replay a CU) where both are available
-    static const int opts[5][3] = {{2, 4, 8}, {64, 128, 256}, {1, 2, 0}, {1, 2, 0}, {2, 1, 0}};
+    // parameter 4, tried first: the decision structure (2: the coding workers
+    // beside the replays, as many as leave every replay a CU, where that is
+    // available; 3: every worker after the replays, which run beside the costs)
+    static const int opts[5][3] = {{2, 4, 8}, {64, 128, 256}, {1, 2, 0}, {1, 2, 0}, {2, 3, 0}};
     static const int order[5] = {4, 0, 1, 2, 3};
     const bool par = me_mode == FDVGPU_ME_LEARNED || me_mode == FDVGPU_ME_PARALLEL;
     for (int pi = 0; pi < 5; ++pi) {
         const int prm = order[pi];
         if (prm == 2 && me_mode != FDVGPU_ME_LEARNED) continue;
-        if (prm == 3 && !(par && g->overlap == 1)) continue;
-        if (prm == 4 && !(par && g->overlap_avail && !g->overlap_fixed)) continue;
+        if (prm == 3 && !(par && (g->overlap == 1 || g->overlap == 3))) continue;
+        if (prm == 4 && !(par && !g->overlap_fixed)) continue;
         float best = 1e30f;
         int bestv = 0;
         for (int k = 0; k < 3 && opts[prm][k]; ++k) {
@@ -947,6 +968,7 @@ extern "C" int fdvgpu_autotune(fdvgpu *g, const int *src_slot, const int *qp, co
             else if (prm == 1) t.deblock_bs = opts[1][k];
             else if (prm == 2) t.nn_wpc = opts[2][k];
             else if (prm == 3) t.pd_wpc = opts[3][k];
+            else if (opts[4][k] == 2 && !g->overlap_avail) continue;
             else g->overlap = opts[4][k];
             g->tune = t;
             drop_graphs(g);

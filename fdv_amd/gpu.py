@@ -230,7 +230,8 @@ class Analyser:
         (costs, replays, coding kernels); 1 the costs, the replays, then the
         coding workers on one stream; 2 the replays beside the costs and the
         coding workers beside the replays (the default where the device holds
-        them side by side; True means 2)."""
+        them side by side; True means 2); 3 the replays beside the costs, then
+        the coding workers.  A structure set here is kept by autotune()."""
         mode = 2 if mode is True else int(mode)
         self._ck(self.L.fdvgpu_set_overlap(self.h, mode), "set_overlap")
 

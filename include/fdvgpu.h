@@ -182,11 +182,12 @@ int fdvgpu_set_graphs(fdvgpu *g, int on);
  * coding workers, on one stream; 2, the replays on a second stream beside the
  * costs and, after the costs, the coding workers beside the replays, each MB
  * coded as soon as it is decided (an intra MB's chroma once its neighbours are
- * coded).  2 is available where every replay still finds a CU beside the
- * workers (at most 7/8 of the CUs' worth of MB rows) and is then the default;
- * fdvgpu_autotune then keeps the faster of 1 and 2 unless the caller set the
- * structure here.  1 and 2 return -1 where unavailable.  No counterpart in the
- * reference. */
+ * coded); 3, the replays beside the costs as in 2, then as many coding workers
+ * as in 1 once the replays are done.  2 is available where every replay still
+ * finds a CU beside the workers (at most 7/8 of the CUs' worth of MB rows) and
+ * is then the default, 3 otherwise; fdvgpu_autotune keeps the faster of 2 and
+ * 3 unless the caller set the structure here.  2 returns -1 where unavailable.
+ * No counterpart in the reference. */
 int fdvgpu_set_overlap(fdvgpu *g, int mode);
 /* The decision structure in use (0, 1 or 2), or -1 for a bad context. */
 int fdvgpu_get_overlap(fdvgpu *g);
