@@ -484,7 +484,8 @@ struct __align__(16) NiWave {
     int pad_[1];
 };
 struct __align__(16) NiShared {
-    u4v wl[2][18][64];                     // conv2 / conv3 B operands, bf16 x 8 per (step, lane)
+    u4v wl[2][18][64];                     // conv2 / conv3 weight operands, bf16 x 8 per (step, lane)
+    u4v w1[2][64];                         // conv1's, per K step and lane
     float b1[32], b2[32], b3[32], wf[64], bf[2];
     NiWave wave[NI_WAVES];
 };
@@ -705,33 +706,23 @@ k_nn_infer(const fdvg_seq *__restrict__ seqs, int nseq, int total, const float *
 #pragma unroll
             for (int c = 0; c < NI_RS / 2; ++c) ((uint32_t *)&Wv.act[e * NI_RS])[c] = 0;
         }
-    // conv1's B operands in registers: k = 16 s + 8 g + j (k = tap*2 + ci, zero beyond 18)
-    u4v w1r[2];
-#pragma unroll
-    for (int s2 = 0; s2 < 2; ++s2) {
+    // conv1's weight operands: k = 16 s + 8 g + j (k = tap*2 + ci, zero beyond 18)
+    if (wv < 2) {
         float f[8];
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
-            const int k = 16 * s2 + 8 * g + j;
+            const int k = 16 * wv + 8 * g + j;
             f[j] = k < NN_K1 ? W[OFF_W1 + i32 * NN_K1 + k] : 0.f;
         }
-        w1r[s2] = ni_pack8(f);
+        S.w1[wv][l] = ni_pack8(f);
     }
-    // per tile: this lane's padded position (the last tile's lanes past the map
-    // take position 0: their outputs are discarded), and the lane half's four
-    // conv1 taps as padded offsets
-    int pb[10];
-#pragma unroll
-    for (int tl = 0; tl < 10; ++tl) {
+    // per tile, this lane's padded position (recomputed where used rather than
+    // held across the MB loop): the last tile's lanes past the map take
+    // position 0, their outputs are discarded
+    auto pb = [&](int tl) {
         const int p = tl * 32 + i32;
-        pb[tl] = p < NN_P ? ni_pad(p) : ni_pad(0);
-    }
-    int sh1[4];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const int tap = 4 * g + j;
-        sh1[j] = (tap / 3 - 1) * NI_PW + tap % 3 - 1;
-    }
+        return p < NN_P ? ni_pad(p) : ni_pad(0);
+    };
     __syncthreads();
     const uint8_t *actb = (const uint8_t *)Wv.act;
     for (int gi = blockIdx.x * NI_WAVES + wv; gi < total; gi += gridDim.x * NI_WAVES) {
@@ -754,12 +745,17 @@ k_nn_infer(const fdvg_seq *__restrict__ seqs, int nseq, int total, const float *
             for (int tl = 0; tl < 10; ++tl) {
                 f16v c = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
                 u4v a0, a1 = {0u, 0u, 0u, 0u};
+                const int pt = pb(tl);
 #pragma unroll
-                for (int j = 0; j < 4; ++j) a0[j] = Wv.fb[pb[tl] + sh1[j]];
-                if (g == 0) a1[0] = Wv.fb[pb[tl] + NI_PW + 1];
-                c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf8, w1r[0]), __builtin_bit_cast(bf8, a0), c, 0, 0, 0);
-                c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf8, w1r[1]), __builtin_bit_cast(bf8, a1), c, 0, 0, 0);
-                ni_put(Wv.act, pb[tl], tl * 32 + i32 < NN_P, c, S.b1, g);
+                for (int j = 0; j < 4; ++j) {
+                    // the lane half's four conv1 taps 4 g + j as padded offsets
+                    const int tap = 4 * g + j;
+                    a0[j] = Wv.fb[pt + (tap / 3 - 1) * NI_PW + tap % 3 - 1];
+                }
+                if (g == 0) a1[0] = Wv.fb[pt + NI_PW + 1];
+                c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf8, S.w1[0][l]), __builtin_bit_cast(bf8, a0), c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf8, S.w1[1][l]), __builtin_bit_cast(bf8, a1), c, 0, 0, 0);
+                ni_put(Wv.act, pt, tl * 32 + i32 < NN_P, c, S.b1, g);
             }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -784,7 +780,7 @@ k_nn_infer(const fdvg_seq *__restrict__ seqs, int nseq, int total, const float *
                 u4v av[10];
 #pragma unroll
                 for (int tl = 0; tl < 10; ++tl) {
-                    const int off = (pb[tl] - NI_PW - 1) * (NI_RS * 2) + 16 * g + (sh + NI_PW + 1) * (NI_RS * 2) + 32 * (st & 1);
+                    const int off = (pb(tl) - NI_PW - 1) * (NI_RS * 2) + 16 * g + (sh + NI_PW + 1) * (NI_RS * 2) + 32 * (st & 1);
                     av[tl] = *(const u4v *)(actb + off);
                 }
                 __builtin_amdgcn_sched_barrier(0);
@@ -795,7 +791,7 @@ k_nn_infer(const fdvg_seq *__restrict__ seqs, int nseq, int total, const float *
             if (layer == 0) {
                 // every read of the buffer has completed (its values fed the MFMAs): rewrite it with conv2
 #pragma unroll
-                for (int tl = 0; tl < 10; ++tl) ni_put(Wv.act, pb[tl], tl * 32 + i32 < NN_P, acc[tl], S.b2, g);
+                for (int tl = 0; tl < 10; ++tl) ni_put(Wv.act, pb(tl), tl * 32 + i32 < NN_P, acc[tl], S.b2, g);
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
