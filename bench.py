@@ -189,7 +189,8 @@ def _cpu_model():
 
 def profiled_traffic(mode: str):
     """HBM bytes per launch of the dominant kernel (diamond / exhaustive: the
-    k_analyse wavefront; learned / parallel: the decision stage's five kernels)
+    k_analyse wavefront; learned / parallel: the decision stage's three kernels,
+    k_pcosts, k_pxrep and k_pdecide)
     from the committed rocprofv3 counter summary of the 720p workload
     (FETCH_SIZE and WRITE_SIZE, each calibrated against a plain copy), or None
     when no summary exists for this mode."""
@@ -400,12 +401,12 @@ def main():
     achieved = algo_bytes / (k_ms * 1e-3) / 1e9
     traffic, traffic_src = profiled_traffic(args.mode) if args.workload == "720p" else (None, None)
     par = mode in (G.ME_LEARNED, G.ME_PARALLEL)
-    # the dominant launch: the decision stage (one graph launch of five kernels)
+    # the dominant launch: the decision stage (k_pcosts, k_pxrep, k_pdecide: one graph launch)
     # in the block-parallel modes, the k_analyse wavefront otherwise; timed with
     # HIP events on the stream it runs on
     roof = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic, "traffic_source": traffic_src,
-            "kernel": "decision stage (k_pcost, k_picost, k_prepair, k_plist, k_pencode, k_pintra)" if par else "k_analyse",
+            "kernel": "decision stage (k_pcosts, k_pxrep, k_pdecide)" if par else "k_analyse",
             "kernel_ms": round(k_ms, 4), "bytes_per_launch": int(algo_bytes),
             "path_achieved": round(value * bytes_per_px / 1000.0, 3),
             "path_frac": round(value * bytes_per_px / 1000.0 / HBM_PEAK_GBS, 6)}
