@@ -95,8 +95,8 @@ def test_720p_eight_bands_full_size():
 
 def test_720p_eight_bands_full_size_parallel_mode():
     """The block-parallel decisions at their real shape: every band's table,
-    stand-in costs, replay, coding, streams and recon equal to the oracle's
-    restatement over a chain of P-frames."""
+    decisions, streams and recon equal to the oracle's MB loop on the table
+    over a chain of P-frames."""
     sc = content.scene_pan(1280, 720, 4)
     bands = [_Band(sc, y0, min(96, 720 - y0)) for y0 in range(0, 720, 96)]
     assert run_parity(bands, 4, [24] * 8, me_mode=G.ME_PARALLEL, label="720p-parallel") == 0
