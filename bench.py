@@ -9,7 +9,10 @@ installed as the reference (I-frame coding is outside this path); every step
 is one P-frame of all 8 bands in one batched device pass: motion search, mode
 decision, residual coding, reconstruction, deblocking and serialisation of the
 16 symbol streams, plus their copy back to the host.  Source frames are resident
-in HBM before timing starts.
+in HBM before timing starts.  Steps are queued two deep (a step's
+serialisation and stream staging run beside the next step's motion tables, and
+each step's streams are collected once the next is queued), for every mode
+compared; --sync times them one at a time.
 
 --mode picks the search: learned (default: the learned proposer with the
 exhaustive search below its confidence threshold, block-parallel decisions),
@@ -261,7 +264,38 @@ WORKLOADS = {"720p": _job_720p, "batched": _job_batched, "stress1080": _job_stre
              "stress4k": _job_stress(3840, 2160)}
 
 
-def bench_1080p(content, G, steps, warmup):
+def run_steps(a, m, first, n, ncycle, qps, pipelined):
+    """Frame steps first .. first + n - 1 of analyser a's batch in search mode m:
+    step i codes source slot 1 + i % ncycle with one reference on step 0, two
+    after.  Synchronous: each step queued, then its streams fetched.
+    Pipelined: steps queued two deep from the resident sources
+    (fdvgpu_submit_slots; a step's serialisation runs beside the next step's
+    motion tables), each collected (its streams staged in host memory) once the
+    next is queued.  Returns each step's stage times (ms)."""
+    nb = a.nseq
+
+    def args(i):
+        return [1 + (i % ncycle)] * nb, qps, [1.0] * nb, [1 if i == 0 else 2] * nb, [0] * nb
+
+    kt = []
+    if not pipelined:
+        for i in range(first, first + n):
+            a.pframe(*args(i), m)
+            a.fetch()
+            kt.append(a.timings())
+        return kt
+    if n <= 0:
+        return kt
+    a.submit(0, *args(first), m)
+    for k in range(n):
+        if k + 1 < n:
+            a.submit((k + 1) % 2, *args(first + k + 1), m)
+        a.collect(k % 2)
+        kt.append(a.slot_timings(k % 2))
+    return kt
+
+
+def bench_1080p(content, G, steps, warmup, pipelined=True):
     """Learned and exhaustive Mpix/s of 1920x1088 P-frames (the headline's pan
     content at 1080p), QP 24, as 12 bands of 6 MB rows in one batch."""
     w, h = 1920, 1088
@@ -281,11 +315,9 @@ def bench_1080p(content, G, steps, warmup):
             for i in range(1, nfr):
                 a.put_source(b, i, per[i][b])
         a.autotune([1] * nb, [24] * nb, navail=[1] * nb, me_mode=m, reps=2)
-        for i in range(warmup + steps):
-            if i == warmup:
-                t0 = time.perf_counter()
-            a.pframe([1 + i] * nb, [24] * nb, [1.0] * nb, [1 if i == 0 else 2] * nb, [0] * nb, m)
-            a.fetch()
+        run_steps(a, m, 0, warmup, nfr - 1, [24] * nb, pipelined)
+        t0 = time.perf_counter()
+        run_steps(a, m, warmup, steps, nfr - 1, [24] * nb, pipelined)
         res[name] = steps * w * h / (time.perf_counter() - t0) / 1e6
         a.close()
     return {"frame": [w, h], "bands": nb, "qp": 24, "learned_mpix_s": round(res["learned"], 3),
@@ -311,6 +343,9 @@ def main():
                          "coding), 1 costs, replays, coding workers on one stream, 2 replays beside the costs and "
                          "workers beside the replays, 3 replays beside the costs, then the workers (default: the "
                          "autotuner's choice of 2 and 3)")
+    ap.add_argument("--sync", action="store_true",
+                    help="time frame steps one at a time (queue, then fetch the streams) instead of two deep "
+                         "(each step's serialisation beside the next step's motion tables)")
     ap.add_argument("--conf", type=float, default=None, help="proposer confidence threshold (default 0.7)")
     ap.add_argument("--skip", type=float, default=None, help="proposer SKIP threshold (default: off; 0.7 prunes)")
     ap.add_argument("--baseline-steps", type=int, default=10,
@@ -375,21 +410,10 @@ def main():
         return a
 
     an = open_analyser(mode)
-
-    def step(i, a=None, m=None):
-        a = an if a is None else a
-        src = 1 + (i % (nframes - 1))
-        navail = 1 if i == 0 else 2
-        a.pframe([src] * nb, qps, [1.0] * nb, [navail] * nb, [0] * nb, mode if m is None else m)
-        a.fetch()
-        return a.timings()
-
-    for i in range(args.warmup):
-        step(i)
-    kt = []
+    pipe = not args.sync
+    run_steps(an, mode, 0, args.warmup, nframes - 1, qps, pipe)
     t0 = time.perf_counter()
-    for i in range(args.warmup, nsteps):
-        kt.append(step(i))
+    kt = run_steps(an, mode, args.warmup, args.steps, nframes - 1, qps, pipe)
     dt = time.perf_counter() - t0
     kt = np.array(kt)
 
@@ -419,11 +443,9 @@ def main():
     base = None
     if args.baseline_steps > 0 and mode != G.ME_EXHAUSTIVE:
         ab = open_analyser(G.ME_EXHAUSTIVE)
-        for i in range(min(args.warmup, 2)):
-            step(i, ab, G.ME_EXHAUSTIVE)
+        run_steps(ab, G.ME_EXHAUSTIVE, 0, 2, nframes - 1, qps, pipe)
         tb = time.perf_counter()
-        for i in range(args.baseline_steps):
-            step(i + 2, ab, G.ME_EXHAUSTIVE)
+        run_steps(ab, G.ME_EXHAUSTIVE, 2, args.baseline_steps, nframes - 1, qps, pipe)
         base = args.baseline_steps * px_frame / (time.perf_counter() - tb) / 1e6
         ab.close()
     # the parallel mode (exhaustive tables, the same block-parallel decisions,
@@ -431,18 +453,16 @@ def main():
     par_mpix = None
     if args.baseline_steps > 0 and mode == G.ME_LEARNED:
         ap_ = open_analyser(G.ME_PARALLEL)
-        for i in range(min(args.warmup, 2)):
-            step(i, ap_, G.ME_PARALLEL)
+        run_steps(ap_, G.ME_PARALLEL, 0, 2, nframes - 1, qps, pipe)
         tb = time.perf_counter()
-        for i in range(args.baseline_steps):
-            step(i + 2, ap_, G.ME_PARALLEL)
+        run_steps(ap_, G.ME_PARALLEL, 2, args.baseline_steps, nframes - 1, qps, pipe)
         par_mpix = args.baseline_steps * px_frame / (time.perf_counter() - tb) / 1e6
         ap_.close()
     # the 1080p half of the metric: 1920x1088 (the same pan content) as
     # 6-MB-row bands, learned and exhaustive on the same frames
     line1080 = None
     if args.baseline_steps > 0 and args.workload == "720p" and mode == G.ME_LEARNED:
-        line1080 = bench_1080p(content, G, args.baseline_steps, min(args.warmup, 2))
+        line1080 = bench_1080p(content, G, args.baseline_steps, min(args.warmup, 2), pipe)
     e2e = None
     if args.end_to_end:
         # the drop-in encoder (include/fdv_amd.h): every band's source uploaded,
@@ -535,7 +555,8 @@ def main():
         "data": job["data"],
         "config": {"workload": job["desc"] + ", %s motion search%s" % (
                        args.mode, ", block-parallel decisions" if mode in (G.ME_LEARNED, G.ME_PARALLEL) else ""),
-                   "sequences": nb, "coded_px_per_step": px_frame, "search": args.mode, **job["extra"]},
+                   "sequences": nb, "coded_px_per_step": px_frame, "search": args.mode,
+                   "steps_in_flight": 1 if args.sync else 2, **job["extra"]},
         "stage_ms": stage_ms,
         "launch_shapes": {"serialiser_waves": shapes[0], "deblock_threads": shapes[1], "proposer_wg_per_cu": shapes[2],
                           "decision_workers_per_cu": shapes[3], "decision_structure": structure},

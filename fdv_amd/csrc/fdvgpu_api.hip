@@ -154,6 +154,16 @@ struct fdvgpu {
     size_t compact_cap;
     uint32_t *flags_h;
     hipEvent_t ev[5];
+    // frame steps queued from resident sources (fdvgpu_submit_slots): the
+    // serialisation and staging run on st3 beside the next step's tables
+    hipStream_t st3;
+    hipEvent_t ev_db, ev_ser;       // deblock done (st) / serialisation and staging done (st3)
+    hipEvent_t pev[2][5];           // per staging slot: the step's timing events
+    hipEvent_t *cev;                // the timing events of the step being queued
+    unsigned *flags_snap_d;         // per staging slot: the step's ticket words 0, 1
+    int ser_async;                  // the step being queued serialises on st3
+    int ser_slot;                   // ... into this staging slot
+    int ser_pending;                // a serialisation may still run on st3
     // pipelined frame steps (fdvgpu_submit / fdvgpu_collect), two slots: the
     // pinned upload copies of the sources, the host buffers the staging kernel
     // writes (mapped for the device), and the event after that kernel
@@ -202,6 +212,12 @@ extern "C" void fdvgpu_close(fdvgpu *g) {
     (void)hipFree(g->nnw_d); (void)hipFree(g->nn_d);
     for (int k = 0; k < 96; ++k) if (g->gexec[k]) (void)hipGraphExecDestroy(g->gexec[k]);
     for (int i = 0; i < 5; ++i) if (g->ev[i]) (void)hipEventDestroy(g->ev[i]);
+    for (int k = 0; k < 2; ++k)
+        for (int i = 0; i < 5; ++i) if (g->pev[k][i]) (void)hipEventDestroy(g->pev[k][i]);
+    if (g->ev_db) (void)hipEventDestroy(g->ev_db);
+    if (g->ev_ser) (void)hipEventDestroy(g->ev_ser);
+    if (g->st3) (void)hipStreamDestroy(g->st3);
+    (void)hipFree(g->flags_snap_d);
     for (int i = 0; i < 8; ++i) if (g->ring_ev[i]) (void)hipEventDestroy(g->ring_ev[i]);
     if (g->st) (void)hipStreamDestroy(g->st);
     if (g->st2) (void)hipStreamDestroy(g->st2);
@@ -266,6 +282,13 @@ extern "C" fdvgpu *fdvgpu_open(int nseq, const int *w, const int *h, int nsrc) {
     CKN(hipEventCreateWithFlags(&g->ev_fork, hipEventDisableTiming));
     CKN(hipEventCreateWithFlags(&g->ev_join, hipEventDisableTiming));
     for (int i = 0; i < 5; ++i) CKN(hipEventCreate(&g->ev[i]));
+    CKN(hipStreamCreate(&g->st3));
+    CKN(hipEventCreateWithFlags(&g->ev_db, hipEventDisableTiming));
+    CKN(hipEventCreateWithFlags(&g->ev_ser, hipEventDisableTiming));
+    for (int k = 0; k < 2; ++k)
+        for (int i = 0; i < 5; ++i) CKN(hipEventCreate(&g->pev[k][i]));
+    CKN(hipMalloc(&g->flags_snap_d, 4 * sizeof(unsigned)));
+    g->cev = g->ev;
     for (int i = 0; i < 8; ++i) CKN(hipEventCreateWithFlags(&g->ring_ev[i], hipEventDisableTiming));
     CKN(hipMalloc(&g->seq_d, nseq * sizeof(fdvg_seq)));
     CKN(hipHostMalloc(&g->seq_h, 8 * nseq * sizeof(fdvg_seq), hipHostMallocDefault));
@@ -511,7 +534,8 @@ static int launch_stage(fdvgpu *g, int stage, int me_mode, bool par) {
             seterr("deblock launch failed"); return -1;
         }
     } else if (fdvg_launch_symbols(g->sym_d, g->nseq, g->maxmb, g->recs_d, g->cnt_d, g->totals_d, g->bases_d,
-                                   g->compact_d, g->compact_cap, g->tune.sym_waves, g->st) != 0) {
+                                   g->compact_d, g->compact_cap, g->tune.sym_waves,
+                                   g->ser_async ? g->st3 : g->st) != 0) {
         seterr("serialiser launch failed"); return -1;
     }
     return 0;
@@ -530,11 +554,13 @@ static int run_stage(fdvgpu *g, int stage, int me_mode, bool par) {
         g->warm[key] = 1;
         return 0;
     }
+    // the serialisation of a step queued by fdvgpu_submit_slots runs on st3
+    hipStream_t sst = stage == 3 && g->ser_async ? g->st3 : g->st;
     if (!g->gexec[key]) {
         hipGraph_t gr = NULL;
-        CK(hipStreamBeginCapture(g->st, hipStreamCaptureModeThreadLocal));
+        CK(hipStreamBeginCapture(sst, hipStreamCaptureModeThreadLocal));
         const int rc = launch_stage(g, stage, me_mode, par);
-        const hipError_t e = hipStreamEndCapture(g->st, &gr);
+        const hipError_t e = hipStreamEndCapture(sst, &gr);
         if (rc != 0 || e != hipSuccess) {
             if (gr) (void)hipGraphDestroy(gr);
             seterr("frame graph capture failed: %s", rc != 0 ? fdvgpu_last_error() : hipGetErrorString(e));
@@ -544,7 +570,7 @@ static int run_stage(fdvgpu *g, int stage, int me_mode, bool par) {
         (void)hipGraphDestroy(gr);
         if (ei != hipSuccess) { g->gexec[key] = NULL; seterr("frame graph: %s", hipGetErrorString(ei)); return -1; }
     }
-    CK(hipGraphLaunch(g->gexec[key], g->st));
+    CK(hipGraphLaunch(g->gexec[key], sst));
     return 0;
 }
 
@@ -567,7 +593,8 @@ extern "C" int fdvgpu_pframe(fdvgpu *g, const int *src_slot, const int *qp, cons
     }
     g->step_ctr = g->step_ctr + 1 ? g->step_ctr + 1 : 1;       // never 0 (the flags' initial value)
     const unsigned step = g->step_ctr;
-    CK(hipEventRecord(g->ev[0], g->st));
+    hipEvent_t *ev = g->cev;
+    CK(hipEventRecord(ev[0], g->st));
     // staging slot for this frame's descriptors: the copy of the frame 8 steps
     // back must have run before the slot is rewritten
     const int slot = g->ring_pos;
@@ -616,10 +643,26 @@ extern "C" int fdvgpu_pframe(fdvgpu *g, const int *src_slot, const int *qp, cons
     const bool reuse = g->keep_tables && me_mode == g->last_mode && me_mode != FDVGPU_ME_DIAMOND;
     g->keep_tables = 0;
     for (int stage = 0; stage < 4; ++stage) {
+        if (stage == 3 && g->ser_async) {
+            // the serialisation on st3 once the deblock is done, so that the
+            // next step's tables run beside it; its MB records stay untouched
+            // until then (the next step's decisions wait for ev_ser)
+            CK(hipEventRecord(g->ev_db, g->st));
+            CK(hipStreamWaitEvent(g->st3, g->ev_db, 0));
+            if (run_stage(g, stage, me_mode, par) != 0) return -1;
+            CK(hipEventRecord(ev[4], g->st3));
+            continue;
+        }
         if (stage == 0 && reuse) {
             if (fdvg_launch_zero(g->ticket_d, CTL_WORDS, g->st) != 0) { seterr("ticket reset launch failed"); return -1; }
         } else if (run_stage(g, stage, me_mode, par) != 0) return -1;
-        CK(hipEventRecord(g->ev[stage + 1], g->st));
+        // the decisions rewrite the MB records: an earlier step's serialisation
+        // on st3 must be done (counted in the tables' time)
+        if (stage == 0 && g->ser_pending) CK(hipStreamWaitEvent(g->st, g->ev_ser, 0));
+        if (stage == 1 && g->ser_async)
+            CK(hipMemcpyAsync(g->flags_snap_d + 2 * g->ser_slot, g->ticket_d, 2 * sizeof(unsigned),
+                              hipMemcpyDeviceToDevice, g->st));
+        CK(hipEventRecord(ev[stage + 1], g->st));
     }
     g->last_mode = me_mode;
     for (int s = 0; s < g->nseq; ++s) {
@@ -763,6 +806,37 @@ extern "C" int fdvgpu_collect(fdvgpu *g, int slot) {
     const int n = g->nseq * FDVG_NS;
     if (g->pipe_sb[slot][n + 1]) { seterr("MB wavefront: a row waited too long for the row above"); return -1; }
     if (g->pipe_sb[slot][n + 2]) { seterr("symbol streams overflowed (%u bytes)", g->pipe_sb[slot][n]); return -1; }
+    return 0;
+}
+
+// A frame step from sources already in the context's slots, queued as
+// fdvgpu_submit queues an uploaded one; its serialisation and the staging of
+// its streams run on a third stream beside the next step's motion tables.
+extern "C" int fdvgpu_submit_slots(fdvgpu *g, int slot, const int *src_slot, const int *qp, const double *lscale,
+                                   const int *navail, const int *force_skip, int me_mode) {
+    if (!g || slot < 0 || slot > 1 || !src_slot) { seterr("fdvgpu_submit_slots: bad arguments"); return -1; }
+    if (pipe_alloc(g) != 0) return -1;
+    // the slot's previous staging (and with it its timing events) is complete
+    CK(hipEventSynchronize(g->pipe_ev[slot]));
+    g->ser_async = 1; g->ser_slot = slot; g->cev = g->pev[slot];
+    const int rc = fdvgpu_pframe(g, src_slot, qp, lscale, navail, force_skip, me_mode);
+    g->ser_async = 0; g->cev = g->ev;
+    if (rc != 0) return -1;
+    g->can_rewind = 0;
+    if (fdvg_launch_stage(g->bases_d, g->flags_snap_d + 2 * slot, g->nseq * FDVG_NS, g->compact_d, g->compact_cap,
+                          g->pipe_sb_d[slot], g->pipe_sc_d[slot], g->st3) != 0) {
+        seterr("fdvgpu_submit_slots: staging launch failed: %s", hipGetErrorString(hipGetLastError())); return -1;
+    }
+    CK(hipEventRecord(g->pipe_ev[slot], g->st3));
+    CK(hipEventRecord(g->ev_ser, g->st3));
+    g->ser_pending = 1;
+    return 0;
+}
+
+extern "C" int fdvgpu_slot_timings(fdvgpu *g, int slot, float ms[4]) {
+    if (!g || slot < 0 || slot > 1 || !ms) { seterr("fdvgpu_slot_timings: bad arguments"); return -1; }
+    CK(hipEventSynchronize(g->pev[slot][4]));
+    for (int i = 0; i < 4; ++i) CK(hipEventElapsedTime(&ms[i], g->pev[slot][i], g->pev[slot][i + 1]));
     return 0;
 }
 

@@ -66,6 +66,10 @@ def _signatures():
         "fdvgpu_nn_train_step": (i, [vp, vp, vp, vp, i, f, f, vp, vp, i, vp]),
         "fdvgpu_nn_adam": (i, [vp, vp, vp, vp, i, f, i, i, vp]),
         "fdvgpu_timings": (i, [vp, C.POINTER(C.c_float)]),
+        "fdvgpu_submit_slots": (i, [vp, i, ip, ip, dp, ip, ip, i]),
+        "fdvgpu_collect": (i, [vp, i]),
+        "fdvgpu_collected_streams": (i, [vp, i, i, vp, vp]),
+        "fdvgpu_slot_timings": (i, [vp, i, C.POINTER(C.c_float)]),
         "fdvgpu_decode": (i, [vp, vp, ip, ip]),
         "fdvgpu_motion_search": (i, [vp, vp, vp, i, i, i, i, i, vp, f, vp, vp]),
         "fdvgpu_motion_search_noalloc": (i, [vp, vp, vp, i, i, i, i, i, vp, f, vp, vp]),
@@ -190,6 +194,32 @@ class Analyser:
 
     def fetch(self):
         self._ck(self.L.fdvgpu_fetch(self.h), "fetch")
+
+    def submit(self, slot, src_slot, qp, lscale=None, navail=None, force_skip=None, me_mode=ME_DIAMOND):
+        """Queue a frame step from the source slots into staging slot 0 or 1
+        without waiting (fdvgpu_submit_slots); collect(slot) before reusing it."""
+        n = self.nseq
+        lscale = [1.0] * n if lscale is None else lscale
+        navail = [2] * n if navail is None else navail
+        force_skip = [0] * n if force_skip is None else force_skip
+        ls = (C.c_double * n)(*[float(x) for x in lscale])
+        self._ck(self.L.fdvgpu_submit_slots(self.h, int(slot), _ints(src_slot), _ints(qp), ls, _ints(navail),
+                                            _ints(force_skip), int(me_mode)), "submit_slots")
+
+    def collect(self, slot):
+        """Wait for the step staged in `slot` and check its flags."""
+        self._ck(self.L.fdvgpu_collect(self.h, int(slot)), "collect")
+
+    def collected_streams(self, slot, seq):
+        sv = (C.POINTER(C.c_uint8) * NS)()
+        sn = (C.c_size_t * NS)()
+        self._ck(self.L.fdvgpu_collected_streams(self.h, int(slot), seq, sv, sn), "collected_streams")
+        return [bytes(C.string_at(sv[k], sn[k])) if sn[k] else b"" for k in range(NS)]
+
+    def slot_timings(self, slot):
+        ms = (C.c_float * 4)()
+        self._ck(self.L.fdvgpu_slot_timings(self.h, int(slot), ms), "slot_timings")
+        return list(ms)
 
     def rewind(self, keep_motion=False):
         """Undo the last frame step's reference rotation (fdvgpu_rewind); with

@@ -104,6 +104,17 @@ int fdvgpu_submit(fdvgpu *g, int slot, const uint8_t *const *frames, const int *
                   const int *navail, const int *force_skip, int me_mode);
 int fdvgpu_collect(fdvgpu *g, int slot);
 int fdvgpu_collected_streams(fdvgpu *g, int slot, int seq, const uint8_t **sv, size_t *sn);
+/* The same pipelining for frames already in the context's source slots
+ * (fdvgpu_put_source): src_slot[s] as in fdvgpu_pframe.  The step's
+ * serialisation and the staging of its streams run on a stream of their own,
+ * beside the next step's motion tables (whose decisions wait for them), so
+ * that consecutive steps overlap on the device.  fdvgpu_collect /
+ * fdvgpu_collected_streams as above; fdvgpu_slot_timings gives the step's
+ * stage times (as fdvgpu_timings) once it is collected.  No counterpart in the
+ * reference. */
+int fdvgpu_submit_slots(fdvgpu *g, int slot, const int *src_slot, const int *qp, const double *lscale,
+                        const int *navail, const int *force_skip, int me_mode);
+int fdvgpu_slot_timings(fdvgpu *g, int slot, float ms[4]);
 
 /* Undo the reference rotation of the last fdvgpu_pframe, so that the same
  * frame can be coded again from the same references -- the requantising retry

@@ -49,6 +49,52 @@ def test_graph_replays_match_eager(mode):
         an.close()
 
 
+@pytest.mark.parametrize("mode", [G.ME_DIAMOND, G.ME_PARALLEL, G.ME_LEARNED])
+def test_pipelined_steps_match_synchronous(mode):
+    """Frame steps queued two deep from the source slots (fdvgpu_submit_slots:
+    the serialisation beside the next step's tables, on a stream of its own)
+    give every frame's streams and the final reconstruction of steps run one at
+    a time; then a synchronous step after the pipelined ones, too."""
+    scenes = _scenes()
+    sizes = [(s.w, s.h) for s in scenes]
+    nfr = 7
+    frames = [[s.frame(i) for i in range(nfr)] for s in scenes]
+    ctx = []
+    for _ in range(2):
+        an = G.Analyser(sizes, nsrc=nfr)
+        if mode == G.ME_LEARNED:
+            an.set_proposer()
+        for k in range(2):
+            an.put_ref(k, 0, frames[k][0])
+            for f in range(1, nfr):
+                an.put_source(k, f, frames[k][f])
+        ctx.append(an)
+    sync, pipe = ctx
+    want = []
+    for f in range(1, nfr):
+        sync.pframe([f, f], [24, 30], navail=[1 if f == 1 else 2] * 2, me_mode=mode)
+        sync.fetch()
+        want.append([sync.streams(k) for k in range(2)])
+    last = nfr - 2                      # frames 1 .. nfr - 2 pipelined, the last one synchronous
+    say(f"  pipelined mode {mode}: frames 1..{last}")
+    pipe.submit(0, [1, 1], [24, 30], navail=[1, 1], me_mode=mode)
+    for f in range(1, last + 1):
+        if f < last:
+            pipe.submit(f % 2, [f + 1] * 2, [24, 30], navail=[2, 2], me_mode=mode)
+        pipe.collect((f - 1) % 2)
+        for k in range(2):
+            assert pipe.collected_streams((f - 1) % 2, k) == want[f - 1][k], f"frame {f} seq {k}: streams"
+        assert all(t >= 0.0 for t in pipe.slot_timings((f - 1) % 2))
+        np.sort(np.random.default_rng(f).random(10000))          # host work while the next step runs
+    pipe.pframe([nfr - 1] * 2, [24, 30], navail=[2, 2], me_mode=mode)
+    pipe.fetch()
+    for k in range(2):
+        assert pipe.streams(k) == want[nfr - 2][k], f"synchronous frame after the pipelined ones, seq {k}"
+        assert np.array_equal(pipe.recon(k), sync.recon(k)), f"seq {k}: recon"
+    for an in ctx:
+        an.close()
+
+
 def test_autotune_changes_no_result():
     scenes = _scenes()
     sizes = [(s.w, s.h) for s in scenes]
