@@ -201,6 +201,10 @@ extern "C" void fdvg_tables8(int32_t c8[64], int32_t qs8[52]) {
 
 extern "C" void fdvgpu_close(fdvgpu *g) {
     if (!g) return;
+    // every queued step, on each of the context's streams, ends before its memory goes
+    if (g->st) (void)hipStreamSynchronize(g->st);
+    if (g->st2) (void)hipStreamSynchronize(g->st2);
+    if (g->st3) (void)hipStreamSynchronize(g->st3);
     if (g->S) for (int s = 0; s < g->nseq; ++s) { (void)hipFree(g->S[s].frames); (void)hipFree(g->S[s].src); }
     (void)hipFree(g->seq_d); (void)hipHostFree(g->seq_h); (void)hipFree(g->rowtab_d); (void)hipFree(g->ticket_d);
     (void)hipFree(g->recs_d); (void)hipFree(g->hand_d); (void)hipFree(g->info_d); (void)hipFree(g->given_d);
@@ -873,6 +877,8 @@ extern "C" int fdvgpu_fetch(fdvgpu *g) {
     if (!g || !g->have_frame) { seterr("fdvgpu_fetch: no frame analysed"); return -1; }
     if (g->fetched) return 0;
     const int n = g->nseq * FDVG_NS;
+    // a step queued by fdvgpu_submit_slots serialises on st3
+    if (g->ser_pending) CK(hipStreamWaitEvent(g->st, g->ev_ser, 0));
     CK(hipMemcpyAsync(g->bases_h, g->bases_d, (n + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, g->st));
     CK(hipMemcpyAsync(g->flags_h, g->ticket_d, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, g->st));
     CK(hipStreamSynchronize(g->st));

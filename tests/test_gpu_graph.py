@@ -54,7 +54,8 @@ def test_pipelined_steps_match_synchronous(mode):
     """Frame steps queued two deep from the source slots (fdvgpu_submit_slots:
     the serialisation beside the next step's tables, on a stream of its own)
     give every frame's streams and the final reconstruction of steps run one at
-    a time; then a synchronous step after the pipelined ones, too."""
+    a time; a plain fetch after them returns the last one's streams, and a
+    synchronous step after them is right too."""
     scenes = _scenes()
     sizes = [(s.w, s.h) for s in scenes]
     nfr = 7
@@ -86,6 +87,10 @@ def test_pipelined_steps_match_synchronous(mode):
             assert pipe.collected_streams((f - 1) % 2, k) == want[f - 1][k], f"frame {f} seq {k}: streams"
         assert all(t >= 0.0 for t in pipe.slot_timings((f - 1) % 2))
         np.sort(np.random.default_rng(f).random(10000))          # host work while the next step runs
+    # a plain fetch after the pipelined steps returns the last one's streams
+    pipe.fetch()
+    for k in range(2):
+        assert pipe.streams(k) == want[last - 1][k], f"fetch after the pipelined steps, seq {k}"
     pipe.pframe([nfr - 1] * 2, [24, 30], navail=[2, 2], me_mode=mode)
     pipe.fetch()
     for k in range(2):
