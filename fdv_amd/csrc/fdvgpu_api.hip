@@ -58,6 +58,13 @@ extern "C" int fdvg_xcd_order(const int *, const int *, const int *, int, const 
 extern "C" int fdvg_pdecide_grid(int, int, int);
 extern "C" int fdvg_launch_pdecide(const fdvg_seq *, int, const unsigned *, const int *, int, unsigned *,
                                    const uint64_t *, const uint64_t *, uint64_t *, fdvg_mbrec *, hipStream_t);
+// One captured graph per (stage 0-3, search mode 0-3, block-parallel or not,
+// decision structure 0-3 for the block-parallel decision stage); distinct
+// for every combination.
+#define NGRAPH (4 * 4 * 2 * 4)
+static inline int graph_key(int stage, int me_mode, bool par, int overlap) {
+    return ((stage * 4 + me_mode) * 2 + (par ? 1 : 0)) * 4 + (stage == 1 && par ? overlap : 0);
+}
 // the control words of a frame step: [0] row ticket, [1] error flag, [3..7]
 // kernel tickets, [8..15] and [16..23] the per-XCD tickets of the cost and
 // coding kernels
@@ -144,8 +151,8 @@ struct fdvgpu {
     // launch shapes, and one instantiated graph per (search mode, decision structure)
     Tune tune;
     int graphs;
-    hipGraphExec_t gexec[96];       // per (stage, search mode, decision structure, fused stage)
-    int warm[96];                   // the key has run once eagerly (launch caches filled)
+    hipGraphExec_t gexec[NGRAPH];   // per (stage, search mode, block-parallel, decision structure): graph_key
+    int warm[NGRAPH];               // the key has run once eagerly (launch caches filled)
     // learned proposer
     float *nnw_d;
     fdvg_nnout *nn_d;
@@ -214,7 +221,7 @@ extern "C" void fdvgpu_close(fdvgpu *g) {
     (void)hipFree(g->pcw_d); (void)hipFree(g->pctx_d); (void)hipFree(g->pgiven_d); (void)hipFree(g->mrec_d);
     (void)hipFree(g->ilist_d); (void)hipFree(g->ih_d);
     (void)hipFree(g->nnw_d); (void)hipFree(g->nn_d);
-    for (int k = 0; k < 96; ++k) if (g->gexec[k]) (void)hipGraphExecDestroy(g->gexec[k]);
+    for (int k = 0; k < NGRAPH; ++k) if (g->gexec[k]) (void)hipGraphExecDestroy(g->gexec[k]);
     for (int i = 0; i < 5; ++i) if (g->ev[i]) (void)hipEventDestroy(g->ev[i]);
     for (int k = 0; k < 2; ++k)
         for (int i = 0; i < 5; ++i) if (g->pev[k][i]) (void)hipEventDestroy(g->pev[k][i]);
@@ -548,7 +555,7 @@ static int launch_stage(fdvgpu *g, int stage, int me_mode, bool par) {
 // One stage: eagerly the first time its key is seen (which also fills the
 // launchers' one-time caches), from then on as a captured hipGraph.
 static int run_stage(fdvgpu *g, int stage, int me_mode, bool par) {
-    const int key = ((stage * 4 + me_mode) * 2 + (par ? 1 : 0)) * 3 + (stage == 1 && par ? g->overlap : 0);
+    const int key = graph_key(stage, me_mode, par, g->overlap);
     // the side-by-side stage runs eagerly: a captured graph need not run its
     // two branches at the same time, and run first alone the fused launch
     // would wait for costs that do not come
@@ -1000,7 +1007,7 @@ extern "C" int fdvgpu_set_graphs(fdvgpu *g, int on) {
 }
 
 static void drop_graphs(fdvgpu *g) {
-    for (int k = 0; k < 96; ++k)
+    for (int k = 0; k < NGRAPH; ++k)
         if (g->gexec[k]) { (void)hipGraphExecDestroy(g->gexec[k]); g->gexec[k] = NULL; }
 }
 

@@ -100,6 +100,38 @@ def test_pipelined_steps_match_synchronous(mode):
         an.close()
 
 
+def test_graph_cache_keeps_stages_apart():
+    """Each (stage, mode, decision structure) has its own captured graph: after
+    learned frames whose decision stage ran from a graph in structure 3, a held
+    frame in the parallel mode (which takes the MB wavefront) gives the result
+    of a context that never captures (the two once shared a cache slot)."""
+    scenes = _scenes()
+    sizes = [(s.w, s.h) for s in scenes]
+    nfr = 6
+    frames = [[s.frame(i) for i in range(nfr)] for s in scenes]
+    outs = []
+    for graphs in (False, True):
+        an = G.Analyser(sizes, nsrc=nfr)
+        an.set_graphs(graphs)
+        an.set_overlap(3)
+        an.set_proposer()
+        for k in range(2):
+            an.put_ref(k, 0, frames[k][0])
+            for f in range(1, nfr):
+                an.put_source(k, f, frames[k][f])
+        for f in range(1, 4):
+            an.pframe([f, f], [24, 30], navail=[1 if f == 1 else 2] * 2, me_mode=G.ME_LEARNED)
+            an.fetch()
+        say(f"  graph cache: held parallel-mode frame, graphs {graphs}")
+        an.pframe([4, 4], [24, 30], navail=[2, 2], force_skip=[1, 1], me_mode=G.ME_PARALLEL)
+        an.fetch()
+        outs.append([(an.streams(k), an.recon(k)) for k in range(2)])
+        an.close()
+    for k in range(2):
+        assert outs[0][k][0] == outs[1][k][0], f"seq {k}: streams"
+        assert np.array_equal(outs[0][k][1], outs[1][k][1]), f"seq {k}: recon"
+
+
 def test_autotune_changes_no_result():
     scenes = _scenes()
     sizes = [(s.w, s.h) for s in scenes]
