@@ -1356,7 +1356,7 @@ size_t orc_pframe_encode(const uint8_t *cy, const uint8_t *cu, const uint8_t *cv
 
 
 /* ========================================================= motion tables ==
- * The per-MB motion tables of this build's exhaustive, parallel and learned
+ * The per-MB motion tables of the library's exhaustive, parallel and learned
  * modes (not in the reference): the vectors and SADs the MB loop would get
  * from its searches, computed for every MB at once.  The exhaustive search
  * ignores the predictor, so its table is a pure function of the frame and the

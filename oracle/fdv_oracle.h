@@ -117,7 +117,7 @@ size_t orc_pframe_encode(const uint8_t *cy, const uint8_t *cu, const uint8_t *cv
                          orc_tabcache *tc, double lscale, int me_mode,
                          orc_mbinfo *mbinfo);
 
-/* ---- motion tables (this build's exhaustive, parallel and learned modes; not
+/* ---- motion tables (the library's exhaustive, parallel and learned modes; not
  * in the reference).  A per-MB motion table, as the GPU's table kernels write it. */
 typedef struct {
     int16_t mv16[2][2];

@@ -420,8 +420,9 @@ def test_rdoq_only_truncates_and_empties_at_huge_lambda():
 
 
 def test_exhaustive_search_is_brute_force_minimum_then_subpel():
-    """The exhaustive mode (this build's own search): the whole-pel minimum over
-    +-range (ties to the smaller |dx|+|dy|, then raster order), then the
+    """The exhaustive mode (the library's own search, no reference
+    counterpart): the whole-pel minimum over +-range (ties to the smaller
+    |dx|+|dy|, then raster order), then the
     reference's sub-pel cross; the returned SAD is that of the motion-compensated
     prediction at the returned vector."""
     lib = L.oracle()
