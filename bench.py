@@ -338,11 +338,12 @@ def main():
     ap.add_argument("--end-to-end", action="store_true",
                     help="also time the drop-in encoder end to end (source upload, device analysis, "
                          "stream fetch, host entropy coding) on the same frames")
-    ap.add_argument("--stage", type=int, default=None, choices=[0, 1, 2, 3],
+    ap.add_argument("--stage", type=int, default=None, choices=[0, 1, 2, 3, 4],
                     help="structure of the block-parallel decision stage: 0 separate launches (costs, replays, "
                          "coding), 1 costs, replays, coding workers on one stream, 2 replays beside the costs and "
-                         "workers beside the replays, 3 replays beside the costs, then the workers (default: the "
-                         "autotuner's choice of 2 and 3)")
+                         "workers beside the replays, 3 replays beside the costs, then the workers, 4 as 2 with "
+                         "more workers joining once the replays are done (default: the autotuner's choice of 2, 3, "
+                         "4)")
     ap.add_argument("--sync", action="store_true",
                     help="time frame steps one at a time (queue, then fetch the streams) instead of two deep "
                          "(each step's serialisation beside the next step's motion tables)")

@@ -59,11 +59,11 @@ extern "C" int fdvg_pdecide_grid(int, int, int);
 extern "C" int fdvg_launch_pdecide(const fdvg_seq *, int, const unsigned *, const int *, int, unsigned *,
                                    const uint64_t *, const uint64_t *, uint64_t *, fdvg_mbrec *, hipStream_t);
 // One captured graph per (stage 0-3, search mode 0-3, block-parallel or not,
-// decision structure 0-3 for the block-parallel decision stage); distinct
+// decision structure 0-4 for the block-parallel decision stage); distinct
 // for every combination.
-#define NGRAPH (4 * 4 * 2 * 4)
+#define NGRAPH (4 * 4 * 2 * 5)
 static inline int graph_key(int stage, int me_mode, bool par, int overlap) {
-    return ((stage * 4 + me_mode) * 2 + (par ? 1 : 0)) * 4 + (stage == 1 && par ? overlap : 0);
+    return ((stage * 4 + me_mode) * 2 + (par ? 1 : 0)) * 5 + (stage == 1 && par ? overlap : 0);
 }
 // the control words of a frame step: [0] row ticket, [1] error flag, [3..7]
 // kernel tickets, [8..15] and [16..23] the per-XCD tickets of the cost and
@@ -349,7 +349,7 @@ extern "C" fdvgpu *fdvgpu_open(int nseq, const int *w, const int *h, int nsrc) {
         g->xmax = g->nunit > 0 ? fdvg_xcd_order(wv, wv + nseq, wv + 2 * nseq, nseq, corder_h, g->total_mb, xorder_h, xoff_h) : 0;
         // side by side only where the workers leave room for every replay
         g->overlap_avail = g->nunit > 0 && fdvg_pdecide_grid(g->nunit, g->total_mb, -1) > 0;
-        g->overlap = g->overlap_avail ? 2 : 3;
+        g->overlap = g->overlap_avail ? 4 : 3;
         hipError_t e = g->nunit <= 0 ? hipErrorInvalidValue : hipSuccess;
         if (e == hipSuccess) e = hipMalloc(&g->xorder_d, (size_t)g->total_mb * sizeof(unsigned));
         if (e == hipSuccess)
@@ -474,7 +474,7 @@ static int launch_stage(fdvgpu *g, int stage, int me_mode, bool par) {
         }
         if (fdvg_launch_zero(g->ticket_d, CTL_WORDS, g->st) != 0) { seterr("ticket reset launch failed"); return -1; }
     } else if (stage == 1) {
-        if (par && g->overlap == 2) {
+        if (par && (g->overlap == 2 || g->overlap == 4)) {
             // the costs on this stream, the replays beside them on the second
             // one (they wait for each chunk's cost words), then, after the
             // costs, the coding workers on this stream beside the replays.  The
@@ -493,6 +493,17 @@ static int launch_stage(fdvgpu *g, int stage, int me_mode, bool par) {
                 fdvg_launch_pdecide(g->seq_d, g->nseq, g->xorder_d, g->xoff_d, wgrid, g->ticket_d, g->decw_d,
                                     g->botw_d, g->ih_d, g->recs_d, g->st) != 0) {
                 seterr("parallel analysis launch failed: %s", hipGetErrorString(hipGetLastError())); return -1;
+            }
+            if (g->overlap == 4) {
+                // 4: behind the replays on their stream, as many more workers as
+                // in 3 join those beside them on the same tickets; they start once
+                // every replay is done, so none can keep a replay from a CU
+                const int dgrid = fdvg_pdecide_grid(g->nunit, g->total_mb, g->tune.pd_wpc);
+                if (dgrid <= 0 ||
+                    fdvg_launch_pdecide(g->seq_d, g->nseq, g->xorder_d, g->xoff_d, dgrid, g->ticket_d, g->decw_d,
+                                        g->botw_d, g->ih_d, g->recs_d, g->st2) != 0) {
+                    seterr("parallel analysis launch failed: %s", hipGetErrorString(hipGetLastError())); return -1;
+                }
             }
             CK(hipEventRecord(g->ev_join, g->st2));
             CK(hipStreamWaitEvent(g->st, g->ev_join, 0));
@@ -559,7 +570,7 @@ static int run_stage(fdvgpu *g, int stage, int me_mode, bool par) {
     // the side-by-side stage runs eagerly: a captured graph need not run its
     // two branches at the same time, and run first alone the fused launch
     // would wait for costs that do not come
-    if (stage == 1 && par && g->overlap == 2) return launch_stage(g, stage, me_mode, par);
+    if (stage == 1 && par && (g->overlap == 2 || g->overlap == 4)) return launch_stage(g, stage, me_mode, par);
     if (!g->graphs || !g->warm[key]) {
         if (launch_stage(g, stage, me_mode, par) != 0) return -1;
         g->warm[key] = 1;
@@ -986,8 +997,8 @@ extern "C" int fdvgpu_planes(fdvgpu *g, int seq, int src_slot, const uint8_t **c
 }
 
 extern "C" int fdvgpu_set_overlap(fdvgpu *g, int on) {
-    if (!g || on < 0 || on > 3) { seterr("fdvgpu_set_overlap: bad arguments"); return -1; }
-    if (on == 2 && !g->overlap_avail) {
+    if (!g || on < 0 || on > 4) { seterr("fdvgpu_set_overlap: bad arguments"); return -1; }
+    if ((on == 2 || on == 4) && !g->overlap_avail) {
         seterr("fdvgpu_set_overlap: the replays and the coding workers do not fit side by side"); return -1;
     }
     g->overlap = on;
@@ -1038,14 +1049,15 @@ extern "C" int fdvgpu_autotune(fdvgpu *g, const int *src_slot, const int *qp, co
     if (!g || reps <= 0) { seterr("fdvgpu_autotune: bad arguments"); return -1; }
     // parameter 4, tried first: the decision structure (2: the coding workers
     // beside the replays, as many as leave every replay a CU, where that is
-    // available; 3: every worker after the replays, which run beside the costs)
-    static const int opts[5][3] = {{2, 4, 8}, {64, 128, 256}, {1, 2, 0}, {1, 2, 0}, {2, 3, 0}};
+    // available; 3: every worker after the replays, which run beside the costs;
+    // 4: 2 joined by 3's workers once the replays are done)
+    static const int opts[5][3] = {{2, 4, 8}, {64, 128, 256}, {1, 2, 0}, {1, 2, 0}, {2, 3, 4}};
     static const int order[5] = {4, 0, 1, 2, 3};
     const bool par = me_mode == FDVGPU_ME_LEARNED || me_mode == FDVGPU_ME_PARALLEL;
     for (int pi = 0; pi < 5; ++pi) {
         const int prm = order[pi];
         if (prm == 2 && me_mode != FDVGPU_ME_LEARNED) continue;
-        if (prm == 3 && !(par && (g->overlap == 1 || g->overlap == 3))) continue;
+        if (prm == 3 && !(par && (g->overlap == 1 || g->overlap == 3 || g->overlap == 4))) continue;
         if (prm == 4 && !(par && !g->overlap_fixed)) continue;
         float best = 1e30f;
         int bestv = 0;
@@ -1055,7 +1067,7 @@ extern "C" int fdvgpu_autotune(fdvgpu *g, const int *src_slot, const int *qp, co
             else if (prm == 1) t.deblock_bs = opts[1][k];
             else if (prm == 2) t.nn_wpc = opts[2][k];
             else if (prm == 3) t.pd_wpc = opts[3][k];
-            else if (opts[4][k] == 2 && !g->overlap_avail) continue;
+            else if ((opts[4][k] == 2 || opts[4][k] == 4) && !g->overlap_avail) continue;
             else g->overlap = opts[4][k];
             g->tune = t;
             drop_graphs(g);

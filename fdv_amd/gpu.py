@@ -261,12 +261,14 @@ class Analyser:
         coding workers on one stream; 2 the replays beside the costs and the
         coding workers beside the replays (the default where the device holds
         them side by side; True means 2); 3 the replays beside the costs, then
-        the coding workers.  A structure set here is kept by autotune()."""
+        the coding workers; 4 as 2, with more workers joining once the replays
+        are done (the default where 2 is available).  A structure set here is
+        kept by autotune()."""
         mode = 2 if mode is True else int(mode)
         self._ck(self.L.fdvgpu_set_overlap(self.h, mode), "set_overlap")
 
     def overlap(self):
-        """The decision structure in use (0, 1 or 2; see set_overlap)."""
+        """The decision structure in use (0 to 4; see set_overlap)."""
         v = self.L.fdvgpu_get_overlap(self.h)
         if v < 0:
             raise GpuError("get_overlap: " + self.L.fdvgpu_last_error().decode())

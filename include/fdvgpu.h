@@ -194,13 +194,15 @@ int fdvgpu_set_graphs(fdvgpu *g, int on);
  * costs and, after the costs, the coding workers beside the replays, each MB
  * coded as soon as it is decided (an intra MB's chroma once its neighbours are
  * coded); 3, the replays beside the costs as in 2, then as many coding workers
- * as in 1 once the replays are done.  2 is available where every replay still
- * finds a CU beside the workers (at most 7/8 of the CUs' worth of MB rows) and
- * is then the default, 3 otherwise; fdvgpu_autotune keeps the faster of 2 and
- * 3 unless the caller set the structure here.  2 returns -1 where unavailable.
- * No counterpart in the reference. */
+ * as in 1 once the replays are done; 4, as 2, and once the replays are done as
+ * many more workers as in 3 join those beside them.  2 and 4 are available
+ * where every replay still finds a CU beside the workers (at most 7/8 of the
+ * CUs' worth of MB rows), and 4 is then the default, 3 otherwise;
+ * fdvgpu_autotune keeps the fastest of 2, 3 and 4 unless the caller set the
+ * structure here.  2 and 4 return -1 where unavailable.  No counterpart in the
+ * reference. */
 int fdvgpu_set_overlap(fdvgpu *g, int mode);
-/* The decision structure in use (0, 1 or 2), or -1 for a bad context. */
+/* The decision structure in use (0 to 4), or -1 for a bad context. */
 int fdvgpu_get_overlap(fdvgpu *g);
 /* shapes: serialiser wavefronts per workgroup {1..8}, deblock workgroup size
  * {64, 128, 256}, proposer workgroups per CU {1, 2}, coding workers of the

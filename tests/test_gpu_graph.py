@@ -158,12 +158,13 @@ def test_autotune_changes_no_result():
 
 @pytest.mark.parametrize("mode", [G.ME_PARALLEL, G.ME_LEARNED])
 def test_fused_decision_stage_matches_separate_launches(mode):
-    """The decision stage's four structures give the same results, eagerly and
+    """The decision stage's five structures give the same results, eagerly and
     from captured graphs: separate launches (costs, replays, coding kernels);
     the costs, the replays, then the coding workers on one stream; the replays
     beside the costs on a second stream and the coding workers beside the
-    replays; the replays beside the costs, then the coding workers (tagged words
-    throughout).  Each replay is one MB row; the third
+    replays; the replays beside the costs, then the coding workers; the
+    workers beside the replays joined by more once the replays are done
+    (tagged words throughout).  Each replay is one MB row; the third
     sequence has 11 rows.  In the parallel mode the result is also the oracle's
     MB loop on the exhaustive table, frame by frame."""
     scenes = _scenes() + [content.scene_pan(80, 176, 63, nobj=2, smin=8, smax=24, pan=(2, 1), grain=2.0)]
@@ -171,7 +172,7 @@ def test_fused_decision_stage_matches_separate_launches(mode):
     nfr = 5
     frames = [[s.frame(i) for i in range(nfr)] for s in scenes]
     outs = []
-    for overlap in (0, 1, 2, 3):
+    for overlap in (0, 1, 2, 3, 4):
         an = G.Analyser(sizes, nsrc=nfr)
         an.set_overlap(overlap)
         if mode == G.ME_LEARNED:
@@ -198,7 +199,7 @@ def test_fused_decision_stage_matches_separate_launches(mode):
                 assert outs[0][f - 1][k][0] == st, f"frame {f} seq {k}: streams differ from the oracle"
                 assert np.array_equal(outs[0][f - 1][k][1], rec), f"frame {f} seq {k}: recon differs from the oracle"
                 far, near = near, rec
-    for v in (1, 2, 3):
+    for v in (1, 2, 3, 4):
         for f in range(nfr - 1):
             for k in range(len(scenes)):
                 assert outs[0][f][k][0] == outs[v][f][k][0], f"structure {v} frame {f + 1} seq {k}: streams"
